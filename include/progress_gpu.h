@@ -1,0 +1,188 @@
+/*
+ * progress_gpu.h -- C-ABI of the MI355X density-matrix solver library
+ * (libprogress_gpu.so).
+ *
+ * Entry points keep the names, argument order and meaning of PROGRESS's
+ * C interface (src/prg_c_interface.F90 bind(C) wrappers, prototypes in
+ * src/prg_progress_mod.h).  Each declaration cites the reference wrapper it
+ * replaces.  Differences that are deliberate:
+ *   - the reference returns nothing and calls `stop` on failure; here the
+ *     routines still return void (same ABI) but record a status that
+ *     prg_last_error_code()/prg_last_error() report, and never exit;
+ *   - `sp2conv` is a NUL-terminated string compared like Fortran ("Rel",
+ *     "Abs", trailing blanks ignored); the reference's bind(C) wrapper passes
+ *     one character by value, so through its C path "Rel" never matches;
+ *   - arrays (pp, vv) are plain pointers with the lengths stated per call;
+ *   - out-parameters the reference loses through `value` (mu, occiter,
+ *     totns) are pointers.
+ *
+ * The matrix handle: BML (lanl/bml) is not available to this build, so the
+ * library carries its own bml_matrix_t holding a dense double-precision
+ * column-major matrix with a lazily synchronised HBM mirror.  The handful of
+ * bml_* functions below have BML's names and C signatures (bml.h) for the
+ * dense / double_real / sequential case; other types and precisions are
+ * accepted only where stated and are stored dense.
+ */
+#ifndef PROGRESS_GPU_H
+#define PROGRESS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ BML subset */
+typedef enum { type_uninitialized, dense, ellpack, ellsort, ellblock, csr } bml_matrix_type_t;
+typedef enum {
+  precision_uninitialized,
+  single_real,
+  double_real,
+  single_complex,
+  double_complex
+} bml_matrix_precision_t;
+typedef enum { sequential, distributed, graph_distributed } bml_distribution_mode_t;
+typedef enum { dense_row_major, dense_column_major } bml_dense_order_t;
+
+typedef struct bml_matrix_t bml_matrix_t;
+
+/* N x N zero / identity matrix; any matrix_type is stored dense; only
+ * double_real precision is supported (others: NULL + error). */
+bml_matrix_t *bml_zero_matrix(bml_matrix_type_t matrix_type, bml_matrix_precision_t precision,
+                              int N, int M, bml_distribution_mode_t distrib_mode);
+bml_matrix_t *bml_identity_matrix(bml_matrix_type_t matrix_type,
+                                  bml_matrix_precision_t precision, int N, int M,
+                                  bml_distribution_mode_t distrib_mode);
+bml_matrix_t *bml_import_from_dense(bml_matrix_type_t matrix_type,
+                                    bml_matrix_precision_t precision, bml_dense_order_t order,
+                                    int N, int M, void *A, double threshold,
+                                    bml_distribution_mode_t distrib_mode);
+/* Returns a malloc'ed N*N double array the caller frees with free(). */
+void *bml_export_to_dense(bml_matrix_t *A, bml_dense_order_t order);
+bml_matrix_t *bml_copy_new(bml_matrix_t *A);
+void bml_copy(bml_matrix_t *A, bml_matrix_t *B); /* B <- A */
+bml_matrix_t *bml_transpose_new(bml_matrix_t *A);
+void bml_add(bml_matrix_t *A, bml_matrix_t *B, double alpha, double beta, double threshold);
+void bml_scale(void *scale_factor, bml_matrix_t *A, bml_matrix_t *B); /* B <- s A */
+double bml_trace(bml_matrix_t *A);
+int bml_get_N(bml_matrix_t *A);
+int bml_get_M(bml_matrix_t *A);
+double bml_get_element(bml_matrix_t *A, int i, int j); /* 0-based */
+void bml_read_bml_matrix(bml_matrix_t *A, char *filename); /* Matrix Market */
+void bml_write_bml_matrix(bml_matrix_t *A, char *filename);
+void bml_deallocate(bml_matrix_t **A);
+
+/* ------------------------------------------------------------------ library */
+/* src/prg_c_interface.F90:51-61 */
+void prg_version(void);
+void prg_progress_init(void);
+void prg_progress_shutdown(void);
+
+/* Status of the last call on this thread (0 = success) and its message. */
+int prg_last_error_code(void);
+const char *prg_last_error(void);
+
+/* ------------------------------------------------------------------ normalize */
+/* src/prg_c_interface.F90:849-854 (prg_normalize_mod.F90:32-46) */
+void prg_normalize(bml_matrix_t *h_bml);
+/* src/prg_c_interface.F90:856-865 (prg_normalize_mod.F90:59-72) */
+void prg_normalize_fermi(bml_matrix_t *h_bml, double h1, double hN, double mu);
+/* src/prg_c_interface.F90:867-876 (prg_normalize_mod.F90:83-94) */
+void prg_normalize_implicit_fermi(bml_matrix_t *h_bml, double cnst, double mu);
+/* Gershgorin interval [gbnd[0], gbnd[1]] (bml_gershgorin as used at
+ * src/prg_normalize_mod.F90:41). */
+void prg_gershgorin(bml_matrix_t *h_bml, double *gbnd);
+
+/* ------------------------------------------------------------------ SP2 */
+/* src/prg_c_interface.F90:1073-1089 (prg_sp2_mod.F90:49-135) */
+void prg_sp2_basic(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, double bndfil,
+                   int minsp2iter, int maxsp2iter, const char *sp2conv, double idemtol,
+                   int verbose);
+/* src/prg_c_interface.F90:1113-1129 (prg_sp2_mod.F90:263-380) */
+void prg_sp2_alg2(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, double bndfil,
+                  int minsp2iter, int maxsp2iter, const char *sp2conv, double idemtol,
+                  int verbose);
+/* src/prg_c_interface.F90:1131-1151 (prg_sp2_mod.F90:395-533).
+ * pp, vv: caller arrays of length maxsp2iter; vv gets one entry per
+ * computed product (vv[*icount] holds the discarded last product when the
+ * loop ends in the limdiff branch, as in the reference). */
+void prg_sp2_alg2_genseq(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold,
+                         double bndfil, int minsp2iter, int maxsp2iter, const char *sp2conv,
+                         double idemtol, int *pp, int *icount, double *vv, int verbose);
+/* src/prg_c_interface.F90:1153-1167 (prg_sp2_mod.F90:543-627); pp, vv length >= *icount */
+void prg_sp2_alg2_seq(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, int *pp,
+                      int *icount, double *vv, int verbose);
+/* src/prg_c_interface.F90:1169-1184 (prg_sp2_mod.F90:638-722) */
+void prg_prg_sp2_alg2_seq_inplace(bml_matrix_t *rho_bml, double threshold, int ppsize, int *pp,
+                                  int *icount, double *vv, double mineval, double maxeval);
+/* src/prg_c_interface.F90:1186-1204 (prg_sp2_mod.F90:734-864) */
+void prg_sp2_alg1(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, double bndfil,
+                  int minsp2iter, int maxsp2iter, const char *sp2conv, double idemtol,
+                  int verbose);
+/* src/prg_c_interface.F90:1206-1224 (prg_sp2_mod.F90:879-1000) */
+void prg_sp2_alg1_genseq(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold,
+                         double bndfil, int minsp2iter, int maxsp2iter, const char *sp2conv,
+                         double idemtol, int *pp, int *icount, double *vv);
+/* src/prg_c_interface.F90:1226-1239 (prg_sp2_mod.F90:1010-1089) */
+void prg_sp2_alg1_seq(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, int *pp,
+                      int *icount, double *vv);
+/* src/prg_c_interface.F90:1241-1255 (prg_sp2_mod.F90:1100-1177) */
+void prg_prg_sp2_alg1_seq_inplace(bml_matrix_t *rho_bml, double threshold, int ppsize, int *pp,
+                                  int *icount, double *vv, double mineval, double maxeval);
+/* src/prg_c_interface.F90:1257-1273 (prg_sp2_mod.F90:1190-1246); vv is
+ * accumulated (vv[i] += core-row sum of (X - X^2)^2), length >= *icount.
+ * The reference header's prototype (src/prg_progress_mod.h:601-608) omits the
+ * two matrices; this follows the Fortran wrapper. */
+void prg_sp2_submatrix(bml_matrix_t *ham_bml, bml_matrix_t *rho_bml, double threshold, int *pp,
+                       int *icount, double *vv, double mineval, double maxeval, int core_size);
+/* src/prg_c_interface.F90:1275-1288 (prg_sp2_mod.F90:1259-1310) */
+void prg_sp2_submatrix_inplace(bml_matrix_t *rho_bml, double threshold, int *pp, int *icount,
+                               double *vv, double mineval, double maxeval, int core_size);
+
+/* ------------------------------------------------------------------ homo-lumo / sequence */
+/* New C symbols for src/prg_homolumo_mod.F90:22-89 and :97-147 (no bind(C)
+ * wrapper in the reference).  pp/vv hold the reference's pp(1..), vv(1..). */
+void prg_homolumogap(const double *vv, int imax, const int *pp, double mineval, double maxeval,
+                     double *ehomo, double *elumo, double *egap, int verbose);
+/* pp: caller array of length >= 100; *imax receives the sequence length */
+void prg_sp2sequence(int *pp, int *imax, double mineval, double maxeval, double ehomo,
+                     double elumo, double errlimit, int verbose);
+
+/* ------------------------------------------------------------------ precision modes */
+/* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).
+ *   PRG_PREC_FP64      FP64 MFMA;
+ *   PRG_PREC_F16SPLIT  FP16 hi+lo split products with FP32 accumulation;
+ *   PRG_PREC_INT8X6    exact int8 MFMA products of 6 seven-bit slices with
+ *                      per-row power-of-two scaling (about 42 significant
+ *                      bits), FP64 bookkeeping.
+ * No counterpart in the reference (whose CPU path is FP64 only). */
+#define PRG_PREC_FP64 0
+#define PRG_PREC_F16SPLIT 1
+#define PRG_PREC_INT8X6 2
+void prg_set_precision(int mode);
+int prg_get_precision(void);
+
+/* ------------------------------------------------------------------ device-pointer API */
+/* prg_sp2_alg2 on device-resident data (no host copies): dH and dRho are
+ * ld x ld column-major FP64 device buffers with ld = ((N+127)/128)*128 and
+ * zeros outside the N x N block (dRho gets 2X in the block, zeros outside).
+ * Returns a status; *iters receives the iteration count, pp (may be NULL,
+ * length maxsp2iter) the branch sequence.  Runs on the library stream and
+ * synchronises before returning.  No reference counterpart: the entry point
+ * the benchmark and the Python binding use to keep inputs in HBM. */
+int prg_gpu_sp2_alg2_device(int N, const double *dH, double *dRho, double bndfil, int minsp2iter,
+                            int maxsp2iter, const char *sp2conv, double idemtol, int precision,
+                            int *iters, int *pp);
+/* Padded leading dimension used by the device-pointer API. */
+int prg_gpu_padded_ld(int N);
+
+/* Opt-in timing of the SP2 product kernel launches with HIP events on the
+ * library stream (enable resets the counters); prg_gpu_step_timing returns
+ * the summed milliseconds and the number of timed products.  Measurement
+ * support for the benchmark; no reference counterpart. */
+void prg_gpu_timing(int enable);
+int prg_gpu_step_timing(double *ms_sum, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROGRESS_GPU_H */

@@ -1,0 +1,913 @@
+"""CPU oracle for the PROGRESS density-matrix solvers (TEST INFRASTRUCTURE ONLY).
+
+This module is a plain numpy FP64 restatement of the reference algorithms.  It is
+the checker for the HIP library, never part of the product: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  Every function cites the reference routine it restates
+(paths relative to the lanl/qmd-progress tree).
+
+Matrices are dense, symmetric FP64 ``numpy.ndarray`` objects.  BML (the
+reference's matrix library) is absent; the BML calls used by the reference are
+restated with the dense semantics inferred from their call sites:
+
+* ``bml_multiply_x2(X, X2, thr, trace)`` -> ``X2 = X @ X``, ``trace = (Tr X, Tr X2)``
+* ``bml_multiply(A, B, C, a, b)``        -> ``C = a A @ B + b C``
+* ``bml_add_deprecated(a, A, b, B)``     -> ``A = a A + b B``
+* ``bml_add(A, B, a, b)``                -> ``A = a A + b B``
+* ``bml_scale_add_identity(A, a, b)``    -> ``A = a A + b I``
+* ``bml_gershgorin``                     -> textbook Gershgorin interval
+* ``bml_normalize(A, lo, hi)``           -> ``A = (hi I - A) / (hi - lo)``
+* thresholds are ignored (dense matrices).
+
+Parity pins (checked in ``tests/test_oracle.py``):
+
+* SP2 family, normalisation, sequence/gap helpers: the SP2 iteration is a
+  polynomial in H, so running the same control flow on the eigenvalue vector
+  (``sp2_*_eigen``) must reproduce the matrix oracle's branch sequence and
+  ``Q p(x) Q^T``; the converged result must equal the T=0 eigendecomposition
+  density (``build_density_T0``); the reference's own tolerance tests on the
+  shipped ``poly.512.mtx`` fixture (idempotency <= 1e-6) and the partition
+  KATs (72/6 -> 12 parts, 72/7 -> 11, ``test.part`` -> 104 parts).
+* implicit Fermi: with exact inverses the recursion is the scalar map
+  x <- x^2 / (2x^2 - 2x + 1) on the eigenvalues, whose limit is the Fermi
+  function 1/(1+exp(beta (e-mu))) as the number of steps grows.
+* responses: Daleckii-Krein divided differences of the same scalar maps and
+  central finite differences of the forward solvers.
+
+Functions whose values only BML could pin (``matrix2submatrix_index`` halo
+ordering) are "parity unpinned"; the subgraph loop's result is pinned through
+the block-diagonal special case (parts independent) and the full-halo limit.
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+# --------------------------------------------------------------------------
+# inputs
+# --------------------------------------------------------------------------
+
+
+def model_hamiltonian(n: int, seed: int = 0) -> np.ndarray:
+    """Synthetic dense tight-binding Hamiltonian used by the benchmark.
+
+    Off-diagonal h_ij = -exp(-|i-j|/8) (1 + 0.1 z_ij), z ~ N(0,1) drawn for the
+    full n x n array first, zeroed for |i-j| > 64, symmetrised as
+    (off + off^T)/2; diagonal ~ U[-1, 1] drawn second.  numpy PCG64 via
+    ``default_rng(seed)``.
+    """
+    rng = np.random.default_rng(seed)
+    z = rng.standard_normal((n, n))
+    idx = np.arange(n)
+    dist = np.abs(idx[:, None] - idx[None, :])
+    off = -np.exp(-dist / 8.0) * (1.0 + 0.1 * z)
+    off[dist > 64] = 0.0
+    np.fill_diagonal(off, 0.0)
+    h = 0.5 * (off + off.T)
+    h[idx, idx] = rng.uniform(-1.0, 1.0, n)
+    return h
+
+
+def banded_perturbation(n: int, bandwidth: int = 32, seed: int = 1) -> np.ndarray:
+    """Random symmetric banded matrix (|i-j| <= bandwidth, N(0,1) entries)."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    idx = np.arange(n)
+    a[np.abs(idx[:, None] - idx[None, :]) > bandwidth] = 0.0
+    return np.triu(a) + np.triu(a, 1).T
+
+
+def read_mtx(path: str) -> np.ndarray:
+    """Matrix Market ``coordinate real general`` reader (1-based indices).
+
+    Accepts the ``%%%MatrixMarket`` header variant found in
+    tests/tests_data/hamiltonian-twolevel-ref.mtx.  Missing entries are zero.
+    """
+    import gzip
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as fh:
+        header = fh.readline()
+        if "MatrixMarket" not in header:
+            raise ValueError(f"{path}: not a Matrix Market file")
+        line = fh.readline()
+        while line.startswith("%"):
+            line = fh.readline()
+        nr, nc, nnz = (int(v) for v in line.split())
+        data = np.loadtxt(fh, ndmin=2)
+    if data.shape[0] != nnz:
+        raise ValueError(f"{path}: expected {nnz} entries, got {data.shape[0]}")
+    a = np.zeros((nr, nc))
+    a[data[:, 0].astype(int) - 1, data[:, 1].astype(int) - 1] = data[:, 2]
+    return a
+
+
+# --------------------------------------------------------------------------
+# normalisation (src/prg_normalize_mod.F90)
+# --------------------------------------------------------------------------
+
+
+def gershgorin(h: np.ndarray) -> tuple[float, float]:
+    """Gershgorin interval (bml_gershgorin as used at src/prg_normalize_mod.F90:41)."""
+    d = np.diag(h)
+    r = np.abs(h).sum(axis=1) - np.abs(d)
+    return float((d - r).min()), float((d + r).max())
+
+
+def bml_normalize(x: np.ndarray, mineval: float, maxeval: float) -> np.ndarray:
+    """X = (maxeval I - X)/(maxeval - mineval) (src/prg_normalize_mod.F90:25-29;
+    explicit form at src/prg_response_mod.F90:694-699)."""
+    n = x.shape[0]
+    return (maxeval * np.eye(n) - x) / (maxeval - mineval)
+
+
+def normalize(h: np.ndarray) -> np.ndarray:
+    """prg_normalize (src/prg_normalize_mod.F90:32-46)."""
+    lo, hi = gershgorin(h)
+    return bml_normalize(h, lo, hi)
+
+
+def normalize_fermi(h, h1, hn, mu):
+    """prg_normalize_fermi (src/prg_normalize_mod.F90:59-72)."""
+    n = h.shape[0]
+    return (-h + (hn - mu) * np.eye(n)) / (hn - h1)
+
+
+def normalize_implicit_fermi(h, cnst, mu):
+    """prg_normalize_implicit_fermi (src/prg_normalize_mod.F90:83-94):
+    X0 = (0.5 + cnst mu) I - cnst H."""
+    n = h.shape[0]
+    return -cnst * h + (0.5 + cnst * mu) * np.eye(n)
+
+
+# --------------------------------------------------------------------------
+# SP2 family (src/prg_sp2_mod.F90)
+# --------------------------------------------------------------------------
+
+
+class NotConverging(RuntimeError):
+    """The reference calls ``stop`` / ``error stop`` here."""
+
+
+def _is_rel(sp2conv: str) -> bool:
+    # Fortran character comparison pads the shorter operand with blanks.
+    return sp2conv.rstrip() == "Rel"
+
+
+def sp2_basic(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """prg_sp2_basic (src/prg_sp2_mod.F90:49-135). Returns (rho, iterations)."""
+    n = h.shape[0]
+    occ = bndfil * n
+    x = normalize(h)
+    for it in range(0, maxsp2iter + 1):
+        trx = np.trace(x)
+        x2 = x @ x
+        trx2 = np.trace(x2)
+        if trx - occ <= 0.0:
+            x = 2.0 * x - x2
+            trx = 2.0 * trx - trx2
+        else:
+            x = x2
+            trx = trx2
+        if abs(occ - trx) < idemtol and it > minsp2iter:
+            return 2.0 * x, it
+        if it == maxsp2iter:
+            raise NotConverging("sp2 purification is not converging")
+    raise NotConverging("unreachable")
+
+
+def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace, want_vv):
+    """Shared body of prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380) and
+    prg_sp2_alg2_genseq (:395-533).  ``fresh_trace`` selects genseq's
+    ``trx = trace(1)`` (:452) instead of the recursed trace (:334,341)."""
+    trx = np.trace(x)
+    it = 0
+    pp, vv = [], []
+    e0 = e1 = e2 = 0.0
+    while it < maxsp2iter:
+        it += 1
+        x2 = x @ x
+        if fresh_trace:
+            trx = np.trace(x)
+        trx2 = np.trace(x2)
+        if want_vv:
+            vv.append(math.sqrt(float(np.sum((x - x2) ** 2))))
+        tr2xx2 = 2.0 * trx - trx2
+        trx_old = trx
+        limdiff = abs(trx2 - occ) - abs(tr2xx2 - occ)
+        brk = False
+        if limdiff >= idemtol:
+            x = 2.0 * x - x2
+            trx = 2.0 * trx - trx2
+            pp.append(0)
+        elif limdiff < -idemtol:
+            x = x2
+            trx = trx2
+            pp.append(1)
+        else:
+            it -= 1
+            trx = trx_old
+            brk = True
+        e2, e1 = e1, e0
+        e0 = abs(trx - trx_old)
+        if _is_rel(sp2conv) and it >= minsp2iter and (e0 >= e2 or e0 < idemtol):
+            brk = True
+        if it == maxsp2iter:
+            raise NotConverging("SP2 purification is not converging")
+        if brk:
+            break
+    return x, it, pp, vv
+
+
+def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380). Returns (rho, iterations, pp)."""
+    occ = bndfil * h.shape[0]
+    x, it, pp, _ = _sp2_alg2_core(normalize(h), occ, minsp2iter, maxsp2iter, sp2conv,
+                                  idemtol, False, False)
+    return 2.0 * x, it, pp
+
+
+def sp2_alg2_genseq(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """prg_sp2_alg2_genseq (src/prg_sp2_mod.F90:395-533).
+    Returns (rho, pp, icount, vv); vv has one entry per executed product
+    (including the discarded last product when the loop ends in the
+    limdiff branch, as the reference writes vv(iter) before undoing iter)."""
+    occ = bndfil * h.shape[0]
+    x, it, pp, vv = _sp2_alg2_core(normalize(h), occ, minsp2iter, maxsp2iter, sp2conv,
+                                   idemtol, True, True)
+    return 2.0 * x, pp, it, vv
+
+
+def _sp2_alg2_replay(x, pp, icount):
+    """Loop body of prg_sp2_alg2_seq (:543-627) and the in-place variant (:638-722)."""
+    vv = []
+    for it in range(icount):
+        x2 = x @ x
+        vv.append(math.sqrt(float(np.sum((x - x2) ** 2))))
+        x = 2.0 * x - x2 if pp[it] == 0 else x2
+    return 2.0 * x, vv
+
+
+def sp2_alg2_seq(h, pp, icount):
+    """prg_sp2_alg2_seq (src/prg_sp2_mod.F90:543-627). Returns (rho, vv)."""
+    return _sp2_alg2_replay(normalize(h), pp, icount)
+
+
+def sp2_alg2_seq_inplace(x, pp, icount, mineval, maxeval):
+    """prg_prg_sp2_alg2_seq_inplace (src/prg_sp2_mod.F90:638-722)."""
+    return _sp2_alg2_replay(bml_normalize(x, mineval, maxeval), pp, icount)
+
+
+def sp2_alg1(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, genseq=False):
+    """prg_sp2_alg1 (src/prg_sp2_mod.F90:734-864) and, with genseq=True,
+    prg_sp2_alg1_genseq (:879-1000).  Returns (rho, iterations, pp, vv)."""
+    n = h.shape[0]
+    occ = bndfil * n
+    x = normalize(h)
+    trx = np.trace(x)
+    it = 0
+    pp, vv = [], []
+    e0 = e1 = e2 = 0.0
+    while it < maxsp2iter:
+        it += 1
+        x2 = x - x @ x
+        if genseq:
+            vv.append(math.sqrt(float(np.sum(x2 ** 2))))
+        trx2 = np.trace(x2)
+        limdiff = abs(trx - trx2 - occ) - abs(trx + trx2 - occ)
+        brk = False
+        if limdiff >= idemtol:
+            x = x + x2
+            trx = trx + trx2
+            pp.append(0)
+        elif limdiff < -idemtol:
+            x = x - x2
+            trx = trx - trx2
+            pp.append(1)
+        elif (not genseq) and limdiff == 0.0 and it == 1:
+            x = x - x2
+            trx = trx - trx2
+            pp.append(1)
+        else:
+            it -= 1
+            brk = True
+        e2, e1 = e1, e0
+        e0 = abs(trx2)
+        if _is_rel(sp2conv) and it >= minsp2iter and (e2 <= e0 or e0 < idemtol):
+            brk = True
+        if genseq and sp2conv.rstrip() == "Abs" and abs(limdiff) < idemtol:
+            brk = True
+        if it == maxsp2iter:
+            raise NotConverging("SP2 purification is not converging")
+        if brk:
+            break
+    return 2.0 * x, it, pp, vv
+
+
+def sp2_alg1_genseq(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """prg_sp2_alg1_genseq (src/prg_sp2_mod.F90:879-1000). Returns (rho, pp, icount, vv)."""
+    rho, it, pp, vv = sp2_alg1(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, genseq=True)
+    return rho, pp, it, vv
+
+
+def _sp2_alg1_replay(x, pp, icount):
+    """Loop body of prg_sp2_alg1_seq (:1010-1089) and the in-place variant (:1100-1177)."""
+    vv = []
+    for it in range(icount):
+        x2 = x - x @ x
+        vv.append(math.sqrt(float(np.sum(x2 ** 2))))
+        x = x + x2 if pp[it] == 0 else x - x2
+    return 2.0 * x, vv
+
+
+def sp2_alg1_seq(h, pp, icount):
+    """prg_sp2_alg1_seq (src/prg_sp2_mod.F90:1010-1089)."""
+    return _sp2_alg1_replay(normalize(h), pp, icount)
+
+
+def sp2_alg1_seq_inplace(x, pp, icount, mineval, maxeval):
+    """prg_prg_sp2_alg1_seq_inplace (src/prg_sp2_mod.F90:1100-1177)."""
+    return _sp2_alg1_replay(bml_normalize(x, mineval, maxeval), pp, icount)
+
+
+def sp2_submatrix(x, pp, icount, vv, mineval, maxeval, core_size):
+    """prg_sp2_submatrix[_inplace] (src/prg_sp2_mod.F90:1190-1310).
+    vv(iter) += sum over the first core_size rows of (X - X^2)^2 (no sqrt).
+    Returns (rho, vv) with vv updated in a copy."""
+    vv = list(vv)
+    x = bml_normalize(x, mineval, maxeval)
+    for it in range(icount):
+        x2 = x - x @ x
+        vv[it] += float(np.sum(x2[:core_size, :] ** 2))
+        factor = 1.0 - 2.0 * pp[it]
+        x = x + factor * x2
+    return 2.0 * x, vv
+
+
+# --------------------------------------------------------------------------
+# sequence / gap helpers (src/prg_homolumo_mod.F90)
+# --------------------------------------------------------------------------
+
+
+def homolumogap(vv, imax, pp, mineval, maxeval):
+    """prg_homolumogap (src/prg_homolumo_mod.F90:22-89). vv/pp are 0-based lists
+    holding the reference's vv(1..), pp(1..).  Returns (ehomo, elumo, egap)."""
+    x_a, x_b = 0.0, 1.0
+    i = imax
+    while vv[i - 1] <= 0.0:
+        i -= 1
+    hgamma = 6.0 - 4.0 * math.sqrt(2.0)
+    hgamma = hgamma * (1.0 - hgamma)
+    while vv[i - 1] < hgamma:
+        precomp = math.sqrt(1.0 - 4.0 * vv[i - 1])
+        y_a = 0.5 * (1.0 + precomp)
+        y_b = 0.5 * (1.0 - precomp)
+        for j in range(i - 1, 0, -1):
+            if pp[j - 1] > 0:
+                y_a = math.sqrt(y_a)
+                y_b = math.sqrt(y_b)
+            else:
+                y_a = 1.0 - math.sqrt(1.0 - y_a)
+                y_b = 1.0 - math.sqrt(1.0 - y_b)
+        x_a = max(x_a, y_a)
+        x_b = min(x_b, y_b)
+        i -= 1
+        if i < 1:
+            break
+    ehomo = maxeval - x_a * (maxeval - mineval)
+    elumo = maxeval - x_b * (maxeval - mineval)
+    return ehomo, elumo, elumo - ehomo
+
+
+def sp2sequence(mineval, maxeval, ehomo, elumo, errlimit):
+    """prg_sp2sequence (src/prg_homolumo_mod.F90:97-147). Returns (pp, imax).
+    Keeps the reference's parenthesisation of the second comparison operand."""
+    eh = (maxeval - ehomo) / (maxeval - mineval)
+    el = (maxeval - elumo) / (maxeval - mineval)
+    error = 1.0
+    it = 0
+    pp = []
+    while error > errlimit:
+        it += 1
+        if (abs(1.0 - eh * eh) + abs(el * el)) < abs(1.0 - (2.0 * eh - eh * eh) + abs(2.0 * el - el * el)):
+            pp.append(1)
+        else:
+            pp.append(0)
+        sgm = 1.0 - 2.0 * pp[-1]
+        eh = eh + sgm * (eh - eh * eh)
+        el = el + sgm * (el - el * el)
+        error = abs(1.0 - eh) + abs(el)
+        if it >= 100:
+            error = 0.0
+    return pp, it
+
+
+# --------------------------------------------------------------------------
+# eigendecomposition densities (src/prg_densitymatrix_mod.F90)
+# --------------------------------------------------------------------------
+
+
+def build_density_T0(h, bndfil):
+    """prg_build_density_T0 (src/prg_densitymatrix_mod.F90:36-101): occupation 2
+    for the lowest int(bndfil N) states, 1 for a remaining fractional state."""
+    n = h.shape[0]
+    w, q = np.linalg.eigh(h)
+    nocc = bndfil * n
+    occ = np.zeros(n)
+    full = int(nocc)
+    occ[:full] = 2.0
+    if nocc - full > 1e-12 and full < n:
+        occ[full] = 1.0
+    return (q * occ) @ q.T, w
+
+
+def fermi(e, mu, beta):
+    """fermi (src/prg_implicit_fermi_mod.F90:1192-1198)."""
+    return 1.0 / (1.0 + np.exp(beta * (e - mu)))
+
+
+def get_density_matrix(h, beta, mu):
+    """prg_get_density_matrix (src/prg_implicit_fermi_mod.F90:1056-1100): no factor 2."""
+    w, q = np.linalg.eigh(h)
+    return (q * fermi(w, mu, beta)) @ q.T
+
+
+def test_density_matrix(h, beta, mu, nocc, osteps, occ_err_limit):
+    """prg_test_density_matrix (src/prg_implicit_fermi_mod.F90:1112-1186).
+    Returns (P, mu)."""
+    occ_err = 1000.0
+    it = 0
+    w, q = np.linalg.eigh(h)
+    p = None
+    while (osteps == 0 and occ_err > occ_err_limit) or (osteps > 0 and it < osteps):
+        it += 1
+        p = (q * fermi(w, mu, beta)) @ q.T
+        trp = np.trace(p)
+        dpdmu = beta * (trp - np.sum(p * p))
+        occ_err = abs(trp - nocc)
+        if occ_err > occ_err_limit:
+            mu = mu + (nocc - trp) / dpdmu
+    return p, mu
+
+
+def check_idempotency(a):
+    """prg_check_idempotency (src/prg_densitymatrix_mod.F90:862-880): ||A^2 - A||_F."""
+    return float(np.linalg.norm(a @ a - a))
+
+
+# --------------------------------------------------------------------------
+# implicit Fermi (src/prg_implicit_fermi_mod.F90)
+# --------------------------------------------------------------------------
+
+
+def conjgrad(a, p, b, cg_tol, maxit=50):
+    """prg_conjgrad (src/prg_implicit_fermi_mod.F90:1006-1048): solve A X = B,
+    stop when the squared Frobenius norm of the residual <= cg_tol."""
+    p = p.copy()
+    r = b - a @ p
+    rn = np.sum(r * r)
+    k = 0
+    d = None
+    rn_old = rn
+    while rn > cg_tol:
+        k += 1
+        if k == 1:
+            d = r.copy()
+        else:
+            d = (rn / rn_old) * d + r
+        w = a @ d
+        alpha = rn / np.sum(d * w)
+        p = p + alpha * d
+        r = r - alpha * w
+        rn_old = rn
+        rn = np.sum(r * r)
+        if k > maxit:
+            raise NotConverging("Conjugate gradient is not converging")
+    return p, k
+
+
+def newtonschulz(a, ai, tol, max_iter=200):
+    """prg_newtonschulz (src/prg_implicit_fermi_mod.F90:888-922):
+    R = I - A Ai, err = ||R||_F, Ai <- Ai + Ai R until err <= tol; falls back
+    to CG when the error grows by more than 0.01."""
+    n = a.shape[0]
+    err = 100000.0
+    i = 0
+    while err > tol:
+        tmp = ai.copy()
+        r = np.eye(n) - a @ ai
+        prev = err
+        err = float(np.linalg.norm(r))
+        if prev + 0.01 < err:
+            ai, _ = conjgrad(a, ai, np.eye(n), 0.00001)
+        else:
+            ai = tmp + tmp @ r
+        i += 1
+        if i > max_iter:
+            raise NotConverging("Newton-Schulz is not converging")
+    return ai, i
+
+
+def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, tol):
+    """prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), k = 2.
+    method 0: CG solve per level (:262-277); method 1: the level-1 CG inverse at
+    tol 0.9 reused for every level (:278-298, Newton-Schulz commented out in the
+    reference).  Output P is not scaled by 2.  Returns (P, mu)."""
+    if k != 2:
+        raise ValueError("only expansion order k = 2 is restated")
+    n = h.shape[0]
+    cnst = beta / (4 * k ** nsteps)
+    occ_err = 10000.0
+    it = 0
+    p = None
+    ai = np.eye(n)
+    trp0 = 0.0
+    while (osteps == 0 and occ_err > occ_err_limit) or (osteps > 0 and it < osteps):
+        it += 1
+        p = normalize_implicit_fermi(h, cnst, mu)
+        for i in range(nsteps):
+            p2 = p @ p
+            y = 2.0 * (p2 - p) + np.eye(n)
+            if method == 0:
+                p, _ = conjgrad(y, p, p2, tol)
+            else:
+                if i == 0:
+                    ai, _ = conjgrad(y, ai, np.eye(n), 0.9)
+                p = ai @ p2
+        trp0 = np.trace(p)
+        dpdmu = beta * (trp0 - np.sum(p * p))
+        occ_err = abs(trp0 - nocc)
+        if occ_err > occ_err_limit:
+            mu = mu + (nocc - trp0) / dpdmu
+        if it > 1000:
+            raise NotConverging("chemical potential")
+    return p, mu
+
+
+def implicit_fermi_save_inverse(inv, h, nsteps, nocc, mu, beta, occ_err_limit, tol,
+                                scf_it, maxiter=30):
+    """prg_implicit_fermi_save_inverse (src/prg_implicit_fermi_mod.F90:46-193).
+    ``inv`` is a list of nsteps matrices (updated copies are returned).
+    Returns (P (scaled by 2), mu, inv, occiter_increment, ns_iterations)."""
+    n = h.shape[0]
+    eye = np.eye(n)
+    inv = [m.copy() for m in inv]
+    occ_err = 10.0
+    alpha = 1.0
+    prev = 1
+    it = 0
+    cnst = beta / (2.0 ** (nsteps + 2))
+    if scf_it == 1:
+        alpha = 4.0
+        p = normalize_implicit_fermi(h, cnst, mu)
+        p2 = p @ p
+        y = 2.0 * (p2 - p) + eye
+        ai, _ = conjgrad(y, eye, eye, 0.0001)
+        inv = [eye.copy() for _ in range(nsteps)]
+    totns = 0
+    muadj = 0
+    p = None
+    while (occ_err > occ_err_limit or muadj == 1) and it < maxiter:
+        it += 1
+        muadj = 0
+        p = normalize_implicit_fermi(h, cnst, mu)
+        for i in range(nsteps):
+            p2 = p @ p
+            y = 2.0 * (p2 - p) + eye
+            if it == 1:
+                inv[i], _ = conjgrad(y, inv[i], eye, 0.001)
+            inv[i], nsit = newtonschulz(y, inv[i], tol)
+            totns += nsit
+            p = inv[i] @ p2
+        trp0 = np.trace(p)
+        dpdmu = beta * (trp0 - np.sum(p * p))
+        occ_err = abs(trp0 - nocc)
+        if occ_err > 1.0:
+            if nocc - trp0 < 0.0 and prev == -1:
+                prev = -1
+            elif nocc - trp0 > 0.0 and prev == 1:
+                prev = 1
+            elif nocc - trp0 > 0.0 and prev == -1:
+                prev = 1
+                alpha = alpha / 2
+            else:
+                prev = -1
+                alpha = alpha / 2
+            mu = mu + prev * alpha
+            muadj = 1
+        elif occ_err > occ_err_limit:
+            mu = mu + (nocc - trp0) / dpdmu
+            muadj = 1
+    return 2.0 * p, mu, inv, it, totns
+
+
+def implicit_fermi_first_order_response(h0, h1, inv, nsteps, mu0, beta):
+    """prg_implicit_fermi_first_order_response (src/prg_implicit_fermi_mod.F90:439-529).
+    Returns (P0, P1) (neither scaled by 2)."""
+    n = h0.shape[0]
+    cnst = beta / (2.0 ** (2 + nsteps))
+    p0 = normalize_implicit_fermi(h0, cnst, mu0)
+    p1 = -cnst * h1
+    for i in range(nsteps):
+        c0 = p0 @ p0
+        c = p0 @ p1 + p1 @ p0
+        b = 2.0 * p1 - 2.0 * c
+        p0 = inv[i] @ c0
+        c = c + b @ p0
+        p1 = inv[i] @ c
+    bm = np.eye(n) - p0
+    cm = beta * (p0 @ bm)
+    dpdmu_trace = np.trace(cm)
+    mu1 = -np.trace(p1) / dpdmu_trace
+    if abs(dpdmu_trace) > 1e-8:
+        p1 = p1 + mu1 * cm
+    return p0, p1
+
+
+def implicit_fermi_response(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, max_k=50):
+    """prg_implicit_fermi_response (src/prg_implicit_fermi_mod.F90:547-712),
+    orders 1..3.  The level-1 inverse is a tol-0.01 CG solve from the previous
+    inverse (starting at zero), reused for all levels as in the reference.
+    Returns (P0, [P1..Porder], mu0, mu)."""
+    n = h0.shape[0]
+    eye = np.eye(n)
+    cnst = beta / (2.0 ** (2 + nsteps))
+    mu = [0.0] * order
+    occ_err = 10000.0
+    k = 0
+    ti = np.zeros((n, n))
+    p0 = None
+    ps = []
+    while occ_err > occ_tol:
+        k += 1
+        p0 = normalize_implicit_fermi(h0, cnst, mu0)
+        ps = [normalize_implicit_fermi(hs[j], cnst, mu[j]) - 0.5 * eye for j in range(order)]
+        for i in range(nsteps):
+            c0 = p0 @ p0
+            cs = [None] * order
+            bs = [None] * order
+            cs[0] = p0 @ ps[0] + ps[0] @ p0
+            bs[0] = 2.0 * ps[0] - 2.0 * cs[0]
+            if order > 1:
+                cs[1] = ps[0] @ ps[0] + p0 @ ps[1] + ps[1] @ p0
+                bs[1] = 2.0 * ps[1] - 2.0 * cs[1]
+            if order > 2:
+                cs[2] = ps[0] @ ps[1] + ps[1] @ ps[0] + p0 @ ps[2] + ps[2] @ p0
+                bs[2] = 2.0 * ps[2] - 2.0 * cs[2]
+            t = c0 - p0
+            t = 2.0 * t + eye
+            if i == 0:
+                ti, _ = conjgrad(t, ti, eye, 0.01)
+            p0 = ti @ c0
+            cs[0] = cs[0] + bs[0] @ p0
+            ps[0] = ti @ cs[0]
+            if order > 1:
+                cs[1] = cs[1] + bs[1] @ p0 + bs[0] @ ps[0]
+                ps[1] = ti @ cs[1]
+            if order > 2:
+                cs[2] = cs[2] + bs[2] @ p0 + bs[1] @ ps[0] + bs[0] @ ps[1]
+                ps[2] = ti @ cs[2]
+        pmu = beta * (p0 @ (eye - p0))
+        pmu_trace = np.trace(pmu)
+        p0_trace = np.trace(p0)
+        occ_err = abs(p0_trace - nocc)
+        mu0 = mu0 + (nocc - p0_trace) / pmu_trace
+        for j in range(order):
+            ptr = np.trace(ps[j])
+            mu[j] = mu[j] - ptr / pmu_trace
+            occ_err += abs(ptr)
+        if k > max_k:
+            break
+    return p0, ps, mu0, mu
+
+
+# --------------------------------------------------------------------------
+# density-matrix response (src/prg_response_mod.F90)
+# --------------------------------------------------------------------------
+
+
+def compute_response_sp2(h, h1, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """prg_compute_response_SP2 (src/prg_response_mod.F90:653-782).
+    Returns (rho, rho1, iterations, pp)."""
+    n = h.shape[0]
+    occ = bndfil * n
+    lo, hi = gershgorin(h)
+    mm = hi - lo
+    alpha = -1.0 / mm
+    beta = hi / mm
+    x = alpha * h + beta * np.eye(n)
+    d = alpha * h1
+    trx = np.trace(x)
+    it = 0
+    pp = []
+    e0 = e1 = e2 = 0.0
+    while it < maxsp2iter:
+        it += 1
+        x2 = x @ x
+        a = x @ d + d @ x
+        trx2 = np.trace(x2)
+        tr2xx2 = 2.0 * trx - trx2
+        trx_old = trx
+        limdiff = abs(trx2 - occ) - abs(tr2xx2 - occ)
+        brk = False
+        if limdiff >= idemtol:
+            x = 2.0 * x - x2
+            d = 2.0 * d - a
+            trx = 2.0 * trx - trx2
+            pp.append(0)
+        elif limdiff < -idemtol:
+            x = x2
+            d = a
+            trx = trx2
+            pp.append(1)
+        else:
+            trx = trx_old
+            brk = True
+        e2, e1 = e1, e0
+        e0 = abs(trx - trx_old)
+        if _is_rel(sp2conv) and it >= minsp2iter and (e0 >= e2 or e0 < idemtol):
+            brk = True
+        if it == maxsp2iter:
+            raise NotConverging("SP2 purification is not converging")
+        if brk:
+            break
+    return 2.0 * x, 2.0 * d, it, pp
+
+
+def canon_response(h1_eig, nocc, beta, evals, mu0, m):
+    """prg_canon_response (src/prg_response_mod.F90:947-1022): first-order
+    response of the implicit-Fermi recursion in the eigenbasis of H0, with the
+    canonical trace correction.  ``h1_eig`` is H1 in that eigenbasis."""
+    cnst = beta / (2.0 ** (m + 2))
+    p0 = 0.5 + cnst * (evals - mu0)
+    p1 = -cnst * h1_eig
+    for _ in range(m):
+        p02 = p0 * p0
+        dx1 = p0[:, None] * p1 + p1 * p0[None, :]
+        id0 = 1.0 / (2.0 * (p02 - p0) + 1.0)
+        p0 = id0 * p02
+        p1 = id0[:, None] * (dx1 + 2.0 * (p1 - dx1) * p0[None, :])
+    dpdmu = beta * p0 * (1.0 - p0)
+    mu1 = -np.trace(p1) / np.sum(dpdmu)
+    p1 = p1.copy()
+    p1[np.diag_indices_from(p1)] += mu1 * dpdmu
+    return p1
+
+
+# --------------------------------------------------------------------------
+# graph partitioning and the subgraph SP2 loop
+# (src/prg_graph_mod.F90, src/prg_subgraphloop_mod.F90)
+# --------------------------------------------------------------------------
+
+
+def equal_partition(nodes_per_part, nnodes):
+    """prg_equalPartition (src/prg_graph_mod.F90:359-399): list of 0-based core
+    node lists."""
+    if nnodes <= nodes_per_part:
+        np_ = 1
+    else:
+        np_ = math.ceil(nnodes / nodes_per_part)
+    parts = []
+    for i in range(1, np_ + 1):
+        psize = nodes_per_part if i * nodes_per_part <= nnodes else nnodes - nodes_per_part * (i - 1)
+        parts.append([(i - 1) * nodes_per_part + j for j in range(psize)])
+    return parts
+
+
+def read_part(path):
+    """prg_readPart (src/prg_graph_mod.F90:888-929): returns (name, total_nodes,
+    parts) with parts as 0-based node lists (the file is 0-based; the reference
+    converts to 1-based in memory)."""
+    with open(path) as fh:
+        tokens = fh.read().split()
+    name = tokens[0]
+    total_nodes, total_parts = int(tokens[1]), int(tokens[2])
+    sizes = [int(t) for t in tokens[3:3 + total_parts]]
+    pos = 3 + total_parts
+    parts = []
+    for i in range(total_parts):
+        pos += 1  # part id
+        parts.append([int(t) for t in tokens[pos:pos + sizes[i]]])
+        pos += sizes[i]
+    return name, total_nodes, parts
+
+
+def matrix2submatrix_index(g, core, double_jump=True):
+    """Core+halo index list for one part (the semantics of
+    bml_matrix2submatrix_index as called at src/prg_subgraphloop_mod.F90:60-63;
+    BML is absent so the ordering is parity unpinned): core nodes first in the
+    given order, then every node adjacent in G (nonzero, off-diagonal) to a
+    core node in order of discovery, then with double_jump the neighbours of
+    those halo nodes.  Returns (index_list, core_size)."""
+    seen = set(core)
+    idx = list(core)
+    for c in core:
+        for k in np.nonzero(g[c])[0]:
+            k = int(k)
+            if k not in seen:
+                seen.add(k)
+                idx.append(k)
+    if double_jump:
+        first_halo = idx[len(core):]
+        for c in first_halo:
+            for k in np.nonzero(g[c])[0]:
+                k = int(k)
+                if k not in seen:
+                    seen.add(k)
+                    idx.append(k)
+    return idx, len(core)
+
+
+def subgraph_sp2_loop(h, g, parts, pp, max_iter, mineval, maxeval, double_jump=True):
+    """prg_subgraphSP2Loop (src/prg_subgraphloop_mod.F90:36-126) on one rank:
+    per part extract the core+halo block of H, run prg_sp2_submatrix_inplace,
+    scatter the core rows (and the matching columns, the density being
+    symmetric) back, then prg_fnormGraph (sqrt of the accumulated vv).
+    Returns (rho, vv)."""
+    n = h.shape[0]
+    rho = np.zeros((n, n))
+    vv = [0.0] * max_iter
+    for core in parts:
+        idx, llsize = matrix2submatrix_index(g, core, double_jump)
+        sub = h[np.ix_(idx, idx)]
+        d, vv = sp2_submatrix(sub, pp, max_iter, vv, mineval, maxeval, llsize)
+        cidx = idx[:llsize]
+        rho[np.ix_(cidx, idx)] = d[:llsize, :]
+    vv = [math.sqrt(v) for v in vv]
+    return rho, vv
+
+
+# --------------------------------------------------------------------------
+# closed forms on the eigenvalues (the pins of the matrix oracle)
+# --------------------------------------------------------------------------
+
+
+def sp2_alg2_eigen(evals, mineval, maxeval, occ, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """The prg_sp2_alg2 control flow applied to the eigenvalue vector: X0 has
+    eigenvalues (maxeval - e)/(maxeval - mineval); traces are sums.  Returns
+    (x_final, iterations, pp) with x_final the eigenvalues of the final X."""
+    x = (maxeval - np.asarray(evals)) / (maxeval - mineval)
+    trx = float(np.sum(x))
+    it = 0
+    pp = []
+    e0 = e1 = e2 = 0.0
+    while it < maxsp2iter:
+        it += 1
+        x2 = x * x
+        trx2 = float(np.sum(x2))
+        tr2xx2 = 2.0 * trx - trx2
+        trx_old = trx
+        limdiff = abs(trx2 - occ) - abs(tr2xx2 - occ)
+        brk = False
+        if limdiff >= idemtol:
+            x = 2.0 * x - x2
+            trx = 2.0 * trx - trx2
+            pp.append(0)
+        elif limdiff < -idemtol:
+            x = x2
+            trx = trx2
+            pp.append(1)
+        else:
+            it -= 1
+            trx = trx_old
+            brk = True
+        e2, e1 = e1, e0
+        e0 = abs(trx - trx_old)
+        if _is_rel(sp2conv) and it >= minsp2iter and (e0 >= e2 or e0 < idemtol):
+            brk = True
+        if brk:
+            break
+    return x, it, pp
+
+
+def sp2_poly_eigen(evals, mineval, maxeval, pp):
+    """Replay a branch sequence on the eigenvalues: eigenvalues of the final X."""
+    x = (maxeval - np.asarray(evals)) / (maxeval - mineval)
+    for s in pp:
+        x = x * x if s == 1 else 2.0 * x - x * x
+    return x
+
+
+def implicit_fermi_eigen(evals, mu, beta, nsteps):
+    """Exact-inverse implicit-Fermi recursion on the eigenvalues:
+    x0 = 0.5 - cnst (e - mu), cnst = beta/2^(nsteps+2);
+    x <- x^2 / (2 x^2 - 2 x + 1), nsteps times."""
+    cnst = beta / (2.0 ** (nsteps + 2))
+    x = 0.5 - cnst * (np.asarray(evals) - mu)
+    for _ in range(nsteps):
+        x = x * x / (2.0 * x * x - 2.0 * x + 1.0)
+    return x
+
+
+def divided_differences(f_of_evals, df_of_evals, evals, tol=1e-9):
+    """Daleckii-Krein kernel K_ij = (f(e_i) - f(e_j))/(e_i - e_j), with the
+    derivative on (near-)degenerate pairs."""
+    e = np.asarray(evals)
+    f = f_of_evals(e)
+    de = e[:, None] - e[None, :]
+    df = f[:, None] - f[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        k = np.where(np.abs(de) > tol, df / np.where(np.abs(de) > tol, de, 1.0), 0.0)
+    deg = np.abs(de) <= tol
+    d = df_of_evals(e)
+    k[deg] = (0.5 * (d[:, None] + d[None, :]))[deg]
+    return k

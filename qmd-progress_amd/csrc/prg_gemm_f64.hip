@@ -1,0 +1,428 @@
+// FP64 MFMA GEMM family for gfx950 (v_mfma_f64_16x16x4_f64).
+//
+// Tiling: 128 x 128 output tile per 256-thread workgroup (4 waves, 2 x 2),
+// each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA blocks of 16 x 16.  The K
+// loop stages 16-deep slabs of A and B through LDS with a register prefetch
+// of the next slab.  Workgroup ids are remapped so that each of the 8 XCDs
+// walks a contiguous range of output tiles (its own L2 keeps the shared
+// panels).  For symmetric products (X*X, X*D + D*X, products of commuting
+// symmetric matrices) only the upper-triangle tiles are computed; the
+// epilogue writes each tile and its transpose, so the result is exactly
+// symmetric and the executed flops are half of the algorithmic 2 N^3.
+//
+// Epilogues are fused: the SP2 branch update (X <- X^2 or 2X - X^2, decided on
+// the device before the product from ||X||_F^2 = Tr X^2), the trace and
+// Frobenius partial sums for the next branch decision, and the idempotency
+// sums ||X - X^2||_F^2 (genseq) or its core-row restriction (submatrix).
+#include "prg_internal.h"
+
+namespace prg {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int LROW = BM + 4;  // padded LDS row length in doubles
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
+// contiguous logical range [x*nb/8, (x+1)*nb/8).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8;
+  const int xcd = b % 8, idx = b / 8;
+  return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+// Upper-triangle tile t -> (I, J), I <= J, column-by-column.
+__device__ __forceinline__ void sym_tile(int t, int& I, int& J) {
+  int j = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((j + 1) * (j + 2) / 2 <= t) ++j;
+  while (j * (j + 1) / 2 > t) --j;
+  J = j;
+  I = t - j * (j + 1) / 2;
+}
+
+// Full tile t -> (I, J) with groups of 8 tile-rows sharing B panels.
+__device__ __forceinline__ void full_tile(int t, int nt, int& I, int& J) {
+  const int G = 8;
+  const int group = t / (G * nt);
+  const int first = group * G;
+  const int gsize = min(nt - first, G);
+  const int in = t % (G * nt);
+  I = first + in % gsize;
+  J = in / gsize;
+}
+
+struct Operands {
+  const double* A[2];
+  const double* B[2];
+  int npair;
+};
+
+// Accumulate acc += sum_p A_p[I-panel, :] * B_p[:, J-panel] for one tile.
+// A is read as column-major A[i + k*ld], B as column-major B[k + j*ld].
+__device__ __forceinline__ void mainloop(const Operands& op, int ld, int kdim, int I, int J,
+                                         double (*As)[LROW], double (*Bs)[LROW], d4 acc[4][4]) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+  const int ar = (tid & 63) * 2, ak = tid >> 6;  // A: rows ar,ar+1; k = ak + 4q
+  const int bj = tid >> 1, bk = (tid & 1) * 8;   // B: column bj; k = bk .. bk+7
+  const int ksteps = kdim / BK;
+  const int total = ksteps * op.npair;
+  d2 ra[4], rb[4];
+
+  auto load = [&](int s) {
+    const int p = s / ksteps;
+    const int k0 = (s - p * ksteps) * BK;
+    const double* A = p ? op.A[1] : op.A[0];
+    const double* B = p ? op.B[1] : op.B[0];
+    const double* Ap = A + (size_t)I * BM + ar;
+    const double* Bp = B + (size_t)(J * BN + bj) * ld + k0 + bk;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ra[q] = *(const d2*)(Ap + (size_t)(k0 + ak + 4 * q) * ld);
+      rb[q] = *(const d2*)(Bp + 2 * q);
+    }
+  };
+
+  load(0);
+  for (int s = 0; s < total; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[ak + 4 * q][ar] = ra[q].x;
+      As[ak + 4 * q][ar + 1] = ra[q].y;
+      Bs[bk + 2 * q][bj] = rb[q].x;
+      Bs[bk + 2 * q + 1][bj] = rb[q].y;
+    }
+    __syncthreads();
+    if (s + 1 < total) load(s + 1);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      double a[4], b[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a[m] = As[kr][wm * 64 + m * 16 + (lane & 15)];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) b[n] = Bs[kr][wn * 64 + n * 16 + (lane & 15)];
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[n], a[m], acc[m][n], 0, 0, 0);
+    }
+  }
+}
+
+// Deterministic workgroup sum of NQ per-thread values; thread 0 stores them.
+template <int NQ>
+__device__ __forceinline__ void block_store_partials(double (&v)[NQ], double* dst) {
+  __shared__ double red[NT / 64][NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[q] = x;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) red[w][q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NT / 64; ++ww) s += red[ww][threadIdx.x];
+    dst[threadIdx.x] = s;
+  }
+}
+
+// Element coordinates of accumulator entry (m, n, r) held by this lane.
+// v_mfma_f64_16x16x4 returns D[lane/16 + 4r][lane%16] in register r; the
+// main loop feeds the B fragment as the instruction's first operand, so the
+// block it returns is (A B)^T and lane holds (A B)[lane%16][lane/16 + 4r]:
+// 16 consecutive lanes own 16 consecutive rows of one column (a 128-byte
+// segment of the column-major output).
+#define PRG_ROW(I, m) ((I) * BM + wm * 64 + (m) * 16 + (lane & 15))
+#define PRG_COL(J, n, r) ((J) * BN + wn * 64 + (n) * 16 + (lane >> 4) + 4 * (r))
+
+// ------------------------------------------------------------- general GEMM
+struct GemmArgs {
+  Operands op;
+  int ld, kdim, nt, n, sym;
+  double alpha, beta, gamma;
+  const double* Cin;
+  double* C;
+  double* part;  // per tile [Tr C, ||C||^2] or null
+};
+
+__global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
+  __shared__ double As[BK][LROW];
+  __shared__ double Bs[BK][LROW];
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  if (g.sym) sym_tile(t, I, J); else full_tile(t, g.nt, I, J);
+  d4 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
+  mainloop(g.op, g.ld, g.kdim, I, J, As, Bs, acc);
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
+  const int ld = g.ld;
+  const bool mirror = g.sym && (I != J);
+  const double wgt = mirror ? 2.0 : 1.0;
+  double sums[2] = {0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int row = PRG_ROW(I, m);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = PRG_COL(J, n, r);
+        const size_t idx = row + (size_t)col * ld;
+        double v = acc[m][n][r] * g.alpha;
+        if (g.beta != 0.0) v += g.beta * g.Cin[idx];
+        const bool diag = (row == col);
+        v += (diag && row < g.n) ? g.gamma : 0.0;
+        sums[0] += diag ? v : 0.0;
+        sums[1] += wgt * v * v;
+        g.C[idx] = v;
+        if (mirror) g.C[col + (size_t)row * ld] = v;
+      }
+    }
+  }
+  if (g.part) block_store_partials<2>(sums, g.part + 2 * (size_t)t);
+}
+
+// ------------------------------------------------------------- SP2 step
+struct Sp2Args {
+  const double* buf[2];  // ping-pong X buffers
+  const double* dbuf[2]; // ping-pong D buffers (response step)
+  int ld, kdim, nt;
+  Sp2State* st;
+  double* part;  // per tile [||Xn||^2, Tr Xn, idem, Tr Y]
+};
+
+__global__ __launch_bounds__(NT, 2) void sp2_step_kernel(Sp2Args g) {
+  __shared__ double As[BK][LROW];
+  __shared__ double Bs[BK][LROW];
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
+  const int core = g.st->core;
+  const int variant = g.st->variant;
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
+
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  d4 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
+  Operands op;
+  op.A[0] = X; op.B[0] = X; op.A[1] = X; op.B[1] = X; op.npair = 1;
+  mainloop(op, g.ld, g.kdim, I, J, As, Bs, acc);
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
+  const int ld = g.ld;
+  const bool mirror = (I != J);
+  const bool core_only = (variant == V_SUBMATRIX);
+  // z = cb*x + ca*y: (ca, cb) = (1, 0) for X^2, (-1, 2) for 2X - X^2,
+  // (0, 1) when the product is only measured (X unchanged).
+  const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
+  const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
+  const double wz = mirror ? 2.0 : 1.0;
+  const int core_lim = core_only ? core : 0x7fffffff;
+  double sums[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int row = PRG_ROW(I, m);
+      const double wrow = (row < core_lim) ? 1.0 : 0.0;
+      double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = PRG_COL(J, n, r);
+        const size_t idx = row + (size_t)col * ld;
+        const double y = acc[m][n][r];
+        const double x = X[idx];
+        const double z = fma(cb, x, ca * y);
+        const double dxy = x - y;
+        const double wi = wrow + ((mirror && col < core_lim) ? 1.0 : 0.0);
+        const bool diag = (row == col);
+        q0 += z * z;
+        q2 += wi * dxy * dxy;
+        q1 += diag ? z : 0.0;
+        q3 += diag ? y : 0.0;
+        Xn[idx] = z;
+        // transposed copy; on diagonal tiles the same lane rewrites its own
+        // element with the same value (no branch, no cross-lane race)
+        Xn[mirror ? col + (size_t)row * ld : idx] = z;
+      }
+      sums[0] += wz * q0;
+      sums[1] += q1;
+      sums[2] += q2;
+      sums[3] += q3;
+      // one 16 x 16 block at a time: keeps the X loads of the epilogue from
+      // being hoisted together (register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  block_store_partials<4>(sums, g.part + 4 * (size_t)t);
+}
+
+// Response step: Y = X D + D X (two products accumulated over a 2N-deep K),
+// Dn = Y (sigma=1) or 2D - Y (sigma=0).  X itself is advanced by sp2_step.
+__global__ __launch_bounds__(NT, 2) void sp2_response_kernel(Sp2Args g) {
+  __shared__ double As[BK][LROW];
+  __shared__ double Bs[BK][LROW];
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0] || !s_flags[1]) return;
+  const int sigma = s_flags[2], cur = s_flags[3];
+  const double ca = sigma ? 1.0 : -1.0, cb = sigma ? 0.0 : 2.0;
+  const double* X = g.buf[cur];
+  const double* D = g.dbuf[cur];
+  double* Dn = const_cast<double*>(g.dbuf[cur ^ 1]);
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  d4 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
+  Operands op;
+  op.A[0] = X; op.B[0] = D; op.A[1] = D; op.B[1] = X; op.npair = 2;
+  mainloop(op, g.ld, g.kdim, I, J, As, Bs, acc);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
+  const int ld = g.ld;
+  const bool mirror = (I != J);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int row = PRG_ROW(I, m);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = PRG_COL(J, n, r);
+        const size_t idx = row + (size_t)col * ld;
+        const double z = fma(cb, D[idx], ca * acc[m][n][r]);
+        Dn[idx] = z;
+        Dn[mirror ? col + (size_t)row * ld : idx] = z;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+}  // namespace
+
+int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, const DMat* Cin,
+         double gamma, DMat C, int sym, double* d_part, int* ntiles) {
+  if (npair < 1 || npair > 2) {
+    set_error(ERR_ARG, "gemm: npair must be 1 or 2");
+    return ERR_ARG;
+  }
+  const int ld = C.ld;
+  for (int p = 0; p < npair; ++p) {
+    if (A[p].ld != ld || B[p].ld != ld || !A[p].p || !B[p].p) {
+      set_error(ERR_ARG, "gemm: operand layout mismatch");
+      return ERR_ARG;
+    }
+  }
+  if (beta != 0.0 && (!Cin || Cin->ld != ld || !Cin->p)) {
+    set_error(ERR_ARG, "gemm: beta != 0 needs an input C of the same layout");
+    return ERR_ARG;
+  }
+  if (ld % kTile != 0 || ld <= 0) {
+    set_error(ERR_ARG, "gemm: leading dimension must be a positive multiple of 128");
+    return ERR_ARG;
+  }
+  GemmArgs g;
+  for (int p = 0; p < 2; ++p) {
+    g.op.A[p] = A[p < npair ? p : 0].p;
+    g.op.B[p] = B[p < npair ? p : 0].p;
+  }
+  g.op.npair = npair;
+  g.ld = ld;
+  g.kdim = ld;
+  g.nt = ld / BM;
+  g.n = C.n;
+  g.sym = sym;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.gamma = gamma;
+  g.Cin = Cin ? Cin->p : nullptr;
+  g.C = C.p;
+  g.part = d_part;
+  const int nb = sym ? g.nt * (g.nt + 1) / 2 : g.nt * g.nt;
+  if (ntiles) *ntiles = nb;
+  hipLaunchKernelGGL(gemm_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles) {
+  Sp2Args g;
+  g.buf[0] = x0.p;
+  g.buf[1] = x1.p;
+  g.dbuf[0] = g.dbuf[1] = nullptr;
+  g.ld = x0.ld;
+  g.kdim = x0.ld;
+  g.nt = x0.ld / BM;
+  g.st = d_state;
+  g.part = d_part;
+  const int nb = g.nt * (g.nt + 1) / 2;
+  if (ntiles) *ntiles = nb;
+  hipLaunchKernelGGL(sp2_step_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int sp2_response_step(const DMat& x0, const DMat& x1, const DMat& d0, const DMat& d1,
+                      Sp2State* d_state, double* d_part, int* ntiles) {
+  Sp2Args g;
+  g.buf[0] = x0.p;
+  g.buf[1] = x1.p;
+  g.dbuf[0] = d0.p;
+  g.dbuf[1] = d1.p;
+  g.ld = x0.ld;
+  g.kdim = x0.ld;
+  g.nt = x0.ld / BM;
+  g.st = d_state;
+  g.part = d_part;
+  const int nb = g.nt * (g.nt + 1) / 2;
+  if (ntiles) *ntiles = nb;
+  hipLaunchKernelGGL(sp2_response_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+}  // namespace prg

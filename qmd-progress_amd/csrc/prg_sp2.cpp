@@ -1,0 +1,471 @@
+// SP2 purification drivers (src/prg_sp2_mod.F90) on one GPU.
+//
+// The whole iteration runs on the device: one fused product-and-update launch
+// and one single-workgroup bookkeeping launch per step, enqueued in chunks.
+// The host only reads the `done` flag between chunks (a 64-byte copy), so no
+// scalar crosses PCIe per iteration.  Once the loop has stopped, the
+// remaining launches of a chunk exit at their first instruction.
+#include "prg_sp2.h"
+
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace prg {
+
+namespace {
+int g_precision = PRG_PREC_FP64;
+
+// Optional per-launch timing of the product kernel (HIP events on the library
+// stream), used by the benchmark's roofline figure.
+bool g_timing = false;
+double g_step_ms_sum = 0.0;
+int g_step_count = 0;
+std::vector<hipEvent_t> g_events;
+
+bool fortran_eq(const char* s, const char* lit) {
+  if (!s) return false;
+  size_t n = std::strlen(s);
+  while (n > 0 && s[n - 1] == ' ') --n;
+  return n == std::strlen(lit) && std::strncmp(s, lit, n) == 0;
+}
+}  // namespace
+
+int precision_mode() { return g_precision; }
+void set_precision_mode(int m) { g_precision = m; }
+
+int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const DMat* H1,
+            DMat* out1) {
+  int st = ensure_device();
+  if (st) return st;
+  const int n = H.n, ld = H.ld;
+  if (n <= 0 || ld != padded(n) || out.ld != ld || out.n != n) {
+    set_error(ERR_ARG, "run_sp2: bad matrix layout");
+    return ERR_ARG;
+  }
+  const int nt = ld / kTile;
+  const int ntiles = nt * (nt + 1) / 2;
+  const size_t mat_bytes = sizeof(double) * (size_t)ld * ld;
+  double* x0 = (double*)workspace("sp2_x0", mat_bytes, &st);
+  double* x1 = (double*)workspace("sp2_x1", mat_bytes, &st);
+  double* part = (double*)workspace("sp2_part", sizeof(double) * 4 * (size_t)(ntiles + 2048), &st);
+  double* bounds = (double*)workspace("sp2_bounds", sizeof(double) * 4, &st);
+  Sp2State* dst = (Sp2State*)workspace("sp2_state", sizeof(Sp2State), &st);
+  Sp2State* hst = (Sp2State*)host_staging(sizeof(Sp2State), &st);
+  if (st) return st;
+  DMat X0{n, ld, x0}, X1{n, ld, x1};
+  const bool response = (p.variant == V_RESPONSE);
+  DMat D0{n, ld, nullptr}, D1{n, ld, nullptr};
+  if (response) {
+    D0.p = (double*)workspace("sp2_d0", mat_bytes, &st);
+    D1.p = (double*)workspace("sp2_d1", mat_bytes, &st);
+    if (st) return st;
+  }
+
+  // X0 = (hi I - H)/(hi - lo)
+  int nb = 0;
+  if (p.use_gershgorin) {
+    if ((st = gershgorin(H, bounds))) return st;
+    if ((st = normalize_gershgorin(H, bounds, X0, part, &nb))) return st;
+  } else {
+    if ((st = normalize_bounds(H, p.mineval, p.maxeval, X0, part, &nb))) return st;
+  }
+  double gb[2] = {p.mineval, p.maxeval};
+  if (p.use_gershgorin) {
+    // 16-byte read-back: validates the interval (zero width / NaN input)
+    PRG_HIP_CHECK(hipMemcpyAsync(hst, bounds, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+    gb[0] = ((double*)hst)[0];
+    gb[1] = ((double*)hst)[1];
+  }
+  if (!std::isfinite(gb[0]) || !std::isfinite(gb[1]) || !(gb[1] > gb[0])) {
+    set_error(ERR_NUMERIC, "SP2: spectral interval is empty or not finite (zero or NaN input?)");
+    return ERR_NUMERIC;
+  }
+  if (res) {
+    res->mineval = gb[0];
+    res->maxeval = gb[1];
+  }
+  if (response) {
+    // Delta0 = -H1/(hi - lo)  (src/prg_response_mod.F90:694-700)
+    if (!(gb[1] - gb[0] != 0.0)) {
+      set_error(ERR_NUMERIC, "response: zero-width spectral interval");
+      return ERR_NUMERIC;
+    }
+    if ((st = affine(*H1, -1.0 / (gb[1] - gb[0]), 0.0, D0, nullptr, nullptr))) return st;
+  }
+
+  std::memset(hst, 0, sizeof(Sp2State));
+  hst->occ = p.occ;
+  hst->idemtol = p.idemtol;
+  hst->variant = p.variant;
+  hst->minit = p.minit;
+  hst->maxit = p.maxit;
+  hst->rel = p.rel;
+  hst->absconv = p.absconv;
+  hst->icount = p.icount;
+  hst->core = p.core;
+  if (p.pp_in)
+    for (int i = 0; i < p.icount && i < kMaxIter; ++i) hst->pp[i] = p.pp_in[i];
+  if (p.vv_in)
+    for (int i = 0; i < p.icount && i < kMaxIter; ++i) hst->vv[i] = p.vv_in[i];
+  PRG_HIP_CHECK(hipMemcpyAsync(dst, hst, sizeof(Sp2State), hipMemcpyHostToDevice, stream()));
+  if ((st = sp2_init_state(dst, part, nb))) return st;
+
+  const int maxsteps = (p.variant == V_SEQ_ALG2 || p.variant == V_SEQ_ALG1 ||
+                        p.variant == V_SUBMATRIX)
+                           ? p.icount
+                           : p.maxit + 2;
+  const int chunk = n >= 4096 ? 4 : 16;
+  const size_t head = offsetof(Sp2State, pp);
+  int steps = 0;
+  const int prec = p.precision;
+  if (g_timing && (int)g_events.size() < 2 * (maxsteps + 1)) {
+    const size_t old = g_events.size();
+    g_events.resize(2 * (maxsteps + 1));
+    for (size_t e = old; e < g_events.size(); ++e) PRG_HIP_CHECK(hipEventCreate(&g_events[e]));
+  }
+  while (steps < maxsteps) {
+    for (int c = 0; c < chunk && steps < maxsteps; ++c, ++steps) {
+      int nt_used = 0;
+      if (g_timing) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps], stream()));
+      if (response) {
+        if ((st = sp2_response_step(X0, X1, D0, D1, dst, part, &nt_used))) return st;
+      }
+      if (prec == PRG_PREC_FP64 || response) {
+        st = sp2_step(X0, X1, dst, part, &nt_used);
+      } else if (prec == PRG_PREC_F16SPLIT) {
+        st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
+      } else if (prec == PRG_PREC_INT8X6) {
+        st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
+      } else {
+        set_error(ERR_ARG, "unknown precision mode");
+        return ERR_ARG;
+      }
+      if (st) return st;
+      if (g_timing) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps + 1], stream()));
+      if ((st = sp2_absorb_plan(dst, part, nt_used))) return st;
+    }
+    PRG_HIP_CHECK(hipMemcpyAsync(hst, dst, head, hipMemcpyDeviceToHost, stream()));
+    PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+    if (hst->done) break;
+  }
+  if ((st = sp2_finish(X0, X1, dst, 2.0, out))) return st;
+  if (response && out1) {
+    // rho1 = 2 Delta, Delta in the ping-pong slot named by the state
+    if ((st = sp2_finish(D0, D1, dst, 2.0, *out1))) return st;
+  }
+  PRG_HIP_CHECK(hipMemcpyAsync(hst, dst, sizeof(Sp2State), hipMemcpyDeviceToHost, stream()));
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  if (!hst->done) {
+    set_error(ERR_NOT_CONVERGED, "SP2 loop did not finish within the step budget");
+    return ERR_NOT_CONVERGED;
+  }
+  if (g_timing) {
+    // products actually computed: iter (plus a measured-only last product)
+    const int executed = hst->iter < steps ? hst->iter : steps;
+    for (int e = 0; e < executed; ++e) {
+      float ms = 0.f;
+      PRG_HIP_CHECK(hipEventElapsedTime(&ms, g_events[2 * e], g_events[2 * e + 1]));
+      g_step_ms_sum += ms;
+      ++g_step_count;
+    }
+  }
+  if (res) {
+    res->iter = hst->iter;
+    res->status = hst->status;
+    res->trx = hst->trx;
+    std::memcpy(res->pp, hst->pp, sizeof(res->pp));
+    std::memcpy(res->vv, hst->vv, sizeof(res->vv));
+  }
+  if (hst->status != OK) {
+    set_error(hst->status, "SP2 purification is not converging");
+    return hst->status;
+  }
+  return OK;
+}
+
+}  // namespace prg
+
+using namespace prg;
+
+namespace {
+
+Sp2Params base_params(int variant, int n, double bndfil, int minit, int maxit,
+                      const char* sp2conv, double idemtol) {
+  Sp2Params p;
+  p.variant = variant;
+  p.occ = bndfil * (double)n;
+  p.idemtol = idemtol;
+  p.minit = minit;
+  p.maxit = maxit > kMaxIter - 2 ? kMaxIter - 2 : maxit;
+  p.rel = fortran_rel(sp2conv);
+  p.absconv = fortran_abs(sp2conv);
+  p.precision = precision_mode();
+  return p;
+}
+
+void report(const char* who, int verbose, const Sp2Result& r) {
+  if (verbose >= 1) std::printf("%s: iterations = %d, trace(X) = %.15g\n", who, r.iter, r.trx);
+}
+
+// Shared body of the H -> rho variants.
+void sp2_h_to_rho(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* rho, double bndfil,
+                  int minit, int maxit, const char* sp2conv, double idemtol, int verbose,
+                  int* pp, int* icount, double* vv, int ppsize) {
+  clear_error();
+  if (check_pair(h, rho, who)) return;
+  DMat H, R;
+  if (bml_dev_read(h, &H)) return;
+  if (h == rho) {
+    // in-place call: stage the input in a workspace first
+    int st = OK;
+    double* tmp = (double*)workspace("sp2_hcopy", sizeof(double) * (size_t)H.ld * H.ld, &st);
+    if (st) return;
+    DMat T{H.n, H.ld, tmp};
+    if (copy_block(H, T)) return;
+    H = T;
+  }
+  if (bml_dev_write(rho, &R)) return;
+  Sp2Params p = base_params(variant, h->N, bndfil, minit, maxit, sp2conv, idemtol);
+  Sp2Result r;
+  const int st = run_sp2(H, p, R, &r);
+  if (st != OK && st != ERR_NOT_CONVERGED) return;
+  report(who, verbose, r);
+  if (icount) *icount = r.iter;
+  // vv[iter] is written only when the loop ended on a discarded product
+  const int nrec_full = (r.vv[r.iter] != 0.0) ? r.iter + 1 : r.iter;
+  const int nrec = nrec_full < ppsize ? nrec_full : ppsize;
+  if (pp)
+    for (int i = 0; i < r.iter && i < ppsize; ++i) pp[i] = r.pp[i];
+  if (vv)
+    for (int i = 0; i < nrec; ++i) vv[i] = r.vv[i];
+}
+
+void sp2_replay(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* rho, int* pp,
+                int* icount, double* vv, bool gersh, double mineval, double maxeval, int core) {
+  clear_error();
+  if (!icount || !pp || !vv) {
+    set_error(ERR_ARG, std::string(who) + ": null pp / icount / vv");
+    return;
+  }
+  if (*icount < 0 || *icount > kMaxIter) {
+    set_error(ERR_ARG, std::string(who) + ": icount out of range");
+    return;
+  }
+  if (check_pair(h, rho, who)) return;
+  DMat H, R;
+  if (bml_dev_read(h, &H)) return;
+  if (h == rho) {
+    int st = OK;
+    double* tmp = (double*)workspace("sp2_hcopy", sizeof(double) * (size_t)H.ld * H.ld, &st);
+    if (st) return;
+    DMat T{H.n, H.ld, tmp};
+    if (copy_block(H, T)) return;
+    H = T;
+  }
+  if (bml_dev_write(rho, &R)) return;
+  Sp2Params p;
+  p.variant = variant;
+  p.icount = *icount;
+  p.pp_in = pp;
+  p.vv_in = (variant == V_SUBMATRIX) ? vv : nullptr;
+  p.use_gershgorin = gersh;
+  p.mineval = mineval;
+  p.maxeval = maxeval;
+  p.core = core;
+  p.precision = precision_mode();
+  Sp2Result r;
+  if (run_sp2(H, p, R, &r) != OK) return;
+  for (int i = 0; i < *icount; ++i) vv[i] = r.vv[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+void prg_set_precision(int mode) {
+  clear_error();
+  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_F16SPLIT && mode != PRG_PREC_INT8X6) {
+    set_error(ERR_ARG, "prg_set_precision: unknown mode");
+    return;
+  }
+  set_precision_mode(mode);
+}
+int prg_get_precision(void) { return precision_mode(); }
+
+void prg_sp2_basic(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
+                   int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
+                   int verbose) {
+  (void)threshold;
+  sp2_h_to_rho("prg_sp2_basic", V_BASIC, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
+               idemtol, verbose, nullptr, nullptr, nullptr, 0);
+}
+
+void prg_sp2_alg2(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
+                  int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
+                  int verbose) {
+  (void)threshold;
+  sp2_h_to_rho("prg_sp2_alg2", V_ALG2, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
+               idemtol, verbose, nullptr, nullptr, nullptr, 0);
+}
+
+void prg_sp2_alg2_genseq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold,
+                         double bndfil, int minsp2iter, int maxsp2iter, const char* sp2conv,
+                         double idemtol, int* pp, int* icount, double* vv, int verbose) {
+  (void)threshold;
+  sp2_h_to_rho("prg_sp2_alg2_genseq", V_ALG2_GENSEQ, h_bml, rho_bml, bndfil, minsp2iter,
+               maxsp2iter, sp2conv, idemtol, verbose, pp, icount, vv, maxsp2iter);
+}
+
+void prg_sp2_alg2_seq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, int* pp,
+                      int* icount, double* vv, int verbose) {
+  (void)threshold;
+  (void)verbose;
+  sp2_replay("prg_sp2_alg2_seq", V_SEQ_ALG2, h_bml, rho_bml, pp, icount, vv, true, 0, 0, 0);
+}
+
+void prg_prg_sp2_alg2_seq_inplace(bml_matrix_t* rho_bml, double threshold, int ppsize, int* pp,
+                                  int* icount, double* vv, double mineval, double maxeval) {
+  (void)threshold;
+  (void)ppsize;
+  sp2_replay("prg_prg_sp2_alg2_seq_inplace", V_SEQ_ALG2, rho_bml, rho_bml, pp, icount, vv, false,
+             mineval, maxeval, 0);
+}
+
+void prg_sp2_alg1(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
+                  int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
+                  int verbose) {
+  (void)threshold;
+  sp2_h_to_rho("prg_sp2_alg1", V_ALG1, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
+               idemtol, verbose, nullptr, nullptr, nullptr, 0);
+}
+
+void prg_sp2_alg1_genseq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold,
+                         double bndfil, int minsp2iter, int maxsp2iter, const char* sp2conv,
+                         double idemtol, int* pp, int* icount, double* vv) {
+  (void)threshold;
+  sp2_h_to_rho("prg_sp2_alg1_genseq", V_ALG1_GENSEQ, h_bml, rho_bml, bndfil, minsp2iter,
+               maxsp2iter, sp2conv, idemtol, 0, pp, icount, vv, maxsp2iter);
+}
+
+void prg_sp2_alg1_seq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, int* pp,
+                      int* icount, double* vv) {
+  (void)threshold;
+  sp2_replay("prg_sp2_alg1_seq", V_SEQ_ALG1, h_bml, rho_bml, pp, icount, vv, true, 0, 0, 0);
+}
+
+void prg_prg_sp2_alg1_seq_inplace(bml_matrix_t* rho_bml, double threshold, int ppsize, int* pp,
+                                  int* icount, double* vv, double mineval, double maxeval) {
+  (void)threshold;
+  (void)ppsize;
+  sp2_replay("prg_prg_sp2_alg1_seq_inplace", V_SEQ_ALG1, rho_bml, rho_bml, pp, icount, vv, false,
+             mineval, maxeval, 0);
+}
+
+void prg_sp2_submatrix(bml_matrix_t* ham_bml, bml_matrix_t* rho_bml, double threshold, int* pp,
+                       int* icount, double* vv, double mineval, double maxeval, int core_size) {
+  (void)threshold;
+  sp2_replay("prg_sp2_submatrix", V_SUBMATRIX, ham_bml, rho_bml, pp, icount, vv, false, mineval,
+             maxeval, core_size);
+}
+
+void prg_sp2_submatrix_inplace(bml_matrix_t* rho_bml, double threshold, int* pp, int* icount,
+                               double* vv, double mineval, double maxeval, int core_size) {
+  (void)threshold;
+  sp2_replay("prg_sp2_submatrix_inplace", V_SUBMATRIX, rho_bml, rho_bml, pp, icount, vv, false,
+             mineval, maxeval, core_size);
+}
+
+void prg_gershgorin(bml_matrix_t* h_bml, double* gbnd) {
+  clear_error();
+  if (!gbnd) {
+    set_error(ERR_ARG, "prg_gershgorin: null output");
+    return;
+  }
+  DMat H;
+  if (bml_dev_read(h_bml, &H)) return;
+  int st = OK;
+  double* b = (double*)workspace("gersh_out", 2 * sizeof(double), &st);
+  if (st || gershgorin(H, b)) return;
+  if (hipMemcpyAsync(gbnd, b, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
+      hipStreamSynchronize(stream()) != hipSuccess)
+    set_error(ERR_HIP, "prg_gershgorin: copy failed");
+}
+
+void prg_normalize(bml_matrix_t* h_bml) {
+  clear_error();
+  DMat H;
+  if (bml_dev_read(h_bml, &H)) return;
+  int st = OK;
+  double* b = (double*)workspace("gersh_out", 2 * sizeof(double), &st);
+  if (st || gershgorin(H, b)) return;
+  h_bml->host_valid = false;
+  normalize_gershgorin(H, b, H, nullptr, nullptr);
+}
+
+void prg_normalize_fermi(bml_matrix_t* h_bml, double h1, double hN, double mu) {
+  clear_error();
+  DMat H;
+  if (bml_dev_read(h_bml, &H)) return;
+  h_bml->host_valid = false;
+  const double inv = 1.0 / (hN - h1);
+  affine(H, -inv, (hN - mu) * inv, H, nullptr, nullptr);
+}
+
+void prg_normalize_implicit_fermi(bml_matrix_t* h_bml, double cnst, double mu) {
+  clear_error();
+  DMat H;
+  if (bml_dev_read(h_bml, &H)) return;
+  h_bml->host_valid = false;
+  affine(H, -cnst, 0.5 + cnst * mu, H, nullptr, nullptr);
+}
+
+int prg_gpu_padded_ld(int N) { return padded(N); }
+
+void prg_gpu_timing(int enable) {
+  g_timing = enable != 0;
+  g_step_ms_sum = 0.0;
+  g_step_count = 0;
+}
+
+int prg_gpu_step_timing(double* ms_sum, int* count) {
+  if (ms_sum) *ms_sum = g_step_ms_sum;
+  if (count) *count = g_step_count;
+  return OK;
+}
+
+int prg_gpu_sp2_alg2_device(int N, const double* dH, double* dRho, double bndfil, int minsp2iter,
+                            int maxsp2iter, const char* sp2conv, double idemtol, int precision,
+                            int* iters, int* pp) {
+  clear_error();
+  if (N <= 0 || !dH || !dRho) {
+    set_error(ERR_ARG, "prg_gpu_sp2_alg2_device: bad arguments");
+    return ERR_ARG;
+  }
+  if (precision != PRG_PREC_FP64 && precision != PRG_PREC_F16SPLIT &&
+      precision != PRG_PREC_INT8X6) {
+    set_error(ERR_ARG, "prg_gpu_sp2_alg2_device: unknown precision");
+    return ERR_ARG;
+  }
+  const int ld = padded(N);
+  DMat H{N, ld, const_cast<double*>(dH)}, R{N, ld, dRho};
+  Sp2Params p = base_params(V_ALG2, N, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol);
+  p.precision = precision;
+  Sp2Result r;
+  const int st = run_sp2(H, p, R, &r);
+  if (iters) *iters = r.iter;
+  if (pp)
+    for (int i = 0; i < r.iter && i < maxsp2iter; ++i) pp[i] = r.pp[i];
+  return st;
+}
+
+}  // extern "C"
+
+namespace prg {
+bool fortran_rel(const char* s) { return fortran_eq(s, "Rel"); }
+bool fortran_abs(const char* s) { return fortran_eq(s, "Abs"); }
+}  // namespace prg

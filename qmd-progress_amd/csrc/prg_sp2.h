@@ -1,0 +1,52 @@
+// SP2 driver interface shared by the C-ABI wrappers, the response routines
+// and the subgraph loop.
+#pragma once
+
+#include "prg_bml.h"
+
+namespace prg {
+
+struct Sp2Params {
+  int variant = V_ALG2;
+  double occ = 0.0;
+  double idemtol = 1e-10;
+  int minit = 0;
+  int maxit = 100;
+  int rel = 0;
+  int absconv = 0;
+  int icount = 0;
+  int core = 0;
+  const int* pp_in = nullptr;
+  const double* vv_in = nullptr;
+  bool use_gershgorin = true;  // false: normalise with (mineval, maxeval)
+  bool want_bounds = false;
+  double mineval = 0.0;
+  double maxeval = 0.0;
+  int precision = 0;
+};
+
+struct Sp2Result {
+  int iter = 0;
+  int status = 0;
+  double trx = 0.0;
+  double mineval = 0.0;
+  double maxeval = 0.0;
+  int pp[kMaxIter] = {};
+  double vv[kMaxIter] = {};
+};
+
+// Runs one SP2 variant: out = 2 X_final.  For V_RESPONSE, H1 is the
+// perturbation and out1 receives rho1 = 2 Delta_final.
+int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res,
+            const DMat* H1 = nullptr, DMat* out1 = nullptr);
+
+// Mixed-precision fused SP2 step (prec = PRG_PREC_F16SPLIT or PRG_PREC_INT8X6).
+int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
+                   int* ntiles, int prec);
+
+int precision_mode();
+void set_precision_mode(int m);
+bool fortran_rel(const char* s);
+bool fortran_abs(const char* s);
+
+}  // namespace prg

@@ -1,0 +1,341 @@
+"""Python binding of libprogress_gpu.so (MI355X density-matrix solvers).
+
+Thin ctypes layer over the C-ABI declared in include/progress_gpu.h.  The
+functions below mirror the reference's routines (names, argument meaning and
+order of src/prg_sp2_mod.F90 & co.) on numpy arrays: each call imports the
+Hamiltonian into a library matrix handle, runs the HIP path and exports the
+result.  There is no CPU fallback: if the shared library is missing, or no
+GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE.parent / "lib" / "libprogress_gpu.so"
+
+PREC_FP64 = 0
+PREC_F16SPLIT = 1
+PREC_INT8X6 = 2
+
+_dense, _double_real, _sequential, _col_major = 1, 2, 0, 1
+
+
+class ProgressError(RuntimeError):
+    """A library call recorded a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[status {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `make -C qmd-progress_amd` (or "
+                          "__graft_entry__.build()) first; there is no CPU fallback")
+    # One HIP runtime per process: when PyTorch is installed, load it first so
+    # that this library's libamdhip64.so.7 dependency resolves to the runtime
+    # torch already mapped (torch bundles its own copy; two copies in one
+    # process cannot share the device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(str(LIB_PATH))
+    vp, i, d, ip, dp, cp = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int), \
+        C.POINTER(C.c_double), C.c_char_p
+    sig = {
+        "bml_zero_matrix": (vp, [i, i, i, i, i]),
+        "bml_import_from_dense": (vp, [i, i, i, i, i, vp, d, i]),
+        "bml_export_to_dense": (vp, [vp, i]),
+        "bml_deallocate": (None, [C.POINTER(vp)]),
+        "bml_get_N": (i, [vp]),
+        "bml_trace": (d, [vp]),
+        "bml_read_bml_matrix": (None, [vp, cp]),
+        "prg_last_error_code": (i, []),
+        "prg_last_error": (cp, []),
+        "prg_progress_init": (None, []),
+        "prg_progress_shutdown": (None, []),
+        "prg_set_precision": (None, [i]),
+        "prg_get_precision": (i, []),
+        "prg_gershgorin": (None, [vp, dp]),
+        "prg_normalize": (None, [vp]),
+        "prg_normalize_fermi": (None, [vp, d, d, d]),
+        "prg_normalize_implicit_fermi": (None, [vp, d, d]),
+        "prg_sp2_basic": (None, [vp, vp, d, d, i, i, cp, d, i]),
+        "prg_sp2_alg2": (None, [vp, vp, d, d, i, i, cp, d, i]),
+        "prg_sp2_alg1": (None, [vp, vp, d, d, i, i, cp, d, i]),
+        "prg_sp2_alg2_genseq": (None, [vp, vp, d, d, i, i, cp, d, ip, ip, dp, i]),
+        "prg_sp2_alg1_genseq": (None, [vp, vp, d, d, i, i, cp, d, ip, ip, dp]),
+        "prg_sp2_alg2_seq": (None, [vp, vp, d, ip, ip, dp, i]),
+        "prg_sp2_alg1_seq": (None, [vp, vp, d, ip, ip, dp]),
+        "prg_prg_sp2_alg2_seq_inplace": (None, [vp, d, i, ip, ip, dp, d, d]),
+        "prg_prg_sp2_alg1_seq_inplace": (None, [vp, d, i, ip, ip, dp, d, d]),
+        "prg_sp2_submatrix": (None, [vp, vp, d, ip, ip, dp, d, d, i]),
+        "prg_sp2_submatrix_inplace": (None, [vp, d, ip, ip, dp, d, d, i]),
+        "prg_homolumogap": (None, [dp, i, ip, d, d, dp, dp, dp, i]),
+        "prg_sp2sequence": (None, [ip, ip, d, d, d, d, d, i]),
+        "prg_gpu_sp2_alg2_device": (i, [i, vp, vp, d, i, i, cp, d, i, ip, ip]),
+        "prg_gpu_padded_ld": (i, [i]),
+        "prg_gpu_timing": (None, [i]),
+        "prg_gpu_step_timing": (i, [dp, ip]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check():
+    L = lib()
+    code = L.prg_last_error_code()
+    if code != 0:
+        raise ProgressError(code, L.prg_last_error().decode())
+
+
+def _free(ptr):
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free(ptr)
+
+
+class Matrix:
+    """Owning wrapper of a library matrix handle (dense FP64)."""
+
+    def __init__(self, handle):
+        if not handle:
+            _check()
+            raise ProgressError(-1, "null matrix handle")
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def zeros(cls, n: int) -> "Matrix":
+        h = lib().bml_zero_matrix(_dense, _double_real, n, n, _sequential)
+        _check()
+        return cls(h)
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "Matrix":
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        n = a.shape[0]
+        if a.shape != (n, n):
+            raise ValueError("square matrix expected")
+        h = lib().bml_import_from_dense(_dense, _double_real, _col_major, n, n,
+                                        a.ctypes.data_as(C.c_void_p), 0.0, _sequential)
+        _check()
+        return cls(h)
+
+    @property
+    def n(self) -> int:
+        return lib().bml_get_N(self.h)
+
+    def numpy(self) -> np.ndarray:
+        n = self.n
+        p = lib().bml_export_to_dense(self.h, _col_major)
+        _check()
+        buf = (C.c_double * (n * n)).from_address(p)
+        out = np.frombuffer(buf, dtype=np.float64).copy().reshape((n, n), order="F")
+        _free(p)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().bml_deallocate(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_precision(mode: int):
+    lib().prg_set_precision(mode)
+    _check()
+
+
+def _ivec(n, init=None):
+    a = np.zeros(max(n, 1), dtype=np.int32)
+    if init is not None:
+        a[:len(init)] = init
+    return a
+
+
+def _dvec(n, init=None):
+    a = np.zeros(max(n, 1), dtype=np.float64)
+    if init is not None:
+        a[:len(init)] = init
+    return a
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def gershgorin(h: np.ndarray):
+    m = Matrix.from_numpy(h)
+    g = _dvec(2)
+    lib().prg_gershgorin(m.h, _dp(g))
+    _check()
+    return float(g[0]), float(g[1])
+
+
+def _h_rho(fn, h, *args):
+    mh = Matrix.from_numpy(h)
+    mr = Matrix.zeros(h.shape[0])
+    fn(mh.h, mr.h, *args)
+    _check()
+    return mr.numpy()
+
+
+def sp2_basic(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):
+    """prg_sp2_basic (src/prg_sp2_mod.F90:49-135)."""
+    return _h_rho(lib().prg_sp2_basic, h, threshold, bndfil, minsp2iter, maxsp2iter,
+                  sp2conv.encode(), idemtol, verbose)
+
+
+def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):
+    """prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380)."""
+    return _h_rho(lib().prg_sp2_alg2, h, threshold, bndfil, minsp2iter, maxsp2iter,
+                  sp2conv.encode(), idemtol, verbose)
+
+
+def sp2_alg1(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):
+    """prg_sp2_alg1 (src/prg_sp2_mod.F90:734-864)."""
+    return _h_rho(lib().prg_sp2_alg1, h, threshold, bndfil, minsp2iter, maxsp2iter,
+                  sp2conv.encode(), idemtol, verbose)
+
+
+def sp2_alg2_genseq(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0,
+                    verbose=0):
+    """prg_sp2_alg2_genseq: returns (rho, pp, icount, vv) with vv[:icount] and,
+    when the loop ended on a discarded product, vv[icount]."""
+    pp, vv, ic = _ivec(maxsp2iter), _dvec(maxsp2iter), _ivec(1)
+    rho = _h_rho(lib().prg_sp2_alg2_genseq, h, threshold, bndfil, minsp2iter, maxsp2iter,
+                 sp2conv.encode(), idemtol, _ip(pp), _ip(ic), _dp(vv), verbose)
+    icount = int(ic[0])
+    return rho, pp[:icount].tolist(), icount, vv
+
+
+def sp2_alg1_genseq(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0):
+    """prg_sp2_alg1_genseq: returns (rho, pp, icount, vv)."""
+    pp, vv, ic = _ivec(maxsp2iter), _dvec(maxsp2iter), _ivec(1)
+    rho = _h_rho(lib().prg_sp2_alg1_genseq, h, threshold, bndfil, minsp2iter, maxsp2iter,
+                 sp2conv.encode(), idemtol, _ip(pp), _ip(ic), _dp(vv))
+    icount = int(ic[0])
+    return rho, pp[:icount].tolist(), icount, vv
+
+
+def sp2_alg2_seq(h, pp, icount, threshold=0.0):
+    """prg_sp2_alg2_seq: returns (rho, vv)."""
+    ppa, vv, ic = _ivec(icount, pp), _dvec(icount), _ivec(1, [icount])
+    rho = _h_rho(lib().prg_sp2_alg2_seq, h, threshold, _ip(ppa), _ip(ic), _dp(vv), 0)
+    return rho, vv[:icount].tolist()
+
+
+def sp2_alg1_seq(h, pp, icount, threshold=0.0):
+    """prg_sp2_alg1_seq: returns (rho, vv)."""
+    ppa, vv, ic = _ivec(icount, pp), _dvec(icount), _ivec(1, [icount])
+    rho = _h_rho(lib().prg_sp2_alg1_seq, h, threshold, _ip(ppa), _ip(ic), _dp(vv))
+    return rho, vv[:icount].tolist()
+
+
+def _inplace(fn, x, pp, icount, mineval, maxeval, threshold):
+    m = Matrix.from_numpy(x)
+    ppa, vv, ic = _ivec(icount, pp), _dvec(icount), _ivec(1, [icount])
+    fn(m.h, threshold, icount, _ip(ppa), _ip(ic), _dp(vv), mineval, maxeval)
+    _check()
+    return m.numpy(), vv[:icount].tolist()
+
+
+def sp2_alg2_seq_inplace(x, pp, icount, mineval, maxeval, threshold=0.0):
+    """prg_prg_sp2_alg2_seq_inplace: returns (rho, vv)."""
+    return _inplace(lib().prg_prg_sp2_alg2_seq_inplace, x, pp, icount, mineval, maxeval,
+                    threshold)
+
+
+def sp2_alg1_seq_inplace(x, pp, icount, mineval, maxeval, threshold=0.0):
+    """prg_prg_sp2_alg1_seq_inplace: returns (rho, vv)."""
+    return _inplace(lib().prg_prg_sp2_alg1_seq_inplace, x, pp, icount, mineval, maxeval,
+                    threshold)
+
+
+def sp2_submatrix(x, pp, icount, vv, mineval, maxeval, core_size, threshold=0.0):
+    """prg_sp2_submatrix_inplace: returns (rho, vv) with vv accumulated."""
+    m = Matrix.from_numpy(x)
+    ppa, vva, ic = _ivec(icount, pp), _dvec(icount, vv), _ivec(1, [icount])
+    lib().prg_sp2_submatrix_inplace(m.h, threshold, _ip(ppa), _ip(ic), _dp(vva), mineval,
+                                    maxeval, core_size)
+    _check()
+    return m.numpy(), vva[:icount].tolist()
+
+
+def homolumogap(vv, imax, pp, mineval, maxeval):
+    """prg_homolumogap: returns (ehomo, elumo, egap)."""
+    vva, ppa = _dvec(len(vv), vv), _ivec(len(pp), pp)
+    out = _dvec(3)
+    o = [C.c_double() for _ in range(3)]
+    lib().prg_homolumogap(_dp(vva), imax, _ip(ppa), mineval, maxeval, C.byref(o[0]),
+                          C.byref(o[1]), C.byref(o[2]), 0)
+    _check()
+    del out
+    return o[0].value, o[1].value, o[2].value
+
+
+def sp2sequence(mineval, maxeval, ehomo, elumo, errlimit):
+    """prg_sp2sequence: returns (pp, imax)."""
+    pp, im = _ivec(100), _ivec(1)
+    lib().prg_sp2sequence(_ip(pp), _ip(im), mineval, maxeval, ehomo, elumo, errlimit, 0)
+    _check()
+    return pp[:int(im[0])].tolist(), int(im[0])
+
+
+def padded_ld(n: int) -> int:
+    return lib().prg_gpu_padded_ld(n)
+
+
+def sp2_alg2_device(dH, dRho, n, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol,
+                    precision=PREC_FP64):
+    """Device-pointer prg_sp2_alg2: dH, dRho are torch CUDA tensors of shape
+    (ld, ld), ld = padded_ld(n), zero outside the n x n block.  Returns
+    (iterations, pp).  Synchronises torch's stream before the call."""
+    import torch
+    torch.cuda.synchronize()
+    pp = _ivec(maxsp2iter)
+    it = _ivec(1)
+    st = lib().prg_gpu_sp2_alg2_device(n, C.c_void_p(dH.data_ptr()), C.c_void_p(dRho.data_ptr()),
+                                       bndfil, minsp2iter, maxsp2iter, sp2conv.encode(), idemtol,
+                                       precision, _ip(it), _ip(pp))
+    if st != 0:
+        raise ProgressError(st, lib().prg_last_error().decode())
+    return int(it[0]), pp[:int(it[0])].tolist()
+
+
+def step_timing(enable=None):
+    """Enable/reset (enable=True/False) or read (enable=None) the product-kernel
+    timers: returns (ms_sum, count)."""
+    if enable is not None:
+        lib().prg_gpu_timing(1 if enable else 0)
+        return None
+    ms = C.c_double()
+    cnt = C.c_int()
+    lib().prg_gpu_step_timing(C.byref(ms), C.byref(cnt))
+    return ms.value, cnt.value

@@ -1,0 +1,316 @@
+"""Pins of the CPU oracle (oracle/progress_oracle.py) -- CPU only.
+
+Every check here is independent of the oracle's own formulas: eigenvalue
+closed forms (the SP2 and implicit-Fermi recursions are polynomials /
+rational maps of H), the T=0 eigendecomposition density, the reference's
+known-answer tests and shipped fixtures, divided differences and finite
+differences.  A planted-mistake test shows the closed-form check catches a
+plausible error.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import progress_oracle as O
+
+
+def eig(h):
+    return np.linalg.eigh(h)
+
+
+@pytest.fixture(scope="module")
+def model256():
+    return O.model_hamiltonian(256, seed=0)
+
+
+@pytest.fixture(scope="module")
+def poly16(golden):
+    h = O.read_mtx(str(golden / "poly.16.mtx"))
+    return 0.5 * (h + h.T)
+
+
+@pytest.fixture(scope="module")
+def ortho384(golden):
+    h = O.read_mtx(str(golden / "hamiltonian_ortho_384.mtx"))
+    return 0.5 * (h + h.T)
+
+
+def test_model_hamiltonian_definition():
+    h = O.model_hamiltonian(200, seed=3)
+    assert np.array_equal(h, h.T)
+    i, j = np.triu_indices(200, 65)
+    assert np.all(h[i, j] == 0.0)
+    d = np.diag(h)
+    assert d.min() >= -1.0 and d.max() <= 1.0
+    # first neighbour hopping ~ -exp(-1/8) (1 + 0.1 z)
+    nn = np.diag(h, 1)
+    assert abs(nn.mean() + math.exp(-1 / 8)) < 0.02
+
+
+def test_fixture_reader(golden):
+    h = O.read_mtx(str(golden / "poly.16.mtx"))
+    assert h.shape == (192, 192)
+    assert h[0, 0] == -13.294
+    t = O.read_mtx(str(golden / "hamiltonian-twolevel-ref.mtx"))
+    assert t.shape == (10, 10) and t[0, 0] == -2.0 and t[0, 1] == -1.0
+
+
+@pytest.mark.parametrize("which", ["model256", "poly16", "ortho384"])
+def test_sp2_alg2_closed_form(which, request):
+    h = request.getfixturevalue(which)
+    n = h.shape[0]
+    w, q = eig(h)
+    lo, hi = O.gershgorin(h)
+    rho, it, pp = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    x, ite, ppe = O.sp2_alg2_eigen(w, lo, hi, 0.5 * n, 10, 200, "Abs", 1e-10)
+    assert it == ite and pp == ppe
+    assert np.linalg.norm(rho - (q * (2 * x)) @ q.T) < 1e-10
+    d0, _ = O.build_density_T0(h, 0.5)
+    assert np.linalg.norm(rho - d0) < 1e-8
+    assert abs(np.trace(rho) - n) < 1e-9
+
+
+def test_gershgorin_bounds_contain_spectrum(model256):
+    w, _ = eig(model256)
+    lo, hi = O.gershgorin(model256)
+    assert lo <= w.min() and w.max() <= hi
+    # a diagonal matrix's Gershgorin interval is its exact spectrum
+    d = np.diag([3.0, -2.0, 1.0])
+    assert O.gershgorin(d) == (-2.0, 3.0)
+
+
+@pytest.mark.parametrize("fn", ["alg1", "basic", "alg2_genseq", "alg1_genseq"])
+def test_sp2_flavours_reach_T0_density(fn, model256):
+    h = model256
+    d0, _ = O.build_density_T0(h, 0.5)
+    if fn == "alg1":
+        rho = O.sp2_alg1(h, 0.5, 10, 200, "Abs", 1e-10)[0]
+    elif fn == "basic":
+        rho = O.sp2_basic(h, 0.5, 10, 200, "Abs", 1e-10)[0]
+    elif fn == "alg2_genseq":
+        rho = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)[0]
+    else:
+        rho = O.sp2_alg1_genseq(h, 0.5, 10, 200, "Abs", 1e-10)[0]
+    assert np.linalg.norm(rho - d0) < 1e-8
+
+
+def test_rel_stop_and_idempotency_of_reference_settings(ortho384):
+    # the reference tests run "Rel" with minsp2iter 25 and check idempotency
+    # of rho/2 (tests/src/main.F90:86-91,260-335).  This shipped 384-orbital
+    # fixture has its spectrum clustered at 0 and 1, so half filling is the
+    # gapped occupation (bndfil 2/3 lands inside the upper cluster).
+    rho, it, pp = O.sp2_alg2(ortho384, 0.5, 10, 100, "Rel", 1e-10)
+    assert O.check_idempotency(0.5 * rho) < 1e-8
+    rho1, it1, _, _ = O.sp2_alg1(ortho384, 0.5, 10, 100, "Rel", 1e-10)
+    assert O.check_idempotency(0.5 * rho1) < 1e-8
+
+
+def test_replay_variants_reproduce_genseq(model256):
+    h = model256
+    rho, pp, ic, vv = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    r2, vv2 = O.sp2_alg2_seq(h, pp, ic)
+    assert np.array_equal(r2, rho) and vv2 == vv[:ic]
+    lo, hi = O.gershgorin(h)
+    r3, vv3 = O.sp2_alg2_seq_inplace(h, pp, ic, lo, hi)
+    assert np.abs(r3 - rho).max() < 1e-13
+    r4, vv4 = O.sp2_alg1_seq(h, pp, ic)
+    assert np.abs(r4 - rho).max() < 1e-10
+    assert np.allclose(vv4, vv2, rtol=1e-6, atol=1e-12)
+    # submatrix with every row core = alg1 replay, vv = squared Frobenius norms
+    r5, vv5 = O.sp2_submatrix(h, pp, ic, [0.0] * ic, lo, hi, h.shape[0])
+    assert np.abs(r5 - r4).max() < 1e-12
+    assert np.allclose(np.sqrt(vv5), vv4, rtol=1e-9, atol=1e-14)
+
+
+def test_homolumogap_recovers_gap(model256):
+    h = model256
+    n = h.shape[0]
+    w, _ = eig(h)
+    lo, hi = O.gershgorin(h)
+    rho, pp, ic, vv = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    eh, el, eg = O.homolumogap(vv, ic, pp, lo, hi)
+    assert abs(eh - w[n // 2 - 1]) < 1e-3 * (hi - lo)
+    assert abs(el - w[n // 2]) < 1e-3 * (hi - lo)
+    assert abs(eg - (w[n // 2] - w[n // 2 - 1])) / (w[n // 2] - w[n // 2 - 1]) < 0.05
+
+
+def test_sp2sequence_converges_homo_lumo(model256):
+    h = model256
+    n = h.shape[0]
+    w, _ = eig(h)
+    lo, hi = O.gershgorin(h)
+    ehomo, elumo = w[n // 2 - 1], w[n // 2]
+    pp, imax = O.sp2sequence(lo, hi, ehomo, elumo, 1e-12)
+    assert imax < 100
+    x = O.sp2_poly_eigen(np.array([ehomo, elumo]), lo, hi, pp)
+    # homo pushed to 1, lumo to 0 within the error limit
+    assert abs(1 - x[0]) + abs(x[1]) <= 1e-12
+    # and the sequence purifies the whole spectrum to the T=0 occupations
+    xs = O.sp2_poly_eigen(w, lo, hi, pp)
+    assert np.abs(xs - np.r_[np.ones(n // 2), np.zeros(n // 2)]).max() < 1e-6
+
+
+def test_partition_kats(golden):
+    # tests/src/main.F90:807-843
+    assert len(O.equal_partition(6, 72)) == 12
+    assert len(O.equal_partition(7, 72)) == 11
+    parts7 = O.equal_partition(7, 72)
+    assert [len(p) for p in parts7] == [7] * 10 + [2]
+    name, total, parts = O.read_part(str(golden / "test.part"))
+    assert len(parts) == 104 and total == 673
+    assert sorted(sum(parts, [])) == list(range(673))
+
+
+def test_subgraph_loop_block_diagonal_is_exact():
+    # independent blocks with zero halo: the graph loop equals full SP2 with
+    # the same sequence (special case of src/prg_subgraphloop_mod.F90:36-126)
+    rng = np.random.default_rng(4)
+    blocks = [O.model_hamiltonian(48, seed=s) for s in range(3)]
+    n = 144
+    h = np.zeros((n, n))
+    for b, blk in enumerate(blocks):
+        h[48 * b:48 * b + 48, 48 * b:48 * b + 48] = blk
+    lo, hi = O.gershgorin(h)
+    rho_full, pp, ic, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    g = (np.abs(rho_full) > 1e-12).astype(float)
+    parts = O.equal_partition(24, n)
+    rho, vv = O.subgraph_sp2_loop(h, g, parts, pp, ic, lo, hi)
+    ref, vvref = O.sp2_alg2_seq(h, pp, ic)
+    assert np.abs(rho - ref).max() < 1e-10
+    # core-row vv sums add up to the whole-matrix Frobenius norms
+    assert np.allclose(vv, vvref, rtol=1e-8, atol=1e-12)
+    del rng
+
+
+def test_implicit_fermi_closed_form(model256):
+    h = model256
+    n = h.shape[0]
+    w, q = eig(h)
+    beta, nsteps, mu = 20.0, 8, 0.6
+    p, mu_new = O.implicit_fermi(h, nsteps, 2, 0.5 * n, mu, beta, 0, 1, 1e-9, 1e-18)
+    x = O.implicit_fermi_eigen(w, mu, beta, nsteps)
+    assert np.linalg.norm(p - (q * x) @ q.T) < 1e-7
+    # the recursion approximates the Fermi function
+    f = O.fermi(w, mu, beta)
+    assert np.abs(x - f).max() < 2e-2
+    x12 = O.implicit_fermi_eigen(w, mu, beta, 12)
+    assert np.abs(x12 - f).max() < np.abs(x - f).max()
+
+
+def test_newtonschulz_inverse():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((40, 40))
+    a = a @ a.T / 40 + np.eye(40)
+    ai0 = np.eye(40) / np.linalg.norm(a, 2)
+    ai, it = O.newtonschulz(a, ai0, 1e-12)
+    assert np.linalg.norm(ai @ a - np.eye(40)) < 1e-11
+    assert it > 3
+
+
+def test_implicit_fermi_save_inverse_matches_fermi(model256):
+    h = model256
+    n = h.shape[0]
+    w, q = eig(h)
+    beta, nsteps = 20.0, 8
+    inv = [np.eye(n)] * nsteps
+    p, mu, inv, it, ns = O.implicit_fermi_save_inverse(inv, h, nsteps, 0.5 * n, 0.6, beta,
+                                                      1e-8, 1e-10, 1)
+    assert abs(np.trace(p) / 2 - 0.5 * n) < 1e-6
+    x = O.implicit_fermi_eigen(w, mu, beta, nsteps)
+    assert np.linalg.norm(p - 2 * (q * x) @ q.T) < 1e-7
+
+
+def _exact_inverses(h, mu, beta, nsteps):
+    n = h.shape[0]
+    cnst = beta / 2.0 ** (nsteps + 2)
+    p = O.normalize_implicit_fermi(h, cnst, mu)
+    inv = []
+    for _ in range(nsteps):
+        p2 = p @ p
+        y = 2 * (p2 - p) + np.eye(n)
+        yi = np.linalg.inv(y)
+        inv.append(yi)
+        p = yi @ p2
+    return inv
+
+
+def test_first_order_response_divided_differences():
+    h = O.model_hamiltonian(96, seed=2)
+    h1 = O.banded_perturbation(96, 8, seed=5)
+    w, q = eig(h)
+    beta, nsteps, mu = 10.0, 6, 0.6
+    inv = _exact_inverses(h, mu, beta, nsteps)
+    p0, p1 = O.implicit_fermi_first_order_response(h, h1, inv, nsteps, mu, beta)
+    # Daleckii-Krein response of the scalar map (before the trace correction)
+    f = lambda e: O.implicit_fermi_eigen(e, mu, beta, nsteps)
+    eps = 1e-6
+    df = lambda e: (f(e + eps) - f(e - eps)) / (2 * eps)
+    k = O.divided_differences(f, df, w, tol=1e-7)
+    h1e = q.T @ h1 @ q
+    dk = k * h1e
+    x0 = f(w)
+    dpdmu = beta * x0 * (1 - x0)
+    mu1 = -np.trace(dk) / dpdmu.sum()
+    ref = q @ (dk + np.diag(mu1 * dpdmu)) @ q.T
+    assert np.linalg.norm(p0 - (q * x0) @ q.T) < 1e-9
+    assert np.linalg.norm(p1 - ref) / np.linalg.norm(ref) < 1e-6
+    # canonical eigenbasis routine gives the same response
+    c1 = O.canon_response(h1e, 48, beta, w, mu, nsteps)
+    # prg_canon_response starts from p0 = 0.5 + c(h - mu) (reflected); the
+    # recursion is reflection symmetric, so P1 changes sign except for the
+    # trace correction which is recomputed: compare through the traceless part
+    assert np.allclose(np.abs(c1), np.abs(q.T @ p1 @ q), rtol=1e-6, atol=1e-9)
+
+
+def test_response_sp2_divided_differences():
+    h = O.model_hamiltonian(96, seed=1)
+    h1 = O.banded_perturbation(96, 8, seed=7)
+    n = h.shape[0]
+    w, q = eig(h)
+    lo, hi = O.gershgorin(h)
+    rho, rho1, it, pp = O.compute_response_sp2(h, h1, 0.5, 10, 200, "Abs", 1e-10)
+    # composite map e -> 2 p((hi - e)/(hi - lo)) with the executed sequence
+    f = lambda e: 2 * O.sp2_poly_eigen(e, lo, hi, pp)
+    eps = 1e-7
+    df = lambda e: (f(e + eps) - f(e - eps)) / (2 * eps)
+    k = O.divided_differences(f, df, w, tol=1e-9)
+    ref = q @ (k * (q.T @ h1 @ q)) @ q.T
+    assert np.linalg.norm(rho1 - ref) / np.linalg.norm(ref) < 1e-6
+    # finite differences with frozen sequence and bounds
+    hstep = 1e-6
+    rp, _ = O.sp2_alg2_seq_inplace(h + hstep * h1, pp, len(pp), lo, hi)
+    rm, _ = O.sp2_alg2_seq_inplace(h - hstep * h1, pp, len(pp), lo, hi)
+    fd = (rp - rm) / (2 * hstep)
+    assert np.linalg.norm(rho1 - fd) / np.linalg.norm(fd) < 1e-4
+    assert abs(np.trace(rho) - n) < 1e-8
+
+
+def test_planted_normalisation_sign_error_is_caught(model256, monkeypatch):
+    """A plausible mistake (normalising with (H - lo I) instead of (hi I - H))
+    must fail the closed-form pin."""
+    h = model256
+    n = h.shape[0]
+    w, q = eig(h)
+    lo, hi = O.gershgorin(h)
+
+    def wrong_normalize(hh):
+        a, b = O.gershgorin(hh)
+        return (hh - a * np.eye(hh.shape[0])) / (b - a)
+
+    monkeypatch.setattr(O, "normalize", wrong_normalize)
+    rho, it, pp = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    d0, _ = O.build_density_T0(h, 0.5)
+    assert np.linalg.norm(rho - d0) > 1.0
+
+
+def test_planted_branch_swap_is_caught(model256):
+    """Swapping the SP2 branches (X^2 <-> 2X - X^2) must change the sequence
+    check against the eigenvalue closed form."""
+    h = model256
+    w, q = eig(h)
+    lo, hi = O.gershgorin(h)
+    rho, ic, pp = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    swapped = [1 - s for s in pp]
+    x = O.sp2_poly_eigen(w, lo, hi, swapped)
+    assert np.linalg.norm(rho - (q * (2 * x)) @ q.T) > 1.0

@@ -1,0 +1,141 @@
+"""GPU parity of the SP2 family (HIP path through the C-ABI) against the oracle.
+
+Bar: the same branch sequence and iteration count as the oracle, and the FP64
+density within 1e-9 (Frobenius) of the oracle's on the same input; the
+reference's own tolerance tests (idempotency of rho/2) on its shipped
+fixtures.
+"""
+import numpy as np
+import pytest
+
+import progress_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def P():
+    import progress_amd
+    progress_amd.set_precision(progress_amd.PREC_FP64)
+    return progress_amd
+
+
+@pytest.mark.parametrize("n,seed", [(256, 0), (300, 1), (1024, 0)])
+def test_sp2_alg2_matches_oracle(P, n, seed):
+    print(f"[sp2_alg2] n={n} seed={seed}", flush=True)
+    h = O.model_hamiltonian(n, seed)
+    rho_o, it_o, pp_o = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    rho_g = P.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert np.linalg.norm(rho_g - rho_o) < 1e-9 * max(1.0, n / 256)
+    d0, _ = O.build_density_T0(h, 0.5)
+    assert np.linalg.norm(rho_g - d0) < 1e-8 * max(1.0, n / 256)
+    assert np.array_equal(rho_g, rho_g.T)
+
+
+@pytest.mark.parametrize("n", [200, 256])
+def test_genseq_sequence_and_vv(P, n):
+    print(f"[genseq] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 2)
+    rho_o, pp_o, ic_o, vv_o = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    rho_g, pp_g, ic_g, vv_g = P.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert ic_g == ic_o and pp_g == pp_o
+    assert np.allclose(vv_g[:len(vv_o)], vv_o, rtol=1e-9, atol=1e-11)
+    assert np.linalg.norm(rho_g - rho_o) < 1e-9
+    rho1_o, pp1_o, ic1_o, vv1_o = O.sp2_alg1_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    rho1_g, pp1_g, ic1_g, vv1_g = P.sp2_alg1_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert ic1_g == ic1_o and pp1_g == pp1_o
+    assert np.allclose(vv1_g[:ic1_o], vv1_o[:ic1_o], rtol=1e-9, atol=1e-11)
+    assert np.linalg.norm(rho1_g - rho1_o) < 1e-9
+
+
+def test_alg1_basic_match_oracle(P):
+    print("[alg1/basic] n=256", flush=True)
+    h = O.model_hamiltonian(256, 0)
+    r1o = O.sp2_alg1(h, 0.5, 10, 200, "Abs", 1e-10)[0]
+    r1g = P.sp2_alg1(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert np.linalg.norm(r1g - r1o) < 1e-9
+    rbo = O.sp2_basic(h, 0.5, 10, 200, "Abs", 1e-10)[0]
+    rbg = P.sp2_basic(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert np.linalg.norm(rbg - rbo) < 1e-9
+
+
+def test_rel_convergence_matches_oracle(P, golden):
+    print("[rel] ortho384", flush=True)
+    h = O.read_mtx(str(golden / "hamiltonian_ortho_384.mtx"))
+    h = 0.5 * (h + h.T)
+    for fo, fg in [(O.sp2_alg2, P.sp2_alg2), (O.sp2_alg1, P.sp2_alg1)]:
+        ro = fo(h, 0.5, 10, 100, "Rel", 1e-10)[0]
+        rg = fg(h, 0.5, 10, 100, "Rel", 1e-10)
+        assert np.linalg.norm(rg - ro) < 1e-9
+        assert O.check_idempotency(0.5 * rg) < 1e-8
+
+
+def test_replay_and_inplace_variants(P):
+    print("[replay] n=192", flush=True)
+    h = O.model_hamiltonian(192, 5)
+    lo, hi = O.gershgorin(h)
+    glo, ghi = P.gershgorin(h)
+    assert (glo, ghi) == pytest.approx((lo, hi), rel=1e-14, abs=1e-14)
+    rho, pp, ic, vv = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    for fo, fg in [(O.sp2_alg2_seq, P.sp2_alg2_seq), (O.sp2_alg1_seq, P.sp2_alg1_seq)]:
+        ro, vo = fo(h, pp, ic)
+        rg, vg = fg(h, pp, ic)
+        assert np.linalg.norm(rg - ro) < 1e-9
+        assert np.allclose(vg, vo, rtol=1e-9, atol=1e-11)
+    for fo, fg in [(O.sp2_alg2_seq_inplace, P.sp2_alg2_seq_inplace),
+                   (O.sp2_alg1_seq_inplace, P.sp2_alg1_seq_inplace)]:
+        ro, vo = fo(h, pp, ic, lo, hi)
+        rg, vg = fg(h, pp, ic, lo, hi)
+        assert np.linalg.norm(rg - ro) < 1e-9
+        assert np.allclose(vg, vo, rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("core", [0, 37, 192])
+def test_submatrix_core_rows(P, core):
+    print(f"[submatrix] core={core}", flush=True)
+    h = O.model_hamiltonian(192, 6)
+    lo, hi = O.gershgorin(h)
+    rho, pp, ic, vv = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    vv0 = [0.25] * ic  # accumulation onto existing values
+    ro, vo = O.sp2_submatrix(h, pp, ic, vv0, lo, hi, core)
+    rg, vg = P.sp2_submatrix(h, pp, ic, vv0, lo, hi, core)
+    assert np.linalg.norm(rg - ro) < 1e-9
+    assert np.allclose(vg, vo, rtol=1e-9, atol=1e-12)
+
+
+def test_reference_poly512_idempotency(P, golden):
+    """tests/src/main.F90:336-394: poly.512.mtx, bndfil 0.5, Rel, minsp2iter 25,
+    idempotency of rho/2 <= 1e-6 for alg1 and alg2."""
+    print("[poly512] n=6144", flush=True)
+    h = O.read_mtx(str(golden / "poly.512.mtx.gz"))
+    for fg in (P.sp2_alg2, P.sp2_alg1):
+        rho = fg(h, 0.5, 25, 100, "Rel", 1e-10)
+        x = 0.5 * rho
+        assert np.linalg.norm(x @ x - x) < 1e-6
+        assert abs(np.trace(rho) - 6144) < 1e-6
+
+
+def test_device_pointer_entry_point(P):
+    import torch
+    print("[device api] n=512", flush=True)
+    n = 512
+    h = O.model_hamiltonian(n, 0)
+    ld = P.padded_ld(n)
+    hd = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    hd[:n, :n] = torch.from_numpy(h)
+    rd = torch.zeros_like(hd)
+    it, pp = P.sp2_alg2_device(hd, rd, n, 0.5, 10, 200, "Abs", 1e-10)
+    rho_o, it_o, pp_o = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert it == it_o and pp == pp_o
+    r = rd.cpu().numpy()
+    assert np.linalg.norm(r[:n, :n] - rho_o) < 1e-9
+    assert np.all(r[n:, :] == 0) and np.all(r[:, n:] == 0)
+
+
+def test_bad_inputs_are_errors(P):
+    print("[errors]", flush=True)
+    with pytest.raises(P.ProgressError):
+        P.sp2_alg2(np.zeros((4, 4)), 0.5, 1, 10, "Abs", 1e-10)  # zero-width interval
+    with pytest.raises(P.ProgressError):
+        # maxsp2iter too small: the reference calls stop here
+        P.sp2_alg2(O.model_hamiltonian(128, 0), 0.5, 1, 5, "Abs", 1e-10)
