@@ -223,6 +223,21 @@ __device__ void sp2_plan(Sp2State* s) {
       if (s->iter >= s->maxit) { s->done = 1; return; }
       const double trx = (v == V_ALG2_GENSEQ) ? s->trx_fresh : s->trx;
       const double trx2 = s->fro2;
+      if (s->refine_phase && s->refine_steps >= s->refine_max) { s->done = 1; return; }
+      if (s->noise_stop && !s->refine_phase) {
+        // low-precision products: stop where the idempotency error stops
+        // falling quadratically (numerical noise floor)
+        const double e = trx - trx2;
+        const int it0 = s->iter;  // 0-based index of the step being planned
+        const bool alt = it0 > 2 && s->pp[it0 - 1] != s->pp[it0 - 2];
+        s->ie2 = s->ie1;
+        s->ie1 = s->ie0;
+        s->ie0 = e;
+        if (e <= 0.0 || (alt && e >= 4.5 * s->ie2 * s->ie2)) {
+          s->done = 1;
+          return;
+        }
+      }
       const double tr2xx2 = 2.0 * trx - trx2;
       const double limdiff = fabs(trx2 - occ) - fabs(tr2xx2 - occ);
       s->trx2 = trx2;
@@ -246,6 +261,7 @@ __device__ void sp2_plan(Sp2State* s) {
         s->status = ERR_NOT_CONVERGED;
         last = 1;
       }
+      if (s->refine_phase && s->refine_steps + 1 >= s->refine_max) last = 1;
       s->last = last;
       s->apply = !stop;
       s->planned = !stop || (v == V_ALG2_GENSEQ);
@@ -370,7 +386,18 @@ __global__ void sp2_absorb_plan_kernel(Sp2State* s, const double* __restrict__ p
   s->e2 = s->e1;
   s->e1 = s->e0;
   s->e0 = e_new;
+  if (s->refine_phase && s->apply) s->refine_steps += 1;
   if (s->last) s->done = 1;
+  sp2_plan(s);
+}
+
+__global__ void sp2_begin_refine_kernel(Sp2State* s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (s->refine_max <= 0 || s->status != OK) return;
+  s->refine_phase = 1;
+  s->refine_steps = 0;
+  s->done = 0;
+  s->e0 = s->e1 = s->e2 = 0.0;
   sp2_plan(s);
 }
 
@@ -492,6 +519,12 @@ int sp2_init_state(Sp2State* d_state, const double* d_part, int nblocks) {
 int sp2_absorb_plan(Sp2State* d_state, const double* d_part, int ntiles) {
   hipLaunchKernelGGL(sp2_absorb_plan_kernel, dim3(1), dim3(NT), 0, stream(), d_state, d_part,
                      ntiles);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int sp2_begin_refine(Sp2State* d_state) {
+  hipLaunchKernelGGL(sp2_begin_refine_kernel, dim3(1), dim3(64), 0, stream(), d_state);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

@@ -1,14 +1,237 @@
-// Mixed-precision SP2 products (gfx950): placeholder until the split and
-// sliced kernels land; the selector rejects the modes loudly.
+// Mixed-precision SP2 products for gfx950.
+//
+// PRG_PREC_F16SPLIT: X (symmetric, |x| <= 1) is carried as an FP64 master and
+// two FP16 planes, hi = fp16(x) and lo = fp16((x - hi) * 2^11), so that
+// x ~= hi + 2^-11 lo (about 22 significant bits).  One fused kernel computes
+//   X^2 ~= hi*hi + 2^-11 (hi*lo + lo*hi)
+// with v_mfma_f32_16x16x32_f16 (FP32 accumulation, two accumulator sets),
+// applies the SP2 branch update in FP64 against the master copy, writes the
+// next master and its hi/lo planes, and emits the same trace / Frobenius
+// partial sums as the FP64 kernel, so the FP64 bookkeeping and stopping rules
+// are shared.  The dropped lo*lo term is 2^-22 relative.
+//
+// Because the planes are symmetric, the A operand (rows of the I panel) and
+// the B operand (columns of the J panel) are both read as contiguous column
+// chunks of a plane.
 #include "prg_sp2.h"
+#include "prg_tile.h"
 
 namespace prg {
+namespace {
+
+using tile::block_store_partials;
+using tile::sym_tile;
+using tile::xcd_remap;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BK = 64, NT = 256;
+constexpr int HROW = BK + 8;  // padded LDS row (halves): 144 bytes
+
+struct F16Args {
+  const double* buf[2];         // FP64 master ping-pong
+  const _Float16* hi[2];        // hi planes ping-pong
+  const _Float16* lo[2];        // lo planes ping-pong
+  int ld;
+  Sp2State* st;
+  double* part;
+};
+
+__global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
+  __shared__ _Float16 As[BM][HROW];
+  __shared__ _Float16 Bs[BM][HROW];
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
+  const int ld = g.ld;
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
+  const _Float16* hi = g.hi[cur];
+  const _Float16* lo = g.lo[cur];
+  _Float16* hin = const_cast<_Float16*>(g.hi[cur ^ 1]);
+  _Float16* lon = const_cast<_Float16*>(g.lo[cur ^ 1]);
+
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+
+  f4 acc_hh[4][4], acc_x[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      acc_hh[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+      acc_x[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+
+  // three segments over K: (hi,hi) -> acc_hh ; (hi,lo), (lo,hi) -> acc_x
+  const int ksteps = ld / BK;
+  const int total = 3 * ksteps;
+  const int c = tid >> 1, kh = (tid & 1) * 32;
+  i4 ra[4], rb[4];
+  auto load = [&](int s) {
+    const int seg = s / ksteps;
+    const int k0 = (s - seg * ksteps) * BK;
+    const _Float16* Ap = (seg == 2) ? lo : hi;
+    const _Float16* Bp = (seg == 1) ? lo : hi;
+    const _Float16* pa = Ap + (size_t)(I * BM + c) * ld + k0 + kh;
+    const _Float16* pb = Bp + (size_t)(J * BM + c) * ld + k0 + kh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ra[q] = *(const i4*)(pa + 8 * q);
+      rb[q] = *(const i4*)(pb + 8 * q);
+    }
+  };
+  load(0);
+  for (int s = 0; s < total; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      *(i4*)(&As[c][kh + 8 * q]) = ra[q];
+      *(i4*)(&Bs[c][kh + 8 * q]) = rb[q];
+    }
+    __syncthreads();
+    if (s + 1 < total) load(s + 1);
+    const bool cross = s >= ksteps;
+#pragma unroll
+    for (int kk = 0; kk < BK / 32; ++kk) {
+      const int kc = kk * 32 + 8 * (lane >> 4);
+      h8 a[4], b[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a[m] = *(const h8*)(&As[wm * 64 + m * 16 + (lane & 15)][kc]);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) b[n] = *(const h8*)(&Bs[wn * 64 + n * 16 + (lane & 15)][kc]);
+      if (cross) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int n = 0; n < 4; ++n)
+            acc_x[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[n], a[m], acc_x[m][n], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int n = 0; n < 4; ++n)
+            acc_hh[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[n], a[m], acc_hh[m][n], 0, 0, 0);
+      }
+    }
+  }
+
+  // Epilogue.  Operands were swapped, so this lane holds (X^2)[row][col]
+  // with row = lane%16 and col = 4*(lane/16) + r within each 16 x 16 block.
+  const bool mirror = (I != J);
+  const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
+  const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
+  const double wz = mirror ? 2.0 : 1.0;
+  double sums[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int row = I * BM + wm * 64 + m * 16 + (lane & 15);
+      double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = J * BM + wn * 64 + n * 16 + 4 * (lane >> 4) + r;
+        const size_t idx = row + (size_t)col * ld;
+        const size_t tdx = mirror ? col + (size_t)row * ld : idx;
+        const double y = (double)acc_hh[m][n][r] + 0x1p-11 * (double)acc_x[m][n][r];
+        const double x = X[idx];
+        const double z = fma(cb, x, ca * y);
+        const double dxy = x - y;
+        const bool diag = (row == col);
+        q0 += z * z;
+        q2 += dxy * dxy;
+        q1 += diag ? z : 0.0;
+        q3 += diag ? y : 0.0;
+        const _Float16 h = (_Float16)z;
+        const _Float16 l = (_Float16)((z - (double)h) * 2048.0);
+        Xn[idx] = z;
+        Xn[tdx] = z;
+        hin[idx] = h;
+        hin[tdx] = h;
+        lon[idx] = l;
+        lon[tdx] = l;
+      }
+      sums[0] += wz * q0;
+      sums[1] += q1;
+      sums[2] += wz * q2;
+      sums[3] += q3;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  block_store_partials<4>(sums, g.part + 4 * (size_t)t);
+}
+
+// hi/lo planes of an FP64 matrix (whole ld x ld buffer, padding stays zero).
+__global__ void split_f16_kernel(const double* __restrict__ x, _Float16* __restrict__ hi,
+                                 _Float16* __restrict__ lo, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
+    const double z = x[i];
+    const _Float16 h = (_Float16)z;
+    hi[i] = h;
+    lo[i] = (_Float16)((z - (double)h) * 2048.0);
+  }
+}
+
+}  // namespace
+
+int split_f16(const DMat& x, void* hi, void* lo) {
+  const size_t count = (size_t)x.ld * x.ld;
+  hipLaunchKernelGGL(split_f16_kernel, dim3(2048), dim3(NT), 0, stream(), x.p, (_Float16*)hi,
+                     (_Float16*)lo, count);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
 
 int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
                    int* ntiles, int prec) {
-  (void)x0; (void)x1; (void)d_state; (void)d_part; (void)ntiles; (void)prec;
-  set_error(ERR_UNSUPPORTED, "mixed-precision SP2 products are not built yet");
-  return ERR_UNSUPPORTED;
+  if (prec != PRG_PREC_F16SPLIT) {
+    set_error(ERR_UNSUPPORTED, "int8 sliced products are not built yet");
+    return ERR_UNSUPPORTED;
+  }
+  const int ld = x0.ld;
+  const size_t plane = sizeof(_Float16) * (size_t)ld * ld;
+  int st = OK;
+  char* planes = (char*)workspace("f16_planes", 4 * plane, &st);
+  if (st) return st;
+  F16Args g;
+  g.buf[0] = x0.p;
+  g.buf[1] = x1.p;
+  g.hi[0] = (const _Float16*)(planes);
+  g.lo[0] = (const _Float16*)(planes + plane);
+  g.hi[1] = (const _Float16*)(planes + 2 * plane);
+  g.lo[1] = (const _Float16*)(planes + 3 * plane);
+  g.ld = ld;
+  g.st = d_state;
+  g.part = d_part;
+  const int nt = ld / BM;
+  const int nb = nt * (nt + 1) / 2;
+  if (ntiles) *ntiles = nb;
+  hipLaunchKernelGGL(sp2_f16split_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int sp2_mixed_prepare(const DMat& x0, int prec) {
+  if (prec != PRG_PREC_F16SPLIT) return OK;
+  const int ld = x0.ld;
+  const size_t plane = sizeof(_Float16) * (size_t)ld * ld;
+  int st = OK;
+  char* planes = (char*)workspace("f16_planes", 4 * plane, &st);
+  if (st) return st;
+  return split_f16(x0, planes, planes + plane);
 }
 
 }  // namespace prg

@@ -102,6 +102,14 @@ struct Sp2State {
   int cur;           // which ping-pong buffer holds X
   int icount;        // sequence length for replay variants
   int core;          // core rows for V_SUBMATRIX
+  // low-precision products: stop on the noise floor of the idempotency error
+  // (e = Tr X - Tr X^2; src/sp2_tensorcore/src/prg_sp2_tensorcore.cu:308-317),
+  // then polish with up to refine_max FP64 steps on the alg2 rule
+  int noise_stop;
+  int refine_max;
+  int refine_phase;  // 1 while the FP64 polishing steps run
+  int refine_steps;  // polishing steps taken
+  double ie0, ie1, ie2;  // idempotency errors of the last three X
   int pp[kMaxIter];
   double vv[kMaxIter];
 };
@@ -143,6 +151,8 @@ int sp2_response_step(const DMat& x0, const DMat& x1, const DMat& d0, const DMat
 int sp2_init_state(Sp2State* d_state, const double* d_part, int nblocks);
 int sp2_absorb_plan(Sp2State* d_state, const double* d_part, int ntiles);
 int sp2_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out);
+// Switches a finished low-precision loop into its FP64 polishing phase.
+int sp2_begin_refine(Sp2State* d_state);
 
 
 }  // namespace prg

@@ -98,6 +98,9 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
     if ((st = affine(*H1, -1.0 / (gb[1] - gb[0]), 0.0, D0, nullptr, nullptr))) return st;
   }
 
+  if (p.precision != PRG_PREC_FP64 && !response) {
+    if ((st = sp2_mixed_prepare(X0, p.precision))) return st;
+  }
   std::memset(hst, 0, sizeof(Sp2State));
   hst->occ = p.occ;
   hst->idemtol = p.idemtol;
@@ -108,6 +111,9 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   hst->absconv = p.absconv;
   hst->icount = p.icount;
   hst->core = p.core;
+  const bool lowp = (p.precision != PRG_PREC_FP64) && !response;
+  hst->noise_stop = lowp && (p.variant == V_ALG2);
+  hst->refine_max = hst->noise_stop ? p.refine_max : 0;
   if (p.pp_in)
     for (int i = 0; i < p.icount && i < kMaxIter; ++i) hst->pp[i] = p.pp_in[i];
   if (p.vv_in)
@@ -121,37 +127,48 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
                            : p.maxit + 2;
   const int chunk = n >= 4096 ? 4 : 16;
   const size_t head = offsetof(Sp2State, pp);
-  int steps = 0;
-  const int prec = p.precision;
+  int prec = p.precision;
   if (g_timing && (int)g_events.size() < 2 * (maxsteps + 1)) {
     const size_t old = g_events.size();
     g_events.resize(2 * (maxsteps + 1));
     for (size_t e = old; e < g_events.size(); ++e) PRG_HIP_CHECK(hipEventCreate(&g_events[e]));
   }
-  while (steps < maxsteps) {
-    for (int c = 0; c < chunk && steps < maxsteps; ++c, ++steps) {
-      int nt_used = 0;
-      if (g_timing) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps], stream()));
-      if (response) {
-        if ((st = sp2_response_step(X0, X1, D0, D1, dst, part, &nt_used))) return st;
-      }
-      if (prec == PRG_PREC_FP64 || response) {
-        st = sp2_step(X0, X1, dst, part, &nt_used);
-      } else if (prec == PRG_PREC_F16SPLIT) {
-        st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
-      } else if (prec == PRG_PREC_INT8X6) {
-        st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
-      } else {
-        set_error(ERR_ARG, "unknown precision mode");
-        return ERR_ARG;
-      }
-      if (st) return st;
-      if (g_timing) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps + 1], stream()));
-      if ((st = sp2_absorb_plan(dst, part, nt_used))) return st;
+  int phase0_steps = 0;
+  for (int phase = 0; phase < 2; ++phase) {
+    int steps = 0;
+    int phase_steps = maxsteps;
+    if (phase == 1) {
+      // FP64 polishing steps after a low-precision loop
+      if (!hst->noise_stop || hst->refine_max <= 0) break;
+      prec = PRG_PREC_FP64;
+      if ((st = sp2_begin_refine(dst))) return st;
+      phase_steps = hst->refine_max + 1;
     }
-    PRG_HIP_CHECK(hipMemcpyAsync(hst, dst, head, hipMemcpyDeviceToHost, stream()));
-    PRG_HIP_CHECK(hipStreamSynchronize(stream()));
-    if (hst->done) break;
+    while (steps < phase_steps) {
+      for (int c = 0; c < chunk && steps < phase_steps; ++c, ++steps) {
+        int nt_used = 0;
+        const bool timed = g_timing && phase == 0;
+        if (timed) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps], stream()));
+        if (response) {
+          if ((st = sp2_response_step(X0, X1, D0, D1, dst, part, &nt_used))) return st;
+        }
+        if (prec == PRG_PREC_FP64 || response) {
+          st = sp2_step(X0, X1, dst, part, &nt_used);
+        } else if (prec == PRG_PREC_F16SPLIT || prec == PRG_PREC_INT8X6) {
+          st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
+        } else {
+          set_error(ERR_ARG, "unknown precision mode");
+          return ERR_ARG;
+        }
+        if (st) return st;
+        if (timed) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps + 1], stream()));
+        if ((st = sp2_absorb_plan(dst, part, nt_used))) return st;
+      }
+      PRG_HIP_CHECK(hipMemcpyAsync(hst, dst, head, hipMemcpyDeviceToHost, stream()));
+      PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+      if (hst->done) break;
+    }
+    if (phase == 0) phase0_steps = steps;
   }
   if ((st = sp2_finish(X0, X1, dst, 2.0, out))) return st;
   if (response && out1) {
@@ -165,8 +182,9 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
     return ERR_NOT_CONVERGED;
   }
   if (g_timing) {
-    // products actually computed: iter (plus a measured-only last product)
-    const int executed = hst->iter < steps ? hst->iter : steps;
+    // products of the main (phase 0) loop
+    const int main_iters = hst->iter - hst->refine_steps;
+    const int executed = main_iters < phase0_steps ? main_iters : phase0_steps;
     for (int e = 0; e < executed; ++e) {
       float ms = 0.f;
       PRG_HIP_CHECK(hipEventElapsedTime(&ms, g_events[2 * e], g_events[2 * e + 1]));
@@ -175,6 +193,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
     }
   }
   if (res) {
+    res->refine_steps = hst->refine_steps;
     res->iter = hst->iter;
     res->status = hst->status;
     res->trx = hst->trx;

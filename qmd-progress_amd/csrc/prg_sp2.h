@@ -23,10 +23,12 @@ struct Sp2Params {
   double mineval = 0.0;
   double maxeval = 0.0;
   int precision = 0;
+  int refine_max = 2;  // FP64 polishing steps after a low-precision loop
 };
 
 struct Sp2Result {
   int iter = 0;
+  int refine_steps = 0;
   int status = 0;
   double trx = 0.0;
   double mineval = 0.0;
@@ -43,6 +45,8 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res,
 // Mixed-precision fused SP2 step (prec = PRG_PREC_F16SPLIT or PRG_PREC_INT8X6).
 int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
                    int* ntiles, int prec);
+// Builds the low-precision representation of X0 (after normalisation).
+int sp2_mixed_prepare(const DMat& x0, int prec);
 
 int precision_mode();
 void set_precision_mode(int m);

@@ -139,3 +139,23 @@ def test_bad_inputs_are_errors(P):
     with pytest.raises(P.ProgressError):
         # maxsp2iter too small: the reference calls stop here
         P.sp2_alg2(O.model_hamiltonian(128, 0), 0.5, 1, 5, "Abs", 1e-10)
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_f16split_mode_tracks_fp64(P, n):
+    """FP16 hi+lo split products with FP32 accumulation: same iteration count
+    (+-2) and an FP32-class density (||D - D_oracle||_F within 1e-3 at these
+    sizes; the measured value is reported by bench.py at N=8192)."""
+    print(f"[f16split] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 0)
+    rho_o, it_o, pp_o = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    P.set_precision(P.PREC_F16SPLIT)
+    try:
+        rho_g = P.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    finally:
+        P.set_precision(P.PREC_FP64)
+    err = np.linalg.norm(rho_g - rho_o)
+    print(f"  ||D_f16split - D_oracle||_F = {err:.3e}")
+    assert err < 1e-3
+    assert abs(np.trace(rho_g) - n) < 1e-3
+    assert np.linalg.norm(rho_g @ rho_g / 2 - rho_g) < 1e-3
