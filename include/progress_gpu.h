@@ -146,6 +146,69 @@ void prg_homolumogap(const double *vv, int imax, const int *pp, double mineval, 
 void prg_sp2sequence(int *pp, int *imax, double mineval, double maxeval, double ehomo,
                      double elumo, double errlimit, int verbose);
 
+/* ------------------------------------------------------------------ eigensolver densities */
+/* src/prg_c_interface.F90:68-86 (prg_densitymatrix_mod.F90:36-101); rocSOLVER dsyevd */
+void prg_build_density_T0(int norbs, bml_matrix_t *ham_bml, bml_matrix_t *rho_bml,
+                          double threshold, double bndfil, double *eigenvalues_out);
+/* src/prg_c_interface.F90:227-241 (prg_densitymatrix_mod.F90:509-582) */
+void prg_build_density_T_fermi(bml_matrix_t *ham_bml, bml_matrix_t *rho_bml, double threshold,
+                               double kbt, double ef, int verbose, double *drho);
+/* src/prg_c_interface.F90:281-293 (prg_densitymatrix_mod.F90:817-854) */
+void prg_get_eigenvalues(int norbs, bml_matrix_t *ham_bml, double *eigenvalues_out, int verbose);
+/* src/prg_c_interface.F90:295-304 (prg_densitymatrix_mod.F90:862-880) */
+void prg_check_idempotency(bml_matrix_t *mat_bml, double threshold, double *idempotency);
+/* prg_get_density_matrix (prg_implicit_fermi_mod.F90:1056-1100): P = f(H), no factor 2 */
+void prg_get_density_matrix(bml_matrix_t *ham_bml, bml_matrix_t *p_bml, double beta, double mu,
+                            double threshold);
+/* src/prg_c_interface.F90:535-549 (prg_implicit_fermi_mod.F90:1112-1186); mu by value as in
+ * the reference; prg_test_density_matrix_mu() returns the converged value. */
+void prg_test_density_matrix(bml_matrix_t *ham_bml, bml_matrix_t *p_bml, double beta, double mu,
+                             double nocc, int osteps, double occErrLimit, double threshold);
+double prg_test_density_matrix_mu(void);
+
+/* ------------------------------------------------------------------ implicit Fermi */
+/* src/prg_c_interface.F90:470-490 (prg_implicit_fermi_mod.F90:209-336); k = 2 only.
+ * mu by value as in the reference; prg_implicit_fermi_last_mu() returns the update. */
+void prg_implicit_fermi(bml_matrix_t *h_bml, bml_matrix_t *p_bml, int nsteps, int k, double nocc,
+                        double mu, double beta, int method, int osteps, double occErrLimit,
+                        double threshold, double tol);
+double prg_implicit_fermi_last_mu(void);
+/* src/prg_c_interface.F90:440-468 (prg_implicit_fermi_mod.F90:46-193); Inv_bml[nsteps] are
+ * caller-owned and persist across calls (warm starts); occiter / totns by value as in the
+ * reference header, prg_implicit_fermi_counters() returns the updated counts. */
+void prg_implicit_fermi_save_inverse(bml_matrix_t **Inv_bml, bml_matrix_t *h_bml,
+                                     bml_matrix_t *p_bml, int nsteps, double nocc, double *mu,
+                                     double beta, double occErrLimit, double threshold, double tol,
+                                     int SCF_IT, int occiter, int totns);
+void prg_implicit_fermi_counters(int *occiter, int *totns);
+/* src/prg_c_interface.F90:492-506 (prg_implicit_fermi_mod.F90:347-425) */
+void prg_implicit_fermi_zero(bml_matrix_t *h_bml, bml_matrix_t *p_bml, int nsteps, double mu,
+                             int method, double threshold, double tol);
+/* src/prg_c_interface.F90:508-533 (prg_implicit_fermi_mod.F90:439-529) */
+void prg_implicit_fermi_first_order_response(bml_matrix_t *H0_bml, bml_matrix_t *H1_bml,
+                                             bml_matrix_t *P0_bml, bml_matrix_t *P1_bml,
+                                             bml_matrix_t **Inv_bml, int nsteps, double mu0,
+                                             double beta, double nocc, double threshold);
+/* Declared at src/prg_progress_mod.h:240-258 without an implementation there
+ * (prg_implicit_fermi_mod.F90:547-712): H_bml[order] = H1..H3, P_bml[order] = P1..P3,
+ * *mu0 updated, mu[order] receives the response chemical potentials. */
+void prg_implicit_fermi_response(bml_matrix_t *H0_bml, bml_matrix_t **H_bml, bml_matrix_t *P0_bml,
+                                 bml_matrix_t **P_bml, int nsteps, double *mu0, double *mu,
+                                 double beta, double nocc, double occ_tol, double lin_tol,
+                                 int order, double threshold);
+
+/* ------------------------------------------------------------------ response */
+/* New C symbol for prg_compute_response_SP2 (prg_response_mod.F90:653-782), Fortran order:
+ * rho = 2X, rsp = 2 Delta (first-order response to prt). */
+void prg_compute_response_SP2(bml_matrix_t *ham_bml, bml_matrix_t *prt_bml, bml_matrix_t *rsp_bml,
+                              bml_matrix_t *rho_bml, double lambda, double bndfil, int minsp2iter,
+                              int maxsp2iter, const char *sp2conv, double idemtol,
+                              double threshold, int verbose);
+/* New C symbol for prg_canon_response (prg_response_mod.F90:947-1022): H1 and P1 in the
+ * eigenbasis of H0, evals[HDIM] its eigenvalues (host). */
+void prg_canon_response(bml_matrix_t *P1_bml, bml_matrix_t *H1_bml, double nocc, double beta,
+                        const double *evals, double mu0, int m, int HDIM);
+
 /* ------------------------------------------------------------------ precision modes */
 /* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).
  *   PRG_PREC_FP64      FP64 MFMA;

@@ -406,18 +406,58 @@ def sp2sequence(mineval, maxeval, ehomo, elumo, errlimit):
 # --------------------------------------------------------------------------
 
 
+def t0_occupations(n, bndfil):
+    """Occupation rule of prg_build_density_T0 (src/prg_densitymatrix_mod.F90:73-84):
+    2 for state i (1-based) when i - nocc < 1e-4, else 0; state int(nocc)+1
+    gets 1 when nocc has a fractional part above 0.01."""
+    nocc = n * bndfil
+    occ = np.zeros(n)
+    for i in range(1, n + 1):
+        if i - nocc < 0.0001 or abs(i - nocc) < 0.0001:
+            occ[i - 1] = 2.0
+    if abs(nocc - int(nocc)) > 0.01:
+        occ[int(nocc)] = 1.0
+    return occ
+
+
 def build_density_T0(h, bndfil):
-    """prg_build_density_T0 (src/prg_densitymatrix_mod.F90:36-101): occupation 2
-    for the lowest int(bndfil N) states, 1 for a remaining fractional state."""
+    """prg_build_density_T0 (src/prg_densitymatrix_mod.F90:36-101)."""
     n = h.shape[0]
     w, q = np.linalg.eigh(h)
-    nocc = bndfil * n
-    occ = np.zeros(n)
-    full = int(nocc)
-    occ[:full] = 2.0
-    if nocc - full > 1e-12 and full < n:
-        occ[full] = 1.0
+    occ = t0_occupations(n, bndfil)
     return (q * occ) @ q.T, w
+
+
+def fermi_dm(e, ef, kbt):
+    """fermi of src/prg_densitymatrix_mod.F90:886-896 (0 beyond (e-ef)/kbt > 100)."""
+    e = np.asarray(e, dtype=float)
+    x = (e - ef) / kbt
+    out = np.zeros_like(e)
+    ok = x <= 100.0
+    out[ok] = 1.0 / (1.0 + np.exp(x[ok]))
+    return out
+
+
+def dfermi_dm(e, ef, kbt):
+    """dfermi of src/prg_densitymatrix_mod.F90:902-916 (with its -1000 sentinel)."""
+    e = np.asarray(e, dtype=float)
+    x = (e - ef) / kbt
+    out = np.zeros_like(e)
+    big = x > 100.0
+    out[big & (np.abs(e - ef) < 0.001)] = -1000.0
+    ok = ~big
+    f = 1.0 / (1.0 + np.exp(x[ok]))
+    out[ok] = f * f * np.exp(x[ok]) / kbt
+    return out
+
+
+def build_density_T_fermi(h, kbt, ef):
+    """prg_build_density_T_Fermi (src/prg_densitymatrix_mod.F90:509-582):
+    returns (rho, drho) with rho = Q 2 f(e) Q^T and drho = Tr(Q 2 f'(e) Q^T)."""
+    w, q = np.linalg.eigh(h)
+    rho = (q * (2.0 * fermi_dm(w, ef, kbt))) @ q.T
+    drho = float(np.sum(2.0 * dfermi_dm(w, ef, kbt)))
+    return rho, drho
 
 
 def fermi(e, mu, beta):
@@ -509,9 +549,12 @@ def newtonschulz(a, ai, tol, max_iter=200):
 
 def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, tol):
     """prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), k = 2.
-    method 0: CG solve per level (:262-277); method 1: the level-1 CG inverse at
-    tol 0.9 reused for every level (:278-298, Newton-Schulz commented out in the
-    reference).  Output P is not scaled by 2.  Returns (P, mu)."""
+    method 0: CG solve per level (:262-277).  method 1: a level-1 CG inverse at
+    tol 0.9 refined by Newton-Schulz (to `tol`) at every level (:278-298).  The
+    reference has that Newton-Schulz call commented out, which reuses the
+    level-1 inverse for all levels and makes the recursion diverge (||P|| ~ 1e30
+    on the model Hamiltonian); this restates the intended iteration.
+    Output P is not scaled by 2.  Returns (P, mu)."""
     if k != 2:
         raise ValueError("only expansion order k = 2 is restated")
     n = h.shape[0]
@@ -532,6 +575,7 @@ def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, 
             else:
                 if i == 0:
                     ai, _ = conjgrad(y, ai, np.eye(n), 0.9)
+                ai, _ = newtonschulz(y, ai, tol)
                 p = ai @ p2
         trp0 = np.trace(p)
         dpdmu = beta * (trp0 - np.sum(p * p))
@@ -600,6 +644,28 @@ def implicit_fermi_save_inverse(inv, h, nsteps, nocc, mu, beta, occ_err_limit, t
     return 2.0 * p, mu, inv, it, totns
 
 
+def implicit_fermi_zero(h, nsteps, mu, method, tol):
+    """prg_implicit_fermi_zero (src/prg_implicit_fermi_mod.F90:347-425) with
+    cnst = 0.5 min(1/(mu - lo), 1/(hi - mu)) from the Gershgorin interval.
+    Method 1 is restated as intended (P2 = P^2 before the solve); the
+    reference's method-1 branch multiplies by a P2 it never computed."""
+    n = h.shape[0]
+    lo, hi = gershgorin(h)
+    cnst = 0.5 * min(1.0 / (mu - lo), 1.0 / (hi - mu))
+    p = normalize_implicit_fermi(h, cnst, mu)
+    ai = np.eye(n)
+    for i in range(nsteps):
+        p2 = p @ p
+        y = 2.0 * (p2 - p) + np.eye(n)
+        if method == 0:
+            p, _ = conjgrad(y, p, p2, tol)
+        else:
+            if i == 0:
+                ai, _ = conjgrad(y, ai, np.eye(n), 0.9)
+            p = ai @ p2
+    return p
+
+
 def implicit_fermi_first_order_response(h0, h1, inv, nsteps, mu0, beta):
     """prg_implicit_fermi_first_order_response (src/prg_implicit_fermi_mod.F90:439-529).
     Returns (P0, P1) (neither scaled by 2)."""
@@ -623,11 +689,15 @@ def implicit_fermi_first_order_response(h0, h1, inv, nsteps, mu0, beta):
     return p0, p1
 
 
-def implicit_fermi_response(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, max_k=50):
+def implicit_fermi_response(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, lin_tol=1e-10,
+                            max_k=50):
     """prg_implicit_fermi_response (src/prg_implicit_fermi_mod.F90:547-712),
-    orders 1..3.  The level-1 inverse is a tol-0.01 CG solve from the previous
-    inverse (starting at zero), reused for all levels as in the reference.
-    Returns (P0, [P1..Porder], mu0, mu)."""
+    orders 1..3, with its own chemical-potential loop.  Per level the inverse
+    T^-1 is a tol-0.01 CG solve at level 1 (from the previous inverse, starting
+    at zero) refined by Newton-Schulz to lin_tol at every level.  The reference
+    has the Newton-Schulz call commented out (it reuses the level-1 inverse
+    for all levels, and the recursion diverges); this restates the intended
+    iteration.  Returns (P0, [P1..Porder], mu0, mu)."""
     n = h0.shape[0]
     eye = np.eye(n)
     cnst = beta / (2.0 ** (2 + nsteps))
@@ -657,6 +727,7 @@ def implicit_fermi_response(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, max
             t = 2.0 * t + eye
             if i == 0:
                 ti, _ = conjgrad(t, ti, eye, 0.01)
+            ti, _ = newtonschulz(t, ti, lin_tol)
             p0 = ti @ c0
             cs[0] = cs[0] + bs[0] @ p0
             ps[0] = ti @ cs[0]

@@ -427,6 +427,46 @@ __global__ void sp2_finish_kernel(const double* __restrict__ x0, const double* _
 
 int grid_cols(int ld) { return ld < 2048 ? ld : 2048; }
 
+// 64 x 64 tiles through LDS (padded rows): coalesced reads and writes.
+__global__ void transpose_kernel(const double* __restrict__ a, double* __restrict__ out, int ld) {
+  __shared__ double t[64][65];
+  const int bx = blockIdx.x * 64, by = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+  for (int j = ty; j < 64; j += 4) t[j][tx] = a[(bx + tx) + (size_t)(by + j) * ld];
+  __syncthreads();
+  for (int j = ty; j < 64; j += 4) out[(by + tx) + (size_t)(bx + j) * ld] = t[tx][j];
+}
+
+__global__ void scale_cols_kernel(const double* __restrict__ a, const double* __restrict__ f,
+                                  double* __restrict__ out, int n, int ld) {
+  for (int col = blockIdx.x; col < ld; col += gridDim.x) {
+    const double s = col < n ? f[col] : 0.0;
+    for (int row = threadIdx.x; row < ld; row += NT)
+      out[row + (size_t)col * ld] = s * a[row + (size_t)col * ld];
+  }
+}
+
+}  // namespace
+
+int transpose_kernel_launch(const DMat& A, DMat out) {
+  if (A.ld != out.ld || A.p == out.p) {
+    set_error(ERR_ARG, "transpose: layout mismatch or aliasing");
+    return ERR_ARG;
+  }
+  const int nb = A.ld / 64;
+  hipLaunchKernelGGL(transpose_kernel, dim3(nb, nb), dim3(NT), 0, stream(), A.p, out.p, A.ld);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int scale_cols_kernel_launch(const DMat& A, const double* d_f, DMat out) {
+  hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_cols(A.ld)), dim3(NT), 0, stream(), A.p, d_f,
+                     out.p, A.n, A.ld);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+namespace {
 }  // namespace
 
 int gershgorin(const DMat& h, double* d_bounds) {

@@ -1,0 +1,503 @@
+// Recursive implicit Fermi-Dirac expansion (src/prg_implicit_fermi_mod.F90)
+// on one GPU.
+//
+// Every matrix product is an FP64 MFMA GEMM (symmetric-tile variant whenever
+// both factors are polynomials of the same normalised Hamiltonian, i.e. they
+// commute and the product is symmetric); the scalar recurrences -- CG step
+// lengths, Newton-Schulz residual norms, the chemical-potential Newton /
+// bisection updates -- run on the host from reductions fused into the
+// producing kernels, as in the reference.
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "prg_bml.h"
+#include "prg_eigen.h"
+#include "prg_ops.h"
+
+namespace prg {
+
+using ops::Sums;
+
+// prg_conjgrad (src/prg_implicit_fermi_mod.F90:1006-1048): solve A X = B,
+// X holding the initial guess, until ||R||_F^2 <= tol.  `commuting`: X0, B
+// are polynomials of A (products then symmetric).
+int conjgrad(const DMat& A, DMat X, const DMat& B, double tol, bool commuting, int* iters) {
+  DMat R, D, W;
+  int st;
+  if ((st = ops::temp("cg_r", A.n, &R)) || (st = ops::temp("cg_d", A.n, &D)) ||
+      (st = ops::temp("cg_w", A.n, &W)))
+    return st;
+  Sums s;
+  // R = B - A X
+  if ((st = ops::mm(A, X, R, commuting, -1.0, 1.0, &B, 0.0, &s))) return st;
+  double rn = s.fro2, rn_old = rn;
+  int k = 0;
+  while (rn > tol) {
+    ++k;
+    if (k == 1) {
+      if ((st = ops::copy(R, D))) return st;
+    } else {
+      if ((st = ops::lin(D, rn / rn_old, &R, 1.0, 0.0, D))) return st;
+    }
+    if ((st = ops::mm(A, D, W, commuting, 1.0, 0.0, nullptr, 0.0, &s, &D))) return st;
+    const double alpha = rn / s.dot;  // <D, A D> = Tr(D^T W)
+    if ((st = ops::lin(X, 1.0, &D, alpha, 0.0, X))) return st;
+    if ((st = ops::lin(R, 1.0, &W, -alpha, 0.0, R, &s))) return st;
+    rn_old = rn;
+    rn = s.fro2;
+    if (!std::isfinite(rn)) {
+      set_error(ERR_NUMERIC, "conjugate gradient: non-finite residual");
+      return ERR_NUMERIC;
+    }
+    if (k > 50) {
+      set_error(ERR_NOT_CONVERGED, "Conjugate gradient is not converging");
+      return ERR_NOT_CONVERGED;
+    }
+  }
+  if (iters) *iters = k;
+  return OK;
+}
+
+// prg_newtonschulz (src/prg_implicit_fermi_mod.F90:888-922): Ai <- Ai + Ai R,
+// R = I - A Ai, until ||R||_F <= tol; CG fallback when the error grows.
+int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters) {
+  DMat R, T, I;
+  int st;
+  if ((st = ops::temp("ns_r", A.n, &R)) || (st = ops::temp("ns_t", A.n, &T)) ||
+      (st = ops::temp("ns_i", A.n, &I)))
+    return st;
+  double err = 100000.0;
+  int i = 0;
+  Sums s;
+  while (err > tol) {
+    if ((st = ops::mm(A, Ai, R, false, -1.0, 0.0, nullptr, 1.0, &s))) return st;
+    const double prev = err;
+    err = std::sqrt(s.fro2);
+    if (!std::isfinite(err)) {
+      set_error(ERR_NUMERIC, "Newton-Schulz: non-finite residual");
+      return ERR_NUMERIC;
+    }
+    if (prev + 0.01 < err) {
+      if ((st = ops::identity(I))) return st;
+      if ((st = conjgrad(A, Ai, I, 0.00001, false, nullptr))) return st;
+    } else {
+      if ((st = ops::mm(Ai, R, T, false, 1.0, 1.0, &Ai))) return st;
+      if ((st = ops::copy(T, Ai))) return st;
+    }
+    ++i;
+    if (i > 200) {
+      set_error(ERR_NOT_CONVERGED, "Newton-Schulz is not converging");
+      return ERR_NOT_CONVERGED;
+    }
+  }
+  if (iters) *iters = i;
+  return OK;
+}
+
+namespace {
+
+int g_last_occiter = 0, g_last_totns = 0;
+double g_last_mu = 0.0;
+
+// P0 = (0.5 + cnst mu) I - cnst H   (prg_normalize_implicit_fermi)
+int normalize_if(const DMat& H, double cnst, double mu, DMat P) {
+  return ops::lin(H, -cnst, nullptr, 0.0, 0.5 + cnst * mu, P);
+}
+
+}  // namespace
+
+// prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), k = 2.
+int implicit_fermi(const DMat& H, DMat P, int nsteps, int k, double nocc, double* mu,
+                   double beta, int method, int osteps, double occ_err_limit, double tol) {
+  if (k != 2) {
+    set_error(ERR_UNSUPPORTED, "prg_implicit_fermi: only expansion order k = 2 is supported");
+    return ERR_UNSUPPORTED;
+  }
+  if (nsteps < 1 || nsteps > 60) {
+    set_error(ERR_ARG, "prg_implicit_fermi: nsteps out of range");
+    return ERR_ARG;
+  }
+  DMat P2, Y, AI, I;
+  int st;
+  if ((st = ops::temp("if_p2", H.n, &P2)) || (st = ops::temp("if_y", H.n, &Y)) ||
+      (st = ops::temp("if_ai", H.n, &AI)) || (st = ops::temp("if_i", H.n, &I)))
+    return st;
+  if (method == 1) {
+    if ((st = ops::identity(AI)) || (st = ops::identity(I))) return st;
+  }
+  const double cnst = beta / (4.0 * std::pow(2.0, nsteps));
+  double occ_err = 10000.0;
+  int iter = 0;
+  Sums s;
+  while ((osteps == 0 && occ_err > occ_err_limit) || (osteps > 0 && iter < osteps)) {
+    ++iter;
+    if ((st = normalize_if(H, cnst, *mu, P))) return st;
+    for (int i = 0; i < nsteps; ++i) {
+      if ((st = ops::mm(P, P, P2, true))) return st;
+      if ((st = ops::lin(P2, 2.0, &P, -2.0, 1.0, Y))) return st;  // Y = 2(P2 - P) + I
+      if (method == 0) {
+        if ((st = conjgrad(Y, P, P2, tol, true, nullptr))) return st;
+      } else {
+        // level-1 CG inverse (tol 0.9) refined by Newton-Schulz at every level;
+        // the reference comments the refinement out and then diverges
+        if (i == 0 && (st = conjgrad(Y, AI, I, 0.9, false, nullptr))) return st;
+        if ((st = newtonschulz(Y, AI, tol, nullptr))) return st;
+        if ((st = ops::mm(AI, P2, P, false))) return st;
+      }
+    }
+    if ((st = ops::sums_of(P, &s))) return st;
+    const double trp0 = s.tr;
+    const double dpdmu = beta * (trp0 - s.fro2);
+    occ_err = std::fabs(trp0 - nocc);
+    if (occ_err > occ_err_limit) *mu = *mu + (nocc - trp0) / dpdmu;
+    if (iter > 1000) {
+      set_error(ERR_NOT_CONVERGED, "prg_implicit_fermi: chemical potential not converging");
+      return ERR_NOT_CONVERGED;
+    }
+  }
+  return OK;
+}
+
+// prg_implicit_fermi_save_inverse (src/prg_implicit_fermi_mod.F90:46-193).
+int implicit_fermi_save_inverse(std::vector<DMat>& inv, const DMat& H, DMat P, int nsteps,
+                                double nocc, double* mu, double beta, double occ_err_limit,
+                                double tol, int scf_it, int* occiter, int* totns) {
+  DMat P2, Y, I;
+  int st;
+  if ((st = ops::temp("ifs_p2", H.n, &P2)) || (st = ops::temp("ifs_y", H.n, &Y)) ||
+      (st = ops::temp("ifs_i", H.n, &I)))
+    return st;
+  if ((st = ops::identity(I))) return st;
+  double occ_err = 10.0, alpha = 1.0;
+  int prev = 1, iter = 0, muadj = 0;
+  const int maxiter = 30;
+  const double cnst = beta / std::pow(2.0, nsteps + 2);
+  if (scf_it == 1) {
+    // the reference also runs a CG solve here whose result it discards
+    // (it resets every inverse to the identity right after); only the reset
+    // is observable
+    alpha = 4.0;
+    for (int i = 0; i < nsteps; ++i)
+      if ((st = ops::identity(inv[i]))) return st;
+  }
+  Sums s;
+  int ns_total = 0;
+  while ((occ_err > occ_err_limit || muadj == 1) && iter < maxiter) {
+    ++iter;
+    muadj = 0;
+    if ((st = normalize_if(H, cnst, *mu, P))) return st;
+    for (int i = 0; i < nsteps; ++i) {
+      if ((st = ops::mm(P, P, P2, true))) return st;
+      if ((st = ops::lin(P2, 2.0, &P, -2.0, 1.0, Y))) return st;
+      if (iter == 1 && (st = conjgrad(Y, inv[i], I, 0.001, false, nullptr))) return st;
+      int nsit = 0;
+      if ((st = newtonschulz(Y, inv[i], tol, &nsit))) return st;
+      ns_total += nsit;
+      if ((st = ops::mm(inv[i], P2, P, false))) return st;
+    }
+    if ((st = ops::sums_of(P, &s))) return st;
+    const double trp0 = s.tr;
+    const double dpdmu = beta * (trp0 - s.fro2);
+    occ_err = std::fabs(trp0 - nocc);
+    if (occ_err > 1.0) {
+      if (nocc - trp0 < 0.0 && prev == -1) {
+        prev = -1;
+      } else if (nocc - trp0 > 0.0 && prev == 1) {
+        prev = 1;
+      } else if (nocc - trp0 > 0.0 && prev == -1) {
+        prev = 1;
+        alpha = alpha / 2;
+      } else {
+        prev = -1;
+        alpha = alpha / 2;
+      }
+      *mu = *mu + prev * alpha;
+      muadj = 1;
+    } else if (occ_err > occ_err_limit) {
+      *mu = *mu + (nocc - trp0) / dpdmu;
+      muadj = 1;
+    }
+  }
+  if (occiter) *occiter += iter;
+  if (totns) *totns += ns_total;
+  if (iter >= maxiter)
+    std::printf("Could not converge chemical potential in prg_implicit_fermi_save_inverse\n");
+  return ops::lin(P, 2.0, nullptr, 0.0, 0.0, P);
+}
+
+// prg_implicit_fermi_zero (src/prg_implicit_fermi_mod.F90:347-425).  Method 1
+// follows the intended algorithm (P2 = P^2 before the solve); the reference's
+// method-1 branch multiplies by a P2 that it never computed.
+int implicit_fermi_zero(const DMat& H, DMat P, int nsteps, double mu, int method, double tol) {
+  DMat P2, Y, AI, I;
+  int st;
+  if ((st = ops::temp("if_p2", H.n, &P2)) || (st = ops::temp("if_y", H.n, &Y)) ||
+      (st = ops::temp("if_ai", H.n, &AI)) || (st = ops::temp("if_i", H.n, &I)))
+    return st;
+  double* d_b = (double*)workspace("ifz_bounds", 2 * sizeof(double), &st);
+  if (st || (st = gershgorin(H, d_b))) return st;
+  double gb[2];
+  PRG_HIP_CHECK(hipMemcpyAsync(gb, d_b, sizeof gb, hipMemcpyDeviceToHost, stream()));
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  const double cnst = 0.5 * std::fmin(1.0 / (mu - gb[0]), 1.0 / (gb[1] - mu));
+  if ((st = normalize_if(H, cnst, mu, P))) return st;
+  if (method == 1 && ((st = ops::identity(AI)) || (st = ops::identity(I)))) return st;
+  for (int i = 0; i < nsteps; ++i) {
+    if ((st = ops::mm(P, P, P2, true))) return st;
+    if ((st = ops::lin(P2, 2.0, &P, -2.0, 1.0, Y))) return st;
+    if (method == 0) {
+      if ((st = conjgrad(Y, P, P2, tol, true, nullptr))) return st;
+    } else {
+      if (i == 0 && (st = conjgrad(Y, AI, I, 0.9, true, nullptr))) return st;
+      if ((st = ops::mm(AI, P2, P, true))) return st;
+    }
+  }
+  return OK;
+}
+
+// prg_implicit_fermi_first_order_response (src/prg_implicit_fermi_mod.F90:439-529).
+int implicit_fermi_first_order_response(const DMat& H0, const DMat& H1, DMat P0, DMat P1,
+                                        const std::vector<DMat>& inv, int nsteps, double mu0,
+                                        double beta) {
+  DMat C0, C, B, T;
+  int st;
+  if ((st = ops::temp("r1_c0", H0.n, &C0)) || (st = ops::temp("r1_c", H0.n, &C)) ||
+      (st = ops::temp("r1_b", H0.n, &B)) || (st = ops::temp("r1_t", H0.n, &T)))
+    return st;
+  const double cnst = beta / std::pow(2.0, 2 + nsteps);
+  if ((st = normalize_if(H0, cnst, mu0, P0))) return st;
+  if ((st = ops::lin(H1, -cnst, nullptr, 0.0, 0.0, P1))) return st;
+  for (int i = 0; i < nsteps; ++i) {
+    if ((st = ops::mm(P0, P0, C0, true))) return st;              // C0 = P0^2
+    if ((st = ops::mm2(P0, P1, P1, P0, C, true))) return st;      // C = P0 P1 + P1 P0
+    if ((st = ops::lin(P1, 2.0, &C, -2.0, 0.0, B))) return st;    // B = 2 (P1 - C)
+    if ((st = ops::mm(inv[i], C0, P0, false))) return st;         // P0 = Inv C0
+    if ((st = ops::mm(B, P0, T, false, 1.0, 1.0, &C))) return st; // T = C + B P0
+    if ((st = ops::mm(inv[i], T, P1, false))) return st;          // P1 = Inv T
+  }
+  // dPdmu = beta P0 (I - P0); trace correction mu1 = -Tr P1 / Tr dPdmu
+  Sums s0, s1;
+  if ((st = ops::mm(P0, P0, C0, false, -beta, beta, &P0, 0.0, &s0))) return st;
+  if ((st = ops::sums_of(P1, &s1))) return st;
+  const double dpdmu_tr = s0.tr;
+  if (std::fabs(dpdmu_tr) > 1e-8) {
+    const double mu1 = -s1.tr / dpdmu_tr;
+    if ((st = ops::lin(P1, 1.0, &C0, mu1, 0.0, P1))) return st;
+  }
+  return OK;
+}
+
+// prg_implicit_fermi_response (src/prg_implicit_fermi_mod.F90:547-712), orders
+// 1..3, with its own chemical-potential loop (at most 50 passes).  The level-1
+// inverse (CG, tol 0.01) is refined by Newton-Schulz at every level; the
+// reference comments that refinement out and its recursion diverges.
+int implicit_fermi_response(const DMat& H0, const std::vector<DMat>& Hs, DMat P0,
+                            std::vector<DMat>& Ps, int nsteps, double* mu0, double* mu,
+                            double beta, double nocc, double occ_tol, double lin_tol, int order) {
+  if (order < 1 || order > 3) {
+    set_error(ERR_ARG, "prg_implicit_fermi_response: order must be 1..3");
+    return ERR_ARG;
+  }
+  const int n = H0.n;
+  DMat C0, T, Ti, I, Cm[3], Bm[3], Tmp;
+  int st;
+  if ((st = ops::temp("rs_c0", n, &C0)) || (st = ops::temp("rs_t", n, &T)) ||
+      (st = ops::temp("rs_ti", n, &Ti)) || (st = ops::temp("rs_i", n, &I)) ||
+      (st = ops::temp("rs_tmp", n, &Tmp)))
+    return st;
+  const char* cn[3] = {"rs_c1", "rs_c2", "rs_c3"};
+  const char* bn[3] = {"rs_b1", "rs_b2", "rs_b3"};
+  for (int j = 0; j < order; ++j)
+    if ((st = ops::temp(cn[j], n, &Cm[j])) || (st = ops::temp(bn[j], n, &Bm[j]))) return st;
+  if ((st = ops::identity(I)) || (st = ops::lin(I, 0.0, nullptr, 0.0, 0.0, Ti))) return st;
+  const double cnst = beta / std::pow(2.0, 2 + nsteps);
+  for (int j = 0; j < order; ++j) mu[j] = 0.0;
+  double occ_err = 10000.0;
+  int k = 0;
+  Sums s;
+  while (occ_err > occ_tol) {
+    ++k;
+    if ((st = normalize_if(H0, cnst, *mu0, P0))) return st;
+    for (int j = 0; j < order; ++j)  // P(j) = -cnst (H(j) - mu(j) I)
+      if ((st = ops::lin(Hs[j], -cnst, nullptr, 0.0, cnst * mu[j], Ps[j]))) return st;
+    for (int i = 0; i < nsteps; ++i) {
+      if ((st = ops::mm(P0, P0, C0, true))) return st;
+      if ((st = ops::mm2(P0, Ps[0], Ps[0], P0, Cm[0], true))) return st;
+      if ((st = ops::lin(Ps[0], 2.0, &Cm[0], -2.0, 0.0, Bm[0]))) return st;
+      if (order > 1) {
+        if ((st = ops::mm2(P0, Ps[1], Ps[1], P0, Tmp, true))) return st;
+        if ((st = ops::mm(Ps[0], Ps[0], Cm[1], true, 1.0, 1.0, &Tmp))) return st;
+        if ((st = ops::lin(Ps[1], 2.0, &Cm[1], -2.0, 0.0, Bm[1]))) return st;
+      }
+      if (order > 2) {
+        if ((st = ops::mm2(Ps[0], Ps[1], Ps[1], Ps[0], Tmp, true))) return st;
+        if ((st = ops::mm2(P0, Ps[2], Ps[2], P0, Cm[2], true, 1.0, 1.0, &Tmp))) return st;
+        if ((st = ops::lin(Ps[2], 2.0, &Cm[2], -2.0, 0.0, Bm[2]))) return st;
+      }
+      if ((st = ops::lin(C0, 2.0, &P0, -2.0, 1.0, T))) return st;  // T = 2 P0^2 - 2 P0 + I
+      if (i == 0 && (st = conjgrad(T, Ti, I, 0.01, false, nullptr))) return st;
+      if ((st = newtonschulz(T, Ti, lin_tol, nullptr))) return st;
+      if ((st = ops::mm(Ti, C0, P0, false))) return st;
+      if ((st = ops::mm(Bm[0], P0, Tmp, false, 1.0, 1.0, &Cm[0]))) return st;
+      if ((st = ops::mm(Ti, Tmp, Ps[0], false))) return st;
+      if (order > 1) {
+        if ((st = ops::mm(Bm[1], P0, Cm[1], false, 1.0, 1.0, &Cm[1]))) return st;
+        if ((st = ops::mm(Bm[0], Ps[0], Tmp, false, 1.0, 1.0, &Cm[1]))) return st;
+        if ((st = ops::mm(Ti, Tmp, Ps[1], false))) return st;
+      }
+      if (order > 2) {
+        if ((st = ops::mm(Bm[2], P0, Cm[2], false, 1.0, 1.0, &Cm[2]))) return st;
+        if ((st = ops::mm(Bm[1], Ps[0], Cm[2], false, 1.0, 1.0, &Cm[2]))) return st;
+        if ((st = ops::mm(Bm[0], Ps[1], Tmp, false, 1.0, 1.0, &Cm[2]))) return st;
+        if ((st = ops::mm(Ti, Tmp, Ps[2], false))) return st;
+      }
+    }
+    // Pmu = beta P0 (I - P0)
+    Sums sm, s0;
+    if ((st = ops::mm(P0, P0, Tmp, false, -beta, beta, &P0, 0.0, &sm))) return st;
+    if ((st = ops::sums_of(P0, &s0))) return st;
+    occ_err = std::fabs(s0.tr - nocc);
+    *mu0 = *mu0 + (nocc - s0.tr) / sm.tr;
+    for (int j = 0; j < order; ++j) {
+      Sums sj;
+      if ((st = ops::sums_of(Ps[j], &sj))) return st;
+      mu[j] = mu[j] - sj.tr / sm.tr;
+      occ_err += std::fabs(sj.tr);
+    }
+    if (k > 50) {
+      std::printf("Chemical potential is not converging\n");
+      break;
+    }
+  }
+  return OK;
+}
+
+}  // namespace prg
+
+using namespace prg;
+
+namespace {
+bool dev_inv(bml_matrix_t** inv, int nsteps, std::vector<DMat>& out, bool write) {
+  out.resize(nsteps);
+  for (int i = 0; i < nsteps; ++i) {
+    if (!inv || !inv[i]) {
+      set_error(ERR_ARG, "null inverse handle");
+      return false;
+    }
+    if (bml_dev_read(inv[i], &out[i])) return false;
+    if (write) inv[i]->host_valid = false;
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+// src/prg_c_interface.F90:470-490 (mu by value, as the reference's C path)
+void prg_implicit_fermi(bml_matrix_t* h_bml, bml_matrix_t* p_bml, int nsteps, int k, double nocc,
+                        double mu, double beta, int method, int osteps, double occErrLimit,
+                        double threshold, double tol) {
+  (void)threshold;
+  clear_error();
+  if (check_pair(h_bml, p_bml, "prg_implicit_fermi")) return;
+  DMat H, P;
+  if (bml_dev_read(h_bml, &H) || bml_dev_write(p_bml, &P)) return;
+  double m = mu;
+  if (implicit_fermi(H, P, nsteps, k, nocc, &m, beta, method, osteps, occErrLimit, tol)) return;
+  g_last_mu = m;
+}
+
+// Chemical potential after the last prg_implicit_fermi call (the reference's
+// C wrapper passes mu by value and loses the update).
+double prg_implicit_fermi_last_mu(void) { return g_last_mu; }
+
+// src/prg_c_interface.F90:440-468; occiter / totns by value as in the
+// reference header; prg_implicit_fermi_counters returns the updated counts.
+void prg_implicit_fermi_save_inverse(bml_matrix_t** Inv_bml, bml_matrix_t* h_bml,
+                                     bml_matrix_t* p_bml, int nsteps, double nocc, double* mu,
+                                     double beta, double occErrLimit, double threshold, double tol,
+                                     int SCF_IT, int occiter, int totns) {
+  (void)threshold;
+  clear_error();
+  if (!mu) {
+    set_error(ERR_ARG, "prg_implicit_fermi_save_inverse: null mu");
+    return;
+  }
+  if (check_pair(h_bml, p_bml, "prg_implicit_fermi_save_inverse")) return;
+  DMat H, P;
+  std::vector<DMat> inv;
+  if (bml_dev_read(h_bml, &H) || bml_dev_write(p_bml, &P)) return;
+  if (!dev_inv(Inv_bml, nsteps, inv, true)) return;
+  int oc = occiter, tn = totns;
+  if (implicit_fermi_save_inverse(inv, H, P, nsteps, nocc, mu, beta, occErrLimit, tol, SCF_IT,
+                                  &oc, &tn))
+    return;
+  g_last_occiter = oc;
+  g_last_totns = tn;
+}
+
+void prg_implicit_fermi_counters(int* occiter, int* totns) {
+  if (occiter) *occiter = g_last_occiter;
+  if (totns) *totns = g_last_totns;
+}
+
+// src/prg_c_interface.F90:492-506
+void prg_implicit_fermi_zero(bml_matrix_t* h_bml, bml_matrix_t* p_bml, int nsteps, double mu,
+                             int method, double threshold, double tol) {
+  (void)threshold;
+  clear_error();
+  if (check_pair(h_bml, p_bml, "prg_implicit_fermi_zero")) return;
+  DMat H, P;
+  if (bml_dev_read(h_bml, &H) || bml_dev_write(p_bml, &P)) return;
+  implicit_fermi_zero(H, P, nsteps, mu, method, tol);
+}
+
+// src/prg_c_interface.F90:508-533
+void prg_implicit_fermi_first_order_response(bml_matrix_t* H0_bml, bml_matrix_t* H1_bml,
+                                             bml_matrix_t* P0_bml, bml_matrix_t* P1_bml,
+                                             bml_matrix_t** Inv_bml, int nsteps, double mu0,
+                                             double beta, double nocc, double threshold) {
+  (void)threshold;
+  (void)nocc;
+  clear_error();
+  if (check_pair(H0_bml, H1_bml, "first_order_response") ||
+      check_pair(H0_bml, P0_bml, "first_order_response") ||
+      check_pair(H0_bml, P1_bml, "first_order_response"))
+    return;
+  DMat H0, H1, P0, P1;
+  std::vector<DMat> inv;
+  if (bml_dev_read(H0_bml, &H0) || bml_dev_read(H1_bml, &H1) || bml_dev_write(P0_bml, &P0) ||
+      bml_dev_write(P1_bml, &P1))
+    return;
+  if (!dev_inv(Inv_bml, nsteps, inv, false)) return;
+  implicit_fermi_first_order_response(H0, H1, P0, P1, inv, nsteps, mu0, beta);
+}
+
+// Declared (src/prg_progress_mod.h:240-258) but not implemented by the
+// reference's C interface.  Here: H_bml[0..order-1] are H1..H3, P_bml the
+// outputs P1..P3; mu0 is updated in place and mu[0..order-1] receives the
+// response chemical potentials.
+void prg_implicit_fermi_response(bml_matrix_t* H0_bml, bml_matrix_t** H_bml, bml_matrix_t* P0_bml,
+                                 bml_matrix_t** P_bml, int nsteps, double* mu0, double* mu,
+                                 double beta, double nocc, double occ_tol, double lin_tol,
+                                 int order, double threshold) {
+  (void)threshold;
+  clear_error();
+  if (!mu0 || !mu || !H_bml || !P_bml || order < 1 || order > 3) {
+    set_error(ERR_ARG, "prg_implicit_fermi_response: bad arguments");
+    return;
+  }
+  DMat H0, P0;
+  if (bml_dev_read(H0_bml, &H0) || bml_dev_write(P0_bml, &P0)) return;
+  std::vector<DMat> Hs(order), Ps(order);
+  for (int j = 0; j < order; ++j) {
+    if (check_pair(H0_bml, H_bml[j], "response") || check_pair(H0_bml, P_bml[j], "response"))
+      return;
+    if (bml_dev_read(H_bml[j], &Hs[j]) || bml_dev_write(P_bml[j], &Ps[j])) return;
+  }
+  implicit_fermi_response(H0, Hs, P0, Ps, nsteps, mu0, mu, beta, nocc, occ_tol, lin_tol, order);
+}
+
+}  // extern "C"

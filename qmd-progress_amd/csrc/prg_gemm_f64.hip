@@ -109,7 +109,8 @@ struct GemmArgs {
   double alpha, beta, gamma;
   const double* Cin;
   double* C;
-  double* part;  // per tile [Tr C, ||C||^2] or null
+  const double* dotE;  // optional: partial <C, E> (elementwise product sum)
+  double* part;        // per tile [Tr C, ||C||^2, <C, E>, 0] or null
 };
 
 __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
   const int ld = g.ld;
   const bool mirror = g.sym && (I != J);
   const double wgt = mirror ? 2.0 : 1.0;
-  double sums[2] = {0.0, 0.0};
+  double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
 #pragma unroll
@@ -146,12 +147,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
         v += (diag && row < g.n) ? g.gamma : 0.0;
         sums[0] += diag ? v : 0.0;
         sums[1] += wgt * v * v;
+        if (g.dotE) sums[2] += wgt * v * g.dotE[idx];
         g.C[idx] = v;
         if (mirror) g.C[col + (size_t)row * ld] = v;
       }
     }
   }
-  if (g.part) block_store_partials<2>(sums, g.part + 2 * (size_t)t);
+  if (g.part) block_store_partials<4>(sums, g.part + 4 * (size_t)t);
 }
 
 // ------------------------------------------------------------- SP2 step
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(NT, 2) void sp2_response_kernel(Sp2Args g) {
 }  // namespace
 
 int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, const DMat* Cin,
-         double gamma, DMat C, int sym, double* d_part, int* ntiles) {
+         double gamma, DMat C, int sym, double* d_part, int* ntiles, const DMat* dotE) {
   if (npair < 1 || npair > 2) {
     set_error(ERR_ARG, "gemm: npair must be 1 or 2");
     return ERR_ARG;
@@ -334,6 +336,7 @@ int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, con
   g.gamma = gamma;
   g.Cin = Cin ? Cin->p : nullptr;
   g.C = C.p;
+  g.dotE = dotE ? dotE->p : nullptr;
   g.part = d_part;
   const int nb = sym ? g.nt * (g.nt + 1) / 2 : g.nt * g.nt;
   if (ntiles) *ntiles = nb;

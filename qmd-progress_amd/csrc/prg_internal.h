@@ -137,9 +137,11 @@ int reduce_partials(const double* d_part, int nblocks, int nq, double* d_out);
 // GEMM family (FP64 MFMA 16x16x4).
 // C = alpha * sum_p A_p B_p + beta * Cin + gamma * I ; npair in {1, 2}.
 // sym != 0: the product is symmetric; only upper-triangle tiles are computed
-// and mirrored.  d_part (optional) receives per-tile [Tr(C), ||C||^2].
+// and mirrored (Cin must then be symmetric too).  d_part (optional) receives
+// per-tile [Tr(C), ||C||^2, <C, dotE>, 0].  C must not alias A_p or B_p.
 int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, const DMat* Cin,
-         double gamma, DMat C, int sym, double* d_part, int* ntiles);
+         double gamma, DMat C, int sym, double* d_part, int* ntiles,
+         const DMat* dotE = nullptr);
 
 // Fused SP2 step: Y = X*X, then per the device state's planned branch
 // Xn = Y or 2X - Y; epilogue partials [||Xn||^2, Tr Xn, idem, Tr Y].

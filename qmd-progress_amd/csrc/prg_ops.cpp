@@ -1,0 +1,108 @@
+// Host-side wrappers of the device matrix operations (see prg_ops.h).
+#include "prg_ops.h"
+
+#include <cstring>
+#include <string>
+
+namespace prg {
+
+int transpose_kernel_launch(const DMat& A, DMat out);
+int scale_cols_kernel_launch(const DMat& A, const double* d_f, DMat out);
+
+namespace ops {
+
+int temp(const char* name, int n, DMat* out) {
+  int st = OK;
+  const int ld = padded(n);
+  double* p = (double*)workspace(name, sizeof(double) * (size_t)ld * ld, &st);
+  if (st) return st;
+  out->n = n;
+  out->ld = ld;
+  out->p = p;
+  return OK;
+}
+
+int fetch(const double* d_part, int nb, int nq, double* host) {
+  int st = OK;
+  double* d_out = (double*)workspace("ops_sums", 8 * sizeof(double), &st);
+  double* h = (double*)host_staging(sizeof(Sp2State), &st);
+  if (st) return st;
+  if ((st = reduce_partials(d_part, nb, nq, d_out))) return st;
+  PRG_HIP_CHECK(hipMemcpyAsync(h, d_out, nq * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  std::memcpy(host, h, nq * sizeof(double));
+  return OK;
+}
+
+static double* part_buf(int ld, int* st) {
+  const int nt = ld / kTile;
+  return (double*)workspace("ops_part", sizeof(double) * 4 * (size_t)(nt * nt + 2048), st);
+}
+
+int mm(const DMat& A, const DMat& B, DMat C, bool sym, double alpha, double beta,
+       const DMat* Cin, double gamma, Sums* sums, const DMat* dotE) {
+  int st = OK;
+  double* part = sums ? part_buf(C.ld, &st) : nullptr;
+  if (st) return st;
+  int nt = 0;
+  if ((st = gemm(1, &A, &B, alpha, beta, Cin, gamma, C, sym ? 1 : 0, part, &nt, dotE))) return st;
+  if (sums) {
+    double h[4];
+    if ((st = fetch(part, nt, 4, h))) return st;
+    sums->tr = h[0];
+    sums->fro2 = h[1];
+    sums->dot = h[2];
+  }
+  return OK;
+}
+
+int mm2(const DMat& A0, const DMat& B0, const DMat& A1, const DMat& B1, DMat C, bool sym,
+        double alpha, double beta, const DMat* Cin, double gamma, Sums* sums) {
+  int st = OK;
+  double* part = sums ? part_buf(C.ld, &st) : nullptr;
+  if (st) return st;
+  const DMat A[2] = {A0, A1}, B[2] = {B0, B1};
+  int nt = 0;
+  if ((st = gemm(2, A, B, alpha, beta, Cin, gamma, C, sym ? 1 : 0, part, &nt))) return st;
+  if (sums) {
+    double h[4];
+    if ((st = fetch(part, nt, 4, h))) return st;
+    sums->tr = h[0];
+    sums->fro2 = h[1];
+    sums->dot = h[2];
+  }
+  return OK;
+}
+
+int lin(const DMat& A, double a, const DMat* B, double b, double c, DMat out, Sums* sums) {
+  int st = OK;
+  double* part = sums ? part_buf(out.ld, &st) : nullptr;
+  if (st) return st;
+  int nb = 0;
+  if (B) st = axpby(A, a, *B, b, c, out, part, &nb);
+  else st = affine(A, a, c, out, part, &nb);
+  if (st) return st;
+  if (sums) {
+    double h[2];
+    if ((st = fetch(part, nb, 2, h))) return st;
+    sums->tr = h[0];
+    sums->fro2 = h[1];
+  }
+  return OK;
+}
+
+int copy(const DMat& src, DMat dst) { return lin(src, 1.0, nullptr, 0.0, 0.0, dst); }
+int identity(DMat dst, double diag) { return set_identity(dst, diag); }
+int sums_of(const DMat& A, Sums* s) {
+  DMat T;
+  int st = temp("ops_sums_tmp", A.n, &T);
+  if (st) return st;
+  return lin(A, 1.0, nullptr, 0.0, 0.0, T, s);
+}
+int transpose(const DMat& A, DMat out) { return transpose_kernel_launch(A, out); }
+int scale_cols(const DMat& A, const double* d_f, DMat out) {
+  return scale_cols_kernel_launch(A, d_f, out);
+}
+
+}  // namespace ops
+}  // namespace prg

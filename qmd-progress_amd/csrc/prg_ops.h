@@ -1,0 +1,43 @@
+// Host-side matrix operations on device matrices: thin wrappers over the
+// GEMM / elementwise kernels that also return the fused reductions (trace,
+// squared Frobenius norm, <C, E>) to the host.  Used by the iterative
+// solvers whose scalar recurrences (CG step lengths, Newton-Schulz residuals,
+// chemical-potential updates) the reference runs on the host.
+#pragma once
+
+#include "prg_internal.h"
+
+namespace prg {
+namespace ops {
+
+struct Sums {
+  double tr = 0.0;
+  double fro2 = 0.0;
+  double dot = 0.0;
+};
+
+// n x n temporary held in a named workspace slot (contents undefined).
+int temp(const char* name, int n, DMat* out);
+// C = alpha A B + beta Cin + gamma I   (sym: product known symmetric)
+int mm(const DMat& A, const DMat& B, DMat C, bool sym, double alpha = 1.0, double beta = 0.0,
+       const DMat* Cin = nullptr, double gamma = 0.0, Sums* sums = nullptr,
+       const DMat* dotE = nullptr);
+// C = alpha (A0 B0 + A1 B1) + beta Cin + gamma I
+int mm2(const DMat& A0, const DMat& B0, const DMat& A1, const DMat& B1, DMat C, bool sym,
+        double alpha = 1.0, double beta = 0.0, const DMat* Cin = nullptr, double gamma = 0.0,
+        Sums* sums = nullptr);
+// out = a A + b B + c I   (B may be null; out may alias A or B)
+int lin(const DMat& A, double a, const DMat* B, double b, double c, DMat out,
+        Sums* sums = nullptr);
+int copy(const DMat& src, DMat dst);
+int identity(DMat dst, double diag = 1.0);
+int sums_of(const DMat& A, Sums* s);
+// out = A^T (out must not alias A)
+int transpose(const DMat& A, DMat out);
+// out[:, j] = A[:, j] * f[j]  (f: device vector of length n)
+int scale_cols(const DMat& A, const double* d_f, DMat out);
+// Reduce [nb][nq] device partials and copy the nq sums to the host.
+int fetch(const double* d_part, int nb, int nq, double* host);
+
+}  // namespace ops
+}  // namespace prg

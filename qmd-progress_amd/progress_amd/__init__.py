@@ -89,6 +89,25 @@ def lib():
         "prg_gpu_sp2_alg2_device": (i, [i, vp, vp, d, i, i, cp, d, i, ip, ip]),
         "prg_gpu_padded_ld": (i, [i]),
         "prg_gpu_timing": (None, [i]),
+        "prg_build_density_T0": (None, [i, vp, vp, d, d, dp]),
+        "prg_build_density_T_fermi": (None, [vp, vp, d, d, d, i, dp]),
+        "prg_get_eigenvalues": (None, [i, vp, dp, i]),
+        "prg_check_idempotency": (None, [vp, d, dp]),
+        "prg_get_density_matrix": (None, [vp, vp, d, d, d]),
+        "prg_test_density_matrix": (None, [vp, vp, d, d, d, i, d, d]),
+        "prg_test_density_matrix_mu": (d, []),
+        "prg_implicit_fermi": (None, [vp, vp, i, i, d, d, d, i, i, d, d, d]),
+        "prg_implicit_fermi_last_mu": (d, []),
+        "prg_implicit_fermi_save_inverse": (None, [C.POINTER(vp), vp, vp, i, d, dp, d, d, d, d,
+                                                   i, i, i]),
+        "prg_implicit_fermi_counters": (None, [ip, ip]),
+        "prg_implicit_fermi_zero": (None, [vp, vp, i, d, i, d, d]),
+        "prg_implicit_fermi_first_order_response": (None, [vp, vp, vp, vp, C.POINTER(vp), i, d,
+                                                           d, d, d]),
+        "prg_implicit_fermi_response": (None, [vp, C.POINTER(vp), vp, C.POINTER(vp), i, dp, dp,
+                                               d, d, d, d, i, d]),
+        "prg_compute_response_SP2": (None, [vp, vp, vp, vp, d, d, i, i, cp, d, d, i]),
+        "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
         "prg_gpu_step_timing": (i, [dp, ip]),
     }
     for name, (res, args) in sig.items():
@@ -339,3 +358,132 @@ def step_timing(enable=None):
     cnt = C.c_int()
     lib().prg_gpu_step_timing(C.byref(ms), C.byref(cnt))
     return ms.value, cnt.value
+
+
+# ---------------------------------------------------------------- eigensolver densities
+
+def build_density_T0(h, bndfil):
+    """prg_build_density_T0: returns (rho, eigenvalues)."""
+    n = h.shape[0]
+    w = _dvec(n)
+    rho = _h_rho(lambda a, b: lib().prg_build_density_T0(n, a, b, 0.0, bndfil, _dp(w)), h)
+    return rho, w
+
+
+def build_density_T_fermi(h, kbt, ef):
+    """prg_build_density_T_fermi: returns (rho, drho)."""
+    dr = C.c_double()
+    rho = _h_rho(lambda a, b: lib().prg_build_density_T_fermi(a, b, 0.0, kbt, ef, 0,
+                                                              C.byref(dr)), h)
+    return rho, dr.value
+
+
+def get_eigenvalues(h):
+    n = h.shape[0]
+    m = Matrix.from_numpy(h)
+    w = _dvec(n)
+    lib().prg_get_eigenvalues(n, m.h, _dp(w), 0)
+    _check()
+    return w
+
+
+def check_idempotency(a):
+    m = Matrix.from_numpy(a)
+    v = C.c_double()
+    lib().prg_check_idempotency(m.h, 0.0, C.byref(v))
+    _check()
+    return v.value
+
+
+def get_density_matrix(h, beta, mu):
+    return _h_rho(lambda a, b: lib().prg_get_density_matrix(a, b, beta, mu, 0.0), h)
+
+
+def test_density_matrix(h, beta, mu, nocc, osteps, occ_err_limit):
+    """prg_test_density_matrix: returns (P, mu)."""
+    p = _h_rho(lambda a, b: lib().prg_test_density_matrix(a, b, beta, mu, nocc, osteps,
+                                                          occ_err_limit, 0.0), h)
+    return p, lib().prg_test_density_matrix_mu()
+
+
+# ---------------------------------------------------------------- implicit Fermi
+
+def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, tol):
+    """prg_implicit_fermi: returns (P, mu)."""
+    p = _h_rho(lambda a, b: lib().prg_implicit_fermi(a, b, nsteps, k, nocc, mu, beta, method,
+                                                     osteps, occ_err_limit, 0.0, tol), h)
+    return p, lib().prg_implicit_fermi_last_mu()
+
+
+def _handles(mats):
+    arr = (C.c_void_p * len(mats))(*[m.h.value for m in mats])
+    return arr
+
+
+def implicit_fermi_save_inverse(inv, h, nsteps, nocc, mu, beta, occ_err_limit, tol, scf_it,
+                                occiter=0, totns=0):
+    """prg_implicit_fermi_save_inverse.  `inv` is a list of nsteps Matrix
+    handles (updated in place on the device).  Returns (P, mu, occiter, totns)."""
+    mu_c = C.c_double(mu)
+    hh = _handles(inv)
+    p = _h_rho(lambda a, b: lib().prg_implicit_fermi_save_inverse(
+        hh, a, b, nsteps, nocc, C.byref(mu_c), beta, occ_err_limit, 0.0, tol, scf_it, occiter,
+        totns), h)
+    oc, tn = C.c_int(), C.c_int()
+    lib().prg_implicit_fermi_counters(C.byref(oc), C.byref(tn))
+    return p, mu_c.value, oc.value, tn.value
+
+
+def implicit_fermi_zero(h, nsteps, mu, method, tol):
+    return _h_rho(lambda a, b: lib().prg_implicit_fermi_zero(a, b, nsteps, mu, method, 0.0, tol),
+                  h)
+
+
+def implicit_fermi_first_order_response(h0, h1, inv, nsteps, mu0, beta, nocc):
+    """Returns (P0, P1); `inv` is the list of Matrix handles of the inverses."""
+    n = h0.shape[0]
+    m0, m1 = Matrix.from_numpy(h0), Matrix.from_numpy(h1)
+    p0, p1 = Matrix.zeros(n), Matrix.zeros(n)
+    lib().prg_implicit_fermi_first_order_response(m0.h, m1.h, p0.h, p1.h, _handles(inv), nsteps,
+                                                  mu0, beta, nocc, 0.0)
+    _check()
+    return p0.numpy(), p1.numpy()
+
+
+def implicit_fermi_response(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, lin_tol=1e-10):
+    """Returns (P0, [P1..], mu0, [mu1..])."""
+    n = h0.shape[0]
+    m0 = Matrix.from_numpy(h0)
+    mh = [Matrix.from_numpy(x) for x in hs[:order]]
+    p0 = Matrix.zeros(n)
+    ps = [Matrix.zeros(n) for _ in range(order)]
+    mu0_c = C.c_double(mu0)
+    mu = _dvec(order)
+    lib().prg_implicit_fermi_response(m0.h, _handles(mh), p0.h, _handles(ps), nsteps,
+                                      C.byref(mu0_c), _dp(mu), beta, nocc, occ_tol, lin_tol,
+                                      order, 0.0)
+    _check()
+    return p0.numpy(), [p.numpy() for p in ps], mu0_c.value, mu.tolist()
+
+
+# ---------------------------------------------------------------- response
+
+def compute_response_sp2(h, h1, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
+    """prg_compute_response_SP2: returns (rho, rho1)."""
+    n = h.shape[0]
+    mh, m1 = Matrix.from_numpy(h), Matrix.from_numpy(h1)
+    rsp, rho = Matrix.zeros(n), Matrix.zeros(n)
+    lib().prg_compute_response_SP2(mh.h, m1.h, rsp.h, rho.h, 0.0, bndfil, minsp2iter,
+                                   maxsp2iter, sp2conv.encode(), idemtol, 0.0, 0)
+    _check()
+    return rho.numpy(), rsp.numpy()
+
+
+def canon_response(h1_eig, nocc, beta, evals, mu0, m):
+    n = h1_eig.shape[0]
+    mh = Matrix.from_numpy(h1_eig)
+    p1 = Matrix.zeros(n)
+    e = _dvec(n, evals)
+    lib().prg_canon_response(p1.h, mh.h, nocc, beta, _dp(e), mu0, m, n)
+    _check()
+    return p1.numpy()

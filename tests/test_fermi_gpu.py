@@ -1,0 +1,143 @@
+"""GPU parity of the eigensolver densities, the implicit Fermi family and the
+density-matrix responses against the oracle (FP64; tolerances in each test)."""
+import numpy as np
+import pytest
+
+import progress_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def P():
+    import progress_amd
+    progress_amd.set_precision(progress_amd.PREC_FP64)
+    return progress_amd
+
+
+@pytest.fixture(scope="module")
+def h256():
+    return O.model_hamiltonian(256, 0)
+
+
+def test_eigensolver_densities(P, h256):
+    print("[eigen] n=256", flush=True)
+    h = h256
+    rho, w = P.build_density_T0(h, 0.5)
+    rho_o, w_o = O.build_density_T0(h, 0.5)
+    assert np.allclose(w, w_o, rtol=0, atol=1e-11)
+    assert np.linalg.norm(rho - rho_o) < 1e-9
+    rho3, _ = P.build_density_T0(h, 0.4987)  # fractional occupation rule
+    rho3_o, _ = O.build_density_T0(h, 0.4987)
+    assert np.linalg.norm(rho3 - rho3_o) < 1e-9
+    rf, dr = P.build_density_T_fermi(h, 0.05, 0.6)
+    rf_o, dr_o = O.build_density_T_fermi(h, 0.05, 0.6)
+    assert np.linalg.norm(rf - rf_o) < 1e-9 and abs(dr - dr_o) < 1e-9 * max(1, abs(dr_o))
+    assert np.allclose(P.get_eigenvalues(h), w_o, atol=1e-11)
+    assert abs(P.check_idempotency(0.5 * rho_o) - O.check_idempotency(0.5 * rho_o)) < 1e-10
+    pd = P.get_density_matrix(h, 20.0, 0.6)
+    assert np.linalg.norm(pd - O.get_density_matrix(h, 20.0, 0.6)) < 1e-9
+    pt, mu = P.test_density_matrix(h, 20.0, 0.6, 128.0, 0, 1e-9)
+    pt_o, mu_o = O.test_density_matrix(h, 20.0, 0.6, 128.0, 0, 1e-9)
+    assert abs(mu - mu_o) < 1e-9 and np.linalg.norm(pt - pt_o) < 1e-8
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_implicit_fermi(P, h256, method):
+    print(f"[implicit_fermi] method={method}", flush=True)
+    h = h256
+    # method 0: CG to ||R||_F^2 <= 1e-16; method 1: Newton-Schulz to ||I - Y Ai||_F <= 1e-10
+    args = (8, 2, 128.0, 0.6, 20.0, method, 3, 1e-9, 1e-16 if method == 0 else 1e-10)
+    p, mu = P.implicit_fermi(h, *args)
+    p_o, mu_o = O.implicit_fermi(h, *args)
+    assert abs(mu - mu_o) < 1e-9
+    assert np.linalg.norm(p - p_o) < 1e-8
+    if method == 1:  # the Newton-Schulz variant solves the same recursion as CG
+        p0, _ = O.implicit_fermi(h, 8, 2, 128.0, 0.6, 20.0, 0, 3, 1e-9, 1e-16)
+        assert np.linalg.norm(p - p0) < 1e-6
+
+
+def test_implicit_fermi_reference_tolerance(P, golden):
+    """tests/src/main.F90:202-227 on the shipped 384-orbital Hamiltonian:
+    nsteps 10, k 2, nocc 10, mu 0.2, beta 4, CG, one mu step;
+    ||P - P_diag||_F <= 0.1 against prg_test_density_matrix."""
+    print("[implicit_fermi ref test] n=384", flush=True)
+    h = O.read_mtx(str(golden / "hamiltonian_ortho_384.mtx"))
+    h = 0.5 * (h + h.T)
+    p, _ = P.implicit_fermi(h, 10, 2, 10.0, 0.2, 4.0, 0, 1, 1.0, 10e-8)
+    pt, _ = P.test_density_matrix(h, 4.0, 0.2, 10.0, 1, 1.0)
+    assert np.linalg.norm(p - pt) <= 0.1
+
+
+def test_save_inverse_and_first_order_response(P, h256):
+    print("[save_inverse + first order response] n=256", flush=True)
+    h = h256
+    n, nsteps = 256, 8
+    inv = [P.Matrix.from_numpy(np.eye(n)) for _ in range(nsteps)]
+    p, mu, oc, tn = P.implicit_fermi_save_inverse(inv, h, nsteps, 128.0, 0.6, 20.0, 1e-8, 1e-10, 1)
+    inv_o = [np.eye(n)] * nsteps
+    p_o, mu_o, inv_o, it_o, ns_o = O.implicit_fermi_save_inverse(inv_o, h, nsteps, 128.0, 0.6,
+                                                                  20.0, 1e-8, 1e-10, 1)
+    assert abs(mu - mu_o) < 1e-9
+    assert np.linalg.norm(p - p_o) < 1e-8
+    assert oc == it_o and tn == ns_o
+    for a, b in zip(inv, inv_o):
+        assert np.linalg.norm(a.numpy() - b) < 1e-8
+    # the reference's tolerance check (tests/src/main.F90:229-258)
+    pt, _ = P.test_density_matrix(h, 20.0, mu, 128.0, 1, 1e-4)
+    assert np.linalg.norm(0.5 * p - pt) <= 0.1
+    h1 = O.banded_perturbation(n, 8, seed=3)
+    p0, p1 = P.implicit_fermi_first_order_response(h, h1, inv, nsteps, mu, 20.0, 128.0)
+    p0_o, p1_o = O.implicit_fermi_first_order_response(h, h1, [a.numpy() for a in inv], nsteps,
+                                                       mu, 20.0)
+    assert np.linalg.norm(p0 - p0_o) < 1e-9
+    assert np.linalg.norm(p1 - p1_o) < 1e-9 * max(1.0, np.linalg.norm(p1_o))
+
+
+def test_implicit_fermi_zero(P, h256):
+    print("[implicit_fermi_zero]", flush=True)
+    for method in (0, 1):
+        p = P.implicit_fermi_zero(h256, 6, 0.6, method, 1e-16)
+        p_o = O.implicit_fermi_zero(h256, 6, 0.6, method, 1e-16)
+        # CG stops at ||R||_F^2 <= tol: agreement to ~sqrt(tol) * condition
+        assert np.linalg.norm(p - p_o) < 1e-6
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_implicit_fermi_response_orders(P, order):
+    print(f"[implicit_fermi_response] order={order}", flush=True)
+    n = 128
+    h0 = O.model_hamiltonian(n, 4)
+    hs = [O.banded_perturbation(n, 4, seed=10 + j) * 0.1 for j in range(3)]
+    p0, ps, mu0, mu = P.implicit_fermi_response(h0, hs, 6, 0.6, 10.0, 64.0, 1e-6, order,
+                                                lin_tol=1e-10)
+    p0_o, ps_o, mu0_o, mu_o = O.implicit_fermi_response(h0, hs, 6, 0.6, 10.0, 64.0, 1e-6, order,
+                                                        lin_tol=1e-10)
+    assert abs(mu0 - mu0_o) < 1e-8
+    assert np.allclose(mu, mu_o, atol=1e-8)
+    assert np.linalg.norm(p0 - p0_o) < 1e-8
+    for a, b in zip(ps, ps_o):
+        assert np.linalg.norm(a - b) < 1e-7 * max(1.0, np.linalg.norm(b))
+
+
+def test_response_sp2(P):
+    print("[response_sp2] n=192", flush=True)
+    n = 192
+    h = O.model_hamiltonian(n, 1)
+    h1 = O.banded_perturbation(n, 8, seed=7)
+    rho, rho1 = P.compute_response_sp2(h, h1, 0.5, 10, 200, "Abs", 1e-10)
+    rho_o, rho1_o, it, pp = O.compute_response_sp2(h, h1, 0.5, 10, 200, "Abs", 1e-10)
+    assert np.linalg.norm(rho - rho_o) < 1e-9
+    assert np.linalg.norm(rho1 - rho1_o) < 1e-9 * max(1.0, np.linalg.norm(rho1_o))
+
+
+def test_canon_response(P):
+    print("[canon_response] n=160", flush=True)
+    n = 160
+    h = O.model_hamiltonian(n, 2)
+    w, q = np.linalg.eigh(h)
+    h1e = q.T @ O.banded_perturbation(n, 6, seed=1) @ q
+    h1e = 0.5 * (h1e + h1e.T)
+    p1 = P.canon_response(h1e, 80.0, 10.0, w, 0.6, 6)
+    p1_o = O.canon_response(h1e, 80.0, 10.0, w, 0.6, 6)
+    assert np.linalg.norm(p1 - p1_o) < 1e-10 * max(1.0, np.linalg.norm(p1_o))
