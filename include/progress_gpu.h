@@ -237,6 +237,21 @@ int prg_gpu_sp2_alg2_device(int N, const double *dH, double *dRho, double bndfil
 /* Padded leading dimension used by the device-pointer API. */
 int prg_gpu_padded_ld(int N);
 
+/* Gershgorin interval of a device-resident H (padded layout) into bounds[2] (host). */
+int prg_gpu_gershgorin_device(int N, const double *dH, double *bounds);
+/* prg_prg_sp2_alg2_seq_inplace semantics on device pointers: replay the branch
+ * sequence pp[icount] with the given normalisation bounds; dRho = 2 X. */
+int prg_gpu_sp2_seq_device(int N, const double *dH, double *dRho, const int *pp, int icount,
+                           double mineval, double maxeval);
+/* Reverse-mode adjoint of the SP2 recursion (no reference counterpart; the
+ * transpose of the tangent recursion of prg_compute_response_SP2,
+ * prg_response_mod.F90:719-755): given dL/dD in dGout, writes dL/dH to dGrad for
+ * the forward solve that took the branch sequence pp[iters] with normalisation
+ * bounds (mineval, maxeval), both frozen.  Checkpoints every `ckpt` steps. */
+int prg_gpu_sp2_adjoint_device(int N, const double *dH, const double *dGout, double *dGrad,
+                               const int *pp, int iters, double mineval, double maxeval,
+                               int ckpt);
+
 /* Opt-in timing of the SP2 product kernel launches with HIP events on the
  * library stream (enable resets the counters); prg_gpu_step_timing returns
  * the summed milliseconds and the number of timed products.  Measurement

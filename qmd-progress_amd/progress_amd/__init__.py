@@ -89,6 +89,9 @@ def lib():
         "prg_gpu_sp2_alg2_device": (i, [i, vp, vp, d, i, i, cp, d, i, ip, ip]),
         "prg_gpu_padded_ld": (i, [i]),
         "prg_gpu_timing": (None, [i]),
+        "prg_gpu_gershgorin_device": (i, [i, vp, dp]),
+        "prg_gpu_sp2_seq_device": (i, [i, vp, vp, ip, i, d, d]),
+        "prg_gpu_sp2_adjoint_device": (i, [i, vp, vp, vp, ip, i, d, d, i]),
         "prg_build_density_T0": (None, [i, vp, vp, d, d, dp]),
         "prg_build_density_T_fermi": (None, [vp, vp, d, d, d, i, dp]),
         "prg_get_eigenvalues": (None, [i, vp, dp, i]),
@@ -487,3 +490,36 @@ def canon_response(h1_eig, nocc, beta, evals, mu0, m):
     lib().prg_canon_response(p1.h, mh.h, nocc, beta, _dp(e), mu0, m, n)
     _check()
     return p1.numpy()
+
+
+# ---------------------------------------------------------------- device-pointer helpers
+
+def _st(code):
+    if code != 0:
+        raise ProgressError(code, lib().prg_last_error().decode())
+
+
+def gershgorin_device(dH, n):
+    import torch
+    torch.cuda.synchronize()
+    b = _dvec(2)
+    _st(lib().prg_gpu_gershgorin_device(n, C.c_void_p(dH.data_ptr()), _dp(b)))
+    return float(b[0]), float(b[1])
+
+
+def sp2_seq_device(dH, dRho, n, pp, mineval, maxeval):
+    import torch
+    torch.cuda.synchronize()
+    ppa = _ivec(len(pp), pp)
+    _st(lib().prg_gpu_sp2_seq_device(n, C.c_void_p(dH.data_ptr()), C.c_void_p(dRho.data_ptr()),
+                                     _ip(ppa), len(pp), mineval, maxeval))
+
+
+def sp2_adjoint_device(dH, dGout, dGrad, n, pp, mineval, maxeval, ckpt=8):
+    import torch
+    torch.cuda.synchronize()
+    ppa = _ivec(len(pp), pp)
+    _st(lib().prg_gpu_sp2_adjoint_device(n, C.c_void_p(dH.data_ptr()),
+                                         C.c_void_p(dGout.data_ptr()),
+                                         C.c_void_p(dGrad.data_ptr()), _ip(ppa), len(pp),
+                                         mineval, maxeval, ckpt))
