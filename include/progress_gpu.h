@@ -209,6 +209,47 @@ void prg_compute_response_SP2(bml_matrix_t *ham_bml, bml_matrix_t *prt_bml, bml_
 void prg_canon_response(bml_matrix_t *P1_bml, bml_matrix_t *H1_bml, double nocc, double beta,
                         const double *evals, double mu0, int m, int HDIM);
 
+/* ------------------------------------------------------------------ graph partitioning */
+/* Flat C counterpart of graph_partitioning_t (src/prg_graph_mod.F90:58-140) for
+ * one rank: 0-based node ids; lsize / llsize / core_halo_index are filled by
+ * prg_subgraphSP2Loop (or prg_getPartitionHalosFromGraph); pp / maxIter /
+ * mineval / maxeval are the caller's inputs to the loop (prg_sp2sequence,
+ * prg_gershgorin); vv receives sqrt of the per-iteration core-row sums. */
+typedef struct prg_graph_partitioning {
+  int totalParts;
+  int totalNodes;
+  int *nnodesInPart;     /* [totalParts] core sizes */
+  int **nodeInPart;      /* [totalParts][nnodesInPart[i]] core node ids */
+  int *lsize;            /* [totalParts] core + halo sizes */
+  int *llsize;           /* [totalParts] core sizes */
+  int **core_halo_index; /* [totalParts][lsize[i]] core first, then halo */
+  int pp[100];
+  int maxIter;
+  double ehomo, elumo, mineval, maxeval;
+  double vv[100];
+} prg_graph_partitioning_t;
+
+/* src/prg_graph_mod.F90:359-399 */
+void prg_equalPartition(prg_graph_partitioning_t *gp, int nodesPerPart, int nnodes);
+/* src/prg_graph_mod.F90:880-929 */
+void prg_filePartition(prg_graph_partitioning_t *gp, const char *partFile);
+/* Partition from flat arrays (a partitioner's output); no reference counterpart. */
+void prg_setPartition(prg_graph_partitioning_t *gp, int nparts, int nnodes, const int *sizes,
+                      const int *nodes);
+/* prg_destroyGraphPartitioning (src/prg_graph_mod.F90:268-290) */
+void prg_destroyGraphPartitioning(prg_graph_partitioning_t *gp);
+/* src/prg_subgraphloop_mod.F90:336-368 */
+void prg_getPartitionHalosFromGraph(prg_graph_partitioning_t *gp, bml_matrix_t *g_bml,
+                                    int djflag);
+/* src/prg_graph_mod.F90:933-971 */
+void prg_fnormGraph(prg_graph_partitioning_t *gp);
+/* New C symbol for prg_subgraphSP2Loop (src/prg_subgraphloop_mod.F90:36-126):
+ * all parts purified in lock-step by batched GPU kernels. */
+void prg_subgraphSP2Loop(bml_matrix_t *h_bml, bml_matrix_t *g_bml, bml_matrix_t *rho_bml,
+                         prg_graph_partitioning_t *gp, double threshold);
+/* Device time (ms) of the last prg_subgraphSP2Loop phases. */
+void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scatter);
+
 /* ------------------------------------------------------------------ precision modes */
 /* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).
  *   PRG_PREC_FP64      FP64 MFMA;

@@ -1,0 +1,32 @@
+// Graph-partitioned SP2: packed batch plan shared by the host driver and the
+// batched kernels.
+#pragma once
+
+#include "prg_internal.h"
+
+namespace prg {
+
+struct PartDesc {
+  size_t offset;   // first double of the part's padded m x m block in the packed buffer
+  int m;           // padded size (multiple of 64)
+  int n;           // core + halo size
+  int core;        // core rows (first `core` indices)
+  int idx_offset;  // first entry of the part's index list in the packed index array
+};
+
+struct BatchPlan {
+  int nparts = 0;
+  int ntiles = 0;
+  size_t total = 0;          // doubles in one packed buffer
+  PartDesc* d_parts = nullptr;
+  int4* d_tiles = nullptr;
+  int* d_idx = nullptr;
+};
+
+int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double mineval,
+                   double maxeval);
+int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
+                 double* d_vv, int it);
+int batched_scatter(const BatchPlan& plan, const double* x, DMat rho);
+
+}  // namespace prg

@@ -89,6 +89,14 @@ def lib():
         "prg_gpu_sp2_alg2_device": (i, [i, vp, vp, d, i, i, cp, d, i, ip, ip]),
         "prg_gpu_padded_ld": (i, [i]),
         "prg_gpu_timing": (None, [i]),
+        "prg_equalPartition": (None, [vp, i, i]),
+        "prg_filePartition": (None, [vp, cp]),
+        "prg_setPartition": (None, [vp, i, i, ip, ip]),
+        "prg_destroyGraphPartitioning": (None, [vp]),
+        "prg_getPartitionHalosFromGraph": (None, [vp, vp, i]),
+        "prg_fnormGraph": (None, [vp]),
+        "prg_subgraphSP2Loop": (None, [vp, vp, vp, vp, d]),
+        "prg_gpu_subgraph_timing": (None, [dp, dp, dp]),
         "prg_gpu_gershgorin_device": (i, [i, vp, dp]),
         "prg_gpu_sp2_seq_device": (i, [i, vp, vp, ip, i, d, d]),
         "prg_gpu_sp2_adjoint_device": (i, [i, vp, vp, vp, ip, i, d, d, i]),
@@ -523,3 +531,84 @@ def sp2_adjoint_device(dH, dGout, dGrad, n, pp, mineval, maxeval, ckpt=8):
                                          C.c_void_p(dGout.data_ptr()),
                                          C.c_void_p(dGrad.data_ptr()), _ip(ppa), len(pp),
                                          mineval, maxeval, ckpt))
+
+
+# ---------------------------------------------------------------- graph partitioning
+
+class GraphPartitioning(C.Structure):
+    """prg_graph_partitioning_t (include/progress_gpu.h)."""
+    _fields_ = [("totalParts", C.c_int), ("totalNodes", C.c_int),
+                ("nnodesInPart", C.POINTER(C.c_int)),
+                ("nodeInPart", C.POINTER(C.POINTER(C.c_int))),
+                ("lsize", C.POINTER(C.c_int)), ("llsize", C.POINTER(C.c_int)),
+                ("core_halo_index", C.POINTER(C.POINTER(C.c_int))),
+                ("pp", C.c_int * 100), ("maxIter", C.c_int),
+                ("ehomo", C.c_double), ("elumo", C.c_double),
+                ("mineval", C.c_double), ("maxeval", C.c_double),
+                ("vv", C.c_double * 100)]
+
+    def __init__(self):
+        super().__init__()
+
+    def parts(self):
+        return [[self.nodeInPart[i][j] for j in range(self.nnodesInPart[i])]
+                for i in range(self.totalParts)]
+
+    def halos(self):
+        return [[self.core_halo_index[i][j] for j in range(self.lsize[i])]
+                for i in range(self.totalParts)]
+
+    def destroy(self):
+        lib().prg_destroyGraphPartitioning(C.byref(self))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def equal_partition(nodes_per_part, nnodes):
+    gp = GraphPartitioning()
+    lib().prg_equalPartition(C.byref(gp), nodes_per_part, nnodes)
+    _check()
+    return gp
+
+
+def file_partition(path):
+    gp = GraphPartitioning()
+    lib().prg_filePartition(C.byref(gp), str(path).encode())
+    _check()
+    return gp
+
+
+def set_partition(parts, nnodes):
+    gp = GraphPartitioning()
+    sizes = np.array([len(p) for p in parts], dtype=np.int32)
+    nodes = np.concatenate([np.asarray(p, dtype=np.int32) for p in parts])
+    lib().prg_setPartition(C.byref(gp), len(parts), nnodes, _ip(sizes), _ip(nodes))
+    _check()
+    return gp
+
+
+def subgraph_sp2_loop(h, g, gp, pp, max_iter, mineval, maxeval, vv=None):
+    """prg_subgraphSP2Loop: returns (rho, vv[:max_iter]); gp is filled with
+    the core+halo lists."""
+    n = h.shape[0]
+    mh, mg = Matrix.from_numpy(h), Matrix.from_numpy(g)
+    rho = Matrix.zeros(n)
+    gp.maxIter = max_iter
+    for k, v in enumerate(pp):
+        gp.pp[k] = int(v)
+    for k in range(100):
+        gp.vv[k] = 0.0 if vv is None or k >= len(vv) else float(vv[k])
+    gp.mineval, gp.maxeval = mineval, maxeval
+    lib().prg_subgraphSP2Loop(mh.h, mg.h, rho.h, C.byref(gp), 0.0)
+    _check()
+    return rho.numpy(), [gp.vv[k] for k in range(max_iter)]
+
+
+def subgraph_timing():
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    lib().prg_gpu_subgraph_timing(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value

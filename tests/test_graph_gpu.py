@@ -1,0 +1,75 @@
+"""GPU parity of the graph-partitioned SP2 loop (batched kernels) against the
+oracle, and the reference's own subgraph test on its shipped fixture."""
+import numpy as np
+import pytest
+
+import progress_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def P():
+    import progress_amd
+    progress_amd.set_precision(progress_amd.PREC_FP64)
+    return progress_amd
+
+
+def _sequence(h, P):
+    lo, hi = O.gershgorin(h)
+    rho, pp, ic, vv = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    eh, el, eg = O.homolumogap(vv, ic, pp, lo, hi)
+    seq, imax = O.sp2sequence(lo, hi, eh, el, 1e-12)
+    return rho, lo, hi, seq, imax
+
+
+@pytest.mark.parametrize("n,per_part,gthr", [(384, 48, 1e-4), (500, 37, 1e-3)])
+def test_subgraph_loop_matches_oracle(P, n, per_part, gthr):
+    print(f"[subgraph] n={n} per_part={per_part}", flush=True)
+    h = O.model_hamiltonian(n, 3)
+    rho_full, lo, hi, seq, imax = _sequence(h, P)
+    g = np.where(np.abs(rho_full) >= gthr, rho_full, 0.0)
+    parts = O.equal_partition(per_part, n)
+    rho_o, vv_o = O.subgraph_sp2_loop(h, g, parts, seq, imax, lo, hi)
+    gp = P.equal_partition(per_part, n)
+    rho_g, vv_g = P.subgraph_sp2_loop(h, g, gp, seq, imax, lo, hi)
+    halos_o = [O.matrix2submatrix_index(g, c)[0] for c in parts]
+    assert gp.halos() == halos_o
+    assert np.linalg.norm(rho_g - rho_o) < 1e-9
+    assert np.allclose(vv_g, vv_o, rtol=1e-9, atol=1e-12)
+
+
+def test_subgraph_loop_block_diagonal_is_exact(P):
+    print("[subgraph block diagonal]", flush=True)
+    blocks = [O.model_hamiltonian(80, seed=s) for s in range(3)]
+    n = 240
+    h = np.zeros((n, n))
+    for b, blk in enumerate(blocks):
+        h[80 * b:80 * b + 80, 80 * b:80 * b + 80] = blk
+    rho_full, lo, hi, seq, imax = _sequence(h, P)
+    g = (np.abs(h) > 0).astype(float)
+    gp = P.equal_partition(40, n)
+    rho_g, _ = P.subgraph_sp2_loop(h, g, gp, seq, imax, lo, hi)
+    ref, _ = O.sp2_alg2_seq_inplace(h, seq, imax, lo, hi)
+    assert np.linalg.norm(rho_g - ref) < 1e-9
+
+
+def test_reference_subgraphsp2_equal(P, golden):
+    """tests/src/main.F90:847-883 + tests/src/test_subgraphloop.F90:59-152:
+    poly.512.mtx, genseq (Rel, minsp2iter 25), graph = density thresholded at
+    1e-3, equal parts of 48 nodes, sequence with errlimit 1e-12;
+    idempotency of rho/2 <= 1e-2."""
+    print("[subgraph reference test] n=6144", flush=True)
+    h = O.read_mtx(str(golden / "poly.512.mtx.gz"))
+    n = h.shape[0]
+    rho, pp, ic, vv = P.sp2_alg2_genseq(h, 0.5, 25, 100, "Rel", 1e-10)
+    lo, hi = P.gershgorin(h)
+    eh, el, eg = P.homolumogap(vv, ic, pp, lo, hi)
+    seq, imax = P.sp2sequence(lo, hi, eh, el, 1e-12)
+    g = np.where(np.abs(rho) >= 1e-3, rho, 0.0)
+    gp = P.equal_partition(48, n)
+    rho_g, vvg = P.subgraph_sp2_loop(h, g, gp, seq, imax, lo, hi)
+    x = 0.5 * rho_g
+    assert np.linalg.norm(x @ x - x) <= 1e-2
+    lsz = [gp.lsize[i] for i in range(gp.totalParts)]
+    print(f"  parts={gp.totalParts} lsize range=({min(lsz)}, {max(lsz)}) iters={imax}")
