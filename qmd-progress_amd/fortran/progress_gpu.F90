@@ -1,0 +1,762 @@
+! Fortran module layer over libprogress_gpu.so (iso_c_binding).
+!
+! Callers keep the reference's module and procedure names and Fortran
+! signatures (`use bml`, `use prg_sp2_mod`, ...): src/prg_sp2_mod.F90,
+! src/prg_normalize_mod.F90, src/prg_homolumo_mod.F90,
+! src/prg_densitymatrix_mod.F90, src/prg_implicit_fermi_mod.F90 and
+! src/prg_response_mod.F90.  The `bml` module here is the minimal dense
+! subset those callers use (BML itself is not available to this build); its
+! handle wraps the library's C matrix handle.
+
+module bml
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter, public :: bml_dp = kind(1.0d0)
+  character(len=*), parameter, public :: bml_matrix_dense = "dense"
+  character(len=*), parameter, public :: bml_matrix_ellpack = "ellpack"
+  character(len=*), parameter, public :: bml_element_real = "real"
+
+  type, public :: bml_matrix_t
+    type(c_ptr) :: ptr = c_null_ptr
+  end type bml_matrix_t
+
+  public :: bml_zero_matrix, bml_identity_matrix, bml_read_matrix, bml_deallocate
+  public :: bml_get_n, bml_import_from_dense, bml_export_to_dense, bml_scale, bml_trace
+  public :: bml_copy, bml_write_matrix, prg_gpu_last_error
+
+  interface
+    function c_bml_zero_matrix(t, p, n, m, d) bind(C, name="bml_zero_matrix") result(r)
+      import :: c_int, c_ptr
+      integer(c_int), value :: t, p, n, m, d
+      type(c_ptr) :: r
+    end function
+    function c_bml_identity_matrix(t, p, n, m, d) bind(C, name="bml_identity_matrix") result(r)
+      import :: c_int, c_ptr
+      integer(c_int), value :: t, p, n, m, d
+      type(c_ptr) :: r
+    end function
+    function c_bml_import(t, p, o, n, m, a, thr, d) bind(C, name="bml_import_from_dense") result(r)
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: t, p, o, n, m, d
+      type(c_ptr), value :: a
+      real(c_double), value :: thr
+      type(c_ptr) :: r
+    end function
+    function c_bml_export(a, o) bind(C, name="bml_export_to_dense") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a
+      integer(c_int), value :: o
+      type(c_ptr) :: r
+    end function
+    subroutine c_bml_read(a, f) bind(C, name="bml_read_bml_matrix")
+      import :: c_ptr, c_char
+      type(c_ptr), value :: a
+      character(kind=c_char) :: f(*)
+    end subroutine
+    subroutine c_bml_write(a, f) bind(C, name="bml_write_bml_matrix")
+      import :: c_ptr, c_char
+      type(c_ptr), value :: a
+      character(kind=c_char) :: f(*)
+    end subroutine
+    subroutine c_bml_deallocate(a) bind(C, name="bml_deallocate")
+      import :: c_ptr
+      type(c_ptr) :: a
+    end subroutine
+    function c_bml_get_n(a) bind(C, name="bml_get_N") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: a
+      integer(c_int) :: r
+    end function
+    subroutine c_bml_scale(s, a, b) bind(C, name="bml_scale")
+      import :: c_ptr
+      type(c_ptr), value :: s, a, b
+    end subroutine
+    function c_bml_trace(a) bind(C, name="bml_trace") result(r)
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a
+      real(c_double) :: r
+    end function
+    subroutine c_bml_copy(a, b) bind(C, name="bml_copy")
+      import :: c_ptr
+      type(c_ptr), value :: a, b
+    end subroutine
+    subroutine c_free(p) bind(C, name="free")
+      import :: c_ptr
+      type(c_ptr), value :: p
+    end subroutine
+    function c_last_error_code() bind(C, name="prg_last_error_code") result(r)
+      import :: c_int
+      integer(c_int) :: r
+    end function
+  end interface
+
+contains
+
+  integer function type_code(matrix_type)
+    character(len=*), intent(in) :: matrix_type
+    type_code = 1
+    if (trim(matrix_type) == "ellpack") type_code = 2
+  end function type_code
+
+  !> Status of the last library call (0 = success).
+  integer function prg_gpu_last_error()
+    prg_gpu_last_error = c_last_error_code()
+  end function prg_gpu_last_error
+
+  subroutine bml_zero_matrix(matrix_type, element_type, element_precision, n, m, a, distrib_mode)
+    character(len=*), intent(in) :: matrix_type, element_type
+    integer, intent(in) :: element_precision, n, m
+    type(bml_matrix_t), intent(inout) :: a
+    character(len=*), optional, intent(in) :: distrib_mode
+    if (c_associated(a%ptr)) call c_bml_deallocate(a%ptr)
+    a%ptr = c_bml_zero_matrix(type_code(matrix_type), 2, n, m, 0)
+  end subroutine bml_zero_matrix
+
+  subroutine bml_identity_matrix(matrix_type, element_type, element_precision, n, m, a, &
+       distrib_mode)
+    character(len=*), intent(in) :: matrix_type, element_type
+    integer, intent(in) :: element_precision, n, m
+    type(bml_matrix_t), intent(inout) :: a
+    character(len=*), optional, intent(in) :: distrib_mode
+    if (c_associated(a%ptr)) call c_bml_deallocate(a%ptr)
+    a%ptr = c_bml_identity_matrix(type_code(matrix_type), 2, n, m, 0)
+  end subroutine bml_identity_matrix
+
+  subroutine bml_read_matrix(a, filename)
+    type(bml_matrix_t), intent(inout) :: a
+    character(len=*), intent(in) :: filename
+    call c_bml_read(a%ptr, trim(filename)//c_null_char)
+  end subroutine bml_read_matrix
+
+  subroutine bml_write_matrix(a, filename)
+    type(bml_matrix_t), intent(in) :: a
+    character(len=*), intent(in) :: filename
+    call c_bml_write(a%ptr, trim(filename)//c_null_char)
+  end subroutine bml_write_matrix
+
+  subroutine bml_deallocate(a)
+    type(bml_matrix_t), intent(inout) :: a
+    if (c_associated(a%ptr)) call c_bml_deallocate(a%ptr)
+    a%ptr = c_null_ptr
+  end subroutine bml_deallocate
+
+  integer function bml_get_n(a)
+    type(bml_matrix_t), intent(in) :: a
+    bml_get_n = c_bml_get_n(a%ptr)
+  end function bml_get_n
+
+  subroutine bml_import_from_dense(matrix_type, a_dense, a, threshold, m, distrib_mode)
+    character(len=*), intent(in) :: matrix_type
+    real(bml_dp), target, intent(in) :: a_dense(:, :)
+    type(bml_matrix_t), intent(inout) :: a
+    real(bml_dp), intent(in) :: threshold
+    integer, intent(in) :: m
+    character(len=*), optional, intent(in) :: distrib_mode
+    if (c_associated(a%ptr)) call c_bml_deallocate(a%ptr)
+    a%ptr = c_bml_import(type_code(matrix_type), 2, 1, size(a_dense, 1), m, c_loc(a_dense), &
+         threshold, 0)
+  end subroutine bml_import_from_dense
+
+  subroutine bml_export_to_dense(a, a_dense)
+    type(bml_matrix_t), intent(in) :: a
+    real(bml_dp), intent(inout) :: a_dense(:, :)
+    type(c_ptr) :: p
+    real(bml_dp), pointer :: buf(:, :)
+    integer :: n
+    n = c_bml_get_n(a%ptr)
+    p = c_bml_export(a%ptr, 1)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, buf, [n, n])
+    a_dense(1:n, 1:n) = buf
+    call c_free(p)
+  end subroutine bml_export_to_dense
+
+  subroutine bml_scale(alpha, a)
+    real(bml_dp), target, intent(in) :: alpha
+    type(bml_matrix_t), intent(inout) :: a
+    real(bml_dp), target :: s
+    s = alpha
+    call c_bml_scale(c_loc(s), a%ptr, a%ptr)
+  end subroutine bml_scale
+
+  real(bml_dp) function bml_trace(a)
+    type(bml_matrix_t), intent(in) :: a
+    bml_trace = c_bml_trace(a%ptr)
+  end function bml_trace
+
+  subroutine bml_copy(a, b)
+    type(bml_matrix_t), intent(in) :: a
+    type(bml_matrix_t), intent(inout) :: b
+    call c_bml_copy(a%ptr, b%ptr)
+  end subroutine bml_copy
+
+end module bml
+
+module prg_gpu_cstr
+  use iso_c_binding
+  implicit none
+contains
+  function cstr(s) result(r)
+    character(len=*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s) + 1) :: r
+    r = trim(s)//c_null_char
+  end function cstr
+end module prg_gpu_cstr
+
+module prg_sp2_mod
+  use iso_c_binding
+  use bml
+  use prg_gpu_cstr
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_sp2_basic, prg_sp2_alg1, prg_sp2_alg2, prg_sp2_alg2_genseq, prg_sp2_alg1_genseq
+  public :: prg_sp2_alg2_seq, prg_sp2_alg1_seq, prg_prg_sp2_alg2_seq_inplace
+  public :: prg_prg_sp2_alg1_seq_inplace, prg_sp2_submatrix, prg_sp2_submatrix_inplace
+
+  interface
+    subroutine c_sp2_3(h, r, thr, bndfil, mn, mx, conv, tol, v) bind(C, name="prg_sp2_alg2")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil, tol
+      integer(c_int), value :: mn, mx, v
+      character(kind=c_char) :: conv(*)
+    end subroutine
+    subroutine c_sp2_alg1(h, r, thr, bndfil, mn, mx, conv, tol, v) bind(C, name="prg_sp2_alg1")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil, tol
+      integer(c_int), value :: mn, mx, v
+      character(kind=c_char) :: conv(*)
+    end subroutine
+    subroutine c_sp2_basic(h, r, thr, bndfil, mn, mx, conv, tol, v) bind(C, name="prg_sp2_basic")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil, tol
+      integer(c_int), value :: mn, mx, v
+      character(kind=c_char) :: conv(*)
+    end subroutine
+    subroutine c_alg2_genseq(h, r, thr, bndfil, mn, mx, conv, tol, pp, ic, vv, v) &
+         bind(C, name="prg_sp2_alg2_genseq")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil, tol
+      integer(c_int), value :: mn, mx, v
+      character(kind=c_char) :: conv(*)
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+    end subroutine
+    subroutine c_alg1_genseq(h, r, thr, bndfil, mn, mx, conv, tol, pp, ic, vv) &
+         bind(C, name="prg_sp2_alg1_genseq")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil, tol
+      integer(c_int), value :: mn, mx
+      character(kind=c_char) :: conv(*)
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+    end subroutine
+    subroutine c_alg2_seq(h, r, thr, pp, ic, vv, v) bind(C, name="prg_sp2_alg2_seq")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+      integer(c_int), value :: v
+    end subroutine
+    subroutine c_alg1_seq(h, r, thr, pp, ic, vv) bind(C, name="prg_sp2_alg1_seq")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+    end subroutine
+    subroutine c_alg2_inpl(r, thr, n, pp, ic, vv, mn, mx) bind(C, name="prg_prg_sp2_alg2_seq_inplace")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: r
+      real(c_double), value :: thr, mn, mx
+      integer(c_int), value :: n
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+    end subroutine
+    subroutine c_alg1_inpl(r, thr, n, pp, ic, vv, mn, mx) bind(C, name="prg_prg_sp2_alg1_seq_inplace")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: r
+      real(c_double), value :: thr, mn, mx
+      integer(c_int), value :: n
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+    end subroutine
+    subroutine c_submatrix(h, r, thr, pp, ic, vv, mn, mx, core) bind(C, name="prg_sp2_submatrix")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, mn, mx
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+      integer(c_int), value :: core
+    end subroutine
+    subroutine c_submatrix_inpl(r, thr, pp, ic, vv, mn, mx, core) &
+         bind(C, name="prg_sp2_submatrix_inplace")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: r
+      real(c_double), value :: thr, mn, mx
+      integer(c_int) :: pp(*), ic
+      real(c_double) :: vv(*)
+      integer(c_int), value :: core
+    end subroutine
+  end interface
+
+contains
+
+  integer function vb(verbose)
+    integer, optional, intent(in) :: verbose
+    vb = 0
+    if (present(verbose)) vb = verbose
+  end function vb
+
+  !> src/prg_sp2_mod.F90:49-135
+  subroutine prg_sp2_basic(h_bml, rho_bml, threshold, bndfil, minsp2iter, maxsp2iter, sp2conv, &
+       idemtol, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, bndfil, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer, optional, intent(in) :: verbose
+    call c_sp2_basic(h_bml%ptr, rho_bml%ptr, threshold, bndfil, minsp2iter, maxsp2iter, &
+         cstr(sp2conv), idemtol, vb(verbose))
+  end subroutine prg_sp2_basic
+
+  !> src/prg_sp2_mod.F90:263-380
+  subroutine prg_sp2_alg2(h_bml, rho_bml, threshold, bndfil, minsp2iter, maxsp2iter, sp2conv, &
+       idemtol, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, bndfil, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer, optional, intent(in) :: verbose
+    call c_sp2_3(h_bml%ptr, rho_bml%ptr, threshold, bndfil, minsp2iter, maxsp2iter, &
+         cstr(sp2conv), idemtol, vb(verbose))
+  end subroutine prg_sp2_alg2
+
+  !> src/prg_sp2_mod.F90:734-864
+  subroutine prg_sp2_alg1(h_bml, rho_bml, threshold, bndfil, minsp2iter, maxsp2iter, sp2conv, &
+       idemtol, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, bndfil, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer, optional, intent(in) :: verbose
+    call c_sp2_alg1(h_bml%ptr, rho_bml%ptr, threshold, bndfil, minsp2iter, maxsp2iter, &
+         cstr(sp2conv), idemtol, vb(verbose))
+  end subroutine prg_sp2_alg1
+
+  !> src/prg_sp2_mod.F90:395-533
+  subroutine prg_sp2_alg2_genseq(h_bml, rho_bml, threshold, bndfil, minsp2iter, maxsp2iter, &
+       sp2conv, idemtol, pp, icount, vv, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, bndfil, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    integer, optional, intent(in) :: verbose
+    call c_alg2_genseq(h_bml%ptr, rho_bml%ptr, threshold, bndfil, minsp2iter, &
+         min(maxsp2iter, size(pp)), cstr(sp2conv), idemtol, pp, icount, vv, vb(verbose))
+  end subroutine prg_sp2_alg2_genseq
+
+  !> src/prg_sp2_mod.F90:879-1000
+  subroutine prg_sp2_alg1_genseq(h_bml, rho_bml, threshold, bndfil, minsp2iter, maxsp2iter, &
+       sp2conv, idemtol, pp, icount, vv)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, bndfil, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    call c_alg1_genseq(h_bml%ptr, rho_bml%ptr, threshold, bndfil, minsp2iter, &
+         min(maxsp2iter, size(pp)), cstr(sp2conv), idemtol, pp, icount, vv)
+  end subroutine prg_sp2_alg1_genseq
+
+  !> src/prg_sp2_mod.F90:543-627
+  subroutine prg_sp2_alg2_seq(h_bml, rho_bml, threshold, pp, icount, vv, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    integer, optional, intent(in) :: verbose
+    call c_alg2_seq(h_bml%ptr, rho_bml%ptr, threshold, pp, icount, vv, vb(verbose))
+  end subroutine prg_sp2_alg2_seq
+
+  !> src/prg_sp2_mod.F90:1010-1089
+  subroutine prg_sp2_alg1_seq(h_bml, rho_bml, threshold, pp, icount, vv)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    call c_alg1_seq(h_bml%ptr, rho_bml%ptr, threshold, pp, icount, vv)
+  end subroutine prg_sp2_alg1_seq
+
+  !> src/prg_sp2_mod.F90:638-722
+  subroutine prg_prg_sp2_alg2_seq_inplace(rho_bml, threshold, pp, icount, vv, mineval, maxeval, &
+       verbose)
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    real(dp), intent(in) :: mineval, maxeval
+    integer, optional, intent(in) :: verbose
+    call c_alg2_inpl(rho_bml%ptr, threshold, size(pp), pp, icount, vv, mineval, maxeval)
+  end subroutine prg_prg_sp2_alg2_seq_inplace
+
+  !> src/prg_sp2_mod.F90:1100-1177
+  subroutine prg_prg_sp2_alg1_seq_inplace(rho_bml, threshold, pp, icount, vv, mineval, maxeval)
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    real(dp), intent(in) :: mineval, maxeval
+    call c_alg1_inpl(rho_bml%ptr, threshold, size(pp), pp, icount, vv, mineval, maxeval)
+  end subroutine prg_prg_sp2_alg1_seq_inplace
+
+  !> src/prg_sp2_mod.F90:1190-1246
+  subroutine prg_sp2_submatrix(ham_bml, rho_bml, threshold, pp, icount, vv, mineval, maxeval, &
+       core_size)
+    type(bml_matrix_t), intent(in) :: ham_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, mineval, maxeval
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    integer, intent(in) :: core_size
+    call c_submatrix(ham_bml%ptr, rho_bml%ptr, threshold, pp, icount, vv, mineval, maxeval, &
+         core_size)
+  end subroutine prg_sp2_submatrix
+
+  !> src/prg_sp2_mod.F90:1259-1310
+  subroutine prg_sp2_submatrix_inplace(rho_bml, threshold, pp, icount, vv, mineval, maxeval, &
+       core_size)
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, mineval, maxeval
+    integer, intent(inout) :: pp(:), icount
+    real(dp), intent(inout) :: vv(:)
+    integer, intent(in) :: core_size
+    call c_submatrix_inpl(rho_bml%ptr, threshold, pp, icount, vv, mineval, maxeval, core_size)
+  end subroutine prg_sp2_submatrix_inplace
+
+end module prg_sp2_mod
+
+module prg_normalize_mod
+  use iso_c_binding
+  use bml
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_normalize, prg_normalize_fermi, prg_normalize_implicit_fermi
+  interface
+    subroutine c_normalize(h) bind(C, name="prg_normalize")
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end subroutine
+    subroutine c_normalize_fermi(h, h1, hn, mu) bind(C, name="prg_normalize_fermi")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: h1, hn, mu
+    end subroutine
+    subroutine c_normalize_if(h, c, mu) bind(C, name="prg_normalize_implicit_fermi")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: c, mu
+    end subroutine
+  end interface
+contains
+  !> src/prg_normalize_mod.F90:32-46
+  subroutine prg_normalize(h_bml)
+    type(bml_matrix_t), intent(inout) :: h_bml
+    call c_normalize(h_bml%ptr)
+  end subroutine prg_normalize
+  !> src/prg_normalize_mod.F90:59-72
+  subroutine prg_normalize_fermi(h_bml, h1, hN, mu)
+    type(bml_matrix_t), intent(inout) :: h_bml
+    real(dp), intent(in) :: h1, hN, mu
+    call c_normalize_fermi(h_bml%ptr, h1, hN, mu)
+  end subroutine prg_normalize_fermi
+  !> src/prg_normalize_mod.F90:83-94
+  subroutine prg_normalize_implicit_fermi(h_bml, cnst, mu)
+    type(bml_matrix_t), intent(inout) :: h_bml
+    real(dp), intent(in) :: cnst, mu
+    call c_normalize_if(h_bml%ptr, cnst, mu)
+  end subroutine prg_normalize_implicit_fermi
+end module prg_normalize_mod
+
+module prg_homolumo_mod
+  use iso_c_binding
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_homolumogap, prg_sp2sequence
+  interface
+    subroutine c_homolumogap(vv, imax, pp, mn, mx, eh, el, eg, v) bind(C, name="prg_homolumogap")
+      import :: c_double, c_int
+      real(c_double) :: vv(*), eh, el, eg
+      integer(c_int), value :: imax, v
+      integer(c_int) :: pp(*)
+      real(c_double), value :: mn, mx
+    end subroutine
+    subroutine c_sp2sequence(pp, imax, mn, mx, eh, el, err, v) bind(C, name="prg_sp2sequence")
+      import :: c_double, c_int
+      integer(c_int) :: pp(*), imax
+      real(c_double), value :: mn, mx, eh, el, err
+      integer(c_int), value :: v
+    end subroutine
+  end interface
+contains
+  !> src/prg_homolumo_mod.F90:22-89
+  subroutine prg_homolumogap(vv, imax, pp, mineval, maxeval, ehomo, elumo, egap, verbose)
+    integer, intent(in) :: imax
+    integer, intent(in) :: pp(:)
+    real(dp), intent(in) :: vv(:)
+    real(dp), intent(in) :: mineval, maxeval
+    real(dp), intent(inout) :: ehomo, elumo, egap
+    integer, optional, intent(in) :: verbose
+    integer :: v
+    integer, allocatable :: ppc(:)
+    real(dp), allocatable :: vvc(:)
+    v = 0
+    if (present(verbose)) v = verbose
+    ppc = pp
+    vvc = vv
+    call c_homolumogap(vvc, imax, ppc, mineval, maxeval, ehomo, elumo, egap, v)
+  end subroutine prg_homolumogap
+  !> src/prg_homolumo_mod.F90:97-147
+  subroutine prg_sp2sequence(pp, imax, mineval, maxeval, ehomo, elumo, errlimit, verbose)
+    integer, intent(inout) :: imax
+    integer, intent(inout) :: pp(:)
+    real(dp), intent(in) :: mineval, maxeval, ehomo, elumo, errlimit
+    integer, optional, intent(in) :: verbose
+    integer :: v
+    integer :: tmp(100)
+    v = 0
+    if (present(verbose)) v = verbose
+    call c_sp2sequence(tmp, imax, mineval, maxeval, ehomo, elumo, errlimit, v)
+    pp(1:min(imax, size(pp))) = tmp(1:min(imax, size(pp)))
+  end subroutine prg_sp2sequence
+end module prg_homolumo_mod
+
+module prg_densitymatrix_mod
+  use iso_c_binding
+  use bml
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_build_density_T0, prg_check_idempotency, prg_build_density_T_Fermi
+  interface
+    subroutine c_t0(n, h, r, thr, bndfil, ev) bind(C, name="prg_build_density_T0")
+      import :: c_ptr, c_double, c_int
+      integer(c_int), value :: n
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil
+      real(c_double) :: ev(*)
+    end subroutine
+    subroutine c_idem(a, thr, v) bind(C, name="prg_check_idempotency")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a
+      real(c_double), value :: thr
+      real(c_double) :: v
+    end subroutine
+    subroutine c_tfermi(h, r, thr, kbt, ef, v, drho) bind(C, name="prg_build_density_T_fermi")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, r
+      real(c_double), value :: thr, kbt, ef
+      integer(c_int), value :: v
+      real(c_double) :: drho
+    end subroutine
+  end interface
+contains
+  !> src/prg_densitymatrix_mod.F90:36-101
+  subroutine prg_build_density_T0(ham_bml, rho_bml, threshold, bndfil, eigenvalues_out)
+    type(bml_matrix_t), intent(in) :: ham_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, bndfil
+    real(dp), allocatable, optional, intent(out) :: eigenvalues_out(:)
+    real(dp), allocatable :: ev(:)
+    integer :: n
+    n = bml_get_n(ham_bml)
+    allocate(ev(n))
+    call c_t0(n, ham_bml%ptr, rho_bml%ptr, threshold, bndfil, ev)
+    if (present(eigenvalues_out)) eigenvalues_out = ev
+  end subroutine prg_build_density_T0
+  !> src/prg_densitymatrix_mod.F90:862-880
+  subroutine prg_check_idempotency(mat_bml, threshold, idempotency)
+    type(bml_matrix_t), intent(in) :: mat_bml
+    real(dp), intent(in) :: threshold
+    real(dp), intent(out) :: idempotency
+    call c_idem(mat_bml%ptr, threshold, idempotency)
+  end subroutine prg_check_idempotency
+  !> src/prg_densitymatrix_mod.F90:509-582
+  subroutine prg_build_density_T_Fermi(ham_bml, rho_bml, threshold, kbt, ef, verbose, drho)
+    type(bml_matrix_t), intent(in) :: ham_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp), intent(in) :: threshold, kbt, ef
+    integer, optional, intent(in) :: verbose
+    real(dp), optional, intent(inout) :: drho
+    real(dp) :: d
+    integer :: v
+    v = 0
+    if (present(verbose)) v = verbose
+    call c_tfermi(ham_bml%ptr, rho_bml%ptr, threshold, kbt, ef, v, d)
+    if (present(drho)) drho = d
+  end subroutine prg_build_density_T_Fermi
+end module prg_densitymatrix_mod
+
+module prg_implicit_fermi_mod
+  use iso_c_binding
+  use bml
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_implicit_fermi, prg_implicit_fermi_save_inverse, prg_test_density_matrix
+  public :: prg_implicit_fermi_first_order_response
+  interface
+    subroutine c_if(h, p, ns, k, nocc, mu, beta, method, osteps, oel, thr, tol) &
+         bind(C, name="prg_implicit_fermi")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, p
+      integer(c_int), value :: ns, k, method, osteps
+      real(c_double), value :: nocc, mu, beta, oel, thr, tol
+    end subroutine
+    function c_if_mu() bind(C, name="prg_implicit_fermi_last_mu") result(r)
+      import :: c_double
+      real(c_double) :: r
+    end function
+    subroutine c_ifs(inv, h, p, ns, nocc, mu, beta, oel, thr, tol, scf, oi, tn) &
+         bind(C, name="prg_implicit_fermi_save_inverse")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr) :: inv(*)
+      type(c_ptr), value :: h, p
+      integer(c_int), value :: ns, scf, oi, tn
+      real(c_double), value :: nocc, beta, oel, thr, tol
+      real(c_double) :: mu
+    end subroutine
+    subroutine c_counters(oi, tn) bind(C, name="prg_implicit_fermi_counters")
+      import :: c_int
+      integer(c_int) :: oi, tn
+    end subroutine
+    subroutine c_tdm(h, p, beta, mu, nocc, osteps, oel, thr) bind(C, name="prg_test_density_matrix")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, p
+      real(c_double), value :: beta, mu, nocc, oel, thr
+      integer(c_int), value :: osteps
+    end subroutine
+    function c_tdm_mu() bind(C, name="prg_test_density_matrix_mu") result(r)
+      import :: c_double
+      real(c_double) :: r
+    end function
+    subroutine c_for(h0, h1, p0, p1, inv, ns, mu0, beta, nocc, thr) &
+         bind(C, name="prg_implicit_fermi_first_order_response")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h0, h1, p0, p1
+      type(c_ptr) :: inv(*)
+      integer(c_int), value :: ns
+      real(c_double), value :: mu0, beta, nocc, thr
+    end subroutine
+  end interface
+contains
+  !> src/prg_implicit_fermi_mod.F90:209-336 (mu updated, as in Fortran)
+  subroutine prg_implicit_fermi(h_bml, p_bml, nsteps, k, nocc, mu, beta, method, osteps, &
+       occErrLimit, threshold, tol)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: p_bml
+    integer, intent(in) :: osteps, nsteps, method, k
+    real(dp), intent(in) :: nocc, threshold, tol, occErrLimit, beta
+    real(dp), intent(inout) :: mu
+    call c_if(h_bml%ptr, p_bml%ptr, nsteps, k, nocc, mu, beta, method, osteps, occErrLimit, &
+         threshold, tol)
+    mu = c_if_mu()
+  end subroutine prg_implicit_fermi
+  !> src/prg_implicit_fermi_mod.F90:46-193
+  subroutine prg_implicit_fermi_save_inverse(Inv_bml, h_bml, p_bml, nsteps, nocc, mu, beta, &
+       occErrLimit, threshold, tol, SCF_IT, occiter, totns)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: p_bml, Inv_bml(nsteps)
+    integer, intent(in) :: nsteps, SCF_IT
+    real(dp), intent(in) :: nocc, threshold, tol, occErrLimit, beta
+    real(dp), intent(inout) :: mu
+    integer, intent(inout) :: occiter, totns
+    type(c_ptr) :: h(nsteps)
+    integer :: i
+    do i = 1, nsteps
+      h(i) = Inv_bml(i)%ptr
+    enddo
+    call c_ifs(h, h_bml%ptr, p_bml%ptr, nsteps, nocc, mu, beta, occErrLimit, threshold, tol, &
+         SCF_IT, occiter, totns)
+    call c_counters(occiter, totns)
+  end subroutine prg_implicit_fermi_save_inverse
+  !> src/prg_implicit_fermi_mod.F90:1112-1186 (mu updated, as in Fortran)
+  subroutine prg_test_density_matrix(ham_bml, p_bml, beta, mu, nocc, osteps, occErrLimit, &
+       threshold)
+    type(bml_matrix_t), intent(in) :: ham_bml
+    type(bml_matrix_t), intent(inout) :: p_bml
+    real(dp), intent(in) :: beta, nocc, occErrLimit, threshold
+    real(dp), intent(inout) :: mu
+    integer, intent(in) :: osteps
+    call c_tdm(ham_bml%ptr, p_bml%ptr, beta, mu, nocc, osteps, occErrLimit, threshold)
+    mu = c_tdm_mu()
+  end subroutine prg_test_density_matrix
+  !> src/prg_implicit_fermi_mod.F90:439-529
+  subroutine prg_implicit_fermi_first_order_response(H0_bml, H1_bml, P0_bml, P1_bml, Inv_bml, &
+       nsteps, mu0, beta, nocc, threshold)
+    type(bml_matrix_t), intent(in) :: H0_bml, H1_bml, Inv_bml(nsteps)
+    type(bml_matrix_t), intent(inout) :: P0_bml, P1_bml
+    real(dp), intent(in) :: mu0, threshold, beta, nocc
+    integer, intent(in) :: nsteps
+    type(c_ptr) :: h(nsteps)
+    integer :: i
+    do i = 1, nsteps
+      h(i) = Inv_bml(i)%ptr
+    enddo
+    call c_for(H0_bml%ptr, H1_bml%ptr, P0_bml%ptr, P1_bml%ptr, h, nsteps, mu0, beta, nocc, &
+         threshold)
+  end subroutine prg_implicit_fermi_first_order_response
+end module prg_implicit_fermi_mod
+
+module prg_response_mod
+  use iso_c_binding
+  use bml
+  use prg_gpu_cstr
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_compute_response_SP2
+  interface
+    subroutine c_rsp(h, h1, rsp, rho, lam, bndfil, mn, mx, conv, tol, thr, v) &
+         bind(C, name="prg_compute_response_SP2")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, h1, rsp, rho
+      real(c_double), value :: lam, bndfil, tol, thr
+      integer(c_int), value :: mn, mx, v
+      character(kind=c_char) :: conv(*)
+    end subroutine
+  end interface
+contains
+  !> src/prg_response_mod.F90:653-782
+  subroutine prg_compute_response_SP2(ham_bml, prt_bml, rsp_bml, rho_bml, lambda, bndfil, &
+       minsp2iter, maxsp2iter, sp2conv, idemtol, threshold, verbose)
+    type(bml_matrix_t), intent(in) :: ham_bml, prt_bml
+    type(bml_matrix_t), intent(inout) :: rsp_bml, rho_bml
+    real(dp) :: lambda
+    real(dp), intent(in) :: bndfil, threshold, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer :: verbose
+    call c_rsp(ham_bml%ptr, prt_bml%ptr, rsp_bml%ptr, rho_bml%ptr, lambda, bndfil, minsp2iter, &
+         maxsp2iter, cstr(sp2conv), idemtol, threshold, verbose)
+  end subroutine prg_compute_response_SP2
+end module prg_response_mod
