@@ -254,15 +254,21 @@ void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scat
 /* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).
  *   PRG_PREC_FP64      FP64 MFMA;
  *   PRG_PREC_F16SPLIT  FP16 hi+lo split products with FP32 accumulation;
- *   PRG_PREC_INT8X6    exact int8 MFMA products of 6 seven-bit slices with
- *                      per-row power-of-two scaling (about 42 significant
- *                      bits), FP64 bookkeeping.
+ *   PRG_PREC_INT8CRT   exact int8 MFMA products of X's residues modulo
+ *                      pairwise-coprime moduli <= 256, Chinese-remainder
+ *                      reconstruction in FP64 (11 moduli: X carried to
+ *                      2^-42 absolute), FP64 bookkeeping.
  * No counterpart in the reference (whose CPU path is FP64 only). */
 #define PRG_PREC_FP64 0
 #define PRG_PREC_F16SPLIT 1
-#define PRG_PREC_INT8X6 2
+#define PRG_PREC_INT8CRT 2
 void prg_set_precision(int mode);
 int prg_get_precision(void);
+/* Number of moduli (6..14, default 11) of PRG_PREC_INT8CRT; X is scaled by
+ * 2^a with a = floor((log2 M - 3) / 2): 8 -> 30, 9 -> 34, 10 -> 38, 11 -> 42,
+ * 12 -> 45, 13 -> 49, 14 -> 52 bits.  Returns a status. */
+int prg_set_int8_moduli(int count);
+int prg_get_int8_moduli(void);
 
 /* ------------------------------------------------------------------ device-pointer API */
 /* prg_sp2_alg2 on device-resident data (no host copies): dH and dRho are

@@ -1,0 +1,472 @@
+// FP64-accurate SP2 products from exact int8 MFMA arithmetic (gfx950).
+//
+// PRG_PREC_INT8CRT.  X is symmetric with its spectrum in [0, 1], so every row
+// has ||X_i||_2 <= 1.  With A = rint(2^a X) (an integer matrix) every entry of
+// the integer product C = A A^T obeys |C_ij| <= ||A_i|| ||A_j|| < M/2 when
+// 2^(2a) <= M/8, M being the product of the moduli below.  C is then fixed by
+// its residues modulo each p_k (Chinese remainder theorem), and each residue
+// is one exact int8 product:
+//   C mod p_k = (A mod p_k)(A mod p_k)^T mod p_k,
+// with the residues stored as symmetric int8 values in [-128, 127] and
+// v_mfma_i32_16x16x64_i8 accumulating in int32 (|sum| <= ld * 128^2 < 2^31 for
+// ld < 2^17).  The reconstruction needs no wide integers:
+//   C / M = sum_k ((C mod p_k) * q_k mod p_k) / p_k   (mod 1),
+//   q_k = (M / p_k)^-1 mod p_k,
+// summed in FP64 per output element as the moduli are processed, so that
+// X^2 ~= frac * M / 2^(2a).  The only rounding is that of A (2^-(a+1) per
+// entry) and of the FP64 fraction sum.
+//
+// The fused kernel computes the upper-triangle 128 x 128 tiles of X^2, one
+// modulus after another over the full K range, applies the SP2 branch update
+// in FP64 against the master copy, writes the next master and its residue
+// planes (both triangle positions), and emits the same trace / Frobenius
+// partial sums as the FP64 kernel, so the FP64 bookkeeping is shared.
+// Because X and its residue planes are symmetric, the A operand (rows of the
+// I panel) and the B operand (columns of the J panel) are both contiguous
+// 128-byte chunks of plane rows.
+#include "prg_sp2.h"
+#include "prg_tile.h"
+
+#include <cmath>
+#include <cstdint>
+
+namespace prg {
+namespace {
+
+using tile::sym_tile;
+using tile::xcd_remap;
+
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BK = 64, NT = 256;
+constexpr int NSTAGE = 4;                  // LDS ring of operand stages
+constexpr int STAGE_BYTES = 2 * BM * BK;   // A and B panels of one stage
+constexpr int kMaxModuli = 14;
+
+// Pairwise coprime moduli <= 256, largest first.
+constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
+                                     233, 229, 227, 223, 217, 211, 199};
+
+struct CrtConst {
+  double p[kMaxModuli];    // moduli
+  double inv[kMaxModuli];  // 1 / p
+  double q[kMaxModuli];    // (M / p)^-1 mod p
+  double scale;            // 2^a
+  double cscale;           // M / 2^(2a)
+};
+
+struct CrtArgs {
+  const double* buf[2];     // FP64 master ping-pong
+  const int8_t* planes[2];  // residue planes ping-pong, plane k at k * ld * ld
+  int ld;
+  Sp2State* st;
+  double* part;
+  const double* fold;       // device copy of (p, 1/p, q) per modulus
+  CrtConst c;
+};
+
+// LDS byte address of a __shared__ object.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// 16 bytes per lane from global memory straight into LDS at base + 16 * lane
+// (global_load_lds_dwordx4; base must be wave-uniform).  Written as inline
+// assembly so that the compiler does not add its conservative vmcnt(0) waits
+// before every LDS read: the waits are explicit (wait_vm_barrier).
+__device__ __forceinline__ void dma16(const int8_t* src, uint32_t lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(lds_base)
+               : "memory", "m0");
+}
+
+// Waits until at most N vector-memory operations of this wave are pending and
+// its LDS reads have completed, then joins the workgroup barrier.
+template <int N>
+__device__ __forceinline__ void wait_vm_barrier() {
+  asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" : : "n"(N) : "memory");
+}
+
+// Symmetric residue of the integer-valued double v modulo p, in [-128, 127].
+__device__ __forceinline__ int sym_residue(double v, double p, double inv) {
+  double m = fma(-p, floor(v * inv), v);  // exact; off by at most one p
+  m += (m < 0.0) ? p : 0.0;
+  m -= (m >= p) ? p : 0.0;
+  m -= (m > 127.0) ? p : 0.0;
+  return (int)m;
+}
+
+template <int NP>
+__global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
+  // Operand stages are copied global -> LDS directly (global_load_lds, 16
+  // bytes per lane), NSTAGE - 1 stages ahead of the MFMAs.  A stage holds
+  // the A panel (128 rows x 64 bytes) then the B panel; within a row the
+  // 16-byte chunk g sits in slot g ^ ((row >> 1) & 3), which keeps the
+  // ds_read_b128 of 8 consecutive rows on distinct banks.
+  // The ring is the kernel's only LDS object, so it starts at LDS address 0
+  // and its 64 KB stay within the 16-bit range of the DMA base in M0.
+  __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][STAGE_BYTES];
+  const volatile Sp2State* stv = g.st;
+  if (!stv->planned) return;
+  const int apply = stv->apply, sigma = stv->sigma, cur = stv->cur;
+  const int ld = g.ld;
+  const size_t plane = (size_t)ld * ld;
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
+  const int8_t* R = g.planes[cur];
+  int8_t* Rn = const_cast<int8_t*>(g.planes[cur ^ 1]);
+
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+
+  i4 acc[4][4];
+  d4 S[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      S[m][n] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  // Stage s covers modulus s / ksteps, k range (s % ksteps) * BK + [0, BK).
+  // Wave w fills LDS rows 32w .. 32w + 31 of both panels: two 1 KB chunks per
+  // panel, lane l taking row 16q + l/4 (of the chunk) and slot l % 4.
+  const int ksteps = ld / BK;
+  const int total = NP * ksteps;
+  const int8_t* src_a[2];
+  const int8_t* src_b[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int row = 32 * w + 16 * q + (lane >> 2);
+    const int slot = lane & 3;
+    const int kc = 16 * (slot ^ ((row >> 1) & 3));
+    src_a[q] = R + (size_t)(I * BM + row) * ld + kc;
+    src_b[q] = R + (size_t)(J * BM + row) * ld + kc;
+  }
+  const uint32_t ring0 = lds_addr(&ring[0][0]) + 2048 * w;
+  auto issue = [&](int s) {
+    const int mod = s / ksteps;
+    const size_t off = (size_t)mod * plane + (size_t)(s - mod * ksteps) * BK;
+    const uint32_t dst = __builtin_amdgcn_readfirstlane(ring0 + STAGE_BYTES * (s % NSTAGE));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      dma16(src_a[q] + off, dst + 1024 * q);
+      dma16(src_b[q] + off, dst + BM * BK + 1024 * q);
+    }
+  };
+  // LDS read offsets of this lane's fragments (row r, chunk g = lane / 16)
+  int offa[4], offb[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int ra = wm * 64 + m * 16 + (lane & 15);
+    const int rb = wn * 64 + m * 16 + (lane & 15);
+    offa[m] = ra * BK + 16 * ((lane >> 4) ^ ((ra >> 1) & 3));
+    offb[m] = BM * BK + rb * BK + 16 * ((lane >> 4) ^ ((rb >> 1) & 3));
+  }
+  // Software pipeline: the ring holds NSTAGE stages in flight; fragments of
+  // stage s + 1 are read from LDS while the MFMAs of stage s run.
+  i4 fa[8], fb[8];  // [0..3] A fragments, [4..7] B fragments
+  auto read_frags = [&](int s, i4 (&f)[8]) {
+    const int8_t* st = ring[s % NSTAGE];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) f[m] = *(const i4*)(st + offa[m]);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) f[4 + n] = *(const i4*)(st + offb[n]);
+  };
+  // Iteration s: wait until stage s + 1 has landed in every wave (and this
+  // wave's reads of stage s are done), refill the slot of stage s with stage
+  // s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.
+  // The first step of a modulus starts from zero accumulators (peeled, so
+  // that the loop carries only MFMA results and they stay in AGPRs).
+  auto step = [&](int s, i4 (&use)[8], i4 (&next)[8], bool first) {
+    if (s + NSTAGE - 1 < total)
+      wait_vm_barrier<4 * (NSTAGE - 2)>();
+    else
+      wait_vm_barrier<0>();
+    if (s + NSTAGE < total) issue(s + NSTAGE);
+    // after the last stage this reads a stale slot that is never used
+    read_frags(s + 1, next);
+    const i4 zero = {0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[4 + n], use[m],
+                                                          first ? zero : acc[m][n], 0, 0, 0);
+  };
+#pragma unroll
+  for (int s = 0; s < NSTAGE; ++s)
+    if (s < total) issue(s);
+  if (NSTAGE - 1 < total)
+    wait_vm_barrier<4 * (NSTAGE - 1)>();
+  else
+    wait_vm_barrier<0>();
+  read_frags(0, fa);
+  // ksteps = ld / 64 is even, so every modulus starts on fa
+  for (int mod = 0; mod < NP; ++mod) {
+    const int s0 = mod * ksteps;
+    step(s0, fa, fb, true);
+    step(s0 + 1, fb, fa, false);
+    for (int ks = 2; ks < ksteps; ks += 2) {
+      step(s0 + ks, fa, fb, false);
+      step(s0 + ks + 1, fb, fa, false);
+    }
+    // modulus done: S += ((acc * q) mod p) / p  (mod 1)
+    const double p = g.fold[3 * mod], inv = g.fold[3 * mod + 1], qm = g.fold[3 * mod + 2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^39
+          double md = fma(-p, floor(v * inv), v);
+          md += (md < 0.0) ? p : 0.0;
+          md -= (md >= p) ? p : 0.0;
+          double sm = fma(md, inv, S[m][n][r]);
+          sm -= (sm >= 1.0) ? 1.0 : 0.0;
+          S[m][n][r] = sm;
+        }
+      }
+  }
+
+  // Epilogue.  Operands were swapped, so this lane holds (X^2)[row][col]
+  // with row = lane%16 and col = 4*(lane/16) + r within each 16 x 16 block.
+  const bool mirror = (I != J);
+  const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
+  const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
+  const double wz = mirror ? 2.0 : 1.0;
+  double sums[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int row = I * BM + wm * 64 + m * 16 + (lane & 15);
+      const int col0 = J * BM + wn * 64 + n * 16 + 4 * (lane >> 4);
+      double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+      double zz[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = col0 + r;
+        const size_t idx = row + (size_t)col * ld;
+        const size_t tdx = col + (size_t)row * ld;
+        double f = S[m][n][r];
+        f -= (f >= 0.5) ? 1.0 : 0.0;
+        const double y = f * g.c.cscale;
+        const double x = X[idx];
+        const double z = fma(cb, x, ca * y);
+        const double dxy = x - y;
+        const bool diag = (row == col);
+        q0 += z * z;
+        q2 += dxy * dxy;
+        q1 += diag ? z : 0.0;
+        q3 += diag ? y : 0.0;
+        Xn[idx] = z;
+        Xn[tdx] = z;
+        zz[r] = rint(z * g.c.scale);
+      }
+      // residues: 4 consecutive bytes at the transposed position (one
+      // 32-bit store) and single bytes at the direct one (on diagonal tiles
+      // both land on entries of the same tile with equal values)
+      const size_t tdx0 = col0 + (size_t)row * ld;
+      int8_t* rd = Rn + row + (size_t)col0 * ld;
+#pragma unroll 1
+      for (int k = 0; k < NP; ++k) {
+        const double p = g.fold[3 * k], inv = g.fold[3 * k + 1];
+        int b4 = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int v = sym_residue(zz[r], p, inv);
+          b4 |= (v & 0xff) << (8 * r);
+          rd[(size_t)k * plane + (size_t)r * ld] = (int8_t)v;
+        }
+        *(int*)(Rn + (size_t)k * plane + tdx0) = b4;
+      }
+      sums[0] += wz * q0;
+      sums[1] += q1;
+      sums[2] += wz * q2;
+      sums[3] += q3;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // workgroup sums (fixed order) through the now idle ring
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double x = sums[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    sums[q] = x;
+  }
+  double* red = (double*)&ring[0][0];
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[4 * w + q] = sums[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    g.part[4 * (size_t)t + q] = ((red[q] + red[4 + q]) + red[8 + q]) + red[12 + q];
+  }
+}
+
+// Residue planes of an FP64 symmetric matrix (whole ld x ld buffer; the zero
+// padding maps to zero residues).
+template <int NP>
+__global__ void crt_split_kernel(const double* __restrict__ x, int8_t* __restrict__ planes,
+                                 size_t count, CrtConst c) {
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
+    const double v = rint(x[i] * c.scale);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) planes[(size_t)k * count + i] = (int8_t)sym_residue(v, c.p[k], c.inv[k]);
+  }
+}
+
+int g_moduli = 11;
+
+// Modular inverse of a mod p (p > 1, gcd(a, p) = 1).
+long long inv_mod(long long a, long long p) {
+  long long t = 0, nt = 1, r = p, nr = ((a % p) + p) % p;
+  while (nr) {
+    const long long qq = r / nr;
+    long long tmp = t - qq * nt;
+    t = nt;
+    nt = tmp;
+    tmp = r - qq * nr;
+    r = nr;
+    nr = tmp;
+  }
+  return t < 0 ? t + p : t;
+}
+
+CrtConst make_const(int np) {
+  CrtConst c{};
+  double log2m = 0.0;
+  for (int k = 0; k < np; ++k) log2m += std::log2((double)kModuli[k]);
+  for (int k = 0; k < np; ++k) {
+    const long long p = kModuli[k];
+    long long mk = 1;  // (M / p) mod p
+    for (int j = 0; j < np; ++j)
+      if (j != k) mk = (mk * (kModuli[j] % p)) % p;
+    c.p[k] = (double)p;
+    c.inv[k] = 1.0 / (double)p;
+    c.q[k] = (double)inv_mod(mk, p);
+  }
+  // 2^(2a) <= M / 8 keeps |C| < M / 2 for rows of 2-norm up to sqrt(2)
+  const int a = (int)std::floor((log2m - 3.0) / 2.0);
+  c.scale = std::ldexp(1.0, a);
+  c.cscale = std::exp2(log2m - 2.0 * a);
+  return c;
+}
+
+template <int NP>
+void launch_step(const CrtArgs& g, int nb) {
+  hipLaunchKernelGGL(sp2_crt_kernel<NP>, dim3(nb), dim3(NT), 0, stream(), g);
+}
+
+template <int NP>
+void launch_split(const double* x, int8_t* planes, size_t count, const CrtConst& c) {
+  hipLaunchKernelGGL(crt_split_kernel<NP>, dim3(2048), dim3(NT), 0, stream(), x, planes, count, c);
+}
+
+#define PRG_CRT_DISPATCH(NPV, CALL) \
+  switch (NPV) {                    \
+    case 6: CALL(6); break;         \
+    case 7: CALL(7); break;         \
+    case 8: CALL(8); break;         \
+    case 9: CALL(9); break;         \
+    case 10: CALL(10); break;       \
+    case 11: CALL(11); break;       \
+    case 12: CALL(12); break;       \
+    case 13: CALL(13); break;       \
+    case 14: CALL(14); break;       \
+    default: break;                 \
+  }
+
+}  // namespace
+
+int crt_moduli() { return g_moduli; }
+
+int set_crt_moduli(int np) {
+  if (np < 6 || np > kMaxModuli) {
+    set_error(ERR_ARG, "int8 CRT products: the number of moduli must be in [6, 14]");
+    return ERR_ARG;
+  }
+  g_moduli = np;
+  return OK;
+}
+
+// Device copy of the per-modulus constants the product kernel indexes at run
+// time (refreshed when the number of moduli changes).
+static double* fold_constants(int np, int* st) {
+  static int cached = 0;
+  double* d = (double*)workspace("crt_fold", sizeof(double) * 3 * kMaxModuli, st);
+  if (*st || cached == np) return d;
+  const CrtConst c = make_const(np);
+  double h[3 * kMaxModuli] = {};
+  for (int k = 0; k < np; ++k) {
+    h[3 * k] = c.p[k];
+    h[3 * k + 1] = c.inv[k];
+    h[3 * k + 2] = c.q[k];
+  }
+  if (hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error(ERR_HIP, "int8 CRT products: constant upload failed");
+    *st = ERR_HIP;
+    return nullptr;
+  }
+  cached = np;
+  return d;
+}
+
+int crt_prepare(const DMat& x0, int np) {
+  int st = OK;
+  const size_t plane = (size_t)x0.ld * x0.ld;
+  int8_t* planes = (int8_t*)workspace("crt_planes", 2 * (size_t)np * plane, &st);
+  if (st) return st;
+  if (!fold_constants(np, &st)) return st;
+  const CrtConst c = make_const(np);
+#define PRG_CALL(K) launch_split<K>(x0.p, planes, plane, c)
+  PRG_CRT_DISPATCH(np, PRG_CALL)
+#undef PRG_CALL
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
+             int np) {
+  const int ld = x0.ld;
+  if (ld >= (1 << 17)) {
+    set_error(ERR_ARG, "int8 CRT products: matrix too large for int32 accumulation");
+    return ERR_ARG;
+  }
+  const size_t plane = (size_t)ld * ld;
+  int st = OK;
+  int8_t* planes = (int8_t*)workspace("crt_planes", 2 * (size_t)np * plane, &st);
+  if (st) return st;
+  CrtArgs g;
+  g.buf[0] = x0.p;
+  g.buf[1] = x1.p;
+  g.planes[0] = planes;
+  g.planes[1] = planes + (size_t)np * plane;
+  g.ld = ld;
+  g.st = d_state;
+  g.part = d_part;
+  g.fold = fold_constants(np, &st);
+  if (st) return st;
+  g.c = make_const(np);
+  const int nt = ld / BM;
+  const int nb = nt * (nt + 1) / 2;
+  if (ntiles) *ntiles = nb;
+#define PRG_CALL(K) launch_step<K>(g, nb)
+  PRG_CRT_DISPATCH(np, PRG_CALL)
+#undef PRG_CALL
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+}  // namespace prg

@@ -197,9 +197,10 @@ int split_f16(const DMat& x, void* hi, void* lo) {
 
 int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
                    int* ntiles, int prec) {
+  if (prec == PRG_PREC_INT8CRT) return crt_step(x0, x1, d_state, d_part, ntiles, crt_moduli());
   if (prec != PRG_PREC_F16SPLIT) {
-    set_error(ERR_UNSUPPORTED, "int8 sliced products are not built yet");
-    return ERR_UNSUPPORTED;
+    set_error(ERR_ARG, "unknown precision mode");
+    return ERR_ARG;
   }
   const int ld = x0.ld;
   const size_t plane = sizeof(_Float16) * (size_t)ld * ld;
@@ -225,6 +226,7 @@ int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_
 }
 
 int sp2_mixed_prepare(const DMat& x0, int prec) {
+  if (prec == PRG_PREC_INT8CRT) return crt_prepare(x0, crt_moduli());
   if (prec != PRG_PREC_F16SPLIT) return OK;
   const int ld = x0.ld;
   const size_t plane = sizeof(_Float16) * (size_t)ld * ld;

@@ -112,7 +112,9 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   hst->icount = p.icount;
   hst->core = p.core;
   const bool lowp = (p.precision != PRG_PREC_FP64) && !response;
-  hst->noise_stop = lowp && (p.variant == V_ALG2);
+  // FP32-accumulated products stop on their noise floor and are polished in
+  // FP64; the int8 CRT products carry X to 2^-42 and keep the FP64 rules.
+  hst->noise_stop = lowp && (p.precision == PRG_PREC_F16SPLIT) && (p.variant == V_ALG2);
   hst->refine_max = hst->noise_stop ? p.refine_max : 0;
   if (p.pp_in)
     for (int i = 0; i < p.icount && i < kMaxIter; ++i) hst->pp[i] = p.pp_in[i];
@@ -154,7 +156,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
         }
         if (prec == PRG_PREC_FP64 || response) {
           st = sp2_step(X0, X1, dst, part, &nt_used);
-        } else if (prec == PRG_PREC_F16SPLIT || prec == PRG_PREC_INT8X6) {
+        } else if (prec == PRG_PREC_F16SPLIT || prec == PRG_PREC_INT8CRT) {
           st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
         } else {
           set_error(ERR_ARG, "unknown precision mode");
@@ -308,13 +310,18 @@ extern "C" {
 
 void prg_set_precision(int mode) {
   clear_error();
-  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_F16SPLIT && mode != PRG_PREC_INT8X6) {
+  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_F16SPLIT && mode != PRG_PREC_INT8CRT) {
     set_error(ERR_ARG, "prg_set_precision: unknown mode");
     return;
   }
   set_precision_mode(mode);
 }
 int prg_get_precision(void) { return precision_mode(); }
+int prg_set_int8_moduli(int count) {
+  clear_error();
+  return set_crt_moduli(count);
+}
+int prg_get_int8_moduli(void) { return crt_moduli(); }
 
 void prg_sp2_basic(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
                    int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
@@ -466,7 +473,7 @@ int prg_gpu_sp2_alg2_device(int N, const double* dH, double* dRho, double bndfil
     return ERR_ARG;
   }
   if (precision != PRG_PREC_FP64 && precision != PRG_PREC_F16SPLIT &&
-      precision != PRG_PREC_INT8X6) {
+      precision != PRG_PREC_INT8CRT) {
     set_error(ERR_ARG, "prg_gpu_sp2_alg2_device: unknown precision");
     return ERR_ARG;
   }

@@ -42,11 +42,18 @@ struct Sp2Result {
 int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res,
             const DMat* H1 = nullptr, DMat* out1 = nullptr);
 
-// Mixed-precision fused SP2 step (prec = PRG_PREC_F16SPLIT or PRG_PREC_INT8X6).
+// Mixed-precision fused SP2 step (prec = PRG_PREC_F16SPLIT or PRG_PREC_INT8CRT).
 int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
                    int* ntiles, int prec);
 // Builds the low-precision representation of X0 (after normalisation).
 int sp2_mixed_prepare(const DMat& x0, int prec);
+
+// int8 Chinese-remainder products (prg_gemm_crt.hip)
+int crt_prepare(const DMat& x0, int np);
+int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
+             int np);
+int crt_moduli();
+int set_crt_moduli(int np);
 
 int precision_mode();
 void set_precision_mode(int m);

@@ -20,7 +20,7 @@ LIB_PATH = _HERE.parent / "lib" / "libprogress_gpu.so"
 
 PREC_FP64 = 0
 PREC_F16SPLIT = 1
-PREC_INT8X6 = 2
+PREC_INT8CRT = 2
 
 _dense, _double_real, _sequential, _col_major = 1, 2, 0, 1
 
@@ -69,6 +69,8 @@ def lib():
         "prg_progress_shutdown": (None, []),
         "prg_set_precision": (None, [i]),
         "prg_get_precision": (i, []),
+        "prg_set_int8_moduli": (i, [i]),
+        "prg_get_int8_moduli": (i, []),
         "prg_gershgorin": (None, [vp, dp]),
         "prg_normalize": (None, [vp]),
         "prg_normalize_fermi": (None, [vp, d, d, d]),
@@ -196,6 +198,20 @@ class Matrix:
 def set_precision(mode: int):
     lib().prg_set_precision(mode)
     _check()
+
+
+def get_precision() -> int:
+    return int(lib().prg_get_precision())
+
+
+def set_int8_moduli(count: int):
+    """Number of moduli of the int8 Chinese-remainder products (6..14)."""
+    lib().prg_set_int8_moduli(count)
+    _check()
+
+
+def get_int8_moduli() -> int:
+    return int(lib().prg_get_int8_moduli())
 
 
 def _ivec(n, init=None):

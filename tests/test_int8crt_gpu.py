@@ -1,0 +1,105 @@
+"""GPU parity of the int8 Chinese-remainder SP2 products (PRG_PREC_INT8CRT).
+
+The products are exact integer products of X's residues, so the only error is
+the rounding of X to 2^-a (a = 42 with the default 11 moduli).  Bars (stated
+per test): the same branch sequence as the FP64 oracle, the density within
+1e-7 (Frobenius) of the oracle's, and the error shrinking as moduli are
+added.  Replays of a fixed branch sequence isolate the product accuracy from
+the branch decisions.
+"""
+import numpy as np
+import pytest
+
+import progress_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def P():
+    import progress_amd
+    progress_amd.set_precision(progress_amd.PREC_INT8CRT)
+    progress_amd.set_int8_moduli(11)
+    yield progress_amd
+    progress_amd.set_int8_moduli(11)
+    progress_amd.set_precision(progress_amd.PREC_FP64)
+
+
+@pytest.mark.parametrize("n,seed", [(256, 0), (300, 1), (1024, 0)])
+def test_crt_alg2_matches_oracle(P, n, seed):
+    print(f"[int8crt alg2] n={n} seed={seed}", flush=True)
+    h = O.model_hamiltonian(n, seed)
+    rho_o, pp_o, ic_o, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    rho_g, pp_g, ic_g, _ = P.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert ic_g == ic_o and list(pp_g) == list(pp_o)
+    # rounding X to 2^-42 per entry, amplified by the inverse gap: 1e-7 bar
+    assert np.linalg.norm(rho_g - rho_o) < 1e-7
+    assert np.array_equal(rho_g, rho_g.T)
+    assert abs(np.trace(rho_g) - n) < 1e-8
+
+
+def test_crt_replay_isolates_products(P):
+    """A fixed branch sequence: the int8 path and the FP64 oracle apply the
+    same polynomial, so the difference is the product error alone."""
+    n = 512
+    print(f"[int8crt replay] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 3)
+    _, pp, ic, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    rho_o = O.sp2_alg2_seq(h, pp, ic)
+    rho_o = rho_o[0] if isinstance(rho_o, tuple) else rho_o
+    rho_g, _ = P.sp2_alg2_seq(h, pp, ic)
+    assert np.linalg.norm(rho_g - rho_o) < 1e-7
+
+
+def test_crt_error_falls_with_moduli(P):
+    """The rounding of X is 2^-(a+1) with a = 30, 38, 45, 53 for 8, 10, 12, 14
+    moduli: each step of two moduli must cut the error, and 14 moduli reach
+    the FP64 path's own agreement with the oracle on this input (1e-9, the
+    bar of the FP64 parity tests: summation order alone moves the density by
+    ~1e-10 here)."""
+    n = 384
+    print(f"[int8crt moduli] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 5)
+    d0, _ = O.build_density_T0(h, 0.5)
+    rho_o, _, pp_o = O.sp2_alg2(h, 0.5, 10, 200, "Abs", 1e-10)
+    errs = {}
+    for npm in (8, 10, 12, 14):
+        P.set_int8_moduli(npm)
+        assert P.get_int8_moduli() == npm
+        _, pp, ic, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+        rho_g, _ = P.sp2_alg2_seq(h, pp, ic)
+        errs[npm] = float(np.linalg.norm(rho_g - rho_o))
+    print("errors vs FP64 oracle:", errs, flush=True)
+    assert errs[14] < 1e-9
+    assert errs[12] < 1e-8
+    assert errs[8] > errs[10] > errs[12]
+    assert float(np.linalg.norm(rho_o - d0)) < 1e-9
+
+
+def test_crt_rejects_bad_moduli_count(P):
+    with pytest.raises(P.ProgressError):
+        P.set_int8_moduli(5)
+    with pytest.raises(P.ProgressError):
+        P.set_int8_moduli(15)
+
+
+@pytest.mark.parametrize("n", [2000])
+def test_crt_device_api_accuracy(P, n):
+    """Device-pointer entry point (the benchmark's path) at a size where the
+    oracle's eigendecomposition still runs in seconds; n = 2000 is padded to
+    ld = 2048 and the padding must stay zero."""
+    import torch
+    print(f"[int8crt device api] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 1)
+    ld = P.padded_ld(n)
+    hd = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    hd[:n, :n] = torch.from_numpy(h)
+    rd = torch.zeros_like(hd)
+    it, pp = P.sp2_alg2_device(hd, rd, n, 0.5, 1, 200, "Abs", 1e-10, precision=P.PREC_INT8CRT)
+    rho = rd[:n, :n].cpu().numpy()
+    d0, _ = O.build_density_T0(h, 0.5)
+    rho_o, it_o, pp_o = O.sp2_alg2(h, 0.5, 1, 200, "Abs", 1e-10)
+    assert it == it_o and list(pp[:it]) == list(pp_o)
+    assert np.linalg.norm(rho - rho_o) < 1e-7
+    assert np.linalg.norm(rho - d0) < 1e-6
+    assert float(rd[n:, :].abs().max()) == 0.0 and float(rd[:, n:].abs().max()) == 0.0
