@@ -247,6 +247,19 @@ void prg_fnormGraph(prg_graph_partitioning_t *gp);
  * all parts purified in lock-step by batched GPU kernels. */
 void prg_subgraphSP2Loop(bml_matrix_t *h_bml, bml_matrix_t *g_bml, bml_matrix_t *rho_bml,
                          prg_graph_partitioning_t *gp, double threshold);
+/* The build's graph partitioner in place of METIS (prg_metisPartition,
+ * src/prg_partition_mod.F90:216-311, has no C symbol): recursive bisection of
+ * breadth-first orderings of the graph of g_bml's nonzero off-diagonal
+ * entries into nparts parts (cores in ascending node order).  No reference
+ * counterpart as a C symbol. */
+void prg_bfsPartition(prg_graph_partitioning_t *gp, bml_matrix_t *g_bml, int nparts);
+/* src/prg_progress_mod.h:803-811 (src/prg_graphsolver_mod.F90:39-191): graph-
+ * partitioned finite-temperature density (kT = 0.1, per-part batched
+ * eigensolves) with the reference's Fermi-level update.  Ef by value as in the
+ * reference header; prg_build_densityGP_T0_ef() returns the updated value. */
+void prg_build_densityGP_T0(bml_matrix_t *ham_bml, bml_matrix_t *g_bml, bml_matrix_t *rho_bml,
+                            double threshold, double bndfil, double Ef, int nparts, int verbose);
+double prg_build_densityGP_T0_ef(void);
 /* Device time (ms) of the last prg_subgraphSP2Loop phases. */
 void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scatter);
 

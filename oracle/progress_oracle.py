@@ -907,6 +907,78 @@ def subgraph_sp2_loop(h, g, parts, pp, max_iter, mineval, maxeval, double_jump=T
     return rho, vv
 
 
+def bfs_partition(g, nparts):
+    """The build's replacement for METIS in prg_build_densityGP_T0
+    (prg_metisPartition, src/prg_partition_mod.F90:216-311): recursive
+    bisection of breadth-first orderings.  A bisection orders the subset by
+    BFS (ascending neighbours) from the last node that a BFS from the
+    component's lowest node reaches, components in ascending order of their
+    lowest node, and cuts where the two sides get node counts proportional
+    to their part counts (left = floor(k/2) parts, rounded half up).
+    Returns the list of cores (ascending node ids)."""
+    n = g.shape[0]
+    adj = [[int(j) for j in np.nonzero((g[i] != 0) | (g[:, i] != 0))[0] if j != i]
+           for i in range(n)]
+    part = [-1] * n
+
+    def bfs(start, inset, mark):
+        out = [start]
+        mark[start] = True
+        head = 0
+        while head < len(out):
+            v = out[head]
+            head += 1
+            for nb in adj[v]:
+                if nb in inset and not mark.get(nb, False):
+                    mark[nb] = True
+                    out.append(nb)
+        return out
+
+    def bisect(nodes, k, first):
+        if k == 1:
+            for v in nodes:
+                part[v] = first
+            return
+        inset = set(nodes)
+        mark = {}
+        order = []
+        for v in nodes:
+            if mark.get(v, False):
+                continue
+            probe = bfs(v, inset, {})
+            order.extend(bfs(probe[-1], inset, mark))
+        kl = k // 2
+        nl = int(math.floor(len(nodes) * kl / k + 0.5))
+        bisect(sorted(order[:nl]), kl, first)
+        bisect(sorted(order[nl:]), k - kl, first + kl)
+
+    bisect(list(range(n)), nparts, 0)
+    return [[v for v in range(n) if part[v] == p] for p in range(nparts)]
+
+
+def build_density_gp_t0(h, g, bndfil, ef, parts, kbt=0.1):
+    """prg_build_densityGP_T0 (src/prg_graphsolver_mod.F90:39-191) on one
+    rank for a given partition: core+halo with the double jump (:122-126),
+    per part prg_build_density_T_Fermi at kT = 0.1 (:147), core rows of each
+    part written into rho (:155), then the Fermi-level step (:162-177) from
+    the trace error and the summed dN/dEf of the parts.  Returns (rho, ef)."""
+    n = h.shape[0]
+    rho = np.zeros((n, n))
+    dnocc = 0.0
+    for core in parts:
+        idx, llsize = matrix2submatrix_index(g, core, True)
+        d, dn = build_density_T_fermi(h[np.ix_(idx, idx)], kbt, ef)
+        dnocc += dn
+        rho[np.ix_(idx[:llsize], idx)] = d[:llsize, :]
+    nocc = float(np.trace(rho))
+    error = abs(nocc - 2.0 * bndfil * n)
+    if nocc < 2.0 * bndfil * n:
+        dnocc = -dnocc
+    if error > 1e-6 and abs(dnocc) > 1e-6:
+        ef = ef - (nocc / dnocc if error > 1.0 else error / dnocc)
+    return rho, ef
+
+
 # --------------------------------------------------------------------------
 # closed forms on the eigenvalues (the pins of the matrix oracle)
 # --------------------------------------------------------------------------

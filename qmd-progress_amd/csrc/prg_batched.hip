@@ -139,18 +139,80 @@ __global__ void gather_normalize_kernel(const double* __restrict__ h, int ldh,
   }
 }
 
-// rho[idx[r], idx[c]] = 2 X_p[r][c] for the core rows r < core.
+// rho[idx[r], idx[c]] = scale X_p[r][c] for the core rows r < core.
 __global__ void scatter_core_kernel(const double* __restrict__ x,
                                     const PartDesc* __restrict__ parts,
                                     const int* __restrict__ idx_all, double* __restrict__ rho,
-                                    int ldr) {
+                                    int ldr, double scale) {
   const PartDesc pd = parts[blockIdx.y];
   const int* idx = idx_all + pd.idx_offset;
   const double* X = x + pd.offset;
   for (int c = blockIdx.x; c < pd.n; c += gridDim.x) {
     const int gc = idx[c];
     for (int r = threadIdx.x; r < pd.core; r += NT)
-      rho[idx[r] + (size_t)gc * ldr] = 2.0 * X[r + (size_t)c * pd.m];
+      rho[idx[r] + (size_t)gc * ldr] = scale * X[r + (size_t)c * pd.m];
+  }
+}
+
+// X_p = H[idx, idx] with the padding rows/columns decoupled: pad on their
+// diagonal, zeros elsewhere.
+__global__ void gather_pad_kernel(const double* __restrict__ h, int ldh,
+                                  const PartDesc* __restrict__ parts,
+                                  const int* __restrict__ idx_all, double* __restrict__ x,
+                                  double pad) {
+  const PartDesc pd = parts[blockIdx.y];
+  const int* idx = idx_all + pd.idx_offset;
+  double* X = x + pd.offset;
+  for (int c = blockIdx.x; c < pd.m; c += gridDim.x) {
+    const int gc = c < pd.n ? idx[c] : 0;
+    for (int r = threadIdx.x; r < pd.m; r += NT) {
+      double v = 0.0;
+      if (r < pd.n && c < pd.n)
+        v = h[idx[r] + (size_t)gc * ldh];
+      else if (r == c)
+        v = pad;
+      X[r + (size_t)c * pd.m] = v;
+    }
+  }
+}
+
+// Occupations of one part from its eigenvalues w (m per part, ascending):
+// U[:, j] = V[:, j] * 2 f(w_j) and, per part, sum_j 2 df/def(w_j)
+// (src/prg_densitymatrix_mod.F90:540-578 with fermi / dfermi at :886-916).
+__device__ __forceinline__ double fermi_occ(double e, double ef, double kbt) {
+  const double t = (e - ef) / kbt;
+  return t > 100.0 ? 0.0 : 1.0 / (1.0 + exp(t));
+}
+__device__ __forceinline__ double fermi_docc(double e, double ef, double kbt) {
+  const double t = (e - ef) / kbt;
+  if (t > 100.0) return fabs(e - ef) < 0.001 ? -1000.0 : 0.0;
+  const double f = 1.0 / (1.0 + exp(t));
+  return f * f * exp(t) / kbt;
+}
+
+__global__ void fermi_scale_kernel(const double* __restrict__ v, const double* __restrict__ w,
+                                   const PartDesc* __restrict__ parts,
+                                   const size_t* __restrict__ w_offset, double* __restrict__ u,
+                                   double ef, double kbt, double* __restrict__ dnocc) {
+  const PartDesc pd = parts[blockIdx.y];
+  const double* W = w + w_offset[blockIdx.y];
+  const double* V = v + pd.offset;
+  double* U = u + pd.offset;
+  for (int c = blockIdx.x; c < pd.m; c += gridDim.x) {
+    const double occ = 2.0 * fermi_occ(W[c], ef, kbt);
+    for (int r = threadIdx.x; r < pd.m; r += NT) U[r + (size_t)c * pd.m] = occ * V[r + (size_t)c * pd.m];
+  }
+  if (blockIdx.x == 0) {
+    __shared__ double red[NT];
+    double s = 0.0;
+    for (int c = threadIdx.x; c < pd.m; c += NT) s += 2.0 * fermi_docc(W[c], ef, kbt);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dnocc[blockIdx.y] = red[0];
   }
 }
 
@@ -200,11 +262,27 @@ int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, 
   return OK;
 }
 
-int batched_scatter(const BatchPlan& plan, const double* x, DMat rho) {
+int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale) {
   hipLaunchKernelGGL(zero_kernel, dim3(2048), dim3(NT), 0, stream(), rho.p,
                      (size_t)rho.ld * rho.ld);
   hipLaunchKernelGGL(scatter_core_kernel, dim3(64, plan.nparts), dim3(NT), 0, stream(), x,
-                     plan.d_parts, plan.d_idx, rho.p, rho.ld);
+                     plan.d_parts, plan.d_idx, rho.p, rho.ld, scale);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int batched_gather_pad(const DMat& H, const BatchPlan& plan, double* x, double pad) {
+  hipLaunchKernelGGL(gather_pad_kernel, dim3(64, plan.nparts), dim3(NT), 0, stream(), H.p, H.ld,
+                     plan.d_parts, plan.d_idx, x, pad);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int batched_fermi_scale(const BatchPlan& plan, const double* v, const double* w,
+                        const size_t* d_w_offset, double* u, double ef, double kbt,
+                        double* d_dnocc) {
+  hipLaunchKernelGGL(fermi_scale_kernel, dim3(64, plan.nparts), dim3(NT), 0, stream(), v, w,
+                     plan.d_parts, d_w_offset, u, ef, kbt, d_dnocc);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

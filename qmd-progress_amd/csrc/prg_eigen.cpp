@@ -70,6 +70,50 @@ int eig(const DMat& H, DMat Q, double* d_w, std::vector<double>* h_w) {
   return OK;
 }
 
+int eig_strided_batched(double* a, int m, int count, double* w) {
+  int st = ensure_device();
+  if (st) return st;
+  if (count <= 0) return OK;
+  rocblas_handle h;
+  if ((st = blas(&h))) return st;
+  double* d_e = (double*)workspace("eigb_e", sizeof(double) * (size_t)m * count, &st);
+  int* d_info = (int*)workspace("eigb_info", sizeof(int) * (size_t)count, &st);
+  if (st) return st;
+  if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, m, a, m,
+                                       (rocblas_stride)m * m, w, m, d_e, m, d_info,
+                                       count) != rocblas_status_success) {
+    set_error(ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
+    return ERR_HIP;
+  }
+  std::vector<int> info(count);
+  PRG_HIP_CHECK(hipMemcpyAsync(info.data(), d_info, sizeof(int) * count, hipMemcpyDeviceToHost,
+                               stream()));
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  for (int k = 0; k < count; ++k)
+    if (info[k] != 0) {
+      set_error(ERR_NUMERIC, "batched eigensolver did not converge");
+      return ERR_NUMERIC;
+    }
+  return OK;
+}
+
+int gemm_nt_strided_batched(const double* a, const double* b, double* c, int m, int count) {
+  int st = ensure_device();
+  if (st) return st;
+  if (count <= 0) return OK;
+  rocblas_handle h;
+  if ((st = blas(&h))) return st;
+  const double one = 1.0, zero = 0.0;
+  const rocblas_stride s = (rocblas_stride)m * m;
+  if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m, m,
+                                    m, &one, a, m, s, b, m, s, &zero, c, m, s,
+                                    count) != rocblas_status_success) {
+    set_error(ERR_HIP, "rocblas_dgemm_strided_batched failed");
+    return ERR_HIP;
+  }
+  return OK;
+}
+
 int density_from_eig(const DMat& Q, const std::vector<double>& occ, DMat out) {
   int st = OK;
   const int n = Q.n;

@@ -16,4 +16,11 @@ int density_from_eig(const DMat& Q, const std::vector<double>& occ, DMat out);
 int density_fermi(const DMat& H, double beta, double mu, DMat out, std::vector<double>* w_out);
 std::vector<double> t0_occupations(int n, double bndfil);
 
+// Batched symmetric eigensolves of `count` m x m column-major matrices stored
+// m*m apart from a (overwritten by the eigenvectors); w receives m ascending
+// eigenvalues per matrix.
+int eig_strided_batched(double* a, int m, int count, double* w);
+// c_k = a_k b_k^T for `count` m x m matrices stored m*m apart.
+int gemm_nt_strided_batched(const double* a, const double* b, double* c, int m, int count);
+
 }  // namespace prg

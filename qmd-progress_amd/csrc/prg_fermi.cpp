@@ -316,7 +316,6 @@ int implicit_fermi_response(const DMat& H0, const std::vector<DMat>& Hs, DMat P0
   for (int j = 0; j < order; ++j) mu[j] = 0.0;
   double occ_err = 10000.0;
   int k = 0;
-  Sums s;
   while (occ_err > occ_tol) {
     ++k;
     if ((st = normalize_if(H0, cnst, *mu0, P0))) return st;

@@ -19,6 +19,8 @@
 #include <vector>
 
 #include "prg_bml.h"
+#include "prg_eigen.h"
+#include "prg_ops.h"
 
 namespace prg {
 namespace {
@@ -116,6 +118,57 @@ int halo_indices(bml_matrix_t* g_bml, prg_graph_partitioning_t* gp, int djflag) 
   return OK;
 }
 
+// Packed batch of all parts: largest padded size first, each part m x m with
+// m = ceil64(core + halo), and the upper-triangle 64 x 64 tiles of every part.
+struct HostPlan {
+  std::vector<int> order;  // packed position -> part id
+  std::vector<PartDesc> parts;
+  std::vector<int> idx;
+  std::vector<int4> tiles;
+};
+
+int make_plan(const prg_graph_partitioning_t* gp, HostPlan* hp, BatchPlan* plan) {
+  int st = OK;
+  const int np = gp->totalParts;
+  hp->order.resize(np);
+  for (int p = 0; p < np; ++p) hp->order[p] = p;
+  std::stable_sort(hp->order.begin(), hp->order.end(),
+                   [&](int a, int b) { return gp->lsize[a] > gp->lsize[b]; });
+  hp->parts.assign(np, PartDesc{});
+  hp->idx.clear();
+  hp->tiles.clear();
+  size_t off = 0;
+  for (int q = 0; q < np; ++q) {
+    const int p = hp->order[q];
+    PartDesc d;
+    d.n = gp->lsize[p];
+    d.core = gp->llsize[p];
+    d.m = std::max(64, ((d.n + 63) / 64) * 64);
+    d.offset = off;
+    d.idx_offset = (int)hp->idx.size();
+    off += (size_t)d.m * d.m;
+    hp->idx.insert(hp->idx.end(), gp->core_halo_index[p], gp->core_halo_index[p] + d.n);
+    hp->parts[q] = d;
+    const int nt = d.m / 64;
+    for (int J = 0; J < nt; ++J)
+      for (int I = 0; I <= J; ++I) hp->tiles.push_back(int4{q, I, J, 0});
+  }
+  plan->nparts = np;
+  plan->ntiles = (int)hp->tiles.size();
+  plan->total = off;
+  plan->d_parts = (PartDesc*)workspace("gp_parts", sizeof(PartDesc) * np, &st);
+  plan->d_tiles = (int4*)workspace("gp_tiles", sizeof(int4) * hp->tiles.size(), &st);
+  plan->d_idx = (int*)workspace("gp_idx", sizeof(int) * std::max<size_t>(1, hp->idx.size()), &st);
+  if (st) return st;
+  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_parts, hp->parts.data(), sizeof(PartDesc) * np,
+                               hipMemcpyHostToDevice, stream()));
+  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_tiles, hp->tiles.data(), sizeof(int4) * hp->tiles.size(),
+                               hipMemcpyHostToDevice, stream()));
+  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_idx, hp->idx.data(), sizeof(int) * hp->idx.size(),
+                               hipMemcpyHostToDevice, stream()));
+  return OK;
+}
+
 int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   int st = ensure_device();
   if (st) return st;
@@ -128,48 +181,15 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
     set_error(ERR_NUMERIC, "prg_subgraphSP2Loop: mineval == maxeval");
     return ERR_NUMERIC;
   }
-  // plan: parts by padded size (largest first), tiles per part
-  std::vector<int> order(np);
-  for (int p = 0; p < np; ++p) order[p] = p;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return gp->lsize[a] > gp->lsize[b]; });
-  std::vector<PartDesc> parts(np);
-  std::vector<int> idx;
-  std::vector<int4> tiles;
-  size_t off = 0;
-  for (int q = 0; q < np; ++q) {
-    const int p = order[q];
-    PartDesc d;
-    d.n = gp->lsize[p];
-    d.core = gp->llsize[p];
-    d.m = std::max(64, ((d.n + 63) / 64) * 64);
-    d.offset = off;
-    d.idx_offset = (int)idx.size();
-    off += (size_t)d.m * d.m;
-    idx.insert(idx.end(), gp->core_halo_index[p], gp->core_halo_index[p] + d.n);
-    parts[q] = d;
-    const int nt = d.m / 64;
-    for (int J = 0; J < nt; ++J)
-      for (int I = 0; I <= J; ++I) tiles.push_back(int4{q, I, J, 0});
-  }
+  HostPlan hp;
   BatchPlan plan;
-  plan.nparts = np;
-  plan.ntiles = (int)tiles.size();
-  plan.total = off;
-  plan.d_parts = (PartDesc*)workspace("gp_parts", sizeof(PartDesc) * np, &st);
-  plan.d_tiles = (int4*)workspace("gp_tiles", sizeof(int4) * tiles.size(), &st);
-  plan.d_idx = (int*)workspace("gp_idx", sizeof(int) * std::max<size_t>(1, idx.size()), &st);
+  if ((st = make_plan(gp, &hp, &plan))) return st;
+  const size_t off = plan.total;
   double* x0 = (double*)workspace("gp_x0", sizeof(double) * off, &st);
   double* x1 = (double*)workspace("gp_x1", sizeof(double) * off, &st);
-  double* d_part = (double*)workspace("gp_part", sizeof(double) * tiles.size(), &st);
+  double* d_part = (double*)workspace("gp_part", sizeof(double) * hp.tiles.size(), &st);
   double* d_vv = (double*)workspace("gp_vv", sizeof(double) * 100, &st);
   if (st) return st;
-  PRG_HIP_CHECK(hipMemcpyAsync(plan.d_parts, parts.data(), sizeof(PartDesc) * np,
-                               hipMemcpyHostToDevice, stream()));
-  PRG_HIP_CHECK(hipMemcpyAsync(plan.d_tiles, tiles.data(), sizeof(int4) * tiles.size(),
-                               hipMemcpyHostToDevice, stream()));
-  PRG_HIP_CHECK(hipMemcpyAsync(plan.d_idx, idx.data(), sizeof(int) * idx.size(),
-                               hipMemcpyHostToDevice, stream()));
   PRG_HIP_CHECK(hipMemcpyAsync(d_vv, gp->vv, sizeof(double) * 100, hipMemcpyHostToDevice,
                                stream()));
   hipEvent_t ev[4];
@@ -184,13 +204,210 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
     std::swap(cur, nxt);
   }
   PRG_HIP_CHECK(hipEventRecord(ev[2], stream()));
-  if ((st = batched_scatter(plan, cur, rho))) return st;
+  if ((st = batched_scatter(plan, cur, rho, 2.0))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[3], stream()));
   PRG_HIP_CHECK(hipMemcpyAsync(gp->vv, d_vv, sizeof(double) * 100, hipMemcpyDeviceToHost,
                                stream()));
   PRG_HIP_CHECK(hipStreamSynchronize(stream()));
   for (int k = 0; k < 3; ++k) PRG_HIP_CHECK(hipEventElapsedTime(&g_ms[k], ev[k], ev[k + 1]));
   for (auto& e : ev) PRG_HIP_CHECK(hipEventDestroy(e));
+  return OK;
+}
+
+// ----------------------------------------------------------------------------
+// Graph partitioner standing in for METIS (prg_metisPartition,
+// src/prg_partition_mod.F90:216-311, which the reference links for
+// prg_build_densityGP_T0): recursive bisection of breadth-first orderings.
+// Each bisection orders the nodes of the current subset by BFS from a
+// pseudo-peripheral node (the last node reached by a BFS from the lowest
+// index; neighbours in ascending order; components in ascending order of
+// their lowest node) and cuts the ordering so that the two sides receive
+// node counts proportional to their numbers of parts.  Geometric graphs get
+// compact, slab-like parts; the result is deterministic.
+namespace {
+
+void bfs_from(int start, const std::vector<std::vector<int>>& adj, const std::vector<char>& in,
+              std::vector<char>& mark, std::vector<int>& out) {
+  size_t head = out.size();
+  out.push_back(start);
+  mark[start] = 1;
+  while (head < out.size()) {
+    const int v = out[head++];
+    for (int nb : adj[v])
+      if (in[nb] && !mark[nb]) {
+        mark[nb] = 1;
+        out.push_back(nb);
+      }
+  }
+}
+
+void bisect(std::vector<int> nodes, int k, int first, const std::vector<std::vector<int>>& adj,
+            std::vector<char>& in, std::vector<char>& mark, std::vector<int>& part) {
+  if (k == 1) {
+    for (int v : nodes) part[v] = first;
+    return;
+  }
+  for (int v : nodes) in[v] = 1;
+  std::vector<int> order, probe;
+  order.reserve(nodes.size());
+  for (int v : nodes) {
+    if (mark[v]) continue;
+    probe.clear();
+    bfs_from(v, adj, in, mark, probe);  // the component of v
+    for (int u : probe) mark[u] = 0;
+    bfs_from(probe.back(), adj, in, mark, order);
+  }
+  for (int v : nodes) {
+    in[v] = 0;
+    mark[v] = 0;
+  }
+  const int kl = k / 2;
+  const size_t nl = (size_t)((double)nodes.size() * kl / k + 0.5);
+  std::vector<int> left(order.begin(), order.begin() + nl);
+  std::vector<int> right(order.begin() + nl, order.end());
+  std::sort(left.begin(), left.end());
+  std::sort(right.begin(), right.end());
+  bisect(std::move(left), kl, first, adj, in, mark, part);
+  bisect(std::move(right), k - kl, first + kl, adj, in, mark, part);
+}
+
+// Row adjacency (nonzero off-diagonal entries, ascending) of the graph matrix.
+int graph_adjacency(bml_matrix_t* g_bml, std::vector<std::vector<int>>* adj) {
+  int st;
+  if ((st = bml_host_sync(g_bml))) return st;
+  const int N = g_bml->N;
+  const double* G = g_bml->host;
+  adj->assign(N, {});
+  for (int j = 0; j < N; ++j) {
+    const double* col = G + (size_t)j * N;
+    for (int i = 0; i < N; ++i)
+      if (i != j && (col[i] != 0.0 || G[j + (size_t)i * N] != 0.0)) (*adj)[i].push_back(j);
+  }
+  return OK;
+}
+
+double g_gp_ef = 0.0;
+
+}  // namespace
+
+int bfs_partition(bml_matrix_t* g_bml, int nparts, prg_graph_partitioning_t* gp) {
+  std::vector<std::vector<int>> adj;
+  int st;
+  if ((st = graph_adjacency(g_bml, &adj))) return st;
+  const int N = (int)adj.size();
+  if (nparts <= 0 || nparts > N) {
+    set_error(ERR_ARG, "graph partitioner: nparts must be in [1, N]");
+    return ERR_ARG;
+  }
+  std::vector<int> nodes(N), part(N, -1);
+  for (int i = 0; i < N; ++i) nodes[i] = i;
+  std::vector<char> in(N, 0), mark(N, 0);
+  bisect(std::move(nodes), nparts, 0, adj, in, mark, part);
+  if ((st = init_parts(gp, nparts, N))) return st;
+  for (int v = 0; v < N; ++v) gp->nnodesInPart[part[v]]++;
+  std::vector<int> fill(nparts, 0);
+  for (int p = 0; p < nparts; ++p)
+    gp->nodeInPart[p] = (int*)std::malloc(sizeof(int) * std::max(1, gp->nnodesInPart[p]));
+  for (int v = 0; v < N; ++v) gp->nodeInPart[part[v]][fill[part[v]]++] = v;
+  return OK;
+}
+
+// prg_build_densityGP_T0 (src/prg_graphsolver_mod.F90:39-191) on one GPU:
+// partition the graph (bfs_partition in place of METIS), core+halo lists
+// with the double jump, every part's Fermi density at kT = 0.1 from batched
+// eigensolves (prg_build_density_T_Fermi, src/prg_densitymatrix_mod.F90:509-582,
+// called with bndfil in the threshold slot as the reference does), core rows
+// scattered into rho, then the reference's chemical-potential update from the
+// trace error and the summed dN/dEf of the parts (:162-177).
+int graph_density_t0(const DMat& H, bml_matrix_t* g_bml, DMat rho, double bndfil, double* ef,
+                     int nparts, int verbose) {
+  int st = ensure_device();
+  if (st) return st;
+  prg_graph_partitioning_t gp;
+  std::memset(&gp, 0, sizeof(gp));
+  struct Guard {
+    prg_graph_partitioning_t* g;
+    ~Guard() { free_parts(g); }
+  } guard{&gp};
+  if ((st = bfs_partition(g_bml, nparts, &gp))) return st;
+  if ((st = halo_indices(g_bml, &gp, 1))) return st;
+  HostPlan hp;
+  BatchPlan plan;
+  if ((st = make_plan(&gp, &hp, &plan))) return st;
+  const int np = plan.nparts;
+  // eigenvalue offsets: m per part, packed in plan order
+  std::vector<size_t> woff(np);
+  size_t wtot = 0;
+  for (int q = 0; q < np; ++q) {
+    woff[q] = wtot;
+    wtot += hp.parts[q].m;
+  }
+  double* a = (double*)workspace("gpt0_a", sizeof(double) * plan.total, &st);
+  double* u = (double*)workspace("gpt0_u", sizeof(double) * plan.total, &st);
+  double* w = (double*)workspace("gpt0_w", sizeof(double) * wtot, &st);
+  size_t* d_woff = (size_t*)workspace("gpt0_woff", sizeof(size_t) * np, &st);
+  double* d_dn = (double*)workspace("gpt0_dn", sizeof(double) * np, &st);
+  if (st) return st;
+  PRG_HIP_CHECK(hipMemcpyAsync(d_woff, woff.data(), sizeof(size_t) * np, hipMemcpyHostToDevice,
+                               stream()));
+  // padding eigenvalues far above the spectrum: Fermi occupation 0
+  double hmax = 0.0;
+  {
+    double gb[2];
+    double* d_b = (double*)workspace("gpt0_bounds", sizeof(double) * 4, &st);
+    if (st || (st = gershgorin(H, d_b))) return st;
+    PRG_HIP_CHECK(hipMemcpyAsync(gb, d_b, sizeof(gb), hipMemcpyDeviceToHost, stream()));
+    PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+    if (!std::isfinite(gb[0]) || !std::isfinite(gb[1])) {
+      set_error(ERR_NUMERIC, "prg_build_densityGP_T0: non-finite Hamiltonian");
+      return ERR_NUMERIC;
+    }
+    hmax = std::max(std::fabs(gb[0]), std::fabs(gb[1]));
+  }
+  const double kbt = 0.1;
+  const double pad = std::fabs(*ef) + hmax + 1.0e3 * kbt + 1.0;
+  if ((st = batched_gather_pad(H, plan, a, pad))) return st;
+  // one strided-batched eigensolve per run of equal padded sizes
+  for (int q0 = 0; q0 < np;) {
+    int q1 = q0;
+    while (q1 < np && hp.parts[q1].m == hp.parts[q0].m) ++q1;
+    const int m = hp.parts[q0].m;
+    if ((st = eig_strided_batched(a + hp.parts[q0].offset, m, q1 - q0, w + woff[q0]))) return st;
+    q0 = q1;
+  }
+  if ((st = batched_fermi_scale(plan, a, w, d_woff, u, *ef, kbt, d_dn))) return st;
+  // part densities U V^T
+  double* dens = (double*)workspace("gpt0_d", sizeof(double) * plan.total, &st);
+  if (st) return st;
+  for (int q0 = 0; q0 < np;) {
+    int q1 = q0;
+    while (q1 < np && hp.parts[q1].m == hp.parts[q0].m) ++q1;
+    const size_t o = hp.parts[q0].offset;
+    if ((st = gemm_nt_strided_batched(u + o, a + o, dens + o, hp.parts[q0].m, q1 - q0))) return st;
+    q0 = q1;
+  }
+  if ((st = batched_scatter(plan, dens, rho, 1.0))) return st;
+  std::vector<double> dn(np);
+  PRG_HIP_CHECK(hipMemcpyAsync(dn.data(), d_dn, sizeof(double) * np, hipMemcpyDeviceToHost,
+                               stream()));
+  ops::Sums sm;
+  if ((st = ops::sums_of(rho, &sm))) return st;
+  double dnocc = 0.0;
+  for (int q = 0; q < np; ++q) dnocc += dn[q];  // parts in packed order
+  const double nocc = sm.tr;
+  const double target = 2.0 * bndfil * rho.n;
+  const double error = std::fabs(nocc - target);
+  if (nocc < target) dnocc = -dnocc;
+  if (verbose > 1)
+    std::printf("prg_build_densityGP_T0: Ef %.10g Nocc %.10g error %.3e gradient %.6g\n", *ef,
+                nocc, error, dnocc);
+  if (error > 1.0e-6 && std::fabs(dnocc) > 1.0e-6) {
+    // the reference steps by nocc (not the error) when the error exceeds 1
+    if (error > 1.0)
+      *ef = *ef - nocc / dnocc;
+    else
+      *ef = *ef - error / dnocc;
+  }
   return OK;
 }
 
@@ -301,6 +518,38 @@ void prg_subgraphSP2Loop(bml_matrix_t* h_bml, bml_matrix_t* g_bml, bml_matrix_t*
   if (subgraph_sp2_loop(H, R, gp)) return;
   prg_fnormGraph(gp);
 }
+
+// The build's partitioner (recursive BFS bisection) in place of METIS; no
+// reference counterpart as a C symbol (prg_metisPartition,
+// src/prg_partition_mod.F90:216-311, is what the reference calls).
+void prg_bfsPartition(prg_graph_partitioning_t* gp, bml_matrix_t* g_bml, int nparts) {
+  clear_error();
+  if (!gp || !g_bml) {
+    set_error(ERR_ARG, "prg_bfsPartition: null argument");
+    return;
+  }
+  bfs_partition(g_bml, nparts, gp);
+}
+
+// src/prg_progress_mod.h:803-811 (Fortran src/prg_graphsolver_mod.F90:39-191).
+// Ef by value as the reference header declares it; the updated value is
+// returned by prg_build_densityGP_T0_ef().
+void prg_build_densityGP_T0(bml_matrix_t* ham_bml, bml_matrix_t* g_bml, bml_matrix_t* rho_bml,
+                            double threshold, double bndfil, double Ef, int nparts, int verbose) {
+  (void)threshold;
+  clear_error();
+  g_gp_ef = Ef;
+  if (check_pair(ham_bml, g_bml, "prg_build_densityGP_T0") ||
+      check_pair(ham_bml, rho_bml, "prg_build_densityGP_T0"))
+    return;
+  DMat H, R;
+  if (bml_dev_read(ham_bml, &H) || bml_dev_write(rho_bml, &R)) return;
+  double ef = Ef;
+  if (graph_density_t0(H, g_bml, R, bndfil, &ef, nparts, verbose)) return;
+  g_gp_ef = ef;
+}
+
+double prg_build_densityGP_T0_ef(void) { return g_gp_ef; }
 
 // Device time (ms) of the last prg_subgraphSP2Loop: gather + normalise, the
 // purification loop, the core-row scatter.

@@ -27,6 +27,12 @@ int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double minev
                    double maxeval);
 int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
                  double* d_vv, int it);
-int batched_scatter(const BatchPlan& plan, const double* x, DMat rho);
+int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale);
+// X_p = H[idx, idx], padding decoupled with `pad` on its diagonal
+int batched_gather_pad(const DMat& H, const BatchPlan& plan, double* x, double pad);
+// U_p = V_p diag(2 f(w_p)), dnocc[p] = sum 2 df/def(w_p)  (finite-T occupations)
+int batched_fermi_scale(const BatchPlan& plan, const double* v, const double* w,
+                        const size_t* d_w_offset, double* u, double ef, double kbt,
+                        double* d_dnocc);
 
 }  // namespace prg

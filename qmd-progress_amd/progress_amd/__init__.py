@@ -98,6 +98,9 @@ def lib():
         "prg_getPartitionHalosFromGraph": (None, [vp, vp, i]),
         "prg_fnormGraph": (None, [vp]),
         "prg_subgraphSP2Loop": (None, [vp, vp, vp, vp, d]),
+        "prg_bfsPartition": (None, [vp, vp, i]),
+        "prg_build_densityGP_T0": (None, [vp, vp, vp, d, d, d, i, i]),
+        "prg_build_densityGP_T0_ef": (d, []),
         "prg_gpu_subgraph_timing": (None, [dp, dp, dp]),
         "prg_gpu_gershgorin_device": (i, [i, vp, dp]),
         "prg_gpu_sp2_seq_device": (i, [i, vp, vp, ip, i, d, d]),
@@ -622,6 +625,26 @@ def subgraph_sp2_loop(h, g, gp, pp, max_iter, mineval, maxeval, vv=None):
     lib().prg_subgraphSP2Loop(mh.h, mg.h, rho.h, C.byref(gp), 0.0)
     _check()
     return rho.numpy(), [gp.vv[k] for k in range(max_iter)]
+
+
+def bfs_partition(g, nparts):
+    """The build's graph partitioner (in place of METIS): returns a
+    GraphPartitioning with nparts cores (ascending node ids)."""
+    gp = GraphPartitioning()
+    mg = Matrix.from_numpy(g)
+    lib().prg_bfsPartition(C.byref(gp), mg.h, nparts)
+    _check()
+    return gp
+
+
+def build_density_gp_t0(h, g, bndfil, ef, nparts, threshold=0.0, verbose=0):
+    """prg_build_densityGP_T0: returns (rho, updated Ef)."""
+    n = h.shape[0]
+    mh, mg = Matrix.from_numpy(h), Matrix.from_numpy(g)
+    rho = Matrix.zeros(n)
+    lib().prg_build_densityGP_T0(mh.h, mg.h, rho.h, threshold, bndfil, ef, nparts, verbose)
+    _check()
+    return rho.numpy(), float(lib().prg_build_densityGP_T0_ef())
 
 
 def subgraph_timing():

@@ -73,3 +73,32 @@ def test_reference_subgraphsp2_equal(P, golden):
     assert np.linalg.norm(x @ x - x) <= 1e-2
     lsz = [gp.lsize[i] for i in range(gp.totalParts)]
     print(f"  parts={gp.totalParts} lsize range=({min(lsz)}, {max(lsz)}) iters={imax}")
+
+
+@pytest.mark.parametrize("n,nparts,gthr", [(400, 6, 0.05), (700, 9, 0.02)])
+def test_build_density_gp_t0_matches_oracle(P, n, nparts, gthr):
+    """prg_build_densityGP_T0: the partition (the build's BFS partitioner, in
+    place of METIS), the per-part kT = 0.1 densities from batched eigensolves,
+    the core-row assembly and the Fermi-level step, against the oracle."""
+    print(f"[gp_t0] n={n} nparts={nparts}", flush=True)
+    h = O.model_hamiltonian(n, 4)
+    g = np.where(np.abs(h) >= gthr, h, 0.0)
+    parts = O.bfs_partition(g, nparts)
+    assert P.bfs_partition(g, nparts).parts() == parts
+    w = np.linalg.eigvalsh(h)
+    ef = 0.5 * (w[n // 2 - 1] + w[n // 2])
+    rho_o, ef_o = O.build_density_gp_t0(h, g, 0.5, ef, parts)
+    rho_g, ef_g = P.build_density_gp_t0(h, g, 0.5, ef, nparts)
+    assert np.linalg.norm(rho_g - rho_o) < 1e-9
+    assert abs(ef_g - ef_o) < 1e-9 * max(1.0, abs(ef_o))
+
+
+def test_build_density_gp_t0_one_part_is_exact(P):
+    """One part holding every node: the result is the full kT = 0.1 density."""
+    n = 300
+    print(f"[gp_t0 one part] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 1)
+    g = np.where(h != 0, 1.0, 0.0)
+    rho_g, _ = P.build_density_gp_t0(h, g, 0.5, 0.2, 1)
+    d, _ = O.build_density_T_fermi(h, 0.1, 0.2)
+    assert np.linalg.norm(rho_g - d) < 1e-9

@@ -33,3 +33,62 @@ def test_homolumogap_and_sequence_match_oracle():
     pps, imax = P.sp2sequence(lo, hi, ref[0], ref[1], 1e-12)
     ppo, imo = O.sp2sequence(lo, hi, ref[0], ref[1], 1e-12)
     assert imax == imo and pps == ppo
+
+
+def _grid(m):
+    n = m * m
+    g = np.zeros((n, n))
+    for r in range(m):
+        for c in range(m):
+            i = r * m + c
+            if c + 1 < m:
+                g[i, i + 1] = g[i + 1, i] = 1.0
+            if r + 1 < m:
+                g[i, i + m] = g[i + m, i] = 1.0
+    return g
+
+
+def _geometric(n, cutoff, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.0, (n, 3))
+    d = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=-1)
+    g = np.where((d < cutoff) & (d > 0), np.exp(-d), 0.0)
+    return g
+
+
+@pytest.mark.parametrize("case", ["path", "grid", "geometric", "components"])
+def test_bfs_partition_matches_oracle(case):
+    """The partitioner that replaces METIS: the C-ABI and the oracle produce the
+    same parts; every node lands in exactly one part; part sizes follow the
+    proportional cuts (within one node per bisection level)."""
+    import progress_amd as P
+    if case == "path":
+        n = 10
+        g = np.zeros((n, n))
+        for i in range(n - 1):
+            g[i, i + 1] = g[i + 1, i] = 1.0
+        nparts, expect = 3, [[7, 8, 9], [3, 4, 5, 6], [0, 1, 2]]
+    elif case == "grid":
+        g, nparts, expect = _grid(12), 8, None
+    elif case == "geometric":
+        g, nparts, expect = _geometric(300, 0.22, 4), 7, None
+    else:
+        g = np.zeros((40, 40))
+        g[:25, :25] = _grid(5)
+        g[25:, 25:] = np.eye(15, k=1) + np.eye(15, k=-1)
+        nparts, expect = 5, None
+    ours = P.bfs_partition(g, nparts).parts()
+    ref = O.bfs_partition(g, nparts)
+    assert ours == ref
+    if expect is not None:
+        assert ref == expect
+    allnodes = sorted(v for p in ref for v in p)
+    assert allnodes == list(range(g.shape[0]))
+    sizes = [len(p) for p in ref]
+    assert max(sizes) - min(sizes) <= int(np.ceil(np.log2(nparts))) + 1
+
+
+def test_bfs_partition_rejects_bad_count():
+    import progress_amd as P
+    with pytest.raises(P.ProgressError):
+        P.bfs_partition(np.eye(4), 5)

@@ -314,3 +314,80 @@ def test_planted_branch_swap_is_caught(model256):
     swapped = [1 - s for s in pp]
     x = O.sp2_poly_eigen(w, lo, hi, swapped)
     assert np.linalg.norm(rho - (q * (2 * x)) @ q.T) > 1.0
+
+
+def _fermi_closed_form(h, kbt, ef):
+    w, q = eig(h)
+    f = 1.0 / (1.0 + np.exp((w - ef) / kbt))
+    df = f * f * np.exp((w - ef) / kbt) / kbt
+    return (q * (2.0 * f)) @ q.T, float(np.sum(2.0 * df))
+
+
+def test_gp_t0_single_part_is_the_fermi_density():
+    """prg_build_densityGP_T0 with one part holding every node: the density is
+    2 f(H) at kT = 0.1 and the Fermi-level step follows the reference's rule
+    (error <= 1: ef - error/dN; nocc below target flips the gradient sign)."""
+    n = 120
+    h = O.model_hamiltonian(n, 2)
+    g = np.where(np.abs(h) > 0, 1.0, 0.0)
+    w = np.linalg.eigvalsh(h)
+    branches = set()
+    for shift in (0.0, 0.02, -0.3, 0.4):
+        ef = 0.5 * (w[n // 2 - 1] + w[n // 2]) + shift
+        rho, ef1 = O.build_density_gp_t0(h, g, 0.5, ef, [list(range(n))])
+        d, dn = _fermi_closed_form(h, 0.1, ef)
+        assert np.linalg.norm(rho - d) < 1e-10
+        nocc = np.trace(d)
+        err = abs(nocc - n)
+        signed = dn if nocc >= n else -dn
+        step = (nocc if err > 1.0 else err) / signed
+        branches.add(err > 1.0)
+        assert abs(ef1 - (ef - step)) < 1e-12 * max(1.0, abs(ef1))
+        if err <= 1.0:
+            # the small-error step moves Ef toward the target occupation
+            assert (ef1 - ef) * (n - nocc) > 0
+    assert branches == {True, False}
+
+
+def test_gp_t0_block_diagonal_parts_are_exact():
+    """Block-diagonal H whose blocks are the parts: the halos are empty and the
+    assembled density equals 2 f(H) of the whole matrix."""
+    rng = np.random.default_rng(7)
+    sizes = [30, 45, 25]
+    n = sum(sizes)
+    h = np.zeros((n, n))
+    parts, o = [], 0
+    for s in sizes:
+        a = rng.standard_normal((s, s))
+        h[o:o + s, o:o + s] = 0.5 * (a + a.T)
+        parts.append(list(range(o, o + s)))
+        o += s
+    g = np.where(h != 0, 1.0, 0.0)
+    rho, _ = O.build_density_gp_t0(h, g, 0.5, 0.0, parts)
+    d, _ = _fermi_closed_form(h, 0.1, 0.0)
+    assert np.linalg.norm(rho - d) < 1e-10
+    # coupled parts: every row comes from the part that owns the node as core
+    # (its core+halo solution), with zeros outside that part's index list
+    h2 = h.copy()
+    h2[29, 30] = h2[30, 29] = 0.3
+    g2 = np.where(h2 != 0, 1.0, 0.0)
+    rho2, _ = O.build_density_gp_t0(h2, g2, 0.5, 0.0, parts)
+    for p, core in enumerate(parts):
+        idx, nc = O.matrix2submatrix_index(g2, core, True)
+        dp, _ = _fermi_closed_form(h2[np.ix_(idx, idx)], 0.1, 0.0)
+        assert np.allclose(rho2[np.ix_(idx[:nc], idx)], dp[:nc], atol=1e-12)
+        others = np.setdiff1d(np.arange(n), idx)
+        assert np.all(rho2[np.ix_(idx[:nc], others)] == 0.0)
+
+
+
+def test_bfs_partition_kats():
+    """Path graph of 10 nodes into 3 parts and a 4 x 4 grid into 4 parts (hand
+    traced: BFS from the far end, proportional cuts, left part first)."""
+    n = 10
+    g = np.zeros((n, n))
+    for i in range(n - 1):
+        g[i, i + 1] = g[i + 1, i] = 1.0
+    assert O.bfs_partition(g, 3) == [[7, 8, 9], [3, 4, 5, 6], [0, 1, 2]]
+    assert O.bfs_partition(g, 1) == [list(range(10))]
+    assert O.bfs_partition(g, 2) == [[5, 6, 7, 8, 9], [0, 1, 2, 3, 4]]
