@@ -137,6 +137,36 @@ void prg_sp2_submatrix(bml_matrix_t *ham_bml, bml_matrix_t *rho_bml, double thre
 void prg_sp2_submatrix_inplace(bml_matrix_t *rho_bml, double threshold, int *pp, int *icount,
                                double *vv, double mineval, double maxeval, int core_size);
 
+/* ------------------------------------------------------------------ truncated SP2 (finite T) */
+/* src/prg_c_interface.F90:904-927 (prg_sp2_fermi_mod.F90:43-185).  sgnlist:
+ * caller array of length >= nsteps (+1 = 2X - X^2, -1 = X^2), filled on the
+ * first pass; mu, beta, h1, hN are outputs.  The trailing verbose is the
+ * bind(C) wrapper's (the header at src/prg_progress_mod.h:435-448 omits it). */
+void prg_sp2_fermi_init(bml_matrix_t *h_bml, int nsteps, double nocc, double tscale,
+                        double threshold, double occErrLimit, double traceLimit,
+                        bml_matrix_t *x_bml, double *mu, double *beta, double *h1, double *hN,
+                        int *sgnlist, int verbose);
+/* src/prg_c_interface.F90:929-955 (prg_sp2_fermi_mod.F90:203-387): *beta in is
+ * the target inverse temperature; *nsteps is cut where the SP2 polynomial's
+ * steepest slope first exceeds it. */
+void prg_sp2_fermi_init_norecs(bml_matrix_t *h_bml, int *nsteps, double nocc, double tscale,
+                               double threshold, double occErrLimit, double traceLimit,
+                               bml_matrix_t *x_bml, double *mu, double *beta, double *h1,
+                               double *hN, int *sgnlist, int verbose);
+/* src/prg_c_interface.F90:957-979 (prg_sp2_fermi_mod.F90:405-470); x_bml gets
+ * the occupation-corrected X (not doubled), *mu the converged value. */
+void prg_sp2_fermi(bml_matrix_t *h_bml, int osteps, int nsteps, double nocc, double *mu,
+                   double *beta, double *h1, double *hN, int *sgnlist, double threshold,
+                   double eps, double traceLimit, bml_matrix_t *x_bml);
+/* src/prg_c_interface.F90:981-995 (prg_sp2_fermi_mod.F90:480-531); GG, ee:
+ * caller arrays of 10001 doubles (the mesh 0, 1e-4, ..., 1). */
+void prg_sp2_entropy_function(double mu, double h1, double hN, int nsteps, const int *sgnlist,
+                              double *GG, double *ee);
+/* sp2_inverse (prg_sp2_fermi_mod.F90:720-745) and sp2_entropy_ts (:541-587);
+ * no C symbols in the reference. */
+double prg_sp2_inverse(double f, double mu, double h1, double hN, int nsteps, const int *sgnlist);
+double prg_sp2_entropy_ts(bml_matrix_t *d0_bml, const double *GG, const double *ee);
+
 /* ------------------------------------------------------------------ homo-lumo / sequence */
 /* New C symbols for src/prg_homolumo_mod.F90:22-89 and :97-147 (no bind(C)
  * wrapper in the reference).  pp/vv hold the reference's pp(1..), vv(1..). */

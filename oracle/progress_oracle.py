@@ -907,6 +907,134 @@ def subgraph_sp2_loop(h, g, parts, pp, max_iter, mineval, maxeval, double_jump=T
     return rho, vv
 
 
+def normalize_fermi(h, h1, hn, mu):
+    """prg_normalize_fermi (src/prg_normalize_mod.F90:59-72):
+    X = ((hN - mu) I - H) / (hN - h1)."""
+    n = h.shape[0]
+    return ((hn - mu) * np.eye(n) - h) / (hn - h1)
+
+
+def _absmaxderivative(f, de):
+    """absmaxderivative (src/prg_sp2_fermi_mod.F90:754-768)."""
+    return float(np.max(np.abs(np.diff(f)) / de)) if len(f) > 1 else -10000.0
+
+
+def sp2_fermi_init(h, nsteps, nocc, tscale, occ_err_limit, trace_limit, norecs=False,
+                   beta=0.0, max_outer=200):
+    """prg_sp2_fermi_init (src/prg_sp2_fermi_mod.F90:43-185) and, with norecs,
+    prg_sp2_fermi_init_norecs (:203-387).  Returns
+    dict(x, mu, beta, h1, hN, sgnlist, nsteps)."""
+    lo, hi = gershgorin(h)
+    beta0 = beta
+    mu = 0.5 * (hi + lo)
+    h1, hn = tscale * lo, tscale * hi
+    n = h.shape[0]
+    sgn = [0] * nsteps
+    first = True
+    occ_err = 1.0
+    outer = 0
+    while occ_err > occ_err_limit:
+        outer += 1
+        if outer > max_outer:
+            raise NotConverging("prg_sp2_fermi_init: occupation loop did not converge")
+        x = normalize_fermi(h, h1, hn, mu)
+        probe = 1.0 - 0.001 * np.arange(1, 1001)
+        x1 = -np.eye(n) / (hn - h1)
+        for i in range(nsteps):
+            x2 = x @ x
+            tr0, tr2 = np.trace(x), np.trace(x2)
+            if first:
+                sgn[i] = -1 if abs(tr2 - nocc) < abs(2.0 * tr0 - tr2 - nocc) else 1
+            t = x @ x1 + x1 @ x
+            if sgn[i] == 1:
+                x1 = 2.0 * x1 - t
+                x = 2.0 * x - x2
+                probe = 2.0 * probe - probe * probe
+            else:
+                x1 = t
+                x = x2
+                probe = probe * probe
+            if norecs:
+                beta = 4.0 * _absmaxderivative(probe, 0.001) / (hi - lo)
+                if beta > beta0:
+                    nsteps = i + 1
+                    break
+        first = False
+        tr0, tr1 = np.trace(x), np.trace(x1)
+        occ_err = abs(nocc - tr0)
+        lam = (nocc - tr0) / tr1 if abs(tr1) > trace_limit else 0.0
+        mu += lam
+    trx = np.trace(x @ (np.eye(n) - x))
+    beta = -np.trace(x1) / trx if abs(trx) > trace_limit else -1000.0
+    return {"x": x, "mu": mu, "beta": beta, "h1": h1, "hN": hn, "sgnlist": sgn[:nsteps],
+            "nsteps": nsteps}
+
+
+def sp2_fermi(h, osteps, nsteps, nocc, mu, beta, h1, hn, sgnlist, eps, trace_limit,
+              max_outer=200):
+    """prg_sp2_fermi (src/prg_sp2_fermi_mod.F90:405-470).  Returns (x, mu)."""
+    n = h.shape[0]
+    occ_err = 1.0 + eps
+    it = 0
+    lam = 0.0
+    dx = np.zeros((n, n))
+    x = None
+    while (osteps == 0 and occ_err > eps) or (osteps > 0 and it < osteps):
+        it += 1
+        if osteps == 0 and it > max_outer:
+            raise NotConverging("prg_sp2_fermi: occupation loop did not converge")
+        x = normalize_fermi(h, h1, hn, mu)
+        for i in range(nsteps):
+            x2 = x @ x
+            x = 2.0 * x - x2 if sgnlist[i] == 1 else x2
+        tr0 = np.trace(x)
+        occ_err = abs(nocc - tr0)
+        dx = -beta * (x @ (np.eye(n) - x))
+        trdx = np.trace(dx)
+        lam = (nocc - tr0) / trdx if abs(trdx) > trace_limit else 0.0
+        mu += lam
+    return x + lam * dx, mu
+
+
+def sp2_inverse(f, mu, h1, hn, sgnlist):
+    """sp2_inverse (src/prg_sp2_fermi_mod.F90:720-745)."""
+    e = f
+    for s in reversed(sgnlist):
+        c = 0.5 * (1.0 + s)
+        e = c - s * math.sqrt(abs(c - s * e))
+    return hn - mu - e * (hn - h1)
+
+
+def sp2_entropy_function(mu, h1, hn, sgnlist):
+    """prg_sp2_entropy_function (src/prg_sp2_fermi_mod.F90:480-531): GG on the
+    mesh ee = 0, 1e-4, ..., 1 (two-point Gauss per cell), minus its linear
+    trend.  Returns (GG, ee)."""
+    n = 10001
+    ee = np.arange(n) * 0.0001
+    gg = np.zeros(n)
+    x1 = 0.5773502691896257
+    acc = 0.0
+    for i in range(n - 1):
+        a, b = ee[i], ee[i + 1]
+        c = 0.5 * (b - a)
+        acc += c * sp2_inverse(0.5 * ((b - a) * x1 + (b + a)), mu, h1, hn, sgnlist)
+        acc += c * sp2_inverse(0.5 * ((b - a) * -x1 + (b + a)), mu, h1, hn, sgnlist)
+        gg[i + 1] = acc
+    return gg - gg[-1] * ee, ee
+
+
+def sp2_entropy_ts(d0, gg, ee):
+    """sp2_entropy_ts (src/prg_sp2_fermi_mod.F90:541-587): linear interpolation
+    of GG at |eigenvalue| of D0 (mesh index 1..10000)."""
+    ts = 0.0
+    for x in np.linalg.eigvalsh(d0):
+        hs = abs(x)
+        j = int(math.floor(hs / 0.0001)) + 1
+        if 0 < j <= 10000:
+            ts += ((hs - ee[j - 1]) * gg[j] + (ee[j] - hs) * gg[j - 1]) / (ee[j] - ee[j - 1])
+    return ts
+
+
 def bfs_partition(g, nparts):
     """The build's replacement for METIS in prg_build_densityGP_T0
     (prg_metisPartition, src/prg_partition_mod.F90:216-311): recursive

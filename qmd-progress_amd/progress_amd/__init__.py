@@ -99,6 +99,12 @@ def lib():
         "prg_fnormGraph": (None, [vp]),
         "prg_subgraphSP2Loop": (None, [vp, vp, vp, vp, d]),
         "prg_bfsPartition": (None, [vp, vp, i]),
+        "prg_sp2_fermi_init": (None, [vp, i, d, d, d, d, d, vp, dp, dp, dp, dp, ip, i]),
+        "prg_sp2_fermi_init_norecs": (None, [vp, ip, d, d, d, d, d, vp, dp, dp, dp, dp, ip, i]),
+        "prg_sp2_fermi": (None, [vp, i, i, d, dp, dp, dp, dp, ip, d, d, d, vp]),
+        "prg_sp2_entropy_function": (None, [d, d, d, i, ip, dp, dp]),
+        "prg_sp2_inverse": (d, [d, d, d, d, i, ip]),
+        "prg_sp2_entropy_ts": (d, [vp, dp, dp]),
         "prg_build_densityGP_T0": (None, [vp, vp, vp, d, d, d, i, i]),
         "prg_build_densityGP_T0_ef": (d, []),
         "prg_gpu_subgraph_timing": (None, [dp, dp, dp]),
@@ -625,6 +631,66 @@ def subgraph_sp2_loop(h, g, gp, pp, max_iter, mineval, maxeval, vv=None):
     lib().prg_subgraphSP2Loop(mh.h, mg.h, rho.h, C.byref(gp), 0.0)
     _check()
     return rho.numpy(), [gp.vv[k] for k in range(max_iter)]
+
+
+def sp2_fermi_init(h, nsteps, nocc, tscale, occ_err_limit, trace_limit, norecs=False,
+                   beta=0.0, threshold=0.0):
+    """prg_sp2_fermi_init (norecs=False) or prg_sp2_fermi_init_norecs (norecs=True,
+    beta = target inverse temperature).  Returns
+    dict(x, mu, beta, h1, hN, sgnlist, nsteps)."""
+    n = h.shape[0]
+    mh = Matrix.from_numpy(h)
+    mx = Matrix.zeros(n)
+    mu, be, h1, hn = _dvec(1), _dvec(1, [beta]), _dvec(1), _dvec(1)
+    sg = _ivec(max(nsteps, 1))
+    if norecs:
+        ns = _ivec(1, [nsteps])
+        lib().prg_sp2_fermi_init_norecs(mh.h, _ip(ns), nocc, tscale, threshold, occ_err_limit,
+                                        trace_limit, mx.h, _dp(mu), _dp(be), _dp(h1), _dp(hn),
+                                        _ip(sg), 0)
+        nsteps = int(ns[0])
+    else:
+        lib().prg_sp2_fermi_init(mh.h, nsteps, nocc, tscale, threshold, occ_err_limit,
+                                 trace_limit, mx.h, _dp(mu), _dp(be), _dp(h1), _dp(hn), _ip(sg), 0)
+    _check()
+    return {"x": mx.numpy(), "mu": float(mu[0]), "beta": float(be[0]), "h1": float(h1[0]),
+            "hN": float(hn[0]), "sgnlist": sg[:nsteps].tolist(), "nsteps": nsteps}
+
+
+def sp2_fermi(h, osteps, nsteps, nocc, mu, beta, h1, hN, sgnlist, eps, trace_limit,
+              threshold=0.0):
+    """prg_sp2_fermi: returns (x, mu)."""
+    n = h.shape[0]
+    mh = Matrix.from_numpy(h)
+    mx = Matrix.zeros(n)
+    m, be, a, b = _dvec(1, [mu]), _dvec(1, [beta]), _dvec(1, [h1]), _dvec(1, [hN])
+    sg = _ivec(max(nsteps, 1), list(sgnlist[:nsteps]) + [0] * max(0, 1 - nsteps))
+    lib().prg_sp2_fermi(mh.h, osteps, nsteps, nocc, _dp(m), _dp(be), _dp(a), _dp(b), _ip(sg),
+                        threshold, eps, trace_limit, mx.h)
+    _check()
+    return mx.numpy(), float(m[0])
+
+
+def sp2_entropy_function(mu, h1, hN, sgnlist):
+    """prg_sp2_entropy_function: returns (GG, ee), 10001 points each."""
+    sg = _ivec(max(len(sgnlist), 1), list(sgnlist) or [0])
+    gg, ee = _dvec(10001), _dvec(10001)
+    lib().prg_sp2_entropy_function(mu, h1, hN, len(sgnlist), _ip(sg), _dp(gg), _dp(ee))
+    _check()
+    return gg, ee
+
+
+def sp2_inverse(f, mu, h1, hN, sgnlist):
+    sg = _ivec(max(len(sgnlist), 1), list(sgnlist) or [0])
+    return float(lib().prg_sp2_inverse(f, mu, h1, hN, len(sgnlist), _ip(sg)))
+
+
+def sp2_entropy_ts(d0, gg, ee):
+    md = Matrix.from_numpy(d0)
+    ts = float(lib().prg_sp2_entropy_ts(md.h, _dp(np.ascontiguousarray(gg, dtype=np.float64)),
+                                        _dp(np.ascontiguousarray(ee, dtype=np.float64))))
+    _check()
+    return ts
 
 
 def bfs_partition(g, nparts):

@@ -92,3 +92,16 @@ def test_bfs_partition_rejects_bad_count():
     import progress_amd as P
     with pytest.raises(P.ProgressError):
         P.bfs_partition(np.eye(4), 5)
+
+
+def test_sp2_inverse_and_entropy_function_match_oracle():
+    """Host-side helpers of the truncated SP2 module through the C-ABI."""
+    import progress_amd as P
+    sg = [1, -1, 1, 1, -1, 1, -1]
+    mu, h1, hn = 0.15, -2.0, 3.0
+    for f in (0.01, 0.4, 0.5, 0.93):
+        assert abs(P.sp2_inverse(f, mu, h1, hn, sg) - O.sp2_inverse(f, mu, h1, hn, sg)) < 1e-14
+    gg, ee = P.sp2_entropy_function(mu, h1, hn, sg)
+    go, eo = O.sp2_entropy_function(mu, h1, hn, sg)
+    assert np.array_equal(ee, eo)
+    assert np.max(np.abs(gg - go)) < 1e-12
