@@ -309,7 +309,7 @@ void prg_set_precision(int mode);
 int prg_get_precision(void);
 /* Number of moduli (6..14, default 11) of PRG_PREC_INT8CRT; X is scaled by
  * 2^a with a = floor((log2 M - 3) / 2): 8 -> 30, 9 -> 34, 10 -> 38, 11 -> 42,
- * 12 -> 45, 13 -> 49, 14 -> 52 bits.  Returns a status. */
+ * 12 -> 45, 13 -> 49, 14 -> 53 bits.  Returns a status. */
 int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 

@@ -1035,6 +1035,46 @@ def sp2_entropy_ts(d0, gg, ee):
     return ts
 
 
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199)
+
+
+def crt_constants(nmod):
+    """Moduli, q_k = (M/p_k)^-1 mod p_k and the scaling exponent a (largest with
+    2^(2a) <= M/8) of the int8 Chinese-remainder products (no reference
+    counterpart: the build's FP64-accurate mixed-precision X^2)."""
+    ps = CRT_MODULI[:nmod]
+    m = 1
+    for p in ps:
+        m *= p
+    qs = [pow(m // p % p, -1, p) for p in ps]
+    a = (m.bit_length() - 1 - 3) // 2
+    while 4 ** (a + 1) <= m // 8:
+        a += 1
+    while 4 ** a > m // 8:
+        a -= 1
+    return ps, qs, a, m
+
+
+def crt_square(x, nmod):
+    """X^2 the way the int8 CRT kernel forms it: A = rint(2^a X), per modulus the
+    exact integer product of the symmetric residues, the fraction sum of
+    ((C mod p) q mod p) / p over the moduli, reduced mod 1, times M / 2^(2a).
+    Integer products in int64 (exact: |sum| < K 128^2)."""
+    ps, qs, a, m = crt_constants(nmod)
+    av = np.rint(np.asarray(x) * 2.0 ** a)
+    s = np.zeros_like(av)
+    for p, q in zip(ps, qs):
+        r = av - p * np.rint(av / p)
+        r = np.where(r > 127, r - p, r)
+        r = np.where(r < -128, r + p, r).astype(np.int64)
+        c = r @ r.T
+        v = (c * q).astype(np.float64)
+        md = v - p * np.rint(v / p)
+        s += md / p
+    f = s - np.rint(s)
+    return f * (float(m) / 4.0 ** a)
+
+
 def bfs_partition(g, nparts):
     """The build's replacement for METIS in prg_build_densityGP_T0
     (prg_metisPartition, src/prg_partition_mod.F90:216-311): recursive

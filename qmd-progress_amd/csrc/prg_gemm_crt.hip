@@ -14,7 +14,8 @@
 //   q_k = (M / p_k)^-1 mod p_k,
 // summed in FP64 per output element as the moduli are processed, so that
 // X^2 ~= frac * M / 2^(2a).  The only rounding is that of A (2^-(a+1) per
-// entry) and of the FP64 fraction sum.
+// entry) and of the FP64 fraction sum (each term a symmetric remainder in
+// [-1/2, 1/2], the sum reduced mod 1 once at the end).
 //
 // The fused kernel computes the upper-triangle 128 x 128 tiles of X^2, one
 // modulus after another over the full K range, applies the SP2 branch update
@@ -94,12 +95,15 @@ __device__ __forceinline__ void wait_vm_barrier() {
   asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" : : "n"(N) : "memory");
 }
 
-// Symmetric residue of the integer-valued double v modulo p, in [-128, 127].
+// Symmetric residue of the integer-valued double v (|v| <= 2^53) modulo p,
+// in [-128, 127]: v - p rint(v / p), formed exactly by the fma.  v * inv
+// carries a relative error of 2^-52, so for the largest |v| the quotient may
+// round to a neighbouring multiple; the two range checks move such a result
+// back by one p (and map the p = 256 remainder 128 to -128).
 __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
-  double m = fma(-p, floor(v * inv), v);  // exact; off by at most one p
-  m += (m < 0.0) ? p : 0.0;
-  m -= (m >= p) ? p : 0.0;
-  m -= (m > 127.0) ? p : 0.0;
+  double m = fma(-p, rint(v * inv), v);
+  m = (m > 127.0) ? m - p : m;
+  m = (m < -128.0) ? m + p : m;
   return (int)m;
 }
 
@@ -229,13 +233,13 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
       for (int n = 0; n < 4; ++n) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+          // (acc q mod p) / p as a symmetric remainder in [-1/2, 1/2]; the
+          // sum over the moduli stays below NP / 2 in magnitude, so it is
+          // reduced mod 1 only at the end (a quotient rounded to the
+          // neighbouring integer shifts the term by exactly 1: harmless)
           const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^39
-          double md = fma(-p, floor(v * inv), v);
-          md += (md < 0.0) ? p : 0.0;
-          md -= (md >= p) ? p : 0.0;
-          double sm = fma(md, inv, S[m][n][r]);
-          sm -= (sm >= 1.0) ? 1.0 : 0.0;
-          S[m][n][r] = sm;
+          const double md = fma(-p, rint(v * inv), v);
+          S[m][n][r] = fma(md, inv, S[m][n][r]);
         }
       }
   }
@@ -260,8 +264,7 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
         const int col = col0 + r;
         const size_t idx = row + (size_t)col * ld;
         const size_t tdx = col + (size_t)row * ld;
-        double f = S[m][n][r];
-        f -= (f >= 0.5) ? 1.0 : 0.0;
+        const double f = S[m][n][r] - rint(S[m][n][r]);  // C / M mod 1 in [-1/2, 1/2]
         const double y = f * g.c.cscale;
         const double x = X[idx];
         const double z = fma(cb, x, ca * y);

@@ -103,3 +103,22 @@ def test_crt_device_api_accuracy(P, n):
     assert np.linalg.norm(rho - rho_o) < 1e-7
     assert np.linalg.norm(rho - d0) < 1e-6
     assert float(rd[n:, :].abs().max()) == 0.0 and float(rd[:, n:].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("npm,n", [(11, 300), (8, 256), (14, 200)])
+def test_crt_single_product_matches_oracle_arithmetic(P, npm, n):
+    """One replayed step (pp = [1]: X1 = X0^2) isolates the kernel's product:
+    the same integer residue arithmetic in the oracle (crt_square) must agree
+    to FP64 rounding of the fraction sum, far below the 2^-a rounding of X."""
+    print(f"[int8crt one product] moduli={npm} n={n}", flush=True)
+    P.set_int8_moduli(npm)
+    h = O.model_hamiltonian(n, 7)
+    rho_g, _ = P.sp2_alg2_seq(h, [1], 1)
+    # the GPU's own X0 (an empty replay returns 2 X0), so that both sides
+    # round the same FP64 matrix to 2^-a
+    rho0, _ = P.sp2_alg2_seq(h, [1], 0)
+    x0 = 0.5 * rho0
+    assert np.max(np.abs(x0 - O.normalize(h))) < 1e-15
+    ref = 2.0 * O.crt_square(x0, npm)
+    assert np.max(np.abs(rho_g - ref)) < 1e-13
+    assert np.max(np.abs(rho_g - 2.0 * (x0 @ x0))) < 2.0 ** -(O.crt_constants(npm)[2] - 8)

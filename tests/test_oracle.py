@@ -469,3 +469,54 @@ def test_sp2_inverse_round_trip_and_entropy_quadrature():
         # cell agrees with adaptive quadrature to ~1e-6
         assert abs(gg[k] - (part - total * ee[k])) < 2e-6
     assert abs(gg[0]) < 1e-15 and abs(gg[-1]) < 1e-12
+
+
+@pytest.mark.parametrize("nmod", [8, 11, 14])
+def test_crt_square_reproduces_x_squared(nmod):
+    """The int8 Chinese-remainder product: for X with spectrum in [0, 1] the
+    reconstruction equals X^2 up to the rounding of X to 2^-a (a = 30, 42, 53),
+    i.e. |error| <= 2^-a (||X_i||_1 + ||X_j||_1) / 2 + FP64 noise."""
+    rng = np.random.default_rng(nmod)
+    n = 200
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    x = (q * rng.uniform(0.0, 1.0, n)) @ q.T
+    x = 0.5 * (x + x.T)
+    ps, qs, a, m = O.crt_constants(nmod)
+    assert 4 ** a <= m // 8 < 4 ** (a + 1)
+    got = O.crt_square(x, nmod)
+    ref = x @ x
+    l1 = np.abs(x).sum(axis=1)
+    bound = 2.0 ** -(a + 1) * (l1[:, None] + l1[None, :]) + 1e-13
+    assert np.all(np.abs(got - ref) <= bound)
+
+
+def test_crt_square_planted_mistakes_are_caught():
+    """A wrong CRT weight (q_k = 1) or a dropped modulus breaks the product."""
+    rng = np.random.default_rng(1)
+    n = 64
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    x = (q * rng.uniform(0.0, 1.0, n)) @ q.T
+    ref = x @ x
+    good = O.crt_square(x, 11)
+    assert np.max(np.abs(good - ref)) < 1e-10
+    orig = O.crt_constants
+
+    def wrong_q(nmod):
+        ps, qs, a, m = orig(nmod)
+        return ps, [1] * len(qs), a, m
+
+    O.crt_constants = wrong_q
+    try:
+        assert np.max(np.abs(O.crt_square(x, 11) - ref)) > 1e-3
+    finally:
+        O.crt_constants = orig
+
+    def dropped(nmod):
+        ps, qs, a, m = orig(nmod)
+        return ps[:-1], qs[:-1], a, m
+
+    O.crt_constants = dropped
+    try:
+        assert np.max(np.abs(O.crt_square(x, 11) - ref)) > 1e-3
+    finally:
+        O.crt_constants = orig
