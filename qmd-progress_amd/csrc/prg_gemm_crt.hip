@@ -28,13 +28,14 @@
 #include "prg_sp2.h"
 #include "prg_tile.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 namespace prg {
 namespace {
 
-using tile::sym_tile;
 using tile::xcd_remap;
 
 typedef int i4 __attribute__((ext_vector_type(4)));
@@ -64,6 +65,7 @@ struct CrtArgs {
   Sp2State* st;
   double* part;
   const double* fold;       // device copy of (p, 1/p, q) per modulus
+  const int2* tiles;        // (I, J) of each upper-triangle tile, grouped order
   CrtConst c;
 };
 
@@ -128,8 +130,8 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
   int8_t* Rn = const_cast<int8_t*>(g.planes[cur ^ 1]);
 
   const int t = xcd_remap(blockIdx.x, gridDim.x);
-  int I, J;
-  sym_tile(t, I, J);
+  const int2 tij = g.tiles[t];
+  const int I = tij.x, J = tij.y;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = tid >> 6, wm = w & 1, wn = w >> 1;
 
@@ -411,6 +413,40 @@ int set_crt_moduli(int np) {
 
 // Device copy of the per-modulus constants the product kernel indexes at run
 // time (refreshed when the number of moduli changes).
+// Upper-triangle tiles in groups of G x G tiles (group rows SI <= group columns
+// SJ, tiles column by column inside a group).  Each XCD receives a contiguous
+// range of this list, so the workgroups resident on one XCD at a time cover
+// a compact block of tiles: with G = 8 and 32 resident workgroups, half a
+// group, i.e. 8 A panels and 4 B panels shared through that XCD's L2 instead
+// of 32 A panels and one B panel for a run of one tile column.
+static const int2* tile_order(int nt, int* st) {
+  static int cached_nt = 0;
+  const int nb = nt * (nt + 1) / 2;
+  int2* d = (int2*)workspace("crt_tiles", sizeof(int2) * (size_t)nb, st);
+  if (*st || cached_nt == nt) return d;
+  constexpr int G = 8;
+  std::vector<int2> h;
+  h.reserve(nb);
+  const int ng = (nt + G - 1) / G;
+  for (int SJ = 0; SJ < ng; ++SJ)
+    for (int SI = 0; SI <= SJ; ++SI)
+      for (int J = SJ * G; J < std::min(nt, SJ * G + G); ++J)
+        for (int I = SI * G; I < std::min(nt, SI * G + G); ++I)
+          if (I <= J) h.push_back(int2{I, J});
+  // on the library stream, after the workspace's zero fill, and complete
+  // before the host vector goes away
+  if ((int)h.size() != nb ||
+      hipMemcpyAsync(d, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice, stream()) !=
+          hipSuccess ||
+      hipStreamSynchronize(stream()) != hipSuccess) {
+    set_error(ERR_HIP, "int8 CRT products: tile list upload failed");
+    *st = ERR_HIP;
+    return nullptr;
+  }
+  cached_nt = nt;
+  return d;
+}
+
 static double* fold_constants(int np, int* st) {
   static int cached = 0;
   double* d = (double*)workspace("crt_fold", sizeof(double) * 3 * kMaxModuli, st);
@@ -422,7 +458,8 @@ static double* fold_constants(int np, int* st) {
     h[3 * k + 1] = c.inv[k];
     h[3 * k + 2] = c.q[k];
   }
-  if (hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, stream()) != hipSuccess ||
+      hipStreamSynchronize(stream()) != hipSuccess) {
     set_error(ERR_HIP, "int8 CRT products: constant upload failed");
     *st = ERR_HIP;
     return nullptr;
@@ -469,6 +506,8 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   g.c = make_const(np);
   const int nt = ld / BM;
   const int nb = nt * (nt + 1) / 2;
+  g.tiles = tile_order(nt, &st);
+  if (st) return st;
   if (ntiles) *ntiles = nb;
 #define PRG_CALL(K) launch_step<K>(g, nb)
   PRG_CRT_DISPATCH(np, PRG_CALL)

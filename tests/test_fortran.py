@@ -54,8 +54,8 @@ def test_driver_matches_oracle(golden, tmp_path, method):
     r = subprocess.run([str(SP2RUN), str(golden / "hamiltonian_ortho_384.mtx"), str(out), method,
                         "0.5", "Abs", "1e-10"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    h = O.read_mtx(golden / "hamiltonian_ortho_384.mtx")
-    rho = O.read_mtx(out)
+    h = O.read_mtx(str(golden / "hamiltonian_ortho_384.mtx"))
+    rho = O.read_mtx(str(out))
     f = {"alg2": O.sp2_alg2, "alg1": O.sp2_alg1, "basic": O.sp2_basic}[method]
     ref = f(h, 0.5, 10, 200, "Abs", 1e-10)
     ref = ref[0] if isinstance(ref, tuple) else ref
