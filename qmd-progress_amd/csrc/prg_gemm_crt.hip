@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <vector>
 
@@ -109,15 +110,22 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
   return (int)m;
 }
 
-template <int NP>
-__global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
+// W = 4: 2 x 2 waves of 64 x 64 (one wave per SIMD); W = 8: 2 x 4 waves of
+// 64 x 32 (two waves per SIMD, so one wave's barrier and LDS waits overlap
+// the other's MFMAs).
+template <int NP, int W>
+__global__ __launch_bounds__(64 * W, 1) void sp2_crt_kernel(CrtArgs g) {
+  constexpr int WN = W / 2;             // waves along the tile columns
+  constexpr int MT = 4;                 // 16-row MFMA blocks per wave
+  constexpr int NTL = BM / WN / 16;     // 16-column MFMA blocks per wave
+  constexpr int CPW = 16 / W;           // 1 KB DMA chunks per wave per stage
   // Operand stages are copied global -> LDS directly (global_load_lds, 16
   // bytes per lane), NSTAGE - 1 stages ahead of the MFMAs.  A stage holds
-  // the A panel (128 rows x 64 bytes) then the B panel; within a row the
-  // 16-byte chunk g sits in slot g ^ ((row >> 1) & 3), which keeps the
-  // ds_read_b128 of 8 consecutive rows on distinct banks.
-  // The ring is the kernel's only LDS object, so it starts at LDS address 0
-  // and its 64 KB stay within the 16-bit range of the DMA base in M0.
+  // the A panel (128 rows x 64 bytes) then the B panel, as sixteen 1 KB
+  // chunks of 16 rows; within a row the 16-byte chunk g sits in slot
+  // g ^ ((row >> 1) & 3), which keeps the ds_read_b128 of 8 consecutive rows
+  // on distinct banks.  The ring is the kernel's only LDS object, so it
+  // starts at LDS address 0 and its 64 KB stay within 16-bit DMA bases.
   __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][STAGE_BYTES];
   const volatile Sp2State* stv = g.st;
   if (!stv->planned) return;
@@ -135,68 +143,68 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = tid >> 6, wm = w & 1, wn = w >> 1;
 
-  i4 acc[4][4];
-  d4 S[4][4];
+  i4 acc[MT][NTL];
+  d4 S[MT][NTL];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      S[m][n] = d4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int n = 0; n < NTL; ++n) S[m][n] = d4{0.0, 0.0, 0.0, 0.0};
 
   // Stage s covers modulus s / ksteps, k range (s % ksteps) * BK + [0, BK).
-  // Wave w fills LDS rows 32w .. 32w + 31 of both panels: two 1 KB chunks per
-  // panel, lane l taking row 16q + l/4 (of the chunk) and slot l % 4.
+  // Chunk c = w + W q: panel c / 8 (A, then B), rows 16 (c % 8) + l / 4 for
+  // lane l, slot l % 4.
   const int ksteps = ld / BK;
   const int total = NP * ksteps;
-  const int8_t* src_a[2];
-  const int8_t* src_b[2];
+  const int8_t* src[CPW];
+  uint32_t dsto[CPW];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int row = 32 * w + 16 * q + (lane >> 2);
+  for (int q = 0; q < CPW; ++q) {
+    const int c = w + W * q;
+    const int row = 16 * (c & 7) + (lane >> 2);
     const int slot = lane & 3;
     const int kc = 16 * (slot ^ ((row >> 1) & 3));
-    src_a[q] = R + (size_t)(I * BM + row) * ld + kc;
-    src_b[q] = R + (size_t)(J * BM + row) * ld + kc;
+    src[q] = R + (size_t)(((c < 8) ? I : J) * BM + row) * ld + kc;
+    dsto[q] = (uint32_t)(c * 1024);
   }
-  const uint32_t ring0 = lds_addr(&ring[0][0]) + 2048 * w;
+  const uint32_t ring0 = lds_addr(&ring[0][0]);
   auto issue = [&](int s) {
     const int mod = s / ksteps;
     const size_t off = (size_t)mod * plane + (size_t)(s - mod * ksteps) * BK;
-    const uint32_t dst = __builtin_amdgcn_readfirstlane(ring0 + STAGE_BYTES * (s % NSTAGE));
+    const uint32_t base = ring0 + STAGE_BYTES * (s % NSTAGE);
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      dma16(src_a[q] + off, dst + 1024 * q);
-      dma16(src_b[q] + off, dst + BM * BK + 1024 * q);
-    }
+    for (int q = 0; q < CPW; ++q)
+      dma16(src[q] + off, __builtin_amdgcn_readfirstlane(base + dsto[q]));
   };
   // LDS read offsets of this lane's fragments (row r, chunk g = lane / 16)
-  int offa[4], offb[4];
+  int offa[MT], offb[NTL];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
+  for (int m = 0; m < MT; ++m) {
     const int ra = wm * 64 + m * 16 + (lane & 15);
-    const int rb = wn * 64 + m * 16 + (lane & 15);
     offa[m] = ra * BK + 16 * ((lane >> 4) ^ ((ra >> 1) & 3));
-    offb[m] = BM * BK + rb * BK + 16 * ((lane >> 4) ^ ((rb >> 1) & 3));
+  }
+#pragma unroll
+  for (int n = 0; n < NTL; ++n) {
+    const int rb = wn * (BM / WN) + n * 16 + (lane & 15);
+    offb[n] = BM * BK + rb * BK + 16 * ((lane >> 4) ^ ((rb >> 1) & 3));
   }
   // Software pipeline: the ring holds NSTAGE stages in flight; fragments of
   // stage s + 1 are read from LDS while the MFMAs of stage s run.
-  i4 fa[8], fb[8];  // [0..3] A fragments, [4..7] B fragments
-  auto read_frags = [&](int s, i4 (&f)[8]) {
+  i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
+  auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
     const int8_t* st = ring[s % NSTAGE];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) f[m] = *(const i4*)(st + offa[m]);
+    for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + offa[m]);
 #pragma unroll
-    for (int n = 0; n < 4; ++n) f[4 + n] = *(const i4*)(st + offb[n]);
+    for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb[n]);
   };
   // Iteration s: wait until stage s + 1 has landed in every wave (and this
   // wave's reads of stage s are done), refill the slot of stage s with stage
   // s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.
   // The first step of a modulus starts from zero accumulators (peeled, so
   // that the loop carries only MFMA results and they stay in AGPRs).
-  auto step = [&](int s, i4 (&use)[8], i4 (&next)[8], bool first) {
+  auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
     if (s + NSTAGE - 1 < total)
-      wait_vm_barrier<4 * (NSTAGE - 2)>();
+      wait_vm_barrier<CPW * (NSTAGE - 2)>();
     else
       wait_vm_barrier<0>();
     if (s + NSTAGE < total) issue(s + NSTAGE);
@@ -204,17 +212,17 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
     read_frags(s + 1, next);
     const i4 zero = {0, 0, 0, 0};
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int n = 0; n < 4; ++n)
-        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[4 + n], use[m],
+      for (int n = 0; n < NTL; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
                                                           first ? zero : acc[m][n], 0, 0, 0);
   };
 #pragma unroll
   for (int s = 0; s < NSTAGE; ++s)
     if (s < total) issue(s);
   if (NSTAGE - 1 < total)
-    wait_vm_barrier<4 * (NSTAGE - 1)>();
+    wait_vm_barrier<CPW * (NSTAGE - 1)>();
   else
     wait_vm_barrier<0>();
   read_frags(0, fa);
@@ -230,9 +238,9 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
     // modulus done: S += ((acc * q) mod p) / p  (mod 1)
     const double p = g.fold[3 * mod], inv = g.fold[3 * mod + 1], qm = g.fold[3 * mod + 2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int n = 0; n < 4; ++n) {
+      for (int n = 0; n < NTL; ++n) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           // (acc q mod p) / p as a symmetric remainder in [-1/2, 1/2]; the
@@ -254,11 +262,11 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
   const double wz = mirror ? 2.0 : 1.0;
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
+  for (int m = 0; m < MT; ++m) {
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
+    for (int n = 0; n < NTL; ++n) {
       const int row = I * BM + wm * 64 + m * 16 + (lane & 15);
-      const int col0 = J * BM + wn * 64 + n * 16 + 4 * (lane >> 4);
+      const int col0 = J * BM + wn * (BM / WN) + n * 16 + 4 * (lane >> 4);
       double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
       double zz[4];
 #pragma unroll
@@ -321,7 +329,10 @@ __global__ __launch_bounds__(NT, 1) void sp2_crt_kernel(CrtArgs g) {
   __syncthreads();
   if (threadIdx.x < 4) {
     const int q = threadIdx.x;
-    g.part[4 * (size_t)t + q] = ((red[q] + red[4 + q]) + red[8 + q]) + red[12 + q];
+    double acc_q = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) acc_q += red[4 * ww + q];
+    g.part[4 * (size_t)t + q] = acc_q;
   }
 }
 
@@ -374,9 +385,22 @@ CrtConst make_const(int np) {
   return c;
 }
 
+int g_waves = 0;  // 4 or 8 waves per workgroup (0: read PRG_CRT_WAVES, default 4)
+
+int crt_waves() {
+  if (g_waves == 0) {
+    const char* e = std::getenv("PRG_CRT_WAVES");
+    g_waves = (e && std::atoi(e) == 8) ? 8 : 4;
+  }
+  return g_waves;
+}
+
 template <int NP>
 void launch_step(const CrtArgs& g, int nb) {
-  hipLaunchKernelGGL(sp2_crt_kernel<NP>, dim3(nb), dim3(NT), 0, stream(), g);
+  if (crt_waves() == 4)
+    hipLaunchKernelGGL((sp2_crt_kernel<NP, 4>), dim3(nb), dim3(256), 0, stream(), g);
+  else
+    hipLaunchKernelGGL((sp2_crt_kernel<NP, 8>), dim3(nb), dim3(512), 0, stream(), g);
 }
 
 template <int NP>
