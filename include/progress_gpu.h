@@ -167,6 +167,16 @@ void prg_sp2_entropy_function(double mu, double h1, double hN, int nsteps, const
 double prg_sp2_inverse(double f, double mu, double h1, double hN, int nsteps, const int *sgnlist);
 double prg_sp2_entropy_ts(bml_matrix_t *d0_bml, const double *GG, const double *ee);
 
+/* ------------------------------------------------------------------ congruence transforms */
+/* prg_orthogonalize (src/prg_nonortho_mod.F90:35-68): orthoA = Z^T A Z, and
+ * prg_deorthogonalize (:81-120): A = Z orthoA Z^T.  No C symbols in the
+ * reference; Fortran argument order (bml_type is accepted and unused: every
+ * handle here is dense). */
+void prg_orthogonalize(bml_matrix_t *A_bml, bml_matrix_t *zmat_bml, bml_matrix_t *orthoA_bml,
+                       double threshold, const char *bml_type, int verbose);
+void prg_deorthogonalize(bml_matrix_t *orthoA_bml, bml_matrix_t *zmat_bml, bml_matrix_t *a_bml,
+                         double threshold, const char *bml_type, int verbose);
+
 /* ------------------------------------------------------------------ homo-lumo / sequence */
 /* New C symbols for src/prg_homolumo_mod.F90:22-89 and :97-147 (no bind(C)
  * wrapper in the reference).  pp/vv hold the reference's pp(1..), vv(1..). */

@@ -1075,6 +1075,16 @@ def crt_square(x, nmod):
     return f * (float(m) / 4.0 ** a)
 
 
+def orthogonalize(a, z):
+    """prg_orthogonalize (src/prg_nonortho_mod.F90:35-68): Z^T A Z."""
+    return (z.T @ a) @ z
+
+
+def deorthogonalize(a_ortho, z):
+    """prg_deorthogonalize (src/prg_nonortho_mod.F90:81-120): Z A Z^T."""
+    return z @ (a_ortho @ z.T)
+
+
 def bfs_partition(g, nparts):
     """The build's replacement for METIS in prg_build_densityGP_T0
     (prg_metisPartition, src/prg_partition_mod.F90:216-311): recursive

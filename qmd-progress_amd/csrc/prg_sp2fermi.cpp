@@ -1,6 +1,7 @@
 // Truncated (finite-temperature) SP2: prg_sp2_fermi_init,
 // prg_sp2_fermi_init_norecs and prg_sp2_fermi (src/prg_sp2_fermi_mod.F90),
-// plus the host-side entropy helpers of the same module.
+// plus the host-side entropy helpers of the same module, and the congruence
+// transforms callers apply around the solvers (src/prg_nonortho_mod.F90).
 //
 // The X^2 products, the tangent X X1 + X1 X and the traces run on the GPU
 // through the FP64 MFMA product kernels (symmetric-tile products with fused
@@ -305,6 +306,68 @@ double prg_sp2_entropy_ts(bml_matrix_t* d0_bml, const double* GG, const double* 
       ts += ((hs - ee[j - 1]) * GG[j] + (ee[j] - hs) * GG[j - 1]) / (ee[j] - ee[j - 1]);
   }
   return ts;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Congruence transforms around the solvers (src/prg_nonortho_mod.F90): the
+// FP64 MFMA products of the GEMM family (transpose kernel + two products).
+namespace prg {
+namespace {
+int congruence(const DMat& A, const DMat& Z, DMat out, bool ortho) {
+  int st = ensure_device();
+  if (st) return st;
+  const int n = A.n;
+  DMat Zt, T;
+  if ((st = ops::temp("nonortho_zt", n, &Zt)) || (st = ops::temp("nonortho_t", n, &T))) return st;
+  if ((st = ops::transpose(Z, Zt))) return st;
+  if (ortho) {
+    // out = Z^T A Z  (src/prg_nonortho_mod.F90:58-64)
+    if ((st = ops::mm(Zt, A, T, false))) return st;
+    return ops::mm(T, Z, out, false);
+  }
+  // out = Z A Z^T  (src/prg_nonortho_mod.F90:101-115)
+  if ((st = ops::mm(A, Zt, T, false))) return st;
+  return ops::mm(Z, T, out, false);
+}
+}  // namespace
+}  // namespace prg
+
+extern "C" {
+
+// prg_orthogonalize (src/prg_nonortho_mod.F90:35-68): orthoA = Z^T A Z.  No C
+// symbol in the reference; arguments in the Fortran order (bml_type names the
+// storage of a newly allocated output and is not needed for the dense handle).
+void prg_orthogonalize(bml_matrix_t* A_bml, bml_matrix_t* zmat_bml, bml_matrix_t* orthoA_bml,
+                       double threshold, const char* bml_type, int verbose) {
+  (void)threshold;
+  (void)bml_type;
+  clear_error();
+  if (check_pair(A_bml, zmat_bml, "prg_orthogonalize") ||
+      check_pair(A_bml, orthoA_bml, "prg_orthogonalize"))
+    return;
+  if (verbose == 1) std::printf("In prg_orthogonalize ...\n");
+  DMat A, Z, O;
+  if (bml_dev_read(A_bml, &A) || bml_dev_read(zmat_bml, &Z) || bml_dev_write(orthoA_bml, &O))
+    return;
+  congruence(A, Z, O, true);
+}
+
+// prg_deorthogonalize (src/prg_nonortho_mod.F90:81-120): A = Z orthoA Z^T.
+void prg_deorthogonalize(bml_matrix_t* orthoA_bml, bml_matrix_t* zmat_bml, bml_matrix_t* a_bml,
+                         double threshold, const char* bml_type, int verbose) {
+  (void)threshold;
+  (void)bml_type;
+  clear_error();
+  if (check_pair(orthoA_bml, zmat_bml, "prg_deorthogonalize") ||
+      check_pair(orthoA_bml, a_bml, "prg_deorthogonalize"))
+    return;
+  if (verbose == 1) std::printf("In prg_deorthogonalize ...\n");
+  DMat O, Z, A;
+  if (bml_dev_read(orthoA_bml, &O) || bml_dev_read(zmat_bml, &Z) || bml_dev_write(a_bml, &A))
+    return;
+  congruence(O, Z, A, false);
 }
 
 }  // extern "C"

@@ -105,6 +105,8 @@ def lib():
         "prg_sp2_entropy_function": (None, [d, d, d, i, ip, dp, dp]),
         "prg_sp2_inverse": (d, [d, d, d, d, i, ip]),
         "prg_sp2_entropy_ts": (d, [vp, dp, dp]),
+        "prg_orthogonalize": (None, [vp, vp, vp, d, cp, i]),
+        "prg_deorthogonalize": (None, [vp, vp, vp, d, cp, i]),
         "prg_build_densityGP_T0": (None, [vp, vp, vp, d, d, d, i, i]),
         "prg_build_densityGP_T0_ef": (d, []),
         "prg_gpu_subgraph_timing": (None, [dp, dp, dp]),
@@ -691,6 +693,24 @@ def sp2_entropy_ts(d0, gg, ee):
                                         _dp(np.ascontiguousarray(ee, dtype=np.float64))))
     _check()
     return ts
+
+
+def orthogonalize(a, z):
+    """prg_orthogonalize: Z^T A Z."""
+    ma, mz = Matrix.from_numpy(a), Matrix.from_numpy(z)
+    mo = Matrix.zeros(a.shape[0])
+    lib().prg_orthogonalize(ma.h, mz.h, mo.h, 0.0, b"dense", 0)
+    _check()
+    return mo.numpy()
+
+
+def deorthogonalize(a_ortho, z):
+    """prg_deorthogonalize: Z A_ortho Z^T."""
+    mo, mz = Matrix.from_numpy(a_ortho), Matrix.from_numpy(z)
+    ma = Matrix.zeros(a_ortho.shape[0])
+    lib().prg_deorthogonalize(mo.h, mz.h, ma.h, 0.0, b"dense", 0)
+    _check()
+    return ma.numpy()
 
 
 def bfs_partition(g, nparts):

@@ -520,3 +520,19 @@ def test_crt_square_planted_mistakes_are_caught():
         assert np.max(np.abs(O.crt_square(x, 11) - ref)) > 1e-3
     finally:
         O.crt_constants = orig
+
+
+def test_orthogonalize_round_trip_with_lowdin_factor():
+    """Z = S^(-1/2) (Loewdin): Z^T S Z = I, and deorthogonalize inverts
+    orthogonalize for Z^T Z = S^-1 ... i.e. Z (Z^T A Z) Z^T = S^-1 A S^-1."""
+    rng = np.random.default_rng(3)
+    n = 50
+    b = rng.standard_normal((n, n)) * 0.05
+    s = np.eye(n) + 0.5 * (b + b.T)
+    w, q = np.linalg.eigh(s)
+    z = (q / np.sqrt(w)) @ q.T
+    assert np.allclose(O.orthogonalize(s, z), np.eye(n), atol=1e-12)
+    a = rng.standard_normal((n, n))
+    a = a + a.T
+    sinv = np.linalg.inv(s)
+    assert np.allclose(O.deorthogonalize(O.orthogonalize(a, z), z), sinv @ a @ sinv, atol=1e-10)

@@ -63,3 +63,14 @@ def test_sp2_entropy_ts_matches_oracle(P):
     gg, ee = O.sp2_entropy_function(0.1, lo, hi, sg)
     x, _ = O.sp2_fermi(h, 1, len(sg), 64.0, 0.1, 5.0, lo, hi, sg, 1e-4, 1e-12)
     assert abs(P.sp2_entropy_ts(x, gg, ee) - O.sp2_entropy_ts(x, gg, ee)) < 1e-9
+
+
+def test_orthogonalize_deorthogonalize_match_oracle(P):
+    print("[nonortho]", flush=True)
+    rng = np.random.default_rng(5)
+    n = 300
+    a = rng.standard_normal((n, n))
+    a = a + a.T
+    z = rng.standard_normal((n, n)) / np.sqrt(n)
+    assert np.linalg.norm(P.orthogonalize(a, z) - O.orthogonalize(a, z)) < 1e-11 * n
+    assert np.linalg.norm(P.deorthogonalize(a, z) - O.deorthogonalize(a, z)) < 1e-11 * n
