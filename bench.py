@@ -44,9 +44,11 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=4,
                     help="SP2 iterations timed for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="dense", choices=["dense", "graph"],
+    ap.add_argument("--workload", default="dense", choices=["dense", "graph", "fermi", "adjoint"],
                     help="dense: prg_sp2_alg2 at --n (the headline); graph: "
-                         "prg_subgraphSP2Loop on the water-box system")
+                         "prg_subgraphSP2Loop on the water-box system; fermi: "
+                         "prg_implicit_fermi (kT 0.05 eV, 8 levels) at --n; adjoint: "
+                         "forward + reverse SP2 sweep for dTr(D A)/dH at --n")
     ap.add_argument("--natoms", type=int, default=10000)
     ap.add_argument("--nparts", type=int, default=512)
     ap.add_argument("--graph-threshold", type=float, default=0.11,
@@ -99,6 +101,12 @@ def main():
 
     if args.workload == "graph":
         graph_bench(args, P, torch, np)
+        return
+    if args.workload == "fermi":
+        fermi_bench(args, P, torch, np)
+        return
+    if args.workload == "adjoint":
+        adjoint_bench(args, P, torch, np)
         return
     n = args.n
     dev = torch.device("cuda:0")
@@ -317,6 +325,100 @@ def graph_bench(args, P, torch, np):
         "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def _line(metric, value, unit, dt, args, dtype, config, extra, higher=True):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": higher,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+           "config": config}
+    out.update(extra)
+    print(json.dumps(out))
+
+
+def fermi_bench(args, P, torch, np):
+    """prg_implicit_fermi (recursive Fermi-operator expansion, CG / Newton-Schulz
+    inverses, FP64) on the model Hamiltonian at --n, beta = 20 eV^-1 (kT 0.05),
+    8 levels, k = 2, occupation N/2.  The library matrix handles stay resident
+    across the timed calls (H and P in HBM).  Accuracy: P against the Fermi
+    function of an FP64 eigendecomposition at the converged mu."""
+    import ctypes as C
+    n = args.n
+    h = model_hamiltonian_torch(n, args.seed, torch.device("cuda:0"))
+    hh = h.cpu().numpy()
+    mh = P.Matrix.from_numpy(hh)
+    mp = P.Matrix.zeros(n)
+    beta, nsteps, nocc = 20.0, 8, 0.5 * n
+    L = P.lib()
+    w = torch.linalg.eigvalsh(h)
+    mu = 0.5 * (w[n // 2 - 1] + w[n // 2]).item()
+
+    def solve(mu0, method):
+        L.prg_implicit_fermi(mh.h, mp.h, nsteps, 2, nocc, mu0, beta, method, 30, 1e-9, 0.0,
+                             1e-10)
+        P._check()
+        return L.prg_implicit_fermi_last_mu()
+
+    method = 1  # Newton-Schulz inverses (CG fallback)
+    for _ in range(args.warmup):
+        mu_w = solve(mu, method)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mu1 = solve(mu, method)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    pm = torch.from_numpy(mp.numpy()).cuda()
+    wv, q = torch.linalg.eigh(h)
+    f = 1.0 / (1.0 + torch.exp(beta * (wv - mu1)))
+    pref = (q * f) @ q.T
+    err = torch.linalg.norm(pm - pref).item()
+    tr = abs(torch.trace(pm).item() - nocc)
+    _line("implicit Fermi solves/s (prg_implicit_fermi, FP64 MFMA)", 1.0 / dt, "solves/s", dt,
+          args, "f64",
+          {"workload": f"prg_implicit_fermi model tight-binding H, N={n}, beta=20 (kT 0.05 eV), "
+                       f"8 levels, k=2, method 1 (Newton-Schulz), seed={args.seed}", "n": n,
+           "mu_start": mu, "mu": mu1},
+          {"accuracy": {"fro_err_vs_fermi_eigh": err, "trace_err": tr}, "cpu_baseline": None,
+           "roofline": None})
+
+
+def adjoint_bench(args, P, torch, np):
+    """Forward SP2 + reverse-mode adjoint of L = Tr(D A) on the model
+    Hamiltonian at --n (A banded, bandwidth 32, N(0,1) entries), checkpoints
+    every 8 steps, through the torch.autograd binding (FP64)."""
+    from progress_amd.autograd import sp2_density
+    n = args.n
+    dev = torch.device("cuda:0")
+    h = model_hamiltonian_torch(n, args.seed, dev)
+    g = torch.Generator(device="cpu").manual_seed(args.seed + 1)
+    a = torch.randn((n, n), generator=g, dtype=torch.float64)
+    idx = torch.arange(n)
+    a[(idx[:, None] - idx[None, :]).abs() > 32] = 0.0
+    a = (0.5 * (a + a.T)).to(dev)
+
+    def step():
+        hv = h.clone().requires_grad_(True)
+        d = sp2_density(hv, 0.5, 1e-10, 8)
+        loss = torch.sum(d * a)
+        loss.backward()
+        return loss.item(), hv.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lv, grad = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # symmetry of the gradient and the first-order response cross-check
+    sym = torch.linalg.norm(grad - grad.T).item() / torch.linalg.norm(grad).item()
+    _line("SP2 forward + adjoint solves/s (dTr(D A)/dH, FP64)", 1.0 / dt, "solves/s", dt, args,
+          "f64",
+          {"workload": f"sp2_density forward + backward, model H N={n}, A banded bw 32, "
+                       f"checkpoints every 8 steps, seed={args.seed}", "n": n},
+          {"loss": lv, "grad_asymmetry": sym, "cpu_baseline": None, "roofline": None})
 
 
 def executed_ops(ld, mode, moduli):

@@ -760,3 +760,192 @@ contains
          maxsp2iter, cstr(sp2conv), idemtol, threshold, verbose)
   end subroutine prg_compute_response_SP2
 end module prg_response_mod
+
+module prg_sp2_fermi_mod
+  use iso_c_binding
+  use bml
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_sp2_fermi_init, prg_sp2_fermi_init_norecs, prg_sp2_fermi, sp2_inverse
+  interface
+    subroutine c_init(h, ns, nocc, ts, thr, oel, tl, x, mu, beta, h1, hn, sg, v) &
+         bind(C, name="prg_sp2_fermi_init")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, x
+      integer(c_int), value :: ns, v
+      real(c_double), value :: nocc, ts, thr, oel, tl
+      real(c_double) :: mu, beta, h1, hn
+      integer(c_int) :: sg(*)
+    end subroutine
+    subroutine c_init_nr(h, ns, nocc, ts, thr, oel, tl, x, mu, beta, h1, hn, sg, v) &
+         bind(C, name="prg_sp2_fermi_init_norecs")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, x
+      integer(c_int) :: ns
+      integer(c_int), value :: v
+      real(c_double), value :: nocc, ts, thr, oel, tl
+      real(c_double) :: mu, beta, h1, hn
+      integer(c_int) :: sg(*)
+    end subroutine
+    subroutine c_fermi(h, os, ns, nocc, mu, beta, h1, hn, sg, thr, eps, tl, x) &
+         bind(C, name="prg_sp2_fermi")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, x
+      integer(c_int), value :: os, ns
+      real(c_double), value :: nocc, thr, eps, tl
+      real(c_double) :: mu, beta, h1, hn
+      integer(c_int) :: sg(*)
+    end subroutine
+    function c_inverse(f, mu, h1, hn, ns, sg) bind(C, name="prg_sp2_inverse") result(r)
+      import :: c_double, c_int
+      real(c_double), value :: f, mu, h1, hn
+      integer(c_int), value :: ns
+      integer(c_int) :: sg(*)
+      real(c_double) :: r
+    end function
+  end interface
+contains
+  !> src/prg_sp2_fermi_mod.F90:43-185
+  subroutine prg_sp2_fermi_init(h_bml, nsteps, nocc, tscale, threshold, occErrLimit, &
+       traceLimit, x_bml, mu, beta, h1, hN, sgnlist)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: x_bml
+    integer, intent(in) :: nsteps
+    integer, intent(inout) :: sgnlist(:)
+    real(dp), intent(in) :: nocc, tscale, threshold, occErrLimit, traceLimit
+    real(dp), intent(inout) :: mu, beta, h1, hN
+    call c_init(h_bml%ptr, nsteps, nocc, tscale, threshold, occErrLimit, traceLimit, &
+         x_bml%ptr, mu, beta, h1, hN, sgnlist, 0)
+  end subroutine prg_sp2_fermi_init
+  !> src/prg_sp2_fermi_mod.F90:203-387
+  subroutine prg_sp2_fermi_init_norecs(h_bml, nsteps, nocc, tscale, threshold, occErrLimit, &
+       traceLimit, x_bml, mu, beta, h1, hN, sgnlist, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: x_bml
+    integer, intent(inout) :: nsteps
+    integer, intent(inout) :: sgnlist(:)
+    real(dp), intent(in) :: nocc, tscale, threshold, occErrLimit, traceLimit
+    real(dp), intent(inout) :: mu, beta, h1, hN
+    integer, optional :: verbose
+    integer :: v
+    v = 0
+    if (present(verbose)) v = verbose
+    call c_init_nr(h_bml%ptr, nsteps, nocc, tscale, threshold, occErrLimit, traceLimit, &
+         x_bml%ptr, mu, beta, h1, hN, sgnlist, v)
+  end subroutine prg_sp2_fermi_init_norecs
+  !> src/prg_sp2_fermi_mod.F90:405-470
+  subroutine prg_sp2_fermi(h_bml, osteps, nsteps, nocc, mu, beta, h1, hN, sgnlist, &
+       threshold, eps, traceLimit, x_bml)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: x_bml
+    integer, intent(in) :: osteps, nsteps
+    integer, intent(in) :: sgnlist(:)
+    real(dp), intent(in) :: nocc, threshold, eps, traceLimit
+    real(dp), intent(inout) :: beta, h1, hN, mu
+    integer, allocatable :: sg(:)
+    sg = sgnlist
+    call c_fermi(h_bml%ptr, osteps, nsteps, nocc, mu, beta, h1, hN, sg, threshold, eps, &
+         traceLimit, x_bml%ptr)
+  end subroutine prg_sp2_fermi
+  !> src/prg_sp2_fermi_mod.F90:720-745
+  function sp2_inverse(f, mu, h1, hN, nsteps, sgnlist) result(ee)
+    real(dp), intent(in) :: f, mu, h1, hN
+    integer, intent(in) :: nsteps, sgnlist(:)
+    real(dp) :: ee
+    integer, allocatable :: sg(:)
+    sg = sgnlist
+    ee = c_inverse(f, mu, h1, hN, nsteps, sg)
+  end function sp2_inverse
+end module prg_sp2_fermi_mod
+
+module prg_nonortho_mod
+  use iso_c_binding
+  use bml
+  use prg_gpu_cstr
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_orthogonalize, prg_deorthogonalize
+  interface
+    subroutine c_ortho(a, z, o, thr, t, v) bind(C, name="prg_orthogonalize")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: a, z, o
+      real(c_double), value :: thr
+      character(kind=c_char) :: t(*)
+      integer(c_int), value :: v
+    end subroutine
+    subroutine c_deortho(o, z, a, thr, t, v) bind(C, name="prg_deorthogonalize")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: o, z, a
+      real(c_double), value :: thr
+      character(kind=c_char) :: t(*)
+      integer(c_int), value :: v
+    end subroutine
+  end interface
+contains
+  !> src/prg_nonortho_mod.F90:35-68
+  subroutine prg_orthogonalize(A_bml, zmat_bml, orthoA_bml, threshold, bml_type, verbose)
+    type(bml_matrix_t), intent(inout) :: A_bml, zmat_bml, orthoA_bml
+    real(dp), intent(in) :: threshold
+    character(len=*), intent(in) :: bml_type
+    integer, intent(in) :: verbose
+    if (.not. c_associated(orthoA_bml%ptr)) &
+         call bml_zero_matrix(bml_type, bml_element_real, dp, bml_get_n(A_bml), &
+         bml_get_n(A_bml), orthoA_bml)
+    call c_ortho(A_bml%ptr, zmat_bml%ptr, orthoA_bml%ptr, threshold, cstr(bml_type), verbose)
+  end subroutine prg_orthogonalize
+  !> src/prg_nonortho_mod.F90:81-120
+  subroutine prg_deorthogonalize(orthoA_bml, zmat_bml, a_bml, threshold, bml_type, verbose)
+    type(bml_matrix_t), intent(inout) :: a_bml
+    type(bml_matrix_t), intent(in) :: zmat_bml, orthoA_bml
+    real(dp) :: threshold
+    character(len=*) :: bml_type
+    integer :: verbose
+    if (.not. c_associated(a_bml%ptr)) &
+         call bml_zero_matrix(bml_type, bml_element_real, dp, bml_get_n(orthoA_bml), &
+         bml_get_n(orthoA_bml), a_bml)
+    call c_deortho(orthoA_bml%ptr, zmat_bml%ptr, a_bml%ptr, threshold, cstr(bml_type), verbose)
+  end subroutine prg_deorthogonalize
+end module prg_nonortho_mod
+
+module prg_graphsolver_mod
+  use iso_c_binding
+  use bml
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+  public :: prg_build_densityGP_T0
+  interface
+    subroutine c_gp_t0(h, g, r, thr, bndfil, ef, nparts, v) bind(C, name="prg_build_densityGP_T0")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, g, r
+      real(c_double), value :: thr, bndfil, ef
+      integer(c_int), value :: nparts, v
+    end subroutine
+    function c_gp_t0_ef() bind(C, name="prg_build_densityGP_T0_ef") result(r)
+      import :: c_double
+      real(c_double) :: r
+    end function
+  end interface
+contains
+  !> src/prg_graphsolver_mod.F90:39-191 (partition by the library's BFS
+  !> partitioner in place of METIS; Ef updated as in Fortran)
+  subroutine prg_build_densityGP_T0(ham_bml, g_bml, rho_bml, threshold, bndfil, Ef, nparts, &
+       verbose)
+    type(bml_matrix_t), intent(in) :: ham_bml, g_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    real(dp) :: bndfil, threshold
+    integer, intent(inout) :: nparts
+    real(dp), intent(inout) :: Ef
+    integer, optional, intent(in) :: verbose
+    integer :: v
+    v = 0
+    if (present(verbose)) v = verbose
+    if (.not. c_associated(rho_bml%ptr)) &
+         call bml_zero_matrix(bml_matrix_dense, bml_element_real, dp, bml_get_n(ham_bml), &
+         bml_get_n(ham_bml), rho_bml)
+    call c_gp_t0(ham_bml%ptr, g_bml%ptr, rho_bml%ptr, threshold, bndfil, Ef, nparts, v)
+    Ef = c_gp_t0_ef()
+  end subroutine prg_build_densityGP_T0
+end module prg_graphsolver_mod
