@@ -173,6 +173,8 @@ def main():
             ops = executed_ops(ld, name, args.moduli)
             r["product_kernel_mfma_ops"] = ops
             r["product_kernel_mfma_tops"] = ops / (kms * 1e-3) / 1e12
+            if name == "int8crt":
+                r["int8_waves"] = P.int8_waves()
         if evals is not None:
             rr = rho[:n, :n]
             err, tr, eb = errors_vs_diag(h, rr, evals, evecs, n)

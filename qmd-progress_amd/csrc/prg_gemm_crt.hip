@@ -29,6 +29,7 @@
 #include "prg_tile.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstdint>
@@ -385,22 +386,63 @@ CrtConst make_const(int np) {
   return c;
 }
 
-int g_waves = 0;  // 4 or 8 waves per workgroup (0: read PRG_CRT_WAVES, default 4)
+// Workgroup layout (4 or 8 waves).  PRG_CRT_WAVES=4 or 8 forces one;
+// otherwise the first product of each (ld, moduli) pair times both layouts
+// (each twice, the minimum counts) and keeps the faster; the product is then
+// computed once more with the chosen layout so that results do not depend on
+// the trial order.  Both layouts give bit-identical X^2; only the order of the
+// partial trace sums differs.
+int g_forced_waves = -1;  // -1: not read yet; 0: autotune; 4 / 8: forced
+std::vector<std::array<int, 3>> g_tuned;  // (ld, np, waves)
+int g_last_waves = 0;
 
-int crt_waves() {
-  if (g_waves == 0) {
+int forced_waves() {
+  if (g_forced_waves < 0) {
     const char* e = std::getenv("PRG_CRT_WAVES");
-    g_waves = (e && std::atoi(e) == 8) ? 8 : 4;
+    const int v = e ? std::atoi(e) : 0;
+    g_forced_waves = (v == 4 || v == 8) ? v : 0;
   }
-  return g_waves;
+  return g_forced_waves;
 }
 
 template <int NP>
-void launch_step(const CrtArgs& g, int nb) {
-  if (crt_waves() == 4)
+void launch_step_w(const CrtArgs& g, int nb, int waves) {
+  if (waves == 4)
     hipLaunchKernelGGL((sp2_crt_kernel<NP, 4>), dim3(nb), dim3(256), 0, stream(), g);
   else
     hipLaunchKernelGGL((sp2_crt_kernel<NP, 8>), dim3(nb), dim3(512), 0, stream(), g);
+}
+
+template <int NP>
+int launch_step(const CrtArgs& g, int nb, int ld) {
+  int waves = forced_waves();
+  if (!waves)
+    for (const auto& t : g_tuned)
+      if (t[0] == ld && t[1] == NP) waves = t[2];
+  if (!waves) {
+    hipEvent_t ev[2];
+    PRG_HIP_CHECK(hipEventCreate(&ev[0]));
+    PRG_HIP_CHECK(hipEventCreate(&ev[1]));
+    float best[2] = {1e30f, 1e30f};
+    const int cand[2] = {4, 8};
+    for (int rep = 0; rep < 2; ++rep)
+      for (int c = 0; c < 2; ++c) {
+        PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
+        launch_step_w<NP>(g, nb, cand[c]);
+        PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
+        PRG_HIP_CHECK(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        PRG_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        best[c] = std::min(best[c], ms);
+      }
+    PRG_HIP_CHECK(hipEventDestroy(ev[0]));
+    PRG_HIP_CHECK(hipEventDestroy(ev[1]));
+    waves = (best[1] < best[0]) ? 8 : 4;
+    g_tuned.push_back({ld, NP, waves});
+  }
+  g_last_waves = waves;
+  launch_step_w<NP>(g, nb, waves);
+  return OK;
 }
 
 template <int NP>
@@ -533,11 +575,15 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   g.tiles = tile_order(nt, &st);
   if (st) return st;
   if (ntiles) *ntiles = nb;
-#define PRG_CALL(K) launch_step<K>(g, nb)
+  st = OK;
+#define PRG_CALL(K) st = launch_step<K>(g, nb, ld)
   PRG_CRT_DISPATCH(np, PRG_CALL)
 #undef PRG_CALL
+  if (st) return st;
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
+
+int crt_last_waves() { return g_last_waves; }
 
 }  // namespace prg

@@ -322,6 +322,7 @@ int prg_set_int8_moduli(int count) {
   return set_crt_moduli(count);
 }
 int prg_get_int8_moduli(void) { return crt_moduli(); }
+int prg_gpu_int8_waves(void) { return crt_last_waves(); }
 
 void prg_sp2_basic(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
                    int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,

@@ -71,6 +71,7 @@ def lib():
         "prg_get_precision": (i, []),
         "prg_set_int8_moduli": (i, [i]),
         "prg_get_int8_moduli": (i, []),
+        "prg_gpu_int8_waves": (i, []),
         "prg_gershgorin": (None, [vp, dp]),
         "prg_normalize": (None, [vp]),
         "prg_normalize_fermi": (None, [vp, d, d, d]),
@@ -223,6 +224,11 @@ def set_int8_moduli(count: int):
 
 def get_int8_moduli() -> int:
     return int(lib().prg_get_int8_moduli())
+
+
+def int8_waves() -> int:
+    """Workgroup layout (4 or 8 waves) of the last int8 CRT product."""
+    return int(lib().prg_gpu_int8_waves())
 
 
 def _ivec(n, init=None):
