@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--graph-threshold", type=float, default=0.11,
                     help="|D| threshold of the density graph (graph workload)")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     return ap.parse_args()
 
 
@@ -110,6 +113,12 @@ def main():
         return
     n = args.n
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ld = P.padded_ld(n)
     h = model_hamiltonian_torch(n, args.seed, dev)
     hpad = torch.zeros((ld, ld), dtype=torch.float64, device=dev)

@@ -44,7 +44,6 @@ typedef int i4 __attribute__((ext_vector_type(4)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 128, BK = 64, NT = 256;
-constexpr int NSTAGE = 4;                  // LDS ring of operand stages
 constexpr int STAGE_BYTES = 2 * BM * BK;   // A and B panels of one stage
 constexpr int kMaxModuli = 14;
 
@@ -114,7 +113,7 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
 // W = 4: 2 x 2 waves of 64 x 64 (one wave per SIMD); W = 8: 2 x 4 waves of
 // 64 x 32 (two waves per SIMD, so one wave's barrier and LDS waits overlap
 // the other's MFMAs).
-template <int NP, int W>
+template <int NP, int W, int NSTAGE>
 __global__ __launch_bounds__(64 * W, 1) void sp2_crt_kernel(CrtArgs g) {
   constexpr int WN = W / 2;             // waves along the tile columns
   constexpr int MT = 4;                 // 16-row MFMA blocks per wave
@@ -125,8 +124,8 @@ __global__ __launch_bounds__(64 * W, 1) void sp2_crt_kernel(CrtArgs g) {
   // the A panel (128 rows x 64 bytes) then the B panel, as sixteen 1 KB
   // chunks of 16 rows; within a row the 16-byte chunk g sits in slot
   // g ^ ((row >> 1) & 3), which keeps the ds_read_b128 of 8 consecutive rows
-  // on distinct banks.  The ring is the kernel's only LDS object, so it
-  // starts at LDS address 0 and its 64 KB stay within 16-bit DMA bases.
+  // on distinct banks.  The ring (NSTAGE x 16 KB) is the kernel's only LDS
+  // object, so it starts at LDS address 0.
   __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][STAGE_BYTES];
   const volatile Sp2State* stv = g.st;
   if (!stv->planned) return;
@@ -386,47 +385,61 @@ CrtConst make_const(int np) {
   return c;
 }
 
-// Workgroup layout (4 or 8 waves).  PRG_CRT_WAVES=4 or 8 forces one;
-// otherwise the first product of each (ld, moduli) pair times both layouts
-// (each twice, the minimum counts) and keeps the faster; the product is then
-// computed once more with the chosen layout so that results do not depend on
-// the trial order.  Both layouts give bit-identical X^2; only the order of the
-// partial trace sums differs.
-int g_forced_waves = -1;  // -1: not read yet; 0: autotune; 4 / 8: forced
-std::vector<std::array<int, 3>> g_tuned;  // (ld, np, waves)
-int g_last_waves = 0;
+// Workgroup layout: 4 or 8 waves, and a ring of 4 or 8 operand stages
+// (64 or 128 KB of LDS), coded waves * 100 + stages.  PRG_CRT_WAVES=4|8 and
+// PRG_CRT_STAGES=4|8 force a layout; otherwise the first product of each
+// (ld, moduli) pair times the four layouts (each twice, the minimum counts)
+// and keeps the fastest; the product is then computed once more with the
+// chosen layout so that results do not depend on the trial order.  All
+// layouts give bit-identical X^2; only the order of the partial trace sums
+// differs.
+int g_forced = -1;                        // -1: not read yet; 0: autotune
+std::vector<std::array<int, 3>> g_tuned;  // (ld, np, layout)
+int g_last_layout = 0;
 
-int forced_waves() {
-  if (g_forced_waves < 0) {
-    const char* e = std::getenv("PRG_CRT_WAVES");
-    const int v = e ? std::atoi(e) : 0;
-    g_forced_waves = (v == 4 || v == 8) ? v : 0;
+int forced_layout() {
+  if (g_forced < 0) {
+    const char* ew = std::getenv("PRG_CRT_WAVES");
+    const char* es = std::getenv("PRG_CRT_STAGES");
+    const int w = ew ? std::atoi(ew) : 0, ns = es ? std::atoi(es) : 0;
+    const bool fw = (w == 4 || w == 8), fs = (ns == 4 || ns == 8);
+    g_forced = (fw || fs) ? (fw ? w : 8) * 100 + (fs ? ns : 4) : 0;
   }
-  return g_forced_waves;
+  return g_forced;
 }
 
 template <int NP>
-void launch_step_w(const CrtArgs& g, int nb, int waves) {
-  if (waves == 4)
-    hipLaunchKernelGGL((sp2_crt_kernel<NP, 4>), dim3(nb), dim3(256), 0, stream(), g);
-  else
-    hipLaunchKernelGGL((sp2_crt_kernel<NP, 8>), dim3(nb), dim3(512), 0, stream(), g);
+void launch_step_w(const CrtArgs& g, int nb, int layout) {
+  switch (layout) {
+    case 404:
+      hipLaunchKernelGGL((sp2_crt_kernel<NP, 4, 4>), dim3(nb), dim3(256), 0, stream(), g);
+      break;
+    case 408:
+      hipLaunchKernelGGL((sp2_crt_kernel<NP, 4, 8>), dim3(nb), dim3(256), 0, stream(), g);
+      break;
+    case 804:
+      hipLaunchKernelGGL((sp2_crt_kernel<NP, 8, 4>), dim3(nb), dim3(512), 0, stream(), g);
+      break;
+    default:
+      hipLaunchKernelGGL((sp2_crt_kernel<NP, 8, 8>), dim3(nb), dim3(512), 0, stream(), g);
+      break;
+  }
 }
 
 template <int NP>
 int launch_step(const CrtArgs& g, int nb, int ld) {
-  int waves = forced_waves();
-  if (!waves)
+  int layout = forced_layout();
+  if (!layout)
     for (const auto& t : g_tuned)
-      if (t[0] == ld && t[1] == NP) waves = t[2];
-  if (!waves) {
+      if (t[0] == ld && t[1] == NP) layout = t[2];
+  if (!layout) {
     hipEvent_t ev[2];
     PRG_HIP_CHECK(hipEventCreate(&ev[0]));
     PRG_HIP_CHECK(hipEventCreate(&ev[1]));
-    float best[2] = {1e30f, 1e30f};
-    const int cand[2] = {4, 8};
+    constexpr int cand[4] = {404, 408, 804, 808};
+    float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
     for (int rep = 0; rep < 2; ++rep)
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < 4; ++c) {
         PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
         launch_step_w<NP>(g, nb, cand[c]);
         PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
@@ -437,11 +450,14 @@ int launch_step(const CrtArgs& g, int nb, int ld) {
       }
     PRG_HIP_CHECK(hipEventDestroy(ev[0]));
     PRG_HIP_CHECK(hipEventDestroy(ev[1]));
-    waves = (best[1] < best[0]) ? 8 : 4;
-    g_tuned.push_back({ld, NP, waves});
+    int bi = 0;
+    for (int c = 1; c < 4; ++c)
+      if (best[c] < best[bi]) bi = c;
+    layout = cand[bi];
+    g_tuned.push_back({ld, NP, layout});
   }
-  g_last_waves = waves;
-  launch_step_w<NP>(g, nb, waves);
+  g_last_layout = layout;
+  launch_step_w<NP>(g, nb, layout);
   return OK;
 }
 
@@ -584,6 +600,6 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   return OK;
 }
 
-int crt_last_waves() { return g_last_waves; }
+int crt_last_waves() { return g_last_layout; }
 
 }  // namespace prg
