@@ -322,10 +322,11 @@ int prg_get_precision(void);
  * 12 -> 45, 13 -> 49, 14 -> 53 bits.  Returns a status. */
 int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
-/* Workgroup layout (4 or 8 waves) the int8 CRT product kernel last ran with:
+/* Configuration the int8 CRT product kernel last ran with, coded
+ * 800 + ring stages (8 waves per workgroup, a 3- or 4-stage operand ring):
  * chosen per (padded size, moduli) by timing both on the first product, or
- * forced with the environment variable PRG_CRT_WAVES=4|8.  0 before any int8
- * product.  Measurement support; no reference counterpart. */
+ * forced with the environment variable PRG_CRT_STAGES=3|4.  0 before any
+ * int8 product.  Measurement support; no reference counterpart. */
 int prg_gpu_int8_waves(void);
 
 /* ------------------------------------------------------------------ device-pointer API */

@@ -17,14 +17,21 @@
 // entry) and of the FP64 fraction sum (each term a symmetric remainder in
 // [-1/2, 1/2], the sum reduced mod 1 once at the end).
 //
-// The fused kernel computes the upper-triangle 128 x 128 tiles of X^2, one
-// modulus after another over the full K range, applies the SP2 branch update
-// in FP64 against the master copy, writes the next master and its residue
-// planes (both triangle positions), and emits the same trace / Frobenius
-// partial sums as the FP64 kernel, so the FP64 bookkeeping is shared.
-// Because X and its residue planes are symmetric, the A operand (rows of the
-// I panel) and the B operand (columns of the J panel) are both contiguous
-// 128-byte chunks of plane rows.
+// Two kernels per SP2 step:
+//  * crt_product_kernel: one workgroup per (256 x 256 upper-triangle tile,
+//    modulus) item computes that modulus's int8 product over the full K range
+//    and stores ((C mod p) q mod p) as symmetric int8 values (64 KB per item);
+//  * crt_update_kernel: one workgroup per upper-triangle 64 x 64 block sums
+//    the fractions over the moduli in FP64, applies the SP2 branch update
+//    against the FP64 master X, writes the next master and its residue planes
+//    (both triangle positions) and emits the same trace / Frobenius partial
+//    sums as the FP64 kernel, so the FP64 bookkeeping is shared.
+// Items are independent, so the product grid is tiles x moduli (5808
+// workgroups at N = 8192) and the 256 x 256 tiles halve the operand bytes per
+// MFMA of a 128 x 128 tile.  Residue planes are stored in 64-column blocks
+// (element (r, c) at (c / 64) * pld * 64 + r * 64 + c % 64, pld = ld rounded
+// up to 256), so a 256-row x 64-byte operand stage is one contiguous 16 KB
+// panel and a 64 x 64 block of a plane is one contiguous 4 KB.
 #include "prg_sp2.h"
 #include "prg_tile.h"
 
@@ -35,16 +42,23 @@
 #include <cstdint>
 #include <vector>
 
+
 namespace prg {
 namespace {
 
 using tile::xcd_remap;
 
 typedef int i4 __attribute__((ext_vector_type(4)));
-typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 128, BK = 64, NT = 256;
-constexpr int STAGE_BYTES = 2 * BM * BK;   // A and B panels of one stage
+constexpr int BK = 64;                 // K bytes per operand stage
+constexpr int TM = 256;                // product tile edge
+constexpr int SB = 64;                 // update block edge (and plane column block)
+constexpr int NT = 256;
+constexpr int kPanel = TM * BK;        // one operand panel of a stage (16 KB)
+constexpr int kStage = 2 * kPanel;     // A and B panels (32 KB)
+constexpr int kItemBytes = TM * TM;    // product residues of one (tile, modulus) item
+constexpr int kWaveDwords = 32 * 64;   // one product wave's share of an item (8 KB)
+constexpr int kPitch = SB + 4;         // LDS row pitch of the update kernel's residues
 constexpr int kMaxModuli = 14;
 
 // Pairwise coprime moduli <= 256, largest first.
@@ -59,14 +73,22 @@ struct CrtConst {
   double cscale;           // M / 2^(2a)
 };
 
-struct CrtArgs {
-  const double* buf[2];     // FP64 master ping-pong
-  const int8_t* planes[2];  // residue planes ping-pong, plane k at k * ld * ld
-  int ld;
-  Sp2State* st;
+struct ProductArgs {
+  const int8_t* planes[2];  // residue planes ping-pong, plane k at k * pld * pld
+  int8_t* res;              // product residues, item (t, k) at (t * np + k) * kItemBytes
+  const int4* items;        // (modulus, I, J, upper-triangle index of (I, J))
+  const Sp2State* st;
+  const double* fold;       // (p, 1/p, q) per modulus
+  int pld, np;
+};
+
+struct UpdateArgs {
+  const double* buf[2];     // FP64 master ping-pong (ld x ld, column-major)
+  int8_t* planes[2];
+  const int8_t* res;
+  const Sp2State* st;
   double* part;
-  const double* fold;       // device copy of (p, 1/p, q) per modulus
-  const int2* tiles;        // (I, J) of each upper-triangle tile, grouped order
+  int ld, pld;
   CrtConst c;
 };
 
@@ -110,104 +132,72 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
   return (int)m;
 }
 
-// W = 4: 2 x 2 waves of 64 x 64 (one wave per SIMD); W = 8: 2 x 4 waves of
-// 64 x 32 (two waves per SIMD, so one wave's barrier and LDS waits overlap
-// the other's MFMAs).
-template <int NP, int W, int NSTAGE>
-__global__ __launch_bounds__(64 * W, 1) void sp2_crt_kernel(CrtArgs g) {
-  constexpr int WN = W / 2;             // waves along the tile columns
-  constexpr int MT = 4;                 // 16-row MFMA blocks per wave
-  constexpr int NTL = BM / WN / 16;     // 16-column MFMA blocks per wave
-  constexpr int CPW = 16 / W;           // 1 KB DMA chunks per wave per stage
-  // Operand stages are copied global -> LDS directly (global_load_lds, 16
-  // bytes per lane), NSTAGE - 1 stages ahead of the MFMAs.  A stage holds
-  // the A panel (128 rows x 64 bytes) then the B panel, as sixteen 1 KB
-  // chunks of 16 rows; within a row the 16-byte chunk g sits in slot
-  // g ^ ((row >> 1) & 3), which keeps the ds_read_b128 of 8 consecutive rows
-  // on distinct banks.  The ring (NSTAGE x 16 KB) is the kernel's only LDS
-  // object, so it starts at LDS address 0.
-  __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][STAGE_BYTES];
+// One (tile, modulus) item: C = A_I A_J^T mod p over the full K range, 8 waves
+// in a 2 x 4 grid of 128 x 64 (8 x 4 MFMA blocks of 16 x 16 per wave,
+// accumulators in AGPRs).  Operand stages are copied global -> LDS directly
+// (global_load_lds, 16 bytes per lane), NSTAGE - 1 stages ahead of the MFMAs.
+// A stage holds the A panel (256 rows x 64 bytes) then the B panel, as
+// thirty-two 1 KB chunks of 16 rows; within a row the 16-byte chunk g sits in
+// slot g ^ ((row >> 1) & 3), which keeps the ds_read_b128 of 8 consecutive
+// rows on distinct banks.  The ring is the kernel's only LDS object.
+template <int NSTAGE>
+__global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
+  constexpr int MT = 8, NTL = 4;
+  __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
   const volatile Sp2State* stv = g.st;
   if (!stv->planned) return;
-  const int apply = stv->apply, sigma = stv->sigma, cur = stv->cur;
-  const int ld = g.ld;
-  const size_t plane = (size_t)ld * ld;
-  const double* X = g.buf[cur];
-  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
-  const int8_t* R = g.planes[cur];
-  int8_t* Rn = const_cast<int8_t*>(g.planes[cur ^ 1]);
+  const int cur = stv->cur;
+  const int pld = g.pld;
+  const int4 it = g.items[xcd_remap(blockIdx.x, gridDim.x)];
+  const int k = it.x, I = it.y, J = it.z, t = it.w;
+  const int8_t* R = g.planes[cur] + (size_t)k * pld * pld;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
 
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 tij = g.tiles[t];
-  const int I = tij.x, J = tij.y;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
-
-  i4 acc[MT][NTL];
-  d4 S[MT][NTL];
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int n = 0; n < NTL; ++n) S[m][n] = d4{0.0, 0.0, 0.0, 0.0};
-
-  // Stage s covers modulus s / ksteps, k range (s % ksteps) * BK + [0, BK).
-  // Chunk c = w + W q: panel c / 8 (A, then B), rows 16 (c % 8) + l / 4 for
-  // lane l, slot l % 4.
-  const int ksteps = ld / BK;
-  const int total = NP * ksteps;
-  const int8_t* src[CPW];
-  uint32_t dsto[CPW];
-#pragma unroll
-  for (int q = 0; q < CPW; ++q) {
-    const int c = w + W * q;
-    const int row = 16 * (c & 7) + (lane >> 2);
-    const int slot = lane & 3;
-    const int kc = 16 * (slot ^ ((row >> 1) & 3));
-    src[q] = R + (size_t)(((c < 8) ? I : J) * BM + row) * ld + kc;
-    dsto[q] = (uint32_t)(c * 1024);
-  }
-  const uint32_t ring0 = lds_addr(&ring[0][0]);
+  // DMA chunks of wave w: c = w + 8 q (q < 4), i.e. panel rows 16 w + lane / 4
+  // (+ 128 for odd q) of A (q < 2) or B (q >= 2), slot lane % 4.  Adding 128
+  // rows leaves the swizzle unchanged.
+  const int drow = 16 * w + (lane >> 2);
+  const int kc = 16 * ((lane & 3) ^ ((drow >> 1) & 3));
+  const int8_t* srcA = R + (size_t)(I * TM + drow) * BK + kc;
+  const int8_t* srcB = R + (size_t)(J * TM + drow) * BK + kc;
+  const size_t kstride = (size_t)pld * BK;  // one 64-column block of a plane
+  const uint32_t dst0 = lds_addr(&ring[0][0]) + 1024 * w;
+  const int ksteps = pld / BK;
   auto issue = [&](int s) {
-    const int mod = s / ksteps;
-    const size_t off = (size_t)mod * plane + (size_t)(s - mod * ksteps) * BK;
-    const uint32_t base = ring0 + STAGE_BYTES * (s % NSTAGE);
-#pragma unroll
-    for (int q = 0; q < CPW; ++q)
-      dma16(src[q] + off, __builtin_amdgcn_readfirstlane(base + dsto[q]));
+    const size_t off = (size_t)s * kstride;
+    const uint32_t base = __builtin_amdgcn_readfirstlane(dst0 + kStage * (s % NSTAGE));
+    dma16(srcA + off, base);
+    dma16(srcA + off + 128 * BK, base + 8192);
+    dma16(srcB + off, base + kPanel);
+    dma16(srcB + off + 128 * BK, base + kPanel + 8192);
   };
-  // LDS read offsets of this lane's fragments (row r, chunk g = lane / 16)
-  int offa[MT], offb[NTL];
-#pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const int ra = wm * 64 + m * 16 + (lane & 15);
-    offa[m] = ra * BK + 16 * ((lane >> 4) ^ ((ra >> 1) & 3));
-  }
-#pragma unroll
-  for (int n = 0; n < NTL; ++n) {
-    const int rb = wn * (BM / WN) + n * 16 + (lane & 15);
-    offb[n] = BM * BK + rb * BK + 16 * ((lane >> 4) ^ ((rb >> 1) & 3));
-  }
-  // Software pipeline: the ring holds NSTAGE stages in flight; fragments of
-  // stage s + 1 are read from LDS while the MFMAs of stage s run.
+  // LDS read offsets of this lane's fragments: row r = lane % 16 of each
+  // 16-row block, 16-byte chunk lane / 16; the block offsets are multiples
+  // of 8 rows, so the swizzle depends on the lane only.
+  const int fo = (lane & 15) * BK + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
+  const int offa = wm * 128 * BK + fo;
+  const int offb = kPanel + wn * 64 * BK + fo;
+  i4 acc[MT][NTL];
   i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
   auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
     const int8_t* st = ring[s % NSTAGE];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + offa[m]);
+    for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + offa + 1024 * m);
 #pragma unroll
-    for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb[n]);
+    for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb + 1024 * n);
   };
   // Iteration s: wait until stage s + 1 has landed in every wave (and this
   // wave's reads of stage s are done), refill the slot of stage s with stage
-  // s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.
-  // The first step of a modulus starts from zero accumulators (peeled, so
-  // that the loop carries only MFMA results and they stay in AGPRs).
+  // s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.  The
+  // first step starts from zero accumulators (peeled, so that the loop
+  // carries only MFMA results and they stay in AGPRs).
   auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
-    if (s + NSTAGE - 1 < total)
-      wait_vm_barrier<CPW * (NSTAGE - 2)>();
+    if (s + NSTAGE - 1 < ksteps)
+      wait_vm_barrier<4 * (NSTAGE - 2)>();
     else
       wait_vm_barrier<0>();
-    if (s + NSTAGE < total) issue(s + NSTAGE);
+    if (s + NSTAGE < ksteps) issue(s + NSTAGE);
     // after the last stage this reads a stale slot that is never used
     read_frags(s + 1, next);
     const i4 zero = {0, 0, 0, 0};
@@ -220,99 +210,158 @@ __global__ __launch_bounds__(64 * W, 1) void sp2_crt_kernel(CrtArgs g) {
   };
 #pragma unroll
   for (int s = 0; s < NSTAGE; ++s)
-    if (s < total) issue(s);
-  if (NSTAGE - 1 < total)
-    wait_vm_barrier<CPW * (NSTAGE - 1)>();
+    if (s < ksteps) issue(s);
+  if (NSTAGE - 1 < ksteps)
+    wait_vm_barrier<4 * (NSTAGE - 1)>();
   else
     wait_vm_barrier<0>();
   read_frags(0, fa);
-  // ksteps = ld / 64 is even, so every modulus starts on fa
-  for (int mod = 0; mod < NP; ++mod) {
-    const int s0 = mod * ksteps;
-    step(s0, fa, fb, true);
-    step(s0 + 1, fb, fa, false);
-    for (int ks = 2; ks < ksteps; ks += 2) {
-      step(s0 + ks, fa, fb, false);
-      step(s0 + ks + 1, fb, fa, false);
-    }
-    // modulus done: S += ((acc * q) mod p) / p  (mod 1)
-    const double p = g.fold[3 * mod], inv = g.fold[3 * mod + 1], qm = g.fold[3 * mod + 2];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int n = 0; n < NTL; ++n) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          // (acc q mod p) / p as a symmetric remainder in [-1/2, 1/2]; the
-          // sum over the moduli stays below NP / 2 in magnitude, so it is
-          // reduced mod 1 only at the end (a quotient rounded to the
-          // neighbouring integer shifts the term by exactly 1: harmless)
-          const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^39
-          const double md = fma(-p, rint(v * inv), v);
-          S[m][n][r] = fma(md, inv, S[m][n][r]);
-        }
-      }
+  // ksteps = pld / 64 is a multiple of 4
+  step(0, fa, fb, true);
+  step(1, fb, fa, false);
+  for (int s = 2; s < ksteps; s += 2) {
+    step(s, fa, fb, false);
+    step(s + 1, fb, fa, false);
   }
 
-  // Epilogue.  Operands were swapped, so this lane holds (X^2)[row][col]
-  // with row = lane%16 and col = 4*(lane/16) + r within each 16 x 16 block.
+  // (C q mod p) as symmetric int8.  Operands were swapped, so this lane holds
+  // C[row][col] with row = lane % 16 and col = 4 (lane / 16) + r inside each
+  // 16 x 16 block; the four bytes of r go to one dword, stored in fragment
+  // order (wave, block m, block n, lane): 256 contiguous bytes per block.
+  const double p = g.fold[3 * k], inv = g.fold[3 * k + 1], qm = g.fold[3 * k + 2];
+  int* dst = (int*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + w * kWaveDwords + lane;
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int n = 0; n < NTL; ++n) {
+      int b4 = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^43
+        b4 |= (sym_residue(v, p, inv) & 0xff) << (8 * r);
+      }
+      dst[(m * NTL + n) * 64] = b4;
+    }
+}
+
+// One upper-triangle 64 x 64 block (I, J) of X^2: sum the moduli's fractions,
+// apply the SP2 update, write X_next at (I, J) and (J, I), and the residue
+// planes of X_next (through LDS, as contiguous 4 KB blocks in both
+// positions).  The block lies in product tile (I / 4, J / 4), in the share of
+// product wave (I % 4) / 2 + 2 (J % 4), MFMA rows 4 (I % 2) .. + 3.
+template <int NP>
+__global__ __launch_bounds__(NT) void crt_update_kernel(UpdateArgs g) {
+  __shared__ __attribute__((aligned(16))) int8_t rb[NP][SB][kPitch];
+  __shared__ double red[NT / 64][4];
+  const volatile Sp2State* stv = g.st;
+  if (!stv->planned) return;
+  const int apply = stv->apply, sigma = stv->sigma, cur = stv->cur;
+  const int ld = g.ld, pld = g.pld;
+  int I, J;
+  tile::sym_tile(blockIdx.x, I, J);
+  const int bi = I >> 2, bj = J >> 2;
+  const int t = bj * (bj + 1) / 2 + bi;
+  const int wave = ((I & 3) >> 1) + 2 * (J & 3);
+  const int tid = threadIdx.x, lane = tid & 63, b = tid >> 6;
+  const int m = 4 * (I & 1) + b;
+  const int* src = (const int*)(g.res + (size_t)t * NP * kItemBytes) + wave * kWaveDwords +
+                   m * 4 * 64 + lane;
+
+  // S = sum_k ((C q_k) mod p_k) / p_k for this lane's 16 elements: blocks
+  // n = 0..3, bytes r = 0..3 (element (row, 16 n + 4 (lane / 16) + r))
+  double S[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) S[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    int d[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) d[n] = src[k * (kItemBytes / 4) + 64 * n];
+    const double inv = g.c.inv[k];
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        S[4 * n + r] = fma((double)(int8_t)(d[n] >> (8 * r)), inv, S[4 * n + r]);
+  }
+
   const bool mirror = (I != J);
   const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
   const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
-  const double wz = mirror ? 2.0 : 1.0;
-  double sums[4] = {0.0, 0.0, 0.0, 0.0};
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
+  const int rl = 16 * b + (lane & 15);
+  const int row = SB * I + rl;
+  double zz[16];
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
+  for (int n = 0; n < 4; ++n) {
+    const int col0 = SB * J + 16 * n + 4 * (lane >> 4);
+    double z4[4];
 #pragma unroll
-    for (int n = 0; n < NTL; ++n) {
-      const int row = I * BM + wm * 64 + m * 16 + (lane & 15);
-      const int col0 = J * BM + wn * (BM / WN) + n * 16 + 4 * (lane >> 4);
-      double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
-      double zz[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int col = col0 + r;
-        const size_t idx = row + (size_t)col * ld;
-        const size_t tdx = col + (size_t)row * ld;
-        const double f = S[m][n][r] - rint(S[m][n][r]);  // C / M mod 1 in [-1/2, 1/2]
-        const double y = f * g.c.cscale;
-        const double x = X[idx];
-        const double z = fma(cb, x, ca * y);
-        const double dxy = x - y;
-        const bool diag = (row == col);
-        q0 += z * z;
-        q2 += dxy * dxy;
-        q1 += diag ? z : 0.0;
-        q3 += diag ? y : 0.0;
-        Xn[idx] = z;
-        Xn[tdx] = z;
-        zz[r] = rint(z * g.c.scale);
-      }
-      // residues: 4 consecutive bytes at the transposed position (one
-      // 32-bit store) and single bytes at the direct one (on diagonal tiles
-      // both land on entries of the same tile with equal values)
-      const size_t tdx0 = col0 + (size_t)row * ld;
-      int8_t* rd = Rn + row + (size_t)col0 * ld;
-#pragma unroll 1
-      for (int k = 0; k < NP; ++k) {
-        const double p = g.fold[3 * k], inv = g.fold[3 * k + 1];
-        int b4 = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int v = sym_residue(zz[r], p, inv);
-          b4 |= (v & 0xff) << (8 * r);
-          rd[(size_t)k * plane + (size_t)r * ld] = (int8_t)v;
-        }
-        *(int*)(Rn + (size_t)k * plane + tdx0) = b4;
-      }
-      sums[0] += wz * q0;
-      sums[1] += q1;
-      sums[2] += wz * q2;
-      sums[3] += q3;
-      __builtin_amdgcn_sched_barrier(0);
+    for (int r = 0; r < 4; ++r) {
+      const int col = col0 + r;
+      const size_t idx = row + (size_t)col * ld;
+      const double sr = S[4 * n + r];
+      const double y = (sr - rint(sr)) * g.c.cscale;  // C / M mod 1, scaled
+      const double x = X[idx];
+      const double z = fma(cb, x, ca * y);
+      const double dxy = x - y;
+      const bool diag = (row == col);
+      q0 += z * z;
+      q2 += dxy * dxy;
+      q1 += diag ? z : 0.0;
+      q3 += diag ? y : 0.0;
+      Xn[idx] = z;
+      z4[r] = z;
+      zz[4 * n + r] = rint(z * g.c.scale);
+    }
+    if (mirror) {
+      double2* tp = (double2*)(Xn + col0 + (size_t)row * ld);
+      tp[0] = double2{z4[0], z4[1]};
+      tp[1] = double2{z4[2], z4[3]};
     }
   }
-  // workgroup sums (fixed order) through the now idle ring
+  // residues of X_next: rb[k][row][col] within the block, 4 columns per dword
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const double p = g.c.p[k], inv = g.c.inv[k];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      int b4 = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b4 |= (sym_residue(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+      *(int*)&rb[k][rl][16 * n + 4 * (lane >> 4)] = b4;
+    }
+  }
+  __syncthreads();
+  // Block (I, J) of a plane: rows SB I + [0, 64), columns SB J + [0, 64) at
+  // J * pld * 64 + (SB I + row) * 64 + col; block (J, I) holds the transpose.
+  int8_t* P = g.planes[cur ^ 1];
+  const size_t plane = (size_t)pld * pld;
+  const int dr = tid >> 2, dc = 16 * (tid & 3);     // direct: row dr, bytes dc..dc+15
+  const int tc = tid & 63, tr = 16 * (tid >> 6);    // transposed: column tc, rows tr..tr+15
+  int8_t* pd = P + (size_t)J * pld * SB + (size_t)(SB * I + dr) * SB + dc;
+  int8_t* pt = P + (size_t)I * pld * SB + (size_t)(SB * J + tc) * SB + tr;
+#pragma unroll 1
+  for (int k = 0; k < NP; ++k) {
+    const int* s4 = (const int*)&rb[k][dr][dc];
+    *(i4*)(pd + (size_t)k * plane) = i4{s4[0], s4[1], s4[2], s4[3]};
+    if (mirror) {
+      int v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          v[j] |= (int)(uint8_t)rb[k][tr + 4 * j + i][tc] << (8 * i);
+      }
+      *(i4*)(pt + (size_t)k * plane) = i4{v[0], v[1], v[2], v[3]};
+    }
+  }
+  // block sums (fixed order); off-diagonal blocks stand for their mirror too
+  const double wz = mirror ? 2.0 : 1.0;
+  double sums[4] = {wz * q0, q1, wz * q2, q3};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     double x = sums[q];
@@ -320,31 +369,40 @@ __global__ __launch_bounds__(64 * W, 1) void sp2_crt_kernel(CrtArgs g) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     sums[q] = x;
   }
-  double* red = (double*)&ring[0][0];
-  __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) red[4 * w + q] = sums[q];
+    for (int q = 0; q < 4; ++q) red[b][q] = sums[q];
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
-    const int q = threadIdx.x;
-    double acc_q = 0.0;
+  if (tid < 4) {
+    double a = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < W; ++ww) acc_q += red[4 * ww + q];
-    g.part[4 * (size_t)t + q] = acc_q;
+    for (int ww = 0; ww < NT / 64; ++ww) a += red[ww][tid];
+    g.part[4 * (size_t)blockIdx.x + tid] = a;
   }
 }
 
-// Residue planes of an FP64 symmetric matrix (whole ld x ld buffer; the zero
-// padding maps to zero residues).
+// Residue planes of an FP64 symmetric matrix in the 64-column block layout
+// (pld x pld; entries outside the ld x ld buffer are zero).  The padding of
+// the second plane buffer is cleared too: the update kernel writes only
+// inside ld x ld.
 template <int NP>
-__global__ void crt_split_kernel(const double* __restrict__ x, int8_t* __restrict__ planes,
-                                 size_t count, CrtConst c) {
+__global__ void crt_split_kernel(const double* __restrict__ x, int8_t* __restrict__ p0,
+                                 int8_t* __restrict__ p1, int ld, int pld, CrtConst c) {
+  const size_t count = (size_t)pld * pld;
   for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
-    const double v = rint(x[i] * c.scale);
+    const size_t q = i / SB;
+    const int r = (int)(q % pld);
+    const int col = (int)(q / pld) * SB + (int)(i % SB);
+    const bool in = (r < ld) && (col < ld);
+    // X is symmetric: (r, col) is read as (col, r), contiguous along col
+    const double v = in ? rint(x[col + (size_t)r * ld] * c.scale) : 0.0;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) planes[(size_t)k * count + i] = (int8_t)sym_residue(v, c.p[k], c.inv[k]);
+    for (int k = 0; k < NP; ++k) p0[(size_t)k * count + i] = (int8_t)sym_residue(v, c.p[k], c.inv[k]);
+    if (!in) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) p1[(size_t)k * count + i] = 0;
+    }
   }
 }
 
@@ -385,85 +443,63 @@ CrtConst make_const(int np) {
   return c;
 }
 
-// Workgroup layout: 4 or 8 waves, and a ring of 4 or 8 operand stages
-// (64 or 128 KB of LDS), coded waves * 100 + stages.  PRG_CRT_WAVES=4|8 and
-// PRG_CRT_STAGES=4|8 force a layout; otherwise the first product of each
-// (ld, moduli) pair times the four layouts (each twice, the minimum counts)
-// and keeps the fastest; the product is then computed once more with the
-// chosen layout so that results do not depend on the trial order.  All
-// layouts give bit-identical X^2; only the order of the partial trace sums
-// differs.
-int g_forced = -1;                        // -1: not read yet; 0: autotune
-std::vector<std::array<int, 3>> g_tuned;  // (ld, np, layout)
-int g_last_layout = 0;
+inline int plane_ld(int ld) { return ((ld + TM - 1) / TM) * TM; }
 
-int forced_layout() {
+// Ring depth of the product kernel: 3 or 4 stages (96 or 128 KB of LDS).
+// PRG_CRT_STAGES=3|4 forces one; otherwise the first product of each
+// (pld, moduli) pair times both (twice each, the minimum counts) and keeps
+// the faster.  The product kernel only writes its residue buffer, so the
+// trial launches leave no trace and the results do not depend on the choice.
+int g_forced = -1;                        // -1: not read yet; 0: autotune
+std::vector<std::array<int, 3>> g_tuned;  // (pld, np, stages)
+int g_last_stages = 0;
+
+int forced_stages() {
   if (g_forced < 0) {
-    const char* ew = std::getenv("PRG_CRT_WAVES");
     const char* es = std::getenv("PRG_CRT_STAGES");
-    const int w = ew ? std::atoi(ew) : 0, ns = es ? std::atoi(es) : 0;
-    const bool fw = (w == 4 || w == 8), fs = (ns == 4 || ns == 8);
-    g_forced = (fw || fs) ? (fw ? w : 8) * 100 + (fs ? ns : 4) : 0;
+    const int ns = es ? std::atoi(es) : 0;
+    g_forced = (ns == 3 || ns == 4) ? ns : 0;
   }
   return g_forced;
 }
 
-template <int NP>
-void launch_step_w(const CrtArgs& g, int nb, int layout) {
-  switch (layout) {
-    case 404:
-      hipLaunchKernelGGL((sp2_crt_kernel<NP, 4, 4>), dim3(nb), dim3(256), 0, stream(), g);
-      break;
-    case 408:
-      hipLaunchKernelGGL((sp2_crt_kernel<NP, 4, 8>), dim3(nb), dim3(256), 0, stream(), g);
-      break;
-    case 804:
-      hipLaunchKernelGGL((sp2_crt_kernel<NP, 8, 4>), dim3(nb), dim3(512), 0, stream(), g);
-      break;
-    default:
-      hipLaunchKernelGGL((sp2_crt_kernel<NP, 8, 8>), dim3(nb), dim3(512), 0, stream(), g);
-      break;
-  }
+void launch_product(const ProductArgs& g, int nitems, int stages) {
+  if (stages == 3)
+    hipLaunchKernelGGL((crt_product_kernel<3>), dim3(nitems), dim3(512), 0, stream(), g);
+  else
+    hipLaunchKernelGGL((crt_product_kernel<4>), dim3(nitems), dim3(512), 0, stream(), g);
 }
 
-template <int NP>
-int launch_step(const CrtArgs& g, int nb, int ld) {
-  int layout = forced_layout();
-  if (!layout)
+int run_product(const ProductArgs& g, int nitems, int* used) {
+  int stages = forced_stages();
+  if (!stages)
     for (const auto& t : g_tuned)
-      if (t[0] == ld && t[1] == NP) layout = t[2];
-  if (!layout) {
-    hipEvent_t ev[2];
-    PRG_HIP_CHECK(hipEventCreate(&ev[0]));
-    PRG_HIP_CHECK(hipEventCreate(&ev[1]));
-    constexpr int cand[4] = {404, 408, 804, 808};
-    float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-    for (int rep = 0; rep < 2; ++rep)
-      for (int c = 0; c < 4; ++c) {
-        PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
-        launch_step_w<NP>(g, nb, cand[c]);
-        PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
-        PRG_HIP_CHECK(hipEventSynchronize(ev[1]));
-        float ms = 0.f;
-        PRG_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        best[c] = std::min(best[c], ms);
-      }
-    PRG_HIP_CHECK(hipEventDestroy(ev[0]));
-    PRG_HIP_CHECK(hipEventDestroy(ev[1]));
-    int bi = 0;
-    for (int c = 1; c < 4; ++c)
-      if (best[c] < best[bi]) bi = c;
-    layout = cand[bi];
-    g_tuned.push_back({ld, NP, layout});
+      if (t[0] == g.pld && t[1] == g.np) stages = t[2];
+  if (stages) {
+    launch_product(g, nitems, stages);
+    *used = stages;
+    return OK;
   }
-  g_last_layout = layout;
-  launch_step_w<NP>(g, nb, layout);
+  hipEvent_t ev[2];
+  PRG_HIP_CHECK(hipEventCreate(&ev[0]));
+  PRG_HIP_CHECK(hipEventCreate(&ev[1]));
+  float best[2] = {1e30f, 1e30f};
+  for (int rep = 0; rep < 2; ++rep)
+    for (int c = 0; c < 2; ++c) {
+      PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
+      launch_product(g, nitems, 3 + c);
+      PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
+      PRG_HIP_CHECK(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      PRG_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      best[c] = std::min(best[c], ms);
+    }
+  PRG_HIP_CHECK(hipEventDestroy(ev[0]));
+  PRG_HIP_CHECK(hipEventDestroy(ev[1]));
+  stages = (best[1] <= best[0]) ? 4 : 3;
+  g_tuned.push_back({g.pld, g.np, stages});
+  *used = stages;  // the trials computed the residues already
   return OK;
-}
-
-template <int NP>
-void launch_split(const double* x, int8_t* planes, size_t count, const CrtConst& c) {
-  hipLaunchKernelGGL(crt_split_kernel<NP>, dim3(2048), dim3(NT), 0, stream(), x, planes, count, c);
 }
 
 #define PRG_CRT_DISPATCH(NPV, CALL) \
@@ -480,56 +516,64 @@ void launch_split(const double* x, int8_t* planes, size_t count, const CrtConst&
     default: break;                 \
   }
 
-}  // namespace
-
-int crt_moduli() { return g_moduli; }
-
-int set_crt_moduli(int np) {
-  if (np < 6 || np > kMaxModuli) {
-    set_error(ERR_ARG, "int8 CRT products: the number of moduli must be in [6, 14]");
-    return ERR_ARG;
-  }
-  g_moduli = np;
-  return OK;
+template <int NP>
+void launch_update(const UpdateArgs& g, int nblocks) {
+  hipLaunchKernelGGL(crt_update_kernel<NP>, dim3(nblocks), dim3(NT), 0, stream(), g);
 }
 
-// Device copy of the per-modulus constants the product kernel indexes at run
-// time (refreshed when the number of moduli changes).
-// Upper-triangle tiles in groups of G x G tiles (group rows SI <= group columns
-// SJ, tiles column by column inside a group).  Each XCD receives a contiguous
-// range of this list, so the workgroups resident on one XCD at a time cover
-// a compact block of tiles: with G = 8 and 32 resident workgroups, half a
-// group, i.e. 8 A panels and 4 B panels shared through that XCD's L2 instead
-// of 32 A panels and one B panel for a run of one tile column.
-static const int2* tile_order(int nt, int* st) {
-  static int cached_nt = 0;
+template <int NP>
+void launch_split(const double* x, int8_t* p0, int8_t* p1, int ld, int pld, const CrtConst& c) {
+  hipLaunchKernelGGL(crt_split_kernel<NP>, dim3(2048), dim3(NT), 0, stream(), x, p0, p1, ld, pld,
+                     c);
+}
+
+// Product items: the upper-triangle tiles in groups of G x G tiles (group
+// rows SI <= group columns SJ, tiles column by column inside a group), split
+// into 8 contiguous parts, one per XCD; each part lists its tiles once per
+// modulus.  The 32 workgroups resident on an XCD then work on one modulus and
+// a compact block of tiles (8 A panels and 4 B panels shared through that
+// XCD's L2), and all XCDs move through the moduli together, so one modulus's
+// planes (pld^2 bytes) are what the shared last-level cache holds.
+const int4* item_list(int nt, int np, int* st) {
+  static int cached_nt = 0, cached_np = 0;
   const int nb = nt * (nt + 1) / 2;
-  int2* d = (int2*)workspace("crt_tiles", sizeof(int2) * (size_t)nb, st);
-  if (*st || cached_nt == nt) return d;
+  int4* d = (int4*)workspace("crt_items", sizeof(int4) * (size_t)nb * np, st);
+  if (*st || (cached_nt == nt && cached_np == np)) return d;
   constexpr int G = 8;
-  std::vector<int2> h;
-  h.reserve(nb);
+  std::vector<int2> tiles;
+  tiles.reserve(nb);
   const int ng = (nt + G - 1) / G;
   for (int SJ = 0; SJ < ng; ++SJ)
     for (int SI = 0; SI <= SJ; ++SI)
       for (int J = SJ * G; J < std::min(nt, SJ * G + G); ++J)
         for (int I = SI * G; I < std::min(nt, SI * G + G); ++I)
-          if (I <= J) h.push_back(int2{I, J});
+          if (I <= J) tiles.push_back(int2{I, J});
+  std::vector<int4> h;
+  h.reserve((size_t)nb * np);
+  for (int x = 0; x < 8; ++x) {
+    const int lo = (int)((long long)nb * x / 8), hi = (int)((long long)nb * (x + 1) / 8);
+    for (int k = 0; k < np; ++k)
+      for (int i = lo; i < hi; ++i)
+        h.push_back(int4{k, tiles[i].x, tiles[i].y, tiles[i].y * (tiles[i].y + 1) / 2 + tiles[i].x});
+  }
   // on the library stream, after the workspace's zero fill, and complete
   // before the host vector goes away
-  if ((int)h.size() != nb ||
-      hipMemcpyAsync(d, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice, stream()) !=
+  if ((int)tiles.size() != nb ||
+      hipMemcpyAsync(d, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice, stream()) !=
           hipSuccess ||
       hipStreamSynchronize(stream()) != hipSuccess) {
-    set_error(ERR_HIP, "int8 CRT products: tile list upload failed");
+    set_error(ERR_HIP, "int8 CRT products: item list upload failed");
     *st = ERR_HIP;
     return nullptr;
   }
   cached_nt = nt;
+  cached_np = np;
   return d;
 }
 
-static double* fold_constants(int np, int* st) {
+// Device copy of the per-modulus constants the product kernel indexes at run
+// time (refreshed when the number of moduli changes).
+double* fold_constants(int np, int* st) {
   static int cached = 0;
   double* d = (double*)workspace("crt_fold", sizeof(double) * 3 * kMaxModuli, st);
   if (*st || cached == np) return d;
@@ -550,14 +594,33 @@ static double* fold_constants(int np, int* st) {
   return d;
 }
 
+}  // namespace
+
+int crt_moduli() { return g_moduli; }
+
+int set_crt_moduli(int np) {
+  if (np < 6 || np > kMaxModuli) {
+    set_error(ERR_ARG, "int8 CRT products: the number of moduli must be in [6, 14]");
+    return ERR_ARG;
+  }
+  g_moduli = np;
+  return OK;
+}
+
+int crt_update_blocks(int ld) {
+  const int nb = ld / SB;
+  return nb * (nb + 1) / 2;
+}
+
 int crt_prepare(const DMat& x0, int np) {
   int st = OK;
-  const size_t plane = (size_t)x0.ld * x0.ld;
+  const int ld = x0.ld, pld = plane_ld(ld);
+  const size_t plane = (size_t)pld * pld;
   int8_t* planes = (int8_t*)workspace("crt_planes", 2 * (size_t)np * plane, &st);
   if (st) return st;
   if (!fold_constants(np, &st)) return st;
   const CrtConst c = make_const(np);
-#define PRG_CALL(K) launch_split<K>(x0.p, planes, plane, c)
+#define PRG_CALL(K) launch_split<K>(x0.p, planes, planes + (size_t)np * plane, ld, pld, c)
   PRG_CRT_DISPATCH(np, PRG_CALL)
 #undef PRG_CALL
   PRG_HIP_CHECK(hipGetLastError());
@@ -566,40 +629,51 @@ int crt_prepare(const DMat& x0, int np) {
 
 int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
              int np) {
-  const int ld = x0.ld;
-  if (ld >= (1 << 17)) {
+  const int ld = x0.ld, pld = plane_ld(ld);
+  if (ld % SB != 0 || pld >= (1 << 17)) {
     set_error(ERR_ARG, "int8 CRT products: matrix too large for int32 accumulation");
     return ERR_ARG;
   }
-  const size_t plane = (size_t)ld * ld;
+  const size_t plane = (size_t)pld * pld;
+  const int nt = pld / TM;
+  const int nb = nt * (nt + 1) / 2;
   int st = OK;
   int8_t* planes = (int8_t*)workspace("crt_planes", 2 * (size_t)np * plane, &st);
+  int8_t* res = (int8_t*)workspace("crt_residues", (size_t)nb * np * kItemBytes, &st);
   if (st) return st;
-  CrtArgs g;
-  g.buf[0] = x0.p;
-  g.buf[1] = x1.p;
-  g.planes[0] = planes;
-  g.planes[1] = planes + (size_t)np * plane;
-  g.ld = ld;
-  g.st = d_state;
-  g.part = d_part;
-  g.fold = fold_constants(np, &st);
+  ProductArgs pa;
+  pa.planes[0] = planes;
+  pa.planes[1] = planes + (size_t)np * plane;
+  pa.res = res;
+  pa.st = d_state;
+  pa.pld = pld;
+  pa.np = np;
+  pa.fold = fold_constants(np, &st);
   if (st) return st;
-  g.c = make_const(np);
-  const int nt = ld / BM;
-  const int nb = nt * (nt + 1) / 2;
-  g.tiles = tile_order(nt, &st);
+  pa.items = item_list(nt, np, &st);
   if (st) return st;
-  if (ntiles) *ntiles = nb;
-  st = OK;
-#define PRG_CALL(K) st = launch_step<K>(g, nb, ld)
+  UpdateArgs ua;
+  ua.buf[0] = x0.p;
+  ua.buf[1] = x1.p;
+  ua.planes[0] = planes;
+  ua.planes[1] = planes + (size_t)np * plane;
+  ua.res = res;
+  ua.st = d_state;
+  ua.part = d_part;
+  ua.ld = ld;
+  ua.pld = pld;
+  ua.c = make_const(np);
+  const int nblocks = crt_update_blocks(ld);
+  if (ntiles) *ntiles = nblocks;
+  if ((st = run_product(pa, nb * np, &g_last_stages))) return st;
+  PRG_HIP_CHECK(hipGetLastError());
+#define PRG_CALL(K) launch_update<K>(ua, nblocks)
   PRG_CRT_DISPATCH(np, PRG_CALL)
 #undef PRG_CALL
-  if (st) return st;
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
 
-int crt_last_waves() { return g_last_layout; }
+int crt_last_waves() { return g_last_stages ? 800 + g_last_stages : 0; }
 
 }  // namespace prg

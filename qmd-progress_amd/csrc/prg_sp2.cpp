@@ -7,6 +7,7 @@
 // remaining launches of a chunk exit at their first instruction.
 #include "prg_sp2.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -51,7 +52,10 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   const size_t mat_bytes = sizeof(double) * (size_t)ld * ld;
   double* x0 = (double*)workspace("sp2_x0", mat_bytes, &st);
   double* x1 = (double*)workspace("sp2_x1", mat_bytes, &st);
-  double* part = (double*)workspace("sp2_part", sizeof(double) * 4 * (size_t)(ntiles + 2048), &st);
+  // partial sums: one slot per product tile (the int8 CRT update writes one
+  // per 64 x 64 block) plus the normalisation's
+  const int nslots = std::max(ntiles, crt_update_blocks(ld));
+  double* part = (double*)workspace("sp2_part", sizeof(double) * 4 * (size_t)(nslots + 2048), &st);
   double* bounds = (double*)workspace("sp2_bounds", sizeof(double) * 4, &st);
   Sp2State* dst = (Sp2State*)workspace("sp2_state", sizeof(Sp2State), &st);
   Sp2State* hst = (Sp2State*)host_staging(sizeof(Sp2State), &st);

@@ -53,7 +53,8 @@ int crt_prepare(const DMat& x0, int np);
 int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
              int np);
 int crt_moduli();
-int crt_last_waves();  // workgroup layout of the last int8 CRT product (4 or 8)
+int crt_last_waves();  // 800 + ring stages of the last int8 CRT product kernel (0: none yet)
+int crt_update_blocks(int ld);  // partial-sum slots one int8 CRT step writes
 int set_crt_moduli(int np);
 
 int precision_mode();
