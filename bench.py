@@ -173,17 +173,23 @@ def main():
         P.step_timing(True)
         P.sp2_alg2_device(hpad, rho, n, 0.5, 1, 200, "Abs", 1e-10, precision=mode)
         ms_sum, cnt = P.step_timing()
+        pms_sum, pcnt = P.int8_product_timing()
         P.step_timing(False)
         if cnt:
-            kms = ms_sum / cnt
+            r["step_ms"] = ms_sum / cnt
+            r["step_launches"] = cnt
+            # the product kernel: the whole step for fp64 / f16split; the
+            # int8 CRT product kernel alone (its FP64 update kernel apart)
+            kms = (pms_sum / pcnt) if (name == "int8crt" and pcnt) else ms_sum / cnt
             r["product_kernel_ms"] = kms
-            r["product_kernel_launches"] = cnt
+            r["product_kernel_launches"] = pcnt if name == "int8crt" else cnt
             r["product_kernel_fp64_equiv_tflops"] = 2.0 * float(n) ** 3 / (kms * 1e-3) / 1e12
             ops = executed_ops(ld, name, args.moduli)
             r["product_kernel_mfma_ops"] = ops
             r["product_kernel_mfma_tops"] = ops / (kms * 1e-3) / 1e12
             if name == "int8crt":
-                r["int8_waves"] = P.int8_waves()
+                r["update_kernel_ms"] = r["step_ms"] - kms
+                r["int8_config"] = P.int8_waves()
         if evals is not None:
             rr = rho[:n, :n]
             err, tr, eb = errors_vs_diag(h, rr, evals, evecs, n)
@@ -197,21 +203,29 @@ def main():
     hr = results.get(head, {})
     peak_tf = {"fp64": PEAK["fp64_tflops"], "f16split": PEAK["fp16_tflops"],
                "int8crt": PEAK["int8_tops"]}[head]
+    kms = hr.get("product_kernel_ms")
+    alg = (2.0 * float(n) ** 3 / (kms * 1e-3) / 1e12) if kms else None
     kern = hr.get("product_kernel_mfma_tops")
     roofline = {
         "bound": "mfma",
-        "kernel": {"fp64": "sp2_step_kernel (FP64 MFMA)",
+        "kernel": {"fp64": "sp2_step_kernel (FP64 MFMA, fused SP2 update)",
                    "f16split": "sp2_f16split_kernel (FP16 MFMA, 3 products)",
-                   "int8crt": f"sp2_crt_kernel<{args.moduli}> (int8 MFMA, one product per "
-                              "modulus)"}[head],
-        "achieved": kern,
+                   "int8crt": f"crt_product_kernel (int8 MFMA, {args.moduli} moduli x "
+                              "upper-triangle 256x256 tiles)"}[head],
+        "achieved": alg,
         "peak": peak_tf,
-        "unit": "TOP/s" if head == "int8crt" else "TFLOP/s",
-        "frac": (kern / peak_tf) if kern else None,
-        "traffic": None,
-        "note": "achieved = MFMA work of the symmetric product per launch (upper-triangle "
-                "128x128 tiles: nt(nt+1)/2 * 2*128^2*ld per pass; passes 1 fp64, 3 f16split, "
-                "moduli int8crt) / HIP-event kernel time on the library stream",
+        "unit": "TFLOP/s",
+        "frac": (alg / peak_tf) if alg else None,
+        "traffic": traffic_bytes(head, n, args.moduli),
+        "executed": kern,
+        "executed_frac": (kern / peak_tf) if kern else None,
+        "note": "achieved = algorithmic 2N^3 flop of the X^2 the launch produces / the "
+                "kernel's HIP-event average duration on the library stream; peak = the dense "
+                "MFMA peak of the type that executes it (FP64 78.6 TF, FP16 2.5 PF, int8 "
+                "5.0 POP/s).  executed = MFMA operations the launch issues (upper-triangle "
+                "tiles only; one pass per modulus for int8crt, 3 for f16split) / the same "
+                "time.  traffic = HBM bytes per launch from the committed FETCH_SIZE / "
+                "WRITE_SIZE counter summary of this workload (null when none matches)",
     }
 
     cpu = None
@@ -433,10 +447,27 @@ def adjoint_bench(args, P, torch, np):
 
 
 def executed_ops(ld, mode, moduli):
-    """MFMA operations of one symmetric SP2 product launch."""
-    nt = ld // 128
-    per_pass = nt * (nt + 1) // 2 * 2.0 * 128 * 128 * ld
+    """MFMA operations of one symmetric SP2 product launch: upper-triangle
+    128x128 tiles (fp64, f16split) or 256x256 tiles over the residue planes'
+    leading dimension rounded up to 256 (int8crt)."""
+    tm = 256 if mode == "int8crt" else 128
+    kd = ((ld + 255) // 256) * 256 if mode == "int8crt" else ld
+    nt = kd // tm
+    per_pass = nt * (nt + 1) // 2 * 2.0 * tm * tm * kd
     return per_pass * {"fp64": 1, "f16split": 3, "int8crt": moduli}[mode]
+
+
+def traffic_bytes(mode, n, moduli):
+    """HBM bytes per launch of the dominant kernel from profiles/traffic.json
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the calibration copy),
+    when that file holds this kernel at this size; otherwise None."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    for e in json.loads(path.read_text()).get("kernels", []):
+        if e.get("mode") == mode and e.get("n") == n and e.get("moduli", moduli) == moduli:
+            return e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024
+    return None
 
 
 def cpu_baseline(n, seed, iters, gpu_iters):

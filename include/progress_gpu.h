@@ -358,12 +358,15 @@ int prg_gpu_sp2_adjoint_device(int N, const double *dH, const double *dGout, dou
                                const int *pp, int iters, double mineval, double maxeval,
                                int ckpt);
 
-/* Opt-in timing of the SP2 product kernel launches with HIP events on the
- * library stream (enable resets the counters); prg_gpu_step_timing returns
- * the summed milliseconds and the number of timed products.  Measurement
+/* Opt-in timing of the SP2 product steps with HIP events on the library
+ * stream (enable resets the counters); prg_gpu_step_timing returns the summed
+ * milliseconds and the number of timed steps (all kernels of a step), and
+ * prg_gpu_int8_product_timing the same for the int8 CRT product kernel alone
+ * (the step's other kernel is the FP64 reconstruction / update).  Measurement
  * support for the benchmark; no reference counterpart. */
 void prg_gpu_timing(int enable);
 int prg_gpu_step_timing(double *ms_sum, int *count);
+int prg_gpu_int8_product_timing(double *ms_sum, int *count);
 
 #ifdef __cplusplus
 }

@@ -594,7 +594,42 @@ double* fold_constants(int np, int* st) {
   return d;
 }
 
+// Optional HIP-event timing of the product kernel alone (one event pair per
+// step of the current solve; the caller names how many steps computed).
+bool g_timing = false;
+std::vector<hipEvent_t> g_ev;
+int g_ev_used = 0;
+double g_ms_sum = 0.0;
+int g_ms_count = 0;
+
 }  // namespace
+
+void crt_timing(bool enable) {
+  g_timing = enable;
+  g_ev_used = 0;
+  g_ms_sum = 0.0;
+  g_ms_count = 0;
+}
+
+void crt_timing_begin() { g_ev_used = 0; }
+
+int crt_timing_collect(int executed) {
+  if (!g_timing) return OK;
+  const int n = std::min(executed, g_ev_used);
+  for (int e = 0; e < n; ++e) {
+    float ms = 0.f;
+    PRG_HIP_CHECK(hipEventElapsedTime(&ms, g_ev[2 * e], g_ev[2 * e + 1]));
+    g_ms_sum += ms;
+    ++g_ms_count;
+  }
+  g_ev_used = 0;
+  return OK;
+}
+
+void crt_timing_get(double* ms_sum, int* count) {
+  if (ms_sum) *ms_sum = g_ms_sum;
+  if (count) *count = g_ms_count;
+}
 
 int crt_moduli() { return g_moduli; }
 
@@ -665,8 +700,21 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   ua.c = make_const(np);
   const int nblocks = crt_update_blocks(ld);
   if (ntiles) *ntiles = nblocks;
+  const bool timed = g_timing;
+  if (timed) {
+    if ((int)g_ev.size() < 2 * (g_ev_used + 1)) {
+      const size_t old = g_ev.size();
+      g_ev.resize(2 * (g_ev_used + 1) + 64);
+      for (size_t e = old; e < g_ev.size(); ++e) PRG_HIP_CHECK(hipEventCreate(&g_ev[e]));
+    }
+    PRG_HIP_CHECK(hipEventRecord(g_ev[2 * g_ev_used], stream()));
+  }
   if ((st = run_product(pa, nb * np, &g_last_stages))) return st;
   PRG_HIP_CHECK(hipGetLastError());
+  if (timed) {
+    PRG_HIP_CHECK(hipEventRecord(g_ev[2 * g_ev_used + 1], stream()));
+    ++g_ev_used;
+  }
 #define PRG_CALL(K) launch_update<K>(ua, nblocks)
   PRG_CRT_DISPATCH(np, PRG_CALL)
 #undef PRG_CALL

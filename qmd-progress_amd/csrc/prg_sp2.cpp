@@ -134,6 +134,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   const int chunk = n >= 4096 ? 4 : 16;
   const size_t head = offsetof(Sp2State, pp);
   int prec = p.precision;
+  if (g_timing) crt_timing_begin();
   if (g_timing && (int)g_events.size() < 2 * (maxsteps + 1)) {
     const size_t old = g_events.size();
     g_events.resize(2 * (maxsteps + 1));
@@ -197,6 +198,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
       g_step_ms_sum += ms;
       ++g_step_count;
     }
+    if ((st = crt_timing_collect(executed))) return st;
   }
   if (res) {
     res->refine_steps = hst->refine_steps;
@@ -461,11 +463,17 @@ void prg_gpu_timing(int enable) {
   g_timing = enable != 0;
   g_step_ms_sum = 0.0;
   g_step_count = 0;
+  crt_timing(enable != 0);
 }
 
 int prg_gpu_step_timing(double* ms_sum, int* count) {
   if (ms_sum) *ms_sum = g_step_ms_sum;
   if (count) *count = g_step_count;
+  return OK;
+}
+
+int prg_gpu_int8_product_timing(double* ms_sum, int* count) {
+  crt_timing_get(ms_sum, count);
   return OK;
 }
 

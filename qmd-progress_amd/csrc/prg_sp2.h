@@ -55,6 +55,12 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
 int crt_moduli();
 int crt_last_waves();  // 800 + ring stages of the last int8 CRT product kernel (0: none yet)
 int crt_update_blocks(int ld);  // partial-sum slots one int8 CRT step writes
+// HIP-event timing of the int8 CRT product kernel alone: enable/reset, start of
+// a solve, sum the first `executed` steps of the solve, read the totals.
+void crt_timing(bool enable);
+void crt_timing_begin();
+int crt_timing_collect(int executed);
+void crt_timing_get(double* ms_sum, int* count);
 int set_crt_moduli(int np);
 
 int precision_mode();

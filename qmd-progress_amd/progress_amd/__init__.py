@@ -134,6 +134,7 @@ def lib():
         "prg_compute_response_SP2": (None, [vp, vp, vp, vp, d, d, i, i, cp, d, d, i]),
         "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
         "prg_gpu_step_timing": (i, [dp, ip]),
+        "prg_gpu_int8_product_timing": (i, [dp, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -401,6 +402,15 @@ def step_timing(enable=None):
     ms = C.c_double()
     cnt = C.c_int()
     lib().prg_gpu_step_timing(C.byref(ms), C.byref(cnt))
+    return ms.value, cnt.value
+
+
+def int8_product_timing():
+    """(ms_sum, count) of the int8 CRT product kernel alone, for the steps
+    timed since step_timing(True)."""
+    ms = C.c_double()
+    cnt = C.c_int()
+    lib().prg_gpu_int8_product_timing(C.byref(ms), C.byref(cnt))
     return ms.value, cnt.value
 
 
