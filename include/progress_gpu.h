@@ -323,10 +323,10 @@ int prg_get_precision(void);
 int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 /* Configuration the int8 CRT product kernel last ran with, coded
- * 800 + ring stages (8 waves per workgroup, a 3- or 4-stage operand ring):
- * chosen per (padded size, moduli) by timing both on the first product, or
- * forced with the environment variable PRG_CRT_STAGES=3|4.  0 before any
- * int8 product.  Measurement support; no reference counterpart. */
+ * waves * 100 + ring stages (803, 804, 805: 8 waves; 404, 405: 4 waves):
+ * chosen per (padded size, moduli) by timing each on the first product, or
+ * forced with the environment variable PRG_CRT_CONFIG.  0 before any int8
+ * product.  Measurement support; no reference counterpart. */
 int prg_gpu_int8_waves(void);
 
 /* ------------------------------------------------------------------ device-pointer API */

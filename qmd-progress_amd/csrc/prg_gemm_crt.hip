@@ -57,7 +57,6 @@ constexpr int NT = 256;
 constexpr int kPanel = TM * BK;        // one operand panel of a stage (16 KB)
 constexpr int kStage = 2 * kPanel;     // A and B panels (32 KB)
 constexpr int kItemBytes = TM * TM;    // product residues of one (tile, modulus) item
-constexpr int kWaveDwords = 32 * 64;   // one product wave's share of an item (8 KB)
 constexpr int kPitch = SB + 4;         // LDS row pitch of the update kernel's residues
 constexpr int kMaxModuli = 14;
 
@@ -132,17 +131,22 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
   return (int)m;
 }
 
-// One (tile, modulus) item: C = A_I A_J^T mod p over the full K range, 8 waves
-// in a 2 x 4 grid of 128 x 64 (8 x 4 MFMA blocks of 16 x 16 per wave,
-// accumulators in AGPRs).  Operand stages are copied global -> LDS directly
-// (global_load_lds, 16 bytes per lane), NSTAGE - 1 stages ahead of the MFMAs.
-// A stage holds the A panel (256 rows x 64 bytes) then the B panel, as
-// thirty-two 1 KB chunks of 16 rows; within a row the 16-byte chunk g sits in
-// slot g ^ ((row >> 1) & 3), which keeps the ds_read_b128 of 8 consecutive
-// rows on distinct banks.  The ring is the kernel's only LDS object.
-template <int NSTAGE>
-__global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
-  constexpr int MT = 8, NTL = 4;
+// One (tile, modulus) item: C = A_I A_J^T mod p over the full K range.  W = 8
+// waves in a 2 x 4 grid of 128 x 64 (two waves per SIMD) or W = 4 waves in a
+// 2 x 2 grid of 128 x 128 (one wave per SIMD, a third fewer LDS fragment
+// reads per MFMA); 16 x 16 MFMA blocks, int32 accumulators.  Operand stages
+// are copied global -> LDS directly (global_load_lds, 16 bytes per lane),
+// NSTAGE - 1 stages ahead of the MFMAs.  A stage holds the A panel (256 rows
+// x 64 bytes) then the B panel, as thirty-two 1 KB chunks of 16 rows; within
+// a row the 16-byte chunk g sits in slot g ^ ((row >> 1) & 3), which keeps
+// the ds_read_b128 of 8 consecutive rows on distinct banks.  The ring is the
+// kernel's only LDS object.
+template <int W, int NSTAGE>
+__global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
+  constexpr int WN = W / 2;            // waves along the tile columns
+  constexpr int MT = 8;                // 16-row MFMA blocks per wave
+  constexpr int NTL = TM / WN / 16;    // 16-column MFMA blocks per wave
+  constexpr int CPW = 32 / W;          // 1 KB DMA chunks per wave per stage
   __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
   const volatile Sp2State* stv = g.st;
   if (!stv->planned) return;
@@ -154,9 +158,11 @@ __global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
 
-  // DMA chunks of wave w: c = w + 8 q (q < 4), i.e. panel rows 16 w + lane / 4
-  // (+ 128 for odd q) of A (q < 2) or B (q >= 2), slot lane % 4.  Adding 128
-  // rows leaves the swizzle unchanged.
+  // DMA chunks of wave w: c = w + W q (q < CPW) is chunk c % 16 of panel
+  // c / 16 (A = rows of tile I, then B = rows of tile J): panel rows
+  // 16 (c % 16) + lane / 4, slot lane % 4.  c / 16 = W q / 16 and the row
+  // offset from 16 w + lane / 4 are compile-time; multiples of 16 rows leave
+  // the swizzle unchanged.
   const int drow = 16 * w + (lane >> 2);
   const int kc = 16 * ((lane & 3) ^ ((drow >> 1) & 3));
   const int8_t* srcA = R + (size_t)(I * TM + drow) * BK + kc;
@@ -167,17 +173,19 @@ __global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
   auto issue = [&](int s) {
     const size_t off = (size_t)s * kstride;
     const uint32_t base = __builtin_amdgcn_readfirstlane(dst0 + kStage * (s % NSTAGE));
-    dma16(srcA + off, base);
-    dma16(srcA + off + 128 * BK, base + 8192);
-    dma16(srcB + off, base + kPanel);
-    dma16(srcB + off + 128 * BK, base + kPanel + 8192);
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int panel = (W * q) >> 4;
+      const int rows = 16 * W * q - TM * panel;
+      dma16((panel ? srcB : srcA) + off + rows * BK, base + 1024 * W * q);
+    }
   };
   // LDS read offsets of this lane's fragments: row r = lane % 16 of each
   // 16-row block, 16-byte chunk lane / 16; the block offsets are multiples
   // of 8 rows, so the swizzle depends on the lane only.
   const int fo = (lane & 15) * BK + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
   const int offa = wm * 128 * BK + fo;
-  const int offb = kPanel + wn * 64 * BK + fo;
+  const int offb = kPanel + wn * (TM / WN) * BK + fo;
   i4 acc[MT][NTL];
   i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
   auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
   // carries only MFMA results and they stay in AGPRs).
   auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
     if (s + NSTAGE - 1 < ksteps)
-      wait_vm_barrier<4 * (NSTAGE - 2)>();
+      wait_vm_barrier<CPW * (NSTAGE - 2)>();
     else
       wait_vm_barrier<0>();
     if (s + NSTAGE < ksteps) issue(s + NSTAGE);
@@ -212,7 +220,7 @@ __global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
   for (int s = 0; s < NSTAGE; ++s)
     if (s < ksteps) issue(s);
   if (NSTAGE - 1 < ksteps)
-    wait_vm_barrier<4 * (NSTAGE - 1)>();
+    wait_vm_barrier<CPW * (NSTAGE - 1)>();
   else
     wait_vm_barrier<0>();
   read_frags(0, fa);
@@ -226,32 +234,45 @@ __global__ __launch_bounds__(512, 1) void crt_product_kernel(ProductArgs g) {
 
   // (C q mod p) as symmetric int8.  Operands were swapped, so this lane holds
   // C[row][col] with row = lane % 16 and col = 4 (lane / 16) + r inside each
-  // 16 x 16 block; the four bytes of r go to one dword, stored in fragment
-  // order (wave, block m, block n, lane): 256 contiguous bytes per block.
+  // 16 x 16 block; the four bytes of r form one dword.  Records follow the
+  // 8-wave layout whatever W is: the 64-column group g = (wn * NTL + n) / 4
+  // and row half wm name record wave wm + 2 g; (record wave, m, lane) holds
+  // the four blocks n % 4 of that group as one 16-byte store (1 KB contiguous
+  // per record wave and m).
   const double p = g.fold[3 * k], inv = g.fold[3 * k + 1], qm = g.fold[3 * k + 2];
-  int* dst = (int*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + w * kWaveDwords + lane;
+  i4* dst = (i4*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + lane;
 #pragma unroll
-  for (int m = 0; m < MT; ++m)
+  for (int m = 0; m < MT; ++m) {
 #pragma unroll
-    for (int n = 0; n < NTL; ++n) {
-      int b4 = 0;
+    for (int h = 0; h < NTL / 4; ++h) {
+      i4 out;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^43
-        b4 |= (sym_residue(v, p, inv) & 0xff) << (8 * r);
+      for (int n4 = 0; n4 < 4; ++n4) {
+        const int n = 4 * h + n4;
+        int b4 = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^43
+          b4 |= (sym_residue(v, p, inv) & 0xff) << (8 * r);
+        }
+        out[n4] = b4;
       }
-      dst[(m * NTL + n) * 64] = b4;
+      const int rw = wm + 2 * (wn * (NTL / 4) + h);
+      dst[(rw * 8 + m) * 64] = out;
     }
+  }
 }
 
 // One upper-triangle 64 x 64 block (I, J) of X^2: sum the moduli's fractions,
 // apply the SP2 update, write X_next at (I, J) and (J, I), and the residue
-// planes of X_next (through LDS, as contiguous 4 KB blocks in both
-// positions).  The block lies in product tile (I / 4, J / 4), in the share of
-// product wave (I % 4) / 2 + 2 (J % 4), MFMA rows 4 (I % 2) .. + 3.
+// planes of X_next (through LDS, one modulus at a time, as contiguous 4 KB
+// blocks in both positions).  The block lies in product tile (I / 4, J / 4),
+// in the share of product wave (I % 4) / 2 + 2 (J % 4), MFMA rows
+// 4 (I % 2) .. + 3.  Thread (b, lane) holds elements (row, col0(n) + r) with
+// row = 64 I + 16 b + lane % 16, col0(n) = 64 J + 16 n + 4 (lane / 16).
 template <int NP>
-__global__ __launch_bounds__(NT) void crt_update_kernel(UpdateArgs g) {
-  __shared__ __attribute__((aligned(16))) int8_t rb[NP][SB][kPitch];
+__global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
+  __shared__ __attribute__((aligned(16))) int8_t rb[2][SB][kPitch];
   __shared__ double red[NT / 64][4];
   const volatile Sp2State* stv = g.st;
   if (!stv->planned) return;
@@ -264,88 +285,96 @@ __global__ __launch_bounds__(NT) void crt_update_kernel(UpdateArgs g) {
   const int wave = ((I & 3) >> 1) + 2 * (J & 3);
   const int tid = threadIdx.x, lane = tid & 63, b = tid >> 6;
   const int m = 4 * (I & 1) + b;
-  const int* src = (const int*)(g.res + (size_t)t * NP * kItemBytes) + wave * kWaveDwords +
-                   m * 4 * 64 + lane;
+  const i4* src = (const i4*)(g.res + (size_t)t * NP * kItemBytes) + (wave * 8 + m) * 64 + lane;
+  const int rl = 16 * b + (lane & 15);
+  const int row = SB * I + rl;
+  const int cl0 = 4 * (lane >> 4);
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
 
-  // S = sum_k ((C q_k) mod p_k) / p_k for this lane's 16 elements: blocks
-  // n = 0..3, bytes r = 0..3 (element (row, 16 n + 4 (lane / 16) + r))
+  // X at (row, col0 + r) read as (col0 + r, row): 32 contiguous bytes per n
+  double2 xv[8];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const double2* xp = (const double2*)(X + SB * J + 16 * n + cl0 + (size_t)row * ld);
+    xv[2 * n] = xp[0];
+    xv[2 * n + 1] = xp[1];
+  }
+  // S = sum_k ((C q_k) mod p_k) / p_k, bytes (n, r) of the 16-byte record
+  i4 d[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) d[k] = src[k * (kItemBytes / 16)];
   double S[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) S[i] = 0.0;
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    int d[4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) d[n] = src[k * (kItemBytes / 4) + 64 * n];
     const double inv = g.c.inv[k];
 #pragma unroll
     for (int n = 0; n < 4; ++n)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        S[4 * n + r] = fma((double)(int8_t)(d[n] >> (8 * r)), inv, S[4 * n + r]);
+        S[4 * n + r] = fma((double)(int8_t)(d[k][n] >> (8 * r)), inv, S[4 * n + r]);
   }
 
   const bool mirror = (I != J);
   const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
   const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
-  const double* X = g.buf[cur];
-  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
-  const int rl = 16 * b + (lane & 15);
-  const int row = SB * I + rl;
   double zz[16];
   double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    const int col0 = SB * J + 16 * n + 4 * (lane >> 4);
+    const int col0 = SB * J + 16 * n + cl0;
     double z4[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int col = col0 + r;
-      const size_t idx = row + (size_t)col * ld;
       const double sr = S[4 * n + r];
       const double y = (sr - rint(sr)) * g.c.cscale;  // C / M mod 1, scaled
-      const double x = X[idx];
+      const double x = (r & 1) ? xv[2 * n + (r >> 1)].y : xv[2 * n + (r >> 1)].x;
       const double z = fma(cb, x, ca * y);
       const double dxy = x - y;
-      const bool diag = (row == col);
+      const bool diag = (row == col0 + r);
       q0 += z * z;
       q2 += dxy * dxy;
       q1 += diag ? z : 0.0;
       q3 += diag ? y : 0.0;
-      Xn[idx] = z;
       z4[r] = z;
       zz[4 * n + r] = rint(z * g.c.scale);
     }
+    // (col0 + r, row): contiguous; on diagonal blocks these cover the block
+    double2* tp = (double2*)(Xn + col0 + (size_t)row * ld);
+    tp[0] = double2{z4[0], z4[1]};
+    tp[1] = double2{z4[2], z4[3]};
     if (mirror) {
-      double2* tp = (double2*)(Xn + col0 + (size_t)row * ld);
-      tp[0] = double2{z4[0], z4[1]};
-      tp[1] = double2{z4[2], z4[3]};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Xn[row + (size_t)(col0 + r) * ld] = z4[r];
     }
   }
-  // residues of X_next: rb[k][row][col] within the block, 4 columns per dword
-#pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    const double p = g.c.p[k], inv = g.c.inv[k];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      int b4 = 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) b4 |= (sym_residue(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
-      *(int*)&rb[k][rl][16 * n + 4 * (lane >> 4)] = b4;
-    }
-  }
-  __syncthreads();
-  // Block (I, J) of a plane: rows SB I + [0, 64), columns SB J + [0, 64) at
-  // J * pld * 64 + (SB I + row) * 64 + col; block (J, I) holds the transpose.
+  // Residue planes of X_next, one modulus at a time through a double-buffered
+  // LDS block rb[row][col] (4 columns per dword).  Block (I, J) of a plane:
+  // rows SB I + [0, 64), columns SB J + [0, 64) at J * pld * 64 +
+  // (SB I + row) * 64 + col; block (J, I) holds the transpose.
   int8_t* P = g.planes[cur ^ 1];
   const size_t plane = (size_t)pld * pld;
   const int dr = tid >> 2, dc = 16 * (tid & 3);     // direct: row dr, bytes dc..dc+15
   const int tc = tid & 63, tr = 16 * (tid >> 6);    // transposed: column tc, rows tr..tr+15
   int8_t* pd = P + (size_t)J * pld * SB + (size_t)(SB * I + dr) * SB + dc;
   int8_t* pt = P + (size_t)I * pld * SB + (size_t)(SB * J + tc) * SB + tr;
-#pragma unroll 1
+#pragma unroll
   for (int k = 0; k < NP; ++k) {
-    const int* s4 = (const int*)&rb[k][dr][dc];
+    const double p = g.c.p[k], inv = g.c.inv[k];
+    int8_t (*buf)[kPitch] = rb[k & 1];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      int b4 = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b4 |= (sym_residue(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+      *(int*)&buf[rl][16 * n + cl0] = b4;
+    }
+    // the writes of modulus k - 1 from this buffer's twin are done by every
+    // thread once all pass this barrier; modulus k + 1 fills the twin next
+    __syncthreads();
+    const int* s4 = (const int*)&buf[dr][dc];
     *(i4*)(pd + (size_t)k * plane) = i4{s4[0], s4[1], s4[2], s4[3]};
     if (mirror) {
       int v[4];
@@ -353,8 +382,7 @@ __global__ __launch_bounds__(NT) void crt_update_kernel(UpdateArgs g) {
       for (int j = 0; j < 4; ++j) {
         v[j] = 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          v[j] |= (int)(uint8_t)rb[k][tr + 4 * j + i][tc] << (8 * i);
+        for (int i = 0; i < 4; ++i) v[j] |= (int)(uint8_t)buf[tr + 4 * j + i][tc] << (8 * i);
       }
       *(i4*)(pt + (size_t)k * plane) = i4{v[0], v[1], v[2], v[3]};
     }
@@ -445,49 +473,69 @@ CrtConst make_const(int np) {
 
 inline int plane_ld(int ld) { return ((ld + TM - 1) / TM) * TM; }
 
-// Ring depth of the product kernel: 3 or 4 stages (96 or 128 KB of LDS).
-// PRG_CRT_STAGES=3|4 forces one; otherwise the first product of each
-// (pld, moduli) pair times both (twice each, the minimum counts) and keeps
-// the faster.  The product kernel only writes its residue buffer, so the
-// trial launches leave no trace and the results do not depend on the choice.
+// Product kernel configuration, coded W * 100 + NSTAGE: 8 waves with a 3-,
+// 4- or 5-stage ring (96, 128, 160 KB of LDS) or 4 waves with 4 or 5 stages.
+// PRG_CRT_CONFIG forces one; otherwise the first product of each
+// (pld, moduli) pair times every configuration (twice each, the minimum
+// counts) and keeps the fastest.  The product kernel only writes its residue
+// buffer, so the trial launches leave no trace and the results do not depend
+// on the choice.
+constexpr int kConfigs[] = {803, 804, 805, 404, 405};
+constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 int g_forced = -1;                        // -1: not read yet; 0: autotune
-std::vector<std::array<int, 3>> g_tuned;  // (pld, np, stages)
-int g_last_stages = 0;
+std::vector<std::array<int, 3>> g_tuned;  // (pld, np, config)
+int g_last_config = 0;
 
-int forced_stages() {
+int forced_config() {
   if (g_forced < 0) {
-    const char* es = std::getenv("PRG_CRT_STAGES");
-    const int ns = es ? std::atoi(es) : 0;
-    g_forced = (ns == 3 || ns == 4) ? ns : 0;
+    const char* e = std::getenv("PRG_CRT_CONFIG");
+    const int v = e ? std::atoi(e) : 0;
+    g_forced = 0;
+    for (int c : kConfigs)
+      if (c == v) g_forced = v;
   }
   return g_forced;
 }
 
-void launch_product(const ProductArgs& g, int nitems, int stages) {
-  if (stages == 3)
-    hipLaunchKernelGGL((crt_product_kernel<3>), dim3(nitems), dim3(512), 0, stream(), g);
-  else
-    hipLaunchKernelGGL((crt_product_kernel<4>), dim3(nitems), dim3(512), 0, stream(), g);
+void launch_product(const ProductArgs& g, int nitems, int config) {
+  switch (config) {
+    case 803:
+      hipLaunchKernelGGL((crt_product_kernel<8, 3>), dim3(nitems), dim3(512), 0, stream(), g);
+      break;
+    case 805:
+      hipLaunchKernelGGL((crt_product_kernel<8, 5>), dim3(nitems), dim3(512), 0, stream(), g);
+      break;
+    case 404:
+      hipLaunchKernelGGL((crt_product_kernel<4, 4>), dim3(nitems), dim3(256), 0, stream(), g);
+      break;
+    case 405:
+      hipLaunchKernelGGL((crt_product_kernel<4, 5>), dim3(nitems), dim3(256), 0, stream(), g);
+      break;
+    default:
+      hipLaunchKernelGGL((crt_product_kernel<8, 4>), dim3(nitems), dim3(512), 0, stream(), g);
+      break;
+  }
 }
 
 int run_product(const ProductArgs& g, int nitems, int* used) {
-  int stages = forced_stages();
-  if (!stages)
+  int config = forced_config();
+  if (!config)
     for (const auto& t : g_tuned)
-      if (t[0] == g.pld && t[1] == g.np) stages = t[2];
-  if (stages) {
-    launch_product(g, nitems, stages);
-    *used = stages;
+      if (t[0] == g.pld && t[1] == g.np) config = t[2];
+  if (config) {
+    launch_product(g, nitems, config);
+    *used = config;
     return OK;
   }
   hipEvent_t ev[2];
   PRG_HIP_CHECK(hipEventCreate(&ev[0]));
   PRG_HIP_CHECK(hipEventCreate(&ev[1]));
-  float best[2] = {1e30f, 1e30f};
+  float best[kNumConfigs];
+  for (int c = 0; c < kNumConfigs; ++c) best[c] = 1e30f;
   for (int rep = 0; rep < 2; ++rep)
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < kNumConfigs; ++c) {
       PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
-      launch_product(g, nitems, 3 + c);
+      launch_product(g, nitems, kConfigs[c]);
       PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
       PRG_HIP_CHECK(hipEventSynchronize(ev[1]));
       float ms = 0.f;
@@ -496,9 +544,11 @@ int run_product(const ProductArgs& g, int nitems, int* used) {
     }
   PRG_HIP_CHECK(hipEventDestroy(ev[0]));
   PRG_HIP_CHECK(hipEventDestroy(ev[1]));
-  stages = (best[1] <= best[0]) ? 4 : 3;
-  g_tuned.push_back({g.pld, g.np, stages});
-  *used = stages;  // the trials computed the residues already
+  int bi = 0;
+  for (int c = 1; c < kNumConfigs; ++c)
+    if (best[c] < best[bi]) bi = c;
+  g_tuned.push_back({g.pld, g.np, kConfigs[bi]});
+  *used = kConfigs[bi];  // the trials computed the residues already
   return OK;
 }
 
@@ -534,12 +584,23 @@ void launch_split(const double* x, int8_t* p0, int8_t* p1, int ld, int pld, cons
 // a compact block of tiles (8 A panels and 4 B panels shared through that
 // XCD's L2), and all XCDs move through the moduli together, so one modulus's
 // planes (pld^2 bytes) are what the shared last-level cache holds.
+// PRG_CRT_GROUP=2..32 sets G (default 8).
+int group_size() {
+  static int g = -1;
+  if (g < 0) {
+    const char* e = std::getenv("PRG_CRT_GROUP");
+    const int v = e ? std::atoi(e) : 0;
+    g = (v >= 2 && v <= 32) ? v : 8;
+  }
+  return g;
+}
+
 const int4* item_list(int nt, int np, int* st) {
   static int cached_nt = 0, cached_np = 0;
   const int nb = nt * (nt + 1) / 2;
   int4* d = (int4*)workspace("crt_items", sizeof(int4) * (size_t)nb * np, st);
   if (*st || (cached_nt == nt && cached_np == np)) return d;
-  constexpr int G = 8;
+  const int G = group_size();
   std::vector<int2> tiles;
   tiles.reserve(nb);
   const int ng = (nt + G - 1) / G;
@@ -709,7 +770,7 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
     }
     PRG_HIP_CHECK(hipEventRecord(g_ev[2 * g_ev_used], stream()));
   }
-  if ((st = run_product(pa, nb * np, &g_last_stages))) return st;
+  if ((st = run_product(pa, nb * np, &g_last_config))) return st;
   PRG_HIP_CHECK(hipGetLastError());
   if (timed) {
     PRG_HIP_CHECK(hipEventRecord(g_ev[2 * g_ev_used + 1], stream()));
@@ -722,6 +783,6 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   return OK;
 }
 
-int crt_last_waves() { return g_last_stages ? 800 + g_last_stages : 0; }
+int crt_last_waves() { return g_last_config; }
 
 }  // namespace prg
