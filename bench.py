@@ -379,7 +379,8 @@ def fermi_bench(args, P, torch, np):
     mu = 0.5 * (w[n // 2 - 1] + w[n // 2]).item()
 
     def solve(mu0, method):
-        L.prg_implicit_fermi(mh.h, mp.h, nsteps, 2, nocc, mu0, beta, method, 30, 1e-9, 0.0,
+        # osteps = 0: Newton steps on mu until |Tr P - nocc| <= 1e-6
+        L.prg_implicit_fermi(mh.h, mp.h, nsteps, 2, nocc, mu0, beta, method, 0, 1e-6, 0.0,
                              1e-10)
         P._check()
         return L.prg_implicit_fermi_last_mu()
@@ -388,11 +389,13 @@ def fermi_bench(args, P, torch, np):
     for _ in range(args.warmup):
         mu_w = solve(mu, method)
     torch.cuda.synchronize()
+    P.gemm_counters(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         mu1 = solve(mu, method)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    nprod, alg, exe = (v / args.steps for v in P.gemm_counters(reset=True))
     pm = torch.from_numpy(mp.numpy()).cuda()
     wv, q = torch.linalg.eigh(h)
     f = 1.0 / (1.0 + torch.exp(beta * (wv - mu1)))
@@ -404,8 +407,16 @@ def fermi_bench(args, P, torch, np):
           {"workload": f"prg_implicit_fermi model tight-binding H, N={n}, beta=20 (kT 0.05 eV), "
                        f"8 levels, k=2, method 1 (Newton-Schulz), seed={args.seed}", "n": n,
            "mu_start": mu, "mu": mu1},
-          {"accuracy": {"fro_err_vs_fermi_eigh": err, "trace_err": tr}, "cpu_baseline": None,
-           "roofline": None})
+          {"accuracy": {"fro_err_vs_fermi_eigh": err, "trace_err": tr},
+           "products_per_solve": nprod, "alg_tflops": alg / dt / 1e12,
+           "cpu_baseline": None,
+           "roofline": {"bound": "mfma", "kernel": "gemm_kernel (FP64 MFMA)",
+                        "achieved": exe / dt / 1e12, "peak": PEAK["fp64_tflops"],
+                        "unit": "TFLOP/s", "frac": exe / dt / 1e12 / PEAK["fp64_tflops"],
+                        "traffic": None,
+                        "note": "MFMA flops executed by the solve's products / whole-solve "
+                                "wall time (host recurrences, reductions and syncs included: "
+                                "a lower bound of the kernel rate)"}})
 
 
 def adjoint_bench(args, P, torch, np):
@@ -432,18 +443,21 @@ def adjoint_bench(args, P, torch, np):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    P.gemm_counters(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         lv, grad = step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    nprod, alg, exe = (v / args.steps for v in P.gemm_counters(reset=True))
     # symmetry of the gradient and the first-order response cross-check
     sym = torch.linalg.norm(grad - grad.T).item() / torch.linalg.norm(grad).item()
     _line("SP2 forward + adjoint solves/s (dTr(D A)/dH, FP64)", 1.0 / dt, "solves/s", dt, args,
           "f64",
           {"workload": f"sp2_density forward + backward, model H N={n}, A banded bw 32, "
                        f"checkpoints every 8 steps, seed={args.seed}", "n": n},
-          {"loss": lv, "grad_asymmetry": sym, "cpu_baseline": None, "roofline": None})
+          {"loss": lv, "grad_asymmetry": sym, "reverse_products_per_solve": nprod,
+           "reverse_alg_tflops": alg / dt / 1e12, "cpu_baseline": None, "roofline": None})
 
 
 def executed_ops(ld, mode, moduli):

@@ -367,6 +367,12 @@ int prg_gpu_sp2_adjoint_device(int N, const double *dH, const double *dGout, dou
 void prg_gpu_timing(int enable);
 int prg_gpu_step_timing(double *ms_sum, int *count);
 int prg_gpu_int8_product_timing(double *ms_sum, int *count);
+/* FP64 products issued by the iterative solvers (implicit Fermi and its
+ * responses, SP2-Fermi, the adjoint) since the last reset: count, algorithmic
+ * flops (2 N^3 each) and MFMA flops executed (symmetric products compute the
+ * upper-triangle tiles only).  reset != 0 clears the counters after reading.
+ * Any pointer may be NULL.  Measurement support; no reference counterpart. */
+void prg_gpu_gemm_counters(int reset, double *products, double *alg_flops, double *exec_flops);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,11 @@ int sp2_adjoint(const DMat& H, const DMat& Gout, DMat grad, const int* pp, int i
   }
   // backward
   if ((st = ops::lin(Gout, 2.0, nullptr, 0.0, 0.0, G))) return st;
+  // A symmetric G stays symmetric (S = X G + (X G)^T), and then only the
+  // upper-triangle tiles of S are computed; any other G takes full products.
+  ops::Sums asym;
+  if ((st = ops::transpose(G, S)) || (st = ops::lin(G, 1.0, &S, -1.0, 0.0, S, &asym))) return st;
+  const bool sym = (asym.fro2 == 0.0);
   int seg_start = -1;
   for (int k = iters - 1; k >= 0; --k) {
     const int c = k / ckpt;
@@ -76,7 +81,7 @@ int sp2_adjoint(const DMat& H, const DMat& Gout, DMat grad, const int* pp, int i
     }
     const DMat Xk = segm(k - seg_start);
     // S = X_k G + G X_k
-    if ((st = ops::mm2(Xk, G, G, Xk, S, false))) return st;
+    if ((st = ops::mm2(Xk, G, G, Xk, S, sym))) return st;
     if (pp[k]) {
       if ((st = ops::copy(S, G))) return st;
     } else {

@@ -62,7 +62,10 @@ int conjgrad(const DMat& A, DMat X, const DMat& B, double tol, bool commuting, i
 
 // prg_newtonschulz (src/prg_implicit_fermi_mod.F90:888-922): Ai <- Ai + Ai R,
 // R = I - A Ai, until ||R||_F <= tol; CG fallback when the error grows.
-int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters) {
+// `commuting`: Ai is a polynomial of A (true of every inverse the implicit
+// Fermi recursion refines: A and the initial Ai are functions of H), so both
+// products are symmetric and only their upper-triangle tiles are computed.
+int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters, bool commuting) {
   DMat R, T, I;
   int st;
   if ((st = ops::temp("ns_r", A.n, &R)) || (st = ops::temp("ns_t", A.n, &T)) ||
@@ -72,7 +75,7 @@ int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters) {
   int i = 0;
   Sums s;
   while (err > tol) {
-    if ((st = ops::mm(A, Ai, R, false, -1.0, 0.0, nullptr, 1.0, &s))) return st;
+    if ((st = ops::mm(A, Ai, R, commuting, -1.0, 0.0, nullptr, 1.0, &s))) return st;
     const double prev = err;
     err = std::sqrt(s.fro2);
     if (!std::isfinite(err)) {
@@ -81,9 +84,9 @@ int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters) {
     }
     if (prev + 0.01 < err) {
       if ((st = ops::identity(I))) return st;
-      if ((st = conjgrad(A, Ai, I, 0.00001, false, nullptr))) return st;
+      if ((st = conjgrad(A, Ai, I, 0.00001, commuting, nullptr))) return st;
     } else {
-      if ((st = ops::mm(Ai, R, T, false, 1.0, 1.0, &Ai))) return st;
+      if ((st = ops::mm(Ai, R, T, commuting, 1.0, 1.0, &Ai))) return st;
       if ((st = ops::copy(T, Ai))) return st;
     }
     ++i;
@@ -142,9 +145,9 @@ int implicit_fermi(const DMat& H, DMat P, int nsteps, int k, double nocc, double
       } else {
         // level-1 CG inverse (tol 0.9) refined by Newton-Schulz at every level;
         // the reference comments the refinement out and then diverges
-        if (i == 0 && (st = conjgrad(Y, AI, I, 0.9, false, nullptr))) return st;
-        if ((st = newtonschulz(Y, AI, tol, nullptr))) return st;
-        if ((st = ops::mm(AI, P2, P, false))) return st;
+        if (i == 0 && (st = conjgrad(Y, AI, I, 0.9, true, nullptr))) return st;
+        if ((st = newtonschulz(Y, AI, tol, nullptr, true))) return st;
+        if ((st = ops::mm(AI, P2, P, true))) return st;
       }
     }
     if ((st = ops::sums_of(P, &s))) return st;
@@ -191,11 +194,14 @@ int implicit_fermi_save_inverse(std::vector<DMat>& inv, const DMat& H, DMat P, i
     for (int i = 0; i < nsteps; ++i) {
       if ((st = ops::mm(P, P, P2, true))) return st;
       if ((st = ops::lin(P2, 2.0, &P, -2.0, 1.0, Y))) return st;
-      if (iter == 1 && (st = conjgrad(Y, inv[i], I, 0.001, false, nullptr))) return st;
+      // the saved inverses are functions of this H (and commute with Y)
+      // unless they come from an earlier SCF iteration's Hamiltonian
+      const bool commuting = (scf_it == 1 || iter > 1);
+      if (iter == 1 && (st = conjgrad(Y, inv[i], I, 0.001, commuting, nullptr))) return st;
       int nsit = 0;
-      if ((st = newtonschulz(Y, inv[i], tol, &nsit))) return st;
+      if ((st = newtonschulz(Y, inv[i], tol, &nsit, commuting))) return st;
       ns_total += nsit;
-      if ((st = ops::mm(inv[i], P2, P, false))) return st;
+      if ((st = ops::mm(inv[i], P2, P, commuting))) return st;
     }
     if ((st = ops::sums_of(P, &s))) return st;
     const double trp0 = s.tr;
@@ -337,7 +343,7 @@ int implicit_fermi_response(const DMat& H0, const std::vector<DMat>& Hs, DMat P0
       }
       if ((st = ops::lin(C0, 2.0, &P0, -2.0, 1.0, T))) return st;  // T = 2 P0^2 - 2 P0 + I
       if (i == 0 && (st = conjgrad(T, Ti, I, 0.01, false, nullptr))) return st;
-      if ((st = newtonschulz(T, Ti, lin_tol, nullptr))) return st;
+      if ((st = newtonschulz(T, Ti, lin_tol, nullptr, false))) return st;
       if ((st = ops::mm(Ti, C0, P0, false))) return st;
       if ((st = ops::mm(Bm[0], P0, Tmp, false, 1.0, 1.0, &Cm[0]))) return st;
       if ((st = ops::mm(Ti, Tmp, Ps[0], false))) return st;

@@ -129,8 +129,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
   const int ld = g.ld;
-  const bool mirror = g.sym && (I != J);
-  const double wgt = mirror ? 2.0 : 1.0;
+  // Symmetric products: off-diagonal tiles are mirrored; inside a diagonal
+  // tile the upper half is mirrored onto the lower one, so that the output is
+  // exactly symmetric also for two-product sums (A0 B0 + A1 B1 accumulates its
+  // (r, c) and (c, r) entries in different orders).
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -140,6 +142,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int col = PRG_COL(J, n, r);
+        if (g.sym && row > col) continue;  // lower half of a diagonal tile
+        const bool mirror = g.sym && (row != col);
+        const double wgt = mirror ? 2.0 : 1.0;
         const size_t idx = row + (size_t)col * ld;
         double v = acc[m][n][r] * g.alpha;
         if (g.beta != 0.0) v += g.beta * g.Cin[idx];

@@ -11,6 +11,27 @@ int scale_cols_kernel_launch(const DMat& A, const double* d_f, DMat out);
 
 namespace ops {
 
+// Products issued through mm / mm2 since the last reset: count, algorithmic
+// flops (2 n^3 each) and MFMA flops executed (symmetric products compute the
+// upper-triangle 128 x 128 tiles only).
+namespace {
+double g_products = 0.0, g_alg_flops = 0.0, g_exec_flops = 0.0;
+
+void count_products(int nprod, const DMat& C, bool sym) {
+  const double n = C.n, ld = C.ld, nt = C.ld / kTile;
+  g_products += nprod;
+  g_alg_flops += nprod * 2.0 * n * n * n;
+  g_exec_flops += nprod * (sym ? nt * (nt + 1) / 2 * 2.0 * kTile * kTile * ld : 2.0 * ld * ld * ld);
+}
+}  // namespace
+
+void gemm_counters(double* products, double* alg_flops, double* exec_flops, bool reset) {
+  if (products) *products = g_products;
+  if (alg_flops) *alg_flops = g_alg_flops;
+  if (exec_flops) *exec_flops = g_exec_flops;
+  if (reset) g_products = g_alg_flops = g_exec_flops = 0.0;
+}
+
 int temp(const char* name, int n, DMat* out) {
   int st = OK;
   const int ld = padded(n);
@@ -46,6 +67,7 @@ int mm(const DMat& A, const DMat& B, DMat C, bool sym, double alpha, double beta
   if (st) return st;
   int nt = 0;
   if ((st = gemm(1, &A, &B, alpha, beta, Cin, gamma, C, sym ? 1 : 0, part, &nt, dotE))) return st;
+  count_products(1, C, sym);
   if (sums) {
     double h[4];
     if ((st = fetch(part, nt, 4, h))) return st;
@@ -64,6 +86,7 @@ int mm2(const DMat& A0, const DMat& B0, const DMat& A1, const DMat& B1, DMat C, 
   const DMat A[2] = {A0, A1}, B[2] = {B0, B1};
   int nt = 0;
   if ((st = gemm(2, A, B, alpha, beta, Cin, gamma, C, sym ? 1 : 0, part, &nt))) return st;
+  count_products(2, C, sym);
   if (sums) {
     double h[4];
     if ((st = fetch(part, nt, 4, h))) return st;
@@ -106,3 +129,13 @@ int scale_cols(const DMat& A, const double* d_f, DMat out) {
 
 }  // namespace ops
 }  // namespace prg
+
+extern "C" {
+
+// Counters of the FP64 products issued by the iterative solvers (implicit
+// Fermi, responses, SP2-Fermi, adjoint): measurement support.
+void prg_gpu_gemm_counters(int reset, double* products, double* alg_flops, double* exec_flops) {
+  prg::ops::gemm_counters(products, alg_flops, exec_flops, reset != 0);
+}
+
+}  // extern "C"

@@ -16,6 +16,10 @@ struct Sums {
   double dot = 0.0;
 };
 
+// Products issued through mm / mm2 since the last reset (count, 2 n^3 per
+// product, MFMA flops executed).
+void gemm_counters(double* products, double* alg_flops, double* exec_flops, bool reset);
+
 // n x n temporary held in a named workspace slot (contents undefined).
 int temp(const char* name, int n, DMat* out);
 // C = alpha A B + beta Cin + gamma I   (sym: product known symmetric)

@@ -135,6 +135,7 @@ def lib():
         "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
         "prg_gpu_step_timing": (i, [dp, ip]),
         "prg_gpu_int8_product_timing": (i, [dp, ip]),
+        "prg_gpu_gemm_counters": (None, [i, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -403,6 +404,14 @@ def step_timing(enable=None):
     cnt = C.c_int()
     lib().prg_gpu_step_timing(C.byref(ms), C.byref(cnt))
     return ms.value, cnt.value
+
+
+def gemm_counters(reset=False):
+    """(products, algorithmic flops, executed MFMA flops) of the FP64 products
+    the iterative solvers issued since the last reset."""
+    v = [C.c_double(), C.c_double(), C.c_double()]
+    lib().prg_gpu_gemm_counters(1 if reset else 0, *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def int8_product_timing():

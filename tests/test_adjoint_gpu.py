@@ -67,3 +67,33 @@ def test_sp2_adjoint_vs_response_and_fd():
         P.sp2_adjoint_device(hp, gp, out, n, pp, lo, hi, ck)
         outs.append(out.cpu().numpy())
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+
+
+def test_sp2_adjoint_nonsymmetric_seed():
+    """A non-symmetric dL/dD takes full products (the symmetric seed takes
+    upper-triangle tiles only).  The adjoint map is linear and commutes with
+    transposition (S(G^T) = S(G)^T for symmetric X_k), so for any B:
+    adj(B) + adj(B)^T = adj(B + B^T), the right side on the symmetric path."""
+    import torch
+    import progress_amd as P
+    n = 320
+    print(f"[adjoint non-symmetric seed] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 4)
+    ld = P.padded_ld(n)
+    hp = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    hp[:n, :n] = torch.from_numpy(h)
+    rd = torch.zeros_like(hp)
+    lo, hi = P.gershgorin_device(hp, n)
+    it, pp = P.sp2_alg2_device(hp, rd, n, 0.5, 1, 500, "Abs", 1e-10)
+    b = np.random.default_rng(5).standard_normal((n, n))
+    outs = []
+    for seed in (b, b + b.T):
+        gp = torch.zeros_like(hp)
+        gp[:n, :n] = torch.from_numpy(seed)
+        out = torch.zeros_like(hp)
+        P.sp2_adjoint_device(hp, gp, out, n, pp, lo, hi, 8)
+        outs.append(out[:n, :n].cpu().numpy())
+    g, gs = outs
+    assert np.linalg.norm(g - g.T) > 1e-3 * np.linalg.norm(g)  # genuinely non-symmetric
+    assert np.array_equal(gs, gs.T)                            # symmetric path is exact
+    assert np.linalg.norm(g + g.T - gs) <= 1e-12 * np.linalg.norm(gs)
