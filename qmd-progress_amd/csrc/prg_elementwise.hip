@@ -347,6 +347,39 @@ __device__ void sp2_plan(Sp2State* s) {
   }
 }
 
+// Fixed-order block sum of nb records of 4 partials (32 bytes each): one
+// pass over the records, wave shuffles, then the NT / 64 wave totals.
+__device__ void block_sum4(const double* part, int nb, double* out /*shared*/) {
+  __shared__ double red[NT / 64][4];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nb; b += NT) {
+    const double2 lo = *(const double2*)(part + 4 * (size_t)b);
+    const double2 hi = *(const double2*)(part + 4 * (size_t)b + 2);
+    a[0] += lo.x;
+    a[1] += lo.y;
+    a[2] += hi.x;
+    a[3] += hi.y;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_xor(a[q], off);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[w][q] = a[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NT / 64; ++ww) t += red[ww][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
 // Absorb the partials of the product just computed, then plan the next step.
 __global__ void sp2_absorb_plan_kernel(Sp2State* s, const double* __restrict__ part, int ntiles) {
   __shared__ double sums[4];
@@ -354,7 +387,7 @@ __global__ void sp2_absorb_plan_kernel(Sp2State* s, const double* __restrict__ p
   if (threadIdx.x == 0) planned = s->planned;
   __syncthreads();
   if (!planned) return;
-  block_sum(part, ntiles, 4, sums);
+  block_sum4(part, ntiles, sums);
   if (threadIdx.x != 0) return;
   const double fro2 = sums[0], trf = sums[1], idem = sums[2];
   s->x2_trace = sums[3];
@@ -423,6 +456,37 @@ __global__ void sp2_finish_kernel(const double* __restrict__ x0, const double* _
   for (int col = blockIdx.x; col < n; col += gridDim.x)
     for (int row = threadIdx.x; row < n; row += NT)
       out[row + (size_t)col * ld_out] = scale * x[row + (size_t)col * ld];
+}
+
+// Same as sp2_finish_kernel for an X whose lower triangle (and diagonal
+// 64 x 64 blocks) alone is current: 64 x 64 output tiles on or below the
+// diagonal are copied, the others are the transposes of their mirror tiles
+// (through LDS, padded rows).
+__global__ void sp2_finish_lower_kernel(const double* __restrict__ x0,
+                                        const double* __restrict__ x1,
+                                        const Sp2State* __restrict__ s, int ld, double scale,
+                                        double* __restrict__ out, int ld_out, int n) {
+  __shared__ double t[64][65];
+  const double* x = s->cur ? x1 : x0;
+  const int br = blockIdx.x * 64, bc = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+  if (br >= bc) {
+    for (int j = ty; j < 64; j += 4) {
+      const int row = br + tx, col = bc + j;
+      if (row < n && col < n) out[row + (size_t)col * ld_out] = scale * x[row + (size_t)col * ld];
+    }
+    return;
+  }
+  // out(br + i, bc + j) = x(bc + j, br + i)
+  for (int j = ty; j < 64; j += 4) {
+    const int row = bc + tx, col = br + j;
+    t[j][tx] = (row < n && col < n) ? x[row + (size_t)col * ld] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 64; j += 4) {
+    const int row = br + tx, col = bc + j;
+    if (row < n && col < n) out[row + (size_t)col * ld_out] = scale * t[tx][j];
+  }
 }
 
 int grid_cols(int ld) { return ld < 2048 ? ld : 2048; }
@@ -569,9 +633,16 @@ int sp2_begin_refine(Sp2State* d_state) {
   return OK;
 }
 
-int sp2_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out) {
-  hipLaunchKernelGGL(sp2_finish_kernel, dim3(grid_cols(out.n)), dim3(NT), 0, stream(), x0.p,
-                     x1.p, d_state, x0.ld, scale, out.p, out.ld, out.n);
+int sp2_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out,
+               bool lower_only) {
+  if (lower_only) {
+    const int nb = (out.n + 63) / 64;
+    hipLaunchKernelGGL(sp2_finish_lower_kernel, dim3(nb, nb), dim3(NT), 0, stream(), x0.p, x1.p,
+                       d_state, x0.ld, scale, out.p, out.ld, out.n);
+  } else {
+    hipLaunchKernelGGL(sp2_finish_kernel, dim3(grid_cols(out.n)), dim3(NT), 0, stream(), x0.p,
+                       x1.p, d_state, x0.ld, scale, out.p, out.ld, out.n);
+  }
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

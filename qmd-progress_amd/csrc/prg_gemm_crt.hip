@@ -23,7 +23,8 @@
 //    and stores ((C mod p) q mod p) as symmetric int8 values (64 KB per item);
 //  * crt_update_kernel: one workgroup per upper-triangle 64 x 64 block sums
 //    the fractions over the moduli in FP64, applies the SP2 branch update
-//    against the FP64 master X, writes the next master and its residue planes
+//    against the FP64 master X, writes the next master (lower triangle; the
+//    final copy of the solve restores the upper one) and its residue planes
 //    (both triangle positions) and emits the same trace / Frobenius partial
 //    sums as the FP64 kernel, so the FP64 bookkeeping is shared.
 // Items are independent, so the product grid is tiles x moduli (5808
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
 }
 
 // One upper-triangle 64 x 64 block (I, J) of X^2: sum the moduli's fractions,
-// apply the SP2 update, write X_next at (I, J) and (J, I), and the residue
+// apply the SP2 update, write X_next at (J, I) (the lower triangle), and the residue
 // planes of X_next (through LDS, one modulus at a time, as contiguous 4 KB
 // blocks in both positions).  The block lies in product tile (I / 4, J / 4),
 // in the share of product wave (I % 4) / 2 + 2 (J % 4), MFMA rows
@@ -341,14 +342,12 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
       z4[r] = z;
       zz[4 * n + r] = rint(z * g.c.scale);
     }
-    // (col0 + r, row): contiguous; on diagonal blocks these cover the block
+    // X_next at (col0 + r, row) only: the lower triangle, plus the whole of
+    // each diagonal block.  These are the positions the next step reads; the
+    // solve's final copy rebuilds the upper triangle by symmetry.
     double2* tp = (double2*)(Xn + col0 + (size_t)row * ld);
     tp[0] = double2{z4[0], z4[1]};
     tp[1] = double2{z4[2], z4[3]};
-    if (mirror) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Xn[row + (size_t)(col0 + r) * ld] = z4[r];
-    }
   }
   // Residue planes of X_next, one modulus at a time through a double-buffered
   // LDS block rb[row][col] (4 columns per dword).  Block (I, J) of a plane:

@@ -152,7 +152,10 @@ int sp2_response_step(const DMat& x0, const DMat& x1, const DMat& d0, const DMat
 // Device-side SP2 bookkeeping kernels.
 int sp2_init_state(Sp2State* d_state, const double* d_part, int nblocks);
 int sp2_absorb_plan(Sp2State* d_state, const double* d_part, int ntiles);
-int sp2_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out);
+// out = scale X[cur]; lower_only: only X's lower triangle (and its diagonal
+// 64 x 64 blocks) is current, the upper triangle is rebuilt by symmetry.
+int sp2_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out,
+               bool lower_only = false);
 // Switches a finished low-precision loop into its FP64 polishing phase.
 int sp2_begin_refine(Sp2State* d_state);
 

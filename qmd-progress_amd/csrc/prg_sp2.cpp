@@ -131,7 +131,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
                         p.variant == V_SUBMATRIX)
                            ? p.icount
                            : p.maxit + 2;
-  const int chunk = n >= 4096 ? 4 : 16;
+  const int chunk = n >= 4096 ? 8 : 16;
   const size_t head = offsetof(Sp2State, pp);
   int prec = p.precision;
   if (g_timing) crt_timing_begin();
@@ -177,7 +177,9 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
     }
     if (phase == 0) phase0_steps = steps;
   }
-  if ((st = sp2_finish(X0, X1, dst, 2.0, out))) return st;
+  // the int8 CRT update writes only the lower triangle of X between steps
+  const bool lower_only = (p.precision == PRG_PREC_INT8CRT) && !response;
+  if ((st = sp2_finish(X0, X1, dst, 2.0, out, lower_only))) return st;
   if (response && out1) {
     // rho1 = 2 Delta, Delta in the ping-pong slot named by the state
     if ((st = sp2_finish(D0, D1, dst, 2.0, *out1))) return st;
