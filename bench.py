@@ -38,9 +38,10 @@ def parse():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="auto",
-                    help="fp64 | f16split | int8crt | auto (fp64 and every mixed mode)")
-    ap.add_argument("--moduli", type=int, default=11,
-                    help="moduli of the int8 Chinese-remainder products (6..14)")
+                    help="fp64 | f16split | int8crt (every --moduli count) | int8crt<m> | auto (all)")
+    ap.add_argument("--moduli", default="10,11",
+                    help="comma-separated moduli counts of the int8 Chinese-remainder products "
+                         "(6..14); each runs as mode int8crt<m>")
     ap.add_argument("--cpu-iters", type=int, default=4,
                     help="SP2 iterations timed for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -125,12 +126,16 @@ def main():
     hpad[:n, :n] = h  # symmetric: row/column-major views coincide
     rho = torch.zeros_like(hpad)
 
-    modes = {"fp64": P.PREC_FP64, "f16split": P.PREC_F16SPLIT, "int8crt": P.PREC_INT8CRT}
+    moduli = [int(m) for m in str(args.moduli).split(",") if m]
+    modes = {"fp64": (P.PREC_FP64, 0), "f16split": (P.PREC_F16SPLIT, 0)}
+    for m in moduli:
+        modes[f"int8crt{m}"] = (P.PREC_INT8CRT, m)
     if args.precision == "auto":
-        run_modes = ["fp64", "f16split", "int8crt"]
+        run_modes = list(modes)
+    elif args.precision == "int8crt":
+        run_modes = [f"int8crt{m}" for m in moduli]
     else:
         run_modes = [args.precision]
-    P.set_int8_moduli(args.moduli)
 
     evals = evecs = None
     if not args.no_check:
@@ -145,7 +150,9 @@ def main():
 
     results = {}
     for name in run_modes:
-        mode = modes[name]
+        mode, nmod = modes[name]
+        if nmod:
+            P.set_int8_moduli(nmod)
         try:
             for _ in range(args.warmup):
                 P.sp2_alg2_device(hpad, rho, n, 0.5, 1, 200, "Abs", 1e-10, precision=mode)
@@ -180,14 +187,15 @@ def main():
             r["step_launches"] = cnt
             # the product kernel: the whole step for fp64 / f16split; the
             # int8 CRT product kernel alone (its FP64 update kernel apart)
-            kms = (pms_sum / pcnt) if (name == "int8crt" and pcnt) else ms_sum / cnt
+            kms = (pms_sum / pcnt) if (nmod and pcnt) else ms_sum / cnt
             r["product_kernel_ms"] = kms
-            r["product_kernel_launches"] = pcnt if name == "int8crt" else cnt
+            r["product_kernel_launches"] = pcnt if nmod else cnt
             r["product_kernel_fp64_equiv_tflops"] = 2.0 * float(n) ** 3 / (kms * 1e-3) / 1e12
-            ops = executed_ops(ld, name, args.moduli)
+            ops = executed_ops(ld, "int8crt" if nmod else name, nmod)
             r["product_kernel_mfma_ops"] = ops
             r["product_kernel_mfma_tops"] = ops / (kms * 1e-3) / 1e12
-            if name == "int8crt":
+            if nmod:
+                r["moduli"] = nmod
                 r["update_kernel_ms"] = r["step_ms"] - kms
                 r["int8_config"] = P.int8_waves()
         if evals is not None:
@@ -201,8 +209,10 @@ def main():
     ok = [k for k, v in results.items() if "solves_per_s" in v and v.get("meets_1e-6", True)]
     head = max(ok, key=lambda k: results[k]["solves_per_s"]) if ok else "fp64"
     hr = results.get(head, {})
+    kind = "int8crt" if head.startswith("int8crt") else head
+    hmod = hr.get("moduli")
     peak_tf = {"fp64": PEAK["fp64_tflops"], "f16split": PEAK["fp16_tflops"],
-               "int8crt": PEAK["int8_tops"]}[head]
+               "int8crt": PEAK["int8_tops"]}[kind]
     kms = hr.get("product_kernel_ms")
     alg = (2.0 * float(n) ** 3 / (kms * 1e-3) / 1e12) if kms else None
     kern = hr.get("product_kernel_mfma_tops")
@@ -210,13 +220,13 @@ def main():
         "bound": "mfma",
         "kernel": {"fp64": "sp2_step_kernel (FP64 MFMA, fused SP2 update)",
                    "f16split": "sp2_f16split_kernel (FP16 MFMA, 3 products)",
-                   "int8crt": f"crt_product_kernel (int8 MFMA, {args.moduli} moduli x "
-                              "upper-triangle 256x256 tiles)"}[head],
+                   "int8crt": f"crt_product_kernel (int8 MFMA, {hmod} moduli x "
+                              "upper-triangle 256x256 tiles)"}[kind],
         "achieved": alg,
         "peak": peak_tf,
         "unit": "TFLOP/s",
         "frac": (alg / peak_tf) if alg else None,
-        "traffic": traffic_bytes(head, n, args.moduli),
+        "traffic": traffic_bytes(kind, n, hmod),
         "executed": kern,
         "executed_frac": (kern / peak_tf) if kern else None,
         "note": "achieved = algorithmic 2N^3 flop of the X^2 the launch produces / the "
@@ -245,11 +255,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "f16split": "f16 (hi+lo split), f64 bookkeeping",
-                  "int8crt": "int8 residues (exact), f64 reconstruction"}[head],
+                  "int8crt": "int8 residues (exact), f64 reconstruction"}[kind],
         "data": "synthetic",
         "config": {"workload": f"prg_sp2_alg2 dense model tight-binding H, N={n}, occ=N/2, "
                                f"idemtol=1e-10, seed={args.seed}",
-                   "n": n, "headline_mode": head, "gap_eV": gap,
+                   "n": n, "headline_mode": head, "moduli": hmod, "gap_eV": gap,
                    "eigh_s": t_eigh},
         "tflops": hr.get("alg_tflops"),
         "modes": results,
@@ -281,7 +291,7 @@ def graph_bench(args, P, torch, np):
     hd[torch.from_numpy(rows).to(dev), torch.from_numpy(cols).to(dev)] = \
         torch.from_numpy(vals).to(dev)
     rd = torch.zeros_like(hd)
-    P.set_int8_moduli(args.moduli)
+    P.set_int8_moduli(max(int(m) for m in str(args.moduli).split(",") if m))
     t0 = time.time()
     it, pp = P.sp2_alg2_device(hd, rd, n, 0.5, 1, 200, "Abs", 1e-10, precision=P.PREC_INT8CRT)
     torch.cuda.synchronize()

@@ -318,8 +318,8 @@ void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scat
 void prg_set_precision(int mode);
 int prg_get_precision(void);
 /* Number of moduli (6..14, default 11) of PRG_PREC_INT8CRT; X is scaled by
- * 2^a with a = floor((log2 M - 3) / 2): 8 -> 30, 9 -> 34, 10 -> 38, 11 -> 42,
- * 12 -> 45, 13 -> 49, 14 -> 53 bits.  Returns a status. */
+ * 2^a with a = min(52, floor((log2 M - 1.001) / 2)): 8 -> 31, 9 -> 35,
+ * 10 -> 39, 11 -> 43, 12 -> 46, 13 -> 50, 14 -> 52 bits.  Returns a status. */
 int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 /* Configuration the int8 CRT product kernel last ran with, coded

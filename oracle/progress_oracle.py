@@ -1047,11 +1047,9 @@ def crt_constants(nmod):
     for p in ps:
         m *= p
     qs = [pow(m // p % p, -1, p) for p in ps]
-    a = (m.bit_length() - 1 - 3) // 2
-    while 4 ** (a + 1) <= m // 8:
-        a += 1
-    while 4 ** a > m // 8:
-        a -= 1
+    # largest a <= 52 with 2^(2a + 1.001) <= M: |C| <= 2^(2a) (1 + 1e-10) < M / 2
+    # for rows of X of 2-norm at most 1
+    a = min(52, int(math.floor((math.log2(m) - 1.001) / 2)))
     return ps, qs, a, m
 
 

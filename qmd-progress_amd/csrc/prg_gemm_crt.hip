@@ -3,7 +3,7 @@
 // PRG_PREC_INT8CRT.  X is symmetric with its spectrum in [0, 1], so every row
 // has ||X_i||_2 <= 1.  With A = rint(2^a X) (an integer matrix) every entry of
 // the integer product C = A A^T obeys |C_ij| <= ||A_i|| ||A_j|| < M/2 when
-// 2^(2a) <= M/8, M being the product of the moduli below.  C is then fixed by
+// 2^(2a + 1.001) <= M, M being the product of the moduli below.  C is then fixed by
 // its residues modulo each p_k (Chinese remainder theorem), and each residue
 // is one exact int8 product:
 //   C mod p_k = (A mod p_k)(A mod p_k)^T mod p_k,
@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 
@@ -171,67 +172,77 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
   const size_t kstride = (size_t)pld * BK;  // one 64-column block of a plane
   const uint32_t dst0 = lds_addr(&ring[0][0]) + 1024 * w;
   const int ksteps = pld / BK;
-  auto issue = [&](int s) {
-    const size_t off = (size_t)s * kstride;
-    const uint32_t base = __builtin_amdgcn_readfirstlane(dst0 + kStage * (s % NSTAGE));
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-      const int panel = (W * q) >> 4;
-      const int rows = 16 * W * q - TM * panel;
-      dma16((panel ? srcB : srcA) + off + rows * BK, base + 1024 * W * q);
-    }
-  };
   // LDS read offsets of this lane's fragments: row r = lane % 16 of each
   // 16-row block, 16-byte chunk lane / 16; the block offsets are multiples
   // of 8 rows, so the swizzle depends on the lane only.
   const int fo = (lane & 15) * BK + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
   const int offa = wm * 128 * BK + fo;
-  const int offb = kPanel + wn * (TM / WN) * BK + fo;
   i4 acc[MT][NTL];
-  i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
-  auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
-    const int8_t* st = ring[s % NSTAGE];
+  // DIAG (I == J): both operands are the same panel, which is loaded once
+  // (half the DMA chunks; the B fragments are read from the A panel).
+  auto run = [&](auto diag_tag) {
+    constexpr bool DIAG = decltype(diag_tag)::value;
+    constexpr int NQ = DIAG ? CPW / 2 : CPW;  // DMA chunks per wave per stage
+    const int offb = (DIAG ? 0 : kPanel) + wn * (TM / WN) * BK + fo;
+    auto issue = [&](int s) {
+      const size_t off = (size_t)s * kstride;
+      const uint32_t base = __builtin_amdgcn_readfirstlane(dst0 + kStage * (s % NSTAGE));
 #pragma unroll
-    for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + offa + 1024 * m);
+      for (int q = 0; q < NQ; ++q) {
+        const int panel = (W * q) >> 4;
+        const int rows = 16 * W * q - TM * panel;
+        dma16((panel ? srcB : srcA) + off + rows * BK, base + 1024 * W * q);
+      }
+    };
+    i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
+    auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
+      const int8_t* st = ring[s % NSTAGE];
 #pragma unroll
-    for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb + 1024 * n);
-  };
-  // Iteration s: wait until stage s + 1 has landed in every wave (and this
-  // wave's reads of stage s are done), refill the slot of stage s with stage
-  // s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.  The
-  // first step starts from zero accumulators (peeled, so that the loop
-  // carries only MFMA results and they stay in AGPRs).
-  auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
-    if (s + NSTAGE - 1 < ksteps)
-      wait_vm_barrier<CPW * (NSTAGE - 2)>();
+      for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + offa + 1024 * m);
+#pragma unroll
+      for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb + 1024 * n);
+    };
+    // Iteration s: wait until stage s + 1 has landed in every wave (and this
+    // wave's reads of stage s are done), refill the slot of stage s with
+    // stage s + NSTAGE, then run the MFMAs of stage s while reading stage
+    // s + 1.  The first step starts from zero accumulators (peeled, so that
+    // the loop carries only MFMA results and they stay in AGPRs).
+    auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
+      if (s + NSTAGE - 1 < ksteps)
+        wait_vm_barrier<NQ * (NSTAGE - 2)>();
+      else
+        wait_vm_barrier<0>();
+      if (s + NSTAGE < ksteps) issue(s + NSTAGE);
+      // after the last stage this reads a stale slot that is never used
+      read_frags(s + 1, next);
+      const i4 zero = {0, 0, 0, 0};
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NTL; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
+                                                            first ? zero : acc[m][n], 0, 0, 0);
+    };
+#pragma unroll
+    for (int s = 0; s < NSTAGE; ++s)
+      if (s < ksteps) issue(s);
+    if (NSTAGE - 1 < ksteps)
+      wait_vm_barrier<NQ * (NSTAGE - 1)>();
     else
       wait_vm_barrier<0>();
-    if (s + NSTAGE < ksteps) issue(s + NSTAGE);
-    // after the last stage this reads a stale slot that is never used
-    read_frags(s + 1, next);
-    const i4 zero = {0, 0, 0, 0};
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int n = 0; n < NTL; ++n)
-        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
-                                                          first ? zero : acc[m][n], 0, 0, 0);
+    read_frags(0, fa);
+    // ksteps = pld / 64 is a multiple of 4
+    step(0, fa, fb, true);
+    step(1, fb, fa, false);
+    for (int s = 2; s < ksteps; s += 2) {
+      step(s, fa, fb, false);
+      step(s + 1, fb, fa, false);
+    }
   };
-#pragma unroll
-  for (int s = 0; s < NSTAGE; ++s)
-    if (s < ksteps) issue(s);
-  if (NSTAGE - 1 < ksteps)
-    wait_vm_barrier<CPW * (NSTAGE - 1)>();
+  if (I == J)
+    run(std::true_type{});
   else
-    wait_vm_barrier<0>();
-  read_frags(0, fa);
-  // ksteps = pld / 64 is a multiple of 4
-  step(0, fa, fb, true);
-  step(1, fb, fa, false);
-  for (int s = 2; s < ksteps; s += 2) {
-    step(s, fa, fb, false);
-    step(s + 1, fb, fa, false);
-  }
+    run(std::false_type{});
 
   // (C q mod p) as symmetric int8.  Operands were swapped, so this lane holds
   // C[row][col] with row = lane % 16 and col = 4 (lane / 16) + r inside each
@@ -463,8 +474,11 @@ CrtConst make_const(int np) {
     c.inv[k] = 1.0 / (double)p;
     c.q[k] = (double)inv_mod(mk, p);
   }
-  // 2^(2a) <= M / 8 keeps |C| < M / 2 for rows of 2-norm up to sqrt(2)
-  const int a = (int)std::floor((log2m - 3.0) / 2.0);
+  // |C_ij| <= ||A_i|| ||A_j|| <= 2^(2a) (1 + 1e-10) for rows of X of 2-norm
+  // at most 1 (the spectrum of every SP2 iterate lies in [0, 1]; the
+  // rounding of A adds at most sqrt(ld) / 2 to a row norm).  2^(2a + 1.001)
+  // <= M keeps |C| < M / 2; a <= 52 (X carries 53 bits).
+  const int a = std::min(52, (int)std::floor((log2m - 1.001) / 2.0));
   c.scale = std::ldexp(1.0, a);
   c.cscale = std::exp2(log2m - 2.0 * a);
   return c;

@@ -474,7 +474,7 @@ def test_sp2_inverse_round_trip_and_entropy_quadrature():
 @pytest.mark.parametrize("nmod", [8, 11, 14])
 def test_crt_square_reproduces_x_squared(nmod):
     """The int8 Chinese-remainder product: for X with spectrum in [0, 1] the
-    reconstruction equals X^2 up to the rounding of X to 2^-a (a = 30, 42, 53),
+    reconstruction equals X^2 up to the rounding of X to 2^-a (a = 31, 43, 52),
     i.e. |error| <= 2^-a (||X_i||_1 + ||X_j||_1) / 2 + FP64 noise."""
     rng = np.random.default_rng(nmod)
     n = 200
@@ -482,7 +482,9 @@ def test_crt_square_reproduces_x_squared(nmod):
     x = (q * rng.uniform(0.0, 1.0, n)) @ q.T
     x = 0.5 * (x + x.T)
     ps, qs, a, m = O.crt_constants(nmod)
-    assert 4 ** a <= m // 8 < 4 ** (a + 1)
+    assert a == {8: 31, 11: 43, 14: 52}[nmod]
+    assert 2 * 4 ** a * 2.0 ** 0.001 <= m
+    assert a == 52 or 2 * 4 ** (a + 1) * 2.0 ** 0.001 > m
     got = O.crt_square(x, nmod)
     ref = x @ x
     l1 = np.abs(x).sum(axis=1)
@@ -536,3 +538,25 @@ def test_orthogonalize_round_trip_with_lowdin_factor():
     a = a + a.T
     sinv = np.linalg.inv(s)
     assert np.allclose(O.deorthogonalize(O.orthogonalize(a, z), z), sinv @ a @ sinv, atol=1e-10)
+
+
+@pytest.mark.parametrize("nmod", [8, 10, 11, 13])
+def test_crt_range_bound_is_tight(nmod):
+    """The largest product the bound admits: X = e_1 e_1^T (a row of 2-norm 1,
+    C_11 = 2^(2a)) reconstructs to FP64 rounding of the fraction sum; one more
+    bit of scale (a + 1) makes C_11 exceed M / 2 and the reconstruction wraps
+    (a planted mistake)."""
+    n = 8
+    x = np.zeros((n, n))
+    x[0, 0] = 1.0
+    assert np.max(np.abs(O.crt_square(x, nmod) - x)) < 1e-12
+    orig = O.crt_constants
+
+    def one_more_bit(k):
+        ps, qs, a, m = orig(k)
+        return ps, qs, a + 1, m
+    O.crt_constants = one_more_bit
+    try:
+        assert abs(O.crt_square(x, nmod)[0, 0] - 1.0) > 0.5
+    finally:
+        O.crt_constants = orig
