@@ -3,10 +3,10 @@
 //
 // The partition structure is a flat C struct (include/progress_gpu.h) standing
 // in for the reference's Fortran graph_partitioning_t.  Core+halo index lists
-// are extracted on the host from the graph matrix (as the reference's OpenMP
-// loop does); all SP2 work -- gather and normalisation of every part, the
-// lock-step purification of all parts, and the scatter of the core rows into
-// the density -- runs on the GPU as batched kernels.
+// are extracted on the GPU from the graph matrix (prg_halo.hip); all SP2 work
+// -- gather and normalisation of every part, the lock-step purification of
+// all parts, and the scatter of the core rows into the density -- runs on the
+// GPU as batched kernels.
 #include "prg_graph.h"
 
 #include <algorithm>
@@ -63,59 +63,14 @@ float g_ms[3] = {0.f, 0.f, 0.f};
 // bml_matrix2submatrix_index as called at src/prg_subgraphloop_mod.F90:60-63:
 // core nodes first (given order), then every node adjacent in G (nonzero,
 // off-diagonal entry of the core node's row) in order of discovery, then with
-// the double-jump flag the neighbours of those halo nodes.
+// the double-jump flag the neighbours of those halo nodes.  The lists are
+// built on the GPU from G's device copy (halo_indices_device).
 int halo_indices(bml_matrix_t* g_bml, prg_graph_partitioning_t* gp, int djflag) {
-  int st;
-  if ((st = bml_host_sync(g_bml))) return st;
-  const int N = g_bml->N;
-  const double* G = g_bml->host;
-  // row adjacency lists (ascending column order), built column by column
-  std::vector<std::vector<int>> adj(N);
-  for (int j = 0; j < N; ++j) {
-    const double* col = G + (size_t)j * N;
-    for (int i = 0; i < N; ++i)
-      if (i != j && col[i] != 0.0) adj[i].push_back(j);
-  }
-  std::vector<char> seen(N, 0);
-  std::vector<int> list;
-  for (int p = 0; p < gp->totalParts; ++p) {
-    list.clear();
-    const int nc = gp->nnodesInPart[p];
-    for (int k = 0; k < nc; ++k) {
-      const int c = gp->nodeInPart[p][k];
-      if (c < 0 || c >= N) {
-        set_error(ERR_ARG, "graph partitioning: node index out of range");
-        return ERR_ARG;
-      }
-      if (!seen[c]) {
-        seen[c] = 1;
-        list.push_back(c);
-      }
-    }
-    const size_t ncore = list.size();
-    for (size_t k = 0; k < ncore; ++k)
-      for (int nb : adj[list[k]])
-        if (!seen[nb]) {
-          seen[nb] = 1;
-          list.push_back(nb);
-        }
-    if (djflag) {
-      const size_t nh = list.size();
-      for (size_t k = ncore; k < nh; ++k)
-        for (int nb : adj[list[k]])
-          if (!seen[nb]) {
-            seen[nb] = 1;
-            list.push_back(nb);
-          }
-    }
-    for (int v : list) seen[v] = 0;
-    std::free(gp->core_halo_index[p]);
-    gp->core_halo_index[p] = (int*)std::malloc(sizeof(int) * std::max<size_t>(1, list.size()));
-    std::memcpy(gp->core_halo_index[p], list.data(), sizeof(int) * list.size());
-    gp->lsize[p] = (int)list.size();
-    gp->llsize[p] = (int)ncore;
-  }
-  return OK;
+  int st = ensure_device();
+  if (st) return st;
+  DMat G;
+  if ((st = bml_dev_read(g_bml, &G))) return st;
+  return halo_indices_device(G, gp, djflag);
 }
 
 // Packed batch of all parts: largest padded size first, each part m x m with

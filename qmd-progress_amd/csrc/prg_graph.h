@@ -3,6 +3,7 @@
 #pragma once
 
 #include "prg_internal.h"
+#include "../../include/progress_gpu.h"
 
 namespace prg {
 
@@ -22,6 +23,11 @@ struct BatchPlan {
   int4* d_tiles = nullptr;
   int* d_idx = nullptr;
 };
+
+// Core+halo lists of every part from the device graph G (nonzero off-diagonal
+// entries are edges), in the reference's discovery order; fills
+// core_halo_index, lsize and llsize of gp (prg_halo.hip).
+int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag);
 
 int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double mineval,
                    double maxeval);

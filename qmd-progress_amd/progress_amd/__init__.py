@@ -643,12 +643,14 @@ def set_partition(parts, nnodes):
     return gp
 
 
-def subgraph_sp2_loop(h, g, gp, pp, max_iter, mineval, maxeval, vv=None):
+def subgraph_sp2_loop(h, g, gp, pp, max_iter, mineval, maxeval, vv=None, out=None):
     """prg_subgraphSP2Loop: returns (rho, vv[:max_iter]); gp is filled with
-    the core+halo lists."""
-    n = h.shape[0]
-    mh, mg = Matrix.from_numpy(h), Matrix.from_numpy(g)
-    rho = Matrix.zeros(n)
+    the core+halo lists.  h and g are numpy arrays or Matrix handles (whose
+    device copies stay resident across calls); with `out` (a Matrix) the
+    density stays in that handle and is returned as the handle."""
+    mh = h if isinstance(h, Matrix) else Matrix.from_numpy(h)
+    mg = g if isinstance(g, Matrix) else Matrix.from_numpy(g)
+    rho = out if out is not None else Matrix.zeros(mh.n)
     gp.maxIter = max_iter
     for k, v in enumerate(pp):
         gp.pp[k] = int(v)
@@ -657,7 +659,7 @@ def subgraph_sp2_loop(h, g, gp, pp, max_iter, mineval, maxeval, vv=None):
     gp.mineval, gp.maxeval = mineval, maxeval
     lib().prg_subgraphSP2Loop(mh.h, mg.h, rho.h, C.byref(gp), 0.0)
     _check()
-    return rho.numpy(), [gp.vv[k] for k in range(max_iter)]
+    return (rho if out is not None else rho.numpy()), [gp.vv[k] for k in range(max_iter)]
 
 
 def sp2_fermi_init(h, nsteps, nocc, tscale, occ_err_limit, trace_limit, norecs=False,
