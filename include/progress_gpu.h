@@ -96,6 +96,14 @@ void prg_gershgorin(bml_matrix_t *h_bml, double *gbnd);
 void prg_sp2_basic(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, double bndfil,
                    int minsp2iter, int maxsp2iter, const char *sp2conv, double idemtol,
                    int verbose);
+/* src/prg_c_interface.F90:1091-1111 (prg_sp2_mod.F90:138-246; prototype
+ * src/prg_progress_mod.h:503-513): the basic branch rule for exactly
+ * maxsp2iter + 1 iterations (no convergence exit, as in the reference), D = 2X.
+ * rhofull is read only for the verbose > 1 diagnostic (the final
+ * ||rhofull/2 - X||_F / ||rhofull||_F); it may be NULL. */
+void prg_sp2_basic_tcore(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, bml_matrix_t *rhofull_bml,
+                         double threshold, double bndfil, int minsp2iter, int maxsp2iter,
+                         const char *sp2conv, double idemtol, int verbose);
 /* src/prg_c_interface.F90:1113-1129 (prg_sp2_mod.F90:263-380) */
 void prg_sp2_alg2(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, double bndfil,
                   int minsp2iter, int maxsp2iter, const char *sp2conv, double idemtol,

@@ -175,6 +175,27 @@ def sp2_basic(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
     raise NotConverging("unreachable")
 
 
+def sp2_basic_tcore(h, bndfil, maxsp2iter):
+    """prg_sp2_basic_tcore (src/prg_sp2_mod.F90:138-246): the prg_sp2_basic
+    branch rule (fresh trace of X each iteration, :177, :198-212) for
+    iterations 0..maxsp2iter with the convergence exit commented out (:214);
+    D = 2X (:241).  Returns (rho, pp) with pp the branch sequence taken."""
+    n = h.shape[0]
+    occ = bndfil * n
+    x = normalize(h)
+    pp = []
+    for _ in range(0, maxsp2iter + 1):
+        trx = np.trace(x)
+        x2 = x @ x
+        if trx - occ <= 0.0:
+            x = 2.0 * x - x2
+            pp.append(0)
+        else:
+            x = x2
+            pp.append(1)
+    return 2.0 * x, pp
+
+
 def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace, want_vv):
     """Shared body of prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380) and
     prg_sp2_alg2_genseq (:395-533).  ``fresh_trace`` selects genseq's

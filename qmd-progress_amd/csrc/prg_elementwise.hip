@@ -308,7 +308,11 @@ __device__ void sp2_plan(Sp2State* s) {
       if (!s->planned) s->done = 1;
       break;
     }
-    case V_BASIC: {
+    case V_BASIC:
+    case V_BASIC_FIXED: {
+      // V_BASIC_FIXED (src/prg_sp2_mod.F90:138-246): the same branch rule run
+      // for iterations 0..maxit with the convergence exit commented out
+      // (:214) and only a message at maxit (:216-219)
       const int it = s->iter;  // reference loop index, starting at 0
       if (it > s->maxit) { s->done = 1; return; }
       const double trx = s->trx_fresh;
@@ -320,7 +324,8 @@ __device__ void sp2_plan(Sp2State* s) {
         s->sigma = 1;
         s->trx_next = s->fro2;
       }
-      int last = (fabs(occ - s->trx_next) < tol && it > s->minit);
+      int last = (v == V_BASIC_FIXED) ? (it == s->maxit)
+                                      : (fabs(occ - s->trx_next) < tol && it > s->minit);
       if (!last && it == s->maxit) {
         s->status = ERR_NOT_CONVERGED;
         last = 1;
@@ -400,7 +405,7 @@ __global__ void sp2_absorb_plan_kernel(Sp2State* s, const double* __restrict__ p
     if (idx < kMaxIter) s->vv[idx] += idem;
   }
   double trx_old = (v == V_ALG2_GENSEQ) ? s->trx_fresh : s->trx;
-  if (v == V_BASIC) trx_old = s->trx_fresh;
+  if (v == V_BASIC || v == V_BASIC_FIXED) trx_old = s->trx_fresh;
   if (s->apply) {
     if (v != V_SEQ_ALG2 && v != V_SEQ_ALG1 && v != V_SUBMATRIX && idx < kMaxIter)
       s->pp[idx] = s->sigma;

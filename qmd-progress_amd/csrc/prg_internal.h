@@ -77,6 +77,7 @@ enum Sp2Variant : int {
   V_SEQ_ALG1 = 6,     // prg_sp2_alg1_seq / prg_prg_sp2_alg1_seq_inplace
   V_SUBMATRIX = 7,    // prg_sp2_submatrix / _inplace (vv accumulates core rows)
   V_RESPONSE = 8,     // prg_compute_response_SP2 (alg2 control flow)
+  V_BASIC_FIXED = 9,  // prg_sp2_basic_tcore (basic branch rule, maxit + 1 steps, no early stop)
 };
 
 // Device-resident SP2 bookkeeping.  All FP64 scalar work of the loop (traces,

@@ -6,6 +6,7 @@
 // scalar crosses PCIe per iteration.  Once the loop has stopped, the
 // remaining launches of a chunk exit at their first instruction.
 #include "prg_sp2.h"
+#include "prg_ops.h"
 
 #include <algorithm>
 #include <cmath>
@@ -338,6 +339,29 @@ void prg_sp2_basic(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold,
   (void)threshold;
   sp2_h_to_rho("prg_sp2_basic", V_BASIC, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
                idemtol, verbose, nullptr, nullptr, nullptr, 0);
+}
+
+// src/prg_c_interface.F90:1091-1111 (Fortran src/prg_sp2_mod.F90:138-246): the
+// basic branch rule for exactly maxsp2iter + 1 iterations (the reference
+// comments its convergence exit out), D = 2X.  rhofull is the reference
+// density its per-iteration printout compares against; here, with
+// verbose > 1, the final ||rhofull/2 - X||_F / ||rhofull||_F is printed.
+void prg_sp2_basic_tcore(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, bml_matrix_t* rhofull_bml,
+                         double threshold, double bndfil, int minsp2iter, int maxsp2iter,
+                         const char* sp2conv, double idemtol, int verbose) {
+  (void)threshold;
+  sp2_h_to_rho("prg_sp2_basic_tcore", V_BASIC_FIXED, h_bml, rho_bml, bndfil, minsp2iter,
+               maxsp2iter, sp2conv, idemtol, verbose, nullptr, nullptr, nullptr, 0);
+  if (prg_last_error_code() != OK || verbose <= 1 || !rhofull_bml) return;
+  if (check_pair(rho_bml, rhofull_bml, "prg_sp2_basic_tcore")) return;
+  DMat R, F, T;
+  if (bml_dev_read(rho_bml, &R) || bml_dev_read(rhofull_bml, &F) ||
+      ops::temp("tcore_diff", R.n, &T))
+    return;
+  ops::Sums d, f;
+  if (ops::lin(F, 0.5, &R, -0.5, 0.0, T, &d) || ops::sums_of(F, &f)) return;
+  std::printf("prg_sp2_basic_tcore: ||rhofull/2 - X||_F / ||rhofull||_F = %.6e\n",
+              f.fro2 > 0.0 ? std::sqrt(d.fro2 / f.fro2) : 0.0);
 }
 
 void prg_sp2_alg2(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,

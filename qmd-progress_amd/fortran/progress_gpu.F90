@@ -212,7 +212,8 @@ module prg_sp2_mod
   implicit none
   private
   integer, parameter :: dp = kind(1.0d0)
-  public :: prg_sp2_basic, prg_sp2_alg1, prg_sp2_alg2, prg_sp2_alg2_genseq, prg_sp2_alg1_genseq
+  public :: prg_sp2_basic, prg_sp2_basic_tcore, prg_sp2_alg1, prg_sp2_alg2
+  public :: prg_sp2_alg2_genseq, prg_sp2_alg1_genseq
   public :: prg_sp2_alg2_seq, prg_sp2_alg1_seq, prg_prg_sp2_alg2_seq_inplace
   public :: prg_prg_sp2_alg1_seq_inplace, prg_sp2_submatrix, prg_sp2_submatrix_inplace
 
@@ -234,6 +235,14 @@ module prg_sp2_mod
     subroutine c_sp2_basic(h, r, thr, bndfil, mn, mx, conv, tol, v) bind(C, name="prg_sp2_basic")
       import :: c_ptr, c_double, c_int, c_char
       type(c_ptr), value :: h, r
+      real(c_double), value :: thr, bndfil, tol
+      integer(c_int), value :: mn, mx, v
+      character(kind=c_char) :: conv(*)
+    end subroutine
+    subroutine c_sp2_basic_tcore(h, r, rf, thr, bndfil, mn, mx, conv, tol, v) &
+         bind(C, name="prg_sp2_basic_tcore")
+      import :: c_ptr, c_double, c_int, c_char
+      type(c_ptr), value :: h, r, rf
       real(c_double), value :: thr, bndfil, tol
       integer(c_int), value :: mn, mx, v
       character(kind=c_char) :: conv(*)
@@ -328,6 +337,19 @@ contains
     call c_sp2_basic(h_bml%ptr, rho_bml%ptr, threshold, bndfil, minsp2iter, maxsp2iter, &
          cstr(sp2conv), idemtol, vb(verbose))
   end subroutine prg_sp2_basic
+
+  !> src/prg_sp2_mod.F90:138-246
+  subroutine prg_sp2_basic_tcore(h_bml, rho_bml, rhofull_bml, threshold, bndfil, minsp2iter, &
+       maxsp2iter, sp2conv, idemtol, verbose)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml, rhofull_bml
+    real(dp), intent(in) :: threshold, bndfil, idemtol
+    integer, intent(in) :: minsp2iter, maxsp2iter
+    character(len=*), intent(in) :: sp2conv
+    integer, optional, intent(in) :: verbose
+    call c_sp2_basic_tcore(h_bml%ptr, rho_bml%ptr, rhofull_bml%ptr, threshold, bndfil, &
+         minsp2iter, maxsp2iter, cstr(sp2conv), idemtol, vb(verbose))
+  end subroutine prg_sp2_basic_tcore
 
   !> src/prg_sp2_mod.F90:263-380
   subroutine prg_sp2_alg2(h_bml, rho_bml, threshold, bndfil, minsp2iter, maxsp2iter, sp2conv, &

