@@ -104,6 +104,15 @@ void prg_sp2_basic(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold,
 void prg_sp2_basic_tcore(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, bml_matrix_t *rhofull_bml,
                          double threshold, double bndfil, int minsp2iter, int maxsp2iter,
                          const char *sp2conv, double idemtol, int verbose);
+/* src/sp2_tensorcore/src/prg_sp2_tensorcore.cu:131-141: the reference's
+ * mixed-precision SP2 entry (host float H, host double D = X, not 2X).  Runs
+ * this library's FP16 hi+lo split SP2 with the reference's sigma rule,
+ * noise-floor stop and FP64 refinement, Nocc = int(bndfil N).  Differences:
+ * Gershgorin bounds instead of the reference's hard-coded (-27.55, 25.0); up
+ * to two FP64 SP2 polishing steps in place of its single 2T^2 - T^4.  eps,
+ * minsp2iter, sp2conv and idemtol are accepted and, as there, not used. */
+void prg_sp2_tensorcore(int N, float *H, double *D, float eps, float bndfil, int minsp2iter,
+                        int maxsp2iter, char sp2conv, float idemtol, int verbose);
 /* src/prg_c_interface.F90:1113-1129 (prg_sp2_mod.F90:263-380) */
 void prg_sp2_alg2(bml_matrix_t *h_bml, bml_matrix_t *rho_bml, double threshold, double bndfil,
                   int minsp2iter, int maxsp2iter, const char *sp2conv, double idemtol,

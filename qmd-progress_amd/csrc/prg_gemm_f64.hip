@@ -283,7 +283,6 @@ __global__ __launch_bounds__(NT, 2) void sp2_response_kernel(Sp2Args g) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
   const int ld = g.ld;
-  const bool mirror = (I != J);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
 #pragma unroll
@@ -292,10 +291,13 @@ __global__ __launch_bounds__(NT, 2) void sp2_response_kernel(Sp2Args g) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int col = PRG_COL(J, n, r);
+        // X D + D X accumulates its (r, c) and (c, r) entries in different
+        // orders: inside a diagonal tile the upper half is mirrored
+        if (row > col) continue;
         const size_t idx = row + (size_t)col * ld;
         const double z = fma(cb, D[idx], ca * acc[m][n][r]);
         Dn[idx] = z;
-        Dn[mirror ? col + (size_t)row * ld : idx] = z;
+        Dn[col + (size_t)row * ld] = z;
       }
       __builtin_amdgcn_sched_barrier(0);
     }

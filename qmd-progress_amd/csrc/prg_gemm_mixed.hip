@@ -130,10 +130,11 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
 
   // Epilogue.  Operands were swapped, so this lane holds (X^2)[row][col]
   // with row = lane%16 and col = 4*(lane/16) + r within each 16 x 16 block.
-  const bool mirror = (I != J);
+  // Off-diagonal tiles are mirrored; inside a diagonal tile the upper half is
+  // mirrored onto the lower one (the hi.lo and lo.hi terms of (r, c) and
+  // (c, r) accumulate in different orders), so X_next is exactly symmetric.
   const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
   const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
-  const double wz = mirror ? 2.0 : 1.0;
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -144,6 +145,9 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int col = J * BM + wn * 64 + n * 16 + 4 * (lane >> 4) + r;
+        if (row > col) continue;  // lower half of a diagonal tile
+        const bool mirror = (row != col);
+        const double we = mirror ? 2.0 : 1.0;
         const size_t idx = row + (size_t)col * ld;
         const size_t tdx = mirror ? col + (size_t)row * ld : idx;
         const double y = (double)acc_hh[m][n][r] + 0x1p-11 * (double)acc_x[m][n][r];
@@ -151,8 +155,8 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
         const double z = fma(cb, x, ca * y);
         const double dxy = x - y;
         const bool diag = (row == col);
-        q0 += z * z;
-        q2 += dxy * dxy;
+        q0 += we * z * z;
+        q2 += we * dxy * dxy;
         q1 += diag ? z : 0.0;
         q3 += diag ? y : 0.0;
         const _Float16 h = (_Float16)z;
@@ -164,9 +168,9 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
         lon[idx] = l;
         lon[tdx] = l;
       }
-      sums[0] += wz * q0;
+      sums[0] += q0;
       sums[1] += q1;
-      sums[2] += wz * q2;
+      sums[2] += q2;
       sums[3] += q3;
       __builtin_amdgcn_sched_barrier(0);
     }

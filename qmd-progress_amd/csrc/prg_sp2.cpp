@@ -364,6 +364,50 @@ void prg_sp2_basic_tcore(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, bml_matrix_
               f.fro2 > 0.0 ? std::sqrt(d.fro2 / f.fro2) : 0.0);
 }
 
+// src/sp2_tensorcore/src/prg_sp2_tensorcore.cu:131-141 (Fortran binding
+// src/sp2_tensorcore/prg_sp2_tensorcore_mod.F90): host single-precision H in,
+// host double D = X out (not 2X).  Runs this library's FP16 hi+lo split SP2
+// (the reference routine's split products, sigma rule, noise-floor stop and
+// FP64 refinement) with Nocc = int(bndfil N) as the reference counts it.
+// Differences: Gershgorin bounds instead of the reference's hard-coded
+// (-27.55, 25.0), and up to two FP64 SP2 polishing steps in place of its
+// single 2T^2 - T^4 step.  eps, minsp2iter, sp2conv and idemtol are accepted
+// as in the reference, which does not use them on this path either.
+void prg_sp2_tensorcore(int N, float* H, double* D, float eps, float bndfil, int minsp2iter,
+                        int maxsp2iter, char sp2conv, float idemtol, int verbose) {
+  (void)eps;
+  (void)sp2conv;
+  clear_error();
+  if (N <= 0 || !H || !D || maxsp2iter < 1) {
+    set_error(ERR_ARG, "prg_sp2_tensorcore: bad arguments");
+    return;
+  }
+  const int nocc = (int)(bndfil * (float)N);
+  bml_matrix_t* h = bml_zero_matrix(dense, double_real, N, N, sequential);
+  bml_matrix_t* r = h ? bml_zero_matrix(dense, double_real, N, N, sequential) : nullptr;
+  if (!h || !r) {
+    if (h) bml_deallocate(&h);
+    return;
+  }
+  for (size_t i = 0; i < (size_t)N * N; ++i) h->host[i] = (double)H[i];
+  int st = OK;
+  DMat Hd, Rd;
+  if (!(st = bml_dev_read(h, &Hd)) && !(st = bml_dev_write(r, &Rd))) {
+    Sp2Params p = base_params(V_ALG2, N, (double)nocc / (double)N, minsp2iter, maxsp2iter, "Abs",
+                              (double)idemtol);
+    p.precision = PRG_PREC_F16SPLIT;
+    Sp2Result res;
+    st = run_sp2(Hd, p, Rd, &res);
+    if (st == OK || st == ERR_NOT_CONVERGED) {
+      report("prg_sp2_tensorcore", verbose, res);
+      if (!(st = bml_host_sync(r)))
+        for (size_t i = 0; i < (size_t)N * N; ++i) D[i] = 0.5 * r->host[i];
+    }
+  }
+  bml_deallocate(&h);
+  bml_deallocate(&r);
+}
+
 void prg_sp2_alg2(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
                   int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
                   int verbose) {

@@ -79,6 +79,8 @@ def lib():
         "prg_sp2_basic": (None, [vp, vp, d, d, i, i, cp, d, i]),
         "prg_sp2_alg2": (None, [vp, vp, d, d, i, i, cp, d, i]),
         "prg_sp2_basic_tcore": (None, [vp, vp, vp, d, d, i, i, cp, d, i]),
+        "prg_sp2_tensorcore": (None, [i, C.POINTER(C.c_float), dp, C.c_float, C.c_float, i, i,
+                                      C.c_char, C.c_float, i]),
         "prg_sp2_alg1": (None, [vp, vp, d, d, i, i, cp, d, i]),
         "prg_sp2_alg2_genseq": (None, [vp, vp, d, d, i, i, cp, d, ip, ip, dp, i]),
         "prg_sp2_alg1_genseq": (None, [vp, vp, d, d, i, i, cp, d, ip, ip, dp]),
@@ -290,6 +292,20 @@ def sp2_basic_tcore(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, rhofull
                               minsp2iter, maxsp2iter, sp2conv.encode(), idemtol, verbose)
     _check()
     return rho.numpy()
+
+
+def sp2_tensorcore(h, bndfil, minsp2iter=1, maxsp2iter=100, sp2conv=b"R", idemtol=1e-5,
+                   eps=1e-5, verbose=0):
+    """prg_sp2_tensorcore (src/sp2_tensorcore/src/prg_sp2_tensorcore.cu:131-141):
+    float32 H in, float64 D = X (not 2X) out, FP16 split products."""
+    hf = np.asfortranarray(np.asarray(h, dtype=np.float32))
+    n = hf.shape[0]
+    out = np.zeros((n, n), dtype=np.float64, order="F")
+    lib().prg_sp2_tensorcore(n, hf.ctypes.data_as(C.POINTER(C.c_float)),
+                             out.ctypes.data_as(C.POINTER(C.c_double)), eps, bndfil, minsp2iter,
+                             maxsp2iter, sp2conv, idemtol, verbose)
+    _check()
+    return out
 
 
 def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):

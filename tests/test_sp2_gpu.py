@@ -179,3 +179,22 @@ def test_sp2_basic_tcore_matches_oracle():
     assert np.linalg.norm(rho_g - rho_o) < 5e-8
     assert np.linalg.norm(rho_g - rho_full) < 5e-8
     assert abs(np.trace(rho_g) - n) < 1e-7
+
+
+def test_sp2_tensorcore_entry():
+    """prg_sp2_tensorcore: float32 H in, D = X out (not 2X), FP16 split
+    products with the noise-floor stop and FP64 refinement.  Bars (the split
+    products' accuracy on this near-gapless model input, as for the f16split
+    mode of prg_sp2_alg2): D within 5e-3 (Frobenius) of the FP64 oracle's X
+    and Tr D within 1e-3 of int(bndfil N); D symmetric."""
+    import progress_amd as P
+    n = 512
+    print(f"[sp2_tensorcore] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 2)
+    d = P.sp2_tensorcore(h, 0.5, 1, 100)
+    x_o = 0.5 * O.build_density_T0(h.astype(np.float32).astype(np.float64), 0.5)[0]
+    err, tr = float(np.linalg.norm(d - x_o)), abs(float(np.trace(d)) - n // 2)
+    print(f"  ||D - X_oracle||_F = {err:.3e}, |Tr D - Nocc| = {tr:.3e}", flush=True)
+    assert err < 5e-3
+    assert tr < 1e-3
+    assert np.array_equal(d, d.T)
