@@ -971,3 +971,39 @@ contains
     Ef = c_gp_t0_ef()
   end subroutine prg_build_densityGP_T0
 end module prg_graphsolver_mod
+
+! src/sp2_tensorcore/src/prg_sp2_tensorcore_mod.F90: the mixed-precision SP2
+! entry with host arrays (H single precision in, D = X double precision out).
+! The C symbol takes sp2conv as one character by value (the C prototype's
+! form); the library does not use it.
+module prg_sp2_tensorcore_mod
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+  public :: prg_sp2_tensorcore_f
+
+  interface
+    subroutine c_sp2_tensorcore(n, h, d, eps, bndfil, mn, mx, conv, tol, v) &
+         bind(C, name="prg_sp2_tensorcore")
+      import :: c_ptr, c_int, c_float, c_char
+      integer(c_int), value :: n, mn, mx, v
+      type(c_ptr), value :: h, d
+      real(c_float), value :: eps, bndfil, tol
+      character(kind=c_char), value :: conv
+    end subroutine
+  end interface
+
+contains
+
+  subroutine prg_sp2_tensorcore_f(N, H, D, eps, bndfil, minsp2iter, maxsp2iter, sp2conv, &
+       idemtol, verbose)
+    integer(c_int), intent(in) :: N, minsp2iter, maxsp2iter, verbose
+    real(c_double), target, intent(inout) :: D(*)
+    real(c_float), target, intent(in) :: H(*)
+    real(c_float), intent(in) :: eps, bndfil, idemtol
+    character(c_char), intent(in) :: sp2conv
+    call c_sp2_tensorcore(N, c_loc(H), c_loc(D), eps, bndfil, minsp2iter, maxsp2iter, &
+         sp2conv, idemtol, verbose)
+  end subroutine prg_sp2_tensorcore_f
+
+end module prg_sp2_tensorcore_mod
