@@ -233,10 +233,19 @@ int graph_adjacency(bml_matrix_t* g_bml, std::vector<std::vector<int>>* adj) {
   const int N = g_bml->N;
   const double* G = g_bml->host;
   adj->assign(N, {});
+  // one contiguous pass over the columns: G(i, j) != 0 makes i and j
+  // neighbours both ways; each list is then sorted and deduplicated
   for (int j = 0; j < N; ++j) {
     const double* col = G + (size_t)j * N;
     for (int i = 0; i < N; ++i)
-      if (i != j && (col[i] != 0.0 || G[j + (size_t)i * N] != 0.0)) (*adj)[i].push_back(j);
+      if (i != j && col[i] != 0.0) {
+        (*adj)[i].push_back(j);
+        (*adj)[j].push_back(i);
+      }
+  }
+  for (auto& l : *adj) {
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
   }
   return OK;
 }

@@ -781,14 +781,15 @@ def bfs_partition(g, nparts):
     return gp
 
 
-def build_density_gp_t0(h, g, bndfil, ef, nparts, threshold=0.0, verbose=0):
-    """prg_build_densityGP_T0: returns (rho, updated Ef)."""
-    n = h.shape[0]
-    mh, mg = Matrix.from_numpy(h), Matrix.from_numpy(g)
-    rho = Matrix.zeros(n)
+def build_density_gp_t0(h, g, bndfil, ef, nparts, threshold=0.0, verbose=0, out=None):
+    """prg_build_densityGP_T0: returns (rho, updated Ef).  h and g are numpy
+    arrays or Matrix handles; with `out` (a Matrix) rho stays in that handle."""
+    mh = h if isinstance(h, Matrix) else Matrix.from_numpy(h)
+    mg = g if isinstance(g, Matrix) else Matrix.from_numpy(g)
+    rho = out if out is not None else Matrix.zeros(mh.n)
     lib().prg_build_densityGP_T0(mh.h, mg.h, rho.h, threshold, bndfil, ef, nparts, verbose)
     _check()
-    return rho.numpy(), float(lib().prg_build_densityGP_T0_ef())
+    return (rho if out is not None else rho.numpy()), float(lib().prg_build_densityGP_T0_ef())
 
 
 def subgraph_timing():
