@@ -66,12 +66,18 @@ constexpr int kMaxModuli = 14;
 constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
                                      233, 229, 227, 223, 217, 211, 199};
 
+constexpr int kRecon = 6;  // moduli that rebuild A itself (M6 = 2^47.8)
+
 struct CrtConst {
   double p[kMaxModuli];    // moduli
   double inv[kMaxModuli];  // 1 / p
   double q[kMaxModuli];    // (M / p)^-1 mod p
+  double q6[kRecon];       // (M6 / p)^-1 mod p over the first kRecon moduli
+  double m6;               // M6, the product of the first kRecon moduli
   double scale;            // 2^a
+  double iscale;           // 2^-a
   double cscale;           // M / 2^(2a)
+  int recon;               // X is carried by its residues alone (a <= 45)
 };
 
 struct ProductArgs {
@@ -275,6 +281,19 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
   }
 }
 
+// X entry from its residues r_k (symmetric int8 of A mod p_k, k < kRecon):
+// A/M6 = sum_k ((r_k q6_k) mod p_k) / p_k (mod 1), X = A 2^-a.
+__device__ __forceinline__ double crt_rebuild(const int (&r)[kRecon], const CrtConst& c) {
+  double S = 0.0;
+#pragma unroll
+  for (int k = 0; k < kRecon; ++k) {
+    const double v = (double)r[k] * c.q6[k];  // exact: |v| < 2^15
+    const double md = fma(-c.p[k], rint(v * c.inv[k]), v);
+    S = fma(md, c.inv[k], S);
+  }
+  return rint((S - rint(S)) * c.m6) * c.iscale;
+}
+
 // One upper-triangle 64 x 64 block (I, J) of X^2: sum the moduli's fractions,
 // apply the SP2 update, write X_next at (J, I) (the lower triangle), and the residue
 // planes of X_next (through LDS, one modulus at a time, as contiguous 4 KB
@@ -282,7 +301,7 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
 // in the share of product wave (I % 4) / 2 + 2 (J % 4), MFMA rows
 // 4 (I % 2) .. + 3.  Thread (b, lane) holds elements (row, col0(n) + r) with
 // row = 64 I + 16 b + lane % 16, col0(n) = 64 J + 16 n + 4 (lane / 16).
-template <int NP>
+template <int NP, bool RECON>
 __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
   __shared__ __attribute__((aligned(16))) int8_t rb[2][SB][kPitch];
   __shared__ double red[NT / 64][4];
@@ -304,13 +323,38 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
   const double* X = g.buf[cur];
   double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
 
-  // X at (row, col0 + r) read as (col0 + r, row): 32 contiguous bytes per n
-  double2 xv[8];
+  // X at (row, col0 + r): with RECON rebuilt from the first kRecon residue
+  // planes of X (4 bytes per plane and n at J * pld * 64 + row * 64 +
+  // 16 n + cl0), otherwise read from the FP64 master as (col0 + r, row),
+  // 32 contiguous bytes per n
+  double xs[16];
+  if (RECON) {
+    const int8_t* Pc = g.planes[cur] + (size_t)J * pld * SB + (size_t)row * SB + cl0;
+    const size_t plane = (size_t)pld * pld;
+    int rw[kRecon][4];
 #pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    const double2* xp = (const double2*)(X + SB * J + 16 * n + cl0 + (size_t)row * ld);
-    xv[2 * n] = xp[0];
-    xv[2 * n + 1] = xp[1];
+    for (int k = 0; k < kRecon; ++k)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) rw[k][n] = *(const int*)(Pc + (size_t)k * plane + 16 * n);
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int rk[kRecon];
+#pragma unroll
+        for (int k = 0; k < kRecon; ++k) rk[k] = (int)(int8_t)(rw[k][n] >> (8 * r));
+        xs[4 * n + r] = crt_rebuild(rk, g.c);
+      }
+  } else {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const double2* xp = (const double2*)(X + SB * J + 16 * n + cl0 + (size_t)row * ld);
+      const double2 u = xp[0], v = xp[1];
+      xs[4 * n] = u.x;
+      xs[4 * n + 1] = u.y;
+      xs[4 * n + 2] = v.x;
+      xs[4 * n + 3] = v.y;
+    }
   }
   // S = sum_k ((C q_k) mod p_k) / p_k, bytes (n, r) of the 16-byte record
   i4 d[NP];
@@ -342,7 +386,7 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
     for (int r = 0; r < 4; ++r) {
       const double sr = S[4 * n + r];
       const double y = (sr - rint(sr)) * g.c.cscale;  // C / M mod 1, scaled
-      const double x = (r & 1) ? xv[2 * n + (r >> 1)].y : xv[2 * n + (r >> 1)].x;
+      const double x = xs[4 * n + r];
       const double z = fma(cb, x, ca * y);
       const double dxy = x - y;
       const bool diag = (row == col0 + r);
@@ -353,12 +397,15 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
       z4[r] = z;
       zz[4 * n + r] = rint(z * g.c.scale);
     }
-    // X_next at (col0 + r, row) only: the lower triangle, plus the whole of
-    // each diagonal block.  These are the positions the next step reads; the
-    // solve's final copy rebuilds the upper triangle by symmetry.
-    double2* tp = (double2*)(Xn + col0 + (size_t)row * ld);
-    tp[0] = double2{z4[0], z4[1]};
-    tp[1] = double2{z4[2], z4[3]};
+    // without RECON, X_next at (col0 + r, row) only: the lower triangle,
+    // plus the whole of each diagonal block.  These are the positions the
+    // next step reads; the solve's final copy rebuilds the upper triangle by
+    // symmetry.  With RECON the residue planes below are X_next.
+    if (!RECON) {
+      double2* tp = (double2*)(Xn + col0 + (size_t)row * ld);
+      tp[0] = double2{z4[0], z4[1]};
+      tp[1] = double2{z4[2], z4[3]};
+    }
   }
   // Residue planes of X_next, one modulus at a time through a double-buffered
   // LDS block rb[row][col] (4 columns per dword).  Block (I, J) of a plane:
@@ -444,6 +491,35 @@ __global__ void crt_split_kernel(const double* __restrict__ x, int8_t* __restric
   }
 }
 
+// out = scale X for the n x n block, X rebuilt from the first kRecon residue
+// planes of the current buffer (used when the update kernel keeps no FP64
+// master).  A thread rebuilds rows r0 .. r0 + 3 of one column, reading
+// element (r, col) as (col, r): 4 contiguous bytes per plane.
+__global__ void crt_finish_kernel(const int8_t* __restrict__ p0, const int8_t* __restrict__ p1,
+                                  const Sp2State* __restrict__ s, int pld, CrtConst c,
+                                  double scale, double* __restrict__ out, int ld_out, int n) {
+  const int8_t* P = s->cur ? p1 : p0;
+  const size_t plane = (size_t)pld * pld;
+  const int rq = (n + 3) / 4;
+  const size_t total = (size_t)rq * n;
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < total; i += (size_t)gridDim.x * NT) {
+    const int r0 = 4 * (int)(i % rq), col = (int)(i / rq);
+    const size_t off = (size_t)(r0 / SB) * pld * SB + (size_t)col * SB + (r0 % SB);
+    int w[kRecon];
+#pragma unroll
+    for (int k = 0; k < kRecon; ++k) w[k] = *(const int*)(P + (size_t)k * plane + off);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = r0 + j;
+      if (row >= n) break;
+      int rk[kRecon];
+#pragma unroll
+      for (int k = 0; k < kRecon; ++k) rk[k] = (int)(int8_t)(w[k] >> (8 * j));
+      out[row + (size_t)col * ld_out] = scale * crt_rebuild(rk, c);
+    }
+  }
+}
+
 int g_moduli = 11;
 
 // Modular inverse of a mod p (p > 1, gcd(a, p) = 1).
@@ -480,7 +556,19 @@ CrtConst make_const(int np) {
   // <= M keeps |C| < M / 2; a <= 52 (X carries 53 bits).
   const int a = std::min(52, (int)std::floor((log2m - 1.001) / 2.0));
   c.scale = std::ldexp(1.0, a);
+  c.iscale = std::ldexp(1.0, -a);
   c.cscale = std::exp2(log2m - 2.0 * a);
+  // A = rint(2^a X) itself from its first kRecon residues: |A| <= 2^a < M6 / 2
+  // and the FP64 fraction sum is off by at most kRecon 2^-53, i.e. < 0.1 after
+  // scaling by M6 = 2^47.8, so rint recovers A exactly for a <= 45
+  long long m6 = 1;
+  for (int k = 0; k < kRecon; ++k) m6 *= kModuli[k];
+  for (int k = 0; k < kRecon; ++k) {
+    const long long p = kModuli[k];
+    c.q6[k] = (double)inv_mod((m6 / p) % p, p);
+  }
+  c.m6 = (double)m6;
+  c.recon = (a <= 45) ? 1 : 0;
   return c;
 }
 
@@ -581,7 +669,10 @@ int run_product(const ProductArgs& g, int nitems, int* used) {
 
 template <int NP>
 void launch_update(const UpdateArgs& g, int nblocks) {
-  hipLaunchKernelGGL(crt_update_kernel<NP>, dim3(nblocks), dim3(NT), 0, stream(), g);
+  if (g.c.recon)
+    hipLaunchKernelGGL((crt_update_kernel<NP, true>), dim3(nblocks), dim3(NT), 0, stream(), g);
+  else
+    hipLaunchKernelGGL((crt_update_kernel<NP, false>), dim3(nblocks), dim3(NT), 0, stream(), g);
 }
 
 template <int NP>
@@ -792,6 +883,21 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
 #define PRG_CALL(K) launch_update<K>(ua, nblocks)
   PRG_CRT_DISPATCH(np, PRG_CALL)
 #undef PRG_CALL
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int crt_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out,
+               int np) {
+  const CrtConst c = make_const(np);
+  if (!c.recon) return sp2_finish(x0, x1, d_state, scale, out, true);
+  const int ld = x0.ld, pld = plane_ld(ld);
+  const size_t plane = (size_t)pld * pld;
+  int st = OK;
+  int8_t* planes = (int8_t*)workspace("crt_planes", 2 * (size_t)np * plane, &st);
+  if (st) return st;
+  hipLaunchKernelGGL(crt_finish_kernel, dim3(2048), dim3(NT), 0, stream(), planes,
+                     planes + (size_t)np * plane, d_state, pld, c, scale, out.p, out.ld, out.n);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

@@ -178,9 +178,13 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
     }
     if (phase == 0) phase0_steps = steps;
   }
-  // the int8 CRT update writes only the lower triangle of X between steps
-  const bool lower_only = (p.precision == PRG_PREC_INT8CRT) && !response;
-  if ((st = sp2_finish(X0, X1, dst, 2.0, out, lower_only))) return st;
+  // the int8 CRT update keeps X in its residue planes (or the lower
+  // triangle of an FP64 master at 12 or more moduli) between steps
+  if (p.precision == PRG_PREC_INT8CRT && !response) {
+    if ((st = crt_finish(X0, X1, dst, 2.0, out, crt_moduli()))) return st;
+  } else if ((st = sp2_finish(X0, X1, dst, 2.0, out))) {
+    return st;
+  }
   if (response && out1) {
     // rho1 = 2 Delta, Delta in the ping-pong slot named by the state
     if ((st = sp2_finish(D0, D1, dst, 2.0, *out1))) return st;

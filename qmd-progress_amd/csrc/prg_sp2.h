@@ -55,6 +55,10 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
 int crt_moduli();
 int crt_last_waves();  // 800 + ring stages of the last int8 CRT product kernel (0: none yet)
 int crt_update_blocks(int ld);  // partial-sum slots one int8 CRT step writes
+// out = scale X[cur] after an int8 CRT solve (from the residue planes when no
+// FP64 master is kept, from the master's lower triangle otherwise)
+int crt_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out,
+               int np);
 // HIP-event timing of the int8 CRT product kernel alone: enable/reset, start of
 // a solve, sum the first `executed` steps of the solve, read the totals.
 void crt_timing(bool enable);
