@@ -277,6 +277,20 @@ def main():
     print(json.dumps(out))
 
 
+def batched_executed_flops(m, panel):
+    """MFMA flops one iteration of the batched SP2 kernel executes on a part
+    of padded size m: the upper-triangle tiles over panels of `panel` (the
+    last one 64 wide when panel does not divide m); a diagonal tile leaves
+    its strictly lower quadrant to the mirror write (3/4 of its product)."""
+    widths = [min(panel, m - s) for s in range(0, m, panel)]
+    f = 0.0
+    for b, wb in enumerate(widths):
+        for a in range(b + 1):
+            wa = widths[a]
+            f += 2.0 * wa * wb * m * (0.75 if a == b else 1.0)
+    return f
+
+
 def graph_bench(args, P, torch, np):
     """Graph-partitioned SP2 (prg_subgraphSP2Loop) on the water-box system:
     natoms atoms x 3 orbitals.  Setup (untimed): a dense SP2 solve of the whole
@@ -330,8 +344,9 @@ def graph_bench(args, P, torch, np):
     dt = sum(walls) / len(walls)
     dev_s = sum(times) / len(times)
     alg = sum(2.0 * float(m) ** 3 for m in ls) * it
+    panel, launch = P.subgraph_config()
     mpad = [max(64, ((m + 63) // 64) * 64) for m in ls]
-    executed = sum((m // 64) * (m // 64 + 1) // 2 * 2.0 * 64 * 64 * m for m in mpad) * it
+    executed = sum(batched_executed_flops(m, panel) for m in mpad) * it
     loop_s = last[1] * 1e-3
     dd = d_dense.cpu().numpy()
     err = float(np.linalg.norm(rho - dd))
@@ -365,9 +380,13 @@ def graph_bench(args, P, torch, np):
         "setup_s": {"generate": t_gen, "dense_int8crt_solve": t_dense},
         "alg_tflops": alg / dt / 1e12,
         "accuracy_vs_dense": {"fro": err, "rel_fro": rel, "trace_err": tr_err},
-        "roofline": {"bound": "mfma", "kernel": "batched_sp2_kernel (FP64 MFMA, 64x64 tiles)",
+        "tiles": {"panel": panel,
+                  "launch": {0: "eager", 1: "captured hipGraph", 2: "hipGraph replay"}[launch]},
+        "roofline": {"bound": "mfma",
+                     "kernel": f"batched_sp2_kernel (FP64 MFMA, {panel}-wide panels)",
                      "achieved": achieved, "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp64_tflops"], "traffic": None,
+                     "algorithmic_tflops": alg / loop_s / 1e12,
                      "note": "executed MFMA work of the padded upper-triangle tiles per "
                              "iteration / loop device time"},
         "cpu_baseline": None,

@@ -3,13 +3,17 @@
 // src/prg_sp2_mod.F90:1259-1310).
 //
 // All parts share the branch sequence pp and the normalisation bounds, so
-// they advance in lock-step: one launch per iteration processes every part.
-// Parts are packed one after another in HBM, each zero-padded to a multiple
-// of 64.  The work list holds the upper-triangle 64 x 64 tiles of every part,
-// largest parts first; consecutive workgroups (per XCD) walk consecutive
-// tiles, so a part's panels stay in one XCD's L2.  Each tile runs the FP64
-// MFMA product and the fused update X <- X + (1 - 2 pp)(X - X^2) with the
-// core-row sum of (X - X^2)^2 for vv.
+// they advance in lock-step: one launch per iteration processes every part,
+// and the iteration launches are replayed from a captured hipGraph.  Parts
+// are packed one after another in HBM, each zero-padded to a multiple of 64.
+// The work list holds the upper-triangle tiles of every part, largest parts
+// first; consecutive workgroups (per XCD) walk consecutive tiles, so a part's
+// panels stay in one XCD's L2.  A part is cut into panels of 64 or of 128
+// (the last one 64 wide when the size is an odd multiple of 64), giving
+// 64 x 64, 128 x 64 and 128 x 128 tiles; the panel width is picked per batch
+// by timing both (prg_graph.cpp).  Each tile runs the FP64 MFMA product and
+// the fused update X <- X + (1 - 2 pp)(X - X^2) with the core-row sum of
+// (X - X^2)^2 for vv.
 #include "prg_graph.h"
 #include "prg_tile.h"
 
@@ -22,99 +26,136 @@ using tile::xcd_remap;
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
 
-constexpr int TB = 64, BK = 16, NT = 256;
-constexpr int LROW = TB + 2;
+constexpr int BK = 16, NT = 256;
 
 struct BatchArgs {
   const double* x;      // packed current X
   double* xn;           // packed next X
   const PartDesc* parts;
-  const int4* tiles;    // (part, I, J, 0)
+  const int4* tiles;    // (part, first row, first column, shape: 0 64x64, 1 128x64, 3 128x128)
   int sigma;            // 1: X^2 ; 0: 2X - X^2
   double* part_sums;    // per tile: core-row sum of (X - X^2)^2
 };
 
-__global__ __launch_bounds__(NT, 4) void batched_sp2_kernel(BatchArgs g) {
-  __shared__ double As[BK][LROW];
-  __shared__ double Bs[BK][LROW];
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int4 tl = g.tiles[t];
+// One TM x TN tile of X^2 (rows r0.., columns c0..) of one part and its fused
+// SP2 update.  Four waves (2 x 2), each (TM/2) x (TN/2) = (TM/32) x (TN/32)
+// MFMA blocks of 16 x 16; 16-deep K slabs of both panels staged through LDS
+// with a register prefetch of the next slab.  In a diagonal tile the wave of
+// the strictly lower quadrant idles and the upper off-diagonal wave writes
+// its block and the transpose.
+template <int TM, int TN>
+__device__ __forceinline__ void sp2_tile(const BatchArgs& g, int t, int4 tl, double* smem) {
+  constexpr int LA = TM + (TM == 128 ? 4 : 2), LB = TN + (TN == 128 ? 4 : 2);
+  constexpr int MI = TM / 32, NJ = TN / 32;
+  constexpr int AR = TM / 2, AKS = NT / AR, QA = BK / AKS;    // A slab loaders
+  constexpr int TPC = NT / TN, KPT = BK / TPC, QB = KPT / 2;  // B slab loaders
+  static_assert(QA >= 1 && QB >= 1 && AKS * QA == BK && TPC * KPT == BK, "tile loaders");
+  double (*As)[LA] = reinterpret_cast<double (*)[LA]>(smem);
+  double (*Bs)[LB] = reinterpret_cast<double (*)[LB]>(smem + BK * LA);
   const PartDesc pd = g.parts[tl.x];
-  const int I = tl.y, J = tl.z;
+  const int r0 = tl.y, c0 = tl.z;
   const int m = pd.m;
   const double* X = g.x + pd.offset;
   double* Xn = g.xn + pd.offset;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = tid >> 6, wm = w & 1, wn = w >> 1;
-  // loaders: A slab = rows I*64..+63, k0..k0+15 ; B slab = k0..k0+15, cols J*64..+63
-  const int ar = (tid & 31) * 2, ak = tid >> 5;  // rows ar, ar+1 ; k = ak, ak + 8
-  const int bj = tid >> 2, bk = (tid & 3) * 4;   // column bj ; k = bk .. bk+3
-  d4 acc[2][2];
+  const bool diag = (r0 == c0);
+  const bool idle = diag && wm == 1 && wn == 0;
+  const bool mirror = !diag || (wm == 0 && wn == 1);
+  const int ar = (tid % AR) * 2, ak = tid / AR;     // rows ar, ar+1 ; k = ak + AKS q
+  const int bj = tid / TPC, bk = (tid % TPC) * KPT;  // column bj ; k = bk .. bk+KPT-1
+  d4 acc[MI][NJ];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MI; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  d2 ra[2], rb[2];
+    for (int b = 0; b < NJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  d2 ra[QA], rb[QB];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      ra[q] = *(const d2*)(X + (size_t)(I * TB + ar) + (size_t)(k0 + ak + 8 * q) * m);
-      rb[q] = *(const d2*)(X + (size_t)(J * TB + bj) * m + k0 + bk + 2 * q);
-    }
+    for (int q = 0; q < QA; ++q)
+      ra[q] = *(const d2*)(X + (size_t)(r0 + ar) + (size_t)(k0 + ak + AKS * q) * m);
+#pragma unroll
+    for (int q = 0; q < QB; ++q) rb[q] = *(const d2*)(X + (size_t)(c0 + bj) * m + k0 + bk + 2 * q);
   };
   load(0);
   for (int k0 = 0; k0 < m; k0 += BK) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      As[ak + 8 * q][ar] = ra[q].x;
-      As[ak + 8 * q][ar + 1] = ra[q].y;
+    for (int q = 0; q < QA; ++q) {
+      As[ak + AKS * q][ar] = ra[q].x;
+      As[ak + AKS * q][ar + 1] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
       Bs[bk + 2 * q][bj] = rb[q].x;
       Bs[bk + 2 * q + 1][bj] = rb[q].y;
     }
     __syncthreads();
     if (k0 + BK < m) load(k0 + BK);
+    if (!idle) {
 #pragma unroll
-    for (int kk = 0; kk < BK / 4; ++kk) {
-      const int kr = kk * 4 + (lane >> 4);
-      double a[2], b[2];
+      for (int kk = 0; kk < BK / 4; ++kk) {
+        const int kr = kk * 4 + (lane >> 4);
+        double a[MI], b[NJ];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kr][wm * 32 + i * 16 + (lane & 15)];
+        for (int i = 0; i < MI; ++i) a[i] = As[kr][wm * (TM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+        for (int j = 0; j < NJ; ++j) b[j] = Bs[kr][wn * (TN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < NJ; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+      }
     }
   }
   // epilogue: lane holds (X^2)[row][col], row = lane%16, col = lane/16 + 4r
-  const bool mirror = (I != J);
   const double ca = g.sigma ? 1.0 : -1.0, cb = g.sigma ? 0.0 : 2.0;
   const int core = pd.core;
   double s[1] = {0.0};
+  if (!idle) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < MI; ++i) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int row = I * TB + wm * 32 + i * 16 + (lane & 15);
+      for (int j = 0; j < NJ; ++j) {
+        const int row = r0 + wm * (TM / 2) + i * 16 + (lane & 15);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int col = J * TB + wn * 32 + j * 16 + (lane >> 4) + 4 * r;
-        const size_t idx = row + (size_t)col * m;
-        const double y = acc[i][j][r];
-        const double x = X[idx];
-        const double z = fma(cb, x, ca * y);
-        const double d = x - y;
-        const double wi = ((row < core) ? 1.0 : 0.0) + ((mirror && col < core) ? 1.0 : 0.0);
-        s[0] += wi * d * d;
-        Xn[idx] = z;
-        Xn[mirror ? col + (size_t)row * m : idx] = z;
+        for (int r = 0; r < 4; ++r) {
+          const int col = c0 + wn * (TN / 2) + j * 16 + (lane >> 4) + 4 * r;
+          const size_t idx = row + (size_t)col * m;
+          const double y = acc[i][j][r];
+          const double x = X[idx];
+          const double z = fma(cb, x, ca * y);
+          const double d = x - y;
+          const double wi = ((row < core) ? 1.0 : 0.0) + ((mirror && col < core) ? 1.0 : 0.0);
+          s[0] += wi * d * d;
+          Xn[idx] = z;
+          Xn[mirror ? col + (size_t)row * m : idx] = z;
+        }
       }
     }
   }
   block_store_partials<1>(s, g.part_sums + t);
+}
+
+// WIDE: the tile list holds 128 x 128, 128 x 64 and 64 x 64 tiles (two
+// workgroups per CU); otherwise 64 x 64 tiles only (four per CU).
+template <bool WIDE>
+__global__ __launch_bounds__(NT, WIDE ? 2 : 4) void batched_sp2_kernel(BatchArgs g) {
+  constexpr int LDS = WIDE ? 2 * BK * (128 + 4) : 2 * BK * (64 + 2);
+  __shared__ double smem[LDS];
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int4 tl = g.tiles[t];
+  if constexpr (WIDE) {
+    if (tl.w == 3)
+      sp2_tile<128, 128>(g, t, tl, smem);
+    else if (tl.w == 1)
+      sp2_tile<128, 64>(g, t, tl, smem);
+    else
+      sp2_tile<64, 64>(g, t, tl, smem);
+  } else {
+    sp2_tile<64, 64>(g, t, tl, smem);
+  }
 }
 
 // X_p = (maxeval I - H[idx, idx]) / (maxeval - mineval), zero padded.
@@ -216,19 +257,20 @@ __global__ void fermi_scale_kernel(const double* __restrict__ v, const double* _
   }
 }
 
-// vv[it] += sum of the tile partials (fixed order).
+// vv[it] += sum of iteration it's tile partials (fixed order); block it.
 __global__ void vv_accumulate_kernel(const double* __restrict__ part, int ntiles,
-                                     double* __restrict__ vv, int it) {
+                                     double* __restrict__ vv) {
   __shared__ double red[NT];
+  const double* p = part + (size_t)blockIdx.x * ntiles;
   double s = 0.0;
-  for (int b = threadIdx.x; b < ntiles; b += NT) s += part[b];
+  for (int b = threadIdx.x; b < ntiles; b += NT) s += p[b];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int st = NT / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) vv[it] += red[0];
+  if (threadIdx.x == 0) vv[blockIdx.x] += red[0];
 }
 
 __global__ void zero_kernel(double* __restrict__ p, size_t count) {
@@ -246,8 +288,7 @@ int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double minev
   return OK;
 }
 
-int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
-                 double* d_vv, int it) {
+int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part) {
   BatchArgs g;
   g.x = x;
   g.xn = xn;
@@ -255,9 +296,18 @@ int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, 
   g.tiles = plan.d_tiles;
   g.sigma = sigma;
   g.part_sums = d_part;
-  hipLaunchKernelGGL(batched_sp2_kernel, dim3(plan.ntiles), dim3(NT), 0, stream(), g);
-  hipLaunchKernelGGL(vv_accumulate_kernel, dim3(1), dim3(NT), 0, stream(), d_part, plan.ntiles,
-                     d_vv, it);
+  if (plan.wide)
+    hipLaunchKernelGGL(batched_sp2_kernel<true>, dim3(plan.ntiles), dim3(NT), 0, stream(), g);
+  else
+    hipLaunchKernelGGL(batched_sp2_kernel<false>, dim3(plan.ntiles), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int batched_vv(const BatchPlan& plan, const double* d_part, int iters, double* d_vv) {
+  if (iters <= 0) return OK;
+  hipLaunchKernelGGL(vv_accumulate_kernel, dim3(iters), dim3(NT), 0, stream(), d_part,
+                     plan.ntiles, d_vv);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

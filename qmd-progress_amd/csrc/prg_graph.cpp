@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -74,12 +75,11 @@ int halo_indices(bml_matrix_t* g_bml, prg_graph_partitioning_t* gp, int djflag) 
 }
 
 // Packed batch of all parts: largest padded size first, each part m x m with
-// m = ceil64(core + halo), and the upper-triangle 64 x 64 tiles of every part.
+// m = ceil64(core + halo).
 struct HostPlan {
   std::vector<int> order;  // packed position -> part id
   std::vector<PartDesc> parts;
   std::vector<int> idx;
-  std::vector<int4> tiles;
 };
 
 int make_plan(const prg_graph_partitioning_t* gp, HostPlan* hp, BatchPlan* plan) {
@@ -91,7 +91,6 @@ int make_plan(const prg_graph_partitioning_t* gp, HostPlan* hp, BatchPlan* plan)
                    [&](int a, int b) { return gp->lsize[a] > gp->lsize[b]; });
   hp->parts.assign(np, PartDesc{});
   hp->idx.clear();
-  hp->tiles.clear();
   size_t off = 0;
   for (int q = 0; q < np; ++q) {
     const int p = hp->order[q];
@@ -104,23 +103,166 @@ int make_plan(const prg_graph_partitioning_t* gp, HostPlan* hp, BatchPlan* plan)
     off += (size_t)d.m * d.m;
     hp->idx.insert(hp->idx.end(), gp->core_halo_index[p], gp->core_halo_index[p] + d.n);
     hp->parts[q] = d;
-    const int nt = d.m / 64;
-    for (int J = 0; J < nt; ++J)
-      for (int I = 0; I <= J; ++I) hp->tiles.push_back(int4{q, I, J, 0});
   }
   plan->nparts = np;
-  plan->ntiles = (int)hp->tiles.size();
   plan->total = off;
   plan->d_parts = (PartDesc*)workspace("gp_parts", sizeof(PartDesc) * np, &st);
-  plan->d_tiles = (int4*)workspace("gp_tiles", sizeof(int4) * hp->tiles.size(), &st);
   plan->d_idx = (int*)workspace("gp_idx", sizeof(int) * std::max<size_t>(1, hp->idx.size()), &st);
   if (st) return st;
   PRG_HIP_CHECK(hipMemcpyAsync(plan->d_parts, hp->parts.data(), sizeof(PartDesc) * np,
                                hipMemcpyHostToDevice, stream()));
-  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_tiles, hp->tiles.data(), sizeof(int4) * hp->tiles.size(),
-                               hipMemcpyHostToDevice, stream()));
   PRG_HIP_CHECK(hipMemcpyAsync(plan->d_idx, hp->idx.data(), sizeof(int) * hp->idx.size(),
                                hipMemcpyHostToDevice, stream()));
+  return OK;
+}
+
+// Upper-triangle tiles (part, first row, first column, shape) of every part
+// over panels of width `panel` (64 or 128); the last panel of a part is 64
+// wide when 128 does not divide its size, so the shapes are 128 x 128
+// (code 3), 128 x 64 (1) and 64 x 64 (0).  Tiles go column by column within
+// a part, parts in packed order (largest first).
+std::vector<int4> make_tiles(const HostPlan& hp, int panel) {
+  std::vector<int4> tiles;
+  std::vector<int> start, width;
+  for (int q = 0; q < (int)hp.parts.size(); ++q) {
+    const int m = hp.parts[q].m;
+    start.clear();
+    width.clear();
+    for (int s = 0; s < m; s += panel) {
+      start.push_back(s);
+      width.push_back(std::min(panel, m - s));
+    }
+    for (size_t b = 0; b < start.size(); ++b)
+      for (size_t a = 0; a <= b; ++a)
+        tiles.push_back(int4{q, start[a], start[b], (width[a] == 128 ? 1 : 0) | (width[b] == 128 ? 2 : 0)});
+  }
+  return tiles;
+}
+
+int upload_tiles(const std::vector<int4>& tiles, int panel, BatchPlan* plan) {
+  int st = OK;
+  plan->wide = panel == 128 ? 1 : 0;
+  plan->ntiles = (int)tiles.size();
+  plan->d_tiles = (int4*)workspace(panel == 128 ? "gp_tiles128" : "gp_tiles64",
+                                   sizeof(int4) * std::max<size_t>(1, tiles.size()), &st);
+  if (st) return st;
+  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_tiles, tiles.data(), sizeof(int4) * tiles.size(),
+                               hipMemcpyHostToDevice, stream()));
+  return OK;
+}
+
+// Tile panel width per batch: PRG_BATCH_TILE=64|128 forces it; otherwise
+// both widths are timed on the batch itself (one untimed and two timed
+// iterations each on the gathered first iterate, results discarded) and the
+// faster is kept for every later call with the same padded part sizes.
+std::map<uint64_t, int>& panel_cache() {
+  static std::map<uint64_t, int> c;
+  return c;
+}
+
+uint64_t size_signature(const HostPlan& hp) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the padded sizes
+  for (const PartDesc& d : hp.parts) {
+    h ^= (uint64_t)d.m;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+int time_step(const BatchPlan& plan, const double* x, double* xn, double* part, float* ms) {
+  hipEvent_t a, b;
+  PRG_HIP_CHECK(hipEventCreate(&a));
+  PRG_HIP_CHECK(hipEventCreate(&b));
+  int st = batched_step(plan, x, xn, 1, part);
+  PRG_HIP_CHECK(hipEventRecord(a, stream()));
+  for (int r = 0; r < 2 && !st; ++r) st = batched_step(plan, x, xn, 1, part);
+  PRG_HIP_CHECK(hipEventRecord(b, stream()));
+  PRG_HIP_CHECK(hipEventSynchronize(b));
+  PRG_HIP_CHECK(hipEventElapsedTime(ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return st;
+}
+
+// The captured iteration launches of the last subgraph loop, replayed while
+// the buffers, the tile list and the branch sequence stay the same.
+struct LoopGraph {
+  std::vector<uint64_t> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+};
+LoopGraph& loop_graph() {
+  static LoopGraph* g = new LoopGraph;  // never destroyed: outlives the HIP runtime at exit
+  return *g;
+}
+
+int g_cfg[2] = {0, 0};  // panel width, loop launch (0 eager, 1 captured, 2 replayed)
+
+// iters SP2 iterations of every part from x0, ping-ponging with x1 (the
+// result is in x0 for even iters), then vv[it] += the iteration's core-row
+// idempotency sums.  d_part holds iters x ntiles partials.
+int enqueue_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int iters,
+                 double* d_part, double* d_vv) {
+  int st = OK;
+  const double* cur = x0;
+  double* nxt = x1;
+  for (int it = 0; it < iters && !st; ++it) {
+    st = batched_step(plan, cur, nxt, pp[it] != 0, d_part + (size_t)it * plan.ntiles);
+    const double* t = cur;
+    cur = nxt;
+    nxt = const_cast<double*>(t);
+  }
+  if (!st) st = batched_vv(plan, d_part, iters, d_vv);
+  return st;
+}
+
+int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int iters,
+             double* d_part, double* d_vv) {
+  const char* e = std::getenv("PRG_BATCH_GRAPH");
+  if (e && std::atoi(e) == 0) {
+    g_cfg[1] = 0;
+    return enqueue_loop(plan, x0, x1, pp, iters, d_part, d_vv);
+  }
+  std::vector<uint64_t> key = {(uint64_t)(uintptr_t)x0,           (uint64_t)(uintptr_t)x1,
+                               (uint64_t)(uintptr_t)d_part,       (uint64_t)(uintptr_t)d_vv,
+                               (uint64_t)(uintptr_t)plan.d_parts, (uint64_t)(uintptr_t)plan.d_tiles,
+                               (uint64_t)plan.ntiles,             (uint64_t)plan.wide,
+                               (uint64_t)iters};
+  for (int it = 0; it < iters; ++it) key.push_back(pp[it] != 0);
+  LoopGraph& lg = loop_graph();
+  if (lg.exec && lg.key == key) {
+    g_cfg[1] = 2;
+    PRG_HIP_CHECK(hipGraphLaunch(lg.exec, stream()));
+    return OK;
+  }
+  lg.reset();
+  PRG_HIP_CHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+  int st = enqueue_loop(plan, x0, x1, pp, iters, d_part, d_vv);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(stream(), &graph);
+  if (st || ec != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    if (!st) set_error(ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    return st ? st : ERR_HIP;
+  }
+  lg.graph = graph;
+  const hipError_t ei = hipGraphInstantiate(&lg.exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    lg.exec = nullptr;
+    lg.reset();
+    set_error(ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    return ERR_HIP;
+  }
+  lg.key = key;
+  g_cfg[1] = 1;
+  PRG_HIP_CHECK(hipGraphLaunch(lg.exec, stream()));
   return OK;
 }
 
@@ -140,11 +282,40 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   BatchPlan plan;
   if ((st = make_plan(gp, &hp, &plan))) return st;
   const size_t off = plan.total;
+  const int iters = gp->maxIter;
+  const std::vector<int4> tiles64 = make_tiles(hp, 64);
   double* x0 = (double*)workspace("gp_x0", sizeof(double) * off, &st);
   double* x1 = (double*)workspace("gp_x1", sizeof(double) * off, &st);
-  double* d_part = (double*)workspace("gp_part", sizeof(double) * hp.tiles.size(), &st);
+  // iters x ntiles loop partials + one slice for the tile-width timing
+  double* d_part =
+      (double*)workspace("gp_part", sizeof(double) * tiles64.size() * (iters + 1), &st);
   double* d_vv = (double*)workspace("gp_vv", sizeof(double) * 100, &st);
   if (st) return st;
+  int panel = 0;
+  if (const char* e = std::getenv("PRG_BATCH_TILE")) panel = std::atoi(e);
+  if (panel != 0 && panel != 64 && panel != 128) {
+    set_error(ERR_ARG, "PRG_BATCH_TILE must be 64 or 128");
+    return ERR_ARG;
+  }
+  const uint64_t sig = size_signature(hp);
+  if (!panel) {
+    auto it = panel_cache().find(sig);
+    if (it != panel_cache().end()) panel = it->second;
+  }
+  if (!panel) {
+    if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval))) return st;
+    float ms[2] = {0.f, 0.f};
+    for (int k = 0; k < 2 && !st; ++k) {
+      const int w = k ? 128 : 64;
+      if ((st = upload_tiles(k ? make_tiles(hp, 128) : tiles64, w, &plan))) return st;
+      st = time_step(plan, x0, x1, d_part + (size_t)iters * tiles64.size(), &ms[k]);
+    }
+    if (st) return st;
+    panel = ms[1] < ms[0] ? 128 : 64;
+    panel_cache()[sig] = panel;
+  }
+  if ((st = upload_tiles(panel == 128 ? make_tiles(hp, 128) : tiles64, panel, &plan))) return st;
+  g_cfg[0] = panel;
   PRG_HIP_CHECK(hipMemcpyAsync(d_vv, gp->vv, sizeof(double) * 100, hipMemcpyHostToDevice,
                                stream()));
   hipEvent_t ev[4];
@@ -152,14 +323,9 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
   if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
-  double* cur = x0;
-  double* nxt = x1;
-  for (int it = 0; it < gp->maxIter; ++it) {
-    if ((st = batched_step(plan, cur, nxt, gp->pp[it] != 0, d_part, d_vv, it))) return st;
-    std::swap(cur, nxt);
-  }
+  if ((st = run_loop(plan, x0, x1, gp->pp, iters, d_part, d_vv))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[2], stream()));
-  if ((st = batched_scatter(plan, cur, rho, 2.0))) return st;
+  if ((st = batched_scatter(plan, iters % 2 ? x1 : x0, rho, 2.0))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[3], stream()));
   PRG_HIP_CHECK(hipMemcpyAsync(gp->vv, d_vv, sizeof(double) * 100, hipMemcpyDeviceToHost,
                                stream()));
@@ -521,6 +687,13 @@ void prg_gpu_subgraph_timing(double* ms_gather, double* ms_loop, double* ms_scat
   if (ms_gather) *ms_gather = g_ms[0];
   if (ms_loop) *ms_loop = g_ms[1];
   if (ms_scatter) *ms_scatter = g_ms[2];
+}
+
+// Tile panel width (64 or 128) of the last prg_subgraphSP2Loop and how its
+// iterations were launched (0 eager, 1 captured into a hipGraph, 2 replayed).
+void prg_gpu_subgraph_config(int* panel, int* graph) {
+  if (panel) *panel = g_cfg[0];
+  if (graph) *graph = g_cfg[1];
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@ struct PartDesc {
 struct BatchPlan {
   int nparts = 0;
   int ntiles = 0;
+  int wide = 0;              // 1: tiles of 128-wide panels (see batched_sp2_kernel)
   size_t total = 0;          // doubles in one packed buffer
   PartDesc* d_parts = nullptr;
   int4* d_tiles = nullptr;
@@ -31,8 +32,10 @@ int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag)
 
 int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double mineval,
                    double maxeval);
-int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
-                 double* d_vv, int it);
+// One SP2 iteration of every part: xn from x, per-tile vv partials in d_part.
+int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part);
+// vv[it] += sum of d_part[it * ntiles + t] over the tiles, it < iters.
+int batched_vv(const BatchPlan& plan, const double* d_part, int iters, double* d_vv);
 int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale);
 // X_p = H[idx, idx], padding decoupled with `pad` on its diagonal
 int batched_gather_pad(const DMat& H, const BatchPlan& plan, double* x, double pad);

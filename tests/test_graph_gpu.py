@@ -39,6 +39,56 @@ def test_subgraph_loop_matches_oracle(P, n, per_part, gthr):
     assert np.allclose(vv_g, vv_o, rtol=1e-9, atol=1e-12)
 
 
+@pytest.fixture(scope="module")
+def wide_case():
+    """Parts of 254-426 core+halo rows (padded 256-448): 128 x 128,
+    128 x 64 and 64 x 64 tiles with 128-wide panels."""
+    n = 1200
+    h = O.model_hamiltonian(n, 5)
+    rho_full, lo, hi, seq, imax = _sequence(h, None)
+    g = np.where(np.abs(rho_full) >= 1e-3, rho_full, 0.0)
+    rho_o, vv_o = O.subgraph_sp2_loop(h, g, O.equal_partition(150, n), seq, imax, lo, hi)
+    return dict(n=n, h=h, g=g, lo=lo, hi=hi, seq=seq, imax=imax, rho=rho_o, vv=vv_o)
+
+
+@pytest.mark.parametrize("panel", [64, 128])
+def test_subgraph_loop_tile_widths_match_oracle(P, wide_case, panel, monkeypatch):
+    """Both panel widths of the batched SP2 kernel against the oracle."""
+    print(f"[subgraph tiles] panel={panel}", flush=True)
+    c = wide_case
+    monkeypatch.setenv("PRG_BATCH_TILE", str(panel))
+    gp = P.equal_partition(150, c["n"])
+    rho_g, vv_g = P.subgraph_sp2_loop(c["h"], c["g"], gp, c["seq"], c["imax"], c["lo"], c["hi"])
+    assert P.subgraph_config()[0] == panel
+    assert max(gp.lsize[i] for i in range(gp.totalParts)) > 384
+    assert np.linalg.norm(rho_g - c["rho"]) < 1e-9
+    assert np.allclose(vv_g, c["vv"], rtol=1e-9, atol=1e-12)
+
+
+def test_subgraph_loop_graph_replays_equal_eager(P, wide_case, monkeypatch):
+    """The iteration launches captured in a hipGraph and replayed by later
+    calls give the eager launches' result bit for bit, on every replay, with
+    other library work and synchronisations between the calls."""
+    print("[subgraph hipGraph replays]", flush=True)
+    c = wide_case
+    mh, mg = P.Matrix.from_numpy(c["h"]), P.Matrix.from_numpy(c["g"])
+    gp = P.equal_partition(150, c["n"])
+    args = (c["seq"], c["imax"], c["lo"], c["hi"])
+    monkeypatch.setenv("PRG_BATCH_GRAPH", "0")
+    ref, vv_ref = P.subgraph_sp2_loop(mh, mg, gp, *args)
+    assert P.subgraph_config()[1] == 0
+    monkeypatch.delenv("PRG_BATCH_GRAPH")
+    modes = []
+    for r in range(4):
+        rho, vv = P.subgraph_sp2_loop(mh, mg, gp, *args)
+        modes.append(P.subgraph_config()[1])
+        assert np.array_equal(rho, ref), f"call {r}"
+        assert vv == vv_ref, f"call {r}"
+        P.gershgorin(c["h"])
+    assert modes[0] in (1, 2) and modes[1:] == [2, 2, 2]
+    assert np.linalg.norm(ref - c["rho"]) < 1e-9
+
+
 def test_subgraph_loop_block_diagonal_is_exact(P):
     print("[subgraph block diagonal]", flush=True)
     blocks = [O.model_hamiltonian(80, seed=s) for s in range(3)]
