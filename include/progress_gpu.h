@@ -319,13 +319,15 @@ void prg_build_densityGP_T0(bml_matrix_t *ham_bml, bml_matrix_t *g_bml, bml_matr
 double prg_build_densityGP_T0_ef(void);
 /* Device time (ms) of the last prg_subgraphSP2Loop phases. */
 void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scatter);
-/* Tile panel width (64 or 128) that the last prg_subgraphSP2Loop used for
- * its batched SP2 kernel, and how its iterations were launched: 0 eager
+/* Kernel configuration of the last prg_subgraphSP2Loop: the tile panel width
+ * (64 or 128) and the number of K slabs its batched SP2 kernel prefetches
+ * (1 or 2), and how its iterations were launched: 0 eager
  * (PRG_BATCH_GRAPH=0), 1 captured into a hipGraph, 2 replay of the graph
  * captured by an earlier call with the same buffers, tile list and branch
- * sequence.  The panel width is timed per batch of part sizes on first use
- * (PRG_BATCH_TILE=64|128 forces it).  No counterpart in the reference. */
-void prg_gpu_subgraph_config(int *panel, int *graph);
+ * sequence.  The configuration is timed per batch of part sizes on first use
+ * (PRG_BATCH_TILE=64|128 and PRG_BATCH_DEPTH=1|2 force it).  No counterpart
+ * in the reference. */
+void prg_gpu_subgraph_config(int *panel, int *depth, int *graph);
 
 /* ------------------------------------------------------------------ precision modes */
 /* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).

@@ -35,15 +35,17 @@ struct BatchArgs {
   const int4* tiles;    // (part, first row, first column, shape: 0 64x64, 1 128x64, 3 128x128)
   int sigma;            // 1: X^2 ; 0: 2X - X^2
   double* part_sums;    // per tile: core-row sum of (X - X^2)^2
+  const int* queue_off; // queue q holds tiles [queue_off[q], queue_off[q + 1]), q < 8
+  int* tickets;         // 8 ticket counters, zero at launch
 };
 
 // One TM x TN tile of X^2 (rows r0.., columns c0..) of one part and its fused
 // SP2 update.  Four waves (2 x 2), each (TM/2) x (TN/2) = (TM/32) x (TN/32)
 // MFMA blocks of 16 x 16; 16-deep K slabs of both panels staged through LDS
-// with a register prefetch of the next slab.  In a diagonal tile the wave of
+// with a register prefetch PF slabs ahead.  In a diagonal tile the wave of
 // the strictly lower quadrant idles and the upper off-diagonal wave writes
 // its block and the transpose.
-template <int TM, int TN>
+template <int TM, int TN, int PF>
 __device__ __forceinline__ void sp2_tile(const BatchArgs& g, int t, int4 tl, double* smem) {
   constexpr int LA = TM + (TM == 128 ? 4 : 2), LB = TN + (TN == 128 ? 4 : 2);
   constexpr int MI = TM / 32, NJ = TN / 32;
@@ -69,43 +71,51 @@ __device__ __forceinline__ void sp2_tile(const BatchArgs& g, int t, int4 tl, dou
   for (int a = 0; a < MI; ++a)
 #pragma unroll
     for (int b = 0; b < NJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  d2 ra[QA], rb[QB];
-  auto load = [&](int k0) {
+  // PF slabs in flight in registers: slab k is loaded PF slabs before it is
+  // staged in LDS (m is a multiple of 64, so of PF * BK)
+  d2 ra[PF][QA], rb[PF][QB];
+  auto load = [&](int slot, int k0) {
 #pragma unroll
     for (int q = 0; q < QA; ++q)
-      ra[q] = *(const d2*)(X + (size_t)(r0 + ar) + (size_t)(k0 + ak + AKS * q) * m);
+      ra[slot][q] = *(const d2*)(X + (size_t)(r0 + ar) + (size_t)(k0 + ak + AKS * q) * m);
 #pragma unroll
-    for (int q = 0; q < QB; ++q) rb[q] = *(const d2*)(X + (size_t)(c0 + bj) * m + k0 + bk + 2 * q);
+    for (int q = 0; q < QB; ++q)
+      rb[slot][q] = *(const d2*)(X + (size_t)(c0 + bj) * m + k0 + bk + 2 * q);
   };
-  load(0);
-  for (int k0 = 0; k0 < m; k0 += BK) {
-    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < QA; ++q) {
-      As[ak + AKS * q][ar] = ra[q].x;
-      As[ak + AKS * q][ar + 1] = ra[q].y;
-    }
+  for (int p = 0; p < PF; ++p) load(p, p * BK);
+  for (int k0 = 0; k0 < m; k0 += PF * BK) {
 #pragma unroll
-    for (int q = 0; q < QB; ++q) {
-      Bs[bk + 2 * q][bj] = rb[q].x;
-      Bs[bk + 2 * q + 1][bj] = rb[q].y;
-    }
-    __syncthreads();
-    if (k0 + BK < m) load(k0 + BK);
-    if (!idle) {
+    for (int p = 0; p < PF; ++p) {
+      const int kc = k0 + p * BK;
+      __syncthreads();
 #pragma unroll
-      for (int kk = 0; kk < BK / 4; ++kk) {
-        const int kr = kk * 4 + (lane >> 4);
-        double a[MI], b[NJ];
+      for (int q = 0; q < QA; ++q) {
+        As[ak + AKS * q][ar] = ra[p][q].x;
+        As[ak + AKS * q][ar + 1] = ra[p][q].y;
+      }
 #pragma unroll
-        for (int i = 0; i < MI; ++i) a[i] = As[kr][wm * (TM / 2) + i * 16 + (lane & 15)];
+      for (int q = 0; q < QB; ++q) {
+        Bs[bk + 2 * q][bj] = rb[p][q].x;
+        Bs[bk + 2 * q + 1][bj] = rb[p][q].y;
+      }
+      __syncthreads();
+      if (kc + PF * BK < m) load(p, kc + PF * BK);
+      if (!idle) {
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) b[j] = Bs[kr][wn * (TN / 2) + j * 16 + (lane & 15)];
+        for (int kk = 0; kk < BK / 4; ++kk) {
+          const int kr = kk * 4 + (lane >> 4);
+          double a[MI], b[NJ];
 #pragma unroll
-        for (int i = 0; i < MI; ++i)
+          for (int i = 0; i < MI; ++i) a[i] = As[kr][wm * (TM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-          for (int j = 0; j < NJ; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < NJ; ++j) b[j] = Bs[kr][wn * (TN / 2) + j * 16 + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+        }
       }
     }
   }
@@ -138,23 +148,43 @@ __device__ __forceinline__ void sp2_tile(const BatchArgs& g, int t, int4 tl, dou
   block_store_partials<1>(s, g.part_sums + t);
 }
 
+// Persistent work-queue launch: one workgroup per resident slot.  The tile
+// list is split into 8 queues, one per XCD (whole parts, balanced by work);
+// workgroup b (dispatched to XCD b mod 8) takes tickets from its own queue,
+// then from the others in turn, and exits when every queue is drained.
+// Tickets only grow, so every workgroup reaches its exit; the tiles are
+// independent and each stores its own partial, so results do not depend on
+// which workgroup ran which tile.
 // WIDE: the tile list holds 128 x 128, 128 x 64 and 64 x 64 tiles (two
-// workgroups per CU); otherwise 64 x 64 tiles only (four per CU).
-template <bool WIDE>
+// workgroups per CU); otherwise 64 x 64 tiles only.  PF: slabs of prefetch.
+template <bool WIDE, int PF>
 __global__ __launch_bounds__(NT, WIDE ? 2 : 4) void batched_sp2_kernel(BatchArgs g) {
   constexpr int LDS = WIDE ? 2 * BK * (128 + 4) : 2 * BK * (64 + 2);
   __shared__ double smem[LDS];
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int4 tl = g.tiles[t];
-  if constexpr (WIDE) {
-    if (tl.w == 3)
-      sp2_tile<128, 128>(g, t, tl, smem);
-    else if (tl.w == 1)
-      sp2_tile<128, 64>(g, t, tl, smem);
-    else
-      sp2_tile<64, 64>(g, t, tl, smem);
-  } else {
-    sp2_tile<64, 64>(g, t, tl, smem);
+  __shared__ int s_ticket;
+  const int home = blockIdx.x & 7;
+  for (int q = 0; q < 8; ++q) {
+    const int qi = (home + q) & 7;
+    const int first = g.queue_off[qi], count = g.queue_off[qi + 1] - first;
+    for (;;) {
+      __syncthreads();  // every thread has read the previous ticket
+      if (threadIdx.x == 0) s_ticket = atomicAdd(&g.tickets[qi], 1);
+      __syncthreads();
+      const int k = s_ticket;
+      if (k >= count) break;
+      const int t = first + k;
+      const int4 tl = g.tiles[t];
+      if constexpr (WIDE) {
+        if (tl.w == 3)
+          sp2_tile<128, 128, PF>(g, t, tl, smem);
+        else if (tl.w == 1)
+          sp2_tile<128, 64, PF>(g, t, tl, smem);
+        else
+          sp2_tile<64, 64, PF>(g, t, tl, smem);
+      } else {
+        sp2_tile<64, 64, PF>(g, t, tl, smem);
+      }
+    }
   }
 }
 
@@ -273,6 +303,10 @@ __global__ void vv_accumulate_kernel(const double* __restrict__ part, int ntiles
   if (threadIdx.x == 0) vv[blockIdx.x] += red[0];
 }
 
+__global__ void zero_int_kernel(int* __restrict__ p, int count) {
+  for (int i = threadIdx.x; i < count; i += NT) p[i] = 0;
+}
+
 __global__ void zero_kernel(double* __restrict__ p, size_t count) {
   for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT)
     p[i] = 0.0;
@@ -288,7 +322,8 @@ int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double minev
   return OK;
 }
 
-int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part) {
+int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
+                 int* d_tickets) {
   BatchArgs g;
   g.x = x;
   g.xn = xn;
@@ -296,10 +331,38 @@ int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, 
   g.tiles = plan.d_tiles;
   g.sigma = sigma;
   g.part_sums = d_part;
-  if (plan.wide)
-    hipLaunchKernelGGL(batched_sp2_kernel<true>, dim3(plan.ntiles), dim3(NT), 0, stream(), g);
-  else
-    hipLaunchKernelGGL(batched_sp2_kernel<false>, dim3(plan.ntiles), dim3(NT), 0, stream(), g);
+  g.queue_off = plan.d_queue_off;
+  g.tickets = d_tickets;
+  void (*kernel)(BatchArgs) =
+      plan.wide ? (plan.depth == 2 ? batched_sp2_kernel<true, 2> : batched_sp2_kernel<true, 1>)
+                : (plan.depth == 2 ? batched_sp2_kernel<false, 2> : batched_sp2_kernel<false, 1>);
+  if (plan.grid <= 0 || plan.grid % 8) {
+    set_error(ERR_ARG, "batched SP2: launch slots not set");
+    return ERR_ARG;
+  }
+  const int grid = plan.grid;
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int batched_slots(BatchPlan* plan) {
+  void (*kernel)(BatchArgs) =
+      plan->wide ? (plan->depth == 2 ? batched_sp2_kernel<true, 2> : batched_sp2_kernel<true, 1>)
+                 : (plan->depth == 2 ? batched_sp2_kernel<false, 2> : batched_sp2_kernel<false, 1>);
+  static int per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int& nb = per_cu[plan->wide ? 1 : 0][plan->depth == 2 ? 2 : 1];
+  if (!nb) PRG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, 0));
+  int cus = 0, dev = 0;
+  PRG_HIP_CHECK(hipGetDevice(&dev));
+  PRG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // one workgroup per resident slot, a multiple of 8 (workgroup b runs on XCD b mod 8)
+  plan->grid = ((cus * std::max(nb, 1) + 7) / 8) * 8;
+  return OK;
+}
+
+int batched_zero_tickets(int* d_tickets, int count) {
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(NT), 0, stream(), d_tickets, count);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

@@ -119,12 +119,21 @@ int make_plan(const prg_graph_partitioning_t* gp, HostPlan* hp, BatchPlan* plan)
 // Upper-triangle tiles (part, first row, first column, shape) of every part
 // over panels of width `panel` (64 or 128); the last panel of a part is 64
 // wide when 128 does not divide its size, so the shapes are 128 x 128
-// (code 3), 128 x 64 (1) and 64 x 64 (0).  Tiles go column by column within
-// a part, parts in packed order (largest first).
-std::vector<int4> make_tiles(const HostPlan& hp, int panel) {
+// (code 3), 128 x 64 (1) and 64 x 64 (0).  The parts are dealt to 8 queues,
+// one per XCD, largest work first to the least loaded queue (work of a tile:
+// rows x columns x m); a queue lists its parts' tiles part by part, column
+// by column.  off[q] is the first tile of queue q, off[8] the total.
+struct TileQueues {
   std::vector<int4> tiles;
+  std::vector<int> off;
+};
+
+TileQueues make_tiles(const HostPlan& hp, int panel) {
+  const int np = (int)hp.parts.size();
+  std::vector<std::vector<int4>> per(np);
+  std::vector<double> work(np, 0.0);
   std::vector<int> start, width;
-  for (int q = 0; q < (int)hp.parts.size(); ++q) {
+  for (int q = 0; q < np; ++q) {
     const int m = hp.parts[q].m;
     start.clear();
     width.clear();
@@ -133,30 +142,57 @@ std::vector<int4> make_tiles(const HostPlan& hp, int panel) {
       width.push_back(std::min(panel, m - s));
     }
     for (size_t b = 0; b < start.size(); ++b)
-      for (size_t a = 0; a <= b; ++a)
-        tiles.push_back(int4{q, start[a], start[b], (width[a] == 128 ? 1 : 0) | (width[b] == 128 ? 2 : 0)});
+      for (size_t a = 0; a <= b; ++a) {
+        per[q].push_back(int4{q, start[a], start[b], (width[a] == 128 ? 1 : 0) | (width[b] == 128 ? 2 : 0)});
+        work[q] += (double)width[a] * width[b] * m;
+      }
   }
-  return tiles;
+  std::vector<int> order(np);
+  for (int q = 0; q < np; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+  std::vector<std::vector<int>> bins(8);
+  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q : order) {
+    const int b = (int)(std::min_element(load, load + 8) - load);
+    bins[b].push_back(q);
+    load[b] += work[q];
+  }
+  TileQueues out;
+  out.off.push_back(0);
+  for (int b = 0; b < 8; ++b) {
+    for (int q : bins[b]) out.tiles.insert(out.tiles.end(), per[q].begin(), per[q].end());
+    out.off.push_back((int)out.tiles.size());
+  }
+  return out;
 }
 
-int upload_tiles(const std::vector<int4>& tiles, int panel, BatchPlan* plan) {
+int upload_tiles(const TileQueues& tq, int panel, BatchPlan* plan) {
   int st = OK;
   plan->wide = panel == 128 ? 1 : 0;
-  plan->ntiles = (int)tiles.size();
+  plan->ntiles = (int)tq.tiles.size();
   plan->d_tiles = (int4*)workspace(panel == 128 ? "gp_tiles128" : "gp_tiles64",
-                                   sizeof(int4) * std::max<size_t>(1, tiles.size()), &st);
+                                   sizeof(int4) * std::max<size_t>(1, tq.tiles.size()), &st);
+  plan->d_queue_off =
+      (int*)workspace(panel == 128 ? "gp_qoff128" : "gp_qoff64", sizeof(int) * 9, &st);
   if (st) return st;
-  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_tiles, tiles.data(), sizeof(int4) * tiles.size(),
+  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_tiles, tq.tiles.data(), sizeof(int4) * tq.tiles.size(),
+                               hipMemcpyHostToDevice, stream()));
+  PRG_HIP_CHECK(hipMemcpyAsync(plan->d_queue_off, tq.off.data(), sizeof(int) * 9,
                                hipMemcpyHostToDevice, stream()));
   return OK;
 }
 
-// Tile panel width per batch: PRG_BATCH_TILE=64|128 forces it; otherwise
-// both widths are timed on the batch itself (one untimed and two timed
-// iterations each on the gathered first iterate, results discarded) and the
-// faster is kept for every later call with the same padded part sizes.
-std::map<uint64_t, int>& panel_cache() {
-  static std::map<uint64_t, int> c;
+// Kernel configuration per batch: the tile panel width (64 or 128) and the
+// number of K slabs prefetched ahead (1 or 2).  PRG_BATCH_TILE=64|128 and
+// PRG_BATCH_DEPTH=1|2 force either; the remaining candidates are timed on
+// the batch itself (one untimed and two timed iterations each on the
+// gathered first iterate, results discarded) and the fastest is kept for
+// every later call with the same padded part sizes.
+struct BatchConfig {
+  int panel = 0, depth = 0;
+};
+std::map<std::pair<uint64_t, int>, BatchConfig>& config_cache() {
+  static std::map<std::pair<uint64_t, int>, BatchConfig> c;
   return c;
 }
 
@@ -169,16 +205,23 @@ uint64_t size_signature(const HostPlan& hp) {
   return h;
 }
 
-int time_step(const BatchPlan& plan, const double* x, double* xn, double* part, float* ms) {
+// Device time of one batched iteration (min of three, after one untimed).
+int time_step(const BatchPlan& plan, const double* x, double* xn, double* part, int* tickets,
+              float* ms) {
   hipEvent_t a, b;
   PRG_HIP_CHECK(hipEventCreate(&a));
   PRG_HIP_CHECK(hipEventCreate(&b));
-  int st = batched_step(plan, x, xn, 1, part);
-  PRG_HIP_CHECK(hipEventRecord(a, stream()));
-  for (int r = 0; r < 2 && !st; ++r) st = batched_step(plan, x, xn, 1, part);
-  PRG_HIP_CHECK(hipEventRecord(b, stream()));
-  PRG_HIP_CHECK(hipEventSynchronize(b));
-  PRG_HIP_CHECK(hipEventElapsedTime(ms, a, b));
+  int st = OK;
+  for (int r = 0; r < 4 && !st; ++r) {
+    if ((st = batched_zero_tickets(tickets, 8))) break;
+    PRG_HIP_CHECK(hipEventRecord(a, stream()));
+    if ((st = batched_step(plan, x, xn, 1, part, tickets))) break;
+    PRG_HIP_CHECK(hipEventRecord(b, stream()));
+    PRG_HIP_CHECK(hipEventSynchronize(b));
+    float t = 0.f;
+    PRG_HIP_CHECK(hipEventElapsedTime(&t, a, b));
+    if (r == 1 || (r > 1 && t < *ms)) *ms = t;
+  }
   hipEventDestroy(a);
   hipEventDestroy(b);
   return st;
@@ -203,18 +246,19 @@ LoopGraph& loop_graph() {
   return *g;
 }
 
-int g_cfg[2] = {0, 0};  // panel width, loop launch (0 eager, 1 captured, 2 replayed)
+int g_cfg[3] = {0, 0, 0};  // panel width, loop launch (0 eager, 1 captured, 2 replayed), depth
 
 // iters SP2 iterations of every part from x0, ping-ponging with x1 (the
 // result is in x0 for even iters), then vv[it] += the iteration's core-row
 // idempotency sums.  d_part holds iters x ntiles partials.
 int enqueue_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int iters,
-                 double* d_part, double* d_vv) {
-  int st = OK;
+                 double* d_part, int* d_tickets, double* d_vv) {
+  int st = batched_zero_tickets(d_tickets, 8 * iters);
   const double* cur = x0;
   double* nxt = x1;
   for (int it = 0; it < iters && !st; ++it) {
-    st = batched_step(plan, cur, nxt, pp[it] != 0, d_part + (size_t)it * plan.ntiles);
+    st = batched_step(plan, cur, nxt, pp[it] != 0, d_part + (size_t)it * plan.ntiles,
+                      d_tickets + 8 * it);
     const double* t = cur;
     cur = nxt;
     nxt = const_cast<double*>(t);
@@ -224,16 +268,18 @@ int enqueue_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, i
 }
 
 int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int iters,
-             double* d_part, double* d_vv) {
+             double* d_part, int* d_tickets, double* d_vv) {
   const char* e = std::getenv("PRG_BATCH_GRAPH");
   if (e && std::atoi(e) == 0) {
     g_cfg[1] = 0;
-    return enqueue_loop(plan, x0, x1, pp, iters, d_part, d_vv);
+    return enqueue_loop(plan, x0, x1, pp, iters, d_part, d_tickets, d_vv);
   }
   std::vector<uint64_t> key = {(uint64_t)(uintptr_t)x0,           (uint64_t)(uintptr_t)x1,
                                (uint64_t)(uintptr_t)d_part,       (uint64_t)(uintptr_t)d_vv,
-                               (uint64_t)(uintptr_t)plan.d_parts, (uint64_t)(uintptr_t)plan.d_tiles,
+                               (uint64_t)(uintptr_t)d_tickets,    (uint64_t)(uintptr_t)plan.d_parts,
+                               (uint64_t)(uintptr_t)plan.d_tiles, (uint64_t)(uintptr_t)plan.d_queue_off,
                                (uint64_t)plan.ntiles,             (uint64_t)plan.wide,
+                               (uint64_t)plan.depth,              (uint64_t)plan.grid,
                                (uint64_t)iters};
   for (int it = 0; it < iters; ++it) key.push_back(pp[it] != 0);
   LoopGraph& lg = loop_graph();
@@ -244,7 +290,7 @@ int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int i
   }
   lg.reset();
   PRG_HIP_CHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-  int st = enqueue_loop(plan, x0, x1, pp, iters, d_part, d_vv);
+  int st = enqueue_loop(plan, x0, x1, pp, iters, d_part, d_tickets, d_vv);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(stream(), &graph);
   if (st || ec != hipSuccess) {
@@ -283,39 +329,60 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   if ((st = make_plan(gp, &hp, &plan))) return st;
   const size_t off = plan.total;
   const int iters = gp->maxIter;
-  const std::vector<int4> tiles64 = make_tiles(hp, 64);
+  const TileQueues tiles64 = make_tiles(hp, 64);
+  const size_t maxtiles = tiles64.tiles.size();  // the 64-wide tiling has the most tiles
   double* x0 = (double*)workspace("gp_x0", sizeof(double) * off, &st);
   double* x1 = (double*)workspace("gp_x1", sizeof(double) * off, &st);
-  // iters x ntiles loop partials + one slice for the tile-width timing
-  double* d_part =
-      (double*)workspace("gp_part", sizeof(double) * tiles64.size() * (iters + 1), &st);
+  // iters x ntiles loop partials and iters x 8 queue tickets, + one slice each
+  // for the configuration timing
+  double* d_part = (double*)workspace("gp_part", sizeof(double) * maxtiles * (iters + 1), &st);
+  int* d_tickets = (int*)workspace("gp_tickets", sizeof(int) * 8 * (iters + 1), &st);
   double* d_vv = (double*)workspace("gp_vv", sizeof(double) * 100, &st);
   if (st) return st;
-  int panel = 0;
-  if (const char* e = std::getenv("PRG_BATCH_TILE")) panel = std::atoi(e);
-  if (panel != 0 && panel != 64 && panel != 128) {
-    set_error(ERR_ARG, "PRG_BATCH_TILE must be 64 or 128");
+  BatchConfig force;
+  if (const char* e = std::getenv("PRG_BATCH_TILE")) force.panel = std::atoi(e);
+  if (const char* e = std::getenv("PRG_BATCH_DEPTH")) force.depth = std::atoi(e);
+  if ((force.panel != 0 && force.panel != 64 && force.panel != 128) ||
+      (force.depth != 0 && force.depth != 1 && force.depth != 2)) {
+    set_error(ERR_ARG, "PRG_BATCH_TILE must be 64 or 128, PRG_BATCH_DEPTH 1 or 2");
     return ERR_ARG;
   }
-  const uint64_t sig = size_signature(hp);
-  if (!panel) {
-    auto it = panel_cache().find(sig);
-    if (it != panel_cache().end()) panel = it->second;
+  const auto key = std::make_pair(size_signature(hp), force.panel * 8 + force.depth);
+  BatchConfig cfg = force;
+  if (!cfg.panel || !cfg.depth) {
+    auto it = config_cache().find(key);
+    if (it != config_cache().end()) cfg = it->second;
   }
-  if (!panel) {
+  if (!cfg.panel || !cfg.depth) {
     if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval))) return st;
-    float ms[2] = {0.f, 0.f};
-    for (int k = 0; k < 2 && !st; ++k) {
-      const int w = k ? 128 : 64;
-      if ((st = upload_tiles(k ? make_tiles(hp, 128) : tiles64, w, &plan))) return st;
-      st = time_step(plan, x0, x1, d_part + (size_t)iters * tiles64.size(), &ms[k]);
+    const TileQueues tiles128 = make_tiles(hp, 128);
+    float best = 0.f;
+    for (int panel : {64, 128}) {
+      if (force.panel && panel != force.panel) continue;
+      if ((st = upload_tiles(panel == 128 ? tiles128 : tiles64, panel, &plan))) return st;
+      for (int depth : {1, 2}) {
+        if (force.depth && depth != force.depth) continue;
+        plan.depth = depth;
+        float ms = 0.f;
+        if ((st = batched_slots(&plan)) ||
+            (st = time_step(plan, x0, x1, d_part + (size_t)iters * maxtiles, d_tickets + 8 * iters,
+                            &ms)))
+          return st;
+        if (!cfg.panel || !cfg.depth || ms < best) {
+          best = ms;
+          cfg.panel = panel;
+          cfg.depth = depth;
+        }
+      }
     }
-    if (st) return st;
-    panel = ms[1] < ms[0] ? 128 : 64;
-    panel_cache()[sig] = panel;
+    config_cache()[key] = cfg;
   }
-  if ((st = upload_tiles(panel == 128 ? make_tiles(hp, 128) : tiles64, panel, &plan))) return st;
-  g_cfg[0] = panel;
+  if ((st = upload_tiles(cfg.panel == 128 ? make_tiles(hp, 128) : tiles64, cfg.panel, &plan)))
+    return st;
+  plan.depth = cfg.depth;
+  if ((st = batched_slots(&plan))) return st;
+  g_cfg[0] = cfg.panel;
+  g_cfg[2] = cfg.depth;
   PRG_HIP_CHECK(hipMemcpyAsync(d_vv, gp->vv, sizeof(double) * 100, hipMemcpyHostToDevice,
                                stream()));
   hipEvent_t ev[4];
@@ -323,7 +390,7 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
   if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
-  if ((st = run_loop(plan, x0, x1, gp->pp, iters, d_part, d_vv))) return st;
+  if ((st = run_loop(plan, x0, x1, gp->pp, iters, d_part, d_tickets, d_vv))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[2], stream()));
   if ((st = batched_scatter(plan, iters % 2 ? x1 : x0, rho, 2.0))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[3], stream()));
@@ -689,10 +756,12 @@ void prg_gpu_subgraph_timing(double* ms_gather, double* ms_loop, double* ms_scat
   if (ms_scatter) *ms_scatter = g_ms[2];
 }
 
-// Tile panel width (64 or 128) of the last prg_subgraphSP2Loop and how its
-// iterations were launched (0 eager, 1 captured into a hipGraph, 2 replayed).
-void prg_gpu_subgraph_config(int* panel, int* graph) {
+// Tile panel width (64 or 128) and prefetch depth (K slabs) of the last
+// prg_subgraphSP2Loop and how its iterations were launched (0 eager,
+// 1 captured into a hipGraph, 2 replayed).
+void prg_gpu_subgraph_config(int* panel, int* depth, int* graph) {
   if (panel) *panel = g_cfg[0];
+  if (depth) *depth = g_cfg[2];
   if (graph) *graph = g_cfg[1];
 }
 

@@ -19,9 +19,12 @@ struct BatchPlan {
   int nparts = 0;
   int ntiles = 0;
   int wide = 0;              // 1: tiles of 128-wide panels (see batched_sp2_kernel)
+  int depth = 1;             // K slabs prefetched ahead (1 or 2)
+  int grid = 0;              // workgroups of the persistent batched SP2 launch
   size_t total = 0;          // doubles in one packed buffer
   PartDesc* d_parts = nullptr;
   int4* d_tiles = nullptr;
+  int* d_queue_off = nullptr;  // 9 offsets: the tiles of XCD queue q
   int* d_idx = nullptr;
 };
 
@@ -32,8 +35,14 @@ int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag)
 
 int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double mineval,
                    double maxeval);
-// One SP2 iteration of every part: xn from x, per-tile vv partials in d_part.
-int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part);
+// One SP2 iteration of every part: xn from x, per-tile vv partials in
+// d_part; d_tickets: 8 queue counters, zero before the launch.
+int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
+                 int* d_tickets);
+// plan->grid = resident workgroups of the kernel that wide/depth select
+int batched_slots(BatchPlan* plan);
+// d_tickets[0, count) = 0 by a kernel (safe inside a captured graph)
+int batched_zero_tickets(int* d_tickets, int count);
 // vv[it] += sum of d_part[it * ntiles + t] over the tiles, it < iters.
 int batched_vv(const BatchPlan& plan, const double* d_part, int iters, double* d_vv);
 int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale);
