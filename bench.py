@@ -344,7 +344,7 @@ def graph_bench(args, P, torch, np):
     dt = sum(walls) / len(walls)
     dev_s = sum(times) / len(times)
     alg = sum(2.0 * float(m) ** 3 for m in ls) * it
-    panel, depth, launch = P.subgraph_config()
+    panel, depth, kslab, launch = P.subgraph_config()
     mpad = [max(64, ((m + 63) // 64) * 64) for m in ls]
     executed = sum(batched_executed_flops(m, panel) for m in mpad) * it
     loop_s = last[1] * 1e-3
@@ -380,7 +380,7 @@ def graph_bench(args, P, torch, np):
         "setup_s": {"generate": t_gen, "dense_int8crt_solve": t_dense},
         "alg_tflops": alg / dt / 1e12,
         "accuracy_vs_dense": {"fro": err, "rel_fro": rel, "trace_err": tr_err},
-        "tiles": {"panel": panel, "prefetch_slabs": depth,
+        "tiles": {"panel": panel, "prefetch_slabs": depth, "k_slab": kslab,
                   "launch": {0: "eager", 1: "captured hipGraph", 2: "hipGraph replay"}[launch]},
         "roofline": {"bound": "mfma",
                      "kernel": f"batched_sp2_kernel (FP64 MFMA, {panel}-wide panels)",

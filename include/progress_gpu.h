@@ -320,14 +320,15 @@ double prg_build_densityGP_T0_ef(void);
 /* Device time (ms) of the last prg_subgraphSP2Loop phases. */
 void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scatter);
 /* Kernel configuration of the last prg_subgraphSP2Loop: the tile panel width
- * (64 or 128) and the number of K slabs its batched SP2 kernel prefetches
- * (1 or 2), and how its iterations were launched: 0 eager
- * (PRG_BATCH_GRAPH=0), 1 captured into a hipGraph, 2 replay of the graph
- * captured by an earlier call with the same buffers, tile list and branch
- * sequence.  The configuration is timed per batch of part sizes on first use
- * (PRG_BATCH_TILE=64|128 and PRG_BATCH_DEPTH=1|2 force it).  No counterpart
- * in the reference. */
-void prg_gpu_subgraph_config(int *panel, int *depth, int *graph);
+ * (64 or 128), the number of K slabs its batched SP2 kernel prefetches
+ * (1 or 2), the K slab depth (16 or 32), and how its iterations were
+ * launched: 0 eager (PRG_BATCH_GRAPH=0), 1 captured into a hipGraph, 2 replay
+ * of the graph captured by an earlier call with the same buffers, tile list
+ * and branch sequence.  The configuration is timed per batch of part sizes
+ * on first use among the compiled ones (64/1/16, 64/2/16, 64/1/32,
+ * 128/1/16); PRG_BATCH_TILE, PRG_BATCH_DEPTH and PRG_BATCH_KSLAB restrict
+ * the choice.  No counterpart in the reference. */
+void prg_gpu_subgraph_config(int *panel, int *depth, int *kslab, int *graph);
 
 /* ------------------------------------------------------------------ precision modes */
 /* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).

@@ -26,7 +26,7 @@ using tile::xcd_remap;
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
 
-constexpr int BK = 16, NT = 256;
+constexpr int NT = 256;
 
 struct BatchArgs {
   const double* x;      // packed current X
@@ -45,7 +45,7 @@ struct BatchArgs {
 // with a register prefetch PF slabs ahead.  In a diagonal tile the wave of
 // the strictly lower quadrant idles and the upper off-diagonal wave writes
 // its block and the transpose.
-template <int TM, int TN, int PF>
+template <int TM, int TN, int PF, int BK>
 __device__ __forceinline__ void sp2_tile(const BatchArgs& g, int t, int4 tl, double* smem) {
   constexpr int LA = TM + (TM == 128 ? 4 : 2), LB = TN + (TN == 128 ? 4 : 2);
   constexpr int MI = TM / 32, NJ = TN / 32;
@@ -156,8 +156,9 @@ __device__ __forceinline__ void sp2_tile(const BatchArgs& g, int t, int4 tl, dou
 // independent and each stores its own partial, so results do not depend on
 // which workgroup ran which tile.
 // WIDE: the tile list holds 128 x 128, 128 x 64 and 64 x 64 tiles (two
-// workgroups per CU); otherwise 64 x 64 tiles only.  PF: slabs of prefetch.
-template <bool WIDE, int PF>
+// workgroups per CU); otherwise 64 x 64 tiles only.  PF: slabs of prefetch,
+// BK: slab depth.
+template <bool WIDE, int PF, int BK>
 __global__ __launch_bounds__(NT, WIDE ? 2 : 4) void batched_sp2_kernel(BatchArgs g) {
   constexpr int LDS = WIDE ? 2 * BK * (128 + 4) : 2 * BK * (64 + 2);
   __shared__ double smem[LDS];
@@ -176,16 +177,28 @@ __global__ __launch_bounds__(NT, WIDE ? 2 : 4) void batched_sp2_kernel(BatchArgs
       const int4 tl = g.tiles[t];
       if constexpr (WIDE) {
         if (tl.w == 3)
-          sp2_tile<128, 128, PF>(g, t, tl, smem);
+          sp2_tile<128, 128, PF, BK>(g, t, tl, smem);
         else if (tl.w == 1)
-          sp2_tile<128, 64, PF>(g, t, tl, smem);
+          sp2_tile<128, 64, PF, BK>(g, t, tl, smem);
         else
-          sp2_tile<64, 64, PF>(g, t, tl, smem);
+          sp2_tile<64, 64, PF, BK>(g, t, tl, smem);
       } else {
-        sp2_tile<64, 64, PF>(g, t, tl, smem);
+        sp2_tile<64, 64, PF, BK>(g, t, tl, smem);
       }
     }
   }
+}
+
+typedef void (*BatchKernel)(BatchArgs);
+
+// The compiled configurations: (panel, slabs of prefetch, slab depth).
+BatchKernel batch_kernel(const BatchPlan& p) {
+  if (p.wide) return (p.depth == 1 && p.kslab == 16) ? batched_sp2_kernel<true, 1, 16> : nullptr;
+  if (p.kslab == 32) return p.depth == 1 ? batched_sp2_kernel<false, 1, 32> : nullptr;
+  if (p.kslab == 16)
+    return p.depth == 2 ? batched_sp2_kernel<false, 2, 16>
+                        : (p.depth == 1 ? batched_sp2_kernel<false, 1, 16> : nullptr);
+  return nullptr;
 }
 
 // X_p = (maxeval I - H[idx, idx]) / (maxeval - mineval), zero padded.
@@ -333,9 +346,11 @@ int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, 
   g.part_sums = d_part;
   g.queue_off = plan.d_queue_off;
   g.tickets = d_tickets;
-  void (*kernel)(BatchArgs) =
-      plan.wide ? (plan.depth == 2 ? batched_sp2_kernel<true, 2> : batched_sp2_kernel<true, 1>)
-                : (plan.depth == 2 ? batched_sp2_kernel<false, 2> : batched_sp2_kernel<false, 1>);
+  BatchKernel kernel = batch_kernel(plan);
+  if (!kernel) {
+    set_error(ERR_ARG, "batched SP2: no kernel for this configuration");
+    return ERR_ARG;
+  }
   if (plan.grid <= 0 || plan.grid % 8) {
     set_error(ERR_ARG, "batched SP2: launch slots not set");
     return ERR_ARG;
@@ -346,14 +361,16 @@ int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, 
   return OK;
 }
 
+bool batched_config_valid(const BatchPlan& plan) { return batch_kernel(plan) != nullptr; }
+
 int batched_slots(BatchPlan* plan) {
-  void (*kernel)(BatchArgs) =
-      plan->wide ? (plan->depth == 2 ? batched_sp2_kernel<true, 2> : batched_sp2_kernel<true, 1>)
-                 : (plan->depth == 2 ? batched_sp2_kernel<false, 2> : batched_sp2_kernel<false, 1>);
-  static int per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};
-  int& nb = per_cu[plan->wide ? 1 : 0][plan->depth == 2 ? 2 : 1];
-  if (!nb) PRG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, 0));
-  int cus = 0, dev = 0;
+  BatchKernel kernel = batch_kernel(*plan);
+  if (!kernel) {
+    set_error(ERR_ARG, "batched SP2: no kernel for this configuration");
+    return ERR_ARG;
+  }
+  int nb = 0, cus = 0, dev = 0;
+  PRG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, 0));
   PRG_HIP_CHECK(hipGetDevice(&dev));
   PRG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   // one workgroup per resident slot, a multiple of 8 (workgroup b runs on XCD b mod 8)

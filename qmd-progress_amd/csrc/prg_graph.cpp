@@ -182,15 +182,18 @@ int upload_tiles(const TileQueues& tq, int panel, BatchPlan* plan) {
   return OK;
 }
 
-// Kernel configuration per batch: the tile panel width (64 or 128) and the
-// number of K slabs prefetched ahead (1 or 2).  PRG_BATCH_TILE=64|128 and
-// PRG_BATCH_DEPTH=1|2 force either; the remaining candidates are timed on
-// the batch itself (one untimed and two timed iterations each on the
-// gathered first iterate, results discarded) and the fastest is kept for
-// every later call with the same padded part sizes.
+// Kernel configuration per batch: the tile panel width (64 or 128), the
+// number of K slabs prefetched ahead (1 or 2) and the slab depth (16 or 32);
+// the compiled combinations are the candidates below.  PRG_BATCH_TILE,
+// PRG_BATCH_DEPTH and PRG_BATCH_KSLAB restrict them; the remaining ones are
+// timed on the batch itself (one untimed and three timed iterations each on
+// the gathered first iterate, results discarded) and the fastest is kept for
+// every later call with the same padded part sizes and restrictions.
 struct BatchConfig {
-  int panel = 0, depth = 0;
+  int panel = 0, depth = 0, kslab = 0;
 };
+constexpr BatchConfig kCandidates[] = {{64, 1, 16}, {64, 2, 16}, {64, 1, 32}, {128, 1, 16}};
+
 std::map<std::pair<uint64_t, int>, BatchConfig>& config_cache() {
   static std::map<std::pair<uint64_t, int>, BatchConfig> c;
   return c;
@@ -246,7 +249,8 @@ LoopGraph& loop_graph() {
   return *g;
 }
 
-int g_cfg[3] = {0, 0, 0};  // panel width, loop launch (0 eager, 1 captured, 2 replayed), depth
+// panel width, loop launch (0 eager, 1 captured, 2 replayed), prefetch depth, slab depth
+int g_cfg[4] = {0, 0, 0, 0};
 
 // iters SP2 iterations of every part from x0, ping-ponging with x1 (the
 // result is in x0 for even iters), then vv[it] += the iteration's core-row
@@ -280,6 +284,7 @@ int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int i
                                (uint64_t)(uintptr_t)plan.d_tiles, (uint64_t)(uintptr_t)plan.d_queue_off,
                                (uint64_t)plan.ntiles,             (uint64_t)plan.wide,
                                (uint64_t)plan.depth,              (uint64_t)plan.grid,
+                               (uint64_t)plan.kslab,
                                (uint64_t)iters};
   for (int it = 0; it < iters; ++it) key.push_back(pp[it] != 0);
   LoopGraph& lg = loop_graph();
@@ -342,47 +347,52 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   BatchConfig force;
   if (const char* e = std::getenv("PRG_BATCH_TILE")) force.panel = std::atoi(e);
   if (const char* e = std::getenv("PRG_BATCH_DEPTH")) force.depth = std::atoi(e);
-  if ((force.panel != 0 && force.panel != 64 && force.panel != 128) ||
-      (force.depth != 0 && force.depth != 1 && force.depth != 2)) {
-    set_error(ERR_ARG, "PRG_BATCH_TILE must be 64 or 128, PRG_BATCH_DEPTH 1 or 2");
+  if (const char* e = std::getenv("PRG_BATCH_KSLAB")) force.kslab = std::atoi(e);
+  std::vector<BatchConfig> cand;
+  for (const BatchConfig& c : kCandidates)
+    if ((!force.panel || c.panel == force.panel) && (!force.depth || c.depth == force.depth) &&
+        (!force.kslab || c.kslab == force.kslab))
+      cand.push_back(c);
+  if (cand.empty()) {
+    set_error(ERR_ARG, "PRG_BATCH_TILE / PRG_BATCH_DEPTH / PRG_BATCH_KSLAB: no compiled batched "
+                       "SP2 kernel has this configuration");
     return ERR_ARG;
   }
-  const auto key = std::make_pair(size_signature(hp), force.panel * 8 + force.depth);
-  BatchConfig cfg = force;
-  if (!cfg.panel || !cfg.depth) {
-    auto it = config_cache().find(key);
-    if (it != config_cache().end()) cfg = it->second;
-  }
-  if (!cfg.panel || !cfg.depth) {
+  const auto key =
+      std::make_pair(size_signature(hp), (force.panel * 4 + force.depth) * 64 + force.kslab);
+  BatchConfig cfg = cand[0];
+  auto hit = config_cache().find(key);
+  if (hit != config_cache().end()) {
+    cfg = hit->second;
+  } else if (cand.size() > 1) {
     if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval))) return st;
     const TileQueues tiles128 = make_tiles(hp, 128);
     float best = 0.f;
-    for (int panel : {64, 128}) {
-      if (force.panel && panel != force.panel) continue;
-      if ((st = upload_tiles(panel == 128 ? tiles128 : tiles64, panel, &plan))) return st;
-      for (int depth : {1, 2}) {
-        if (force.depth && depth != force.depth) continue;
-        plan.depth = depth;
-        float ms = 0.f;
-        if ((st = batched_slots(&plan)) ||
-            (st = time_step(plan, x0, x1, d_part + (size_t)iters * maxtiles, d_tickets + 8 * iters,
-                            &ms)))
-          return st;
-        if (!cfg.panel || !cfg.depth || ms < best) {
-          best = ms;
-          cfg.panel = panel;
-          cfg.depth = depth;
-        }
+    for (size_t k = 0; k < cand.size(); ++k) {
+      const BatchConfig& c = cand[k];
+      if ((st = upload_tiles(c.panel == 128 ? tiles128 : tiles64, c.panel, &plan))) return st;
+      plan.depth = c.depth;
+      plan.kslab = c.kslab;
+      float ms = 0.f;
+      if ((st = batched_slots(&plan)) ||
+          (st = time_step(plan, x0, x1, d_part + (size_t)iters * maxtiles, d_tickets + 8 * iters,
+                          &ms)))
+        return st;
+      if (k == 0 || ms < best) {
+        best = ms;
+        cfg = c;
       }
     }
-    config_cache()[key] = cfg;
   }
+  config_cache()[key] = cfg;
   if ((st = upload_tiles(cfg.panel == 128 ? make_tiles(hp, 128) : tiles64, cfg.panel, &plan)))
     return st;
   plan.depth = cfg.depth;
+  plan.kslab = cfg.kslab;
   if ((st = batched_slots(&plan))) return st;
   g_cfg[0] = cfg.panel;
   g_cfg[2] = cfg.depth;
+  g_cfg[3] = cfg.kslab;
   PRG_HIP_CHECK(hipMemcpyAsync(d_vv, gp->vv, sizeof(double) * 100, hipMemcpyHostToDevice,
                                stream()));
   hipEvent_t ev[4];
@@ -756,12 +766,13 @@ void prg_gpu_subgraph_timing(double* ms_gather, double* ms_loop, double* ms_scat
   if (ms_scatter) *ms_scatter = g_ms[2];
 }
 
-// Tile panel width (64 or 128) and prefetch depth (K slabs) of the last
-// prg_subgraphSP2Loop and how its iterations were launched (0 eager,
-// 1 captured into a hipGraph, 2 replayed).
-void prg_gpu_subgraph_config(int* panel, int* depth, int* graph) {
+// Kernel configuration of the last prg_subgraphSP2Loop: tile panel width
+// (64 or 128), prefetch depth (K slabs), K slab depth, and how its iterations
+// were launched (0 eager, 1 captured into a hipGraph, 2 replayed).
+void prg_gpu_subgraph_config(int* panel, int* depth, int* kslab, int* graph) {
   if (panel) *panel = g_cfg[0];
   if (depth) *depth = g_cfg[2];
+  if (kslab) *kslab = g_cfg[3];
   if (graph) *graph = g_cfg[1];
 }
 

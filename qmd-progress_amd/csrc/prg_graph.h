@@ -20,6 +20,7 @@ struct BatchPlan {
   int ntiles = 0;
   int wide = 0;              // 1: tiles of 128-wide panels (see batched_sp2_kernel)
   int depth = 1;             // K slabs prefetched ahead (1 or 2)
+  int kslab = 16;            // K slab depth (16 or 32)
   int grid = 0;              // workgroups of the persistent batched SP2 launch
   size_t total = 0;          // doubles in one packed buffer
   PartDesc* d_parts = nullptr;
@@ -39,7 +40,9 @@ int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double minev
 // d_part; d_tickets: 8 queue counters, zero before the launch.
 int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
                  int* d_tickets);
-// plan->grid = resident workgroups of the kernel that wide/depth select
+// whether a kernel is compiled for plan's wide / depth / kslab
+bool batched_config_valid(const BatchPlan& plan);
+// plan->grid = resident workgroups of the kernel that wide/depth/kslab select
 int batched_slots(BatchPlan* plan);
 // d_tickets[0, count) = 0 by a kernel (safe inside a captured graph)
 int batched_zero_tickets(int* d_tickets, int count);

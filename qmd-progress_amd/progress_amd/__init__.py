@@ -114,7 +114,7 @@ def lib():
         "prg_build_densityGP_T0": (None, [vp, vp, vp, d, d, d, i, i]),
         "prg_build_densityGP_T0_ef": (d, []),
         "prg_gpu_subgraph_timing": (None, [dp, dp, dp]),
-        "prg_gpu_subgraph_config": (None, [ip, ip, ip]),
+        "prg_gpu_subgraph_config": (None, [ip, ip, ip, ip]),
         "prg_gpu_gershgorin_device": (i, [i, vp, dp]),
         "prg_gpu_sp2_seq_device": (i, [i, vp, vp, ip, i, d, d]),
         "prg_gpu_sp2_adjoint_device": (i, [i, vp, vp, vp, ip, i, d, d, i]),
@@ -800,8 +800,8 @@ def subgraph_timing():
 
 
 def subgraph_config():
-    """(panel width, prefetch depth, launch) of the last subgraph loop:
-    launch 0 eager, 1 captured into a hipGraph, 2 replayed graph."""
-    a, b, c = C.c_int(), C.c_int(), C.c_int()
-    lib().prg_gpu_subgraph_config(C.byref(a), C.byref(b), C.byref(c))
-    return a.value, b.value, c.value
+    """(panel width, prefetch slabs, slab depth, launch) of the last subgraph
+    loop: launch 0 eager, 1 captured into a hipGraph, 2 replayed graph."""
+    a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    lib().prg_gpu_subgraph_config(C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+    return a.value, b.value, c.value, d.value

@@ -51,17 +51,19 @@ def wide_case():
     return dict(n=n, h=h, g=g, lo=lo, hi=hi, seq=seq, imax=imax, rho=rho_o, vv=vv_o)
 
 
-@pytest.mark.parametrize("panel,depth", [(64, 1), (64, 2), (128, 1), (128, 2)])
-def test_subgraph_loop_tile_configs_match_oracle(P, wide_case, panel, depth, monkeypatch):
-    """Every configuration of the batched SP2 kernel (panel width, slabs of
-    prefetch) against the oracle."""
-    print(f"[subgraph tiles] panel={panel} depth={depth}", flush=True)
+@pytest.mark.parametrize("panel,depth,kslab", [(64, 1, 16), (64, 2, 16), (64, 1, 32),
+                                               (128, 1, 16)])
+def test_subgraph_loop_tile_configs_match_oracle(P, wide_case, panel, depth, kslab, monkeypatch):
+    """Every compiled configuration of the batched SP2 kernel (panel width,
+    slabs of prefetch, slab depth) against the oracle."""
+    print(f"[subgraph tiles] panel={panel} depth={depth} kslab={kslab}", flush=True)
     c = wide_case
     monkeypatch.setenv("PRG_BATCH_TILE", str(panel))
     monkeypatch.setenv("PRG_BATCH_DEPTH", str(depth))
+    monkeypatch.setenv("PRG_BATCH_KSLAB", str(kslab))
     gp = P.equal_partition(150, c["n"])
     rho_g, vv_g = P.subgraph_sp2_loop(c["h"], c["g"], gp, c["seq"], c["imax"], c["lo"], c["hi"])
-    assert P.subgraph_config()[:2] == (panel, depth)
+    assert P.subgraph_config()[:3] == (panel, depth, kslab)
     assert max(gp.lsize[i] for i in range(gp.totalParts)) > 384
     assert np.linalg.norm(rho_g - c["rho"]) < 1e-9
     assert np.allclose(vv_g, c["vv"], rtol=1e-9, atol=1e-12)
@@ -78,12 +80,12 @@ def test_subgraph_loop_graph_replays_equal_eager(P, wide_case, monkeypatch):
     args = (c["seq"], c["imax"], c["lo"], c["hi"])
     monkeypatch.setenv("PRG_BATCH_GRAPH", "0")
     ref, vv_ref = P.subgraph_sp2_loop(mh, mg, gp, *args)
-    assert P.subgraph_config()[2] == 0
+    assert P.subgraph_config()[3] == 0
     monkeypatch.delenv("PRG_BATCH_GRAPH")
     modes = []
     for r in range(4):
         rho, vv = P.subgraph_sp2_loop(mh, mg, gp, *args)
-        modes.append(P.subgraph_config()[2])
+        modes.append(P.subgraph_config()[3])
         assert np.array_equal(rho, ref), f"call {r}"
         assert vv == vv_ref, f"call {r}"
         P.gershgorin(c["h"])
