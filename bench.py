@@ -353,6 +353,7 @@ def graph_bench(args, P, torch, np):
     rel = err / float(np.linalg.norm(dd))
     tr_err = abs(float(np.trace(rho)) - n)
     achieved = executed / loop_s / 1e12
+    cpu = None if args.no_cpu else graph_cpu_baseline(h, gp, pp, it, lo, hi)
     out = {
         "metric": "graph-partitioned SP2 solves/s (prg_subgraphSP2Loop, batched FP64 MFMA)",
         "value": 1.0 / dt,
@@ -385,11 +386,13 @@ def graph_bench(args, P, torch, np):
         "roofline": {"bound": "mfma",
                      "kernel": f"batched_sp2_kernel (FP64 MFMA, {panel}-wide panels)",
                      "achieved": achieved, "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp64_tflops"], "traffic": None,
+                     "frac": achieved / PEAK["fp64_tflops"],
+                     "traffic": traffic_bytes("graph", n, None) if (panel, depth, kslab) ==
+                     (64, 2, 16) else None,
                      "algorithmic_tflops": alg / loop_s / 1e12,
                      "note": "executed MFMA work of the padded upper-triangle tiles per "
                              "iteration / loop device time"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -581,6 +584,38 @@ def traffic_bytes(mode, n, moduli):
         if e.get("mode") == mode and e.get("n") == n and e.get("moduli", moduli) == moduli:
             return e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024
     return None
+
+
+def graph_cpu_baseline(h, gp, pp, iters, lo, hi, stride=8):
+    """The oracle's per-part prg_sp2_submatrix_inplace restatement (numpy /
+    OpenBLAS dgemm) on every stride-th part of the partitioning, all
+    iterations, scaled to the whole partitioning by sum n^3."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import progress_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max(i.get("num_threads", 1) for i in threadpool_info()
+                      if i.get("internal_api") == "openblas") if threadpool_info() else os.cpu_count()
+    except Exception:
+        threads = os.cpu_count()
+    halos = gp.halos()
+    sample = list(range(0, len(halos), stride))
+    vv = [0.0] * iters
+    t = 0.0
+    for p in sample:
+        idx = np.asarray(halos[p])
+        sub = h[np.ix_(idx, idx)]
+        t0 = time.perf_counter()
+        O.sp2_submatrix(sub, pp, iters, vv, lo, hi, gp.llsize[p])
+        t += time.perf_counter() - t0
+    cube = [float(len(x)) ** 3 for x in halos]
+    scale = sum(cube) / sum(cube[p] for p in sample)
+    solve_s = t * scale
+    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 per-part SP2 (numpy/OpenBLAS dgemm) on {len(sample)} of "
+                      f"{len(halos)} parts (every {stride}th), {iters} iterations each, "
+                      f"{t:.2f} s, scaled by sum n^3 ({scale:.2f}x)"}
 
 
 def cpu_baseline(n, seed, iters, gpu_iters):
