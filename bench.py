@@ -291,6 +291,12 @@ def batched_executed_flops(m, panel):
     return f
 
 
+def batched_tile_count(m, panel):
+    """Upper-triangle tiles of a part of padded size m over panels of `panel`."""
+    k = len(range(0, m, panel))
+    return k * (k + 1) // 2
+
+
 def graph_bench(args, P, torch, np):
     """Graph-partitioned SP2 (prg_subgraphSP2Loop) on the water-box system:
     natoms atoms x 3 orbitals.  Setup (untimed): a dense SP2 solve of the whole
@@ -382,6 +388,7 @@ def graph_bench(args, P, torch, np):
         "alg_tflops": alg / dt / 1e12,
         "accuracy_vs_dense": {"fro": err, "rel_fro": rel, "trace_err": tr_err},
         "tiles": {"panel": panel, "prefetch_slabs": depth, "k_slab": kslab,
+                  "count": sum(batched_tile_count(m, panel) for m in mpad),
                   "launch": {0: "eager", 1: "captured hipGraph", 2: "hipGraph replay"}[launch]},
         "roofline": {"bound": "mfma",
                      "kernel": f"batched_sp2_kernel (FP64 MFMA, {panel}-wide panels)",
