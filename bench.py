@@ -56,6 +56,11 @@ def parse():
     ap.add_argument("--nparts", type=int, default=512)
     ap.add_argument("--graph-threshold", type=float, default=0.11,
                     help="|D| threshold of the density graph (graph workload)")
+    ap.add_argument("--products", default="fp64",
+                    help="fermi workload: arithmetic of the symmetric products, fp64 or "
+                         "int8crt<m> (int8 CRT products with m moduli)")
+    ap.add_argument("--ns-tol", type=float, default=1e-10,
+                    help="fermi workload: Newton-Schulz tolerance ||I - Y Inv||_F")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for the "
@@ -472,11 +477,12 @@ def _line(metric, value, unit, dt, args, dtype, config, extra, higher=True):
 
 def fermi_bench(args, P, torch, np):
     """prg_implicit_fermi (recursive Fermi-operator expansion, CG / Newton-Schulz
-    inverses, FP64) on the model Hamiltonian at --n, beta = 20 eV^-1 (kT 0.05),
-    8 levels, k = 2, occupation N/2.  The library matrix handles stay resident
-    across the timed calls (H and P in HBM).  Accuracy: P against the Fermi
-    function of an FP64 eigendecomposition at the converged mu."""
-    import ctypes as C
+    inverses) on the model Hamiltonian at --n, beta = 20 eV^-1 (kT 0.05),
+    8 levels, k = 2, occupation N/2, its symmetric products in FP64 or (with
+    --products int8crt<m>) as int8 CRT products.  The library matrix handles
+    stay resident across the timed calls (H and P in HBM).  Accuracy: P
+    against the Fermi function of an FP64 eigendecomposition at the converged
+    mu, and (int8 products) against the FP64-product solve."""
     n = args.n
     h = model_hamiltonian_torch(n, args.seed, torch.device("cuda:0"))
     hh = h.cpu().numpy()
@@ -486,46 +492,77 @@ def fermi_bench(args, P, torch, np):
     L = P.lib()
     w = torch.linalg.eigvalsh(h)
     mu = 0.5 * (w[n // 2 - 1] + w[n // 2]).item()
+    products = args.products
+    if products == "fp64":
+        moduli = None
+    elif products.startswith("int8crt") and products[7:].isdigit():
+        moduli = int(products[7:])
+    else:
+        raise SystemExit(f"--products: fp64 or int8crt<m>, not {products}")
 
     def solve(mu0, method):
         # osteps = 0: Newton steps on mu until |Tr P - nocc| <= 1e-6
         L.prg_implicit_fermi(mh.h, mp.h, nsteps, 2, nocc, mu0, beta, method, 0, 1e-6, 0.0,
-                             1e-10)
+                             args.ns_tol)
         P._check()
         return L.prg_implicit_fermi_last_mu()
 
     method = 1  # Newton-Schulz inverses (CG fallback)
-    for _ in range(args.warmup):
-        mu_w = solve(mu, method)
-    torch.cuda.synchronize()
-    P.gemm_counters(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mu1 = solve(mu, method)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    nprod, alg, exe = (v / args.steps for v in P.gemm_counters(reset=True))
+    p_fp64 = None
+    if moduli is not None:
+        P.set_product_precision(P.PREC_FP64)
+        solve(mu, method)
+        p_fp64 = torch.from_numpy(mp.numpy()).cuda()
+        P.set_int8_moduli(moduli)
+        P.set_product_precision(P.PREC_INT8CRT)
+    try:
+        for _ in range(args.warmup):
+            solve(mu, method)
+        torch.cuda.synchronize()
+        P.gemm_counters(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mu1 = solve(mu, method)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        nprod, alg, exe, i8 = (v / args.steps for v in P.gemm_counters(reset=True, int8=True))
+    finally:
+        P.set_product_precision(P.PREC_FP64)
     pm = torch.from_numpy(mp.numpy()).cuda()
     wv, q = torch.linalg.eigh(h)
     f = 1.0 / (1.0 + torch.exp(beta * (wv - mu1)))
     pref = (q * f) @ q.T
     err = torch.linalg.norm(pm - pref).item()
     tr = abs(torch.trace(pm).item() - nocc)
-    _line("implicit Fermi solves/s (prg_implicit_fermi, FP64 MFMA)", 1.0 / dt, "solves/s", dt,
-          args, "f64",
+    acc = {"fro_err_vs_fermi_eigh": err, "trace_err": tr}
+    if p_fp64 is not None:
+        acc["fro_diff_vs_fp64_products"] = torch.linalg.norm(pm - p_fp64).item()
+    if moduli is None:
+        roof = {"bound": "mfma", "kernel": "gemm_kernel (FP64 MFMA)",
+                "achieved": exe / dt / 1e12, "peak": PEAK["fp64_tflops"],
+                "unit": "TFLOP/s", "frac": exe / dt / 1e12 / PEAK["fp64_tflops"],
+                "traffic": None,
+                "note": "MFMA flops executed by the solve's products / whole-solve "
+                        "wall time (host recurrences, reductions and syncs included: "
+                        "a lower bound of the kernel rate)"}
+        metric = "implicit Fermi solves/s (prg_implicit_fermi, FP64 MFMA)"
+        dtype = "f64"
+    else:
+        roof = {"bound": "mfma", "kernel": "crt_product_kernel (int8 MFMA)",
+                "achieved": i8 / dt / 1e12, "peak": PEAK["int8_tops"], "unit": "TOP/s",
+                "frac": i8 / dt / 1e12 / PEAK["int8_tops"], "traffic": None,
+                "note": "int8 MFMA operations executed by the solve's CRT products / "
+                        "whole-solve wall time (splits, reconstructions, host recurrences "
+                        "included: a lower bound of the kernel rate)"}
+        metric = "implicit Fermi solves/s (prg_implicit_fermi, int8 CRT products)"
+        dtype = "int8/f64"
+    _line(metric, 1.0 / dt, "solves/s", dt, args, dtype,
           {"workload": f"prg_implicit_fermi model tight-binding H, N={n}, beta=20 (kT 0.05 eV), "
-                       f"8 levels, k=2, method 1 (Newton-Schulz), seed={args.seed}", "n": n,
+                       f"8 levels, k=2, method 1 (Newton-Schulz, tol {args.ns_tol:g}), "
+                       f"seed={args.seed}", "n": n, "products": products,
            "mu_start": mu, "mu": mu1},
-          {"accuracy": {"fro_err_vs_fermi_eigh": err, "trace_err": tr},
-           "products_per_solve": nprod, "alg_tflops": alg / dt / 1e12,
-           "cpu_baseline": None,
-           "roofline": {"bound": "mfma", "kernel": "gemm_kernel (FP64 MFMA)",
-                        "achieved": exe / dt / 1e12, "peak": PEAK["fp64_tflops"],
-                        "unit": "TFLOP/s", "frac": exe / dt / 1e12 / PEAK["fp64_tflops"],
-                        "traffic": None,
-                        "note": "MFMA flops executed by the solve's products / whole-solve "
-                                "wall time (host recurrences, reductions and syncs included: "
-                                "a lower bound of the kernel rate)"}})
+          {"accuracy": acc, "products_per_solve": nprod, "alg_tflops": alg / dt / 1e12,
+           "cpu_baseline": None, "roofline": roof})
 
 
 def adjoint_bench(args, P, torch, np):

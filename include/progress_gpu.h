@@ -394,12 +394,30 @@ int prg_gpu_sp2_adjoint_device(int N, const double *dH, const double *dGout, dou
 void prg_gpu_timing(int enable);
 int prg_gpu_step_timing(double *ms_sum, int *count);
 int prg_gpu_int8_product_timing(double *ms_sum, int *count);
-/* FP64 products issued by the iterative solvers (implicit Fermi and its
+/* Products issued by the iterative solvers (implicit Fermi and its
  * responses, SP2-Fermi, the adjoint) since the last reset: count, algorithmic
- * flops (2 N^3 each) and MFMA flops executed (symmetric products compute the
- * upper-triangle tiles only).  reset != 0 clears the counters after reading.
- * Any pointer may be NULL.  Measurement support; no reference counterpart. */
-void prg_gpu_gemm_counters(int reset, double *products, double *alg_flops, double *exec_flops);
+ * flops (2 N^3 each), FP64 MFMA flops executed (symmetric products compute
+ * the upper-triangle tiles only) and int8 MFMA operations executed by int8
+ * CRT products.  reset != 0 clears the counters after reading.  Any pointer
+ * may be NULL.  Measurement support; no reference counterpart. */
+void prg_gpu_gemm_counters(int reset, double *products, double *alg_flops, double *exec_flops,
+                           double *int8_ops);
+/* Arithmetic of the symmetric products of commuting symmetric matrices in
+ * the iterative solvers other than the SP2 loops -- implicit Fermi (X^2,
+ * the Newton-Schulz and CG products, Inv X^2), its saved-inverse variant,
+ * SP2-Fermi, the adjoint's forward recomputation: PRG_PREC_FP64 (default) or
+ * PRG_PREC_INT8CRT (exact int8 products modulo prg_get_int8_moduli() moduli,
+ * each operand scaled by the power of two above its largest row 2-norm,
+ * FP64 reconstruction).  Other products stay FP64.  Sets the error state for
+ * any other mode.  No counterpart in the reference. */
+void prg_set_product_precision(int mode);
+int prg_get_product_precision(void);
+/* C = A B on device pointers (padded layout, prg_gpu_padded_ld(N)) for
+ * symmetric A and B that commute (so that A B is symmetric), in the current
+ * product precision; C is exactly symmetric and must not alias A or B.  The
+ * product the implicit Fermi recursion issues as bml_multiply(A, B, C, 1, 0,
+ * threshold) (src/prg_implicit_fermi_mod.F90:296).  Returns a status. */
+int prg_gpu_symmetric_product_device(int N, const double *dA, const double *dB, double *dC);
 
 #ifdef __cplusplus
 }

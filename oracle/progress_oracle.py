@@ -1094,6 +1094,43 @@ def crt_square(x, nmod):
     return f * (float(m) / 4.0 ** a)
 
 
+def crt_scale(x):
+    """Power of two above the largest row 2-norm of a symmetric matrix (1 for a
+    zero matrix): the operand scale of the int8 CRT general product."""
+    norm = math.sqrt(float(np.max(np.sum(np.asarray(x) ** 2, axis=0))))
+    if norm == 0.0:
+        return 1.0
+    return math.ldexp(1.0, math.frexp(norm)[1])
+
+
+def crt_product(x, y, nmod):
+    """X Y for symmetric, commuting X and Y the way the int8 CRT general
+    product forms it: A = rint(2^a X / sX), B = rint(2^a Y / sY) with sX, sY
+    from crt_scale, per modulus the exact integer product A B^T of the
+    symmetric residues, the fraction sum of ((C mod p) q mod p) / p reduced
+    mod 1, times M sX sY / 2^(2a); the upper triangle mirrored onto the lower.
+    Same integer arithmetic as crt_square, which is the case X = Y with both
+    scales 1."""
+    ps, qs, a, m = crt_constants(nmod)
+    sx, sy = crt_scale(x), crt_scale(y)
+    xv = np.rint(np.asarray(x) * (2.0 ** a / sx))
+    yv = np.rint(np.asarray(y) * (2.0 ** a / sy))
+
+    def residues(v, p):
+        r = v - p * np.rint(v / p)
+        r = np.where(r > 127, r - p, r)
+        return np.where(r < -128, r + p, r).astype(np.int64)
+
+    s = np.zeros_like(xv)
+    for p, q in zip(ps, qs):
+        c = residues(xv, p) @ residues(yv, p).T
+        v = (c * q).astype(np.float64)
+        md = v - p * np.rint(v / p)
+        s += md / p
+    f = (s - np.rint(s)) * (float(m) / 4.0 ** a) * sx * sy
+    return np.triu(f) + np.triu(f, 1).T
+
+
 def orthogonalize(a, z):
     """prg_orthogonalize (src/prg_nonortho_mod.F90:35-68): Z^T A Z."""
     return (z.T @ a) @ z

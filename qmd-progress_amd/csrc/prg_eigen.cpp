@@ -125,7 +125,8 @@ int density_from_eig(const DMat& Q, const std::vector<double>& occ, DMat out) {
   if ((st = ops::temp("eig_qf", n, &QF)) || (st = ops::temp("eig_qt", n, &QT))) return st;
   if ((st = ops::scale_cols(Q, d_f, QF))) return st;
   if ((st = ops::transpose(Q, QT))) return st;
-  if ((st = ops::mm(QF, QT, out, true))) return st;
+  // symmetric product of non-symmetric factors: always the FP64 kernel
+  if ((st = ops::mm_symmetric_result(QF, QT, out))) return st;
   // the occupation vector must stay alive until the kernels consumed it
   PRG_HIP_CHECK(hipStreamSynchronize(stream()));
   return OK;

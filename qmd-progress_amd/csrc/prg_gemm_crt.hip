@@ -82,9 +82,10 @@ struct CrtConst {
 
 struct ProductArgs {
   const int8_t* planes[2];  // residue planes ping-pong, plane k at k * pld * pld
+  const int8_t* planes_b;   // B operand's planes (general products), or null: B = A
   int8_t* res;              // product residues, item (t, k) at (t * np + k) * kItemBytes
   const int4* items;        // (modulus, I, J, upper-triangle index of (I, J))
-  const Sp2State* st;
+  const Sp2State* st;       // SP2 state (planned, current buffer), or null: planes[0]
   const double* fold;       // (p, 1/p, q) per modulus
   int pld, np;
 };
@@ -139,7 +140,9 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
   return (int)m;
 }
 
-// One (tile, modulus) item: C = A_I A_J^T mod p over the full K range.  W = 8
+// One (tile, modulus) item: C = A_I B_J^T mod p over the full K range (B = A
+// for the SP2 square; for a general product of symmetric A and B, B_J^T is
+// B's column panel J).  W = 8
 // waves in a 2 x 4 grid of 128 x 64 (two waves per SIMD) or W = 4 waves in a
 // 2 x 2 grid of 128 x 128 (one wave per SIMD, a third fewer LDS fragment
 // reads per MFMA); 16 x 16 MFMA blocks, int32 accumulators.  Operand stages
@@ -156,13 +159,17 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
   constexpr int NTL = TM / WN / 16;    // 16-column MFMA blocks per wave
   constexpr int CPW = 32 / W;          // 1 KB DMA chunks per wave per stage
   __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
-  const volatile Sp2State* stv = g.st;
-  if (!stv->planned) return;
-  const int cur = stv->cur;
+  int cur = 0;
+  if (g.st) {
+    const volatile Sp2State* stv = g.st;
+    if (!stv->planned) return;
+    cur = stv->cur;
+  }
   const int pld = g.pld;
   const int4 it = g.items[xcd_remap(blockIdx.x, gridDim.x)];
   const int k = it.x, I = it.y, J = it.z, t = it.w;
   const int8_t* R = g.planes[cur] + (size_t)k * pld * pld;
+  const int8_t* RB = g.planes_b ? g.planes_b + (size_t)k * pld * pld : R;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
 
@@ -174,7 +181,7 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
   const int drow = 16 * w + (lane >> 2);
   const int kc = 16 * ((lane & 3) ^ ((drow >> 1) & 3));
   const int8_t* srcA = R + (size_t)(I * TM + drow) * BK + kc;
-  const int8_t* srcB = R + (size_t)(J * TM + drow) * BK + kc;
+  const int8_t* srcB = RB + (size_t)(J * TM + drow) * BK + kc;
   const size_t kstride = (size_t)pld * BK;  // one 64-column block of a plane
   const uint32_t dst0 = lds_addr(&ring[0][0]) + 1024 * w;
   const int ksteps = pld / BK;
@@ -184,8 +191,8 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
   const int fo = (lane & 15) * BK + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
   const int offa = wm * 128 * BK + fo;
   i4 acc[MT][NTL];
-  // DIAG (I == J): both operands are the same panel, which is loaded once
-  // (half the DMA chunks; the B fragments are read from the A panel).
+  // DIAG (I == J, B = A): both operands are the same panel, which is loaded
+  // once (half the DMA chunks; the B fragments are read from the A panel).
   auto run = [&](auto diag_tag) {
     constexpr bool DIAG = decltype(diag_tag)::value;
     constexpr int NQ = DIAG ? CPW / 2 : CPW;  // DMA chunks per wave per stage
@@ -245,7 +252,7 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
       step(s + 1, fb, fa, false);
     }
   };
-  if (I == J)
+  if (I == J && !g.planes_b)
     run(std::true_type{});
   else
     run(std::false_type{});
@@ -520,6 +527,179 @@ __global__ void crt_finish_kernel(const int8_t* __restrict__ p0, const int8_t* _
   }
 }
 
+// ---------------------------------------------------------------- general products
+// C = alpha A B + beta Cin + gamma I for symmetric, commuting A and B (so that
+// A B is symmetric).  A is scaled by 2^a / sA and B by 2^a / sB, sA and sB
+// being powers of two above the largest row 2-norm of each, found on the
+// device; then every row of the integer operands has 2-norm at most 2^a (plus
+// sqrt(ld) / 2 from rounding), and the bound of the SP2 square applies to
+// the integer product unchanged.  C = (C~ / M mod 1) M sA sB / 2^(2a).
+
+// Per-block maxima of the column sums of squares of the n x n block of x
+// (for a symmetric matrix, its squared row 2-norms).
+__global__ void crt_colnorm_kernel(const double* __restrict__ x, int ld, int n,
+                                   double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  double best = 0.0;
+  for (int c = blockIdx.x; c < n; c += gridDim.x) {
+    double s = 0.0;
+    for (int r = threadIdx.x; r < n; r += NT) {
+      const double v = x[r + (size_t)c * ld];
+      s = fma(v, v, s);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) t += red[w];
+      best = fmax(best, t);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = best;
+}
+
+// out = (s, 1 / s), s the power of two above sqrt(max part) (1 for a zero
+// matrix; non-finite input gives a non-finite s, which the outputs carry).
+__global__ void crt_scale_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double red[NT];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += NT) m = fmax(m, part[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double norm = sqrt(red[0]);
+    double s = 1.0;
+    if (!(norm < INFINITY)) {
+      s = norm;  // inf or NaN
+    } else if (norm > 0.0) {
+      int e = 0;
+      frexp(norm, &e);  // norm = f 2^e, f in [0.5, 1)
+      s = ldexp(1.0, e);
+    }
+    out[0] = s;
+    out[1] = 1.0 / s;
+  }
+}
+
+// Residue planes of rint(2^a x / s) (s = sc[0], 1 / s = sc[1]) in the
+// 64-column block layout, zero outside the ld x ld buffer.
+template <int NP>
+__global__ void crt_split_scaled_kernel(const double* __restrict__ x, int8_t* __restrict__ p0,
+                                        int ld, int pld, CrtConst c,
+                                        const double* __restrict__ sc) {
+  const size_t count = (size_t)pld * pld;
+  const double f = c.scale * sc[1];
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
+    const size_t q = i / SB;
+    const int r = (int)(q % pld);
+    const int col = (int)(q / pld) * SB + (int)(i % SB);
+    const bool in = (r < ld) && (col < ld);
+    const double v = in ? rint(x[col + (size_t)r * ld] * f) : 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) p0[(size_t)k * count + i] = (int8_t)sym_residue(v, c.p[k], c.inv[k]);
+  }
+}
+
+struct GemmEpiArgs {
+  const int8_t* res;
+  const double* sa;   // (sA, 1 / sA)
+  const double* sb;   // (sB, 1 / sB)
+  const double* Cin;  // beta term, or null
+  const double* E;    // <C, E> partial, or null
+  double* C;
+  double* part;       // per block [Tr C, ||C||_F^2, <C, E>, 0]
+  double alpha, beta, gamma;
+  int ld, n;
+  CrtConst c;
+};
+
+// One upper-triangle 64 x 64 block (I, J) of C (same residue record mapping
+// as crt_update_kernel): the CRT sum over the moduli, C at (row, col) and its
+// mirror; inside a diagonal block the upper half is mirrored onto the lower
+// one, so C is exactly symmetric.
+template <int NP>
+__global__ __launch_bounds__(NT) void crt_gemm_epilogue_kernel(GemmEpiArgs g) {
+  __shared__ double red[NT / 64][4];
+  const int ld = g.ld;
+  int I, J;
+  tile::sym_tile(blockIdx.x, I, J);
+  const int bi = I >> 2, bj = J >> 2;
+  const int t = bj * (bj + 1) / 2 + bi;
+  const int wave = ((I & 3) >> 1) + 2 * (J & 3);
+  const int tid = threadIdx.x, lane = tid & 63, b = tid >> 6;
+  const int m = 4 * (I & 1) + b;
+  const i4* src = (const i4*)(g.res + (size_t)t * NP * kItemBytes) + (wave * 8 + m) * 64 + lane;
+  const int row = SB * I + 16 * b + (lane & 15);
+  const int cl0 = 4 * (lane >> 4);
+  i4 d[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) d[k] = src[k * (kItemBytes / 16)];
+  double S[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) S[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const double inv = g.c.inv[k];
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        S[4 * n + r] = fma((double)(int8_t)(d[k][n] >> (8 * r)), inv, S[4 * n + r]);
+  }
+  const double ys = g.c.cscale * g.sa[0] * g.sb[0];
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = SB * J + 16 * n + cl0 + r;
+      if (I == J && row > col) continue;
+      const bool mirror = (row != col);
+      const double w = mirror ? 2.0 : 1.0;
+      const double sr = S[4 * n + r];
+      const size_t idx = row + (size_t)col * ld;
+      double v = g.alpha * ((sr - rint(sr)) * ys);
+      if (g.Cin) v += g.beta * g.Cin[idx];
+      const bool diag = (row == col);
+      v += (diag && row < g.n) ? g.gamma : 0.0;
+      q0 += diag ? v : 0.0;
+      q1 += w * v * v;
+      if (g.E) q2 += w * v * g.E[idx];
+      g.C[idx] = v;
+      if (mirror) g.C[col + (size_t)row * ld] = v;
+    }
+  }
+  double sums[3] = {q0, q1, q2};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double x = sums[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    sums[q] = x;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) red[b][q] = sums[q];
+  }
+  __syncthreads();
+  if (g.part && tid < 4) {
+    double a = 0.0;
+    if (tid < 3) {
+#pragma unroll
+      for (int ww = 0; ww < NT / 64; ++ww) a += red[ww][tid];
+    }
+    g.part[4 * (size_t)blockIdx.x + tid] = a;
+  }
+}
+
 int g_moduli = 11;
 
 // Modular inverse of a mod p (p > 1, gcd(a, p) = 1).
@@ -673,6 +853,18 @@ void launch_update(const UpdateArgs& g, int nblocks) {
     hipLaunchKernelGGL((crt_update_kernel<NP, true>), dim3(nblocks), dim3(NT), 0, stream(), g);
   else
     hipLaunchKernelGGL((crt_update_kernel<NP, false>), dim3(nblocks), dim3(NT), 0, stream(), g);
+}
+
+template <int NP>
+void launch_split_scaled(const double* x, int8_t* p, int ld, int pld, const CrtConst& c,
+                         const double* sc) {
+  hipLaunchKernelGGL(crt_split_scaled_kernel<NP>, dim3(2048), dim3(NT), 0, stream(), x, p, ld,
+                     pld, c, sc);
+}
+
+template <int NP>
+void launch_gemm_epilogue(const GemmEpiArgs& g, int nblocks) {
+  hipLaunchKernelGGL(crt_gemm_epilogue_kernel<NP>, dim3(nblocks), dim3(NT), 0, stream(), g);
 }
 
 template <int NP>
@@ -844,6 +1036,7 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   ProductArgs pa;
   pa.planes[0] = planes;
   pa.planes[1] = planes + (size_t)np * plane;
+  pa.planes_b = nullptr;
   pa.res = res;
   pa.st = d_state;
   pa.pld = pld;
@@ -903,5 +1096,84 @@ int crt_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double s
 }
 
 int crt_last_waves() { return g_last_config; }
+
+int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
+             double gamma, double* part, int* nparts, const DMat* dotE, int np) {
+  const int ld = C.ld, pld = plane_ld(ld);
+  if (A.ld != ld || B.ld != ld || (Cin && Cin->ld != ld) || (dotE && dotE->ld != ld)) {
+    set_error(ERR_ARG, "int8 CRT products: operand leading dimensions differ");
+    return ERR_ARG;
+  }
+  if (ld % SB != 0 || pld >= (1 << 17)) {
+    set_error(ERR_ARG, "int8 CRT products: matrix too large for int32 accumulation");
+    return ERR_ARG;
+  }
+  if (np < 6 || np > kMaxModuli) {
+    set_error(ERR_ARG, "int8 CRT products: the number of moduli must be in [6, 14]");
+    return ERR_ARG;
+  }
+  const size_t plane = (size_t)pld * pld;
+  const int nt = pld / TM;
+  const int nb = nt * (nt + 1) / 2;
+  const bool same = (A.p == B.p);
+  int st = OK;
+  int8_t* pa_planes = (int8_t*)workspace("crtg_planes_a", (size_t)np * plane, &st);
+  int8_t* pb_planes =
+      same ? nullptr : (int8_t*)workspace("crtg_planes_b", (size_t)np * plane, &st);
+  int8_t* res = (int8_t*)workspace("crt_residues", (size_t)nb * np * kItemBytes, &st);
+  double* sc = (double*)workspace("crtg_scales", sizeof(double) * 4, &st);
+  double* npart = (double*)workspace("crtg_norm_part", sizeof(double) * 1024, &st);
+  if (st) return st;
+  const double* fold = fold_constants(np, &st);
+  if (st) return st;
+  const int4* items = item_list(nt, np, &st);
+  if (st) return st;
+  const CrtConst c = make_const(np);
+  const int nnb = std::max(1, std::min(1024, C.n));
+  auto scale_split = [&](const DMat& X, int8_t* planes, double* s) -> int {
+    hipLaunchKernelGGL(crt_colnorm_kernel, dim3(nnb), dim3(NT), 0, stream(), X.p, ld, X.n, npart);
+    hipLaunchKernelGGL(crt_scale_kernel, dim3(1), dim3(NT), 0, stream(), npart, nnb, s);
+#define PRG_CALL(K) launch_split_scaled<K>(X.p, planes, ld, pld, c, s)
+    PRG_CRT_DISPATCH(np, PRG_CALL)
+#undef PRG_CALL
+    PRG_HIP_CHECK(hipGetLastError());
+    return OK;
+  };
+  if ((st = scale_split(A, pa_planes, sc))) return st;
+  if (!same && (st = scale_split(B, pb_planes, sc + 2))) return st;
+  ProductArgs pa;
+  pa.planes[0] = pa_planes;
+  pa.planes[1] = pa_planes;
+  pa.planes_b = pb_planes;
+  pa.res = res;
+  pa.items = items;
+  pa.st = nullptr;
+  pa.fold = fold;
+  pa.pld = pld;
+  pa.np = np;
+  if ((st = run_product(pa, nb * np, &g_last_config))) return st;
+  PRG_HIP_CHECK(hipGetLastError());
+  GemmEpiArgs ea;
+  ea.res = res;
+  ea.sa = sc;
+  ea.sb = same ? sc : sc + 2;
+  ea.Cin = (Cin && beta != 0.0) ? Cin->p : nullptr;
+  ea.E = dotE ? dotE->p : nullptr;
+  ea.C = C.p;
+  ea.part = part;
+  ea.alpha = alpha;
+  ea.beta = beta;
+  ea.gamma = gamma;
+  ea.ld = ld;
+  ea.n = C.n;
+  ea.c = c;
+  const int nblocks = crt_update_blocks(ld);
+  if (nparts) *nparts = nblocks;
+#define PRG_CALL(K) launch_gemm_epilogue<K>(ea, nblocks)
+  PRG_CRT_DISPATCH(np, PRG_CALL)
+#undef PRG_CALL
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
 
 }  // namespace prg

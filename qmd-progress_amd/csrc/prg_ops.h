@@ -17,15 +17,24 @@ struct Sums {
 };
 
 // Products issued through mm / mm2 since the last reset (count, 2 n^3 per
-// product, MFMA flops executed).
-void gemm_counters(double* products, double* alg_flops, double* exec_flops, bool reset);
+// product, FP64 MFMA flops executed, int8 MFMA operations executed).
+void gemm_counters(double* products, double* alg_flops, double* exec_flops, bool reset,
+                   double* int8_ops = nullptr);
+// Arithmetic of mm's symmetric products: PRG_PREC_FP64 (default) or
+// PRG_PREC_INT8CRT (crt_gemm with the int8 CRT moduli count).
+int product_precision();
+int set_product_precision(int mode);
 
 // n x n temporary held in a named workspace slot (contents undefined).
 int temp(const char* name, int n, DMat* out);
-// C = alpha A B + beta Cin + gamma I   (sym: product known symmetric)
+// C = alpha A B + beta Cin + gamma I   (sym: A and B symmetric and commuting,
+// so the product is symmetric; such products follow product_precision())
 int mm(const DMat& A, const DMat& B, DMat C, bool sym, double alpha = 1.0, double beta = 0.0,
        const DMat* Cin = nullptr, double gamma = 0.0, Sums* sums = nullptr,
        const DMat* dotE = nullptr);
+// C = A B for factors that are not symmetric but whose product is (FP64,
+// upper-triangle tiles mirrored)
+int mm_symmetric_result(const DMat& A, const DMat& B, DMat C);
 // C = alpha (A0 B0 + A1 B1) + beta Cin + gamma I
 int mm2(const DMat& A0, const DMat& B0, const DMat& A1, const DMat& B1, DMat C, bool sym,
         double alpha = 1.0, double beta = 0.0, const DMat* Cin = nullptr, double gamma = 0.0,

@@ -53,6 +53,12 @@ int crt_prepare(const DMat& x0, int np);
 int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
              int np);
 int crt_moduli();
+// C = alpha A B + beta Cin + gamma I (gamma on the n x n diagonal) for
+// symmetric, commuting A and B, from exact int8 products modulo np moduli
+// (np in [6, 14]); part (if not null) receives per-64x64-block [Tr C,
+// ||C||_F^2, <C, dotE>, 0], *nparts blocks.  C is exactly symmetric.
+int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
+             double gamma, double* part, int* nparts, const DMat* dotE, int np);
 int crt_last_waves();  // 800 + ring stages of the last int8 CRT product kernel (0: none yet)
 int crt_update_blocks(int ld);  // partial-sum slots one int8 CRT step writes
 // out = scale X[cur] after an int8 CRT solve (from the residue planes when no

@@ -139,7 +139,10 @@ def lib():
         "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
         "prg_gpu_step_timing": (i, [dp, ip]),
         "prg_gpu_int8_product_timing": (i, [dp, ip]),
-        "prg_gpu_gemm_counters": (None, [i, dp, dp, dp]),
+        "prg_gpu_gemm_counters": (None, [i, dp, dp, dp, dp]),
+        "prg_set_product_precision": (None, [i]),
+        "prg_gpu_symmetric_product_device": (i, [i, vp, vp, vp]),
+        "prg_get_product_precision": (i, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -438,12 +441,37 @@ def step_timing(enable=None):
     return ms.value, cnt.value
 
 
-def gemm_counters(reset=False):
-    """(products, algorithmic flops, executed MFMA flops) of the FP64 products
-    the iterative solvers issued since the last reset."""
-    v = [C.c_double(), C.c_double(), C.c_double()]
+def gemm_counters(reset=False, int8=False):
+    """(products, algorithmic flops, executed FP64 MFMA flops) of the products
+    the iterative solvers issued since the last reset; with int8=True also the
+    int8 MFMA operations of their int8 CRT products."""
+    v = [C.c_double(), C.c_double(), C.c_double(), C.c_double()]
     lib().prg_gpu_gemm_counters(1 if reset else 0, *[C.byref(x) for x in v])
-    return tuple(x.value for x in v)
+    out = tuple(x.value for x in v)
+    return out if int8 else out[:3]
+
+
+def set_product_precision(mode):
+    """Arithmetic of the symmetric products of commuting symmetric matrices in
+    implicit Fermi, SP2-Fermi and the adjoint: PREC_FP64 or PREC_INT8CRT."""
+    lib().prg_set_product_precision(int(mode))
+    _check()
+
+
+def get_product_precision():
+    return int(lib().prg_get_product_precision())
+
+
+def symmetric_product_device(a, b, c, n):
+    """c = a b for commuting symmetric a, b (torch float64 cuda tensors in the
+    padded layout, padded_ld(n) square), in the current product precision."""
+    for t in (a, b, c):
+        if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8 and
+                t.is_contiguous() and t.shape == (padded_ld(n), padded_ld(n))):
+            raise ValueError("symmetric_product_device: padded contiguous float64 cuda tensors")
+    lib().prg_gpu_symmetric_product_device(n, C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                                           C.c_void_p(c.data_ptr()))
+    _check()
 
 
 def int8_product_timing():

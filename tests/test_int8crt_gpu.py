@@ -131,3 +131,91 @@ def test_crt_single_product_matches_oracle_arithmetic(P, npm, n):
     # and X^2 itself to the 2^-a rounding
     bound = max(2.0 ** -(a - 8), noise)
     assert np.max(np.abs(rho_g - 2.0 * (x0 @ x0))) < bound
+
+
+def _commuting_pair(n, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    x = (q * rng.uniform(0.0, 1.0, n)) @ q.T
+    y = (q * rng.uniform(lo, hi, n)) @ q.T
+    return 0.5 * (x + x.T), 0.5 * (y + y.T)
+
+
+def _device(P, z, n):
+    import torch
+    ld = P.padded_ld(n)
+    t = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    if z is not None:
+        t[:n, :n] = torch.from_numpy(z)
+    return t
+
+
+@pytest.mark.parametrize("npm,n", [(10, 200), (11, 256), (14, 300)])
+def test_crt_general_product_matches_oracle(P, npm, n):
+    """C = X Y for commuting symmetric X (spectrum [0, 1]) and Y (spectrum
+    [2, 6], scaled by 8) through the device entry point in int8 CRT product
+    precision: the oracle's integer arithmetic (crt_product) to the FP64
+    rounding of the fraction sum, exactly symmetric, padding untouched."""
+    print(f"[int8crt general product] moduli={npm} n={n}", flush=True)
+    x, y = _commuting_pair(n, npm, 2.0, 6.0)
+    a, b, c = _device(P, x, n), _device(P, y, n), _device(P, None, n)
+    P.set_int8_moduli(npm)
+    P.set_product_precision(P.PREC_INT8CRT)
+    try:
+        P.gemm_counters(reset=True)
+        P.symmetric_product_device(a, b, c, n)
+        nprod, _, exe, int8_ops = P.gemm_counters(reset=True, int8=True)
+    finally:
+        P.set_product_precision(P.PREC_FP64)
+    assert nprod == 1 and exe == 0.0 and int8_ops > 0.0
+    got = c[:n, :n].cpu().numpy()
+    ref = O.crt_product(x, y, npm)
+    _, _, abits, m = O.crt_constants(npm)
+    sxy = O.crt_scale(x) * O.crt_scale(y)
+    noise = max(1e-13, 8.0 * npm * 2.0 ** -52 * (float(m) / 4.0 ** abits)) * sxy
+    assert np.max(np.abs(got - ref)) < noise
+    assert np.array_equal(got, got.T)
+    ld = P.padded_ld(n)
+    if ld > n:
+        assert float(c[n:, :].abs().max()) == 0.0 and float(c[:, n:].abs().max()) == 0.0
+
+
+def test_symmetric_product_fp64_matches_numpy(P):
+    """The same entry point in FP64 product precision: the FP64 MFMA kernel."""
+    n = 200
+    x, y = _commuting_pair(n, 3, -1.0, 1.0)
+    a, b, c = _device(P, x, n), _device(P, y, n), _device(P, None, n)
+    assert P.get_product_precision() == P.PREC_FP64
+    P.symmetric_product_device(a, b, c, n)
+    got = c[:n, :n].cpu().numpy()
+    assert np.max(np.abs(got - x @ y)) < 1e-13
+    assert np.array_equal(got, got.T)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_implicit_fermi_int8_products_match_oracle(P, method):
+    """prg_implicit_fermi with its symmetric products (X^2, the CG and
+    Newton-Schulz products, Inv X^2) as int8 CRT products, 12 moduli (operands
+    rounded to 2^-46 of their scale).  Against the oracle's FP64 recursion:
+    the FP64 path's own bars (P to 1e-8 Frobenius, mu to 1e-9: the CG / NS
+    stopping points differ by rounding); against the GPU's FP64 products:
+    1e-9 (measured 4e-11), which isolates the product arithmetic."""
+    print(f"[int8crt implicit fermi] method={method}", flush=True)
+    h = O.model_hamiltonian(256, 0)
+    args = (8, 2, 128.0, 0.6, 20.0, method, 3, 1e-9, 1e-16 if method == 0 else 1e-10)
+    p_o, mu_o = O.implicit_fermi(h, *args)
+    assert P.get_product_precision() == P.PREC_FP64
+    p64, _ = P.implicit_fermi(h, *args)
+    P.set_int8_moduli(12)
+    P.set_product_precision(P.PREC_INT8CRT)
+    try:
+        P.gemm_counters(reset=True)
+        p, mu = P.implicit_fermi(h, *args)
+        nprod, _, exe, int8_ops = P.gemm_counters(reset=True, int8=True)
+    finally:
+        P.set_product_precision(P.PREC_FP64)
+    assert nprod > 0 and exe == 0.0 and int8_ops > 0.0
+    assert np.array_equal(p, p.T)
+    assert abs(mu - mu_o) < 1e-9
+    assert np.linalg.norm(p - p_o) < 1e-8
+    assert np.linalg.norm(p - p64) < 1e-9

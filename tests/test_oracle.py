@@ -524,6 +524,51 @@ def test_crt_square_planted_mistakes_are_caught():
         O.crt_constants = orig
 
 
+def _commuting_pair(n, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    x = (q * rng.uniform(0.0, 1.0, n)) @ q.T
+    y = (q * rng.uniform(lo, hi, n)) @ q.T
+    return 0.5 * (x + x.T), 0.5 * (y + y.T)
+
+
+@pytest.mark.parametrize("nmod", [8, 11, 14])
+def test_crt_product_reproduces_commuting_product(nmod):
+    """The int8 CRT general product of commuting symmetric X (spectrum in
+    [0, 1]) and Y (spectrum in [2, 6], scale 8): the reconstruction equals X Y
+    up to the rounding of the operands, |error| <= 2^-(a+1) (sX ||Y_j||_1 +
+    sY ||X_i||_1) + 4^-(a+1) sX sY n + FP64 noise."""
+    n = 160
+    x, y = _commuting_pair(n, nmod, 2.0, 6.0)
+    sx, sy = O.crt_scale(x), O.crt_scale(y)
+    assert sx == 1.0 and sy == 8.0
+    assert np.max(np.sqrt(np.sum(y * y, axis=0))) <= sy
+    ps, qs, a, m = O.crt_constants(nmod)
+    got = O.crt_product(x, y, nmod)
+    assert np.array_equal(got, got.T)
+    ref = x @ y
+    lx, ly = np.abs(x).sum(axis=1), np.abs(y).sum(axis=1)
+    bound = (2.0 ** -(a + 1) * (sx * ly[None, :] + sy * lx[:, None]) +
+             4.0 ** -(a + 1) * sx * sy * n + 1e-13 * sx * sy)
+    assert np.all(np.abs(got - ref) <= bound)
+    # the square is the X = Y case of the same arithmetic
+    assert np.max(np.abs(O.crt_product(x, x, nmod) - O.crt_square(x, nmod))) < 1e-13
+
+
+def test_crt_product_planted_scale_mistake_is_caught():
+    """Scaling Y by 1 instead of the power of two above its row norms lets the
+    integer products pass M / 2 and wrap: the reconstruction breaks."""
+    x, y = _commuting_pair(96, 5, 2.0, 6.0)
+    ref = x @ y
+    assert np.max(np.abs(O.crt_product(x, y, 11) - ref)) < 1e-9
+    orig = O.crt_scale
+    O.crt_scale = lambda z: 1.0
+    try:
+        assert np.max(np.abs(O.crt_product(x, y, 11) - ref)) > 1e-3
+    finally:
+        O.crt_scale = orig
+
+
 def test_orthogonalize_round_trip_with_lowdin_factor():
     """Z = S^(-1/2) (Loewdin): Z^T S Z = I, and deorthogonalize inverts
     orthogonalize for Z^T Z = S^-1 ... i.e. Z (Z^T A Z) Z^T = S^-1 A S^-1."""
