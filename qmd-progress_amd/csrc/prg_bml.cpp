@@ -4,6 +4,7 @@
 // solver calls on the same handles never leave the GPU.
 #include "prg_bml.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -44,6 +45,34 @@ static int ensure_dev_alloc(bml_matrix_t* A) {
   return OK;
 }
 
+// Host <-> device copy of the N x N block through the pinned staging buffer
+// (pageable host memory is never handed to the copy engine), in column
+// blocks of at most 64 MiB, each complete before the buffer is reused.
+static int copy_columns(bml_matrix_t* A, bool to_device) {
+  const size_t col_bytes = sizeof(double) * (size_t)A->N;
+  const int per = (int)std::max<size_t>(1, ((size_t)64 << 20) / col_bytes);
+  int st = OK;
+  double* stage = (double*)host_staging(col_bytes * std::min(per, A->N), &st);
+  if (st) return st;
+  for (int j0 = 0; j0 < A->N; j0 += per) {
+    const int nc = std::min(per, A->N - j0);
+    double* h = A->host + (size_t)j0 * A->N;
+    double* d = A->dev + (size_t)j0 * A->ld;
+    if (to_device) {
+      std::memcpy(stage, h, col_bytes * nc);
+      PRG_HIP_CHECK(hipMemcpy2DAsync(d, sizeof(double) * A->ld, stage, col_bytes, col_bytes, nc,
+                                     hipMemcpyHostToDevice, stream()));
+      PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+    } else {
+      PRG_HIP_CHECK(hipMemcpy2DAsync(stage, col_bytes, d, sizeof(double) * A->ld, col_bytes, nc,
+                                     hipMemcpyDeviceToHost, stream()));
+      PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+      std::memcpy(h, stage, col_bytes * nc);
+    }
+  }
+  return OK;
+}
+
 int bml_dev_read(bml_matrix_t* A, DMat* out) {
   if (!A) {
     set_error(ERR_ARG, "null matrix handle");
@@ -52,9 +81,7 @@ int bml_dev_read(bml_matrix_t* A, DMat* out) {
   int st = ensure_dev_alloc(A);
   if (st) return st;
   if (!A->dev_valid) {
-    PRG_HIP_CHECK(hipMemcpy2DAsync(A->dev, sizeof(double) * A->ld, A->host,
-                                   sizeof(double) * A->N, sizeof(double) * A->N, A->N,
-                                   hipMemcpyHostToDevice, stream()));
+    if ((st = copy_columns(A, true))) return st;
     A->dev_valid = true;
   }
   out->n = A->N;
@@ -84,9 +111,8 @@ int bml_host_sync(bml_matrix_t* A) {
     return ERR_ARG;
   }
   if (A->host_valid) return OK;
-  PRG_HIP_CHECK(hipMemcpy2DAsync(A->host, sizeof(double) * A->N, A->dev, sizeof(double) * A->ld,
-                                 sizeof(double) * A->N, A->N, hipMemcpyDeviceToHost, stream()));
-  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  const int st = copy_columns(A, false);
+  if (st) return st;
   A->host_valid = true;
   return OK;
 }

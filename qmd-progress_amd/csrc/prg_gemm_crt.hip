@@ -536,30 +536,33 @@ __global__ void crt_finish_kernel(const int8_t* __restrict__ p0, const int8_t* _
 // the integer product unchanged.  C = (C~ / M mod 1) M sA sB / 2^(2a).
 
 // Per-block maxima of the column sums of squares of the n x n block of x
-// (for a symmetric matrix, its squared row 2-norms).
+// (for a symmetric matrix, its squared row 2-norms); one wave per column.
 __global__ void crt_colnorm_kernel(const double* __restrict__ x, int ld, int n,
                                    double* __restrict__ part) {
   __shared__ double red[NT / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double best = 0.0;
-  for (int c = blockIdx.x; c < n; c += gridDim.x) {
+  for (int c = blockIdx.x * (NT / 64) + w; c < n; c += gridDim.x * (NT / 64)) {
+    const double* col = x + (size_t)c * ld;
     double s = 0.0;
-    for (int r = threadIdx.x; r < n; r += NT) {
-      const double v = x[r + (size_t)c * ld];
-      s = fma(v, v, s);
+    for (int r = 2 * lane; r + 1 < n; r += 128) {
+      const double2 u = *(const double2*)(col + r);
+      s = fma(u.x, u.x, s);
+      s = fma(u.y, u.y, s);
     }
+    if ((n & 1) && lane == 0) s = fma(col[n - 1], col[n - 1], s);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-#pragma unroll
-      for (int w = 0; w < NT / 64; ++w) t += red[w];
-      best = fmax(best, t);
-    }
-    __syncthreads();
+    best = fmax(best, s);
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = best;
+  if (lane == 0) red[w] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; ++k) m = fmax(m, red[k]);
+    part[blockIdx.x] = m;
+  }
 }
 
 // out = (s, 1 / s), s the power of two above sqrt(max part) (1 for a zero
@@ -590,21 +593,48 @@ __global__ void crt_scale_kernel(const double* __restrict__ part, int nb, double
 }
 
 // Residue planes of rint(2^a x / s) (s = sc[0], 1 / s = sc[1]) in the
-// 64-column block layout, zero outside the ld x ld buffer.
+// 64-column block layout, zero outside the ld x ld buffer.  A thread forms 16
+// consecutive bytes of every plane: one row, 16 columns of a 64-column block
+// (read as 128 contiguous bytes of the transposed position; ld is a multiple
+// of 64, so the 16 columns are all inside or all outside).
 template <int NP>
 __global__ void crt_split_scaled_kernel(const double* __restrict__ x, int8_t* __restrict__ p0,
                                         int ld, int pld, CrtConst c,
                                         const double* __restrict__ sc) {
   const size_t count = (size_t)pld * pld;
+  const size_t nchunk = count / 16;
   const double f = c.scale * sc[1];
-  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
+  for (size_t ch = (size_t)blockIdx.x * NT + threadIdx.x; ch < nchunk;
+       ch += (size_t)gridDim.x * NT) {
+    const size_t i = 16 * ch;
     const size_t q = i / SB;
     const int r = (int)(q % pld);
-    const int col = (int)(q / pld) * SB + (int)(i % SB);
-    const bool in = (r < ld) && (col < ld);
-    const double v = in ? rint(x[col + (size_t)r * ld] * f) : 0.0;
+    const int col0 = (int)(q / pld) * SB + (int)(i % SB);
+    double v[16];
+    if (r < ld && col0 < ld) {
+      const double2* xp = (const double2*)(x + col0 + (size_t)r * ld);
 #pragma unroll
-    for (int k = 0; k < NP; ++k) p0[(size_t)k * count + i] = (int8_t)sym_residue(v, c.p[k], c.inv[k]);
+      for (int j = 0; j < 8; ++j) {
+        const double2 u = xp[j];
+        v[2 * j] = rint(u.x * f);
+        v[2 * j + 1] = rint(u.y * f);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      i4 out;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        int b4 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b4 |= (sym_residue(v[4 * w + j], c.p[k], c.inv[k]) & 0xff) << (8 * j);
+        out[w] = b4;
+      }
+      *(i4*)(p0 + (size_t)k * count + i) = out;
+    }
   }
 }
 
@@ -1129,7 +1159,7 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
   const int4* items = item_list(nt, np, &st);
   if (st) return st;
   const CrtConst c = make_const(np);
-  const int nnb = std::max(1, std::min(1024, C.n));
+  const int nnb = std::max(1, std::min(1024, (C.n + NT / 64 - 1) / (NT / 64)));
   auto scale_split = [&](const DMat& X, int8_t* planes, double* s) -> int {
     hipLaunchKernelGGL(crt_colnorm_kernel, dim3(nnb), dim3(NT), 0, stream(), X.p, ld, X.n, npart);
     hipLaunchKernelGGL(crt_scale_kernel, dim3(1), dim3(NT), 0, stream(), npart, nnb, s);
