@@ -33,6 +33,10 @@ Parity pins (checked in ``tests/test_oracle.py``):
   function 1/(1+exp(beta (e-mu))) as the number of steps grows.
 * responses: Daleckii-Krein divided differences of the same scalar maps and
   central finite differences of the forward solvers.
+* int8 Chinese-remainder products (``crt_square``, ``crt_product``; no
+  reference counterpart): against the FP64 products within the operand
+  rounding bound, the scale bound at its edge, and planted mistakes (wrong CRT
+  weights, a dropped modulus, an operand scaled by 1) that must break them.
 
 Functions whose values only BML could pin (``matrix2submatrix_index`` halo
 ordering) are "parity unpinned"; the subgraph loop's result is pinned through
