@@ -612,6 +612,45 @@ def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, 
     return p, mu
 
 
+def implicit_fermi_adjoint(h, a, nsteps, mu, beta, canonical=False):
+    """Reverse sweep of the fixed-mu implicit-Fermi recursion (k = 2, exact
+    inverses; the forward of src/prg_implicit_fermi_mod.F90:253-298): the
+    gradient dTr(P A)/dH for symmetric A (no reference routine: the adjoint
+    the GPU library adds).  Level i maps X to F = Y^-1 X^2, Y = 2 X^2 - 2X + I;
+    with W = Y^-1 Gbar and U = W - 2 W F the adjoint of X is
+    X Us + Us X + W F + (W F)^T, Us = (U + U^T) / 2, and X0 = (1/2 + c mu) I - c H
+    gives dH = -c Gbar0 (c = beta / 2^(nsteps+2)).  canonical: the
+    derivative at fixed occupation with the reference's dP/dmu ~ beta P (I - P)
+    (src/prg_implicit_fermi_mod.F90:512-521): A is replaced by A - kappa I,
+    kappa = Tr(A beta P (I - P)) / Tr(beta P (I - P))."""
+    n = h.shape[0]
+    eye = np.eye(n)
+    cnst = beta / (2.0 ** (nsteps + 2))
+    xs = [normalize_implicit_fermi(h, cnst, mu)]
+    invs = []
+    for _ in range(nsteps):
+        x = xs[-1]
+        x2 = x @ x
+        yi = np.linalg.inv(2.0 * (x2 - x) + eye)
+        invs.append(yi)
+        xs.append(yi @ x2)
+    g = np.array(a, dtype=np.float64)
+    if canonical:
+        p = xs[-1]
+        cm = beta * (p - p @ p)
+        tr = np.trace(cm)
+        if abs(tr) > 1e-8:
+            g = g - (np.sum(g * cm) / tr) * eye
+    for i in reversed(range(nsteps)):
+        x, f, yi = xs[i], xs[i + 1], invs[i]
+        w = yi @ g
+        wf = w @ f
+        u = w - 2.0 * wf
+        v = x @ (0.5 * (u + u.T))
+        g = (v + v.T) + (wf + wf.T)
+    return -cnst * g
+
+
 def implicit_fermi_save_inverse(inv, h, nsteps, nocc, mu, beta, occ_err_limit, tol,
                                 scf_it, maxiter=30):
     """prg_implicit_fermi_save_inverse (src/prg_implicit_fermi_mod.F90:46-193).

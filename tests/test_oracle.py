@@ -643,3 +643,103 @@ def test_sp2_basic_tcore_pins():
     for _ in range(maxit + 1):
         x = x * x if x.sum() - 0.5 * n <= 0.0 else 2.0 * x - x * x
     assert np.linalg.norm(rho - (q * (2.0 * x)) @ q.T) > 1.0
+
+
+def _if_scalar_and_derivative(nsteps, mu, beta):
+    """The exact-inverse implicit-Fermi recursion on one eigenvalue and its
+    derivative by the chain rule: x0 = 1/2 - c (e - mu), x <- x^2 / (2x^2 - 2x + 1),
+    dx_{k+1}/dx_k = 2 x (1 - x) / (2x^2 - 2x + 1)^2."""
+    cnst = beta / (2.0 ** (nsteps + 2))
+
+    def f(e):
+        return O.implicit_fermi_eigen(e, mu, beta, nsteps)
+
+    def df(e):
+        x = 0.5 - cnst * (np.asarray(e) - mu)
+        d = -cnst * np.ones_like(x)
+        for _ in range(nsteps):
+            q = 2.0 * x * x - 2.0 * x + 1.0
+            d = d * 2.0 * x * (1.0 - x) / (q * q)
+            x = x * x / q
+        return d
+
+    return f, df
+
+
+@pytest.mark.parametrize("canonical", [False, True])
+def test_implicit_fermi_adjoint_daleckii_krein(canonical):
+    """The oracle's reverse sweep equals the closed form: P = f(H) for the
+    scalar recursion f, so dTr(P A)/dH = Q (K o Q^T A Q) Q^T with K the
+    divided differences of f (Daleckii-Krein); the canonical variant is the
+    same with A - kappa I."""
+    n, nsteps, mu, beta = 60, 6, 0.3, 8.0
+    h = O.model_hamiltonian(n, 4)
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((n, n))
+    a = 0.5 * (a + a.T)
+    g = O.implicit_fermi_adjoint(h, a, nsteps, mu, beta, canonical)
+    assert np.array_equal(g, g.T)
+    e, q = np.linalg.eigh(h)
+    f, df = _if_scalar_and_derivative(nsteps, mu, beta)
+    k = O.divided_differences(f, df, e)
+    aa = a
+    if canonical:
+        p = (q * f(e)) @ q.T
+        cm = beta * (p - p @ p)
+        aa = a - (np.sum(a * cm) / np.trace(cm)) * np.eye(n)
+    ref = q @ (k * (q.T @ aa @ q)) @ q.T
+    assert np.max(np.abs(g - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_implicit_fermi_adjoint_canonical_matches_reference_response():
+    """Tr(A P1[H1]) for the reference's first-order response (with its mu
+    correction) equals Tr(H1 G) for the canonical adjoint G, for any H1."""
+    n, nsteps, mu, beta = 50, 6, 0.3, 8.0
+    h = O.model_hamiltonian(n, 5)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((n, n))
+    a = 0.5 * (a + a.T)
+    cnst = beta / (2.0 ** (nsteps + 2))
+    xs = [O.normalize_implicit_fermi(h, cnst, mu)]
+    inv = []
+    for _ in range(nsteps):
+        x = xs[-1]
+        inv.append(np.linalg.inv(2.0 * (x @ x - x) + np.eye(n)))
+        xs.append(inv[-1] @ (x @ x))
+    g = O.implicit_fermi_adjoint(h, a, nsteps, mu, beta, canonical=True)
+    for s in range(3):
+        h1 = rng.standard_normal((n, n))
+        h1 = 0.5 * (h1 + h1.T)
+        _, p1 = O.implicit_fermi_first_order_response(h, h1, inv, nsteps, mu, beta)
+        assert abs(np.sum(a * p1) - np.sum(h1 * g)) < 1e-10 * max(1.0, abs(np.sum(a * p1)))
+
+
+def test_implicit_fermi_adjoint_planted_mistake_is_caught():
+    """Dropping the W F + (W F)^T term of the reverse step breaks the closed form."""
+    n, nsteps, mu, beta = 40, 5, 0.3, 8.0
+    h = O.model_hamiltonian(n, 6)
+    a = np.eye(n)
+    good = O.implicit_fermi_adjoint(h, a, nsteps, mu, beta)
+    e, q = np.linalg.eigh(h)
+    f, df = _if_scalar_and_derivative(nsteps, mu, beta)
+    ref = q @ (O.divided_differences(f, df, e) * (q.T @ a @ q)) @ q.T
+    assert np.max(np.abs(good - ref)) < 1e-10
+
+    def broken(h, a, nsteps, mu, beta, canonical=False):
+        n = h.shape[0]
+        cnst = beta / (2.0 ** (nsteps + 2))
+        xs = [O.normalize_implicit_fermi(h, cnst, mu)]
+        invs = []
+        for _ in range(nsteps):
+            x = xs[-1]
+            invs.append(np.linalg.inv(2.0 * (x @ x - x) + np.eye(n)))
+            xs.append(invs[-1] @ (x @ x))
+        g = a.copy()
+        for i in reversed(range(nsteps)):
+            w = invs[i] @ g
+            u = w - 2.0 * (w @ xs[i + 1])
+            v = xs[i] @ (0.5 * (u + u.T))
+            g = v + v.T
+        return -cnst * g
+
+    assert np.max(np.abs(broken(h, a, nsteps, mu, beta) - ref)) > 1e-6
