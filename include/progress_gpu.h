@@ -367,6 +367,22 @@ int prg_gpu_int8_waves(void);
 int prg_gpu_sp2_alg2_device(int N, const double *dH, double *dRho, double bndfil, int minsp2iter,
                             int maxsp2iter, const char *sp2conv, double idemtol, int precision,
                             int *iters, int *pp);
+/* prg_implicit_fermi at a fixed chemical potential (one pass of the level
+ * loop of src/prg_implicit_fermi_mod.F90:259-298, method 1: level-1 CG inverse
+ * refined by Newton-Schulz to tol at every level) on device pointers:
+ * dP = P (not scaled by 2).  Returns a status. */
+int prg_gpu_implicit_fermi_device(int N, const double *dH, double *dP, int nsteps, double mu,
+                                  double beta, double tol);
+/* Gradient dG = dTr(P A)/dH of that fixed-mu map (A symmetric), by a reverse
+ * sweep over the levels (every level's X and inverse held in HBM, (2 nsteps +
+ * 1) N^2 doubles); canonical != 0: the derivative at fixed occupation with the
+ * reference's dP/dmu ~ beta P (I - P) (src/prg_implicit_fermi_mod.F90:512-521),
+ * so that Tr(H1 dG) equals Tr(A P1) for the first-order response P1 of
+ * prg_implicit_fermi_first_order_response.  dP (may be NULL) receives P.
+ * Exactly symmetric.  No reference counterpart.  Returns a status. */
+int prg_gpu_implicit_fermi_adjoint_device(int N, const double *dH, const double *dA, double *dG,
+                                          int nsteps, double mu, double beta, double tol,
+                                          int canonical, double *dP);
 /* Padded leading dimension used by the device-pointer API. */
 int prg_gpu_padded_ld(int N);
 

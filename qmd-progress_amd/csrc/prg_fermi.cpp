@@ -379,6 +379,95 @@ int implicit_fermi_response(const DMat& H0, const std::vector<DMat>& Hs, DMat P0
   return OK;
 }
 
+// ---------------------------------------------------------------- adjoint
+// Fixed-mu forward of prg_implicit_fermi, method 1, k = 2
+// (src/prg_implicit_fermi_mod.F90:259-298 for one mu, the same operations as
+// implicit_fermi), keeping every level: X[0] = P0, X[i + 1] = the P after
+// level i + 1, Inv[i] = that level's converged inverse of Y = 2(X^2 - X) + I.
+int implicit_fermi_levels(const DMat& H, int nsteps, double mu, double beta, double tol,
+                          std::vector<DMat>& X, std::vector<DMat>& Inv) {
+  if (nsteps < 1 || nsteps > 60) {
+    set_error(ERR_ARG, "implicit Fermi adjoint: nsteps out of range");
+    return ERR_ARG;
+  }
+  int st;
+  X.resize(nsteps + 1);
+  Inv.resize(nsteps);
+  for (int i = 0; i <= nsteps; ++i)
+    if ((st = ops::temp(("ifa_x" + std::to_string(i)).c_str(), H.n, &X[i]))) return st;
+  for (int i = 0; i < nsteps; ++i)
+    if ((st = ops::temp(("ifa_inv" + std::to_string(i)).c_str(), H.n, &Inv[i]))) return st;
+  DMat P2, Y, AI, I;
+  if ((st = ops::temp("if_p2", H.n, &P2)) || (st = ops::temp("if_y", H.n, &Y)) ||
+      (st = ops::temp("if_ai", H.n, &AI)) || (st = ops::temp("if_i", H.n, &I)))
+    return st;
+  if ((st = ops::identity(AI)) || (st = ops::identity(I))) return st;
+  const double cnst = beta / (4.0 * std::pow(2.0, nsteps));
+  if ((st = normalize_if(H, cnst, mu, X[0]))) return st;
+  for (int i = 0; i < nsteps; ++i) {
+    if ((st = ops::mm(X[i], X[i], P2, true))) return st;
+    if ((st = ops::lin(P2, 2.0, &X[i], -2.0, 1.0, Y))) return st;
+    if (i == 0 && (st = conjgrad(Y, AI, I, 0.9, true, nullptr))) return st;
+    if ((st = newtonschulz(Y, AI, tol, nullptr, true))) return st;
+    if ((st = ops::copy(AI, Inv[i]))) return st;
+    if ((st = ops::mm(AI, P2, X[i + 1], true))) return st;
+  }
+  return OK;
+}
+
+// G = dTr(P A)/dH for the fixed-mu recursion above (A symmetric): the reverse
+// sweep.  Level i maps X to F = Y^-1 X^2; with W = Y^-1 Gb and U = W - 2 W F
+// the adjoint of X is X Us + Us X + W F + (W F)^T, Us = (U + U^T) / 2 (the
+// symmetric part: only symmetric perturbations of H are meaningful), and
+// X0 = (1/2 + c mu) I - c H gives G = -c Gb0.  canonical: the derivative at
+// fixed occupation with the reference's dP/dmu ~ beta P (I - P)
+// (src/prg_implicit_fermi_mod.F90:512-521), i.e. A replaced by A - kappa I,
+// kappa = Tr(A beta P (I - P)) / Tr(beta P (I - P)).  Every level's X and
+// inverse stay in HBM: (2 nsteps + 1) N^2 doubles.  P (optional) receives
+// the forward result.
+int implicit_fermi_adjoint(const DMat& H, const DMat& A, DMat G, int nsteps, double mu,
+                           double beta, double tol, bool canonical, DMat* P) {
+  std::vector<DMat> X, Inv;
+  int st;
+  if ((st = implicit_fermi_levels(H, nsteps, mu, beta, tol, X, Inv))) return st;
+  DMat Gb, W, WF, U, Ut, V, T1, T2;
+  if ((st = ops::temp("ifa_gb", H.n, &Gb)) || (st = ops::temp("ifa_w", H.n, &W)) ||
+      (st = ops::temp("ifa_wf", H.n, &WF)) || (st = ops::temp("ifa_u", H.n, &U)) ||
+      (st = ops::temp("ifa_ut", H.n, &Ut)) || (st = ops::temp("ifa_v", H.n, &V)) ||
+      (st = ops::temp("ifa_t1", H.n, &T1)) || (st = ops::temp("ifa_t2", H.n, &T2)))
+    return st;
+  if ((st = ops::copy(A, Gb))) return st;
+  const DMat& Pn = X[nsteps];
+  if (canonical) {
+    ops::Sums sa, sx2, sx;
+    if ((st = ops::mm(A, Pn, T1, false, 1.0, 0.0, nullptr, 0.0, &sa))) return st;  // Tr(A P)
+    if ((st = ops::mm(Pn, Pn, T2, false, 1.0, 0.0, nullptr, 0.0, &sx2, &A))) return st;
+    if ((st = ops::sums_of(Pn, &sx))) return st;
+    const double trcm = beta * (sx.tr - sx2.tr);  // Tr(beta P (I - P))
+    if (std::fabs(trcm) > 1e-8) {
+      const double kappa = beta * (sa.tr - sx2.dot) / trcm;
+      if ((st = ops::lin(A, 1.0, nullptr, 0.0, -kappa, Gb))) return st;
+    }
+  }
+  for (int i = nsteps - 1; i >= 0; --i) {
+    if ((st = ops::mm(Inv[i], Gb, W, false))) return st;          // W = Y^-1 Gb
+    if ((st = ops::mm(W, X[i + 1], WF, false))) return st;        // W F
+    if ((st = ops::lin(W, 1.0, &WF, -2.0, 0.0, U))) return st;    // U = W - 2 W F
+    if ((st = ops::transpose(U, Ut))) return st;
+    if ((st = ops::lin(U, 0.5, &Ut, 0.5, 0.0, U))) return st;     // Us
+    if ((st = ops::mm(X[i], U, V, false))) return st;             // X Us
+    if ((st = ops::transpose(V, Ut))) return st;
+    if ((st = ops::lin(V, 1.0, &Ut, 1.0, 0.0, T1))) return st;    // X Us + Us X
+    if ((st = ops::transpose(WF, Ut))) return st;
+    if ((st = ops::lin(WF, 1.0, &Ut, 1.0, 0.0, T2))) return st;   // W F + (W F)^T
+    if ((st = ops::lin(T1, 1.0, &T2, 1.0, 0.0, Gb))) return st;
+  }
+  const double cnst = beta / (4.0 * std::pow(2.0, nsteps));
+  if ((st = ops::lin(Gb, -cnst, nullptr, 0.0, 0.0, G))) return st;
+  if (P && (st = ops::copy(Pn, *P))) return st;
+  return OK;
+}
+
 }  // namespace prg
 
 using namespace prg;
@@ -399,6 +488,47 @@ bool dev_inv(bml_matrix_t** inv, int nsteps, std::vector<DMat>& out, bool write)
 }  // namespace
 
 extern "C" {
+
+// prg_implicit_fermi at a fixed mu (one pass of src/prg_implicit_fermi_mod.F90:
+// 259-298, method 1) on device pointers (padded layout): dP = P (not scaled
+// by 2).
+int prg_gpu_implicit_fermi_device(int N, const double* dH, double* dP, int nsteps, double mu,
+                                  double beta, double tol) {
+  clear_error();
+  if (N <= 0 || !dH || !dP || dP == dH) {
+    set_error(ERR_ARG, "prg_gpu_implicit_fermi_device: bad arguments");
+    return ERR_ARG;
+  }
+  int st = ensure_device();
+  if (st) return st;
+  const int ld = padded(N);
+  DMat H{N, ld, const_cast<double*>(dH)}, P{N, ld, dP};
+  std::vector<DMat> X, Inv;
+  if ((st = implicit_fermi_levels(H, nsteps, mu, beta, tol, X, Inv))) return st;
+  if ((st = ops::copy(X[nsteps], P))) return st;
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  return OK;
+}
+
+int prg_gpu_implicit_fermi_adjoint_device(int N, const double* dH, const double* dA, double* dG,
+                                          int nsteps, double mu, double beta, double tol,
+                                          int canonical, double* dP) {
+  clear_error();
+  if (N <= 0 || !dH || !dA || !dG || dG == dH || dG == dA || (dP && (dP == dH || dP == dA))) {
+    set_error(ERR_ARG, "prg_gpu_implicit_fermi_adjoint_device: bad arguments");
+    return ERR_ARG;
+  }
+  int st = ensure_device();
+  if (st) return st;
+  const int ld = padded(N);
+  DMat H{N, ld, const_cast<double*>(dH)}, A{N, ld, const_cast<double*>(dA)}, G{N, ld, dG};
+  DMat P{N, ld, dP};
+  if ((st = implicit_fermi_adjoint(H, A, G, nsteps, mu, beta, tol, canonical != 0,
+                                   dP ? &P : nullptr)))
+    return st;
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  return OK;
+}
 
 // src/prg_c_interface.F90:470-490 (mu by value, as the reference's C path)
 void prg_implicit_fermi(bml_matrix_t* h_bml, bml_matrix_t* p_bml, int nsteps, int k, double nocc,

@@ -118,6 +118,8 @@ def lib():
         "prg_gpu_gershgorin_device": (i, [i, vp, dp]),
         "prg_gpu_sp2_seq_device": (i, [i, vp, vp, ip, i, d, d]),
         "prg_gpu_sp2_adjoint_device": (i, [i, vp, vp, vp, ip, i, d, d, i]),
+        "prg_gpu_implicit_fermi_device": (i, [i, vp, vp, i, d, d, d]),
+        "prg_gpu_implicit_fermi_adjoint_device": (i, [i, vp, vp, vp, i, d, d, d, i, vp]),
         "prg_build_density_T0": (None, [i, vp, vp, d, d, dp]),
         "prg_build_density_T_fermi": (None, [vp, vp, d, d, d, i, dp]),
         "prg_get_eigenvalues": (None, [i, vp, dp, i]),
@@ -643,6 +645,27 @@ def sp2_adjoint_device(dH, dGout, dGrad, n, pp, mineval, maxeval, ckpt=8):
                                          C.c_void_p(dGout.data_ptr()),
                                          C.c_void_p(dGrad.data_ptr()), _ip(ppa), len(pp),
                                          mineval, maxeval, ckpt))
+
+
+def implicit_fermi_device(dH, dP, n, nsteps, mu, beta, tol=1e-10):
+    """prg_implicit_fermi at fixed mu (method 1) on padded float64 cuda tensors:
+    dP = P (not scaled by 2)."""
+    import torch
+    torch.cuda.synchronize()
+    _st(lib().prg_gpu_implicit_fermi_device(n, C.c_void_p(dH.data_ptr()),
+                                            C.c_void_p(dP.data_ptr()), nsteps, mu, beta, tol))
+
+
+def implicit_fermi_adjoint_device(dH, dA, dG, n, nsteps, mu, beta, tol=1e-10, canonical=False,
+                                  dP=None):
+    """dG = dTr(P A)/dH of the fixed-mu implicit-Fermi map (canonical: at fixed
+    occupation), padded float64 cuda tensors; dP (optional) receives P."""
+    import torch
+    torch.cuda.synchronize()
+    _st(lib().prg_gpu_implicit_fermi_adjoint_device(
+        n, C.c_void_p(dH.data_ptr()), C.c_void_p(dA.data_ptr()), C.c_void_p(dG.data_ptr()),
+        nsteps, mu, beta, tol, 1 if canonical else 0,
+        C.c_void_p(dP.data_ptr()) if dP is not None else None))
 
 
 # ---------------------------------------------------------------- graph partitioning

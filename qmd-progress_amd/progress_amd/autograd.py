@@ -1,4 +1,5 @@
-"""torch.autograd binding of the SP2 density solver and its hand-written adjoint.
+"""torch.autograd bindings of the SP2 and implicit-Fermi density solvers and
+their hand-written adjoints.
 
     D = SP2Density.apply(H, bndfil, idemtol, ckpt)
 
@@ -15,7 +16,8 @@ from __future__ import annotations
 
 import torch
 
-from . import PREC_FP64, padded_ld, sp2_adjoint_device, sp2_alg2_device, gershgorin_device
+from . import (PREC_FP64, gershgorin_device, implicit_fermi_adjoint_device,
+               implicit_fermi_device, padded_ld, sp2_adjoint_device, sp2_alg2_device)
 
 
 def _pad(x, n, ld):
@@ -51,3 +53,39 @@ class SP2Density(torch.autograd.Function):
 
 def sp2_density(h, bndfil=0.5, idemtol=1e-10, ckpt=8):
     return SP2Density.apply(h, bndfil, idemtol, ckpt)
+
+
+class ImplicitFermiDensity(torch.autograd.Function):
+    """P = prg_implicit_fermi(H) at a fixed chemical potential (method 1, k = 2,
+    `nsteps` levels, inverse temperature `beta`, Newton-Schulz tolerance
+    `tol`).  Backward: the reverse sweep of the level recursion
+    (prg_gpu_implicit_fermi_adjoint_device), recomputing the levels from H;
+    `canonical` gives the derivative at fixed occupation instead (the
+    reference's first-order-response convention)."""
+
+    @staticmethod
+    def forward(ctx, h, nsteps, mu, beta, tol=1e-10, canonical=False):
+        if h.dtype != torch.float64 or not h.is_cuda or h.dim() != 2 or h.shape[0] != h.shape[1]:
+            raise ValueError("ImplicitFermiDensity expects an N x N float64 CUDA tensor")
+        n = h.shape[0]
+        ld = padded_ld(n)
+        hp = _pad(h.detach(), n, ld)
+        p = torch.zeros_like(hp)
+        implicit_fermi_device(hp, p, n, nsteps, mu, beta, tol)
+        ctx.save_for_backward(hp)
+        ctx.meta = (n, ld, nsteps, mu, beta, tol, canonical)
+        return p[:n, :n].clone()
+
+    @staticmethod
+    def backward(ctx, grad_p):
+        (hp,) = ctx.saved_tensors
+        n, ld, nsteps, mu, beta, tol, canonical = ctx.meta
+        g = grad_p.contiguous().to(torch.float64)
+        ap = _pad(0.5 * (g + g.T), n, ld)
+        out = torch.zeros_like(ap)
+        implicit_fermi_adjoint_device(hp, ap, out, n, nsteps, mu, beta, tol, canonical)
+        return out[:n, :n].clone(), None, None, None, None, None
+
+
+def implicit_fermi_density(h, nsteps, mu, beta, tol=1e-10, canonical=False):
+    return ImplicitFermiDensity.apply(h, nsteps, mu, beta, tol, canonical)
