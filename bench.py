@@ -61,6 +61,8 @@ def parse():
                          "int8crt<m> (int8 CRT products with m moduli)")
     ap.add_argument("--ns-tol", type=float, default=1e-10,
                     help="fermi workload: Newton-Schulz tolerance ||I - Y Inv||_F")
+    ap.add_argument("--adjoint-of", default="sp2", choices=["sp2", "fermi"],
+                    help="adjoint workload: the SP2 recursion or the fixed-mu implicit-Fermi map")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for the "
@@ -566,9 +568,13 @@ def fermi_bench(args, P, torch, np):
 
 
 def adjoint_bench(args, P, torch, np):
-    """Forward SP2 + reverse-mode adjoint of L = Tr(D A) on the model
-    Hamiltonian at --n (A banded, bandwidth 32, N(0,1) entries), checkpoints
-    every 8 steps, through the torch.autograd binding (FP64)."""
+    """Forward + reverse-mode adjoint of L = Tr(D A) on the model Hamiltonian
+    at --n (A banded, bandwidth 32, N(0,1) entries) through the torch.autograd
+    binding (FP64): SP2 with checkpoints every 8 steps, or (--adjoint-of
+    fermi) the fixed-mu implicit-Fermi map (8 levels, beta 20, mu at the
+    spectrum's midgap) with one central finite difference along A as a check."""
+    if args.adjoint_of == "fermi":
+        return fermi_adjoint_bench(args, P, torch, np)
     from progress_amd.autograd import sp2_density
     n = args.n
     dev = torch.device("cuda:0")
@@ -604,6 +610,63 @@ def adjoint_bench(args, P, torch, np):
                        f"checkpoints every 8 steps, seed={args.seed}", "n": n},
           {"loss": lv, "grad_asymmetry": sym, "reverse_products_per_solve": nprod,
            "reverse_alg_tflops": alg / dt / 1e12, "cpu_baseline": None, "roofline": None})
+
+
+def fermi_adjoint_bench(args, P, torch, np):
+    from progress_amd.autograd import implicit_fermi_density
+    n = args.n
+    dev = torch.device("cuda:0")
+    h = model_hamiltonian_torch(n, args.seed, dev)
+    g = torch.Generator(device="cpu").manual_seed(args.seed + 1)
+    a = torch.randn((n, n), generator=g, dtype=torch.float64)
+    idx = torch.arange(n)
+    a[(idx[:, None] - idx[None, :]).abs() > 32] = 0.0
+    a = (0.5 * (a + a.T)).to(dev)
+    w = torch.linalg.eigvalsh(h)
+    mu = 0.5 * (w[n // 2 - 1] + w[n // 2]).item()
+    nsteps, beta, tol = 8, 20.0, args.ns_tol
+
+    def step():
+        hv = h.clone().requires_grad_(True)
+        p = implicit_fermi_density(hv, nsteps, mu, beta, tol)
+        loss = torch.sum(p * a)
+        loss.backward()
+        return loss.item(), hv.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    P.gemm_counters(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lv, grad = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    nprod, alg, exe = (v / args.steps for v in P.gemm_counters(reset=True))
+    sym = torch.linalg.norm(grad - grad.T).item() / torch.linalg.norm(grad).item()
+    # one central difference along A / ||A|| (step 1e-4) against <G, A> / ||A||
+    ld = P.padded_ld(n)
+    e = a / torch.linalg.norm(a)
+    eps = 1e-4
+    vals = []
+    for sgn in (1.0, -1.0):
+        hp = torch.zeros((ld, ld), dtype=torch.float64, device=dev)
+        hp[:n, :n] = h + sgn * eps * e
+        pp = torch.zeros_like(hp)
+        P.implicit_fermi_device(hp, pp, n, nsteps, mu, beta, tol)
+        vals.append(torch.sum(pp[:n, :n] * a).item())
+    fd = (vals[0] - vals[1]) / (2.0 * eps)
+    an = torch.sum(grad * e).item()
+    _line("implicit Fermi forward + adjoint solves/s (dTr(P A)/dH, fixed mu, FP64)", 1.0 / dt,
+          "solves/s", dt, args, "f64",
+          {"workload": f"implicit_fermi_density forward + backward, model H N={n}, A banded "
+                       f"bw 32, 8 levels, beta 20, mu midgap {mu:.6f}, Newton-Schulz tol "
+                       f"{tol:g}, seed={args.seed}", "n": n},
+          {"loss": lv, "grad_asymmetry": sym, "products_per_solve": nprod,
+           "alg_tflops": alg / dt / 1e12,
+           "fd_check": {"direction": "A/||A||", "step": eps, "fd": fd, "adjoint": an,
+                        "rel_err": abs(fd - an) / abs(an)},
+           "cpu_baseline": None, "roofline": None})
 
 
 def executed_ops(ld, mode, moduli):
