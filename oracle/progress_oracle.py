@@ -888,6 +888,81 @@ def canon_response(h1_eig, nocc, beta, evals, mu0, m):
     return p1
 
 
+def canon_response_p1_dpdmu(h1_eig, beta, evals, mu0, m):
+    """prg_canon_response_p1_dpdmu (src/prg_response_mod.F90:1112-1183): the
+    recursion of canon_response without the trace correction.  Returns
+    (P1, dPdmu) with dPdmu = beta p0 (1 - p0) after the m steps.
+    (prg_canon_response_orig, :1038-1105, is canon_response itself.)"""
+    cnst = beta / (2.0 ** (m + 2))
+    p0 = 0.5 + cnst * (np.asarray(evals) - mu0)
+    p1 = -cnst * np.asarray(h1_eig, dtype=np.float64)
+    for _ in range(m):
+        p02 = p0 * p0
+        dx1 = (p0[:, None] + p0[None, :]) * p1
+        id0 = 1.0 / (2.0 * (p02 - p0) + 1.0)
+        p0 = id0 * p02
+        p1 = id0[:, None] * (dx1 + 2.0 * (p1 - dx1) * p0[None, :])
+    return p1, beta * p0 * (1.0 - p0)
+
+
+def canon_response_vector(h1_eig, beta, evals, mu0):
+    """prg_canon_response_vector (src/prg_response_mod.F90:849-931): the exact
+    Fermi response in the eigenbasis, P1 = K o H1 with K_ji = (f_i - f_j) /
+    (e_i - e_j) off the diagonal and K_ii = df/de = -beta t / (1 + t)^2,
+    t = exp(beta (e_i - mu0)), then the diagonal shifted by -dmu K_ii with
+    dmu = Tr P1 / Tr K (0 when |Tr K| <= 1e-12).  Degenerate eigenvalue pairs
+    divide by zero, as in the reference."""
+    e = np.asarray(evals, dtype=np.float64)
+    n = e.shape[0]
+    f = 1.0 / (1.0 + np.exp(beta * (e - mu0)))
+    x = e[None, :] - e[:, None]
+    y = f[None, :] - f[:, None]
+    x[np.diag_indices(n)] += 1.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        k = y / x
+    t = np.exp((e - mu0) * beta)
+    k[np.diag_indices(n)] += -t / ((1.0 + t) * (1.0 + t)) * beta
+    p1 = k * np.asarray(h1_eig, dtype=np.float64)
+    trk = float(np.sum(np.diag(k)))
+    dmu = float(np.sum(np.diag(p1))) / trk if abs(trk) > 1e-12 else 0.0
+    p1[np.diag_indices(n)] -= dmu * np.diag(k)
+    return p1
+
+
+def rs_occupations(n, bndfil):
+    """Occupations of prg_compute_response_RS (src/prg_response_mod.F90:319-329):
+    2 for state i (1-based) with i <= nocc, else 0; state int(nocc)+1 gets 1
+    when nocc has a fractional part above 0.01."""
+    nocc = n * bndfil
+    occ = np.array([2.0 if i <= nocc else 0.0 for i in range(1, n + 1)])
+    if abs(nocc - int(nocc)) > 0.01:
+        occ[int(nocc)] = 1.0
+    return occ
+
+
+def compute_response_rs(h, h1, bndfil):
+    """prg_compute_response_RS (src/prg_response_mod.F90:250-361): first-order
+    Rayleigh-Schroedinger response at T = 0.  V = C^T H1 C, V~_ij = V_ij /
+    (e_j - e_i) (0 on the diagonal), C1 = C V~, rho1 = C f C1^T + C1 f C^T."""
+    e, c = np.linalg.eigh(h)
+    v = c.T @ h1 @ c
+    n = h.shape[0]
+    de = e[None, :] - e[:, None]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        vt = np.where(np.eye(n, dtype=bool), 0.0, v / np.where(np.eye(n, dtype=bool), 1.0, de))
+    c1 = c @ vt
+    occ = rs_occupations(n, bndfil)
+    return (c * occ) @ c1.T + c1 @ (occ[:, None] * c.T)
+
+
+def compute_response_fd(h, h1, delta, bndfil):
+    """prg_compute_response_FD (src/prg_response_mod.F90:380-424):
+    (rho(H + delta H1) - rho(H - delta H1)) / (2 delta), rho = prg_build_density_T0."""
+    rp, _ = build_density_T0(h + delta * h1, bndfil)
+    rm, _ = build_density_T0(h - delta * h1, bndfil)
+    return (rp - rm) / (2.0 * delta)
+
+
 # --------------------------------------------------------------------------
 # graph partitioning and the subgraph SP2 loop
 # (src/prg_graph_mod.F90, src/prg_subgraphloop_mod.F90)

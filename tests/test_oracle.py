@@ -743,3 +743,71 @@ def test_implicit_fermi_adjoint_planted_mistake_is_caught():
         return -cnst * g
 
     assert np.max(np.abs(broken(h, a, nsteps, mu, beta) - ref)) > 1e-6
+
+
+def test_canon_response_vector_is_the_fermi_derivative():
+    """Off the diagonal, the vector response equals the central difference of
+    the eigenbasis Fermi density f(L + lambda H1) (Daleckii-Krein); the
+    diagonal correction makes the trace vanish (canonical)."""
+    n, beta, mu0 = 40, 6.0, 0.1
+    rng = np.random.default_rng(4)
+    e = np.sort(rng.uniform(-1.0, 1.0, n))
+    h1 = rng.standard_normal((n, n))
+    h1 = 0.5 * (h1 + h1.T)
+    p1 = O.canon_response_vector(h1, beta, e, mu0)
+    assert abs(np.trace(p1)) < 1e-12
+
+    def dens(lam):
+        w, q = np.linalg.eigh(np.diag(e) + lam * h1)
+        return (q / (1.0 + np.exp(beta * (w - mu0)))) @ q.T
+
+    eps = 1e-5
+    fd = (dens(eps) - dens(-eps)) / (2.0 * eps)
+    off = ~np.eye(n, dtype=bool)
+    # central difference error O(eps^2) on the closest eigenvalue pairs: 2e-8
+    assert np.max(np.abs(p1[off] - fd[off])) < 1e-7
+    # the diagonal is the derivative shifted by -dmu f'(e), dmu = Tr(dP) / Tr f'
+    t = np.exp(beta * (e - mu0))
+    df = -beta * t / (1.0 + t) ** 2
+    ref = np.diag(fd) - (np.trace(fd) / np.sum(df)) * df
+    assert np.max(np.abs(np.diag(p1) - ref)) < 1e-7
+
+
+def test_canon_response_p1_dpdmu_plus_correction_is_canon_response():
+    n, beta, mu0, m = 30, 10.0, 0.2, 8
+    rng = np.random.default_rng(5)
+    e = np.sort(rng.uniform(-1.0, 1.0, n))
+    h1 = rng.standard_normal((n, n))
+    h1 = 0.5 * (h1 + h1.T)
+    p1, dpdmu = O.canon_response_p1_dpdmu(h1, beta, e, mu0, m)
+    ref = O.canon_response(h1, n / 2, beta, e, mu0, m)
+    corr = p1.copy()
+    corr[np.diag_indices(n)] += (-np.trace(p1) / np.sum(dpdmu)) * dpdmu
+    assert np.max(np.abs(corr - ref)) < 1e-14
+
+
+def test_response_rs_matches_finite_differences():
+    """Rayleigh-Schroedinger first-order response of the T = 0 density equals
+    the central difference of prg_build_density_T0 on a gapped system, and
+    keeps the trace (fixed occupation)."""
+    n = 60
+    rng = np.random.default_rng(6)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    e = np.concatenate([np.sort(rng.uniform(-3.0, -1.0, n // 2)), np.sort(rng.uniform(1.0, 3.0, n // 2))])
+    h = (q * e) @ q.T
+    h = 0.5 * (h + h.T)
+    h1 = rng.standard_normal((n, n))
+    h1 = 0.5 * (h1 + h1.T)
+    rs = O.compute_response_rs(h, h1, 0.5)
+    fd = O.compute_response_fd(h, h1, 1e-5, 0.5)
+    assert np.max(np.abs(rs - fd)) < 1e-7
+    assert abs(np.trace(rs)) < 1e-10
+    # a planted sign error (e_i - e_j) is caught
+    e0, c = np.linalg.eigh(h)
+    v = c.T @ h1 @ c
+    de = e0[:, None] - e0[None, :]
+    vt = np.where(np.eye(n, dtype=bool), 0.0, v / np.where(np.eye(n, dtype=bool), 1.0, de))
+    occ = O.rs_occupations(n, 0.5)
+    c1 = c @ vt
+    bad = (c * occ) @ c1.T + c1 @ (occ[:, None] * c.T)
+    assert np.max(np.abs(bad - fd)) > 1e-3
