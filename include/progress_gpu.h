@@ -265,6 +265,30 @@ void prg_compute_response_SP2(bml_matrix_t *ham_bml, bml_matrix_t *prt_bml, bml_
  * eigenbasis of H0, evals[HDIM] its eigenvalues (host). */
 void prg_canon_response(bml_matrix_t *P1_bml, bml_matrix_t *H1_bml, double nocc, double beta,
                         const double *evals, double mu0, int m, int HDIM);
+/* New C symbol for prg_canon_response_orig (prg_response_mod.F90:1038-1105): the
+ * same result as prg_canon_response (thresh unused). */
+void prg_canon_response_orig(bml_matrix_t *P1_bml, bml_matrix_t *H1_bml, double nocc, double beta,
+                             const double *evals, double mu0, int m, double thresh, int HDIM);
+/* New C symbol for prg_canon_response_p1_dpdmu (prg_response_mod.F90:1112-1183):
+ * the recursion without the trace correction; dPdMu[HDIM] (host) receives
+ * beta p0 (1 - p0).  Norbs and Q_bml are unused (may be 0 / NULL). */
+void prg_canon_response_p1_dpdmu(bml_matrix_t *P1_bml, double *dPdMu, bml_matrix_t *H1_bml,
+                                 int Norbs, double beta, bml_matrix_t *Q_bml, const double *evals,
+                                 double mu0, int m, int HDIM);
+/* New C symbol for prg_canon_response_vector (prg_response_mod.F90:849-931): the
+ * exact Fermi response in the eigenbasis (element-wise divided differences),
+ * canonical diagonal correction (m, thresh unused). */
+void prg_canon_response_vector(bml_matrix_t *P1_bml, bml_matrix_t *H1_bml, double nocc,
+                               double beta, const double *evals, double mu0, int m, double thresh,
+                               int HDIM);
+/* New C symbol for prg_compute_response_RS (prg_response_mod.F90:250-361): T = 0
+ * Rayleigh-Schroedinger first-order response rsp = C f C1^T + C1 f C^T. */
+void prg_compute_response_RS(bml_matrix_t *ham_bml, bml_matrix_t *prt_bml, bml_matrix_t *rsp_bml,
+                             double lambda, double bndfil, double threshold, int verbose);
+/* New C symbol for prg_compute_response_FD (prg_response_mod.F90:380-424):
+ * (rho(H + delta H1) - rho(H - delta H1)) / (2 delta), T = 0 densities. */
+void prg_compute_response_FD(bml_matrix_t *ham_bml, bml_matrix_t *prt_bml, bml_matrix_t *rsp_bml,
+                             double delta, double bndfil, double threshold, int verbose);
 
 /* ------------------------------------------------------------------ graph partitioning */
 /* Flat C counterpart of graph_partitioning_t (src/prg_graph_mod.F90:58-140) for

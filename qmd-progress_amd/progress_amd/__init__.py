@@ -139,6 +139,11 @@ def lib():
                                                d, d, d, d, i, d]),
         "prg_compute_response_SP2": (None, [vp, vp, vp, vp, d, d, i, i, cp, d, d, i]),
         "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
+        "prg_canon_response_orig": (None, [vp, vp, d, d, dp, d, i, d, i]),
+        "prg_canon_response_p1_dpdmu": (None, [vp, dp, vp, i, d, vp, dp, d, i, i]),
+        "prg_canon_response_vector": (None, [vp, vp, d, d, dp, d, i, d, i]),
+        "prg_compute_response_RS": (None, [vp, vp, vp, d, d, d, i]),
+        "prg_compute_response_FD": (None, [vp, vp, vp, d, d, d, i]),
         "prg_gpu_step_timing": (i, [dp, ip]),
         "prg_gpu_int8_product_timing": (i, [dp, ip]),
         "prg_gpu_gemm_counters": (None, [i, dp, dp, dp, dp]),
@@ -612,6 +617,50 @@ def canon_response(h1_eig, nocc, beta, evals, mu0, m):
     lib().prg_canon_response(p1.h, mh.h, nocc, beta, _dp(e), mu0, m, n)
     _check()
     return p1.numpy()
+
+
+def canon_response_orig(h1_eig, nocc, beta, evals, mu0, m):
+    n = h1_eig.shape[0]
+    mh, p1 = Matrix.from_numpy(h1_eig), Matrix.zeros(n)
+    e = _dvec(n, evals)
+    lib().prg_canon_response_orig(p1.h, mh.h, nocc, beta, _dp(e), mu0, m, 0.0, n)
+    _check()
+    return p1.numpy()
+
+
+def canon_response_p1_dpdmu(h1_eig, beta, evals, mu0, m):
+    """Returns (P1, dPdmu)."""
+    n = h1_eig.shape[0]
+    mh, p1 = Matrix.from_numpy(h1_eig), Matrix.zeros(n)
+    e, dp = _dvec(n, evals), _dvec(n)
+    lib().prg_canon_response_p1_dpdmu(p1.h, _dp(dp), mh.h, 0, beta, None, _dp(e), mu0, m, n)
+    _check()
+    return p1.numpy(), np.array(dp[:n])
+
+
+def canon_response_vector(h1_eig, beta, evals, mu0):
+    n = h1_eig.shape[0]
+    mh, p1 = Matrix.from_numpy(h1_eig), Matrix.zeros(n)
+    e = _dvec(n, evals)
+    lib().prg_canon_response_vector(p1.h, mh.h, 0.0, beta, _dp(e), mu0, 0, 0.0, n)
+    _check()
+    return p1.numpy()
+
+
+def compute_response_rs(h, h1, bndfil):
+    n = h.shape[0]
+    m0, m1, r = Matrix.from_numpy(h), Matrix.from_numpy(h1), Matrix.zeros(n)
+    lib().prg_compute_response_RS(m0.h, m1.h, r.h, 0.0, bndfil, 0.0, 0)
+    _check()
+    return r.numpy()
+
+
+def compute_response_fd(h, h1, delta, bndfil):
+    n = h.shape[0]
+    m0, m1, r = Matrix.from_numpy(h), Matrix.from_numpy(h1), Matrix.zeros(n)
+    lib().prg_compute_response_FD(m0.h, m1.h, r.h, delta, bndfil, 0.0, 0)
+    _check()
+    return r.numpy()
 
 
 # ---------------------------------------------------------------- device-pointer helpers

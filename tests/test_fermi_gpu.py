@@ -141,3 +141,47 @@ def test_canon_response(P):
     p1 = P.canon_response(h1e, 80.0, 10.0, w, 0.6, 6)
     p1_o = O.canon_response(h1e, 80.0, 10.0, w, 0.6, 6)
     assert np.linalg.norm(p1 - p1_o) < 1e-10 * max(1.0, np.linalg.norm(p1_o))
+
+
+def test_canon_response_variants_match_oracle(P, h256):
+    """prg_canon_response_orig / _p1_dpdmu / _vector against the oracle (the
+    eigenbasis recursions are element-wise: FP64 rounding only)."""
+    print("[canon response variants] n=256", flush=True)
+    e = np.linalg.eigvalsh(h256)
+    rng = np.random.default_rng(8)
+    h1 = rng.standard_normal((256, 256))
+    h1 = 0.5 * (h1 + h1.T)
+    beta, mu0, m = 20.0, 0.6, 8
+    p = P.canon_response_orig(h1, 128.0, beta, e, mu0, m)
+    assert np.max(np.abs(p - O.canon_response(h1, 128.0, beta, e, mu0, m))) < 1e-12
+    p1, dp = P.canon_response_p1_dpdmu(h1, beta, e, mu0, m)
+    p1_o, dp_o = O.canon_response_p1_dpdmu(h1, beta, e, mu0, m)
+    # dPdmu up to beta / 4 = 5: a few ulps of the 8-step scalar recursion
+    assert np.max(np.abs(p1 - p1_o)) < 1e-12 and np.max(np.abs(dp - dp_o)) < 1e-12
+    pv = P.canon_response_vector(h1, beta, e, mu0)
+    pv_o = O.canon_response_vector(h1, beta, e, mu0)
+    assert np.max(np.abs(pv - pv_o)) < 1e-10 * max(1.0, np.max(np.abs(pv_o)))
+    assert abs(np.trace(pv)) < 1e-9
+
+
+def test_response_rs_and_fd_match_oracle(P):
+    """prg_compute_response_RS and prg_compute_response_FD on a gapped
+    Hamiltonian (half filling at the gap): each against the oracle, and RS
+    against FD (central difference, delta 1e-4: O(delta^2) apart)."""
+    print("[response RS / FD] n=160", flush=True)
+    n = 160
+    rng = np.random.default_rng(9)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.concatenate([np.sort(rng.uniform(-3.0, -1.0, n // 2)),
+                         np.sort(rng.uniform(1.0, 3.0, n // 2))])
+    h = (q * ev) @ q.T
+    h = 0.5 * (h + h.T)
+    h1 = rng.standard_normal((n, n))
+    h1 = 0.5 * (h1 + h1.T)
+    rs = P.compute_response_rs(h, h1, 0.5)
+    rs_o = O.compute_response_rs(h, h1, 0.5)
+    assert np.max(np.abs(rs - rs_o)) < 1e-9
+    fd = P.compute_response_fd(h, h1, 1e-4, 0.5)
+    fd_o = O.compute_response_fd(h, h1, 1e-4, 0.5)
+    assert np.max(np.abs(fd - fd_o)) < 1e-8
+    assert np.max(np.abs(rs - fd)) < 1e-5
