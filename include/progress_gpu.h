@@ -224,7 +224,8 @@ void prg_test_density_matrix(bml_matrix_t *ham_bml, bml_matrix_t *p_bml, double 
 double prg_test_density_matrix_mu(void);
 
 /* ------------------------------------------------------------------ implicit Fermi */
-/* src/prg_c_interface.F90:470-490 (prg_implicit_fermi_mod.F90:209-336); k = 2 only.
+/* src/prg_c_interface.F90:470-490 (prg_implicit_fermi_mod.F90:209-336); even
+ * expansion order k (k > 2: prg_setup_linsys, :843-876; odd k is an error).
  * mu by value as in the reference; prg_implicit_fermi_last_mu() returns the update. */
 void prg_implicit_fermi(bml_matrix_t *h_bml, bml_matrix_t *p_bml, int nsteps, int k, double nocc,
                         double mu, double beta, int method, int osteps, double occErrLimit,
@@ -391,6 +392,29 @@ int prg_gpu_int8_waves(void);
 int prg_gpu_sp2_alg2_device(int N, const double *dH, double *dRho, double bndfil, int minsp2iter,
                             int maxsp2iter, const char *sp2conv, double idemtol, int precision,
                             int *iters, int *pp);
+/* New C symbols for the linear solvers of prg_implicit_fermi_mod.F90 (Fortran
+ * argument order; the auxiliary matrices are unused and may be NULL):
+ * prg_conjgrad (:1006-1048) and prg_pcg (:934-996, Jacobi preconditioner)
+ * solve A X = B with X in p_bml (initial guess on entry), stopping when
+ * ||R||_F^2 <= cg_tol; prg_pcg leaves the residual in p2_bml as the reference
+ * does.  prg_newtonschulz (:888-922) refines ai_bml towards A^-1 until
+ * ||I - A Ai||_F <= tol and stores the iteration count in *num_iter. */
+void prg_conjgrad(bml_matrix_t *A_bml, bml_matrix_t *p_bml, bml_matrix_t *p2_bml,
+                  bml_matrix_t *tmp_bml, bml_matrix_t *d_bml, bml_matrix_t *w_bml, double cg_tol,
+                  double threshold);
+void prg_pcg(bml_matrix_t *A_bml, bml_matrix_t *p_bml, bml_matrix_t *p2_bml, bml_matrix_t *d_bml,
+             bml_matrix_t *wtmp_bml, double cg_tol, double threshold);
+void prg_newtonschulz(bml_matrix_t *a_bml, bml_matrix_t *ai_bml, bml_matrix_t *r_bml,
+                      bml_matrix_t *tmp_bml, bml_matrix_t *d_bml, bml_matrix_t *I_bml, double tol,
+                      double threshold, int *num_iter);
+/* New C symbol for prg_finite_diff (prg_implicit_fermi_mod.F90:726-831):
+ * finite-difference responses of prg_get_density_matrix along H0 + sum l^i H_i,
+ * mu0 + sum l^i mu_i (i = 1..order, order <= 3) at l = lambda, step h.  The
+ * reference prints them (top-left 10 x 10 times 1000); out_list[k] (order
+ * k + 1, may be NULL: then the corners are printed) receives each. */
+void prg_finite_diff(bml_matrix_t *H0_bml, bml_matrix_t **H_list, double mu0,
+                     const double *mu_list, double beta, int order, double lambda, double h,
+                     double threshold, bml_matrix_t **out_list);
 /* prg_implicit_fermi at a fixed chemical potential (one pass of the level
  * loop of src/prg_implicit_fermi_mod.F90:259-298, method 1: level-1 CG inverse
  * refined by Newton-Schulz to tol at every level) on device pointers:

@@ -550,6 +550,62 @@ def conjgrad(a, p, b, cg_tol, maxit=50):
     return p, k
 
 
+def finite_diff(h0, hs, mu0, mus, beta, order, lam, h):
+    """prg_finite_diff (src/prg_implicit_fermi_mod.F90:726-831): the order-1..3
+    finite differences of get_density_matrix along H0 + sum l^i H_i,
+    mu0 + sum l^i mu_i at l = lam (order 1 forward, 2 and 3 central).
+    Returns the list of difference matrices (the reference prints them)."""
+    def dens(l):
+        hh = np.array(h0, dtype=np.float64)
+        mu = mu0
+        for i in range(order):
+            hh = hh + l ** (i + 1) * hs[i]
+            mu += l ** (i + 1) * mus[i]
+        return get_density_matrix(hh, beta, mu)
+
+    d0, d1p = dens(lam), dens(lam + h)
+    out = [(d1p - d0) / h]
+    if order > 1:
+        d1m = dens(lam - h)
+        out.append((d1p - 2.0 * d0 + d1m) / (2.0 * h * h))
+    if order > 2:
+        d2m, d2p = dens(lam - 2 * h), dens(lam + 2 * h)
+        out.append((d2p - d2m) / (12.0 * h ** 3) - (d1p - d1m) / (6.0 * h ** 3))
+    return out
+
+
+def pcg(a, p, b, cg_tol, maxit=100):
+    """prg_pcg (src/prg_implicit_fermi_mod.F90:934-996): conjugate gradient for
+    A X = B preconditioned by M = diag(A)^-1, run while ||R||_F^2 > cg_tol; the
+    inner products are bml_trace_mult, Tr(X Y).  Returns (X, R, iterations)
+    (the reference leaves R in the right-hand side's matrix)."""
+    def trm(x, y):
+        return float(np.sum(x * y.T))
+
+    minv = 1.0 / np.diag(a)
+    p = p.copy()
+    r = b - a @ p
+    z = minv[:, None] * r
+    r_new = trm(z, r)
+    d = z.copy()
+    k = 0
+    r_old = r_new
+    while np.sum(r * r) > cg_tol:
+        k += 1
+        if k != 1:
+            d = (r_new / r_old) * d + z
+        w = a @ d
+        alpha = trm(r, z) / trm(d, w)
+        p = p + alpha * d
+        r = r - alpha * w
+        z = minv[:, None] * r
+        r_old = r_new
+        r_new = trm(r, z)
+        if k > maxit:
+            raise NotConverging("PCG is not converging")
+    return p, r, k
+
+
 def newtonschulz(a, ai, tol, max_iter=200):
     """prg_newtonschulz (src/prg_implicit_fermi_mod.F90:888-922):
     R = I - A Ai, err = ||R||_F, Ai <- Ai + Ai R until err <= tol; falls back
@@ -572,16 +628,34 @@ def newtonschulz(a, ai, tol, max_iter=200):
     return ai, i
 
 
+def setup_linsys(x, k):
+    """prg_setup_linsys (src/prg_implicit_fermi_mod.F90:843-876): (A, B) of one
+    level, A X_next = B, for even k: k = 2 gives B = X^2, A = 2(X^2 - X) + I;
+    otherwise B = X^k, A = (I - X)^k + X^k by (k - 2)/2 products with X^2 and
+    (I - X)^2."""
+    n = x.shape[0]
+    x2 = x @ x
+    if k == 2:
+        return 2.0 * (x2 - x) + np.eye(n), x2
+    y = x2 - 2.0 * x + np.eye(n)
+    b, a = x2.copy(), y.copy()
+    for _ in range((k - 2) // 2):
+        b = b @ x2
+        a = a @ y
+    return a + b, b
+
+
 def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, tol):
-    """prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), k = 2.
+    """prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), even k
+    (setup_linsys).
     method 0: CG solve per level (:262-277).  method 1: a level-1 CG inverse at
     tol 0.9 refined by Newton-Schulz (to `tol`) at every level (:278-298).  The
     reference has that Newton-Schulz call commented out, which reuses the
     level-1 inverse for all levels and makes the recursion diverge (||P|| ~ 1e30
     on the model Hamiltonian); this restates the intended iteration.
     Output P is not scaled by 2.  Returns (P, mu)."""
-    if k != 2:
-        raise ValueError("only expansion order k = 2 is restated")
+    if k < 2 or k % 2:
+        raise ValueError("the expansion order k must be even")
     n = h.shape[0]
     cnst = beta / (4 * k ** nsteps)
     occ_err = 10000.0
@@ -593,8 +667,7 @@ def implicit_fermi(h, nsteps, k, nocc, mu, beta, method, osteps, occ_err_limit, 
         it += 1
         p = normalize_implicit_fermi(h, cnst, mu)
         for i in range(nsteps):
-            p2 = p @ p
-            y = 2.0 * (p2 - p) + np.eye(n)
+            y, p2 = setup_linsys(p, k)
             if method == 0:
                 p, _ = conjgrad(y, p, p2, tol)
             else:

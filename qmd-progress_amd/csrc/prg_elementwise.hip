@@ -515,7 +515,62 @@ __global__ void scale_cols_kernel(const double* __restrict__ a, const double* __
   }
 }
 
+// out[r][c] = f[r] a[r][c] (rows beyond n scaled by 0)
+__global__ void scale_rows_kernel(const double* __restrict__ a, const double* __restrict__ f,
+                                  double* __restrict__ out, int n, int ld) {
+  for (int col = blockIdx.x; col < ld; col += gridDim.x)
+    for (int row = threadIdx.x; row < ld; row += NT)
+      out[row + (size_t)col * ld] = (row < n ? f[row] : 0.0) * a[row + (size_t)col * ld];
+}
+
+// out[i] = 1 / a[i][i], i < n
+__global__ void inv_diag_kernel(const double* __restrict__ a, double* __restrict__ out, int n,
+                                int ld) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) out[i] = 1.0 / a[i + (size_t)i * ld];
+}
+
+// per-block partial sums of Tr(X Y) = sum_ij X[i][j] Y[j][i]: one column j of
+// X (coalesced) against row j of Y
+__global__ void trace_mult_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                  int n, int ld, double* __restrict__ part) {
+  double s = 0.0;
+  for (int j = blockIdx.x; j < n; j += gridDim.x)
+    for (int i = threadIdx.x; i < n; i += NT) s += x[i + (size_t)j * ld] * y[j + (size_t)i * ld];
+  __shared__ double red[NT];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 }  // namespace
+
+int scale_rows_kernel_launch(const DMat& A, const double* d_f, DMat out) {
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(grid_cols(A.ld)), dim3(NT), 0, stream(), A.p, d_f,
+                     out.p, A.n, A.ld);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int inv_diag_kernel_launch(const DMat& A, double* d_out) {
+  hipLaunchKernelGGL(inv_diag_kernel, dim3((A.n + NT - 1) / NT), dim3(NT), 0, stream(), A.p,
+                     d_out, A.n, A.ld);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int trace_mult_kernel_launch(const DMat& X, const DMat& Y, double* d_part, int* nblocks) {
+  const int nb = X.n < 1024 ? X.n : 1024;
+  hipLaunchKernelGGL(trace_mult_kernel, dim3(nb), dim3(NT), 0, stream(), X.p, Y.p, X.n, X.ld,
+                     d_part);
+  PRG_HIP_CHECK(hipGetLastError());
+  *nblocks = nb;
+  return OK;
+}
 
 int transpose_kernel_launch(const DMat& A, DMat out) {
   if (A.ld != out.ld || A.p == out.p) {

@@ -99,6 +99,54 @@ int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters, bool commuting)
   return OK;
 }
 
+// prg_pcg (src/prg_implicit_fermi_mod.F90:934-996): conjugate gradient for
+// A X = B preconditioned by M = diag(A)^-1, X holding the initial guess; B is
+// overwritten with the residual, as the reference does with p2_bml.  Runs
+// while ||R||_F^2 > tol; more than 100 iterations is "not converging".
+// Inner products are bml_trace_mult, Tr(X Y).
+int pcg(const DMat& A, DMat X, DMat B, double tol, int* iters) {
+  DMat Z, D, W;
+  int st;
+  if ((st = ops::temp("pcg_z", A.n, &Z)) || (st = ops::temp("pcg_d", A.n, &D)) ||
+      (st = ops::temp("pcg_w", A.n, &W)))
+    return st;
+  double* minv = (double*)workspace("pcg_minv", sizeof(double) * (size_t)A.ld, &st);
+  if (st) return st;
+  if ((st = ops::inv_diagonal(A, minv))) return st;
+  Sums s;
+  if ((st = ops::mm(A, X, B, false, -1.0, 1.0, &B, 0.0, &s))) return st;  // R = B - A X
+  if ((st = ops::scale_rows(B, minv, Z))) return st;                         // Z = M R
+  double r_new = 0.0, r_old = 0.0;
+  if ((st = ops::trace_mult(Z, B, &r_new))) return st;
+  if ((st = ops::copy(Z, D))) return st;
+  double rn = s.fro2;
+  int k = 0;
+  while (rn > tol) {
+    ++k;
+    if (k != 1 && (st = ops::lin(D, r_new / r_old, &Z, 1.0, 0.0, D))) return st;
+    if ((st = ops::mm(A, D, W, false))) return st;
+    double num = 0.0, den = 0.0;
+    if ((st = ops::trace_mult(B, Z, &num)) || (st = ops::trace_mult(D, W, &den))) return st;
+    const double alpha = num / den;
+    if ((st = ops::lin(X, 1.0, &D, alpha, 0.0, X))) return st;
+    if ((st = ops::lin(B, 1.0, &W, -alpha, 0.0, B, &s))) return st;
+    if ((st = ops::scale_rows(B, minv, Z))) return st;
+    r_old = r_new;
+    if ((st = ops::trace_mult(B, Z, &r_new))) return st;
+    rn = s.fro2;
+    if (!std::isfinite(rn)) {
+      set_error(ERR_NUMERIC, "preconditioned conjugate gradient: non-finite residual");
+      return ERR_NUMERIC;
+    }
+    if (k > 100) {
+      set_error(ERR_NOT_CONVERGED, "PCG is not converging");
+      return ERR_NOT_CONVERGED;
+    }
+  }
+  if (iters) *iters = k;
+  return OK;
+}
+
 namespace {
 
 int g_last_occiter = 0, g_last_totns = 0;
@@ -111,12 +159,36 @@ int normalize_if(const DMat& H, double cnst, double mu, DMat P) {
 
 }  // namespace
 
-// prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), k = 2.
+// prg_setup_linsys (src/prg_implicit_fermi_mod.F90:843-876): the linear system
+// of one level for even expansion order k, A X_next = B with B = X^k and
+// A = X^k + (I - X)^k (k = 2: A = 2(X^2 - X) + I).  Every factor is a
+// polynomial of X, so all products are symmetric.
+int setup_linsys(const DMat& X, DMat A, DMat B, int k) {
+  int st;
+  if (k == 2) {
+    if ((st = ops::mm(X, X, B, true))) return st;
+    return ops::lin(B, 2.0, &X, -2.0, 1.0, A);
+  }
+  DMat X2, Y, T;
+  if ((st = ops::temp("lin_x2", X.n, &X2)) || (st = ops::temp("lin_y", X.n, &Y)) ||
+      (st = ops::temp("lin_t", X.n, &T)))
+    return st;
+  if ((st = ops::mm(X, X, X2, true))) return st;
+  if ((st = ops::lin(X2, 1.0, &X, -2.0, 1.0, Y))) return st;  // (I - X)^2
+  if ((st = ops::copy(X2, B)) || (st = ops::copy(Y, A))) return st;
+  for (int i = 0; i < (k - 2) / 2; ++i) {
+    if ((st = ops::mm(B, X2, T, true)) || (st = ops::copy(T, B))) return st;
+    if ((st = ops::mm(A, Y, T, true)) || (st = ops::copy(T, A))) return st;
+  }
+  return ops::lin(A, 1.0, &B, 1.0, 0.0, A);
+}
+
+// prg_implicit_fermi (src/prg_implicit_fermi_mod.F90:209-336), even k.
 int implicit_fermi(const DMat& H, DMat P, int nsteps, int k, double nocc, double* mu,
                    double beta, int method, int osteps, double occ_err_limit, double tol) {
-  if (k != 2) {
-    set_error(ERR_UNSUPPORTED, "prg_implicit_fermi: only expansion order k = 2 is supported");
-    return ERR_UNSUPPORTED;
+  if (k < 2 || k % 2 != 0 || k > 64) {
+    set_error(ERR_ARG, "prg_implicit_fermi: the expansion order k must be even (2..64)");
+    return ERR_ARG;
   }
   if (nsteps < 1 || nsteps > 60) {
     set_error(ERR_ARG, "prg_implicit_fermi: nsteps out of range");
@@ -130,7 +202,7 @@ int implicit_fermi(const DMat& H, DMat P, int nsteps, int k, double nocc, double
   if (method == 1) {
     if ((st = ops::identity(AI)) || (st = ops::identity(I))) return st;
   }
-  const double cnst = beta / (4.0 * std::pow(2.0, nsteps));
+  const double cnst = beta / (4.0 * std::pow((double)k, nsteps));
   double occ_err = 10000.0;
   int iter = 0;
   Sums s;
@@ -138,8 +210,8 @@ int implicit_fermi(const DMat& H, DMat P, int nsteps, int k, double nocc, double
     ++iter;
     if ((st = normalize_if(H, cnst, *mu, P))) return st;
     for (int i = 0; i < nsteps; ++i) {
-      if ((st = ops::mm(P, P, P2, true))) return st;
-      if ((st = ops::lin(P2, 2.0, &P, -2.0, 1.0, Y))) return st;  // Y = 2(P2 - P) + I
+      // k = 2: P2 = P^2, Y = 2(P2 - P) + I; otherwise P2 = P^k, Y = P^k + (I - P)^k
+      if ((st = setup_linsys(P, Y, P2, k))) return st;
       if (method == 0) {
         if ((st = conjgrad(Y, P, P2, tol, true, nullptr))) return st;
       } else {
@@ -488,6 +560,142 @@ bool dev_inv(bml_matrix_t** inv, int nsteps, std::vector<DMat>& out, bool write)
 }  // namespace
 
 extern "C" {
+
+// New C symbols for the linear solvers of src/prg_implicit_fermi_mod.F90, with
+// the Fortran argument order; the auxiliary matrices are accepted and unused
+// (the library keeps its own).  prg_conjgrad (:1006-1048) and prg_pcg
+// (:934-996) solve A X = B (X in p_bml, initial guess on entry); prg_pcg
+// overwrites p2_bml with the residual as the reference does.  prg_newtonschulz
+// (:888-922) refines ai_bml towards A^-1 until ||I - A Ai||_F <= tol (CG
+// fallback when the error grows) and reports the iteration count.
+void prg_conjgrad(bml_matrix_t* A_bml, bml_matrix_t* p_bml, bml_matrix_t* p2_bml,
+                  bml_matrix_t* tmp_bml, bml_matrix_t* d_bml, bml_matrix_t* w_bml, double cg_tol,
+                  double threshold) {
+  (void)tmp_bml;
+  (void)d_bml;
+  (void)w_bml;
+  (void)threshold;
+  clear_error();
+  if (check_pair(A_bml, p_bml, "prg_conjgrad") || check_pair(A_bml, p2_bml, "prg_conjgrad")) return;
+  DMat A, X, B;
+  if (bml_dev_read(A_bml, &A) || bml_dev_read(p2_bml, &B) || bml_dev_read(p_bml, &X)) return;
+  p_bml->host_valid = false;
+  conjgrad(A, X, B, cg_tol, false, nullptr);
+}
+
+void prg_pcg(bml_matrix_t* A_bml, bml_matrix_t* p_bml, bml_matrix_t* p2_bml, bml_matrix_t* d_bml,
+             bml_matrix_t* wtmp_bml, double cg_tol, double threshold) {
+  (void)d_bml;
+  (void)wtmp_bml;
+  (void)threshold;
+  clear_error();
+  if (check_pair(A_bml, p_bml, "prg_pcg") || check_pair(A_bml, p2_bml, "prg_pcg")) return;
+  DMat A, X, B;
+  if (bml_dev_read(A_bml, &A) || bml_dev_read(p_bml, &X) || bml_dev_read(p2_bml, &B)) return;
+  p_bml->host_valid = false;
+  p2_bml->host_valid = false;
+  pcg(A, X, B, cg_tol, nullptr);
+}
+
+void prg_newtonschulz(bml_matrix_t* a_bml, bml_matrix_t* ai_bml, bml_matrix_t* r_bml,
+                      bml_matrix_t* tmp_bml, bml_matrix_t* d_bml, bml_matrix_t* I_bml, double tol,
+                      double threshold, int* num_iter) {
+  (void)r_bml;
+  (void)tmp_bml;
+  (void)d_bml;
+  (void)I_bml;
+  (void)threshold;
+  clear_error();
+  if (check_pair(a_bml, ai_bml, "prg_newtonschulz")) return;
+  DMat A, Ai;
+  if (bml_dev_read(a_bml, &A) || bml_dev_read(ai_bml, &Ai)) return;
+  ai_bml->host_valid = false;
+  int it = 0;
+  if (newtonschulz(A, Ai, tol, &it, false)) return;
+  if (num_iter) *num_iter = it;
+}
+
+// New C symbol for prg_finite_diff (src/prg_implicit_fermi_mod.F90:726-831):
+// finite-difference response of prg_get_density_matrix along the series
+// H(l) = H0 + sum_i l^i H_i, mu(l) = mu0 + sum_i l^i mu_i at l = lambda with
+// step h: order 1 the forward difference (D(l + h) - D(l)) / h, order 2
+// (D(l + h) - 2 D(l) + D(l - h)) / (2 h^2), order 3 (D(l + 2h) - D(l - 2h)) /
+// (12 h^3) - (D(l + h) - D(l - h)) / (6 h^3).  The reference only prints them
+// (top-left 10 x 10 times 1000); here out_list[k] (order k + 1) receives each
+// when out_list is given, and with out_list NULL the same corners are printed.
+void prg_finite_diff(bml_matrix_t* H0_bml, bml_matrix_t** H_list, double mu0,
+                     const double* mu_list, double beta, int order, double lambda, double h,
+                     double threshold, bml_matrix_t** out_list) {
+  (void)threshold;
+  clear_error();
+  if (!H0_bml || !H_list || !mu_list || order < 1 || order > 3 || !(h != 0.0)) {
+    set_error(ERR_ARG, "prg_finite_diff: bad arguments");
+    return;
+  }
+  DMat H0;
+  if (bml_dev_read(H0_bml, &H0)) return;
+  std::vector<DMat> Hs(order);
+  for (int i = 0; i < order; ++i) {
+    if (check_pair(H0_bml, H_list[i], "prg_finite_diff") || bml_dev_read(H_list[i], &Hs[i])) return;
+  }
+  const int n = H0.n;
+  DMat T, D0, D1p, D1m, D2p, D2m, F;
+  if (ops::temp("fdf_t", n, &T) || ops::temp("fdf_d0", n, &D0) || ops::temp("fdf_d1p", n, &D1p) ||
+      ops::temp("fdf_d1m", n, &D1m) || ops::temp("fdf_d2p", n, &D2p) ||
+      ops::temp("fdf_d2m", n, &D2m) || ops::temp("fdf_f", n, &F))
+    return;
+  auto density_at = [&](double l, DMat out) -> int {
+    int st;
+    double mu = mu0, lp = 1.0;
+    if ((st = ops::copy(H0, T))) return st;
+    for (int i = 0; i < order; ++i) {
+      lp *= l;
+      if ((st = ops::lin(T, 1.0, &Hs[i], lp, 0.0, T))) return st;
+      mu += lp * mu_list[i];
+    }
+    return density_fermi(T, beta, mu, out, nullptr);
+  };
+  auto emit = [&](int k, const DMat& src) -> int {
+    if (out_list) {
+      if (!out_list[k] || out_list[k]->N != n) {
+        set_error(ERR_ARG, "prg_finite_diff: bad output handle");
+        return ERR_ARG;
+      }
+      DMat o;
+      if (bml_dev_write(out_list[k], &o)) return last_error_code();
+      return ops::copy(src, o);
+    }
+    const int c = n < 10 ? n : 10;
+    std::vector<double> blk((size_t)c * c);
+    PRG_HIP_CHECK(hipMemcpy2DAsync(blk.data(), sizeof(double) * c, src.p, sizeof(double) * src.ld,
+                                   sizeof(double) * c, c, hipMemcpyDeviceToHost, stream()));
+    PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+    std::printf("Finite diff - Order %d * 1000\n", k + 1);
+    for (int r = 0; r < c; ++r) {
+      for (int q = 0; q < c; ++q) std::printf(" %12.6f", 1000.0 * blk[r + (size_t)q * c]);
+      std::printf("\n");
+    }
+    return OK;
+  };
+  const double l = lambda;
+  if (density_at(l, D0) || density_at(l + h, D1p)) return;
+  if (ops::lin(D1p, 1.0 / h, &D0, -1.0 / h, 0.0, F) || emit(0, F)) return;
+  if (order > 1) {
+    if (density_at(l - h, D1m)) return;
+    if (ops::lin(D0, -1.0 / (h * h), &D1m, 0.5 / (h * h), 0.0, F) ||
+        ops::lin(F, 1.0, &D1p, 0.5 / (h * h), 0.0, F) || emit(1, F))
+      return;
+  }
+  if (order > 2) {
+    if (density_at(l - 2.0 * h, D2m) || density_at(l + 2.0 * h, D2p)) return;
+    const double h3 = h * h * h;
+    if (ops::lin(D2p, 1.0 / (12.0 * h3), &D2m, -1.0 / (12.0 * h3), 0.0, F) ||
+        ops::lin(F, 1.0, &D1p, -1.0 / (6.0 * h3), 0.0, F) ||
+        ops::lin(F, 1.0, &D1m, 1.0 / (6.0 * h3), 0.0, F) || emit(2, F))
+      return;
+  }
+  if (hipStreamSynchronize(stream()) != hipSuccess) set_error(ERR_HIP, "prg_finite_diff");
+}
 
 // prg_implicit_fermi at a fixed mu (one pass of src/prg_implicit_fermi_mod.F90:
 // 259-298, method 1) on device pointers (padded layout): dP = P (not scaled

@@ -10,6 +10,9 @@ namespace prg {
 
 int transpose_kernel_launch(const DMat& A, DMat out);
 int scale_cols_kernel_launch(const DMat& A, const double* d_f, DMat out);
+int scale_rows_kernel_launch(const DMat& A, const double* d_f, DMat out);
+int inv_diag_kernel_launch(const DMat& A, double* d_out);
+int trace_mult_kernel_launch(const DMat& X, const DMat& Y, double* d_part, int* nblocks);
 
 namespace ops {
 
@@ -166,6 +169,18 @@ int sums_of(const DMat& A, Sums* s) {
 int transpose(const DMat& A, DMat out) { return transpose_kernel_launch(A, out); }
 int scale_cols(const DMat& A, const double* d_f, DMat out) {
   return scale_cols_kernel_launch(A, d_f, out);
+}
+int scale_rows(const DMat& A, const double* d_f, DMat out) {
+  return scale_rows_kernel_launch(A, d_f, out);
+}
+int inv_diagonal(const DMat& A, double* d_out) { return inv_diag_kernel_launch(A, d_out); }
+int trace_mult(const DMat& X, const DMat& Y, double* out) {
+  int st = OK;
+  double* part = (double*)workspace("ops_trmult", sizeof(double) * 1024, &st);
+  if (st) return st;
+  int nb = 0;
+  if ((st = trace_mult_kernel_launch(X, Y, part, &nb))) return st;
+  return fetch(part, nb, 1, out);
 }
 
 }  // namespace ops

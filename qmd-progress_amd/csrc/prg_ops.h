@@ -49,6 +49,12 @@ int sums_of(const DMat& A, Sums* s);
 int transpose(const DMat& A, DMat out);
 // out[:, j] = A[:, j] * f[j]  (f: device vector of length n)
 int scale_cols(const DMat& A, const double* d_f, DMat out);
+// out[i, :] = f[i] * A[i, :]  (f: device vector of length n)
+int scale_rows(const DMat& A, const double* d_f, DMat out);
+// d_out[i] = 1 / A[i][i]
+int inv_diagonal(const DMat& A, double* d_out);
+// *out = Tr(X Y) = sum_ij X[i][j] Y[j][i] (bml_trace_mult), to the host
+int trace_mult(const DMat& X, const DMat& Y, double* out);
 // Reduce [nb][nq] device partials and copy the nq sums to the host.
 int fetch(const double* d_part, int nb, int nq, double* host);
 

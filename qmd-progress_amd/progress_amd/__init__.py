@@ -140,6 +140,10 @@ def lib():
         "prg_compute_response_SP2": (None, [vp, vp, vp, vp, d, d, i, i, cp, d, d, i]),
         "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
         "prg_canon_response_orig": (None, [vp, vp, d, d, dp, d, i, d, i]),
+        "prg_pcg": (None, [vp, vp, vp, vp, vp, d, d]),
+        "prg_finite_diff": (None, [vp, C.POINTER(vp), d, dp, d, i, d, d, d, C.POINTER(vp)]),
+        "prg_conjgrad": (None, [vp, vp, vp, vp, vp, vp, d, d]),
+        "prg_newtonschulz": (None, [vp, vp, vp, vp, vp, vp, d, d, ip]),
         "prg_canon_response_p1_dpdmu": (None, [vp, dp, vp, i, d, vp, dp, d, i, i]),
         "prg_canon_response_vector": (None, [vp, vp, d, d, dp, d, i, d, i]),
         "prg_compute_response_RS": (None, [vp, vp, vp, d, d, d, i]),
@@ -617,6 +621,45 @@ def canon_response(h1_eig, nocc, beta, evals, mu0, m):
     lib().prg_canon_response(p1.h, mh.h, nocc, beta, _dp(e), mu0, m, n)
     _check()
     return p1.numpy()
+
+
+def finite_diff(h0, hs, mu0, mus, beta, order, lam, h):
+    """prg_finite_diff: returns the list of the order 1..`order` difference
+    matrices (the reference prints them)."""
+    n = h0.shape[0]
+    m0 = Matrix.from_numpy(h0)
+    mh = [Matrix.from_numpy(x) for x in hs[:order]]
+    outs = [Matrix.zeros(n) for _ in range(order)]
+    mu = _dvec(order, mus[:order])
+    lib().prg_finite_diff(m0.h, _handles(mh), mu0, _dp(mu), beta, order, lam, h, 0.0,
+                          _handles(outs))
+    _check()
+    return [o.numpy() for o in outs]
+
+
+def pcg(a, x0, b, cg_tol):
+    """prg_pcg: returns (X, R) with R the residual left in B's matrix."""
+    ma, mx, mb = Matrix.from_numpy(a), Matrix.from_numpy(x0), Matrix.from_numpy(b)
+    lib().prg_pcg(ma.h, mx.h, mb.h, None, None, cg_tol, 0.0)
+    _check()
+    return mx.numpy(), mb.numpy()
+
+
+def conjgrad(a, x0, b, cg_tol):
+    """prg_conjgrad: returns X."""
+    ma, mx, mb = Matrix.from_numpy(a), Matrix.from_numpy(x0), Matrix.from_numpy(b)
+    lib().prg_conjgrad(ma.h, mx.h, mb.h, None, None, None, cg_tol, 0.0)
+    _check()
+    return mx.numpy()
+
+
+def newtonschulz(a, ai0, tol):
+    """prg_newtonschulz: returns (Ai, iterations)."""
+    ma, mi = Matrix.from_numpy(a), Matrix.from_numpy(ai0)
+    it = C.c_int()
+    lib().prg_newtonschulz(ma.h, mi.h, None, None, None, None, tol, 0.0, C.byref(it))
+    _check()
+    return mi.numpy(), it.value
 
 
 def canon_response_orig(h1_eig, nocc, beta, evals, mu0, m):

@@ -185,3 +185,63 @@ def test_response_rs_and_fd_match_oracle(P):
     fd_o = O.compute_response_fd(h, h1, 1e-4, 0.5)
     assert np.max(np.abs(fd - fd_o)) < 1e-8
     assert np.max(np.abs(rs - fd)) < 1e-5
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_implicit_fermi_order_four_matches_oracle(P, h256, method):
+    """Expansion order k = 4 (prg_setup_linsys: A = X^4 + (I - X)^4, B = X^4)."""
+    print(f"[implicit_fermi k=4] method={method}", flush=True)
+    args = (6, 4, 128.0, 0.6, 20.0, method, 3, 1e-9, 1e-16 if method == 0 else 1e-10)
+    p, mu = P.implicit_fermi(h256, *args)
+    p_o, mu_o = O.implicit_fermi(h256, *args)
+    assert abs(mu - mu_o) < 1e-9
+    assert np.linalg.norm(p - p_o) < 1e-8
+
+
+def test_implicit_fermi_rejects_odd_order(P, h256):
+    with pytest.raises(P.ProgressError):
+        P.implicit_fermi(h256, 6, 3, 128.0, 0.6, 20.0, 0, 1, 1e-9, 1e-16)
+
+
+def test_linear_solvers_match_oracle(P):
+    """prg_pcg (Jacobi-preconditioned), prg_conjgrad and prg_newtonschulz
+    through the C ABI against the oracle on an SPD system with a diagonal
+    spread over three decades; B = I (the inverse, as in the recursion)."""
+    print("[pcg / conjgrad / newtonschulz] n=120", flush=True)
+    n = 120
+    rng = np.random.default_rng(11)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    core = (q * rng.uniform(1.0, 2.0, n)) @ q.T
+    dsc = np.diag(np.logspace(0, 1.5, n))
+    a = dsc @ core @ dsc
+    a = 0.5 * (a + a.T)
+    eye = np.eye(n)
+    x, r = P.pcg(a, np.zeros((n, n)), eye, 1e-16)
+    x_o, r_o, _ = O.pcg(a, np.zeros((n, n)), eye, 1e-16)
+    assert np.max(np.abs(a @ x - eye)) < 1e-7
+    assert np.max(np.abs(x - x_o)) < 1e-9 * np.max(np.abs(x_o))
+    assert np.max(np.abs(r - (eye - a @ x))) < 1e-9
+    spd = (q * rng.uniform(1.0, 3.0, n)) @ q.T
+    spd = 0.5 * (spd + spd.T)
+    xc = P.conjgrad(spd, np.zeros((n, n)), eye, 1e-20)
+    assert np.max(np.abs(spd @ xc - eye)) < 1e-9
+    ai, it = P.newtonschulz(spd, 0.25 * eye, 1e-10)
+    ai_o, it_o = O.newtonschulz(spd, 0.25 * eye, 1e-10)
+    assert it == it_o and np.max(np.abs(ai - ai_o)) < 1e-11
+
+
+def test_finite_diff_matches_oracle(P, h256):
+    """prg_finite_diff (orders 1-3) against the oracle: eigensolver densities
+    on both sides, differences scaled by 1/h^k (h = 1e-3), so the bar grows
+    with the order."""
+    print("[finite diff] n=256", flush=True)
+    rng = np.random.default_rng(13)
+    hs = []
+    for _ in range(3):
+        x = rng.standard_normal((256, 256)) * 0.01
+        hs.append(0.5 * (x + x.T))
+    mus = [0.02, -0.01, 0.005]
+    got = P.finite_diff(h256, hs, 0.6, mus, 20.0, 3, 0.1, 1e-3)
+    ref = O.finite_diff(h256, hs, 0.6, mus, 20.0, 3, 0.1, 1e-3)
+    for k, (g, r) in enumerate(zip(got, ref)):
+        assert np.max(np.abs(g - r)) < 1e-11 / 1e-3 ** (k + 1), k

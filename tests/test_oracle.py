@@ -811,3 +811,71 @@ def test_response_rs_matches_finite_differences():
     c1 = c @ vt
     bad = (c * occ) @ c1.T + c1 @ (occ[:, None] * c.T)
     assert np.max(np.abs(bad - fd)) > 1e-3
+
+
+@pytest.mark.parametrize("k", [4, 6])
+def test_implicit_fermi_even_order_is_the_scalar_map(k):
+    """Order k > 2 (prg_setup_linsys): with converged CG solves the level map
+    on every eigenvalue is x <- x^k / (x^k + (1 - x)^k) from
+    x0 = 1/2 - beta / (4 k^nsteps) (e - mu); a mistake in the system (the
+    k = 2 matrix A for k = 4) changes the result."""
+    n, nsteps, mu, beta = 48, 5, 0.1, 6.0
+    h = O.model_hamiltonian(n, 3)
+    p, _ = O.implicit_fermi(h, nsteps, k, n / 2.0, mu, beta, 0, 1, 1e30, 1e-24)
+    e, q = np.linalg.eigh(h)
+    x = 0.5 - beta / (4.0 * k ** nsteps) * (e - mu)
+    for _ in range(nsteps):
+        x = x ** k / (x ** k + (1.0 - x) ** k)
+    assert np.max(np.abs(p - (q * x) @ q.T)) < 1e-10
+    xs = np.array([0.2, 0.4, 0.7, 0.9])
+    a, b = O.setup_linsys(np.diag(xs), k)
+    assert np.allclose(np.diag(b), xs ** k) and np.allclose(np.diag(a), xs ** k + (1 - xs) ** k)
+    wrong_a = 2.0 * (np.diag(xs ** 2) - np.diag(xs)) + np.eye(4)
+    assert not np.allclose(wrong_a, a)
+
+
+def test_pcg_solves_and_beats_plain_cg_on_a_badly_scaled_system():
+    """prg_pcg on the system the implicit-Fermi recursion solves (B = I: the
+    inverse) with a diagonal spread over three decades: it reaches A^-1, the
+    Jacobi preconditioner needs far fewer iterations than plain CG, and the
+    returned residual is B - A X."""
+    n = 80
+    rng = np.random.default_rng(11)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    core = (q * rng.uniform(1.0, 2.0, n)) @ q.T
+    dsc = np.diag(np.logspace(0, 1.5, n))
+    a = dsc @ core @ dsc
+    a = 0.5 * (a + a.T)
+    b = np.eye(n)
+    x, r, k = O.pcg(a, np.zeros((n, n)), b, 1e-18)
+    assert np.max(np.abs(a @ x - b)) < 1e-8
+    assert np.allclose(r, b - a @ x, atol=1e-9)
+    _, k_cg = O.conjgrad(a, np.zeros((n, n)), b, 1e-18, maxit=1000)
+    assert k < k_cg / 4
+
+
+def test_finite_diff_matches_fermi_taylor_coefficients():
+    """prg_finite_diff on commuting (diagonal) H0, H1..H3: the density is the
+    Fermi function of x(l) = e + l a + l^2 b + l^3 c - mu(l) element-wise, so
+    the three differences approach its Taylor coefficients f' x', (f'' x'^2 +
+    f' x'')/2 and (f''' x'^3 + 3 f'' x' x'' + f' x''')/6 at l = 0, with errors
+    O(h) (forward) and O(h^2) (central)."""
+    rng = np.random.default_rng(12)
+    n, beta, mu0 = 12, 4.0, 0.1
+    e = rng.uniform(-1.0, 1.0, n)
+    a, b, c = (rng.uniform(-1.0, 1.0, n) for _ in range(3))
+    mus = [0.2, -0.1, 0.05]
+    h = 1e-3
+    o = O.finite_diff(np.diag(e), [np.diag(a), np.diag(b), np.diag(c)], mu0, mus, beta, 3, 0.0, h)
+    f = 1.0 / (1.0 + np.exp(beta * (e - mu0)))
+    f1 = -beta * f * (1 - f)
+    f2 = beta ** 2 * f * (1 - f) * (1 - 2 * f)
+    f3 = -beta ** 3 * f * (1 - f) * (1 - 6 * f + 6 * f * f)
+    x1, x2, x3 = a - mus[0], 2 * (b - mus[1]), 6 * (c - mus[2])
+    t1 = f1 * x1
+    t2 = (f2 * x1 ** 2 + f1 * x2) / 2
+    t3 = (f3 * x1 ** 3 + 3 * f2 * x1 * x2 + f1 * x3) / 6
+    assert np.max(np.abs(np.diag(o[0]) - t1)) < 20 * h
+    assert np.max(np.abs(np.diag(o[1]) - t2)) < 100 * h * h
+    assert np.max(np.abs(np.diag(o[2]) - t3)) < 1000 * h * h
+    assert np.max(np.abs(o[0] - np.diag(np.diag(o[0])))) == 0.0
