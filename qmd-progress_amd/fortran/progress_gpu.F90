@@ -646,7 +646,33 @@ module prg_implicit_fermi_mod
   integer, parameter :: dp = kind(1.0d0)
   public :: prg_implicit_fermi, prg_implicit_fermi_save_inverse, prg_test_density_matrix
   public :: prg_implicit_fermi_first_order_response
+  public :: prg_conjgrad, prg_newtonschulz, prg_pcg, prg_finite_diff
   interface
+    subroutine c_cg(a, p, p2, t, d, w, tol, thr) bind(C, name="prg_conjgrad")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a, p, p2, t, d, w
+      real(c_double), value :: tol, thr
+    end subroutine
+    subroutine c_pcg(a, p, p2, d, w, tol, thr) bind(C, name="prg_pcg")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a, p, p2, d, w
+      real(c_double), value :: tol, thr
+    end subroutine
+    subroutine c_ns(a, ai, r, t, d, ii, tol, thr, it) bind(C, name="prg_newtonschulz")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: a, ai, r, t, d, ii
+      real(c_double), value :: tol, thr
+      integer(c_int) :: it
+    end subroutine
+    subroutine c_fd(h0, hl, mu0, mul, beta, order, lam, h, thr, outl) &
+         bind(C, name="prg_finite_diff")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h0, outl
+      type(c_ptr) :: hl(*)
+      real(c_double), value :: mu0, beta, lam, h, thr
+      real(c_double) :: mul(*)
+      integer(c_int), value :: order
+    end subroutine
     subroutine c_if(h, p, ns, k, nocc, mu, beta, method, osteps, oel, thr, tol) &
          bind(C, name="prg_implicit_fermi")
       import :: c_ptr, c_double, c_int
@@ -747,6 +773,49 @@ contains
     call c_for(H0_bml%ptr, H1_bml%ptr, P0_bml%ptr, P1_bml%ptr, h, nsteps, mu0, beta, nocc, &
          threshold)
   end subroutine prg_implicit_fermi_first_order_response
+  !> src/prg_implicit_fermi_mod.F90:1006-1048
+  subroutine prg_conjgrad(A_bml, p_bml, p2_bml, tmp_bml, d_bml, w_bml, cg_tol, threshold)
+    type(bml_matrix_t), intent(in) :: A_bml, p2_bml
+    type(bml_matrix_t), intent(inout) :: p_bml, tmp_bml, d_bml, w_bml
+    real(dp), intent(in) :: cg_tol, threshold
+    call c_cg(A_bml%ptr, p_bml%ptr, p2_bml%ptr, tmp_bml%ptr, d_bml%ptr, w_bml%ptr, cg_tol, &
+         threshold)
+  end subroutine prg_conjgrad
+  !> src/prg_implicit_fermi_mod.F90:934-996 (p2_bml returns the residual)
+  subroutine prg_pcg(A_bml, p_bml, p2_bml, d_bml, wtmp_bml, cg_tol, threshold)
+    type(bml_matrix_t), intent(in) :: A_bml
+    type(bml_matrix_t), intent(inout) :: p_bml, p2_bml, d_bml, wtmp_bml
+    real(dp), intent(in) :: cg_tol, threshold
+    call c_pcg(A_bml%ptr, p_bml%ptr, p2_bml%ptr, d_bml%ptr, wtmp_bml%ptr, cg_tol, threshold)
+  end subroutine prg_pcg
+  !> src/prg_implicit_fermi_mod.F90:888-922
+  subroutine prg_newtonschulz(a_bml, ai_bml, r_bml, tmp_bml, d_bml, I_bml, tol, threshold, &
+       num_iter)
+    type(bml_matrix_t), intent(inout) :: ai_bml, r_bml, tmp_bml, d_bml
+    type(bml_matrix_t), intent(in) :: a_bml, I_bml
+    real(dp), intent(in) :: threshold, tol
+    integer, intent(out) :: num_iter
+    integer(c_int) :: it
+    call c_ns(a_bml%ptr, ai_bml%ptr, r_bml%ptr, tmp_bml%ptr, d_bml%ptr, I_bml%ptr, tol, &
+         threshold, it)
+    num_iter = it
+  end subroutine prg_newtonschulz
+  !> src/prg_implicit_fermi_mod.F90:726-831 (prints the differences, as the reference)
+  subroutine prg_finite_diff(H0_bml, H_list, mu0, mu_list, beta, order, lambda, h, threshold)
+    type(bml_matrix_t), intent(in) :: H0_bml
+    type(bml_matrix_t), allocatable, intent(in) :: H_list(:)
+    real(dp), allocatable, intent(in) :: mu_list(:)
+    real(dp), intent(in) :: mu0, lambda, beta, threshold, h
+    integer, intent(in) :: order
+    type(c_ptr) :: hl(order)
+    real(c_double) :: ml(order)
+    integer :: i
+    do i = 1, order
+      hl(i) = H_list(i)%ptr
+      ml(i) = mu_list(i)
+    enddo
+    call c_fd(H0_bml%ptr, hl, mu0, ml, beta, order, lambda, h, threshold, c_null_ptr)
+  end subroutine prg_finite_diff
 end module prg_implicit_fermi_mod
 
 module prg_response_mod
@@ -756,8 +825,53 @@ module prg_response_mod
   implicit none
   private
   integer, parameter :: dp = kind(1.0d0)
-  public :: prg_compute_response_SP2
+  public :: prg_compute_response_SP2, prg_compute_response_RS, prg_compute_response_FD
+  public :: prg_canon_response, prg_canon_response_orig, prg_canon_response_vector
+  public :: prg_canon_response_p1_dpdmu
   interface
+    subroutine c_rs(h, h1, rsp, lam, bndfil, thr, v) bind(C, name="prg_compute_response_RS")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, h1, rsp
+      real(c_double), value :: lam, bndfil, thr
+      integer(c_int), value :: v
+    end subroutine
+    subroutine c_fdr(h, h1, rsp, delta, bndfil, thr, v) bind(C, name="prg_compute_response_FD")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, h1, rsp
+      real(c_double), value :: delta, bndfil, thr
+      integer(c_int), value :: v
+    end subroutine
+    subroutine c_canon(p1, h1, nocc, beta, e, mu0, m, hdim) bind(C, name="prg_canon_response")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: p1, h1
+      real(c_double), value :: nocc, beta, mu0
+      real(c_double) :: e(*)
+      integer(c_int), value :: m, hdim
+    end subroutine
+    subroutine c_canon_o(p1, h1, nocc, beta, e, mu0, m, thr, hdim) &
+         bind(C, name="prg_canon_response_orig")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: p1, h1
+      real(c_double), value :: nocc, beta, mu0, thr
+      real(c_double) :: e(*)
+      integer(c_int), value :: m, hdim
+    end subroutine
+    subroutine c_canon_v(p1, h1, nocc, beta, e, mu0, m, thr, hdim) &
+         bind(C, name="prg_canon_response_vector")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: p1, h1
+      real(c_double), value :: nocc, beta, mu0, thr
+      real(c_double) :: e(*)
+      integer(c_int), value :: m, hdim
+    end subroutine
+    subroutine c_canon_p(p1, dp1, h1, norbs, beta, q, e, mu0, m, hdim) &
+         bind(C, name="prg_canon_response_p1_dpdmu")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: p1, h1, q
+      real(c_double) :: dp1(*), e(*)
+      real(c_double), value :: beta, mu0
+      integer(c_int), value :: norbs, m, hdim
+    end subroutine
     subroutine c_rsp(h, h1, rsp, rho, lam, bndfil, mn, mx, conv, tol, thr, v) &
          bind(C, name="prg_compute_response_SP2")
       import :: c_ptr, c_double, c_int, c_char
@@ -781,6 +895,65 @@ contains
     call c_rsp(ham_bml%ptr, prt_bml%ptr, rsp_bml%ptr, rho_bml%ptr, lambda, bndfil, minsp2iter, &
          maxsp2iter, cstr(sp2conv), idemtol, threshold, verbose)
   end subroutine prg_compute_response_SP2
+  !> src/prg_response_mod.F90:250-361
+  subroutine prg_compute_response_RS(ham_bml, prt_bml, rsp_bml, lambda, bndfil, threshold, verbose)
+    type(bml_matrix_t), intent(in) :: ham_bml, prt_bml
+    type(bml_matrix_t), intent(inout) :: rsp_bml
+    real(dp) :: lambda
+    real(dp), intent(in) :: bndfil, threshold
+    integer :: verbose
+    call c_rs(ham_bml%ptr, prt_bml%ptr, rsp_bml%ptr, lambda, bndfil, threshold, verbose)
+  end subroutine prg_compute_response_RS
+  !> src/prg_response_mod.F90:380-424
+  subroutine prg_compute_response_FD(ham_bml, prt_bml, rsp_bml, prg_delta, bndfil, threshold, &
+       verbose)
+    type(bml_matrix_t), intent(in) :: ham_bml, prt_bml
+    type(bml_matrix_t), intent(inout) :: rsp_bml
+    real(dp) :: prg_delta
+    real(dp), intent(in) :: bndfil, threshold
+    integer :: verbose
+    call c_fdr(ham_bml%ptr, prt_bml%ptr, rsp_bml%ptr, prg_delta, bndfil, threshold, verbose)
+  end subroutine prg_compute_response_FD
+  !> src/prg_response_mod.F90:947-1022
+  subroutine prg_canon_response(P1_bml, H1_bml, Nocc, beta, evals, mu0, m, HDIM)
+    type(bml_matrix_t), intent(inout) :: P1_bml, H1_bml
+    integer, intent(in) :: HDIM, m
+    real(dp), intent(in) :: evals(HDIM), beta, mu0, nocc
+    real(c_double) :: e(HDIM)
+    e = evals
+    call c_canon(P1_bml%ptr, H1_bml%ptr, nocc, beta, e, mu0, m, HDIM)
+  end subroutine prg_canon_response
+  !> src/prg_response_mod.F90:1038-1105
+  subroutine prg_canon_response_orig(P1_bml, H1_bml, Nocc, beta, evals, mu0, m, thresh, HDIM)
+    type(bml_matrix_t), intent(inout) :: P1_bml, H1_bml
+    integer, intent(in) :: HDIM, m
+    real(dp), intent(in) :: evals(HDIM), thresh, beta, mu0, nocc
+    real(c_double) :: e(HDIM)
+    e = evals
+    call c_canon_o(P1_bml%ptr, H1_bml%ptr, nocc, beta, e, mu0, m, thresh, HDIM)
+  end subroutine prg_canon_response_orig
+  !> src/prg_response_mod.F90:849-931
+  subroutine prg_canon_response_vector(P1_bml, H1_bml, Nocc, beta, evals, mu0, m, thresh, HDIM)
+    type(bml_matrix_t), intent(inout) :: P1_bml, H1_bml
+    integer, intent(in) :: HDIM, m
+    real(dp), intent(in) :: evals(HDIM), thresh, beta, mu0, nocc
+    real(c_double) :: e(HDIM)
+    e = evals
+    call c_canon_v(P1_bml%ptr, H1_bml%ptr, nocc, beta, e, mu0, m, thresh, HDIM)
+  end subroutine prg_canon_response_vector
+  !> src/prg_response_mod.F90:1112-1183
+  subroutine prg_canon_response_p1_dpdmu(P1_bml, dPdMu, H1_bml, Norbs, beta, Q_bml, evals, mu0, &
+       m, HDIM)
+    type(bml_matrix_t), intent(inout) :: P1_bml, H1_bml
+    type(bml_matrix_t) :: Q_bml
+    integer, intent(in) :: HDIM, m, Norbs
+    real(dp), intent(in) :: evals(HDIM), beta, mu0
+    real(dp), intent(inout) :: dPdMu(HDIM)
+    real(c_double) :: e(HDIM), d(HDIM)
+    e = evals
+    call c_canon_p(P1_bml%ptr, d, H1_bml%ptr, Norbs, beta, Q_bml%ptr, e, mu0, m, HDIM)
+    dPdMu = d
+  end subroutine prg_canon_response_p1_dpdmu
 end module prg_response_mod
 
 module prg_sp2_fermi_mod
