@@ -51,6 +51,8 @@ bml_matrix_t *bml_zero_matrix(bml_matrix_type_t matrix_type, bml_matrix_precisio
 bml_matrix_t *bml_identity_matrix(bml_matrix_type_t matrix_type,
                                   bml_matrix_precision_t precision, int N, int M,
                                   bml_distribution_mode_t distrib_mode);
+/* Sparse types (ellpack, ellsort, ellblock, csr) keep only the entries with
+ * |a| > threshold, as the BML formats do; every type is stored dense. */
 bml_matrix_t *bml_import_from_dense(bml_matrix_type_t matrix_type,
                                     bml_matrix_precision_t precision, bml_dense_order_t order,
                                     int N, int M, void *A, double threshold,

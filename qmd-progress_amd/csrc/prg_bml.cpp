@@ -163,7 +163,6 @@ bml_matrix_t* bml_import_from_dense(bml_matrix_type_t matrix_type,
                                     bml_matrix_precision_t precision, bml_dense_order_t order,
                                     int N, int M, void* Av, double threshold,
                                     bml_distribution_mode_t distrib_mode) {
-  (void)threshold;
   bml_matrix_t* A = bml_zero_matrix(matrix_type, precision, N, M, distrib_mode);
   if (!A) return nullptr;
   const double* src = (const double*)Av;
@@ -177,6 +176,12 @@ bml_matrix_t* bml_import_from_dense(bml_matrix_type_t matrix_type,
   } else {
     for (int i = 0; i < N; ++i)
       for (int j = 0; j < N; ++j) A->host[i + (size_t)j * N] = src[(size_t)i * N + j];
+  }
+  // The sparse BML formats keep only the entries above the threshold
+  // (|a| > threshold); stored dense here, the others become zeros.
+  if (matrix_type != dense && threshold > 0.0) {
+    for (size_t k = 0; k < (size_t)N * N; ++k)
+      if (!(std::fabs(A->host[k]) > threshold)) A->host[k] = 0.0;
   }
   return A;
 }

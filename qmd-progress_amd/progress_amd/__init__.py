@@ -192,13 +192,17 @@ class Matrix:
         return cls(h)
 
     @classmethod
-    def from_numpy(cls, a: np.ndarray) -> "Matrix":
+    def from_numpy(cls, a: np.ndarray, matrix_type: int = None,
+                   threshold: float = 0.0) -> "Matrix":
+        """bml_import_from_dense; a sparse matrix_type (ellpack = 2, ...) keeps
+        only the entries above `threshold`, as BML's sparse formats do."""
         a = np.asfortranarray(np.asarray(a, dtype=np.float64))
         n = a.shape[0]
         if a.shape != (n, n):
             raise ValueError("square matrix expected")
-        h = lib().bml_import_from_dense(_dense, _double_real, _col_major, n, n,
-                                        a.ctypes.data_as(C.c_void_p), 0.0, _sequential)
+        h = lib().bml_import_from_dense(_dense if matrix_type is None else matrix_type,
+                                        _double_real, _col_major, n, n,
+                                        a.ctypes.data_as(C.c_void_p), threshold, _sequential)
         _check()
         return cls(h)
 

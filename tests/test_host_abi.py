@@ -105,3 +105,13 @@ def test_sp2_inverse_and_entropy_function_match_oracle():
     go, eo = O.sp2_entropy_function(mu, h1, hn, sg)
     assert np.array_equal(ee, eo)
     assert np.max(np.abs(gg - go)) < 1e-12
+
+
+def test_sparse_import_keeps_entries_above_threshold():
+    """bml_import_from_dense with a sparse type (ellpack) keeps |a| > threshold,
+    the dense type keeps every entry (host-side, no GPU compute)."""
+    import progress_amd as P
+    a = np.array([[1.0, 1e-6, -2e-5], [1e-6, 2.0, 3e-4], [-2e-5, 3e-4, -1.0]])
+    ell = P.Matrix.from_numpy(a, matrix_type=2, threshold=1e-5).numpy()
+    assert np.array_equal(ell, np.where(np.abs(a) > 1e-5, a, 0.0))
+    assert np.array_equal(P.Matrix.from_numpy(a, threshold=1e-5).numpy(), a)
