@@ -365,6 +365,8 @@ def graph_bench(args, P, torch, np):
     err = float(np.linalg.norm(rho - dd))
     rel = err / float(np.linalg.norm(dd))
     tr_err = abs(float(np.trace(rho)) - n)
+    # band energy Tr(rho H) against the dense solve's (rho = 2 X on both sides)
+    eb_err = abs(float(np.sum(rho * h)) - float(np.sum(dd * h)))
     achieved = executed / loop_s / 1e12
     cpu = None if args.no_cpu else graph_cpu_baseline(h, gp, pp, it, lo, hi)
     out = {
@@ -393,7 +395,8 @@ def graph_bench(args, P, torch, np):
                       "extraction_and_plan": (dt - dev_s) * 1e3},
         "setup_s": {"generate": t_gen, "dense_int8crt_solve": t_dense},
         "alg_tflops": alg / dt / 1e12,
-        "accuracy_vs_dense": {"fro": err, "rel_fro": rel, "trace_err": tr_err},
+        "accuracy_vs_dense": {"fro": err, "rel_fro": rel, "trace_err": tr_err,
+                              "band_energy_err": eb_err},
         "tiles": {"panel": panel, "prefetch_slabs": depth, "k_slab": kslab,
                   "count": sum(batched_tile_count(m, panel) for m in mpad),
                   "launch": {0: "eager", 1: "captured hipGraph", 2: "hipGraph replay"}[launch]},
@@ -536,7 +539,9 @@ def fermi_bench(args, P, torch, np):
     pref = (q * f) @ q.T
     err = torch.linalg.norm(pm - pref).item()
     tr = abs(torch.trace(pm).item() - nocc)
-    acc = {"fro_err_vs_fermi_eigh": err, "trace_err": tr}
+    # band energy 2 Tr(P H) (spin-paired occupation) against the eigendecomposition's
+    eb = abs(2.0 * torch.sum(pm * h).item() - 2.0 * torch.sum(f * wv).item())
+    acc = {"fro_err_vs_fermi_eigh": err, "trace_err": tr, "band_energy_err": eb}
     if p_fp64 is not None:
         acc["fro_diff_vs_fp64_products"] = torch.linalg.norm(pm - p_fp64).item()
     if moduli is None:
