@@ -563,13 +563,14 @@ def fermi_bench(args, P, torch, np):
                         "included: a lower bound of the kernel rate)"}
         metric = "implicit Fermi solves/s (prg_implicit_fermi, int8 CRT products)"
         dtype = "int8/f64"
+    cpu = None if args.no_cpu else products_cpu_baseline(hh, nprod, args.cpu_iters)
     _line(metric, 1.0 / dt, "solves/s", dt, args, dtype,
           {"workload": f"prg_implicit_fermi model tight-binding H, N={n}, beta=20 (kT 0.05 eV), "
                        f"8 levels, k=2, method 1 (Newton-Schulz, tol {args.ns_tol:g}), "
                        f"seed={args.seed}", "n": n, "products": products,
            "mu_start": mu, "mu": mu1},
           {"accuracy": acc, "products_per_solve": nprod, "alg_tflops": alg / dt / 1e12,
-           "cpu_baseline": None, "roofline": roof})
+           "cpu_baseline": cpu, "roofline": roof})
 
 
 def adjoint_bench(args, P, torch, np):
@@ -728,6 +729,31 @@ def graph_cpu_baseline(h, gp, pp, iters, lo, hi, stride=8):
             "sample": f"oracle FP64 per-part SP2 (numpy/OpenBLAS dgemm) on {len(sample)} of "
                       f"{len(halos)} parts (every {stride}th), {iters} iterations each, "
                       f"{t:.2f} s, scaled by sum n^3 ({scale:.2f}x)"}
+
+
+def products_cpu_baseline(h, nprod, iters):
+    """The oracle's FP64 products (numpy / OpenBLAS dgemm) at the full N: `iters`
+    products of the normalised H timed, scaled to the solve's product count
+    (the recurrences and element-wise updates of the solve are not counted,
+    which favours the CPU)."""
+    import numpy as np
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max(i.get("num_threads", 1) for i in threadpool_info()
+                      if i.get("internal_api") == "openblas") if threadpool_info() else os.cpu_count()
+    except Exception:
+        threads = os.cpu_count()
+    x = np.asfortranarray(h / np.abs(h).sum(axis=0).max())
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        x = x @ x
+    t = (time.perf_counter() - t0) / iters
+    n = h.shape[0]
+    solve_s = t * nprod
+    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{iters} FP64 products at N={n} (numpy/OpenBLAS dgemm, {t:.2f} s each), "
+                      f"scaled to the solve's {nprod:.0f} products",
+            "tflops": 2.0 * float(n) ** 3 / t / 1e12}
 
 
 def cpu_baseline(n, seed, iters, gpu_iters):
