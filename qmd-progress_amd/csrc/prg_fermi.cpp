@@ -26,6 +26,7 @@ using ops::Sums;
 int conjgrad(const DMat& A, DMat X, const DMat& B, double tol, bool commuting, int* iters) {
   DMat R, D, W;
   int st;
+  ops::HoldOperand hold(A);  // A is not changed here
   if ((st = ops::temp("cg_r", A.n, &R)) || (st = ops::temp("cg_d", A.n, &D)) ||
       (st = ops::temp("cg_w", A.n, &W)))
     return st;
@@ -68,6 +69,7 @@ int conjgrad(const DMat& A, DMat X, const DMat& B, double tol, bool commuting, i
 int newtonschulz(const DMat& A, DMat Ai, double tol, int* iters, bool commuting) {
   DMat R, T, I;
   int st;
+  ops::HoldOperand hold(A);  // A is not changed here
   if ((st = ops::temp("ns_r", A.n, &R)) || (st = ops::temp("ns_t", A.n, &T)) ||
       (st = ops::temp("ns_i", A.n, &I)))
     return st;

@@ -1127,6 +1127,34 @@ int crt_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double s
 
 int crt_last_waves() { return g_last_config; }
 
+// One operand that a caller promises not to change while held (the matrix of
+// a Newton-Schulz or CG solve): its residue planes and scale are formed once
+// and reused by every product that names it until the hold ends.
+namespace {
+struct HeldOperand {
+  const double* key = nullptr;
+  int depth = 0;
+  bool valid = false;
+  int np = 0, ld = 0;
+};
+HeldOperand g_held;
+}  // namespace
+
+void crt_hold_operand(const double* p) {
+  if (g_held.depth == 0) {
+    g_held.key = p;
+    g_held.valid = false;
+  }
+  if (g_held.key == p) ++g_held.depth;
+}
+
+void crt_release_operand(const double* p) {
+  if (g_held.depth > 0 && g_held.key == p && --g_held.depth == 0) {
+    g_held.key = nullptr;
+    g_held.valid = false;
+  }
+}
+
 int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
              double gamma, double* part, int* nparts, const DMat* dotE, int np) {
   const int ld = C.ld, pld = plane_ld(ld);
@@ -1146,14 +1174,19 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
   const int nt = pld / TM;
   const int nb = nt * (nt + 1) / 2;
   const bool same = (A.p == B.p);
+  const bool held_a = g_held.depth > 0 && A.p == g_held.key;
+  const bool held_b = g_held.depth > 0 && B.p == g_held.key && !same;
   int st = OK;
-  int8_t* pa_planes = (int8_t*)workspace("crtg_planes_a", (size_t)np * plane, &st);
-  int8_t* pb_planes =
-      same ? nullptr : (int8_t*)workspace("crtg_planes_b", (size_t)np * plane, &st);
+  int8_t* pa_planes = (int8_t*)workspace(held_a ? "crtg_planes_held" : "crtg_planes_a",
+                                         (size_t)np * plane, &st);
+  int8_t* pb_planes = same ? nullptr
+                           : (int8_t*)workspace(held_b ? "crtg_planes_held" : "crtg_planes_b",
+                                                (size_t)np * plane, &st);
   int8_t* res = (int8_t*)workspace("crt_residues", (size_t)nb * np * kItemBytes, &st);
-  double* sc = (double*)workspace("crtg_scales", sizeof(double) * 4, &st);
+  double* sc = (double*)workspace("crtg_scales", sizeof(double) * 6, &st);
   double* npart = (double*)workspace("crtg_norm_part", sizeof(double) * 1024, &st);
   if (st) return st;
+  if ((held_a || held_b) && (g_held.np != np || g_held.ld != ld)) g_held.valid = false;
   const double* fold = fold_constants(np, &st);
   if (st) return st;
   const int4* items = item_list(nt, np, &st);
@@ -1169,8 +1202,16 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
     PRG_HIP_CHECK(hipGetLastError());
     return OK;
   };
-  if ((st = scale_split(A, pa_planes, sc))) return st;
-  if (!same && (st = scale_split(B, pb_planes, sc + 2))) return st;
+  // scale slots: A at sc, B at sc + 2, the held operand at sc + 4
+  double* sa = held_a ? sc + 4 : sc;
+  double* sb = same ? sa : (held_b ? sc + 4 : sc + 2);
+  if (!(held_a && g_held.valid) && (st = scale_split(A, pa_planes, sa))) return st;
+  if (!same && !(held_b && g_held.valid) && (st = scale_split(B, pb_planes, sb))) return st;
+  if (held_a || held_b) {
+    g_held.valid = true;
+    g_held.np = np;
+    g_held.ld = ld;
+  }
   ProductArgs pa;
   pa.planes[0] = pa_planes;
   pa.planes[1] = pa_planes;
@@ -1185,8 +1226,8 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
   PRG_HIP_CHECK(hipGetLastError());
   GemmEpiArgs ea;
   ea.res = res;
-  ea.sa = sc;
-  ea.sb = same ? sc : sc + 2;
+  ea.sa = sa;
+  ea.sb = sb;
   ea.Cin = (Cin && beta != 0.0) ? Cin->p : nullptr;
   ea.E = dotE ? dotE->p : nullptr;
   ea.C = C.p;

@@ -60,6 +60,9 @@ int set_product_precision(int mode) {
   return OK;
 }
 
+HoldOperand::HoldOperand(const DMat& A) : p(A.p) { crt_hold_operand(p); }
+HoldOperand::~HoldOperand() { crt_release_operand(p); }
+
 int temp(const char* name, int n, DMat* out) {
   int st = OK;
   const int ld = padded(n);

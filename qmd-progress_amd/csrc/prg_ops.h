@@ -25,6 +25,14 @@ void gemm_counters(double* products, double* alg_flops, double* exec_flops, bool
 int product_precision();
 int set_product_precision(int mode);
 
+// Scope during which A is not modified: int8 CRT products reuse its residue
+// planes (no effect on FP64 products).
+struct HoldOperand {
+  explicit HoldOperand(const DMat& A);
+  ~HoldOperand();
+  const double* p;
+};
+
 // n x n temporary held in a named workspace slot (contents undefined).
 int temp(const char* name, int n, DMat* out);
 // C = alpha A B + beta Cin + gamma I   (sym: A and B symmetric and commuting,

@@ -59,6 +59,11 @@ int crt_moduli();
 // ||C||_F^2, <C, dotE>, 0], *nparts blocks.  C is exactly symmetric.
 int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
              double gamma, double* part, int* nparts, const DMat* dotE, int np);
+// Hold operand p across crt_gemm calls (its planes are formed once); the
+// caller must not change the matrix until the matching release.  Holds of the
+// same pointer nest; a hold of another pointer while one is active is ignored.
+void crt_hold_operand(const double* p);
+void crt_release_operand(const double* p);
 int crt_last_waves();  // 800 + ring stages of the last int8 CRT product kernel (0: none yet)
 int crt_update_blocks(int ld);  // partial-sum slots one int8 CRT step writes
 // out = scale X[cur] after an int8 CRT solve (from the residue planes when no
