@@ -62,3 +62,57 @@ def test_driver_matches_oracle(golden, tmp_path, method):
     # rho is written with 17 significant digits
     assert np.linalg.norm(rho - ref) < 1e-9
     assert "sp2run: norb=384" in r.stdout
+
+
+RESPSOLVE = ROOT / "qmd-progress_amd" / "bin" / "respsolve"
+
+
+def _write_mtx(path, a):
+    n = a.shape[0]
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{n} {n} {n * n}\n")
+        for j in range(n):
+            for i in range(n):
+                f.write(f"{i + 1} {j + 1} {a[i, j]:.17e}\n")
+
+
+@pytest.mark.gpu
+def test_respsolve_driver_matches_oracle(tmp_path):
+    """The Fortran modules' prg_compute_response_RS / _FD, prg_pcg and
+    prg_newtonschulz (fortran/respsolve.F90) against the oracle."""
+    if not RESPSOLVE.exists():
+        _need_driver()
+        pytest.fail("amdflang is present but the respsolve driver was not built")
+    print("[respsolve] n=96", flush=True)
+    n = 96
+    rng = np.random.default_rng(14)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.concatenate([np.sort(rng.uniform(-3.0, -1.0, n // 2)),
+                         np.sort(rng.uniform(1.0, 3.0, n // 2))])
+    h = (q * ev) @ q.T
+    h = 0.5 * (h + h.T)
+    h1 = rng.standard_normal((n, n))
+    h1 = 0.5 * (h1 + h1.T)
+    a = (q * rng.uniform(1.0, 3.0, n)) @ q.T
+    a = 0.5 * (a + a.T)
+    for name, m in (("h", h), ("h1", h1), ("a", a)):
+        _write_mtx(tmp_path / f"{name}.mtx", m)
+    pre = tmp_path / "out"
+    r = subprocess.run([str(RESPSOLVE), str(tmp_path / "h.mtx"), str(tmp_path / "h1.mtx"),
+                        str(tmp_path / "a.mtx"), str(pre)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the files hold what the library computed from the 17-digit inputs
+    h, h1, a = (O.read_mtx(str(tmp_path / f"{x}.mtx")) for x in ("h", "h1", "a"))
+    rs = O.read_mtx(f"{pre}_rs.mtx")
+    fd = O.read_mtx(f"{pre}_fd.mtx")
+    x = O.read_mtx(f"{pre}_pcg.mtx")
+    ai = O.read_mtx(f"{pre}_ns.mtx")
+    assert np.max(np.abs(rs - O.compute_response_rs(h, h1, 0.5))) < 1e-9
+    assert np.max(np.abs(fd - O.compute_response_fd(h, h1, 1e-4, 0.5))) < 1e-8
+    x_o, _, _ = O.pcg(a, np.zeros((n, n)), np.eye(n), 1e-16)
+    assert np.max(np.abs(x - x_o)) < 1e-9
+    ai_o, it_o = O.newtonschulz(a, 0.25 * np.eye(n), 1e-10)
+    assert np.max(np.abs(ai - ai_o)) < 1e-11
+    assert f"iterations={it_o}" in r.stdout
