@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="auto",
-                    help="fp64 | f16split | int8crt (every --moduli count) | int8crt<m> | auto (all)")
+                    help="fp64 | f16split | bf16split | int8crt (every --moduli count) | "
+                         "int8crt<m> | auto (all)")
     ap.add_argument("--moduli", default="10,11",
                     help="comma-separated moduli counts of the int8 Chinese-remainder products "
                          "(6..14); each runs as mode int8crt<m>")
@@ -139,7 +140,8 @@ def main():
     rho = torch.zeros_like(hpad)
 
     moduli = [int(m) for m in str(args.moduli).split(",") if m]
-    modes = {"fp64": (P.PREC_FP64, 0), "f16split": (P.PREC_F16SPLIT, 0)}
+    modes = {"fp64": (P.PREC_FP64, 0), "f16split": (P.PREC_F16SPLIT, 0),
+             "bf16split": (P.PREC_BF16SPLIT, 0)}
     for m in moduli:
         modes[f"int8crt{m}"] = (P.PREC_INT8CRT, m)
     if args.precision == "auto":
@@ -224,7 +226,7 @@ def main():
     kind = "int8crt" if head.startswith("int8crt") else head
     hmod = hr.get("moduli")
     peak_tf = {"fp64": PEAK["fp64_tflops"], "f16split": PEAK["fp16_tflops"],
-               "int8crt": PEAK["int8_tops"]}[kind]
+               "bf16split": PEAK["fp16_tflops"], "int8crt": PEAK["int8_tops"]}[kind]
     kms = hr.get("product_kernel_ms")
     alg = (2.0 * float(n) ** 3 / (kms * 1e-3) / 1e12) if kms else None
     kern = hr.get("product_kernel_mfma_tops")
@@ -232,6 +234,7 @@ def main():
         "bound": "mfma",
         "kernel": {"fp64": "sp2_step_kernel (FP64 MFMA, fused SP2 update)",
                    "f16split": "sp2_f16split_kernel (FP16 MFMA, 3 products)",
+                   "bf16split": "sp2_f16split_kernel<bf16> (BF16 MFMA, 3 products)",
                    "int8crt": f"crt_product_kernel (int8 MFMA, {hmod} moduli x "
                               "upper-triangle 256x256 tiles)"}[kind],
         "achieved": alg,
@@ -267,6 +270,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "f16split": "f16 (hi+lo split), f64 bookkeeping",
+                  "bf16split": "bf16 (hi+lo split), f64 bookkeeping",
                   "int8crt": "int8 residues (exact), f64 reconstruction"}[kind],
         "data": "synthetic",
         "config": {"workload": f"prg_sp2_alg2 dense model tight-binding H, N={n}, occ=N/2, "
@@ -677,13 +681,13 @@ def fermi_adjoint_bench(args, P, torch, np):
 
 def executed_ops(ld, mode, moduli):
     """MFMA operations of one symmetric SP2 product launch: upper-triangle
-    128x128 tiles (fp64, f16split) or 256x256 tiles over the residue planes'
+    128x128 tiles (fp64, f16split, bf16split) or 256x256 tiles over the residue planes'
     leading dimension rounded up to 256 (int8crt)."""
     tm = 256 if mode == "int8crt" else 128
     kd = ((ld + 255) // 256) * 256 if mode == "int8crt" else ld
     nt = kd // tm
     per_pass = nt * (nt + 1) // 2 * 2.0 * tm * tm * kd
-    return per_pass * {"fp64": 1, "f16split": 3, "int8crt": moduli}[mode]
+    return per_pass * {"fp64": 1, "f16split": 3, "bf16split": 3, "int8crt": moduli}[mode]
 
 
 def traffic_bytes(mode, n, moduli):

@@ -361,6 +361,9 @@ void prg_gpu_subgraph_config(int *panel, int *depth, int *kslab, int *graph);
 /* Selects how prg_sp2_* evaluate the X^2 products (default PRG_PREC_FP64).
  *   PRG_PREC_FP64      FP64 MFMA;
  *   PRG_PREC_F16SPLIT  FP16 hi+lo split products with FP32 accumulation;
+ *   PRG_PREC_BF16SPLIT BF16 hi+lo split products with FP32 accumulation
+ *                      (about 16 significant bits; same stopping and FP64
+ *                      polishing as the FP16 split);
  *   PRG_PREC_INT8CRT   exact int8 MFMA products of X's residues modulo
  *                      pairwise-coprime moduli <= 256, Chinese-remainder
  *                      reconstruction in FP64 (11 moduli: X carried to
@@ -369,6 +372,7 @@ void prg_gpu_subgraph_config(int *panel, int *depth, int *kslab, int *graph);
 #define PRG_PREC_FP64 0
 #define PRG_PREC_F16SPLIT 1
 #define PRG_PREC_INT8CRT 2
+#define PRG_PREC_BF16SPLIT 3
 void prg_set_precision(int mode);
 int prg_get_precision(void);
 /* Number of moduli (6..14, default 11) of PRG_PREC_INT8CRT; X is scaled by

@@ -10,6 +10,10 @@
 // partial sums as the FP64 kernel, so the FP64 bookkeeping and stopping rules
 // are shared.  The dropped lo*lo term is 2^-22 relative.
 //
+// PRG_PREC_BF16SPLIT is the same kernel on BF16 planes, hi = bf16(x) and
+// lo = bf16((x - hi) * 2^8) (about 16 significant bits), with
+// v_mfma_f32_16x16x32_bf16.
+//
 // Because the planes are symmetric, the A operand (rows of the I panel) and
 // the B operand (columns of the J panel) are both read as contiguous column
 // chunks of a plane.
@@ -24,24 +28,47 @@ using tile::sym_tile;
 using tile::xcd_remap;
 
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef int i4 __attribute__((ext_vector_type(4)));
+
+// The two 16-bit split formats: element type, lo scale (2^(mantissa bits)),
+// and the 16x16x32 MFMA with FP32 accumulation.
+template <typename T>
+struct Split;
+template <>
+struct Split<_Float16> {
+  typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+  static constexpr double kScale = 2048.0;
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Split<__bf16> {
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  static constexpr double kScale = 256.0;
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
 
 constexpr int BM = 128, BK = 64, NT = 256;
 constexpr int HROW = BK + 8;  // padded LDS row (halves): 144 bytes
 
 struct F16Args {
   const double* buf[2];         // FP64 master ping-pong
-  const _Float16* hi[2];        // hi planes ping-pong
-  const _Float16* lo[2];        // lo planes ping-pong
+  const void* hi[2];            // hi planes ping-pong (T)
+  const void* lo[2];            // lo planes ping-pong (T)
   int ld;
   Sp2State* st;
   double* part;
 };
 
+template <typename T>
 __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
-  __shared__ _Float16 As[BM][HROW];
-  __shared__ _Float16 Bs[BM][HROW];
+  typedef typename Split<T>::v8 h8;
+  constexpr double kScale = Split<T>::kScale;
+  __shared__ T As[BM][HROW];
+  __shared__ T Bs[BM][HROW];
   __shared__ int s_flags[4];
   if (threadIdx.x == 0) {
     s_flags[0] = g.st->planned;
@@ -55,10 +82,10 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
   const int ld = g.ld;
   const double* X = g.buf[cur];
   double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
-  const _Float16* hi = g.hi[cur];
-  const _Float16* lo = g.lo[cur];
-  _Float16* hin = const_cast<_Float16*>(g.hi[cur ^ 1]);
-  _Float16* lon = const_cast<_Float16*>(g.lo[cur ^ 1]);
+  const T* hi = (const T*)g.hi[cur];
+  const T* lo = (const T*)g.lo[cur];
+  T* hin = (T*)const_cast<void*>(g.hi[cur ^ 1]);
+  T* lon = (T*)const_cast<void*>(g.lo[cur ^ 1]);
 
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   int I, J;
@@ -83,10 +110,10 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
   auto load = [&](int s) {
     const int seg = s / ksteps;
     const int k0 = (s - seg * ksteps) * BK;
-    const _Float16* Ap = (seg == 2) ? lo : hi;
-    const _Float16* Bp = (seg == 1) ? lo : hi;
-    const _Float16* pa = Ap + (size_t)(I * BM + c) * ld + k0 + kh;
-    const _Float16* pb = Bp + (size_t)(J * BM + c) * ld + k0 + kh;
+    const T* Ap = (seg == 2) ? lo : hi;
+    const T* Bp = (seg == 1) ? lo : hi;
+    const T* pa = Ap + (size_t)(I * BM + c) * ld + k0 + kh;
+    const T* pb = Bp + (size_t)(J * BM + c) * ld + k0 + kh;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       ra[q] = *(const i4*)(pa + 8 * q);
@@ -117,13 +144,13 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
         for (int m = 0; m < 4; ++m)
 #pragma unroll
           for (int n = 0; n < 4; ++n)
-            acc_x[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[n], a[m], acc_x[m][n], 0, 0, 0);
+            acc_x[m][n] = Split<T>::mfma(b[n], a[m], acc_x[m][n]);
       } else {
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
           for (int n = 0; n < 4; ++n)
-            acc_hh[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[n], a[m], acc_hh[m][n], 0, 0, 0);
+            acc_hh[m][n] = Split<T>::mfma(b[n], a[m], acc_hh[m][n]);
       }
     }
   }
@@ -150,7 +177,7 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
         const double we = mirror ? 2.0 : 1.0;
         const size_t idx = row + (size_t)col * ld;
         const size_t tdx = mirror ? col + (size_t)row * ld : idx;
-        const double y = (double)acc_hh[m][n][r] + 0x1p-11 * (double)acc_x[m][n][r];
+        const double y = (double)acc_hh[m][n][r] + (1.0 / kScale) * (double)acc_x[m][n][r];
         const double x = X[idx];
         const double z = fma(cb, x, ca * y);
         const double dxy = x - y;
@@ -159,8 +186,8 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
         q2 += we * dxy * dxy;
         q1 += diag ? z : 0.0;
         q3 += diag ? y : 0.0;
-        const _Float16 h = (_Float16)z;
-        const _Float16 l = (_Float16)((z - (double)h) * 2048.0);
+        const T h = (T)z;
+        const T l = (T)((z - (double)h) * kScale);
         Xn[idx] = z;
         Xn[tdx] = z;
         hin[idx] = h;
@@ -179,22 +206,27 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
 }
 
 // hi/lo planes of an FP64 matrix (whole ld x ld buffer, padding stays zero).
-__global__ void split_f16_kernel(const double* __restrict__ x, _Float16* __restrict__ hi,
-                                 _Float16* __restrict__ lo, size_t count) {
+template <typename T>
+__global__ void split_f16_kernel(const double* __restrict__ x, T* __restrict__ hi,
+                                 T* __restrict__ lo, size_t count) {
   for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
     const double z = x[i];
-    const _Float16 h = (_Float16)z;
+    const T h = (T)z;
     hi[i] = h;
-    lo[i] = (_Float16)((z - (double)h) * 2048.0);
+    lo[i] = (T)((z - (double)h) * Split<T>::kScale);
   }
 }
 
 }  // namespace
 
-int split_f16(const DMat& x, void* hi, void* lo) {
+int split_f16(const DMat& x, void* hi, void* lo, bool bf16) {
   const size_t count = (size_t)x.ld * x.ld;
-  hipLaunchKernelGGL(split_f16_kernel, dim3(2048), dim3(NT), 0, stream(), x.p, (_Float16*)hi,
-                     (_Float16*)lo, count);
+  if (bf16)
+    hipLaunchKernelGGL(split_f16_kernel<__bf16>, dim3(2048), dim3(NT), 0, stream(), x.p,
+                       (__bf16*)hi, (__bf16*)lo, count);
+  else
+    hipLaunchKernelGGL(split_f16_kernel<_Float16>, dim3(2048), dim3(NT), 0, stream(), x.p,
+                       (_Float16*)hi, (_Float16*)lo, count);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
@@ -202,42 +234,47 @@ int split_f16(const DMat& x, void* hi, void* lo) {
 int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
                    int* ntiles, int prec) {
   if (prec == PRG_PREC_INT8CRT) return crt_step(x0, x1, d_state, d_part, ntiles, crt_moduli());
-  if (prec != PRG_PREC_F16SPLIT) {
+  if (prec != PRG_PREC_F16SPLIT && prec != PRG_PREC_BF16SPLIT) {
     set_error(ERR_ARG, "unknown precision mode");
     return ERR_ARG;
   }
+  const bool bf16 = (prec == PRG_PREC_BF16SPLIT);
   const int ld = x0.ld;
-  const size_t plane = sizeof(_Float16) * (size_t)ld * ld;
+  const size_t plane = 2 * (size_t)ld * ld;  // bytes of one 16-bit plane
   int st = OK;
-  char* planes = (char*)workspace("f16_planes", 4 * plane, &st);
+  char* planes = (char*)workspace(bf16 ? "bf16_planes" : "f16_planes", 4 * plane, &st);
   if (st) return st;
   F16Args g;
   g.buf[0] = x0.p;
   g.buf[1] = x1.p;
-  g.hi[0] = (const _Float16*)(planes);
-  g.lo[0] = (const _Float16*)(planes + plane);
-  g.hi[1] = (const _Float16*)(planes + 2 * plane);
-  g.lo[1] = (const _Float16*)(planes + 3 * plane);
+  g.hi[0] = planes;
+  g.lo[0] = planes + plane;
+  g.hi[1] = planes + 2 * plane;
+  g.lo[1] = planes + 3 * plane;
   g.ld = ld;
   g.st = d_state;
   g.part = d_part;
   const int nt = ld / BM;
   const int nb = nt * (nt + 1) / 2;
   if (ntiles) *ntiles = nb;
-  hipLaunchKernelGGL(sp2_f16split_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  if (bf16)
+    hipLaunchKernelGGL(sp2_f16split_kernel<__bf16>, dim3(nb), dim3(NT), 0, stream(), g);
+  else
+    hipLaunchKernelGGL(sp2_f16split_kernel<_Float16>, dim3(nb), dim3(NT), 0, stream(), g);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
 
 int sp2_mixed_prepare(const DMat& x0, int prec) {
   if (prec == PRG_PREC_INT8CRT) return crt_prepare(x0, crt_moduli());
-  if (prec != PRG_PREC_F16SPLIT) return OK;
+  if (prec != PRG_PREC_F16SPLIT && prec != PRG_PREC_BF16SPLIT) return OK;
+  const bool bf16 = (prec == PRG_PREC_BF16SPLIT);
   const int ld = x0.ld;
-  const size_t plane = sizeof(_Float16) * (size_t)ld * ld;
+  const size_t plane = 2 * (size_t)ld * ld;
   int st = OK;
-  char* planes = (char*)workspace("f16_planes", 4 * plane, &st);
+  char* planes = (char*)workspace(bf16 ? "bf16_planes" : "f16_planes", 4 * plane, &st);
   if (st) return st;
-  return split_f16(x0, planes, planes + plane);
+  return split_f16(x0, planes, planes + plane, bf16);
 }
 
 }  // namespace prg

@@ -119,7 +119,9 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   const bool lowp = (p.precision != PRG_PREC_FP64) && !response;
   // FP32-accumulated products stop on their noise floor and are polished in
   // FP64; the int8 CRT products carry X to 2^-42 and keep the FP64 rules.
-  hst->noise_stop = lowp && (p.precision == PRG_PREC_F16SPLIT) && (p.variant == V_ALG2);
+  hst->noise_stop = lowp &&
+                    (p.precision == PRG_PREC_F16SPLIT || p.precision == PRG_PREC_BF16SPLIT) &&
+                    (p.variant == V_ALG2);
   hst->refine_max = hst->noise_stop ? p.refine_max : 0;
   if (p.pp_in)
     for (int i = 0; i < p.icount && i < kMaxIter; ++i) hst->pp[i] = p.pp_in[i];
@@ -162,7 +164,8 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
         }
         if (prec == PRG_PREC_FP64 || response) {
           st = sp2_step(X0, X1, dst, part, &nt_used);
-        } else if (prec == PRG_PREC_F16SPLIT || prec == PRG_PREC_INT8CRT) {
+        } else if (prec == PRG_PREC_F16SPLIT || prec == PRG_PREC_BF16SPLIT ||
+                   prec == PRG_PREC_INT8CRT) {
           st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
         } else {
           set_error(ERR_ARG, "unknown precision mode");
@@ -323,7 +326,8 @@ extern "C" {
 
 void prg_set_precision(int mode) {
   clear_error();
-  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_F16SPLIT && mode != PRG_PREC_INT8CRT) {
+  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_F16SPLIT && mode != PRG_PREC_INT8CRT &&
+      mode != PRG_PREC_BF16SPLIT) {
     set_error(ERR_ARG, "prg_set_precision: unknown mode");
     return;
   }
@@ -560,7 +564,7 @@ int prg_gpu_sp2_alg2_device(int N, const double* dH, double* dRho, double bndfil
     return ERR_ARG;
   }
   if (precision != PRG_PREC_FP64 && precision != PRG_PREC_F16SPLIT &&
-      precision != PRG_PREC_INT8CRT) {
+      precision != PRG_PREC_INT8CRT && precision != PRG_PREC_BF16SPLIT) {
     set_error(ERR_ARG, "prg_gpu_sp2_alg2_device: unknown precision");
     return ERR_ARG;
   }

@@ -21,6 +21,7 @@ LIB_PATH = _HERE.parent / "lib" / "libprogress_gpu.so"
 PREC_FP64 = 0
 PREC_F16SPLIT = 1
 PREC_INT8CRT = 2
+PREC_BF16SPLIT = 3
 
 _dense, _double_real, _sequential, _col_major = 1, 2, 0, 1
 
