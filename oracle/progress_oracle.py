@@ -32,7 +32,9 @@ Parity pins (checked in ``tests/test_oracle.py``):
   x <- x^2 / (2x^2 - 2x + 1) on the eigenvalues, whose limit is the Fermi
   function 1/(1+exp(beta (e-mu))) as the number of steps grows.
 * responses: Daleckii-Krein divided differences of the same scalar maps and
-  central finite differences of the forward solvers.
+  central finite differences of the forward solvers; the order-1..3 implicit
+  Fermi responses as the Taylor coefficients of the scalar map applied to the
+  eigenvalues of H0 + l H1 + l^2 H2 + l^3 H3 (a polynomial fit in l).
 * int8 Chinese-remainder products (``crt_square``, ``crt_product``; no
   reference counterpart): against the FP64 products within the operand
   rounding bound, the scale bound at its edge, and planted mistakes (wrong CRT

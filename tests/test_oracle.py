@@ -263,6 +263,41 @@ def test_first_order_response_divided_differences():
     assert np.allclose(np.abs(c1), np.abs(q.T @ p1 @ q), rtol=1e-6, atol=1e-9)
 
 
+def test_implicit_fermi_response_orders_are_taylor_coefficients():
+    """prg_implicit_fermi_response orders 1-3: along H(l) = H0 + l H1 + l^2 H2
+    + l^3 H3 at the returned mu(l) = mu0 + l mu1 + l^2 mu2 + l^3 mu3, the
+    density is the scalar recursion applied to the eigenvalues of H(l), so
+    P1..P3 are the Taylor coefficients of Q(l) f(w(l)) Q(l)^T.  They are
+    read off a degree-6 fit through l = -3s..3s (s = 0.005; fit error 7e-7
+    relative at order 3), with non-commuting perturbations.  Dropping the
+    cross terms of the higher orders (the response to H2, H3 alone) moves P2
+    and P3 by about 80 %, far outside the 1e-5 bar."""
+    n, nsteps, beta = 48, 6, 10.0
+    h0 = O.model_hamiltonian(n, 3)
+    hs = [0.25 * O.banded_perturbation(n, 4, seed=s) for s in (11, 12, 13)]
+    p0, ps, mu0, mu = O.implicit_fermi_response(h0, hs, nsteps, 0.6, beta, 0.5 * n, 1e-12, 3,
+                                                1e-13)
+    for p in ps:
+        assert abs(np.trace(p)) < 1e-10
+
+    def density(lam):
+        hl = h0 + lam * hs[0] + lam ** 2 * hs[1] + lam ** 3 * hs[2]
+        ml = mu0 + lam * mu[0] + lam ** 2 * mu[1] + lam ** 3 * mu[2]
+        w, q = eig(hl)
+        return (q * O.implicit_fermi_eigen(w, ml, beta, nsteps)) @ q.T
+
+    lams = 0.005 * np.arange(-3, 4)
+    vals = np.stack([density(lam) for lam in lams]).reshape(7, -1)
+    coef = np.linalg.solve(np.vander(lams, 7, increasing=True), vals).reshape(7, n, n)
+    assert np.linalg.norm(coef[0] - p0) < 1e-12
+    for j in range(3):
+        assert np.linalg.norm(coef[j + 1] - ps[j]) / np.linalg.norm(ps[j]) < 1e-5
+    _, ps_x, _, _ = O.implicit_fermi_response(h0, [0.0 * hs[0], hs[1], hs[2]], nsteps, 0.6, beta,
+                                              0.5 * n, 1e-12, 3, 1e-13)
+    for j in (1, 2):
+        assert np.linalg.norm(coef[j + 1] - ps_x[j]) / np.linalg.norm(ps[j]) > 0.5
+
+
 def test_response_sp2_divided_differences():
     h = O.model_hamiltonian(96, seed=1)
     h1 = O.banded_perturbation(96, 8, seed=7)
