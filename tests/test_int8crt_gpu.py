@@ -254,3 +254,42 @@ def test_full_size_n8192_fp64_and_int8_properties(P):
     (it64, pp64, r64), (it8, pp8, r8) = out[P.PREC_FP64], out[P.PREC_INT8CRT]
     assert it8 == it64 == 66 and pp8 == pp64
     assert float(torch.linalg.norm(r8 - r64)) < 1e-6
+
+
+def test_full_size_n8192_implicit_fermi_properties(P):
+    """prg_implicit_fermi at fixed mu (8 levels, beta 20, Newton-Schulz to
+    1e-10) on the N = 8192 model Hamiltonian, with FP64 products and with
+    12-moduli int8 CRT products.  With exact inverses the recursion maps each
+    eigenvalue through the oracle's scalar map, so P must equal
+    Q f(e) Q^T built from a GPU eigendecomposition and the oracle's
+    implicit_fermi_eigen on the 8192 eigenvalues, to 1e-10 (Frobenius) with
+    FP64 products and 1e-8 with int8 products (measured 1.3e-12 and 3.5e-10:
+    the Newton-Schulz stopping point and the operands' rounding to 2^-46 of
+    their scale are the only approximations), exactly symmetric, and the two
+    product precisions within 1e-8 of each other."""
+    import torch
+    n = 8192
+    print(f"[implicit fermi full size] n={n}", flush=True)
+    h = torch.from_numpy(O.model_hamiltonian(n, 0)).cuda()
+    assert P.padded_ld(n) == n
+    e, q = torch.linalg.eigh(h)
+    mu = 0.5 * float(e[n // 2 - 1] + e[n // 2])
+    nsteps, beta = 8, 20.0
+    f = torch.from_numpy(O.implicit_fermi_eigen(e.cpu().numpy(), mu, beta, nsteps)).cuda()
+    ref = (q * f) @ q.T
+    del q
+    out = []
+    for prec in (P.PREC_FP64, P.PREC_INT8CRT):
+        P.set_int8_moduli(12)
+        P.set_product_precision(prec)
+        try:
+            p = torch.zeros_like(h)
+            P.implicit_fermi_device(h, p, n, nsteps, mu, beta, 1e-10)
+        finally:
+            P.set_product_precision(P.PREC_FP64)
+        err = float(torch.linalg.norm(p - ref))
+        print(f"  product precision {prec}: ||P - Q f(e) Q^T||_F = {err:.3e}", flush=True)
+        assert err < (1e-10 if prec == P.PREC_FP64 else 1e-8)
+        assert bool(torch.equal(p, p.T))
+        out.append(p)
+    assert float(torch.linalg.norm(out[0] - out[1])) < 1e-8
