@@ -250,12 +250,25 @@ void prg_implicit_fermi_first_order_response(bml_matrix_t *H0_bml, bml_matrix_t 
                                              bml_matrix_t **Inv_bml, int nsteps, double mu0,
                                              double beta, double nocc, double threshold);
 /* Declared at src/prg_progress_mod.h:240-258 without an implementation there
- * (prg_implicit_fermi_mod.F90:547-712): H_bml[order] = H1..H3, P_bml[order] = P1..P3,
- * *mu0 updated, mu[order] receives the response chemical potentials. */
-void prg_implicit_fermi_response(bml_matrix_t *H0_bml, bml_matrix_t **H_bml, bml_matrix_t *P0_bml,
-                                 bml_matrix_t **P_bml, int nsteps, double *mu0, double *mu,
-                                 double beta, double nocc, double occ_tol, double lin_tol,
-                                 int order, double threshold);
+ * (prg_implicit_fermi_mod.F90:547-712); the declared arity: H1..H3 and P1..P3 as
+ * separate handles (those above `order` may be NULL), mu0 and mu by value (the
+ * Fortran routine zeroes mu(:) on entry, so mu is unused).  The updated
+ * chemical potentials are returned by prg_implicit_fermi_response_mu. */
+void prg_implicit_fermi_response(bml_matrix_t *H0_bml, bml_matrix_t *H1_bml, bml_matrix_t *H2_bml,
+                                 bml_matrix_t *H3_bml, bml_matrix_t *P0_bml, bml_matrix_t *P1_bml,
+                                 bml_matrix_t *P2_bml, bml_matrix_t *P3_bml, int nsteps,
+                                 double mu0, double mu, double beta, double nocc, double occ_tol,
+                                 double lin_tol, int order, double threshold);
+/* mu0 and mu[0..2] after the last prg_implicit_fermi_response call. */
+void prg_implicit_fermi_response_mu(double *mu0, double *mu);
+/* The same routine with handle arrays and pointer out-parameters (what the
+ * Fortran module calls): H_bml[order] = H1..H3, P_bml[order] = P1..P3, *mu0
+ * updated, mu[order] receives the response chemical potentials.  No
+ * counterpart in the reference's C header. */
+void prg_gpu_implicit_fermi_response(bml_matrix_t *H0_bml, bml_matrix_t **H_bml,
+                                     bml_matrix_t *P0_bml, bml_matrix_t **P_bml, int nsteps,
+                                     double *mu0, double *mu, double beta, double nocc,
+                                     double occ_tol, double lin_tol, int order, double threshold);
 
 /* ------------------------------------------------------------------ response */
 /* New C symbol for prg_compute_response_SP2 (prg_response_mod.F90:653-782), Fortran order:
@@ -413,14 +426,21 @@ void prg_pcg(bml_matrix_t *A_bml, bml_matrix_t *p_bml, bml_matrix_t *p2_bml, bml
 void prg_newtonschulz(bml_matrix_t *a_bml, bml_matrix_t *ai_bml, bml_matrix_t *r_bml,
                       bml_matrix_t *tmp_bml, bml_matrix_t *d_bml, bml_matrix_t *I_bml, double tol,
                       double threshold, int *num_iter);
-/* New C symbol for prg_finite_diff (prg_implicit_fermi_mod.F90:726-831):
- * finite-difference responses of prg_get_density_matrix along H0 + sum l^i H_i,
- * mu0 + sum l^i mu_i (i = 1..order, order <= 3) at l = lambda, step h.  The
- * reference prints them (top-left 10 x 10 times 1000); out_list[k] (order
- * k + 1, may be NULL: then the corners are printed) receives each. */
-void prg_finite_diff(bml_matrix_t *H0_bml, bml_matrix_t **H_list, double mu0,
-                     const double *mu_list, double beta, int order, double lambda, double h,
-                     double threshold, bml_matrix_t **out_list);
+/* prg_finite_diff (prg_implicit_fermi_mod.F90:726-831) at the arity and types
+ * of its declaration, src/prg_progress_mod.h:259-268: one perturbation handle
+ * (H_list = H1) and one coefficient (mu_list = mu1) by value, so order must be
+ * 1; the difference is printed as the reference does (top-left 10 x 10 times
+ * 1000).  Orders 2-3: prg_gpu_finite_diff. */
+void prg_finite_diff(bml_matrix_t *H0_bml, bml_matrix_t *H_list, double mu0, double mu_list,
+                     double beta, int order, double lambda, double h, double threshold);
+/* The extended form (what the Fortran module calls): finite-difference
+ * responses of prg_get_density_matrix along H0 + sum l^i H_i, mu0 + sum l^i mu_i
+ * (i = 1..order, order <= 3) at l = lambda, step h; out_list[k] (order k + 1;
+ * out_list may be NULL: then the corners are printed) receives each.  No
+ * counterpart in the reference's C header. */
+void prg_gpu_finite_diff(bml_matrix_t *H0_bml, bml_matrix_t **H_list, double mu0,
+                         const double *mu_list, double beta, int order, double lambda, double h,
+                         double threshold, bml_matrix_t **out_list);
 /* prg_implicit_fermi at a fixed chemical potential (one pass of the level
  * loop of src/prg_implicit_fermi_mod.F90:259-298, method 1: level-1 CG inverse
  * refined by Newton-Schulz to tol at every level) on device pointers:

@@ -132,12 +132,6 @@ def normalize(h: np.ndarray) -> np.ndarray:
     return bml_normalize(h, lo, hi)
 
 
-def normalize_fermi(h, h1, hn, mu):
-    """prg_normalize_fermi (src/prg_normalize_mod.F90:59-72)."""
-    n = h.shape[0]
-    return (-h + (hn - mu) * np.eye(n)) / (hn - h1)
-
-
 def normalize_implicit_fermi(h, cnst, mu):
     """prg_normalize_implicit_fermi (src/prg_normalize_mod.F90:83-94):
     X0 = (0.5 + cnst mu) I - cnst H."""
@@ -1076,43 +1070,78 @@ def read_part(path):
     return name, total_nodes, parts
 
 
-def matrix2submatrix_index(g, core, double_jump=True):
-    """Core+halo index list for one part (the semantics of
-    bml_matrix2submatrix_index as called at src/prg_subgraphloop_mod.F90:60-63;
-    BML is absent so the ordering is parity unpinned): core nodes first in the
-    given order, then every node adjacent in G (nonzero, off-diagonal) to a
-    core node in order of discovery, then with double_jump the neighbours of
-    those halo nodes.  Returns (index_list, core_size)."""
-    seen = set(core)
-    idx = list(core)
+def matrix2submatrix_index(g, core, double_jump=True, h=None):
+    """Core+halo index list for one part: bml_matrix2submatrix_index as the
+    reference calls it (graph only at src/prg_subgraphloop_mod.F90:355-358 and
+    src/prg_graphsolver_mod.F90:112-115; graph plus the Hamiltonian as the
+    trailing optional matrix at src/prg_subgraphloop_mod.F90:60-63).  BML
+    (lanl/bml, master, commit unpinned) is absent; this restates its published
+    dense algorithm:
+
+    1. the core nodes in the given order (repeats dropped);
+    2. for each core node in that order, its unseen neighbours in G
+       (nonzero off-diagonal entries of its row) in ascending index order;
+    3. when h is given, the same walk over the rows of H (H's sparsity
+       adds halo nodes the graph misses);
+    4. with double_jump, the unseen G-neighbours of every node listed so far,
+       in list order (a symbolic X^2 over the graph).
+
+    So the set is S1 u N_G(S1) with S1 = C u N_G(C) u N_H(C), i.e. the
+    nonzero columns of the core rows of (I + |G| + |H|)(I + |G|)
+    (``halo_set_bruteforce``).  Returns (index_list, core_size)."""
+    seen = set()
+    idx = []
     for c in core:
-        for k in np.nonzero(g[c])[0]:
-            k = int(k)
-            if k not in seen:
-                seen.add(k)
-                idx.append(k)
-    if double_jump:
-        first_halo = idx[len(core):]
-        for c in first_halo:
-            for k in np.nonzero(g[c])[0]:
+        c = int(c)
+        if c not in seen:
+            seen.add(c)
+            idx.append(c)
+    ncore = len(idx)
+
+    def walk(m, rows):
+        for c in rows:
+            for k in np.nonzero(m[c])[0]:
                 k = int(k)
                 if k not in seen:
                     seen.add(k)
                     idx.append(k)
-    return idx, len(core)
+
+    walk(g, list(idx))
+    if h is not None:
+        walk(h, idx[:ncore])
+    if double_jump:
+        walk(g, idx[ncore:])
+    return idx, ncore
 
 
-def subgraph_sp2_loop(h, g, parts, pp, max_iter, mineval, maxeval, double_jump=True):
+def halo_set_bruteforce(g, core, double_jump=True, h=None):
+    """The core+halo set of ``matrix2submatrix_index`` by symbolic products:
+    the nonzero columns of the core rows of (I + |G| + |H|) (I + |G|) with
+    double jump, of (I + |G| + |H|) without.  Ordering-free check of the set."""
+    n = g.shape[0]
+    pg = (np.abs(g) > 0).astype(np.int64)
+    first = np.eye(n, dtype=np.int64) + pg
+    if h is not None:
+        first = first + (np.abs(h) > 0).astype(np.int64)
+    row = first[list(core)].sum(axis=0) > 0
+    if double_jump:
+        row = (row.astype(np.int64) @ (np.eye(n, dtype=np.int64) + pg)) > 0
+    return set(int(k) for k in np.nonzero(row)[0])
+
+
+def subgraph_sp2_loop(h, g, parts, pp, max_iter, mineval, maxeval, double_jump=True,
+                      halo_from_h=True):
     """prg_subgraphSP2Loop (src/prg_subgraphloop_mod.F90:36-126) on one rank:
-    per part extract the core+halo block of H, run prg_sp2_submatrix_inplace,
-    scatter the core rows (and the matching columns, the density being
-    symmetric) back, then prg_fnormGraph (sqrt of the accumulated vv).
-    Returns (rho, vv)."""
+    per part extract the core+halo block of H (halos from G and, as the
+    reference passes h_bml to bml_matrix2submatrix_index, from H), run
+    prg_sp2_submatrix_inplace, scatter the core rows (and the matching
+    columns, the density being symmetric) back, then prg_fnormGraph (sqrt of
+    the accumulated vv).  Returns (rho, vv)."""
     n = h.shape[0]
     rho = np.zeros((n, n))
     vv = [0.0] * max_iter
     for core in parts:
-        idx, llsize = matrix2submatrix_index(g, core, double_jump)
+        idx, llsize = matrix2submatrix_index(g, core, double_jump, h if halo_from_h else None)
         sub = h[np.ix_(idx, idx)]
         d, vv = sp2_submatrix(sub, pp, max_iter, vv, mineval, maxeval, llsize)
         cidx = idx[:llsize]

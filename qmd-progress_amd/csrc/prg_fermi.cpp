@@ -153,6 +153,7 @@ namespace {
 
 int g_last_occiter = 0, g_last_totns = 0;
 double g_last_mu = 0.0;
+double g_resp_mu0 = 0.0, g_resp_mu[3] = {0.0, 0.0, 0.0};
 
 // P0 = (0.5 + cnst mu) I - cnst H   (prg_normalize_implicit_fermi)
 int normalize_if(const DMat& H, double cnst, double mu, DMat P) {
@@ -617,7 +618,7 @@ void prg_newtonschulz(bml_matrix_t* a_bml, bml_matrix_t* ai_bml, bml_matrix_t* r
   if (num_iter) *num_iter = it;
 }
 
-// New C symbol for prg_finite_diff (src/prg_implicit_fermi_mod.F90:726-831):
+// prg_finite_diff (src/prg_implicit_fermi_mod.F90:726-831), extended form:
 // finite-difference response of prg_get_density_matrix along the series
 // H(l) = H0 + sum_i l^i H_i, mu(l) = mu0 + sum_i l^i mu_i at l = lambda with
 // step h: order 1 the forward difference (D(l + h) - D(l)) / h, order 2
@@ -625,9 +626,9 @@ void prg_newtonschulz(bml_matrix_t* a_bml, bml_matrix_t* ai_bml, bml_matrix_t* r
 // (12 h^3) - (D(l + h) - D(l - h)) / (6 h^3).  The reference only prints them
 // (top-left 10 x 10 times 1000); here out_list[k] (order k + 1) receives each
 // when out_list is given, and with out_list NULL the same corners are printed.
-void prg_finite_diff(bml_matrix_t* H0_bml, bml_matrix_t** H_list, double mu0,
-                     const double* mu_list, double beta, int order, double lambda, double h,
-                     double threshold, bml_matrix_t** out_list) {
+void prg_gpu_finite_diff(bml_matrix_t* H0_bml, bml_matrix_t** H_list, double mu0,
+                         const double* mu_list, double beta, int order, double lambda, double h,
+                         double threshold, bml_matrix_t** out_list) {
   (void)threshold;
   clear_error();
   if (!H0_bml || !H_list || !mu_list || order < 1 || order > 3 || !(h != 0.0)) {
@@ -697,6 +698,25 @@ void prg_finite_diff(bml_matrix_t* H0_bml, bml_matrix_t** H_list, double mu0,
       return;
   }
   if (hipStreamSynchronize(stream()) != hipSuccess) set_error(ERR_HIP, "prg_finite_diff");
+}
+
+// prg_finite_diff at the arity and types of its declaration in the
+// reference's C header (src/prg_progress_mod.h:259-268): one perturbation
+// handle H_list (= H1) and one chemical-potential coefficient mu_list (= mu1)
+// by value, so only order 1 can be carried; the difference is printed as the
+// reference does (top-left 10 x 10 times 1000).  Orders 2 and 3 need the
+// lists of prg_gpu_finite_diff (an argument error here).
+void prg_finite_diff(bml_matrix_t* H0_bml, bml_matrix_t* H_list, double mu0, double mu_list,
+                     double beta, int order, double lambda, double h, double threshold) {
+  clear_error();
+  if (order != 1 || !H_list) {
+    set_error(ERR_ARG, "prg_finite_diff: the reference C prototype carries order 1 only "
+                       "(use prg_gpu_finite_diff for orders 2-3)");
+    return;
+  }
+  bml_matrix_t* hl[1] = {H_list};
+  const double ml[1] = {mu_list};
+  prg_gpu_finite_diff(H0_bml, hl, mu0, ml, beta, 1, lambda, h, threshold, nullptr);
 }
 
 // prg_implicit_fermi at a fixed mu (one pass of src/prg_implicit_fermi_mod.F90:
@@ -820,14 +840,15 @@ void prg_implicit_fermi_first_order_response(bml_matrix_t* H0_bml, bml_matrix_t*
   implicit_fermi_first_order_response(H0, H1, P0, P1, inv, nsteps, mu0, beta);
 }
 
-// Declared (src/prg_progress_mod.h:240-258) but not implemented by the
-// reference's C interface.  Here: H_bml[0..order-1] are H1..H3, P_bml the
-// outputs P1..P3; mu0 is updated in place and mu[0..order-1] receives the
-// response chemical potentials.
-void prg_implicit_fermi_response(bml_matrix_t* H0_bml, bml_matrix_t** H_bml, bml_matrix_t* P0_bml,
-                                 bml_matrix_t** P_bml, int nsteps, double* mu0, double* mu,
-                                 double beta, double nocc, double occ_tol, double lin_tol,
-                                 int order, double threshold) {
+// prg_implicit_fermi_response (src/prg_implicit_fermi_mod.F90:547-712), the
+// extended form: H_bml[0..order-1] are H1..H3, P_bml the outputs P1..P3; *mu0
+// is updated in place and mu[0..order-1] receives the response chemical
+// potentials.
+void prg_gpu_implicit_fermi_response(bml_matrix_t* H0_bml, bml_matrix_t** H_bml,
+                                     bml_matrix_t* P0_bml, bml_matrix_t** P_bml, int nsteps,
+                                     double* mu0, double* mu, double beta, double nocc,
+                                     double occ_tol, double lin_tol, int order,
+                                     double threshold) {
   (void)threshold;
   clear_error();
   if (!mu0 || !mu || !H_bml || !P_bml || order < 1 || order > 3) {
@@ -843,6 +864,46 @@ void prg_implicit_fermi_response(bml_matrix_t* H0_bml, bml_matrix_t** H_bml, bml
     if (bml_dev_read(H_bml[j], &Hs[j]) || bml_dev_write(P_bml[j], &Ps[j])) return;
   }
   implicit_fermi_response(H0, Hs, P0, Ps, nsteps, mu0, mu, beta, nocc, occ_tol, lin_tol, order);
+}
+
+// Declared at src/prg_progress_mod.h:240-258 (no implementation in the
+// reference's C interface): the header's arity, H1..H3 and P1..P3 as separate
+// handles (H2, H3, P2, P3 may be NULL above `order`), mu0 and mu by value as
+// declared.  The Fortran routine sets mu(:) to zero on entry, so the input mu
+// is unused; the updated chemical potentials are returned by
+// prg_implicit_fermi_response_mu().
+void prg_implicit_fermi_response(bml_matrix_t* H0_bml, bml_matrix_t* H1_bml, bml_matrix_t* H2_bml,
+                                 bml_matrix_t* H3_bml, bml_matrix_t* P0_bml, bml_matrix_t* P1_bml,
+                                 bml_matrix_t* P2_bml, bml_matrix_t* P3_bml, int nsteps, double mu0,
+                                 double mu, double beta, double nocc, double occ_tol,
+                                 double lin_tol, int order, double threshold) {
+  (void)mu;
+  clear_error();
+  bml_matrix_t* hs[3] = {H1_bml, H2_bml, H3_bml};
+  bml_matrix_t* ps[3] = {P1_bml, P2_bml, P3_bml};
+  if (order < 1 || order > 3) {
+    set_error(ERR_ARG, "prg_implicit_fermi_response: order must be 1..3");
+    return;
+  }
+  for (int j = 0; j < order; ++j)
+    if (!hs[j] || !ps[j]) {
+      set_error(ERR_ARG, "prg_implicit_fermi_response: missing H or P handle below order");
+      return;
+    }
+  double m0 = mu0, mr[3] = {0.0, 0.0, 0.0};
+  prg_gpu_implicit_fermi_response(H0_bml, hs, P0_bml, ps, nsteps, &m0, mr, beta, nocc, occ_tol,
+                                  lin_tol, order, threshold);
+  if (last_error_code() != OK) return;
+  g_resp_mu0 = m0;
+  for (int j = 0; j < 3; ++j) g_resp_mu[j] = mr[j];
+}
+
+// Chemical potentials after the last prg_implicit_fermi_response call (the
+// declared C prototype passes mu0 and mu by value): *mu0 and mu[0..2].
+void prg_implicit_fermi_response_mu(double* mu0, double* mu) {
+  if (mu0) *mu0 = g_resp_mu0;
+  if (mu)
+    for (int j = 0; j < 3; ++j) mu[j] = g_resp_mu[j];
 }
 
 }  // extern "C"

@@ -315,6 +315,9 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
   const volatile Sp2State* stv = g.st;
   if (!stv->planned) return;
   const int apply = stv->apply, sigma = stv->sigma, cur = stv->cur;
+  // prg_sp2_submatrix: the idempotency sum runs over the first `core` rows
+  // (and, by symmetry, the mirrored columns) only
+  const int core_lim = (stv->variant == V_SUBMATRIX) ? stv->core : 0x7fffffff;
   const int ld = g.ld, pld = g.pld;
   int I, J;
   tile::sym_tile(blockIdx.x, I, J);
@@ -397,8 +400,10 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
       const double z = fma(cb, x, ca * y);
       const double dxy = x - y;
       const bool diag = (row == col0 + r);
+      // weight 1 + mirror for the whole matrix; core rows only for submatrix
+      const double wi = (row < core_lim ? 1.0 : 0.0) + ((mirror && col0 + r < core_lim) ? 1.0 : 0.0);
       q0 += z * z;
-      q2 += dxy * dxy;
+      q2 += wi * dxy * dxy;
       q1 += diag ? z : 0.0;
       q3 += diag ? y : 0.0;
       z4[r] = z;
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
   }
   // block sums (fixed order); off-diagonal blocks stand for their mirror too
   const double wz = mirror ? 2.0 : 1.0;
-  double sums[4] = {wz * q0, q1, wz * q2, q3};
+  double sums[4] = {wz * q0, q1, q2, q3};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     double x = sums[q];
@@ -922,10 +927,13 @@ int group_size() {
 }
 
 const int4* item_list(int nt, int np, int* st) {
+  // the upload is kept while the workspace buffer stays the same one
   static int cached_nt = 0, cached_np = 0;
+  static unsigned cached_gen = 0;
   const int nb = nt * (nt + 1) / 2;
   int4* d = (int4*)workspace("crt_items", sizeof(int4) * (size_t)nb * np, st);
-  if (*st || (cached_nt == nt && cached_np == np)) return d;
+  const unsigned gen = workspace_generation();
+  if (*st || (cached_nt == nt && cached_np == np && cached_gen == gen)) return d;
   const int G = group_size();
   std::vector<int2> tiles;
   tiles.reserve(nb);
@@ -955,6 +963,7 @@ const int4* item_list(int nt, int np, int* st) {
   }
   cached_nt = nt;
   cached_np = np;
+  cached_gen = gen;
   return d;
 }
 
@@ -962,8 +971,10 @@ const int4* item_list(int nt, int np, int* st) {
 // time (refreshed when the number of moduli changes).
 double* fold_constants(int np, int* st) {
   static int cached = 0;
+  static unsigned cached_gen = 0;
   double* d = (double*)workspace("crt_fold", sizeof(double) * 3 * kMaxModuli, st);
-  if (*st || cached == np) return d;
+  const unsigned gen = workspace_generation();
+  if (*st || (cached == np && cached_gen == gen)) return d;
   const CrtConst c = make_const(np);
   double h[3 * kMaxModuli] = {};
   for (int k = 0; k < np; ++k) {
@@ -978,6 +989,7 @@ double* fold_constants(int np, int* st) {
     return nullptr;
   }
   cached = np;
+  cached_gen = gen;
   return d;
 }
 

@@ -162,6 +162,9 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
   // (c, r) accumulate in different orders), so X_next is exactly symmetric.
   const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
   const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
+  // prg_sp2_submatrix: the idempotency sum runs over the first `core` rows
+  // (and, by symmetry, the mirrored columns) only
+  const int core_lim = (g.st->variant == V_SUBMATRIX) ? g.st->core : 0x7fffffff;
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -182,8 +185,9 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
         const double z = fma(cb, x, ca * y);
         const double dxy = x - y;
         const bool diag = (row == col);
+        const double wc = (row < core_lim ? 1.0 : 0.0) + ((mirror && col < core_lim) ? 1.0 : 0.0);
         q0 += we * z * z;
-        q2 += we * dxy * dxy;
+        q2 += wc * dxy * dxy;
         q1 += diag ? z : 0.0;
         q3 += diag ? y : 0.0;
         const T h = (T)z;

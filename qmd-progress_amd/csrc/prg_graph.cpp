@@ -61,17 +61,22 @@ float g_ms[3] = {0.f, 0.f, 0.f};
 
 }  // namespace
 
-// bml_matrix2submatrix_index as called at src/prg_subgraphloop_mod.F90:60-63:
-// core nodes first (given order), then every node adjacent in G (nonzero,
-// off-diagonal entry of the core node's row) in order of discovery, then with
-// the double-jump flag the neighbours of those halo nodes.  The lists are
-// built on the GPU from G's device copy (halo_indices_device).
-int halo_indices(bml_matrix_t* g_bml, prg_graph_partitioning_t* gp, int djflag) {
+// bml_matrix2submatrix_index (lanl/bml; restated, BML is absent): core nodes
+// first (given order), then every node adjacent in G (nonzero, off-diagonal
+// entry of the core node's row) in order of discovery; with h_bml -- passed by
+// the subgraph SP2 loop, src/prg_subgraphloop_mod.F90:60-63 -- the core
+// nodes' neighbours in H next; then with the double-jump flag the graph
+// neighbours of those halo nodes.  Graph-only callers:
+// src/prg_subgraphloop_mod.F90:355-358, src/prg_graphsolver_mod.F90:112-115.
+// The lists are built on the GPU from the device copies (halo_indices_device).
+int halo_indices(bml_matrix_t* g_bml, prg_graph_partitioning_t* gp, int djflag,
+                 bml_matrix_t* h_bml = nullptr) {
   int st = ensure_device();
   if (st) return st;
-  DMat G;
+  DMat G, H;
   if ((st = bml_dev_read(g_bml, &G))) return st;
-  return halo_indices_device(G, gp, djflag);
+  if (h_bml && (st = bml_dev_read(h_bml, &H))) return st;
+  return halo_indices_device(G, gp, djflag, h_bml ? &H : nullptr);
 }
 
 // Packed batch of all parts: largest padded size first, each part m x m with
@@ -285,7 +290,8 @@ int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int i
                                (uint64_t)plan.ntiles,             (uint64_t)plan.wide,
                                (uint64_t)plan.depth,              (uint64_t)plan.grid,
                                (uint64_t)plan.kslab,
-                               (uint64_t)iters};
+                               (uint64_t)iters,
+                               (uint64_t)workspace_generation()};
   for (int it = 0; it < iters; ++it) key.push_back(pp[it] != 0);
   LoopGraph& lg = loop_graph();
   if (lg.exec && lg.key == key) {
@@ -707,8 +713,8 @@ void prg_fnormGraph(prg_graph_partitioning_t* gp) {
   for (int i = 0; i < gp->maxIter && i < 100; ++i) gp->vv[i] = std::sqrt(gp->vv[i]);
 }
 
-// src/prg_subgraphloop_mod.F90:36-126 on one GPU: core+halo extraction with
-// the double-jump flag set (as the reference calls it), lock-step SP2 of all
+// src/prg_subgraphloop_mod.F90:36-126 on one GPU: core+halo extraction from
+// G and H with the double-jump flag set (as the reference calls it), lock-step SP2 of all
 // parts with gp->pp / gp->maxIter and the (mineval, maxeval) normalisation,
 // core rows scattered into rho, vv accumulated then square-rooted by
 // prg_fnormGraph.  The threshold is not applied (dense storage).
@@ -719,7 +725,7 @@ void prg_subgraphSP2Loop(bml_matrix_t* h_bml, bml_matrix_t* g_bml, bml_matrix_t*
   if (!gp || check_pair(h_bml, g_bml, "prg_subgraphSP2Loop") ||
       check_pair(h_bml, rho_bml, "prg_subgraphSP2Loop"))
     return;
-  if (halo_indices(g_bml, gp, 1)) return;
+  if (halo_indices(g_bml, gp, 1, h_bml)) return;
   DMat H, R;
   if (bml_dev_read(h_bml, &H) || bml_dev_write(rho_bml, &R)) return;
   if (subgraph_sp2_loop(H, R, gp)) return;

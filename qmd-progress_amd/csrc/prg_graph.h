@@ -30,9 +30,11 @@ struct BatchPlan {
 };
 
 // Core+halo lists of every part from the device graph G (nonzero off-diagonal
-// entries are edges), in the reference's discovery order; fills
-// core_halo_index, lsize and llsize of gp (prg_halo.hip).
-int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag);
+// entries are edges), in the reference's discovery order; with H (may be
+// null) the core nodes' neighbours in H join the halo before the double jump.
+// Fills core_halo_index, lsize and llsize of gp (prg_halo.hip).
+int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag,
+                        const DMat* H = nullptr);
 
 int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double mineval,
                    double maxeval);

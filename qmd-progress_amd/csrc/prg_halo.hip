@@ -7,8 +7,9 @@
 // i != j.  One workgroup per part then walks the reference's discovery order
 // with a W-word "seen" bitmap in LDS: the core nodes in their given order
 // (duplicates dropped), then for every core node in list order its unseen
-// neighbours in ascending index order, then (double jump) the same for every
-// first-shell halo node.  Each step is one pass over the node's W words, a
+// neighbours in ascending index order, then -- when the Hamiltonian is given,
+// as prg_subgraphSP2Loop passes it -- the core nodes' unseen neighbours in H,
+// then (double jump) the unseen graph neighbours of every halo node so far.  Each step is one pass over the node's W words, a
 // workgroup prefix sum of the new bits, and an ordered write, so the lists
 // are exactly those of the sequential loop.
 #include "prg_graph.h"
@@ -77,7 +78,8 @@ __device__ int block_exclusive_scan(int v, int* total) {
 
 // One part per workgroup.  out: the part's list (capacity cap); cnt[2p] =
 // list length, cnt[2p + 1] = core size after dropping repeated core nodes.
-__global__ __launch_bounds__(NT) void halo_lists_kernel(const uint32_t* __restrict__ bits, int W,
+__global__ __launch_bounds__(NT) void halo_lists_kernel(const uint32_t* __restrict__ bits,
+                                                        const uint32_t* __restrict__ hbits, int W,
                                                         const int* __restrict__ core,
                                                         const int* __restrict__ core_off,
                                                         int djflag, int cap,
@@ -110,10 +112,10 @@ __global__ __launch_bounds__(NT) void halo_lists_kernel(const uint32_t* __restri
   const int ncore = s_n;
   int n = ncore;
   // expand: for each node of list positions [from, to) in order, append its
-  // unseen neighbours in ascending order
-  auto expand = [&](int from, int to) {
+  // unseen neighbours (bit matrix m) in ascending order
+  auto expand = [&](const uint32_t* m, int from, int to) {
     for (int k = from; k < to; ++k) {
-      const uint32_t* row = bits + (size_t)out[k] * W;
+      const uint32_t* row = m + (size_t)out[k] * W;
       uint32_t fresh[8];  // per <= 8 for W <= 2048; larger W loops below
       int c = 0;
       if (per <= 8) {
@@ -160,8 +162,9 @@ __global__ __launch_bounds__(NT) void halo_lists_kernel(const uint32_t* __restri
       __syncthreads();
     }
   };
-  expand(0, ncore);
-  if (djflag) expand(ncore, n);
+  expand(bits, 0, ncore);
+  if (hbits) expand(hbits, 0, ncore);
+  if (djflag) expand(bits, ncore, n);
   if (threadIdx.x == 0) {
     cnt[2 * p] = n;
     cnt[2 * p + 1] = ncore;
@@ -178,7 +181,7 @@ __global__ void pack_lists_kernel(const int* __restrict__ lists, int cap,
 
 }  // namespace
 
-int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag) {
+int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag, const DMat* H) {
   const int n = G.n;
   const int np = gp->totalParts;
   if (np <= 0 || !gp->nnodesInPart || !gp->nodeInPart) {
@@ -208,7 +211,13 @@ int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag)
     core_off[p + 1] = (int)core.size();
   }
   int st = OK;
+  if (H && H->n != n) {
+    set_error(ERR_ARG, "core+halo extraction: graph and Hamiltonian sizes differ");
+    return ERR_ARG;
+  }
   uint32_t* bits = (uint32_t*)workspace("halo_bits", sizeof(uint32_t) * (size_t)n * W, &st);
+  uint32_t* hbits =
+      H ? (uint32_t*)workspace("halo_hbits", sizeof(uint32_t) * (size_t)n * W, &st) : nullptr;
   int* d_core = (int*)workspace("halo_core", sizeof(int) * std::max<size_t>(1, core.size()), &st);
   int* d_coff = (int*)workspace("halo_core_off", sizeof(int) * (np + 1), &st);
   int* d_lists = (int*)workspace("halo_lists", sizeof(int) * (size_t)np * n, &st);
@@ -222,8 +231,13 @@ int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag)
   hipLaunchKernelGGL(graph_bits_kernel, dim3((n + 63) / 64, (W + 63) / 64), dim3(NT), 0, stream(),
                      G.p, G.ld, n, W, bits);
   PRG_HIP_CHECK(hipGetLastError());
+  if (H) {
+    hipLaunchKernelGGL(graph_bits_kernel, dim3((n + 63) / 64, (W + 63) / 64), dim3(NT), 0,
+                       stream(), H->p, H->ld, n, W, hbits);
+    PRG_HIP_CHECK(hipGetLastError());
+  }
   hipLaunchKernelGGL(halo_lists_kernel, dim3(np), dim3(NT), sizeof(uint32_t) * W, stream(), bits,
-                     W, d_core, d_coff, djflag ? 1 : 0, n, d_lists, d_cnt);
+                     hbits, W, d_core, d_coff, djflag ? 1 : 0, n, d_lists, d_cnt);
   PRG_HIP_CHECK(hipGetLastError());
   std::vector<int> cnt(2 * (size_t)np);
   PRG_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int) * 2 * np, hipMemcpyDeviceToHost,

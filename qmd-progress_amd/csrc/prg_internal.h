@@ -59,6 +59,10 @@ int ensure_device();  // initialises the device once; returns Status
 // Scratch buffer cache keyed by name; grows on demand and is reused across
 // calls on the library stream (calls are serialised on that stream).
 void* workspace(const char* name, size_t bytes, int* status);
+// Incremented whenever a workspace buffer is freed (growth or
+// release_workspaces).  Host-side caches of what was uploaded into a
+// workspace record it and upload again once it has changed.
+unsigned workspace_generation();
 // Pinned host staging buffer for small device->host result copies.
 void* host_staging(size_t bytes, int* status);
 void release_workspaces();

@@ -20,6 +20,7 @@ struct Runtime {
   int init_status = OK;
   hipStream_t stream = nullptr;
   std::map<std::string, std::pair<void*, size_t>> ws;
+  unsigned generation = 1;
   void* host_stage = nullptr;
   size_t host_stage_bytes = 0;
 };
@@ -77,6 +78,7 @@ void* workspace(const char* name, size_t bytes, int* status) {
     hipStreamSynchronize(r.stream);
     hipFree(it->second.first);
     r.ws.erase(it);
+    ++r.generation;
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
@@ -88,6 +90,12 @@ void* workspace(const char* name, size_t bytes, int* status) {
   hipMemsetAsync(p, 0, bytes ? bytes : 256, r.stream);
   r.ws[name] = {p, bytes};
   return p;
+}
+
+unsigned workspace_generation() {
+  Runtime& r = rt();
+  std::lock_guard<std::mutex> lk(r.mu);
+  return r.generation;
 }
 
 void* host_staging(size_t bytes, int* status) {
@@ -116,6 +124,7 @@ void release_workspaces() {
   hipStreamSynchronize(r.stream);
   for (auto& kv : r.ws) hipFree(kv.second.first);
   r.ws.clear();
+  ++r.generation;
   if (r.host_stage) hipHostFree(r.host_stage);
   r.host_stage = nullptr;
   r.host_stage_bytes = 0;

@@ -645,7 +645,8 @@ module prg_implicit_fermi_mod
   private
   integer, parameter :: dp = kind(1.0d0)
   public :: prg_implicit_fermi, prg_implicit_fermi_save_inverse, prg_test_density_matrix
-  public :: prg_implicit_fermi_first_order_response
+  public :: prg_implicit_fermi_first_order_response, prg_implicit_fermi_zero
+  public :: prg_implicit_fermi_response
   public :: prg_conjgrad, prg_newtonschulz, prg_pcg, prg_finite_diff
   interface
     subroutine c_cg(a, p, p2, t, d, w, tol, thr) bind(C, name="prg_conjgrad")
@@ -665,7 +666,7 @@ module prg_implicit_fermi_mod
       integer(c_int) :: it
     end subroutine
     subroutine c_fd(h0, hl, mu0, mul, beta, order, lam, h, thr, outl) &
-         bind(C, name="prg_finite_diff")
+         bind(C, name="prg_gpu_finite_diff")
       import :: c_ptr, c_double, c_int
       type(c_ptr), value :: h0, outl
       type(c_ptr) :: hl(*)
@@ -707,6 +708,21 @@ module prg_implicit_fermi_mod
       import :: c_double
       real(c_double) :: r
     end function
+    subroutine c_ifz(h, p, ns, mu, method, thr, tol) bind(C, name="prg_implicit_fermi_zero")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h, p
+      integer(c_int), value :: ns, method
+      real(c_double), value :: mu, thr, tol
+    end subroutine
+    subroutine c_ifr(h0, hl, p0, pl, ns, mu0, mu, beta, nocc, otol, ltol, order, thr) &
+         bind(C, name="prg_gpu_implicit_fermi_response")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: h0, p0
+      type(c_ptr) :: hl(*), pl(*)
+      integer(c_int), value :: ns, order
+      real(c_double) :: mu0, mu(*)
+      real(c_double), value :: beta, nocc, otol, ltol, thr
+    end subroutine
     subroutine c_for(h0, h1, p0, p1, inv, ns, mu0, beta, nocc, thr) &
          bind(C, name="prg_implicit_fermi_first_order_response")
       import :: c_ptr, c_double, c_int
@@ -773,6 +789,42 @@ contains
     call c_for(H0_bml%ptr, H1_bml%ptr, P0_bml%ptr, P1_bml%ptr, h, nsteps, mu0, beta, nocc, &
          threshold)
   end subroutine prg_implicit_fermi_first_order_response
+  !> src/prg_implicit_fermi_mod.F90:347-425 (tol optional, as there)
+  subroutine prg_implicit_fermi_zero(h_bml, p_bml, nsteps, mu, method, threshold, tol)
+    type(bml_matrix_t), intent(in) :: h_bml
+    type(bml_matrix_t), intent(inout) :: p_bml
+    integer, intent(in) :: nsteps, method
+    real(dp), intent(in) :: mu, threshold
+    real(dp), intent(inout), optional :: tol
+    real(dp) :: t
+    t = 1.0e-10_dp
+    if (present(tol)) t = tol
+    call c_ifz(h_bml%ptr, p_bml%ptr, nsteps, mu, method, threshold, t)
+  end subroutine prg_implicit_fermi_zero
+  !> src/prg_implicit_fermi_mod.F90:547-712: H2/H3, P2/P3 are read only up to
+  !> `order`; mu0 and mu(1:order) are updated
+  subroutine prg_implicit_fermi_response(H0_bml, H1_bml, H2_bml, H3_bml, P0_bml, P1_bml, &
+       P2_bml, P3_bml, nsteps, mu0, mu, beta, nocc, occ_tol, lin_tol, order, threshold)
+    type(bml_matrix_t), intent(in) :: H0_bml, H1_bml, H2_bml, H3_bml
+    type(bml_matrix_t), intent(inout) :: P0_bml, P1_bml, P2_bml, P3_bml
+    real(dp), intent(inout) :: mu0
+    real(dp), allocatable, intent(inout) :: mu(:)
+    real(dp), intent(in) :: beta, occ_tol, lin_tol, nocc
+    integer, intent(in) :: nsteps
+    integer :: order
+    real(dp) :: threshold
+    type(c_ptr) :: hl(3), pl(3)
+    real(c_double) :: m0, mr(3)
+    hl = (/ H1_bml%ptr, H2_bml%ptr, H3_bml%ptr /)
+    pl = (/ P1_bml%ptr, P2_bml%ptr, P3_bml%ptr /)
+    m0 = mu0
+    mr = 0.0_dp
+    call c_ifr(H0_bml%ptr, hl, P0_bml%ptr, pl, nsteps, m0, mr, beta, nocc, occ_tol, lin_tol, &
+         order, threshold)
+    mu0 = m0
+    if (.not. allocated(mu)) allocate(mu(order))
+    mu(1:order) = mr(1:order)
+  end subroutine prg_implicit_fermi_response
   !> src/prg_implicit_fermi_mod.F90:1006-1048
   subroutine prg_conjgrad(A_bml, p_bml, p2_bml, tmp_bml, d_bml, w_bml, cg_tol, threshold)
     type(bml_matrix_t), intent(in) :: A_bml, p2_bml

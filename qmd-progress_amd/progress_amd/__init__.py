@@ -136,13 +136,17 @@ def lib():
         "prg_implicit_fermi_zero": (None, [vp, vp, i, d, i, d, d]),
         "prg_implicit_fermi_first_order_response": (None, [vp, vp, vp, vp, C.POINTER(vp), i, d,
                                                            d, d, d]),
-        "prg_implicit_fermi_response": (None, [vp, C.POINTER(vp), vp, C.POINTER(vp), i, dp, dp,
-                                               d, d, d, d, i, d]),
+        "prg_implicit_fermi_response": (None, [vp, vp, vp, vp, vp, vp, vp, vp, i, d, d, d, d,
+                                               d, d, i, d]),
+        "prg_implicit_fermi_response_mu": (None, [dp, dp]),
+        "prg_gpu_implicit_fermi_response": (None, [vp, C.POINTER(vp), vp, C.POINTER(vp), i, dp,
+                                                   dp, d, d, d, d, i, d]),
         "prg_compute_response_SP2": (None, [vp, vp, vp, vp, d, d, i, i, cp, d, d, i]),
         "prg_canon_response": (None, [vp, vp, d, d, dp, d, i, i]),
         "prg_canon_response_orig": (None, [vp, vp, d, d, dp, d, i, d, i]),
         "prg_pcg": (None, [vp, vp, vp, vp, vp, d, d]),
-        "prg_finite_diff": (None, [vp, C.POINTER(vp), d, dp, d, i, d, d, d, C.POINTER(vp)]),
+        "prg_finite_diff": (None, [vp, vp, d, d, d, i, d, d, d]),
+        "prg_gpu_finite_diff": (None, [vp, C.POINTER(vp), d, dp, d, i, d, d, d, C.POINTER(vp)]),
         "prg_conjgrad": (None, [vp, vp, vp, vp, vp, vp, d, d]),
         "prg_newtonschulz": (None, [vp, vp, vp, vp, vp, vp, d, d, ip]),
         "prg_canon_response_p1_dpdmu": (None, [vp, dp, vp, i, d, vp, dp, d, i, i]),
@@ -598,11 +602,32 @@ def implicit_fermi_response(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, lin
     ps = [Matrix.zeros(n) for _ in range(order)]
     mu0_c = C.c_double(mu0)
     mu = _dvec(order)
-    lib().prg_implicit_fermi_response(m0.h, _handles(mh), p0.h, _handles(ps), nsteps,
-                                      C.byref(mu0_c), _dp(mu), beta, nocc, occ_tol, lin_tol,
-                                      order, 0.0)
+    lib().prg_gpu_implicit_fermi_response(m0.h, _handles(mh), p0.h, _handles(ps), nsteps,
+                                          C.byref(mu0_c), _dp(mu), beta, nocc, occ_tol, lin_tol,
+                                          order, 0.0)
     _check()
     return p0.numpy(), [p.numpy() for p in ps], mu0_c.value, mu.tolist()
+
+
+def implicit_fermi_response_c(h0, hs, nsteps, mu0, beta, nocc, occ_tol, order, lin_tol=1e-10):
+    """The same through the reference header's arity (prg_implicit_fermi_response
+    with separate H1..H3 / P1..P3 handles and mu0 / mu by value), the updated
+    chemical potentials read back with prg_implicit_fermi_response_mu.
+    Returns (P0, [P1..], mu0, [mu1..])."""
+    n = h0.shape[0]
+    m0 = Matrix.from_numpy(h0)
+    mh = [Matrix.from_numpy(x) for x in hs[:order]]
+    p0 = Matrix.zeros(n)
+    ps = [Matrix.zeros(n) for _ in range(order)]
+    hv = [m.h for m in mh] + [None] * (3 - order)
+    pv = [m.h for m in ps] + [None] * (3 - order)
+    lib().prg_implicit_fermi_response(m0.h, hv[0], hv[1], hv[2], p0.h, pv[0], pv[1], pv[2],
+                                      nsteps, mu0, 0.0, beta, nocc, occ_tol, lin_tol, order, 0.0)
+    _check()
+    m0_out = C.c_double()
+    mu = _dvec(3)
+    lib().prg_implicit_fermi_response_mu(C.byref(m0_out), _dp(mu))
+    return p0.numpy(), [p.numpy() for p in ps], m0_out.value, mu[:order].tolist()
 
 
 # ---------------------------------------------------------------- response
@@ -636,8 +661,8 @@ def finite_diff(h0, hs, mu0, mus, beta, order, lam, h):
     mh = [Matrix.from_numpy(x) for x in hs[:order]]
     outs = [Matrix.zeros(n) for _ in range(order)]
     mu = _dvec(order, mus[:order])
-    lib().prg_finite_diff(m0.h, _handles(mh), mu0, _dp(mu), beta, order, lam, h, 0.0,
-                          _handles(outs))
+    lib().prg_gpu_finite_diff(m0.h, _handles(mh), mu0, _dp(mu), beta, order, lam, h, 0.0,
+                              _handles(outs))
     _check()
     return [o.numpy() for o in outs]
 

@@ -293,3 +293,21 @@ def test_full_size_n8192_implicit_fermi_properties(P):
         assert bool(torch.equal(p, p.T))
         out.append(p)
     assert float(torch.linalg.norm(out[0] - out[1])) < 1e-8
+
+
+def test_crt_after_shutdown_and_init(P):
+    """prg_progress_shutdown frees every workspace, the int8 product's item
+    list and per-modulus constants included; the next int8 solve must upload
+    them again (a stale cache would run every item as modulus 0, tile 0).
+    Same density and branch sequence before and after, and the oracle's."""
+    n = 300
+    print(f"[int8crt shutdown/init] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 1)
+    rho_o, pp_o, ic_o, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    before, pp_b, ic_b, _ = P.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    P.lib().prg_progress_shutdown()
+    P.lib().prg_progress_init()
+    after, pp_a, ic_a, _ = P.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert ic_a == ic_b == ic_o and list(pp_a) == list(pp_b) == list(pp_o)
+    assert np.array_equal(after, before)
+    assert np.linalg.norm(after - rho_o) < 1e-7

@@ -914,3 +914,216 @@ def test_finite_diff_matches_fermi_taylor_coefficients():
     assert np.max(np.abs(np.diag(o[1]) - t2)) < 100 * h * h
     assert np.max(np.abs(np.diag(o[2]) - t3)) < 1000 * h * h
     assert np.max(np.abs(o[0] - np.diag(np.diag(o[0])))) == 0.0
+
+
+# --------------------------------------------------------------------------
+# core+halo sets (bml_matrix2submatrix_index), halo order, and the remaining
+# oracle functions: implicit_fermi_zero, sp2_entropy_ts, test_density_matrix
+# --------------------------------------------------------------------------
+
+
+def _random_sparse_sym(n, density, rng):
+    m = (rng.random((n, n)) < density) * rng.normal(size=(n, n))
+    m = m + m.T
+    np.fill_diagonal(m, rng.normal(size=n))
+    return m
+
+
+def _halo_cases():
+    rng = np.random.default_rng(21)
+    n = 60
+    g = _random_sparse_sym(n, 0.03, rng)
+    h = _random_sparse_sym(n, 0.04, rng)
+    cores = [list(range(0, 6)), [7, 3, 3, 41], [59], list(rng.permutation(n)[:10])]
+    return g, h, cores
+
+
+@pytest.mark.parametrize("double_jump", [True, False])
+@pytest.mark.parametrize("with_h", [True, False])
+def test_halo_set_is_the_symbolic_product_pattern(double_jump, with_h):
+    """The set matrix2submatrix_index lists equals the nonzero columns of the
+    core rows of (I + |G| + |H|)(I + |G|) (double jump) or (I + |G| + |H|)
+    (single jump), computed here by integer matrix products.  The list holds
+    the cores first (repeats dropped) and no node twice."""
+    g, h, cores = _halo_cases()
+    hh = h if with_h else None
+    for core in cores:
+        idx, nc = O.matrix2submatrix_index(g, core, double_jump, hh)
+        uniq = list(dict.fromkeys(int(c) for c in core))
+        assert idx[:nc] == uniq
+        assert len(set(idx)) == len(idx)
+        assert set(idx) == O.halo_set_bruteforce(g, core, double_jump, hh)
+
+
+def test_halo_set_planted_mistakes_are_caught():
+    """A single-jump extraction and one that ignores H both fail the
+    double-jump symbolic pattern with H on these graphs."""
+    g, h, cores = _halo_cases()
+    ref = [O.halo_set_bruteforce(g, c, True, h) for c in cores]
+    single = [set(O.matrix2submatrix_index(g, c, False, h)[0]) for c in cores]
+    no_h = [set(O.matrix2submatrix_index(g, c, True, None)[0]) for c in cores]
+    assert any(a != b for a, b in zip(single, ref))
+    assert any(a != b for a, b in zip(no_h, ref))
+
+
+def test_halo_order_does_not_move_the_core_rows():
+    """The order of the halo nodes inside a part is a similarity permutation of
+    the block, which commutes with the SP2 polynomial: shuffling the halo
+    moves the core rows of the part's density (and its vv) by rounding only."""
+    rng = np.random.default_rng(5)
+    n = 96
+    h = O.model_hamiltonian(n, seed=2)
+    lo, hi = O.gershgorin(h)
+    _, pp, ic, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    g = (np.abs(h) > 0.05).astype(float)
+    core = list(range(20, 36))
+    idx, nc = O.matrix2submatrix_index(g, core, True, h)
+    halo = idx[nc:]
+    perm = idx[:nc] + list(rng.permutation(halo))
+    d1, vv1 = O.sp2_submatrix(h[np.ix_(idx, idx)], pp, ic, [0.0] * ic, lo, hi, nc)
+    d2, vv2 = O.sp2_submatrix(h[np.ix_(perm, perm)], pp, ic, [0.0] * ic, lo, hi, nc)
+    pos = {k: i for i, k in enumerate(perm)}
+    back = [pos[k] for k in idx]
+    assert np.max(np.abs(d1[:nc, :] - d2[:nc, :][:, back])) < 1e-12
+    assert np.allclose(vv1, vv2, rtol=1e-12, atol=1e-14)
+
+
+def _gapped_hamiltonian(n, seed, gap=2.0):
+    """Symmetric H with eigenvalues in [-3, -gap/2] u [gap/2, 3] (half each)."""
+    rng = np.random.default_rng(seed)
+    e = np.concatenate([rng.uniform(-3.0, -gap / 2, n // 2), rng.uniform(gap / 2, 3.0, n - n // 2)])
+    q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    return (q * e) @ q.T
+
+
+def test_implicit_fermi_zero_closed_form():
+    """prg_implicit_fermi_zero, method 0 (CG solves): with converged solves the
+    recursion is the scalar map x <- x^2/(2x^2 - 2x + 1) on the eigenvalues of
+    X0 = (0.5 + c mu) I - c H, c = 0.5 min(1/(mu - lo), 1/(hi - mu)) from the
+    Gershgorin interval (src/prg_implicit_fermi_mod.F90:379-382), i.e.
+    implicit_fermi_eigen at beta = c 2^(nsteps + 2).  On a gapped H it tends
+    to the T = 0 projector onto the states below mu."""
+    n, mu = 40, 0.1
+    h = _gapped_hamiltonian(n, 3)
+    w, q = eig(h)
+    lo, hi = O.gershgorin(h)
+    for nsteps in (4, 10):
+        c = 0.5 * min(1.0 / (mu - lo), 1.0 / (hi - mu))
+        x = O.implicit_fermi_eigen(w, mu, c * 2.0 ** (nsteps + 2), nsteps)
+        p = O.implicit_fermi_zero(h, nsteps, mu, 0, 1e-24)
+        assert np.linalg.norm(p - (q * x) @ q.T) < 1e-9
+    proj = (q[:, w < mu]) @ q[:, w < mu].T
+    assert np.linalg.norm(O.implicit_fermi_zero(h, 14, mu, 0, 1e-24) - proj) < 1e-8
+
+
+def test_implicit_fermi_zero_planted_mistakes_are_caught(monkeypatch):
+    """A constant without the 0.5 factor, or the reflected map (X0 = 0.5 +
+    c (H - mu)), must fail the closed form (model H, states at mu: the map's
+    slope there is what the constant sets)."""
+    n, mu, nsteps = 40, 0.0, 3
+    h = O.model_hamiltonian(n, seed=3)
+    w, q = eig(h)
+    lo, hi = O.gershgorin(h)
+    c = 0.5 * min(1.0 / (mu - lo), 1.0 / (hi - mu))
+    ref = (q * O.implicit_fermi_eigen(w, mu, c * 2.0 ** (nsteps + 2), nsteps)) @ q.T
+    real = O.normalize_implicit_fermi
+    monkeypatch.setattr(O, "normalize_implicit_fermi", lambda hh, cn, m: real(hh, 2.0 * cn, m))
+    assert np.linalg.norm(O.implicit_fermi_zero(h, nsteps, mu, 0, 1e-24) - ref) > 1e-3
+    monkeypatch.setattr(O, "normalize_implicit_fermi", lambda hh, cn, m: real(hh, -cn, m))
+    assert np.linalg.norm(O.implicit_fermi_zero(h, nsteps, mu, 0, 1e-24) - ref) > 1.0
+
+
+def test_implicit_fermi_zero_method1_is_a_function_of_h():
+    """Method 1 reuses one CG inverse (tolerance 0.9) of the first level's
+    Y = 2(X^2 - X) + I; that inverse is a polynomial in Y, so the result is a
+    spectral function of H: it commutes with H and equals the same recursion
+    run on the diagonal matrix of eigenvalues, rotated back."""
+    n, mu = 40, 0.1
+    h = _gapped_hamiltonian(n, 4)
+    w, q = eig(h)
+    p = O.implicit_fermi_zero(h, 6, mu, 1, 1e-12)
+    pd = O.implicit_fermi_zero(np.diag(w), 6, mu, 1, 1e-12)
+    # the Gershgorin interval of diag(w) differs from H's: run both with the
+    # same interval by comparing only what the interval does not change
+    assert np.linalg.norm(p @ h - h @ p) < 1e-9
+    lo, hi = O.gershgorin(h)
+    lod, hid = O.gershgorin(np.diag(w))
+    if (lo, hi) == (lod, hid):
+        assert np.linalg.norm(p - (q * np.diag(pd)) @ q.T) < 1e-9
+
+
+def _entropy_G(x, mu, h1, hn, sg, total):
+    from scipy.integrate import quad
+    return quad(lambda t: O.sp2_inverse(t, mu, h1, hn, sg), 0.0, x, limit=200)[0] - total * x
+
+
+def test_sp2_entropy_ts_is_the_quadrature_of_the_inverse():
+    """sp2_entropy_ts on D0 = Q diag(d) Q^T: the sum over |d_i| of the entropy
+    function G(x) = int_0^x sp2_inverse - x int_0^1 sp2_inverse, evaluated
+    here by adaptive quadrature (independent of the 1e-4 mesh); eigenvalue
+    invariance through the rotation Q."""
+    from scipy.integrate import quad
+    sg = [1, -1, 1, -1, -1, 1]
+    mu, h1, hn = 0.2, -3.0, 2.5
+    gg, ee = O.sp2_entropy_function(mu, h1, hn, sg)
+    total = quad(lambda t: O.sp2_inverse(t, mu, h1, hn, sg), 0.0, 1.0, limit=200)[0]
+    rng = np.random.default_rng(8)
+    d = np.concatenate([rng.uniform(0.02, 0.98, 10), [-0.3, 0.51234567]])
+    qm, _ = np.linalg.qr(rng.normal(size=(12, 12)))
+    d0 = (qm * d) @ qm.T
+    ts = O.sp2_entropy_ts(d0, gg, ee)
+    gx = [_entropy_G(abs(x), mu, h1, hn, sg, total) for x in d]
+    assert abs(ts - sum(gx)) < 5e-5
+    # one eigenvalue at a time (errors of opposite sign cannot cancel)
+    err = [abs(O.sp2_entropy_ts(np.array([[x]]), gg, ee) - g) for x, g in zip(d, gx)]
+    assert max(err) < 1e-5
+    # planted mistake: the mesh read one cell off (GG(j) for GG(j+1))
+    off = [abs(O.sp2_entropy_ts(np.array([[x]]), np.roll(gg, 1), ee) - g) for x, g in zip(d, gx)]
+    assert max(off) > 5e-5
+
+
+def test_test_density_matrix_converges_to_the_occupation_root():
+    """prg_test_density_matrix (osteps = 0): the returned mu satisfies
+    |sum_i f(e_i; mu) - nocc| <= occErrLimit, P = Q f Q^T at that mu, and mu
+    is within the Newton bound of the root found by bracketing."""
+    from scipy.optimize import brentq
+    n, beta = 50, 4.0
+    h = O.model_hamiltonian(n, seed=7)
+    w, q = eig(h)
+    nocc = 0.5 * n + 0.3
+    limit = 1e-9
+    # undamped Newton, as the reference: start near the root (mu = 0.73)
+    p, mu = O.test_density_matrix(h, beta, 0.5, nocc, 0, limit)
+    occ = np.sum(1.0 / (1.0 + np.exp(beta * (w - mu))))
+    assert abs(occ - nocc) <= limit
+    f = 1.0 / (1.0 + np.exp(beta * (w - mu)))
+    assert np.linalg.norm(p - (q * f) @ q.T) < 1e-12
+    root = brentq(lambda m: np.sum(1.0 / (1.0 + np.exp(beta * (w - m)))) - nocc, -10, 10,
+                  xtol=1e-15)
+    dndmu = beta * np.sum(f * (1 - f))
+    assert abs(mu - root) <= 2 * limit / dndmu
+    # osteps > 0: exactly osteps Newton steps; from a start near the root
+    # three steps already converge quadratically
+    _, mu3 = O.test_density_matrix(h, beta, root + 0.05, nocc, 3, limit)
+    assert abs(mu3 - root) < 1e-10
+
+
+def test_test_density_matrix_planted_mistake_is_caught():
+    """A Newton step that drops the sum of squares (dN/dmu = beta Tr P instead
+    of beta (Tr P - sum P_ij^2)) does not reach the root in three steps."""
+    from scipy.optimize import brentq
+    n, beta = 50, 4.0
+    h = O.model_hamiltonian(n, seed=7)
+    w, q = eig(h)
+    nocc = 0.5 * n + 0.3
+    root = brentq(lambda m: np.sum(1.0 / (1.0 + np.exp(beta * (w - m)))) - nocc, -10, 10,
+                  xtol=1e-15)
+
+    def broken(h, beta, mu, nocc, osteps):
+        for _ in range(osteps):
+            f = 1.0 / (1.0 + np.exp(beta * (w - mu)))
+            p = (q * f) @ q.T
+            mu = mu + (nocc - np.trace(p)) / (beta * np.trace(p))
+        return mu
+
+    assert abs(broken(h, beta, root + 0.05, nocc, 3) - root) > 1e-6

@@ -221,3 +221,68 @@ def test_sp2_tensorcore_entry():
     assert err < 5e-3
     assert tr < 1e-3
     assert np.array_equal(d, d.T)
+
+
+@pytest.mark.parametrize("prec,moduli", [("int8crt", 10), ("int8crt", 11), ("f16split", 0),
+                                         ("bf16split", 0)])
+@pytest.mark.parametrize("core", [37, 120])
+def test_submatrix_core_rows_low_precision(P, prec, moduli, core):
+    """prg_sp2_submatrix under the other product precisions: vv accumulates the
+    core rows only, as with FP64 products (src/prg_sp2_mod.F90:1224).  Bars:
+    int8 CRT (X rounded to 2^-39 / 2^-43) density within 1e-7 and vv within
+    1e-6 relative; the FP32-accumulated split products track the oracle to
+    their own accuracy (vv within 3 % of its largest entry for FP16, 20 % for
+    BF16) -- both far from the whole-matrix sums, which are about n / core
+    times larger."""
+    print(f"[submatrix low precision] {prec} moduli={moduli} core={core}", flush=True)
+    n = 192
+    h = O.model_hamiltonian(n, 6)
+    lo, hi = O.gershgorin(h)
+    _, pp, ic, _ = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    vv0 = [0.0] * ic
+    ro, vo = O.sp2_submatrix(h, pp, ic, vv0, lo, hi, core)
+    _, vwhole = O.sp2_submatrix(h, pp, ic, vv0, lo, hi, n)
+    mode = {"int8crt": P.PREC_INT8CRT, "f16split": P.PREC_F16SPLIT,
+            "bf16split": P.PREC_BF16SPLIT}[prec]
+    P.set_precision(mode)
+    if moduli:
+        P.set_int8_moduli(moduli)
+    try:
+        rg, vg = P.sp2_submatrix(h, pp, ic, vv0, lo, hi, core)
+    finally:
+        P.set_int8_moduli(11)
+        P.set_precision(P.PREC_FP64)
+    vo, vg, vwhole = np.asarray(vo), np.asarray(vg[:ic]), np.asarray(vwhole)
+    if prec == "int8crt":
+        assert np.linalg.norm(rg - ro) < 1e-7
+        assert np.allclose(vg, vo, rtol=1e-6, atol=1e-12)
+    else:
+        tol = (0.03 if prec == "f16split" else 0.2) * vo.max()
+        assert np.max(np.abs(vg - vo)) < tol
+    assert np.max(np.abs(vg - vwhole)) > 0.5 * vo.max()
+
+
+def test_homolumogap_and_sequence_from_gpu_genseq(P):
+    """prg_homolumogap and prg_sp2sequence fed by the GPU's genseq, as
+    examples/gptest/gptest.F90:57-115 chains them: the same gap estimate and
+    the same generated sequence as the oracle's routines on the oracle's
+    genseq (pp identical, vv within 1e-9 relative), and the estimate brackets
+    the true HOMO/LUMO of an eigendecomposition to 1e-3 of the interval."""
+    print("[homolumogap/sp2sequence from GPU genseq] n=256", flush=True)
+    n = 256
+    h = O.model_hamiltonian(n, 0)
+    lo, hi = P.gershgorin(h)
+    lo_o, hi_o = O.gershgorin(h)
+    assert abs(lo - lo_o) < 1e-12 and abs(hi - hi_o) < 1e-12
+    _, pp_g, ic_g, vv_g = P.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    _, pp_o, ic_o, vv_o = O.sp2_alg2_genseq(h, 0.5, 10, 200, "Abs", 1e-10)
+    assert ic_g == ic_o and list(pp_g) == list(pp_o)
+    ours = P.homolumogap(vv_g, ic_g, pp_g, lo, hi)
+    ref = O.homolumogap(vv_o, ic_o, pp_o, lo_o, hi_o)
+    assert np.allclose(ours, ref, rtol=1e-9, atol=1e-12)
+    w = np.linalg.eigvalsh(h)
+    assert abs(ours[0] - w[n // 2 - 1]) < 1e-3 * (hi - lo)
+    assert abs(ours[1] - w[n // 2]) < 1e-3 * (hi - lo)
+    pps, imax = P.sp2sequence(lo, hi, ours[0], ours[1], 1e-12)
+    ppo, imo = O.sp2sequence(lo_o, hi_o, ref[0], ref[1], 1e-12)
+    assert imax == imo and list(pps) == list(ppo)
