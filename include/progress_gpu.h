@@ -71,6 +71,14 @@ double bml_get_element(bml_matrix_t *A, int i, int j); /* 0-based */
 void bml_read_bml_matrix(bml_matrix_t *A, char *filename); /* Matrix Market */
 void bml_write_bml_matrix(bml_matrix_t *A, char *filename);
 void bml_deallocate(bml_matrix_t **A);
+/* BML accessors the graph-partitioned SP2 callers use (examples/gptest/
+ * gptest.F90:68,82,100,131,144): thresholding (|a| <= threshold -> 0),
+ * Tr(A B), ||A - B||_F, and the Gershgorin interval as a malloc'ed {min, max}
+ * pair the caller frees. */
+void bml_threshold(bml_matrix_t *A, double threshold);
+double bml_trace_mult(bml_matrix_t *A, bml_matrix_t *B);
+double bml_fnorm2(bml_matrix_t *A, bml_matrix_t *B);
+void *bml_gershgorin(bml_matrix_t *A);
 
 /* ------------------------------------------------------------------ library */
 /* src/prg_c_interface.F90:51-61 */

@@ -267,6 +267,53 @@ double bml_trace(bml_matrix_t* A) {
   return s;
 }
 
+// BML's thresholding (entries with |a| <= threshold become zero), as the
+// graph construction of examples/gptest/gptest.F90:100 uses it.
+void bml_threshold(bml_matrix_t* A, double threshold) {
+  clear_error();
+  if (bml_host_sync(A)) return;
+  for (size_t k = 0; k < (size_t)A->N * A->N; ++k)
+    if (!(std::fabs(A->host[k]) > threshold)) A->host[k] = 0.0;
+  host_modified(A);
+}
+
+// Tr(A B) = sum_ij A_ij B_ji (examples/gptest/gptest.F90:68,131: band energy).
+double bml_trace_mult(bml_matrix_t* A, bml_matrix_t* B) {
+  clear_error();
+  if (check_pair(A, B, "bml_trace_mult") || bml_host_sync(A) || bml_host_sync(B)) return NAN;
+  const int N = A->N;
+  double s = 0.0;
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < N; ++i) s += A->host[i + (size_t)j * N] * B->host[j + (size_t)i * N];
+  return s;
+}
+
+// ||A - B||_F (examples/gptest/gptest.F90:144).
+double bml_fnorm2(bml_matrix_t* A, bml_matrix_t* B) {
+  clear_error();
+  if (check_pair(A, B, "bml_fnorm2") || bml_host_sync(A) || bml_host_sync(B)) return NAN;
+  double s = 0.0;
+  for (size_t k = 0; k < (size_t)A->N * A->N; ++k) {
+    const double d = A->host[k] - B->host[k];
+    s += d * d;
+  }
+  return std::sqrt(s);
+}
+
+// Gershgorin interval as BML returns it: a malloc'ed pair {min, max} the
+// caller frees (examples/gptest/gptest.F90:82).
+void* bml_gershgorin(bml_matrix_t* A) {
+  clear_error();
+  double* g = (double*)std::malloc(2 * sizeof(double));
+  if (!g) return nullptr;
+  prg_gershgorin(A, g);
+  if (last_error_code() != OK) {
+    std::free(g);
+    return nullptr;
+  }
+  return g;
+}
+
 int bml_get_N(bml_matrix_t* A) { return A ? A->N : -1; }
 int bml_get_M(bml_matrix_t* A) { return A ? A->M : -1; }
 

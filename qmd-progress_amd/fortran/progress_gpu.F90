@@ -25,6 +25,7 @@ module bml
   public :: bml_zero_matrix, bml_identity_matrix, bml_read_matrix, bml_deallocate
   public :: bml_get_n, bml_import_from_dense, bml_export_to_dense, bml_scale, bml_trace
   public :: bml_copy, bml_write_matrix, prg_gpu_last_error
+  public :: bml_copy_new, bml_get_m, bml_threshold, bml_trace_mult, bml_fnorm2, bml_gershgorin
 
   interface
     function c_bml_zero_matrix(t, p, n, m, d) bind(C, name="bml_zero_matrix") result(r)
@@ -82,6 +83,36 @@ module bml
       import :: c_ptr
       type(c_ptr), value :: a, b
     end subroutine
+    function c_bml_copy_new(a) bind(C, name="bml_copy_new") result(r)
+      import :: c_ptr
+      type(c_ptr), value :: a
+      type(c_ptr) :: r
+    end function
+    function c_bml_get_m(a) bind(C, name="bml_get_M") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: a
+      integer(c_int) :: r
+    end function
+    subroutine c_bml_threshold(a, thr) bind(C, name="bml_threshold")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a
+      real(c_double), value :: thr
+    end subroutine
+    function c_bml_trace_mult(a, b) bind(C, name="bml_trace_mult") result(r)
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a, b
+      real(c_double) :: r
+    end function
+    function c_bml_fnorm2(a, b) bind(C, name="bml_fnorm2") result(r)
+      import :: c_ptr, c_double
+      type(c_ptr), value :: a, b
+      real(c_double) :: r
+    end function
+    function c_bml_gershgorin(a) bind(C, name="bml_gershgorin") result(r)
+      import :: c_ptr
+      type(c_ptr), value :: a
+      type(c_ptr) :: r
+    end function
     subroutine c_free(p) bind(C, name="free")
       import :: c_ptr
       type(c_ptr), value :: p
@@ -192,7 +223,68 @@ contains
     call c_bml_copy(a%ptr, b%ptr)
   end subroutine bml_copy
 
+  subroutine bml_copy_new(a, b)
+    type(bml_matrix_t), intent(in) :: a
+    type(bml_matrix_t), intent(inout) :: b
+    if (c_associated(b%ptr)) call c_bml_deallocate(b%ptr)
+    b%ptr = c_bml_copy_new(a%ptr)
+  end subroutine bml_copy_new
+
+  integer function bml_get_m(a)
+    type(bml_matrix_t), intent(in) :: a
+    bml_get_m = c_bml_get_m(a%ptr)
+  end function bml_get_m
+
+  !> entries with |a| <= threshold become zero
+  subroutine bml_threshold(a, threshold)
+    type(bml_matrix_t), intent(inout) :: a
+    real(bml_dp), intent(in) :: threshold
+    call c_bml_threshold(a%ptr, threshold)
+  end subroutine bml_threshold
+
+  !> Tr(A B)
+  real(bml_dp) function bml_trace_mult(a, b)
+    type(bml_matrix_t), intent(in) :: a, b
+    bml_trace_mult = c_bml_trace_mult(a%ptr, b%ptr)
+  end function bml_trace_mult
+
+  !> ||A - B||_F
+  real(bml_dp) function bml_fnorm2(a, b)
+    type(bml_matrix_t), intent(in) :: a, b
+    bml_fnorm2 = c_bml_fnorm2(a%ptr, b%ptr)
+  end function bml_fnorm2
+
+  !> Gershgorin interval: gbnd(1) = min, gbnd(2) = max
+  subroutine bml_gershgorin(a, gbnd)
+    type(bml_matrix_t), intent(in) :: a
+    real(bml_dp), allocatable, intent(inout) :: gbnd(:)
+    type(c_ptr) :: p
+    real(bml_dp), pointer :: g(:)
+    if (.not. allocated(gbnd)) allocate(gbnd(2))
+    p = c_bml_gershgorin(a%ptr)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, g, [2])
+    gbnd(1:2) = g(1:2)
+    call c_free(p)
+  end subroutine bml_gershgorin
+
 end module bml
+
+! src/prg_progress_mod.F90 (library start / stop / version) over
+! src/prg_c_interface.F90:51-61's C symbols.
+module prg_progress_mod
+  implicit none
+  private
+  public :: prg_progress_init, prg_progress_shutdown, prg_version
+  interface
+    subroutine prg_progress_init() bind(C, name="prg_progress_init")
+    end subroutine
+    subroutine prg_progress_shutdown() bind(C, name="prg_progress_shutdown")
+    end subroutine
+    subroutine prg_version() bind(C, name="prg_version")
+    end subroutine
+  end interface
+end module prg_progress_mod
 
 module prg_gpu_cstr
   use iso_c_binding
@@ -1155,6 +1247,371 @@ contains
     call c_deortho(orthoA_bml%ptr, zmat_bml%ptr, a_bml%ptr, threshold, cstr(bml_type), verbose)
   end subroutine prg_deorthogonalize
 end module prg_nonortho_mod
+
+! Graph partitioning (src/prg_graph_mod.F90) on one rank.  The Fortran type
+! keeps the reference's fields; node ids are 0-based throughout (the C
+! routines and the core+halo lists are 0-based; the reference's own
+! prg_readPart adds 1 to the ids it reads, which its 0-based equal partitions
+! do not -- here both are 0-based).  The library routines work on the flat C
+! struct prg_graph_partitioning_t, filled from and copied back into this type.
+module prg_graph_mod
+  use iso_c_binding
+  use prg_gpu_cstr
+  implicit none
+  private
+  integer, parameter :: dp = kind(1.0d0)
+
+  !> src/prg_graph_mod.F90:32-56
+  type, public :: subgraph_t
+    integer :: part = 0
+    integer :: hsize = 0
+    integer :: lsize = 0
+    integer :: llsize = 0
+    integer, allocatable :: core_halo_index(:)
+    integer, allocatable :: nodeInPart(:)
+  end type subgraph_t
+
+  !> src/prg_graph_mod.F90:58-140
+  type, public :: graph_partitioning_t
+    character(len=100) :: pname = ""
+    integer :: myRank = 0
+    integer :: totalProcs = 1
+    integer :: totalParts = 0
+    integer :: totalNodes = 0
+    integer :: totalNodes2 = 0
+    integer :: globalPartMin = 0
+    integer :: globalPartMax = 0
+    integer :: globalPartExtent = 0
+    integer, allocatable :: localPartMin(:)
+    integer, allocatable :: localPartMax(:)
+    integer, allocatable :: localPartExtent(:)
+    integer, allocatable :: order(:)
+    integer, allocatable :: reorder(:)
+    integer :: nparts = 0
+    integer :: nx = 0, ny = 0, nz = 0
+    integer, allocatable :: nnodesInPart(:)
+    integer, allocatable :: nnodesInPartAll(:)
+    integer :: pp(100) = 0
+    integer :: maxIter = 0
+    real(dp) :: ehomo = 0.0_dp
+    real(dp) :: elumo = 0.0_dp
+    real(dp) :: mineval = 0.0_dp
+    real(dp) :: maxeval = 0.0_dp
+    real(dp) :: vv(100) = 0.0_dp
+    type(subgraph_t), allocatable :: sgraph(:)
+  end type graph_partitioning_t
+
+  !> the library's flat prg_graph_partitioning_t (include/progress_gpu.h)
+  type, bind(C), public :: prg_c_partitioning_t
+    integer(c_int) :: totalParts
+    integer(c_int) :: totalNodes
+    type(c_ptr) :: nnodesInPart
+    type(c_ptr) :: nodeInPart
+    type(c_ptr) :: lsize
+    type(c_ptr) :: llsize
+    type(c_ptr) :: core_halo_index
+    integer(c_int) :: pp(100)
+    integer(c_int) :: maxIter
+    real(c_double) :: ehomo, elumo, mineval, maxeval
+    real(c_double) :: vv(100)
+  end type prg_c_partitioning_t
+
+  public :: prg_initSubgraph, prg_destroySubgraph, prg_initGraphPartitioning
+  public :: prg_destroyGraphPartitioning, prg_equalPartition, prg_filePartition
+  public :: prg_fnormGraph
+  public :: prg_gp_to_c, prg_gp_from_c, prg_gp_c_free
+
+  interface
+    subroutine c_set_partition(gp, np, nn, sizes, nodes) bind(C, name="prg_setPartition")
+      import :: prg_c_partitioning_t, c_int
+      type(prg_c_partitioning_t) :: gp
+      integer(c_int), value :: np, nn
+      integer(c_int) :: sizes(*), nodes(*)
+    end subroutine
+    subroutine c_file_partition(gp, f) bind(C, name="prg_filePartition")
+      import :: prg_c_partitioning_t, c_char
+      type(prg_c_partitioning_t) :: gp
+      character(kind=c_char) :: f(*)
+    end subroutine
+    subroutine c_destroy(gp) bind(C, name="prg_destroyGraphPartitioning")
+      import :: prg_c_partitioning_t
+      type(prg_c_partitioning_t) :: gp
+    end subroutine
+    function c_err() bind(C, name="prg_last_error_code") result(r)
+      import :: c_int
+      integer(c_int) :: r
+    end function
+  end interface
+
+contains
+
+  subroutine c_clear(cg)
+    type(prg_c_partitioning_t), intent(out) :: cg
+    cg%totalParts = 0
+    cg%totalNodes = 0
+    cg%nnodesInPart = c_null_ptr
+    cg%nodeInPart = c_null_ptr
+    cg%lsize = c_null_ptr
+    cg%llsize = c_null_ptr
+    cg%core_halo_index = c_null_ptr
+    cg%pp = 0
+    cg%maxIter = 0
+    cg%ehomo = 0.0_dp
+    cg%elumo = 0.0_dp
+    cg%mineval = 0.0_dp
+    cg%maxeval = 0.0_dp
+    cg%vv = 0.0_dp
+  end subroutine c_clear
+
+  !> src/prg_graph_mod.F90:148-159
+  subroutine prg_initSubgraph(sg, pnum, hsize)
+    type(subgraph_t), intent(inout) :: sg
+    integer, intent(in) :: pnum, hsize
+    sg%part = pnum
+    sg%hsize = hsize
+    sg%lsize = 0
+    sg%llsize = 0
+    if (allocated(sg%core_halo_index)) deallocate(sg%core_halo_index)
+    allocate(sg%core_halo_index(hsize))
+  end subroutine prg_initSubgraph
+
+  !> src/prg_graph_mod.F90:164-172
+  subroutine prg_destroySubgraph(sg)
+    type(subgraph_t), intent(inout) :: sg
+    if (allocated(sg%core_halo_index)) deallocate(sg%core_halo_index)
+    if (allocated(sg%nodeInPart)) deallocate(sg%nodeInPart)
+  end subroutine prg_destroySubgraph
+
+  !> src/prg_graph_mod.F90:179-260, one rank: every part is local
+  subroutine prg_initGraphPartitioning(gp, pname, np, nnodes, nnodes2)
+    type(graph_partitioning_t), intent(inout) :: gp
+    character(len=*), intent(in) :: pname
+    integer, intent(in) :: np, nnodes, nnodes2
+    gp%myRank = 0
+    gp%pname = pname
+    gp%totalProcs = 1
+    gp%totalParts = np
+    gp%totalNodes = nnodes
+    gp%totalNodes2 = nnodes2
+    gp%globalPartMin = 1
+    gp%globalPartMax = np
+    gp%globalPartExtent = np
+    allocate(gp%localPartMin(1), gp%localPartMax(1), gp%localPartExtent(1))
+    gp%localPartMin(1) = 1
+    gp%localPartMax(1) = np
+    gp%localPartExtent(1) = np
+    gp%nparts = np
+    allocate(gp%nnodesInPart(np), gp%nnodesInPartAll(np), gp%sgraph(np))
+    allocate(gp%order(nnodes), gp%reorder(nnodes))
+    gp%maxIter = 0
+    gp%mineval = 0.0_dp
+    gp%maxeval = 0.0_dp
+  end subroutine prg_initGraphPartitioning
+
+  !> src/prg_graph_mod.F90:268-290
+  subroutine prg_destroyGraphPartitioning(gp)
+    type(graph_partitioning_t), intent(inout) :: gp
+    integer :: i
+    if (allocated(gp%localPartMin)) deallocate(gp%localPartMin)
+    if (allocated(gp%localPartMax)) deallocate(gp%localPartMax)
+    if (allocated(gp%localPartExtent)) deallocate(gp%localPartExtent)
+    if (allocated(gp%order)) deallocate(gp%order)
+    if (allocated(gp%reorder)) deallocate(gp%reorder)
+    if (allocated(gp%nnodesInPart)) deallocate(gp%nnodesInPart)
+    if (allocated(gp%nnodesInPartAll)) deallocate(gp%nnodesInPartAll)
+    if (allocated(gp%sgraph)) then
+      do i = 1, size(gp%sgraph)
+        call prg_destroySubgraph(gp%sgraph(i))
+      enddo
+      deallocate(gp%sgraph)
+    endif
+  end subroutine prg_destroyGraphPartitioning
+
+  !> src/prg_graph_mod.F90:359-399: ceiling(nnodes / nodesPerPart) parts of
+  !> consecutive 0-based node ids
+  subroutine prg_equalPartition(gp, nodesPerPart, nnodes)
+    type(graph_partitioning_t), intent(inout) :: gp
+    integer, intent(in) :: nodesPerPart, nnodes
+    integer :: i, j, np, psize
+    if (nnodes <= nodesPerPart) then
+      np = 1
+    else
+      np = ceiling(real(nnodes) / real(nodesPerPart))
+    endif
+    call prg_destroyGraphPartitioning(gp)
+    call prg_initGraphPartitioning(gp, "equalParts", np, nnodes, nnodes)
+    do i = 1, np
+      call prg_initSubgraph(gp%sgraph(i), i, nnodes)
+      if (i * nodesPerPart <= nnodes) then
+        psize = nodesPerPart
+      else
+        psize = nnodes - nodesPerPart * (i - 1)
+      endif
+      allocate(gp%sgraph(i)%nodeInPart(psize))
+      do j = 1, psize
+        gp%sgraph(i)%nodeInPart(j) = (i - 1) * nodesPerPart + j - 1
+      enddo
+      gp%nnodesInPart(i) = psize
+      gp%nnodesInPartAll(i) = psize
+    enddo
+  end subroutine prg_equalPartition
+
+  !> src/prg_graph_mod.F90:880-929 (the library's reader; ids kept 0-based)
+  subroutine prg_filePartition(gp, partFile)
+    type(graph_partitioning_t), intent(inout) :: gp
+    character(len=*), intent(in) :: partFile
+    type(prg_c_partitioning_t) :: cg
+    integer, pointer :: sizes(:), nodes(:)
+    type(c_ptr), pointer :: lists(:)
+    character(len=100) :: pname
+    integer :: i, np, u
+    call c_clear(cg)
+    call c_file_partition(cg, cstr(partFile))
+    if (c_err() /= 0) return
+    pname = "filePart"
+    open(newunit=u, file=partFile, status='old')
+    read(u, *) pname
+    close(u)
+    np = cg%totalParts
+    call prg_destroyGraphPartitioning(gp)
+    call prg_initGraphPartitioning(gp, trim(pname), np, cg%totalNodes, cg%totalNodes)
+    call c_f_pointer(cg%nnodesInPart, sizes, [np])
+    call c_f_pointer(cg%nodeInPart, lists, [np])
+    do i = 1, np
+      call prg_initSubgraph(gp%sgraph(i), i, cg%totalNodes)
+      allocate(gp%sgraph(i)%nodeInPart(sizes(i)))
+      if (sizes(i) > 0) then
+        call c_f_pointer(lists(i), nodes, [sizes(i)])
+        gp%sgraph(i)%nodeInPart(:) = nodes(:)
+      endif
+      gp%nnodesInPart(i) = sizes(i)
+      gp%nnodesInPartAll(i) = sizes(i)
+    enddo
+    call c_destroy(cg)
+  end subroutine prg_filePartition
+
+  !> src/prg_graph_mod.F90:933-971 (one rank): vv <- sqrt(vv)
+  subroutine prg_fnormGraph(gp)
+    type(graph_partitioning_t), intent(inout) :: gp
+    integer :: i
+    do i = 1, gp%maxIter
+      gp%vv(i) = sqrt(gp%vv(i))
+    enddo
+  end subroutine prg_fnormGraph
+
+  !> fills the library's flat struct from gp (cores, SP2 sequence, bounds, vv)
+  subroutine prg_gp_to_c(gp, cg)
+    type(graph_partitioning_t), intent(in) :: gp
+    type(prg_c_partitioning_t), intent(inout) :: cg
+    integer(c_int), allocatable :: sizes(:), nodes(:)
+    integer :: i, k
+    call c_clear(cg)
+    allocate(sizes(gp%totalParts))
+    do i = 1, gp%totalParts
+      sizes(i) = gp%nnodesInPart(i)
+    enddo
+    allocate(nodes(max(1, sum(sizes))))
+    k = 0
+    do i = 1, gp%totalParts
+      nodes(k + 1:k + sizes(i)) = gp%sgraph(i)%nodeInPart(1:sizes(i))
+      k = k + sizes(i)
+    enddo
+    call c_set_partition(cg, gp%totalParts, gp%totalNodes, sizes, nodes)
+    cg%pp = gp%pp
+    cg%maxIter = gp%maxIter
+    cg%ehomo = gp%ehomo
+    cg%elumo = gp%elumo
+    cg%mineval = gp%mineval
+    cg%maxeval = gp%maxeval
+    cg%vv = gp%vv
+  end subroutine prg_gp_to_c
+
+  !> copies the core+halo lists (lsize, llsize) and vv back into gp
+  subroutine prg_gp_from_c(cg, gp)
+    type(prg_c_partitioning_t), intent(in) :: cg
+    type(graph_partitioning_t), intent(inout) :: gp
+    integer, pointer :: ls(:), lls(:), idx(:)
+    type(c_ptr), pointer :: lists(:)
+    integer :: i, np
+    np = cg%totalParts
+    gp%vv = cg%vv
+    if (.not. c_associated(cg%lsize)) return
+    call c_f_pointer(cg%lsize, ls, [np])
+    call c_f_pointer(cg%llsize, lls, [np])
+    call c_f_pointer(cg%core_halo_index, lists, [np])
+    do i = 1, np
+      gp%sgraph(i)%lsize = ls(i)
+      gp%sgraph(i)%llsize = lls(i)
+      if (allocated(gp%sgraph(i)%core_halo_index)) deallocate(gp%sgraph(i)%core_halo_index)
+      allocate(gp%sgraph(i)%core_halo_index(max(1, ls(i))))
+      if (ls(i) > 0 .and. c_associated(lists(i))) then
+        call c_f_pointer(lists(i), idx, [ls(i)])
+        gp%sgraph(i)%core_halo_index(1:ls(i)) = idx(:)
+      endif
+    enddo
+  end subroutine prg_gp_from_c
+
+  subroutine prg_gp_c_free(cg)
+    type(prg_c_partitioning_t), intent(inout) :: cg
+    call c_destroy(cg)
+  end subroutine prg_gp_c_free
+
+end module prg_graph_mod
+
+! Subgraph SP2 loop (src/prg_subgraphloop_mod.F90) over the library's batched
+! GPU loop.
+module prg_subgraphloop_mod
+  use iso_c_binding
+  use bml
+  use prg_graph_mod
+  implicit none
+  private
+  public :: prg_subgraphSP2Loop, prg_getPartitionHalosFromGraph
+  interface
+    subroutine c_loop(h, g, r, gp, thr) bind(C, name="prg_subgraphSP2Loop")
+      import :: c_ptr, c_double, prg_c_partitioning_t
+      type(c_ptr), value :: h, g, r
+      type(prg_c_partitioning_t) :: gp
+      real(c_double), value :: thr
+    end subroutine
+    subroutine c_halos(gp, g, dj) bind(C, name="prg_getPartitionHalosFromGraph")
+      import :: c_ptr, c_int, prg_c_partitioning_t
+      type(prg_c_partitioning_t) :: gp
+      type(c_ptr), value :: g
+      integer(c_int), value :: dj
+    end subroutine
+  end interface
+contains
+  !> src/prg_subgraphloop_mod.F90:36-126 (one rank): core+halo lists from G
+  !> and H, every part purified with gp%pp / gp%maxIter on the GPU, core rows
+  !> into rho_bml, gp%vv accumulated then square-rooted (prg_fnormGraph)
+  subroutine prg_subgraphSP2Loop(h_bml, g_bml, rho_bml, gp, threshold)
+    type(bml_matrix_t), intent(in) :: h_bml, g_bml
+    type(bml_matrix_t), intent(inout) :: rho_bml
+    type(graph_partitioning_t), intent(inout) :: gp
+    real(kind(1.0d0)), intent(in) :: threshold
+    type(prg_c_partitioning_t) :: cg
+    call prg_gp_to_c(gp, cg)
+    call c_loop(h_bml%ptr, g_bml%ptr, rho_bml%ptr, cg, threshold)
+    call prg_gp_from_c(cg, gp)
+    call prg_gp_c_free(cg)
+  end subroutine prg_subgraphSP2Loop
+
+  !> src/prg_subgraphloop_mod.F90:336-368
+  subroutine prg_getPartitionHalosFromGraph(gp, g_bml, djflag)
+    type(graph_partitioning_t), intent(inout) :: gp
+    type(bml_matrix_t), intent(in) :: g_bml
+    logical, intent(in) :: djflag
+    type(prg_c_partitioning_t) :: cg
+    integer(c_int) :: dj
+    dj = 0
+    if (djflag) dj = 1
+    call prg_gp_to_c(gp, cg)
+    call c_halos(cg, g_bml%ptr, dj)
+    call prg_gp_from_c(cg, gp)
+    call prg_gp_c_free(cg)
+  end subroutine prg_getPartitionHalosFromGraph
+end module prg_subgraphloop_mod
 
 module prg_graphsolver_mod
   use iso_c_binding

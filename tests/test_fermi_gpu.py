@@ -245,3 +245,24 @@ def test_finite_diff_matches_oracle(P, h256):
     ref = O.finite_diff(h256, hs, 0.6, mus, 20.0, 3, 0.1, 1e-3)
     for k, (g, r) in enumerate(zip(got, ref)):
         assert np.max(np.abs(g - r)) < 1e-11 / 1e-3 ** (k + 1), k
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_implicit_fermi_response_reference_arity(P, order):
+    """prg_implicit_fermi_response at the arity of src/prg_progress_mod.h:240-258
+    (separate H1..H3 / P1..P3 handles, NULL above `order`; mu0 and mu by
+    value, read back with prg_implicit_fermi_response_mu) gives the oracle's
+    result, as the array form does."""
+    print(f"[implicit_fermi_response C arity] order={order}", flush=True)
+    n = 128
+    h0 = O.model_hamiltonian(n, 4)
+    hs = [O.banded_perturbation(n, 4, seed=10 + j) * 0.1 for j in range(3)]
+    p0, ps, mu0, mu = P.implicit_fermi_response_c(h0, hs, 6, 0.6, 10.0, 64.0, 1e-6, order,
+                                                  lin_tol=1e-10)
+    p0_o, ps_o, mu0_o, mu_o = O.implicit_fermi_response(h0, hs, 6, 0.6, 10.0, 64.0, 1e-6, order,
+                                                        lin_tol=1e-10)
+    assert abs(mu0 - mu0_o) < 1e-8
+    assert np.allclose(mu, mu_o, atol=1e-8)
+    assert np.linalg.norm(p0 - p0_o) < 1e-8
+    for a, b in zip(ps, ps_o):
+        assert np.linalg.norm(a - b) < 1e-7 * max(1.0, np.linalg.norm(b))

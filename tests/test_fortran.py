@@ -116,3 +116,64 @@ def test_respsolve_driver_matches_oracle(tmp_path):
     ai_o, it_o = O.newtonschulz(a, 0.25 * np.eye(n), 1e-10)
     assert np.max(np.abs(ai - ai_o)) < 1e-11
     assert f"iterations={it_o}" in r.stdout
+    # prg_implicit_fermi_zero and prg_implicit_fermi_response (order 2, H2 =
+    # H1 / 2) at the reference's Fortran signatures
+    ifz = O.read_mtx(f"{pre}_ifz.mtx")
+    assert np.linalg.norm(ifz - O.implicit_fermi_zero(h, 6, 0.0, 0, 1e-18)) < 1e-8
+    p0_o, ps_o, mu0_o, mu_o = O.implicit_fermi_response(h, [h1, 0.5 * h1], 6, 0.0, 10.0,
+                                                        0.5 * n, 1e-8, 2, lin_tol=1e-10)
+    assert np.linalg.norm(O.read_mtx(f"{pre}_ifr0.mtx") - p0_o) < 1e-8
+    for k in (1, 2):
+        pk = O.read_mtx(f"{pre}_ifr{k}.mtx")
+        assert np.linalg.norm(pk - ps_o[k - 1]) < 1e-7 * max(1.0, np.linalg.norm(ps_o[k - 1]))
+    line = [x for x in r.stdout.splitlines() if "mu0=" in x][0]
+    vals = dict(kv.split("=") for kv in line.replace("respsolve: ", "").split())
+    assert abs(float(vals["mu0"]) - mu0_o) < 1e-8
+    assert abs(float(vals["mu1"]) - mu_o[0]) < 1e-8 and abs(float(vals["mu2"]) - mu_o[1]) < 1e-8
+
+
+GPTEST = ROOT / "qmd-progress_amd" / "bin" / "gptest"
+
+
+def test_gptest_usage():
+    if not GPTEST.exists():
+        _need_driver()
+        pytest.fail("amdflang is present but the gptest driver was not built")
+    r = subprocess.run([str(GPTEST)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage: gptest" in r.stdout
+
+
+@pytest.mark.gpu
+def test_gptest_driver_matches_oracle(golden, tmp_path):
+    """The gptest-shaped Fortran caller (prg_graph_mod, prg_subgraphloop_mod,
+    prg_homolumo_mod and the bml accessors, in the call order of
+    examples/gptest/gptest.F90:57-151) on the reference's poly.16.mtx with 16
+    core nodes per part, against the same flow through the oracle."""
+    if not GPTEST.exists():
+        _need_driver()
+        pytest.fail("amdflang is present but the gptest driver was not built")
+    print("[gptest] poly.16 nodesPerPart=16", flush=True)
+    out = tmp_path / "rho.mtx"
+    r = subprocess.run([str(GPTEST), str(golden / "poly.16.mtx"), "16", "1e-5", "1e-12", "0.5",
+                        str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = dict(l.split(" = ") for l in r.stdout.splitlines() if " = " in l)
+    h = O.read_mtx(str(golden / "poly.16.mtx"))
+    n = h.shape[0]
+    rho_full, pp, ic, vv = O.sp2_alg2_genseq(h, 0.5, 25, 100, "Rel", 1e-10)
+    assert int(got["icount"]) == ic
+    assert abs(float(got["trace_hp_sp2"]) - np.sum(h * rho_full)) < 1e-8
+    lo, hi = O.gershgorin(h)
+    eh, el, eg = O.homolumogap(vv, ic, pp, lo, hi)
+    assert abs(float(got["egap_sp2"]) - eg) < 1e-8 * max(1.0, abs(eg))
+    g = np.where(np.abs(rho_full) > 1e-5, rho_full, 0.0)
+    parts = O.equal_partition(16, n)
+    seq, imax = O.sp2sequence(lo, hi, eh, el, 1e-12)
+    assert int(got["parts"]) == len(parts) and int(got["maxIter"]) == imax
+    rho_o, vv_o = O.subgraph_sp2_loop(h, g, parts, seq, imax, lo, hi)
+    rho = O.read_mtx(str(out))
+    assert np.linalg.norm(rho - rho_o) < 1e-9
+    assert abs(float(got["trace_hp_graph"]) - np.sum(h * rho_o)) < 1e-8
+    assert abs(float(got["frob"]) - np.linalg.norm(rho_full - rho_o)) < 1e-8
+    _, _, eg_g = O.homolumogap(vv_o, imax, seq, lo, hi)
+    assert abs(float(got["egap_graph"]) - eg_g) < 1e-6 * max(1.0, abs(eg_g))

@@ -33,8 +33,13 @@ def test_subgraph_loop_matches_oracle(P, n, per_part, gthr):
     rho_o, vv_o = O.subgraph_sp2_loop(h, g, parts, seq, imax, lo, hi)
     gp = P.equal_partition(per_part, n)
     rho_g, vv_g = P.subgraph_sp2_loop(h, g, gp, seq, imax, lo, hi)
-    halos_o = [O.matrix2submatrix_index(g, c)[0] for c in parts]
+    # the loop passes H to the core+halo extraction, as the reference does
+    # (src/prg_subgraphloop_mod.F90:60-63): H's neighbours of the core join
+    # the halo before the double jump; the set is pinned by symbolic products
+    halos_o = [O.matrix2submatrix_index(g, c, True, h)[0] for c in parts]
     assert gp.halos() == halos_o
+    for c, lst in zip(parts, gp.halos()):
+        assert set(lst) == O.halo_set_bruteforce(g, c, True, h)
     assert np.linalg.norm(rho_g - rho_o) < 1e-9
     assert np.allclose(vv_g, vv_o, rtol=1e-9, atol=1e-12)
 
