@@ -43,8 +43,9 @@ def parse():
     ap.add_argument("--moduli", default="10,11",
                     help="comma-separated moduli counts of the int8 Chinese-remainder products "
                          "(6..14); each runs as mode int8crt<m>")
-    ap.add_argument("--cpu-iters", type=int, default=4,
-                    help="SP2 iterations timed for the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=5,
+                    help="full-size SP2 iterations timed for the CPU baseline sample (>= 5, "
+                         "after one warm-up; the median is used)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="dense",
                     choices=["dense", "graph", "gpt0", "fermi", "adjoint"],
@@ -760,35 +761,75 @@ def products_cpu_baseline(h, nprod, iters):
             "tflops": 2.0 * float(n) ** 3 / t / 1e12}
 
 
+def host_description():
+    """CPU model, logical CPUs and the BLAS numpy calls (threadpoolctl)."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    blas, threads = "unknown", os.cpu_count()
+    try:
+        from threadpoolctl import threadpool_info
+        libs = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        if libs:
+            blas = f"{libs[0].get('internal_api')} {libs[0].get('version')}"
+            threads = libs[0].get("num_threads", threads)
+    except Exception:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "blas": blas, "blas_threads": threads}
+
+
 def cpu_baseline(n, seed, iters, gpu_iters):
-    """The oracle's FP64 prg_sp2_alg2 restatement (numpy / OpenBLAS dgemm) timed
-    on a bounded sample: normalisation plus `iters` SP2 iterations at the full
-    N, scaled to solves/s with the GPU run's iteration count."""
+    """The oracle's FP64 prg_sp2_alg2 restatement (numpy dgemm on the host
+    BLAS) on a bounded sample of the same workload: the normalisation, one
+    untimed warm-up iteration, then `iters` (>= 5) full-size N x N iterations
+    timed one by one; solves/s = 1 / (normalisation + median iteration x the
+    GPU run's iteration count)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
     import progress_oracle as O
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max(i.get("num_threads", 1) for i in threadpool_info()
-                      if i.get("internal_api") == "openblas") if threadpool_info() else os.cpu_count()
-    except Exception:
-        threads = os.cpu_count()
+    iters = max(5, iters)
+    host = host_description()
     h = O.model_hamiltonian(n, seed)
+    occ = 0.5 * n
     t0 = time.perf_counter()
     x = O.normalize(h)
     t_norm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for _ in range(iters):
+    trx = float(np.trace(x))
+
+    def step(x, trx):
+        # src/prg_sp2_mod.F90:321-349: X^2, limdiff, branch, recursed trace
         x2 = x @ x
-        tr = float(np.trace(x2))
-        x = 2.0 * x - x2 if tr < 0.5 * n else x2
-    t_it = (time.perf_counter() - t0) / iters
+        trx2 = float(np.trace(x2))
+        limdiff = abs(trx2 - occ) - abs(2.0 * trx - trx2 - occ)
+        if limdiff >= 1e-10:
+            return 2.0 * x - x2, 2.0 * trx - trx2
+        return x2, trx2
+
+    x, trx = step(x, trx)  # warm-up
+    times = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        x, trx = step(x, trx)
+        times.append(time.perf_counter() - t0)
+    t_it = float(np.median(times))
     it_total = gpu_iters or 66
     solve_s = t_norm + it_total * t_it
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 SP2 (numpy/OpenBLAS dgemm), N={n}: normalisation + {iters} "
-                      f"iterations timed ({t_it:.2f} s/iteration), scaled to {it_total} iterations",
-            "tflops": 2.0 * float(n) ** 3 / t_it / 1e12}
+    out = {"value": 1.0 / solve_s, "unit": "solves/s", "cores": host["blas_threads"],
+           "kind": "port",
+           "sample": f"oracle FP64 SP2 alg2 (numpy dgemm, {host['blas']}, "
+                     f"{host['blas_threads']} threads), N={n}: normalisation "
+                     f"({t_norm:.2f} s) + 1 warm-up + {iters} timed full-size iterations, "
+                     f"median {t_it:.3f} s (min {min(times):.3f}, max {max(times):.3f}), "
+                     f"scaled to {it_total} iterations",
+           "tflops": 2.0 * float(n) ** 3 / t_it / 1e12,
+           "iteration_s": times}
+    out.update(host)
+    return out
 
 
 if __name__ == "__main__":

@@ -207,3 +207,52 @@ def test_implicit_fermi_adjoint_vs_oracle_and_fd(canonical):
         e = e / np.linalg.norm(e)
         fd = (np.sum(p_of(h + eps * e) * a) - np.sum(p_of(h - eps * e) * a)) / (2.0 * eps)
         assert abs(fd - np.sum(g * e)) <= 1e-5 * gn
+
+
+def test_sp2_adjoint_fp32_input_c4():
+    """Config C4 names an FP32 Hamiltonian: H given as float32 (N = 2048,
+    seed 0; A banded, bandwidth 32) is promoted exactly to FP64, and D and
+    dL/dH come back as float32.  They equal the FP64 path on the promoted
+    input rounded to float32 (relative 1e-6, the float32 rounding), and the
+    gradient is symmetric."""
+    import torch
+    import progress_amd as P
+    from progress_amd.autograd import sp2_density
+    P.set_precision(P.PREC_FP64)
+    n = 2048
+    print(f"[adjoint C4 fp32 input] n={n}", flush=True)
+    h32 = torch.tensor(O.model_hamiltonian(n, 0), dtype=torch.float32, device="cuda")
+    a = torch.tensor(O.banded_perturbation(n, 32, seed=1), device="cuda")
+    x32 = h32.clone().requires_grad_(True)
+    d32 = sp2_density(x32, 0.5, 1e-10, 8)
+    assert d32.dtype == torch.float32
+    torch.sum(d32.to(torch.float64) * a).backward()
+    g32 = x32.grad
+    assert g32.dtype == torch.float32
+    x64 = h32.to(torch.float64).clone().requires_grad_(True)
+    d64 = sp2_density(x64, 0.5, 1e-10, 8)
+    torch.sum(d64 * a).backward()
+    g64 = x64.grad
+    assert torch.linalg.norm(d32.double() - d64) <= 1e-6 * torch.linalg.norm(d64)
+    assert torch.linalg.norm(g32.double() - g64) <= 1e-6 * torch.linalg.norm(g64)
+    assert torch.linalg.norm(g32 - g32.T) <= 1e-6 * torch.linalg.norm(g32)
+
+
+def test_sp2_adjoint_matches_oracle_response():
+    """dTr(D A)/dH of the SP2 recursion from the autograd binding against the
+    oracle's prg_compute_response_SP2 with H1 = A (the adjoint of the
+    self-adjoint first-order response), FP64, N = 256: 1e-9 relative."""
+    import torch
+    import progress_amd as P
+    from progress_amd.autograd import sp2_density
+    P.set_precision(P.PREC_FP64)
+    n = 256
+    print(f"[adjoint vs oracle response] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 2)
+    a = O.banded_perturbation(n, 32, seed=5)
+    ht = torch.tensor(h, device="cuda", requires_grad=True)
+    d = sp2_density(ht, 0.5, 1e-10, 8)
+    torch.sum(d * torch.tensor(a, device="cuda")).backward()
+    g = ht.grad.detach().cpu().numpy()
+    _, rho1, _, _ = O.compute_response_sp2(h, a, 0.5, 1, 500, "Abs", 1e-10)
+    assert np.linalg.norm(g - rho1) <= 1e-9 * np.linalg.norm(rho1)
