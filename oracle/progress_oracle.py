@@ -1089,22 +1089,22 @@ def matrix2submatrix_index(g, core, double_jump=True, h=None):
     So the set is S1 u N_G(S1) with S1 = C u N_G(C) u N_H(C), i.e. the
     nonzero columns of the core rows of (I + |G| + |H|)(I + |G|)
     (``halo_set_bruteforce``).  Returns (index_list, core_size)."""
-    seen = set()
+    seen = np.zeros(g.shape[0], dtype=bool)
     idx = []
     for c in core:
         c = int(c)
-        if c not in seen:
-            seen.add(c)
+        if not seen[c]:
+            seen[c] = True
             idx.append(c)
     ncore = len(idx)
 
     def walk(m, rows):
+        # each row's unseen nonzero columns in ascending order, rows in order
         for c in rows:
-            for k in np.nonzero(m[c])[0]:
-                k = int(k)
-                if k not in seen:
-                    seen.add(k)
-                    idx.append(k)
+            nz = np.nonzero(m[c])[0]
+            new = nz[~seen[nz]]
+            seen[new] = True
+            idx.extend(int(k) for k in new)
 
     walk(g, list(idx))
     if h is not None:

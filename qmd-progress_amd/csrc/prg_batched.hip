@@ -25,6 +25,8 @@ using tile::xcd_remap;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
 
 constexpr int NT = 256;
 
@@ -37,6 +39,12 @@ struct BatchArgs {
   double* part_sums;    // per tile: core-row sum of (X - X^2)^2
   const int* queue_off; // queue q holds tiles [queue_off[q], queue_off[q + 1]), q < 8
   int* tickets;         // 8 ticket counters, zero at launch
+  // split-precision kernels: 16-bit hi / lo planes of the current and the
+  // next X, packed like x (same element offsets)
+  const void* hi;
+  const void* lo;
+  void* hin;
+  void* lon;
 };
 
 // One TM x TN tile of X^2 (rows r0.., columns c0..) of one part and its fused
@@ -189,10 +197,198 @@ __global__ __launch_bounds__(NT, WIDE ? 2 : 4) void batched_sp2_kernel(BatchArgs
   }
 }
 
+// ------------------------------------------------------------------------
+// Split-precision variant (PRG_PREC_F16SPLIT / PRG_PREC_BF16SPLIT): X is
+// carried as the FP64 master plus two 16-bit planes, hi = t(x) and
+// lo = t((x - hi) s) with s = 2^11 (FP16) or 2^8 (BF16), so that
+//   X^2 ~= hi hi + (hi lo + lo hi) / s
+// runs as three 16x16x32 MFMA products with FP32 accumulation (two
+// accumulator sets); the update, the next planes and the core-row sums are
+// formed in FP64 from the master, as the FP64 tile does.  64 x 64 tiles of
+// 64-wide panels; 64-deep K slabs of the four operand planes (A hi, A lo,
+// B hi, B lo; wave w stages plane w) through LDS with a register prefetch of
+// the next slab.  X symmetric: row r of a panel is column r in HBM,
+// contiguous in k.
+template <typename T>
+struct SplitT;
+template <>
+struct SplitT<_Float16> {
+  typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+  static constexpr double kScale = 2048.0;
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct SplitT<__bf16> {
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  static constexpr double kScale = 256.0;
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int SBK = 64;          // K slab (halves)
+constexpr int SROW = SBK + 8;    // padded LDS row: 144 bytes
+constexpr int kSplitLds = 4 * 64 * SROW * 2;  // bytes: four 64-row planes
+
+template <typename T>
+__device__ __forceinline__ void split_tile(const BatchArgs& g, int t, int4 tl, T* smem) {
+  typedef typename SplitT<T>::v8 h8;
+  constexpr double kScale = SplitT<T>::kScale;
+  T (*Sp)[SROW] = reinterpret_cast<T (*)[SROW]>(smem);  // plane w: rows 64 w .. 64 w + 63
+  const PartDesc pd = g.parts[tl.x];
+  const int r0 = tl.y, c0 = tl.z;
+  const int m = pd.m;
+  const double* X = g.x + pd.offset;
+  double* Xn = g.xn + pd.offset;
+  const T* hi = (const T*)g.hi + pd.offset;
+  const T* lo = (const T*)g.lo + pd.offset;
+  T* hin = (T*)g.hin + pd.offset;
+  T* lon = (T*)g.lon + pd.offset;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+  const bool diag = (r0 == c0);
+  const bool idle = diag && wm == 1 && wn == 0;
+  // loader: wave w stages plane w (A hi, A lo, B hi, B lo); lane reads the
+  // 16-byte chunk (lane & 7) of rows (lane >> 3) + 8 q, q < 8
+  const T* src = (w & 1) ? lo : hi;
+  const int lrow0 = (w >> 1) ? c0 : r0;
+  const int lc = 8 * (lane & 7), lr = lane >> 3;
+  i4 pre[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      pre[q] = *(const i4*)(src + (size_t)(lrow0 + lr + 8 * q) * m + k0 + lc);
+  };
+  f4 acc_hh[2][2], acc_x[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc_hh[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      acc_x[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  load(0);
+  for (int k0 = 0; k0 < m; k0 += SBK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) *(i4*)(&Sp[64 * w + lr + 8 * q][lc]) = pre[q];
+    __syncthreads();
+    if (k0 + SBK < m) load(k0 + SBK);
+    if (!idle) {
+#pragma unroll 1
+      for (int kk = 0; kk < SBK / 32; ++kk) {
+        const int kc = kk * 32 + 8 * (lane >> 4);
+        h8 bh[2], bl[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int rb = wn * 32 + j * 16 + (lane & 15);
+          bh[j] = *(const h8*)(&Sp[128 + rb][kc]);
+          bl[j] = *(const h8*)(&Sp[192 + rb][kc]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const int ra = wm * 32 + i * 16 + (lane & 15);
+          const h8 ah = *(const h8*)(&Sp[ra][kc]);
+          const h8 al = *(const h8*)(&Sp[64 + ra][kc]);
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc_hh[i][j] = SplitT<T>::mfma(bh[j], ah, acc_hh[i][j]);
+            acc_x[i][j] = SplitT<T>::mfma(bl[j], ah, acc_x[i][j]);
+            acc_x[i][j] = SplitT<T>::mfma(bh[j], al, acc_x[i][j]);
+          }
+        }
+      }
+    }
+  }
+  // epilogue: operands were swapped, so this lane holds (X^2)[row][col] with
+  // row = lane % 16, col = 4 (lane / 16) + r inside each 16 x 16 block.  In
+  // the diagonal blocks of a diagonal tile only row <= col is formed and
+  // mirrored (the hi lo and lo hi sums of (r, c) and (c, r) accumulate in
+  // different orders), so X_next is exactly symmetric.
+  const double ca = g.sigma ? 1.0 : -1.0, cb = g.sigma ? 0.0 : 2.0;
+  const int core = pd.core;
+  double s[1] = {0.0};
+  if (!idle) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int row = r0 + wm * 32 + i * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = c0 + wn * 32 + j * 16 + 4 * (lane >> 4) + r;
+          if (row > col) continue;
+          const bool mir = (row != col);
+          const size_t idx = row + (size_t)col * m;
+          const size_t tdx = mir ? col + (size_t)row * m : idx;
+          const double y = (double)acc_hh[i][j][r] + (1.0 / kScale) * (double)acc_x[i][j][r];
+          const double x = X[idx];
+          const double z = fma(cb, x, ca * y);
+          const double d = x - y;
+          const double wi = ((row < core) ? 1.0 : 0.0) + ((mir && col < core) ? 1.0 : 0.0);
+          s[0] += wi * d * d;
+          const T h = (T)z;
+          const T l = (T)((z - (double)h) * kScale);
+          Xn[idx] = z;
+          Xn[tdx] = z;
+          hin[idx] = h;
+          hin[tdx] = h;
+          lon[idx] = l;
+          lon[tdx] = l;
+        }
+        // one 16 x 16 block at a time (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  block_store_partials<1>(s, g.part_sums + t);
+}
+
+// Persistent work-queue launch of split tiles (the queue walk of
+// batched_sp2_kernel over the 64-wide tile list).
+template <typename T>
+__global__ __launch_bounds__(NT, 4) void batched_split_kernel(BatchArgs g) {
+  __shared__ __attribute__((aligned(16))) T smem[kSplitLds / sizeof(T)];
+  __shared__ int s_ticket;
+  const int home = blockIdx.x & 7;
+  for (int q = 0; q < 8; ++q) {
+    const int qi = (home + q) & 7;
+    const int first = g.queue_off[qi], count = g.queue_off[qi + 1] - first;
+    for (;;) {
+      __syncthreads();  // every thread has read the previous ticket
+      if (threadIdx.x == 0) s_ticket = atomicAdd(&g.tickets[qi], 1);
+      __syncthreads();
+      const int k = s_ticket;
+      if (k >= count) break;
+      const int t = first + k;
+      split_tile<T>(g, t, g.tiles[t], smem);
+    }
+  }
+}
+
+// hi / lo planes of the packed FP64 X (padding stays zero).
+template <typename T>
+__global__ void split_planes_kernel(const double* __restrict__ x, T* __restrict__ hi,
+                                    T* __restrict__ lo, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) {
+    const double z = x[i];
+    const T h = (T)z;
+    hi[i] = h;
+    lo[i] = (T)((z - (double)h) * SplitT<T>::kScale);
+  }
+}
+
 typedef void (*BatchKernel)(BatchArgs);
 
-// The compiled configurations: (panel, slabs of prefetch, slab depth).
+// The compiled configurations: (panel, slabs of prefetch, slab depth); the
+// split-precision kernels have one (64, 1, 64).
 BatchKernel batch_kernel(const BatchPlan& p) {
+  if (p.split)
+    return (!p.wide && p.depth == 1 && p.kslab == SBK)
+               ? (p.split == 2 ? batched_split_kernel<__bf16> : batched_split_kernel<_Float16>)
+               : nullptr;
   if (p.wide) return (p.depth == 1 && p.kslab == 16) ? batched_sp2_kernel<true, 1, 16> : nullptr;
   if (p.kslab == 32) return p.depth == 1 ? batched_sp2_kernel<false, 1, 32> : nullptr;
   if (p.kslab == 16)
@@ -336,10 +532,18 @@ int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double minev
 }
 
 int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
-                 int* d_tickets) {
+                 int* d_tickets, int parity) {
   BatchArgs g;
   g.x = x;
   g.xn = xn;
+  g.hi = plan.planes[2 * parity];
+  g.lo = plan.planes[2 * parity + 1];
+  g.hin = plan.planes[2 * (parity ^ 1)];
+  g.lon = plan.planes[2 * (parity ^ 1) + 1];
+  if (plan.split && (!g.hi || !g.lo || !g.hin || !g.lon)) {
+    set_error(ERR_ARG, "batched SP2: split planes not set");
+    return ERR_ARG;
+  }
   g.parts = plan.d_parts;
   g.tiles = plan.d_tiles;
   g.sigma = sigma;
@@ -375,6 +579,17 @@ int batched_slots(BatchPlan* plan) {
   PRG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   // one workgroup per resident slot, a multiple of 8 (workgroup b runs on XCD b mod 8)
   plan->grid = ((cus * std::max(nb, 1) + 7) / 8) * 8;
+  return OK;
+}
+
+int batched_split_prepare(const BatchPlan& plan, const double* x) {
+  if (plan.split == 2)
+    hipLaunchKernelGGL(split_planes_kernel<__bf16>, dim3(2048), dim3(NT), 0, stream(), x,
+                       (__bf16*)plan.planes[0], (__bf16*)plan.planes[1], plan.total);
+  else
+    hipLaunchKernelGGL(split_planes_kernel<_Float16>, dim3(2048), dim3(NT), 0, stream(), x,
+                       (_Float16*)plan.planes[0], (_Float16*)plan.planes[1], plan.total);
+  PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
 

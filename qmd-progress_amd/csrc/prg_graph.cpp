@@ -22,6 +22,7 @@
 #include "prg_bml.h"
 #include "prg_eigen.h"
 #include "prg_ops.h"
+#include "prg_sp2.h"
 
 namespace prg {
 namespace {
@@ -267,7 +268,7 @@ int enqueue_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, i
   double* nxt = x1;
   for (int it = 0; it < iters && !st; ++it) {
     st = batched_step(plan, cur, nxt, pp[it] != 0, d_part + (size_t)it * plan.ntiles,
-                      d_tickets + 8 * it);
+                      d_tickets + 8 * it, it & 1);
     const double* t = cur;
     cur = nxt;
     nxt = const_cast<double*>(t);
@@ -290,6 +291,9 @@ int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int i
                                (uint64_t)plan.ntiles,             (uint64_t)plan.wide,
                                (uint64_t)plan.depth,              (uint64_t)plan.grid,
                                (uint64_t)plan.kslab,
+                               (uint64_t)plan.split,
+                               (uint64_t)(uintptr_t)plan.planes[0],
+                               (uint64_t)(uintptr_t)plan.planes[2],
                                (uint64_t)iters,
                                (uint64_t)workspace_generation()};
   for (int it = 0; it < iters; ++it) key.push_back(pp[it] != 0);
@@ -323,6 +327,9 @@ int run_loop(const BatchPlan& plan, double* x0, double* x1, const int* pp, int i
   return OK;
 }
 
+int run_batch(const DMat& H, DMat rho, prg_graph_partitioning_t* gp, const BatchPlan& plan,
+              double* x0, double* x1, double* d_part, int* d_tickets, double* d_vv);
+
 int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   int st = ensure_device();
   if (st) return st;
@@ -350,6 +357,24 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   int* d_tickets = (int*)workspace("gp_tickets", sizeof(int) * 8 * (iters + 1), &st);
   double* d_vv = (double*)workspace("gp_vv", sizeof(double) * 100, &st);
   if (st) return st;
+  // split-precision products (prg_set_precision FP16 / BF16 split): the
+  // 64-wide split kernel, X also carried as hi / lo planes; int8 CRT has no
+  // batched kernel, so its parts run with FP64 products
+  const int pm = precision_mode();
+  plan.split = pm == PRG_PREC_F16SPLIT ? 1 : (pm == PRG_PREC_BF16SPLIT ? 2 : 0);
+  if (plan.split) {
+    char* pl = (char*)workspace("gp_planes", 4 * sizeof(uint16_t) * off, &st);
+    if (st) return st;
+    for (int k = 0; k < 4; ++k) plan.planes[k] = pl + k * sizeof(uint16_t) * off;
+    if ((st = upload_tiles(tiles64, 64, &plan))) return st;
+    plan.depth = 1;
+    plan.kslab = 64;
+    if ((st = batched_slots(&plan))) return st;
+    g_cfg[0] = 64;
+    g_cfg[2] = 1;
+    g_cfg[3] = 64;
+    return run_batch(H, rho, gp, plan, x0, x1, d_part, d_tickets, d_vv);
+  }
   BatchConfig force;
   if (const char* e = std::getenv("PRG_BATCH_TILE")) force.panel = std::atoi(e);
   if (const char* e = std::getenv("PRG_BATCH_DEPTH")) force.depth = std::atoi(e);
@@ -399,12 +424,22 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   g_cfg[0] = cfg.panel;
   g_cfg[2] = cfg.depth;
   g_cfg[3] = cfg.kslab;
+  return run_batch(H, rho, gp, plan, x0, x1, d_part, d_tickets, d_vv);
+}
+
+// Gather + normalise, the iteration loop, the core-row scatter and the vv
+// read-back of one configured batch, with HIP-event phase timing.
+int run_batch(const DMat& H, DMat rho, prg_graph_partitioning_t* gp, const BatchPlan& plan,
+              double* x0, double* x1, double* d_part, int* d_tickets, double* d_vv) {
+  int st = OK;
+  const int iters = gp->maxIter;
   PRG_HIP_CHECK(hipMemcpyAsync(d_vv, gp->vv, sizeof(double) * 100, hipMemcpyHostToDevice,
                                stream()));
   hipEvent_t ev[4];
   for (auto& e : ev) PRG_HIP_CHECK(hipEventCreate(&e));
   PRG_HIP_CHECK(hipEventRecord(ev[0], stream()));
   if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval))) return st;
+  if (plan.split && (st = batched_split_prepare(plan, x0))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[1], stream()));
   if ((st = run_loop(plan, x0, x1, gp->pp, iters, d_part, d_tickets, d_vv))) return st;
   PRG_HIP_CHECK(hipEventRecord(ev[2], stream()));

@@ -22,6 +22,8 @@ struct BatchPlan {
   int depth = 1;             // K slabs prefetched ahead (1 or 2)
   int kslab = 16;            // K slab depth (16 or 32)
   int grid = 0;              // workgroups of the persistent batched SP2 launch
+  int split = 0;             // 0 FP64 products; 1 FP16 / 2 BF16 hi+lo split products
+  void* planes[4] = {nullptr, nullptr, nullptr, nullptr};  // hi0, lo0, hi1, lo1 (split)
   size_t total = 0;          // doubles in one packed buffer
   PartDesc* d_parts = nullptr;
   int4* d_tiles = nullptr;
@@ -39,9 +41,13 @@ int halo_indices_device(const DMat& G, prg_graph_partitioning_t* gp, int djflag,
 int batched_gather(const DMat& H, const BatchPlan& plan, double* x, double mineval,
                    double maxeval);
 // One SP2 iteration of every part: xn from x, per-tile vv partials in
-// d_part; d_tickets: 8 queue counters, zero before the launch.
+// d_part; d_tickets: 8 queue counters, zero before the launch.  Split plans:
+// parity selects the planes of x (planes[2 parity], [2 parity + 1]); those of
+// xn are the others.
 int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, double* d_part,
-                 int* d_tickets);
+                 int* d_tickets, int parity = 0);
+// hi / lo planes 0 and 1 of a split plan from the packed FP64 x
+int batched_split_prepare(const BatchPlan& plan, const double* x);
 // whether a kernel is compiled for plan's wide / depth / kslab
 bool batched_config_valid(const BatchPlan& plan);
 // plan->grid = resident workgroups of the kernel that wide/depth/kslab select

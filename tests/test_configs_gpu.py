@@ -1,0 +1,222 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+C2: the model Hamiltonian at N = 4096 in every product precision (FP64,
+int8 CRT with 10 and 11 moduli, FP16 and BF16 hi+lo split) and the split
+modes at N = 8192, each against an FP64 eigendecomposition on the GPU
+(D_diag) and the branch sequence of the SP2 control flow run on the
+eigenvalues (O.sp2_alg2_eigen).  C3: the 30,000-orbital water box with 512
+parts: every core+halo list against the oracle's, the core rows of sampled
+parts against O.sp2_submatrix on the same blocks, and their vv.  C5: the
+implicit-Fermi adjoint at N = 8192 against central differences and against
+the first-order-response identity.  Bars are stated per test with the values
+measured on an MI355X.
+"""
+import numpy as np
+import pytest
+
+import progress_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_matrix(P, torch, h):
+    n = h.shape[0]
+    ld = P.padded_ld(n)
+    d = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    d[:n, :n] = torch.from_numpy(h)  # symmetric: row/column-major views coincide
+    return d
+
+
+def _model_case(n, seed):
+    import torch
+    import progress_amd as P
+    h = O.model_hamiltonian(n, seed)
+    hd = _device_matrix(P, torch, h)
+    w, v = torch.linalg.eigh(hd[:n, :n])
+    q = v[:, :n // 2]
+    ddiag = 2.0 * (q @ q.T)
+    del v, q
+    lo, hi = O.gershgorin(h)
+    we = w.cpu().numpy()
+    _, it_e, pp_e = O.sp2_alg2_eigen(we, lo, hi, 0.5 * n, 1, 200, "Abs", 1e-10)
+    return dict(n=n, hd=hd, ddiag=ddiag, it_e=it_e, pp_e=list(pp_e))
+
+
+def _solve(P, torch, case, mode, moduli):
+    n = case["n"]
+    prec = {"fp64": P.PREC_FP64, "int8crt": P.PREC_INT8CRT, "f16split": P.PREC_F16SPLIT,
+            "bf16split": P.PREC_BF16SPLIT}[mode]
+    if moduli:
+        P.set_int8_moduli(moduli)
+    try:
+        rho = torch.zeros_like(case["hd"])
+        it, pp = P.sp2_alg2_device(case["hd"], rho, n, 0.5, 1, 200, "Abs", 1e-10, precision=prec)
+    finally:
+        P.set_int8_moduli(11)
+    r = rho[:n, :n]
+    err = torch.linalg.norm(r - case["ddiag"]).item()
+    tr = abs(torch.trace(r).item() - n)
+    sym = torch.max(torch.abs(r - r.T)).item()
+    return it, pp, err, tr, sym
+
+
+@pytest.fixture(scope="module")
+def c2_4096():
+    return _model_case(4096, 0)
+
+
+# measured on an MI355X (||D - D_diag||_F, seed 0, gap 3.4e-6 eV): FP64 7e-11,
+# int8 CRT 1.6e-7 (10 moduli) / 1.1e-8 (11), FP16 split 5.2e-4, BF16 split 2.0e-2
+C2_BARS = {("fp64", 0): 1e-8, ("int8crt", 10): 1e-6, ("int8crt", 11): 1e-6,
+           ("f16split", 0): 1e-3, ("bf16split", 0): 5e-2}
+
+
+@pytest.mark.parametrize("mode,moduli", list(C2_BARS))
+def test_c2_n4096_every_precision(c2_4096, mode, moduli):
+    """C2 at N = 4096: the density against D_diag within the mode's bar, the
+    trace within 1e-8 (split modes: after their FP64 polish, 1e-6), exact
+    symmetry, and the branch sequence of the eigenvalue control flow on its
+    first 60 steps (near convergence the limdiff test reads values at the
+    products' rounding level; the gap is 3.4e-6 eV)."""
+    import torch
+    import progress_amd as P
+    print(f"[C2 n=4096] {mode} moduli={moduli}", flush=True)
+    it, pp, err, tr, sym = _solve(P, torch, c2_4096, mode, moduli)
+    print(f"  iterations {it} (eigenvalue flow {c2_4096['it_e']}), ||D-D_diag||_F {err:.2e}, "
+          f"|Tr D - N| {tr:.1e}", flush=True)
+    assert err <= C2_BARS[(mode, moduli)]
+    assert tr <= (1e-8 if mode in ("fp64", "int8crt") else 1e-6)
+    assert sym == 0.0
+    assert pp[:60] == c2_4096["pp_e"][:60]
+
+
+@pytest.fixture(scope="module")
+def c2_8192():
+    return _model_case(8192, 0)
+
+
+@pytest.mark.parametrize("mode,bar", [("f16split", 1e-3), ("bf16split", 5e-2)])
+def test_c2_n8192_split_modes(c2_8192, mode, bar):
+    """C2's split modes at N = 8192 (FP16 / BF16 hi+lo products with FP32
+    accumulation, noise-floor stop, FP64 polish): against D_diag at their
+    measured accuracy (5.7e-4 FP16, 2.8e-2 BF16; neither meets the headline's
+    1e-6, which the int8 CRT products do), trace within 1e-6, exact symmetry,
+    the eigenvalue flow's branch sequence on the first 40 steps."""
+    import torch
+    import progress_amd as P
+    print(f"[C2 n=8192] {mode}", flush=True)
+    it, pp, err, tr, sym = _solve(P, torch, c2_8192, mode, 0)
+    print(f"  iterations {it}, ||D-D_diag||_F {err:.2e}, |Tr D - N| {tr:.1e}", flush=True)
+    assert err <= bar and tr <= 1e-6 and sym == 0.0
+    assert pp[:40] == c2_8192["pp_e"][:40]
+
+
+@pytest.fixture(scope="module")
+def water():
+    """The C3 system: 10,000 atoms x 3 orbitals at water number density, the
+    graph |D| >= 0.11 of an int8 CRT dense solve (11 moduli), 512 BFS parts."""
+    import torch
+    import progress_amd as P
+    from progress_amd import workloads as W
+    rows, cols, vals, n, _ = W.water_box(natoms=10000, seed=0)
+    ld = P.padded_ld(n)
+    hd = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+    hd[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()] = \
+        torch.from_numpy(vals).cuda()
+    rd = torch.zeros_like(hd)
+    P.set_int8_moduli(11)
+    it, pp = P.sp2_alg2_device(hd, rd, n, 0.5, 1, 200, "Abs", 1e-10, precision=P.PREC_INT8CRT)
+    lo, hi = P.gershgorin_device(hd, n)
+    g = (rd[:n, :n].abs() >= 0.11).to(torch.float64).cpu().numpy()
+    h = hd[:n, :n].cpu().numpy()
+    del hd, rd
+    torch.cuda.empty_cache()
+    return dict(n=n, h=h, g=g, it=it, pp=pp, lo=lo, hi=hi)
+
+
+def test_c3_water_box_full_size(water):
+    """C3 at full size (30,000 orbitals, 512 parts): every core+halo list of
+    the GPU loop equals the oracle's matrix2submatrix_index with H (ordered
+    lists); the core rows of four parts (largest, smallest, two others)
+    equal O.sp2_submatrix on the same blocks to 1e-9; the GPU loop on a
+    partition of just those four cores gives the oracle's vv for them
+    (1e-9 relative)."""
+    import progress_amd as P
+    n, h, g = water["n"], water["h"], water["g"]
+    it, pp, lo, hi = water["it"], water["pp"], water["lo"], water["hi"]
+    print(f"[C3 water box] n={n} parts=512 iterations={it}", flush=True)
+    gp = P.bfs_partition(g, 512)
+    mh, mg = P.Matrix.from_numpy(h), P.Matrix.from_numpy(g)
+    rho, _ = P.subgraph_sp2_loop(mh, mg, gp, pp, it, lo, hi)
+    cores, halos = gp.parts(), gp.halos()
+    sizes = [len(x) for x in halos]
+    print(f"  lsize {min(sizes)}..{max(sizes)} (mean {np.mean(sizes):.0f})", flush=True)
+    for c, lst in zip(cores, halos):
+        idx, nc = O.matrix2submatrix_index(g, c, True, h)
+        assert lst == idx and nc == len(c)
+    order = np.argsort(sizes)
+    sample = [int(order[-1]), int(order[0]), int(order[len(order) // 2]), 137]
+    vv_o = [0.0] * it
+    for p in sample:
+        idx, nc = halos[p], len(cores[p])
+        d, vv_o = O.sp2_submatrix(h[np.ix_(idx, idx)], pp, it, vv_o, lo, hi, nc)
+        got = rho[np.ix_(idx[:nc], idx)]
+        assert np.max(np.abs(got - d[:nc, :])) < 1e-9
+    gps = P.set_partition([cores[p] for p in sample], n)
+    _, vv_g = P.subgraph_sp2_loop(mh, mg, gps, pp, it, lo, hi)
+    assert np.allclose(vv_g, np.sqrt(vv_o), rtol=1e-9, atol=1e-12)
+
+
+def test_c5_implicit_fermi_adjoint_n8192():
+    """C5 at N = 8192 (beta 20, 8 levels, Newton-Schulz to 1e-10, mu 0.671):
+    (1) the fixed-mu gradient dTr(P A)/dH against FP64 central differences
+    (step 1e-4) along A and along a random symmetric direction, to 1e-6 of
+    the gradient norm; (2) the canonical gradient against the first-order
+    response identity Tr(E dG) = Tr(A P1[E]), P1 from
+    prg_implicit_fermi_first_order_response with the inverses of
+    prg_implicit_fermi_save_inverse at the same chemical potential, along
+    E = A (1e-8 relative)."""
+    import torch
+    import progress_amd as P
+    from progress_amd.autograd import implicit_fermi_density
+    n, nsteps, beta = 8192, 8, 20.0
+    print(f"[C5 implicit-Fermi adjoint] n={n}", flush=True)
+    h = O.model_hamiltonian(n, 0)
+    a = O.banded_perturbation(n, 32, seed=1)
+    inv = [P.Matrix.from_numpy(np.eye(n)) for _ in range(nsteps)]
+    _, mu, _, _ = P.implicit_fermi_save_inverse(inv, h, nsteps, 0.5 * n, 0.671, beta, 1e-6,
+                                                1e-10, 1)
+    at = torch.tensor(a, device="cuda")
+    grads = {}
+    for canonical in (False, True):
+        ht = torch.tensor(h, device="cuda", requires_grad=True)
+        p = implicit_fermi_density(ht, nsteps, mu, beta, 1e-10, canonical)
+        torch.sum(p * at).backward()
+        grads[canonical] = ht.grad.detach()
+        del ht, p
+    g = grads[False]
+    assert torch.equal(g, g.T)
+    hd = _device_matrix(P, torch, h)
+    ld = hd.shape[0]
+
+    def loss(e, s):
+        hp = hd.clone()
+        hp[:n, :n] += s * e
+        pp = torch.zeros((ld, ld), dtype=torch.float64, device="cuda")
+        P.implicit_fermi_device(hp, pp, n, nsteps, mu, beta, 1e-10)
+        return float(torch.sum(pp[:n, :n] * at))
+
+    rng = np.random.default_rng(5)
+    r = torch.tensor(rng.standard_normal((n, n)), device="cuda")
+    gn = float(torch.linalg.norm(g))
+    for e in (at, 0.5 * (r + r.T)):
+        e = e / torch.linalg.norm(e)
+        fd = (loss(e, 1e-4) - loss(e, -1e-4)) / 2e-4
+        an = float(torch.sum(g * e))
+        print(f"  FD {fd:.10e} adjoint {an:.10e}", flush=True)
+        assert abs(fd - an) <= 1e-6 * gn
+    _, p1 = P.implicit_fermi_first_order_response(h, a, inv, nsteps, mu, beta, 0.5 * n)
+    lhs = float(torch.sum(grads[True] * at))
+    rhs = float(np.sum(a * p1))
+    print(f"  canonical: Tr(A dG) {lhs:.10e}, Tr(A P1[A]) {rhs:.10e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-8 * abs(rhs)

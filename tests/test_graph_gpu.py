@@ -161,3 +161,35 @@ def test_build_density_gp_t0_one_part_is_exact(P):
     rho_g, _ = P.build_density_gp_t0(h, g, 0.5, 0.2, 1)
     d, _ = O.build_density_T_fermi(h, 0.1, 0.2)
     assert np.linalg.norm(rho_g - d) < 1e-9
+
+
+@pytest.mark.parametrize("mode", ["f16split", "bf16split"])
+def test_subgraph_loop_split_precision(P, wide_case, mode):
+    """The batched loop with FP16 / BF16 hi+lo split products (prg_set_precision
+    F16SPLIT / BF16SPLIT; three 16-bit MFMA products with FP32 accumulation
+    per tile, FP64 update and core-row sums) on the n = 1200, 8-part case:
+    the same core+halo lists as the FP64 loop, and the density and vv of the
+    oracle's FP64 loop within the split products' accuracy (relative
+    Frobenius bars 1e-3 FP16, 3e-2 BF16; vv within 1e-3 / 3e-2 of its
+    largest entry)."""
+    print(f"[subgraph split] {mode}", flush=True)
+    c = wide_case
+    gp64 = P.equal_partition(150, c["n"])
+    P.subgraph_sp2_loop(c["h"], c["g"], gp64, c["seq"], c["imax"], c["lo"], c["hi"])
+    prec = {"f16split": P.PREC_F16SPLIT, "bf16split": P.PREC_BF16SPLIT}[mode]
+    P.set_precision(prec)
+    try:
+        gp = P.equal_partition(150, c["n"])
+        rho_g, vv_g = P.subgraph_sp2_loop(c["h"], c["g"], gp, c["seq"], c["imax"], c["lo"],
+                                          c["hi"])
+        cfg = P.subgraph_config()
+    finally:
+        P.set_precision(P.PREC_FP64)
+    assert cfg[0] == 64
+    assert gp.halos() == gp64.halos()
+    rel = np.linalg.norm(rho_g - c["rho"]) / np.linalg.norm(c["rho"])
+    vv_o = np.asarray(c["vv"])
+    dvv = np.max(np.abs(np.asarray(vv_g) - vv_o)) / vv_o.max()
+    print(f"  rel. Frobenius {rel:.2e}, vv {dvv:.2e}", flush=True)
+    bar = 1e-3 if mode == "f16split" else 3e-2
+    assert rel < bar and dvv < bar
