@@ -58,6 +58,12 @@ def parse():
     ap.add_argument("--nparts", type=int, default=512)
     ap.add_argument("--graph-threshold", type=float, default=0.11,
                     help="|D| threshold of the density graph (graph workload)")
+    ap.add_argument("--water", default="gapless", choices=["gapless", "gapped"],
+                    help="graph workload: on-site energies (-0.5, 0, 0.5) eV at half filling "
+                         "(gapless) or (-8, 4, 4) eV with one occupied orbital per atom "
+                         "(a 5 eV gap, so the thresholded density graph can hold D)")
+    ap.add_argument("--graph-precision", default="fp64", choices=["fp64", "f16split", "bf16split"],
+                    help="graph workload: products of the batched subgraph kernel")
     ap.add_argument("--products", default="fp64",
                     help="fermi workload: arithmetic of the symmetric products, fp64 or "
                          "int8crt<m> (int8 CRT products with m moduli)")
@@ -140,7 +146,12 @@ def main():
     hpad[:n, :n] = h  # symmetric: row/column-major views coincide
     rho = torch.zeros_like(hpad)
 
+    # the library's default moduli count, fixed before any solve: the headline
+    # mode (the D_diag comparison below only gates it)
+    default_moduli = P.get_int8_moduli()
     moduli = [int(m) for m in str(args.moduli).split(",") if m]
+    if default_moduli not in moduli:
+        moduli.append(default_moduli)
     modes = {"fp64": (P.PREC_FP64, 0), "f16split": (P.PREC_F16SPLIT, 0),
              "bf16split": (P.PREC_BF16SPLIT, 0)}
     for m in moduli:
@@ -220,9 +231,15 @@ def main():
                       "meets_1e-6": bool(err <= 1e-6)})
         results[name] = r
 
-    # headline: the fastest mode meeting the accuracy bar (fp64 otherwise)
+    # headline: int8 CRT products with the library's default moduli count
+    # (chosen before the solve); the check against D_diag is a pass/fail gate
+    # (fp64 when it fails).  The fastest mode meeting the bar on this input is
+    # reported beside it, not as the headline: only D_diag selects it.
     ok = [k for k, v in results.items() if "solves_per_s" in v and v.get("meets_1e-6", True)]
-    head = max(ok, key=lambda k: results[k]["solves_per_s"]) if ok else "fp64"
+    fastest_ok = max(ok, key=lambda k: results[k]["solves_per_s"]) if ok else None
+    head = f"int8crt{default_moduli}"
+    if head not in ok:
+        head = "fp64"
     hr = results.get(head, {})
     kind = "int8crt" if head.startswith("int8crt") else head
     hmod = hr.get("moduli")
@@ -276,7 +293,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"prg_sp2_alg2 dense model tight-binding H, N={n}, occ=N/2, "
                                f"idemtol=1e-10, seed={args.seed}",
-                   "n": n, "headline_mode": head, "moduli": hmod, "gap_eV": gap,
+                   "n": n, "headline_mode": head, "moduli": hmod,
+                   "moduli_rule": "library default, fixed before the solve; D_diag gates only",
+                   "fastest_mode_meeting_bar": fastest_ok, "gap_eV": gap,
                    "eigh_s": t_eigh},
         "tflops": hr.get("alg_tflops"),
         "modes": results,
@@ -321,7 +340,10 @@ def graph_bench(args, P, torch, np):
     Accuracy: the assembled density against the dense solve."""
     from progress_amd import workloads as W
     t0 = time.time()
-    rows, cols, vals, n, box = W.water_box(natoms=args.natoms, seed=args.seed)
+    gapped = args.water == "gapped"
+    onsite = (-8.0, 4.0, 4.0) if gapped else (-0.5, 0.0, 0.5)
+    bndfil = 1.0 / 3.0 if gapped else 0.5
+    rows, cols, vals, n, box = W.water_box(natoms=args.natoms, onsite=onsite, seed=args.seed)
     t_gen = time.time() - t0
     dev = torch.device("cuda:0")
     ld = P.padded_ld(n)
@@ -331,7 +353,8 @@ def graph_bench(args, P, torch, np):
     rd = torch.zeros_like(hd)
     P.set_int8_moduli(max(int(m) for m in str(args.moduli).split(",") if m))
     t0 = time.time()
-    it, pp = P.sp2_alg2_device(hd, rd, n, 0.5, 1, 200, "Abs", 1e-10, precision=P.PREC_INT8CRT)
+    it, pp = P.sp2_alg2_device(hd, rd, n, bndfil, 1, 200, "Abs", 1e-10,
+                               precision=P.PREC_INT8CRT)
     torch.cuda.synchronize()
     t_dense = time.time() - t0
     lo, hi = P.gershgorin_device(hd, n)
@@ -346,16 +369,22 @@ def graph_bench(args, P, torch, np):
     del g
     times = []
     walls = []
-    for k in range(args.warmup + args.steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, vv = P.subgraph_sp2_loop(mh, mg, gp, pp, it, lo, hi, out=mr)
-        wall = time.perf_counter() - t0
-        ga, lp, sc = P.subgraph_timing()
-        if k >= args.warmup:
-            times.append((ga + lp + sc) * 1e-3)
-            walls.append(wall)
-            last = (ga, lp, sc)
+    gprec = {"fp64": P.PREC_FP64, "f16split": P.PREC_F16SPLIT,
+             "bf16split": P.PREC_BF16SPLIT}[args.graph_precision]
+    P.set_precision(gprec)
+    try:
+        for k in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, vv = P.subgraph_sp2_loop(mh, mg, gp, pp, it, lo, hi, out=mr)
+            wall = time.perf_counter() - t0
+            ga, lp, sc = P.subgraph_timing()
+            if k >= args.warmup:
+                times.append((ga + lp + sc) * 1e-3)
+                walls.append(wall)
+                last = (ga, lp, sc)
+    finally:
+        P.set_precision(P.PREC_FP64)
     rho = mr.numpy()
     ls = [gp.lsize[i] for i in range(gp.totalParts)]
     core = [gp.llsize[i] for i in range(gp.totalParts)]
@@ -365,6 +394,8 @@ def graph_bench(args, P, torch, np):
     panel, depth, kslab, launch = P.subgraph_config()
     mpad = [max(64, ((m + 63) // 64) * 64) for m in ls]
     executed = sum(batched_executed_flops(m, panel) for m in mpad) * it
+    if args.graph_precision != "fp64":
+        executed *= 3.0  # hi.hi, hi.lo and lo.hi 16-bit products per tile
     loop_s = last[1] * 1e-3
     dd = d_dense.cpu().numpy()
     err = float(np.linalg.norm(rho - dd))
@@ -374,8 +405,10 @@ def graph_bench(args, P, torch, np):
     eb_err = abs(float(np.sum(rho * h)) - float(np.sum(dd * h)))
     achieved = executed / loop_s / 1e12
     cpu = None if args.no_cpu else graph_cpu_baseline(h, gp, pp, it, lo, hi)
+    kdesc = {"fp64": "batched FP64 MFMA", "f16split": "batched FP16 hi+lo split MFMA",
+             "bf16split": "batched BF16 hi+lo split MFMA"}[args.graph_precision]
     out = {
-        "metric": "graph-partitioned SP2 solves/s (prg_subgraphSP2Loop, batched FP64 MFMA)",
+        "metric": f"graph-partitioned SP2 solves/s (prg_subgraphSP2Loop, {kdesc})",
         "value": 1.0 / dt,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -385,9 +418,11 @@ def graph_bench(args, P, torch, np):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "f16split": "f16 (hi+lo split), f64 update",
+                  "bf16split": "bf16 (hi+lo split), f64 update"}[args.graph_precision],
         "data": "synthetic",
-        "config": {"workload": f"water box: {args.natoms} atoms x 3 orbitals = {n} orbitals, "
+        "config": {"workload": f"water box ({args.water}: on-site {onsite} eV, occupation "
+                               f"{bndfil:.4f} N): {args.natoms} atoms x 3 orbitals = {n} orbitals, "
                                f"box {box:.2f} A, hopping -exp(-r/1.5) (r < 4 A), graph "
                                f"|D| >= {args.graph_threshold}, {args.nparts} BFS parts, "
                                f"double-jump halos, seed {args.seed}",
@@ -406,9 +441,16 @@ def graph_bench(args, P, torch, np):
                   "count": sum(batched_tile_count(m, panel) for m in mpad),
                   "launch": {0: "eager", 1: "captured hipGraph", 2: "hipGraph replay"}[launch]},
         "roofline": {"bound": "mfma",
-                     "kernel": f"batched_sp2_kernel (FP64 MFMA, {panel}-wide panels)",
-                     "achieved": achieved, "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp64_tflops"],
+                     "kernel": (f"batched_sp2_kernel (FP64 MFMA, {panel}-wide panels)"
+                                if args.graph_precision == "fp64" else
+                                f"batched split-product kernel ({args.graph_precision}, 3 "
+                                f"16-bit products per tile)"),
+                     "achieved": achieved,
+                     "peak": PEAK["fp64_tflops" if args.graph_precision == "fp64" else
+                                  "fp16_tflops"],
+                     "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp64_tflops" if args.graph_precision == "fp64"
+                                             else "fp16_tflops"],
                      "traffic": traffic_bytes("graph", n, None) if (panel, depth, kslab) ==
                      (64, 2, 16) else None,
                      "algorithmic_tflops": alg / loop_s / 1e12,

@@ -365,6 +365,18 @@ void prg_bfsPartition(prg_graph_partitioning_t *gp, bml_matrix_t *g_bml, int npa
 void prg_build_densityGP_T0(bml_matrix_t *ham_bml, bml_matrix_t *g_bml, bml_matrix_t *rho_bml,
                             double threshold, double bndfil, double Ef, int nparts, int verbose);
 double prg_build_densityGP_T0_ef(void);
+/* src/prg_progress_mod.h:814-820 (declared there without an implementation;
+ * Fortran src/prg_graphsolver_mod.F90:202-331): the congruence transform
+ * Z = S^-1/2 assembled from parts.  The graph g_bml is partitioned into
+ * nparts parts (the library's BFS partitioner in place of METIS); for each
+ * part the Loewdin factor U s^-1/2 U^T of the overlap's double-jump
+ * core+halo block (prg_buildZdiag, src/prg_genz_mod.F90:176-276) is formed
+ * from batched eigensolves and its core rows are written into zmat_bml
+ * (bml_submatrix2matrix); other entries of zmat_bml keep their values.  A
+ * block that is not positive definite records a numeric error (the
+ * reference stops).  threshold is ignored (dense storage). */
+void prg_build_zmatGP(bml_matrix_t *over_bml, bml_matrix_t *g_bml, bml_matrix_t *zmat_bml,
+                      double threshold, int nparts, int verbose);
 /* Device time (ms) of the last prg_subgraphSP2Loop phases. */
 void prg_gpu_subgraph_timing(double *ms_gather, double *ms_loop, double *ms_scatter);
 /* Kernel configuration of the last prg_subgraphSP2Loop: the tile panel width
@@ -396,17 +408,25 @@ void prg_gpu_subgraph_config(int *panel, int *depth, int *kslab, int *graph);
 #define PRG_PREC_BF16SPLIT 3
 void prg_set_precision(int mode);
 int prg_get_precision(void);
-/* Number of moduli (6..14, default 11) of PRG_PREC_INT8CRT; X is scaled by
- * 2^a with a = min(52, floor((log2 M - 1.001) / 2)): 8 -> 31, 9 -> 35,
- * 10 -> 39, 11 -> 43, 12 -> 46, 13 -> 50, 14 -> 52 bits.  Returns a status. */
+/* Number of moduli (6..14, default 11) of PRG_PREC_INT8CRT.  In the SP2
+ * square X is scaled by 2^a with a = min(52, floor((log2 M + 1 - 0.001) / 2))
+ * (the residues stand for X^2 - X, whose entries are at most 1/4): 8 -> 32,
+ * 9 -> 36, 10 -> 40, 11 -> 44, 12 -> 47, 13 -> 51, 14 -> 52 bits; general
+ * products use a = min(52, floor((log2 M - 1.001) / 2)), one bit less.
+ * Returns a status. */
 int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 /* Configuration the int8 CRT product kernel last ran with, coded
- * waves * 100 + ring stages (803, 804, 805: 8 waves; 404, 405: 4 waves):
+ * waves * 100 + ring stages (803, 804, 805: 8 waves; 404, 405: 4 waves;
+ * 903, 904, 905: the persistent 8-wave kernel, one workgroup per CU):
  * chosen per (padded size, moduli) by timing each on the first product, or
  * forced with the environment variable PRG_CRT_CONFIG.  0 before any int8
  * product.  Measurement support; no reference counterpart. */
 int prg_gpu_int8_waves(void);
+/* Forces the int8 CRT product kernel configuration (one of the codes above;
+ * 0 returns to timing them per size).  Returns a status (1 for an unknown
+ * code).  Test and measurement support; no reference counterpart. */
+int prg_gpu_set_int8_config(int config);
 
 /* ------------------------------------------------------------------ device-pointer API */
 /* prg_sp2_alg2 on device-resident data (no host copies): dH and dRho are

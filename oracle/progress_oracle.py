@@ -196,10 +196,21 @@ def sp2_basic_tcore(h, bndfil, maxsp2iter):
     return 2.0 * x, pp
 
 
-def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace, want_vv):
+def _drop(a, threshold):
+    """A sparse BML format's thresholding: entries with |a| <= threshold are
+    dropped (lanl/bml, absent here: parity unpinned beyond this rule)."""
+    return np.where(np.abs(a) > threshold, a, 0.0) if threshold > 0.0 else a
+
+
+def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace, want_vv,
+                   threshold=0.0):
     """Shared body of prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380) and
     prg_sp2_alg2_genseq (:395-533).  ``fresh_trace`` selects genseq's
-    ``trx = trace(1)`` (:452) instead of the recursed trace (:334,341)."""
+    ``trx = trace(1)`` (:452) instead of the recursed trace (:334,341).
+    threshold > 0 is the sparse formats' arithmetic: bml_multiply_x2 drops
+    the product's entries at or below it (its trace taken before, as the
+    product is formed) and bml_add_deprecated drops those of 2X - X2
+    (:310, :328)."""
     trx = np.trace(x)
     it = 0
     pp, vv = [], []
@@ -210,6 +221,7 @@ def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace
         if fresh_trace:
             trx = np.trace(x)
         trx2 = np.trace(x2)
+        x2 = _drop(x2, threshold)
         if want_vv:
             vv.append(math.sqrt(float(np.sum((x - x2) ** 2))))
         tr2xx2 = 2.0 * trx - trx2
@@ -217,7 +229,7 @@ def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace
         limdiff = abs(trx2 - occ) - abs(tr2xx2 - occ)
         brk = False
         if limdiff >= idemtol:
-            x = 2.0 * x - x2
+            x = _drop(2.0 * x - x2, threshold)
             trx = 2.0 * trx - trx2
             pp.append(0)
         elif limdiff < -idemtol:
@@ -239,11 +251,12 @@ def _sp2_alg2_core(x, occ, minsp2iter, maxsp2iter, sp2conv, idemtol, fresh_trace
     return x, it, pp, vv
 
 
-def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol):
-    """prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380). Returns (rho, iterations, pp)."""
+def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0):
+    """prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380). Returns (rho, iterations, pp).
+    threshold > 0: the arithmetic of a sparse BML format (see _sp2_alg2_core)."""
     occ = bndfil * h.shape[0]
     x, it, pp, _ = _sp2_alg2_core(normalize(h), occ, minsp2iter, maxsp2iter, sp2conv,
-                                  idemtol, False, False)
+                                  idemtol, False, False, threshold)
     return 2.0 * x, it, pp
 
 
@@ -1281,26 +1294,35 @@ def sp2_entropy_ts(d0, gg, ee):
 CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199)
 
 
-def crt_constants(nmod):
-    """Moduli, q_k = (M/p_k)^-1 mod p_k and the scaling exponent a (largest with
-    2^(2a) <= M/8) of the int8 Chinese-remainder products (no reference
-    counterpart: the build's FP64-accurate mixed-precision X^2)."""
+def crt_constants(nmod, square=True):
+    """Moduli, q_k = (M/p_k)^-1 mod p_k and the scaling exponent a of the int8
+    Chinese-remainder products (no reference counterpart: the build's
+    FP64-accurate mixed-precision products).
+
+    square=True (the SP2 square of an iterate with spectrum in [0, 1]): the
+    residues stand for C - 2^a A = 2^(2a) (X~^2 - X~), whose entries lie in
+    [-2^(2a)/4, 2^(2a)/4] (x^2 - x in [-1/4, 0] on [0, 1]; an entry of a
+    symmetric matrix is at most its spectral norm), so the largest a with
+    2^(2a - 1 + 0.001) <= M.  square=False (general product of operands
+    scaled to row 2-norm <= 1): |C| <= 2^(2a) (1 + 1e-10), so 2^(2a + 1.001)
+    <= M.  a <= 52 either way."""
     ps = CRT_MODULI[:nmod]
     m = 1
     for p in ps:
         m *= p
     qs = [pow(m // p % p, -1, p) for p in ps]
-    # largest a <= 52 with 2^(2a + 1.001) <= M: |C| <= 2^(2a) (1 + 1e-10) < M / 2
-    # for rows of X of 2-norm at most 1
-    a = min(52, int(math.floor((math.log2(m) - 1.001) / 2)))
+    slack = -1.001 if square else 1.001
+    a = min(52, int(math.floor((math.log2(m) - slack) / 2)))
     return ps, qs, a, m
 
 
 def crt_square(x, nmod):
-    """X^2 the way the int8 CRT kernel forms it: A = rint(2^a X), per modulus the
-    exact integer product of the symmetric residues, the fraction sum of
-    ((C mod p) q mod p) / p over the moduli, reduced mod 1, times M / 2^(2a).
-    Integer products in int64 (exact: |sum| < K 128^2)."""
+    """X^2 the way the int8 CRT SP2 step forms it: A = rint(2^a X), per modulus
+    the exact integer product of the symmetric residues, the fraction sum
+    S = sum_k ((C mod p) q mod p) / p = C / M (mod 1); then
+    (C - 2^a A) / M = S - A 2^a / M (mod 1), which lies in (-1/2, 1/2), and
+    X~^2 = (C - 2^a A) / 2^(2a) + A 2^-a.  Integer products in int64 (exact:
+    |sum| < K 128^2)."""
     ps, qs, a, m = crt_constants(nmod)
     av = np.rint(np.asarray(x) * 2.0 ** a)
     s = np.zeros_like(av)
@@ -1312,8 +1334,9 @@ def crt_square(x, nmod):
         v = (c * q).astype(np.float64)
         md = v - p * np.rint(v / p)
         s += md / p
-    f = s - np.rint(s)
-    return f * (float(m) / 4.0 ** a)
+    xa = av * 2.0 ** -a
+    t = s - xa * (4.0 ** a / float(m))
+    return (t - np.rint(t)) * (float(m) / 4.0 ** a) + xa
 
 
 def crt_scale(x):
@@ -1333,7 +1356,7 @@ def crt_product(x, y, nmod):
     mod 1, times M sX sY / 2^(2a); the upper triangle mirrored onto the lower.
     Same integer arithmetic as crt_square, which is the case X = Y with both
     scales 1."""
-    ps, qs, a, m = crt_constants(nmod)
+    ps, qs, a, m = crt_constants(nmod, square=False)
     sx, sy = crt_scale(x), crt_scale(y)
     xv = np.rint(np.asarray(x) * (2.0 ** a / sx))
     yv = np.rint(np.asarray(y) * (2.0 ** a / sy))
@@ -1433,6 +1456,31 @@ def build_density_gp_t0(h, g, bndfil, ef, parts, kbt=0.1):
     if error > 1e-6 and abs(dnocc) > 1e-6:
         ef = ef - (nocc / dnocc if error > 1.0 else error / dnocc)
     return rho, ef
+
+
+def buildzdiag(s):
+    """prg_buildZdiag (src/prg_genz_mod.F90:176-276): Z = U s^-1/2 U^T from the
+    eigen-decomposition of the overlap S (the Loewdin factor; a negative
+    eigenvalue is the reference's 'not positive definite' stop)."""
+    w, u = np.linalg.eigh(s)
+    if np.min(w) <= 0.0:
+        raise ValueError("overlap block is not positive definite")
+    return (u / np.sqrt(w)) @ u.T
+
+
+def build_zmat_gp(s, g, parts, z0=None):
+    """prg_build_zmatGP (src/prg_graphsolver_mod.F90:202-331) on one rank for
+    a given partition: the double-jump core+halo lists of the graph alone
+    (:282-285, no Hamiltonian), per part prg_buildZdiag of the overlap's
+    core+halo block (:306), its core rows written into zmat
+    (bml_submatrix2matrix, :308); entries not written keep z0's value."""
+    n = s.shape[0]
+    z = np.zeros((n, n)) if z0 is None else np.array(z0, dtype=float)
+    for core in parts:
+        idx, llsize = matrix2submatrix_index(g, core, True)
+        zp = buildzdiag(s[np.ix_(idx, idx)])
+        z[np.ix_(idx[:llsize], idx)] = zp[:llsize, :]
+    return z
 
 
 # --------------------------------------------------------------------------

@@ -496,6 +496,36 @@ __global__ void fermi_scale_kernel(const double* __restrict__ v, const double* _
   }
 }
 
+// Loewdin factor of each part: U_p = V_p diag(w_p^-1/2) (the congruence
+// transform Z_p = V_p w_p^-1/2 V_p^T of prg_buildZdiag, formed by the U V^T
+// product after this scaling); wmin[p] = the part's smallest eigenvalue
+// (padding eigenvalues are the gather's positive pad).
+__global__ void invsqrt_scale_kernel(const double* __restrict__ v, const double* __restrict__ w,
+                                     const PartDesc* __restrict__ parts,
+                                     const size_t* __restrict__ w_offset, double* __restrict__ u,
+                                     double* __restrict__ wmin) {
+  const PartDesc pd = parts[blockIdx.y];
+  const double* W = w + w_offset[blockIdx.y];
+  const double* V = v + pd.offset;
+  double* U = u + pd.offset;
+  for (int c = blockIdx.x; c < pd.m; c += gridDim.x) {
+    const double f = W[c] > 0.0 ? 1.0 / sqrt(W[c]) : 0.0;
+    for (int r = threadIdx.x; r < pd.m; r += NT) U[r + (size_t)c * pd.m] = f * V[r + (size_t)c * pd.m];
+  }
+  if (blockIdx.x == 0) {
+    __shared__ double red[NT];
+    double s = INFINITY;
+    for (int c = threadIdx.x; c < pd.m; c += NT) s = fmin(s, W[c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) wmin[blockIdx.y] = red[0];
+  }
+}
+
 // vv[it] += sum of iteration it's tile partials (fixed order); block it.
 __global__ void vv_accumulate_kernel(const double* __restrict__ part, int ntiles,
                                      double* __restrict__ vv) {
@@ -607,9 +637,10 @@ int batched_vv(const BatchPlan& plan, const double* d_part, int iters, double* d
   return OK;
 }
 
-int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale) {
-  hipLaunchKernelGGL(zero_kernel, dim3(2048), dim3(NT), 0, stream(), rho.p,
-                     (size_t)rho.ld * rho.ld);
+int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale, bool zero) {
+  if (zero)
+    hipLaunchKernelGGL(zero_kernel, dim3(2048), dim3(NT), 0, stream(), rho.p,
+                       (size_t)rho.ld * rho.ld);
   hipLaunchKernelGGL(scatter_core_kernel, dim3(64, plan.nparts), dim3(NT), 0, stream(), x,
                      plan.d_parts, plan.d_idx, rho.p, rho.ld, scale);
   PRG_HIP_CHECK(hipGetLastError());
@@ -628,6 +659,14 @@ int batched_fermi_scale(const BatchPlan& plan, const double* v, const double* w,
                         double* d_dnocc) {
   hipLaunchKernelGGL(fermi_scale_kernel, dim3(64, plan.nparts), dim3(NT), 0, stream(), v, w,
                      plan.d_parts, d_w_offset, u, ef, kbt, d_dnocc);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
+}
+
+int batched_invsqrt_scale(const BatchPlan& plan, const double* v, const double* w,
+                          const size_t* d_w_offset, double* u, double* d_wmin) {
+  hipLaunchKernelGGL(invsqrt_scale_kernel, dim3(64, plan.nparts), dim3(NT), 0, stream(), v, w,
+                     plan.d_parts, d_w_offset, u, d_wmin);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }

@@ -593,14 +593,26 @@ int scale_cols_kernel_launch(const DMat& A, const double* d_f, DMat out) {
 namespace {
 }  // namespace
 
-int gershgorin(const DMat& h, double* d_bounds) {
+namespace {
+constexpr int kGershChunk = 512;
+}
+
+size_t gershgorin_scratch_doubles(int n) {
+  const size_t nchunks = (n + kGershChunk - 1) / kGershChunk;
+  const size_t rblocks = (n + NT - 1) / NT;
+  return nchunks * n + 2 * rblocks;
+}
+
+int gershgorin(const DMat& h, double* d_bounds, double* scratch) {
   const int n = h.n;
-  const int cchunk = 512;
+  const int cchunk = kGershChunk;
   const int nchunks = (n + cchunk - 1) / cchunk;
   const int rblocks = (n + NT - 1) / NT;
   int st = OK;
-  double* rs = (double*)workspace("gersh_rs", sizeof(double) * (size_t)nchunks * n, &st);
-  double* bmm = (double*)workspace("gersh_bmm", sizeof(double) * 2 * rblocks, &st);
+  double* rs = scratch ? scratch
+                       : (double*)workspace("gersh_rs", sizeof(double) * (size_t)nchunks * n, &st);
+  double* bmm = scratch ? scratch + (size_t)nchunks * n
+                        : (double*)workspace("gersh_bmm", sizeof(double) * 2 * rblocks, &st);
   if (st) return st;
   hipLaunchKernelGGL(gersh_rows_kernel, dim3(rblocks, nchunks), dim3(NT), 0, stream(), h.p, n,
                      h.ld, cchunk, rs);

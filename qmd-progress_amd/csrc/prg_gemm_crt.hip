@@ -77,7 +77,9 @@ struct CrtConst {
   double scale;            // 2^a
   double iscale;           // 2^-a
   double cscale;           // M / 2^(2a)
+  double icscale;          // 2^(2a) / M
   int recon;               // X is carried by its residues alone (a <= 45)
+  int shift;               // SP2 square: the residues stand for C - 2^a A
 };
 
 struct ProductArgs {
@@ -88,6 +90,7 @@ struct ProductArgs {
   const Sp2State* st;       // SP2 state (planned, current buffer), or null: planes[0]
   const double* fold;       // (p, 1/p, q) per modulus
   int pld, np;
+  int nitems;               // items in the list (the persistent kernel's range)
 };
 
 struct UpdateArgs {
@@ -152,120 +155,157 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
 // a row the 16-byte chunk g sits in slot g ^ ((row >> 1) & 3), which keeps
 // the ds_read_b128 of 8 consecutive rows on distinct banks.  The ring is the
 // kernel's only LDS object.
-template <int W, int NSTAGE>
-__global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
-  constexpr int WN = W / 2;            // waves along the tile columns
-  constexpr int MT = 8;                // 16-row MFMA blocks per wave
-  constexpr int NTL = TM / WN / 16;    // 16-column MFMA blocks per wave
-  constexpr int CPW = 32 / W;          // 1 KB DMA chunks per wave per stage
-  __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
-  int cur = 0;
-  if (g.st) {
-    const volatile Sp2State* stv = g.st;
-    if (!stv->planned) return;
-    cur = stv->cur;
+// Per-workgroup constants of the product kernels (W waves, NSTAGE-slot ring).
+template <int W>
+struct ProductLane {
+  static constexpr int WN = W / 2;          // waves along the tile columns
+  static constexpr int MT = 8;              // 16-row MFMA blocks per wave
+  static constexpr int NTL = TM / WN / 16;  // 16-column MFMA blocks per wave
+  static constexpr int CPW = 32 / W;        // 1 KB DMA chunks per wave per stage
+  int lane, w, wm, wn, drow, kc, fo, offa;
+  uint32_t dst0;
+  __device__ ProductLane(const int8_t* ring0) {
+    const int tid = threadIdx.x;
+    lane = tid & 63;
+    w = tid >> 6;
+    wm = w & 1;
+    wn = w >> 1;
+    // DMA chunks of wave w: c = w + W q (q < CPW) is chunk c % 16 of panel
+    // c / 16 (A = rows of tile I, then B = rows of tile J): panel rows
+    // 16 (c % 16) + lane / 4, slot lane % 4.  c / 16 = W q / 16 and the row
+    // offset from 16 w + lane / 4 are compile-time; multiples of 16 rows
+    // leave the swizzle unchanged.
+    drow = 16 * w + (lane >> 2);
+    kc = 16 * ((lane & 3) ^ ((drow >> 1) & 3));
+    dst0 = lds_addr(ring0) + 1024 * w;
+    // LDS read offsets of this lane's fragments: row r = lane % 16 of each
+    // 16-row block, 16-byte chunk lane / 16; the block offsets are multiples
+    // of 8 rows, so the swizzle depends on the lane only.
+    fo = (lane & 15) * BK + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
+    offa = wm * 128 * BK + fo;
   }
-  const int pld = g.pld;
-  const int4 it = g.items[xcd_remap(blockIdx.x, gridDim.x)];
-  const int k = it.x, I = it.y, J = it.z, t = it.w;
-  const int8_t* R = g.planes[cur] + (size_t)k * pld * pld;
-  const int8_t* RB = g.planes_b ? g.planes_b + (size_t)k * pld * pld : R;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w & 1, wn = w >> 1;
+};
 
-  // DMA chunks of wave w: c = w + W q (q < CPW) is chunk c % 16 of panel
-  // c / 16 (A = rows of tile I, then B = rows of tile J): panel rows
-  // 16 (c % 16) + lane / 4, slot lane % 4.  c / 16 = W q / 16 and the row
-  // offset from 16 w + lane / 4 are compile-time; multiples of 16 rows leave
-  // the swizzle unchanged.
-  const int drow = 16 * w + (lane >> 2);
-  const int kc = 16 * ((lane & 3) ^ ((drow >> 1) & 3));
-  const int8_t* srcA = R + (size_t)(I * TM + drow) * BK + kc;
-  const int8_t* srcB = RB + (size_t)(J * TM + drow) * BK + kc;
-  const size_t kstride = (size_t)pld * BK;  // one 64-column block of a plane
-  const uint32_t dst0 = lds_addr(&ring[0][0]) + 1024 * w;
-  const int ksteps = pld / BK;
-  // LDS read offsets of this lane's fragments: row r = lane % 16 of each
-  // 16-row block, 16-byte chunk lane / 16; the block offsets are multiples
-  // of 8 rows, so the swizzle depends on the lane only.
-  const int fo = (lane & 15) * BK + 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
-  const int offa = wm * 128 * BK + fo;
-  i4 acc[MT][NTL];
-  // DIAG (I == J, B = A): both operands are the same panel, which is loaded
-  // once (half the DMA chunks; the B fragments are read from the A panel).
-  auto run = [&](auto diag_tag) {
-    constexpr bool DIAG = decltype(diag_tag)::value;
-    constexpr int NQ = DIAG ? CPW / 2 : CPW;  // DMA chunks per wave per stage
-    const int offb = (DIAG ? 0 : kPanel) + wn * (TM / WN) * BK + fo;
-    auto issue = [&](int s) {
-      const size_t off = (size_t)s * kstride;
-      const uint32_t base = __builtin_amdgcn_readfirstlane(dst0 + kStage * (s % NSTAGE));
+// Operand panels of one (tile, modulus) item for this lane's DMA chunks.
+struct ItemSrc {
+  const int8_t* a;
+  const int8_t* b;
+  bool diag;  // I == J and B = A: one panel, loaded once
+};
+
+template <int W>
+__device__ __forceinline__ ItemSrc item_src(const ProductArgs& g, int cur, const int4& it,
+                                            const ProductLane<W>& L) {
+  const size_t plane = (size_t)g.pld * g.pld;
+  const int8_t* R = g.planes[cur] + (size_t)it.x * plane;
+  const int8_t* RB = g.planes_b ? g.planes_b + (size_t)it.x * plane : R;
+  ItemSrc s;
+  s.a = R + (size_t)(it.y * TM + L.drow) * BK + L.kc;
+  s.b = RB + (size_t)(it.z * TM + L.drow) * BK + L.kc;
+  s.diag = (it.y == it.z) && !g.planes_b;
+  return s;
+}
+
+// Stage s of an item into ring slot s % NSTAGE: NQ 1 KB chunks per wave
+// (half of them, the A panel only, for a diagonal item).
+template <int W, int NSTAGE, bool DIAG>
+__device__ __forceinline__ void issue_stage(const ItemSrc& src, size_t kstride,
+                                            const ProductLane<W>& L, int s) {
+  constexpr int NQ = DIAG ? ProductLane<W>::CPW / 2 : ProductLane<W>::CPW;
+  const size_t off = (size_t)s * kstride;
+  const uint32_t base = __builtin_amdgcn_readfirstlane(L.dst0 + kStage * (s % NSTAGE));
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int panel = (W * q) >> 4;
-        const int rows = 16 * W * q - TM * panel;
-        dma16((panel ? srcB : srcA) + off + rows * BK, base + 1024 * W * q);
-      }
-    };
-    i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
-    auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
-      const int8_t* st = ring[s % NSTAGE];
+  for (int q = 0; q < NQ; ++q) {
+    const int panel = (W * q) >> 4;
+    const int rows = 16 * W * q - TM * panel;
+    dma16((panel ? src.b : src.a) + off + rows * BK, base + 1024 * W * q);
+  }
+}
+
+// The first NSTAGE stages of an item (its prologue).
+template <int W, int NSTAGE>
+__device__ __forceinline__ void issue_prologue(const ItemSrc& src, size_t kstride,
+                                               const ProductLane<W>& L, int ksteps) {
 #pragma unroll
-      for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + offa + 1024 * m);
-#pragma unroll
-      for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb + 1024 * n);
-    };
-    // Iteration s: wait until stage s + 1 has landed in every wave (and this
-    // wave's reads of stage s are done), refill the slot of stage s with
-    // stage s + NSTAGE, then run the MFMAs of stage s while reading stage
-    // s + 1.  The first step starts from zero accumulators (peeled, so that
-    // the loop carries only MFMA results and they stay in AGPRs).
-    auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
-      if (s + NSTAGE - 1 < ksteps)
-        wait_vm_barrier<NQ * (NSTAGE - 2)>();
+  for (int s = 0; s < NSTAGE; ++s) {
+    if (s < ksteps) {
+      if (src.diag)
+        issue_stage<W, NSTAGE, true>(src, kstride, L, s);
       else
-        wait_vm_barrier<0>();
-      if (s + NSTAGE < ksteps) issue(s + NSTAGE);
-      // after the last stage this reads a stale slot that is never used
-      read_frags(s + 1, next);
-      const i4 zero = {0, 0, 0, 0};
+        issue_stage<W, NSTAGE, false>(src, kstride, L, s);
+    }
+  }
+}
+
+// One item's K loop after its prologue was issued: every stage lands through
+// the ring, NSTAGE - 1 stages ahead of the MFMAs.  all_landed: the prologue
+// is complete already (wait for everything once instead of for stage 0).
+// Iteration s: wait until stage s + 1 has landed in every wave (and this
+// wave's reads of stage s are done), refill the slot of stage s with stage
+// s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.  The
+// first step starts from zero accumulators (peeled, so that the loop carries
+// only MFMA results and they stay in AGPRs).
+template <int W, int NSTAGE, bool DIAG>
+__device__ __forceinline__ void item_mainloop(
+    const ItemSrc& src, size_t kstride, const ProductLane<W>& L, int ksteps,
+    const int8_t (*ring)[kStage], bool all_landed,
+    i4 (&acc)[ProductLane<W>::MT][ProductLane<W>::NTL]) {
+  constexpr int MT = ProductLane<W>::MT, NTL = ProductLane<W>::NTL, WN = ProductLane<W>::WN;
+  constexpr int NQ = DIAG ? ProductLane<W>::CPW / 2 : ProductLane<W>::CPW;
+  const int offb = (DIAG ? 0 : kPanel) + L.wn * (TM / WN) * BK + L.fo;
+  i4 fa[MT + NTL], fb[MT + NTL];  // A fragments, then B fragments
+  auto read_frags = [&](int s, i4 (&f)[MT + NTL]) {
+    const int8_t* st = ring[s % NSTAGE];
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
+    for (int m = 0; m < MT; ++m) f[m] = *(const i4*)(st + L.offa + 1024 * m);
 #pragma unroll
-        for (int n = 0; n < NTL; ++n)
-          acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
-                                                            first ? zero : acc[m][n], 0, 0, 0);
-    };
-#pragma unroll
-    for (int s = 0; s < NSTAGE; ++s)
-      if (s < ksteps) issue(s);
-    if (NSTAGE - 1 < ksteps)
-      wait_vm_barrier<NQ * (NSTAGE - 1)>();
+    for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb + 1024 * n);
+  };
+  auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
+    if (s + NSTAGE - 1 < ksteps)
+      wait_vm_barrier<NQ * (NSTAGE - 2)>();
     else
       wait_vm_barrier<0>();
-    read_frags(0, fa);
-    // ksteps = pld / 64 is a multiple of 4
-    step(0, fa, fb, true);
-    step(1, fb, fa, false);
-    for (int s = 2; s < ksteps; s += 2) {
-      step(s, fa, fb, false);
-      step(s + 1, fb, fa, false);
-    }
+    if (s + NSTAGE < ksteps) issue_stage<W, NSTAGE, DIAG>(src, kstride, L, s + NSTAGE);
+    // after the last stage this reads a stale slot that is never used
+    read_frags(s + 1, next);
+    const i4 zero = {0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < NTL; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
+                                                          first ? zero : acc[m][n], 0, 0, 0);
   };
-  if (I == J && !g.planes_b)
-    run(std::true_type{});
+  if (all_landed || NSTAGE - 1 >= ksteps)
+    wait_vm_barrier<0>();
   else
-    run(std::false_type{});
+    wait_vm_barrier<NQ * (NSTAGE - 1)>();
+  read_frags(0, fa);
+  // ksteps = pld / 64 is a multiple of 4
+  step(0, fa, fb, true);
+  step(1, fb, fa, false);
+  for (int s = 2; s < ksteps; s += 2) {
+    step(s, fa, fb, false);
+    step(s + 1, fb, fa, false);
+  }
+}
 
-  // (C q mod p) as symmetric int8.  Operands were swapped, so this lane holds
-  // C[row][col] with row = lane % 16 and col = 4 (lane / 16) + r inside each
-  // 16 x 16 block; the four bytes of r form one dword.  Records follow the
-  // 8-wave layout whatever W is: the 64-column group g = (wn * NTL + n) / 4
-  // and row half wm name record wave wm + 2 g; (record wave, m, lane) holds
-  // the four blocks n % 4 of that group as one 16-byte store (1 KB contiguous
-  // per record wave and m).
+// (C q mod p) as symmetric int8.  Operands were swapped, so this lane holds
+// C[row][col] with row = lane % 16 and col = 4 (lane / 16) + r inside each
+// 16 x 16 block; the four bytes of r form one dword.  Records follow the
+// 8-wave layout whatever W is: the 64-column group g = (wn * NTL + n) / 4
+// and row half wm name record wave wm + 2 g; (record wave, m, lane) holds
+// the four blocks n % 4 of that group as one 16-byte store (1 KB contiguous
+// per record wave and m).
+template <int W>
+__device__ __forceinline__ void item_epilogue(
+    const ProductArgs& g, const int4& it, const ProductLane<W>& L,
+    const i4 (&acc)[ProductLane<W>::MT][ProductLane<W>::NTL]) {
+  constexpr int MT = ProductLane<W>::MT, NTL = ProductLane<W>::NTL;
+  const int k = it.x, t = it.w;
   const double p = g.fold[3 * k], inv = g.fold[3 * k + 1], qm = g.fold[3 * k + 2];
-  i4* dst = (i4*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + lane;
+  i4* dst = (i4*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + L.lane;
 #pragma unroll
   for (int m = 0; m < MT; ++m) {
 #pragma unroll
@@ -282,9 +322,100 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
         }
         out[n4] = b4;
       }
-      const int rw = wm + 2 * (wn * (NTL / 4) + h);
+      const int rw = L.wm + 2 * (L.wn * (NTL / 4) + h);
       dst[(rw * 8 + m) * 64] = out;
     }
+  }
+}
+
+// One (tile, modulus) item per workgroup: C = A_I B_J^T mod p over the full K
+// range (B = A for the SP2 square; for a general product of symmetric A and
+// B, B_J^T is B's column panel J).  W = 8 waves in a 2 x 4 grid of 128 x 64
+// (two waves per SIMD) or W = 4 waves in a 2 x 2 grid of 128 x 128 (one wave
+// per SIMD, a third fewer LDS fragment reads per MFMA); 16 x 16 MFMA blocks,
+// int32 accumulators.  Operand stages are copied global -> LDS directly
+// (global_load_lds, 16 bytes per lane), NSTAGE - 1 stages ahead of the
+// MFMAs.  A stage holds the A panel (256 rows x 64 bytes) then the B panel,
+// as thirty-two 1 KB chunks of 16 rows; within a row the 16-byte chunk g
+// sits in slot g ^ ((row >> 1) & 3), which keeps the ds_read_b128 of 8
+// consecutive rows on distinct banks.  The ring is the kernel's only LDS
+// object.
+template <int W, int NSTAGE>
+__global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
+  __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
+  int cur = 0;
+  if (g.st) {
+    const volatile Sp2State* stv = g.st;
+    if (!stv->planned) return;
+    cur = stv->cur;
+  }
+  const ProductLane<W> L(&ring[0][0]);
+  const int4 it = g.items[xcd_remap(blockIdx.x, gridDim.x)];
+  const ItemSrc src = item_src<W>(g, cur, it, L);
+  const size_t kstride = (size_t)g.pld * BK;  // one 64-column block of a plane
+  const int ksteps = g.pld / BK;
+  i4 acc[ProductLane<W>::MT][ProductLane<W>::NTL];
+  issue_prologue<W, NSTAGE>(src, kstride, L, ksteps);
+  if (src.diag)
+    item_mainloop<W, NSTAGE, true>(src, kstride, L, ksteps, ring, false, acc);
+  else
+    item_mainloop<W, NSTAGE, false>(src, kstride, L, ksteps, ring, false, acc);
+  item_epilogue<W>(g, it, L, acc);
+}
+
+// Persistent form: one workgroup per CU walks its XCD's share of the item
+// list (the same contiguous eighths as xcd_remap gives the one-item form, in
+// strides of the XCD's workgroup count) and issues the next item's prologue
+// before the current item's epilogue, so that the residue arithmetic and
+// stores overlap the next operand loads.  Each item starts with a full wait
+// (the prologue and the previous item's residue stores), so loads and
+// stores never share a partial vmcnt wait.  Every workgroup exits after its
+// last item.
+template <int W, int NSTAGE>
+__global__ __launch_bounds__(64 * W, 1) void crt_product_persistent_kernel(ProductArgs g) {
+  __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
+  int cur = 0;
+  if (g.st) {
+    const volatile Sp2State* stv = g.st;
+    if (!stv->planned) return;
+    cur = stv->cur;
+  }
+  const ProductLane<W> L(&ring[0][0]);
+  // XCD x = blockIdx.x % 8 owns items [lo, hi) (xcd_remap's split of the
+  // list); its workgroups take lo + j, lo + j + nx, ...
+  const int nitems = g.nitems, x = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int nx = gridDim.x >> 3;
+  const int per = nitems / 8, rem = nitems % 8;
+  const int lo = (x < rem) ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+  const int hi = lo + per + (x < rem ? 1 : 0);
+  int idx = lo + j;
+  if (idx >= hi) return;
+  const size_t kstride = (size_t)g.pld * BK;
+  const int ksteps = g.pld / BK;
+  int4 it = g.items[idx];
+  ItemSrc src = item_src<W>(g, cur, it, L);
+  issue_prologue<W, NSTAGE>(src, kstride, L, ksteps);
+  i4 acc[ProductLane<W>::MT][ProductLane<W>::NTL];
+  while (true) {
+    if (src.diag)
+      item_mainloop<W, NSTAGE, true>(src, kstride, L, ksteps, ring, true, acc);
+    else
+      item_mainloop<W, NSTAGE, false>(src, kstride, L, ksteps, ring, true, acc);
+    // every wave has read its last stage (no loads are outstanding): the
+    // ring is free for the next item's prologue
+    wait_vm_barrier<0>();
+    const int nidx = idx + nx;
+    const bool more = nidx < hi;
+    int4 nit = it;
+    if (more) {
+      nit = g.items[nidx];
+      src = item_src<W>(g, cur, nit, L);
+      issue_prologue<W, NSTAGE>(src, kstride, L, ksteps);
+    }
+    item_epilogue<W>(g, it, L, acc);
+    if (!more) break;
+    idx = nidx;
+    it = nit;
   }
 }
 
@@ -394,9 +525,14 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
     double z4[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double sr = S[4 * n + r];
-      const double y = (sr - rint(sr)) * g.c.cscale;  // C / M mod 1, scaled
       const double x = xs[4 * n + r];
+      // X~ = A 2^-a, the operand the product squared (the master rounded to
+      // the residues' grid when an FP64 master is kept)
+      const double xa = RECON ? x : rint(x * g.c.scale) * g.c.iscale;
+      // (C - 2^a A) / M = S - X~ 2^(2a) / M (mod 1) lies in (-1/2, 1/2), and
+      // X~^2 = (C - 2^a A) / 2^(2a) + X~
+      const double sr = fma(-xa, g.c.icscale, S[4 * n + r]);
+      const double y = fma(sr - rint(sr), g.c.cscale, xa);
       const double z = fma(cb, x, ca * y);
       const double dxy = x - y;
       const bool diag = (row == col0 + r);
@@ -752,7 +888,9 @@ long long inv_mod(long long a, long long p) {
   return t < 0 ? t + p : t;
 }
 
-CrtConst make_const(int np) {
+// square: the constants of the SP2 square X^2 (spectrum of X in [0, 1]);
+// otherwise those of a general product of row-norm-scaled operands.
+CrtConst make_const(int np, bool square) {
   CrtConst c{};
   double log2m = 0.0;
   for (int k = 0; k < np; ++k) log2m += std::log2((double)kModuli[k]);
@@ -765,14 +903,22 @@ CrtConst make_const(int np) {
     c.inv[k] = 1.0 / (double)p;
     c.q[k] = (double)inv_mod(mk, p);
   }
-  // |C_ij| <= ||A_i|| ||A_j|| <= 2^(2a) (1 + 1e-10) for rows of X of 2-norm
-  // at most 1 (the spectrum of every SP2 iterate lies in [0, 1]; the
-  // rounding of A adds at most sqrt(ld) / 2 to a row norm).  2^(2a + 1.001)
-  // <= M keeps |C| < M / 2; a <= 52 (X carries 53 bits).
-  const int a = std::min(52, (int)std::floor((log2m - 1.001) / 2.0));
+  // General product: |C_ij| <= ||A_i|| ||A_j|| <= 2^(2a) (1 + 1e-10) for
+  // operand rows of 2-norm at most 2^a (the rounding of A adds at most
+  // sqrt(ld) / 2 to a row norm); 2^(2a + 1.001) <= M keeps |C| < M / 2.
+  // SP2 square: C - 2^a A = 2^(2a) (X~^2 - X~), and X~ = A 2^-a has its
+  // spectrum in [0, 1] up to the rounding of A (the SP2 maps keep [0, 1]),
+  // where x^2 - x lies in [-1/4, 0]; an entry of a symmetric matrix is at
+  // most its spectral norm, so |C_ij - 2^a A_ij| <= 2^(2a) / 4 and
+  // 2^(2a - 1 + 0.001) <= M keeps it below M / 2: one more bit of scale.
+  // a <= 52 (X carries 53 bits).
+  const double slack = square ? -1.001 : 1.001;
+  const int a = std::min(52, (int)std::floor((log2m - slack) / 2.0));
   c.scale = std::ldexp(1.0, a);
   c.iscale = std::ldexp(1.0, -a);
   c.cscale = std::exp2(log2m - 2.0 * a);
+  c.icscale = std::exp2(2.0 * a - log2m);
+  c.shift = square ? 1 : 0;
   // A = rint(2^a X) itself from its first kRecon residues: |A| <= 2^a < M6 / 2
   // and the FP64 fraction sum is off by at most kRecon 2^-53, i.e. < 0.1 after
   // scaling by M6 = 2^47.8, so rint recovers A exactly for a <= 45
@@ -790,13 +936,14 @@ CrtConst make_const(int np) {
 inline int plane_ld(int ld) { return ((ld + TM - 1) / TM) * TM; }
 
 // Product kernel configuration, coded W * 100 + NSTAGE: 8 waves with a 3-,
-// 4- or 5-stage ring (96, 128, 160 KB of LDS) or 4 waves with 4 or 5 stages.
+// 4- or 5-stage ring (96, 128, 160 KB of LDS) or 4 waves with 4 or 5 stages;
+// 900 + NSTAGE is the persistent 8-wave kernel.
 // PRG_CRT_CONFIG forces one; otherwise the first product of each
 // (pld, moduli) pair times every configuration (twice each, the minimum
 // counts) and keeps the fastest.  The product kernel only writes its residue
 // buffer, so the trial launches leave no trace and the results do not depend
 // on the choice.
-constexpr int kConfigs[] = {803, 804, 805, 404, 405};
+constexpr int kConfigs[] = {803, 804, 805, 404, 405, 903, 904, 905};
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 int g_forced = -1;                        // -1: not read yet; 0: autotune
 std::vector<std::array<int, 3>> g_tuned;  // (pld, np, config)
@@ -813,8 +960,36 @@ int forced_config() {
   return g_forced;
 }
 
+// Workgroups of the persistent kernel: one per CU (its ring takes most of
+// the LDS), a multiple of 8 so that every XCD gets the same count.
+int persistent_grid(int nitems) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n < 8)
+      n = 256;
+    cus = n;
+  }
+  const int need = ((nitems + 7) / 8) * 8;
+  return std::max(8, std::min((cus / 8) * 8, need));
+}
+
 void launch_product(const ProductArgs& g, int nitems, int config) {
   switch (config) {
+    case 903:
+      hipLaunchKernelGGL((crt_product_persistent_kernel<8, 3>), dim3(persistent_grid(nitems)),
+                         dim3(512), 0, stream(), g);
+      break;
+    case 904:
+      hipLaunchKernelGGL((crt_product_persistent_kernel<8, 4>), dim3(persistent_grid(nitems)),
+                         dim3(512), 0, stream(), g);
+      break;
+    case 905:
+      hipLaunchKernelGGL((crt_product_persistent_kernel<8, 5>), dim3(persistent_grid(nitems)),
+                         dim3(512), 0, stream(), g);
+      break;
     case 803:
       hipLaunchKernelGGL((crt_product_kernel<8, 3>), dim3(nitems), dim3(512), 0, stream(), g);
       break;
@@ -975,7 +1150,7 @@ double* fold_constants(int np, int* st) {
   double* d = (double*)workspace("crt_fold", sizeof(double) * 3 * kMaxModuli, st);
   const unsigned gen = workspace_generation();
   if (*st || (cached == np && cached_gen == gen)) return d;
-  const CrtConst c = make_const(np);
+  const CrtConst c = make_const(np, false);  // p, 1/p and q only
   double h[3 * kMaxModuli] = {};
   for (int k = 0; k < np; ++k) {
     h[3 * k] = c.p[k];
@@ -1053,7 +1228,7 @@ int crt_prepare(const DMat& x0, int np) {
   int8_t* planes = (int8_t*)workspace("crt_planes", 2 * (size_t)np * plane, &st);
   if (st) return st;
   if (!fold_constants(np, &st)) return st;
-  const CrtConst c = make_const(np);
+  const CrtConst c = make_const(np, true);
 #define PRG_CALL(K) launch_split<K>(x0.p, planes, planes + (size_t)np * plane, ld, pld, c)
   PRG_CRT_DISPATCH(np, PRG_CALL)
 #undef PRG_CALL
@@ -1083,6 +1258,7 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   pa.st = d_state;
   pa.pld = pld;
   pa.np = np;
+  pa.nitems = nb * np;
   pa.fold = fold_constants(np, &st);
   if (st) return st;
   pa.items = item_list(nt, np, &st);
@@ -1097,7 +1273,7 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   ua.part = d_part;
   ua.ld = ld;
   ua.pld = pld;
-  ua.c = make_const(np);
+  ua.c = make_const(np, true);
   const int nblocks = crt_update_blocks(ld);
   if (ntiles) *ntiles = nblocks;
   const bool timed = g_timing;
@@ -1124,7 +1300,7 @@ int crt_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
 
 int crt_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double scale, DMat out,
                int np) {
-  const CrtConst c = make_const(np);
+  const CrtConst c = make_const(np, true);
   if (!c.recon) return sp2_finish(x0, x1, d_state, scale, out, true);
   const int ld = x0.ld, pld = plane_ld(ld);
   const size_t plane = (size_t)pld * pld;
@@ -1138,6 +1314,21 @@ int crt_finish(const DMat& x0, const DMat& x1, const Sp2State* d_state, double s
 }
 
 int crt_last_waves() { return g_last_config; }
+
+int set_crt_config(int config) {
+  forced_config();  // the environment is read first, so that this call wins
+  if (config == 0) {
+    g_forced = 0;
+    return OK;
+  }
+  for (int c : kConfigs)
+    if (c == config) {
+      g_forced = config;
+      return OK;
+    }
+  set_error(ERR_ARG, "int8 CRT products: unknown kernel configuration");
+  return ERR_ARG;
+}
 
 // One operand that a caller promises not to change while held (the matrix of
 // a Newton-Schulz or CG solve): its residue planes and scale are formed once
@@ -1203,7 +1394,7 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
   if (st) return st;
   const int4* items = item_list(nt, np, &st);
   if (st) return st;
-  const CrtConst c = make_const(np);
+  const CrtConst c = make_const(np, false);
   const int nnb = std::max(1, std::min(1024, (C.n + NT / 64 - 1) / (NT / 64)));
   auto scale_split = [&](const DMat& X, int8_t* planes, double* s) -> int {
     hipLaunchKernelGGL(crt_colnorm_kernel, dim3(nnb), dim3(NT), 0, stream(), X.p, ld, X.n, npart);
@@ -1234,6 +1425,7 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
   pa.fold = fold;
   pa.pld = pld;
   pa.np = np;
+  pa.nitems = nb * np;
   if ((st = run_product(pa, nb * np, &g_last_config))) return st;
   PRG_HIP_CHECK(hipGetLastError());
   GemmEpiArgs ea;

@@ -39,8 +39,10 @@ struct Operands {
 
 // Accumulate acc += sum_p A_p[I-panel, :] * B_p[:, J-panel] for one tile.
 // A is read as column-major A[i + k*ld], B as column-major B[k + j*ld].
-__device__ __forceinline__ void mainloop(const Operands& op, int ld, int kdim, int I, int J,
-                                         double (*As)[LROW], double (*Bs)[LROW], d4 acc[4][4]) {
+// K runs over [kbeg, kbeg + kdim) of each product.
+__device__ __forceinline__ void mainloop(const Operands& op, int ld, int kbeg, int kdim, int I,
+                                         int J, double (*As)[LROW], double (*Bs)[LROW],
+                                         d4 acc[4][4]) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6, wm = w & 1, wn = w >> 1;
@@ -52,7 +54,7 @@ __device__ __forceinline__ void mainloop(const Operands& op, int ld, int kdim, i
 
   auto load = [&](int s) {
     const int p = s / ksteps;
-    const int k0 = (s - p * ksteps) * BK;
+    const int k0 = kbeg + (s - p * ksteps) * BK;
     const double* A = p ? op.A[1] : op.A[0];
     const double* B = p ? op.B[1] : op.B[0];
     const double* Ap = A + (size_t)I * BM + ar;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs g) {
   for (int m = 0; m < 4; ++m)
 #pragma unroll
     for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
-  mainloop(g.op, g.ld, g.kdim, I, J, As, Bs, acc);
+  mainloop(g.op, g.ld, 0, g.kdim, I, J, As, Bs, acc);
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
@@ -166,66 +168,60 @@ struct Sp2Args {
   const double* buf[2];  // ping-pong X buffers
   const double* dbuf[2]; // ping-pong D buffers (response step)
   int ld, kdim, nt;
+  int split;             // K slices of the split product (1: fused kernel)
   Sp2State* st;
   double* part;  // per tile [||Xn||^2, Tr Xn, idem, Tr Y]
+  double* acc;   // split product: per (tile, slice) partial accumulators
 };
 
-__global__ __launch_bounds__(NT, 2) void sp2_step_kernel(Sp2Args g) {
-  __shared__ double As[BK][LROW];
-  __shared__ double Bs[BK][LROW];
-  __shared__ int s_flags[4];
-  if (threadIdx.x == 0) {
-    s_flags[0] = g.st->planned;
-    s_flags[1] = g.st->apply;
-    s_flags[2] = g.st->sigma;
-    s_flags[3] = g.st->cur;
-  }
-  __syncthreads();
-  if (!s_flags[0]) return;
-  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
-  const int core = g.st->core;
-  const int variant = g.st->variant;
+// The SP2 update fused into the epilogue of tile (I, J) of Y = X^2: z =
+// cb x + ca y with (ca, cb) = (1, 0) for X^2, (-1, 2) for 2X - X^2, (0, 1)
+// when the product is only measured (X unchanged); the tile's partial sums
+// [||Xn||^2, Tr Xn, ||X - X^2||^2 (core rows for submatrix), Tr Y] go to
+// part[4 t].
+// MN of the tile's 16-row MFMA groups, starting at group m0 (MN = 4: the
+// whole tile); the partial sums go to part[4 slot].
+template <bool THR, int MN>
+__device__ __forceinline__ void sp2_epilogue_impl(const Sp2Args& g, const Sp2State* st, int slot,
+                                                  int I, int J, const d4 (&acc)[MN][4], int m0,
+                                                  int apply, int sigma, int cur) {
+  const int core = st->core;
+  const int variant = st->variant;
   const double* X = g.buf[cur];
   double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
-
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  int I, J;
-  sym_tile(t, I, J);
-  d4 acc[4][4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
-  Operands op;
-  op.A[0] = X; op.B[0] = X; op.A[1] = X; op.B[1] = X; op.npair = 1;
-  mainloop(op, g.ld, g.kdim, I, J, As, Bs, acc);
-
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
   const int ld = g.ld;
   const bool mirror = (I != J);
   const bool core_only = (variant == V_SUBMATRIX);
-  // z = cb*x + ca*y: (ca, cb) = (1, 0) for X^2, (-1, 2) for 2X - X^2,
-  // (0, 1) when the product is only measured (X unchanged).
   const double ca = apply ? (sigma ? 1.0 : -1.0) : 0.0;
   const double cb = apply ? (sigma ? 0.0 : 2.0) : 1.0;
   const double wz = mirror ? 2.0 : 1.0;
   const int core_lim = core_only ? core : 0x7fffffff;
+  const double thr = st->threshold;
+  const bool alg1 = (variant == V_ALG1 || variant == V_ALG1_GENSEQ || variant == V_SEQ_ALG1);
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
+  for (int m = 0; m < MN; ++m) {
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      const int row = PRG_ROW(I, m);
+      const int row = PRG_ROW(I, m0 + m);
       const double wrow = (row < core_lim) ? 1.0 : 0.0;
       double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int col = PRG_COL(J, n, r);
         const size_t idx = row + (size_t)col * ld;
-        const double y = acc[m][n][r];
+        double y = acc[m][n][r];
         const double x = X[idx];
-        const double z = fma(cb, x, ca * y);
+        double z;
+        if (THR) {
+          const tile::SparseUpdate u = tile::sp2_sparse_update(x, y, apply, sigma, alg1, thr);
+          z = u.z;
+          y = u.y;
+        } else {
+          z = fma(cb, x, ca * y);
+        }
         const double dxy = x - y;
         const double wi = wrow + ((mirror && col < core_lim) ? 1.0 : 0.0);
         const bool diag = (row == col);
@@ -247,7 +243,102 @@ __global__ __launch_bounds__(NT, 2) void sp2_step_kernel(Sp2Args g) {
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  block_store_partials<4>(sums, g.part + 4 * (size_t)t);
+  block_store_partials<4>(sums, g.part + 4 * (size_t)slot);
+}
+
+__global__ __launch_bounds__(NT, 2) void sp2_step_kernel(Sp2Args g) {
+  __shared__ double As[BK][LROW];
+  __shared__ double Bs[BK][LROW];
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const int cur = s_flags[3];
+  const double* X = g.buf[cur];
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  d4 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
+  Operands op;
+  op.A[0] = X; op.B[0] = X; op.A[1] = X; op.B[1] = X; op.npair = 1;
+  mainloop(op, g.ld, 0, g.kdim, I, J, As, Bs, acc);
+  sp2_epilogue_impl<false, 4>(g, g.st, t, I, J, acc, 0, s_flags[1], s_flags[2], cur);
+}
+
+// Split product for small matrices (too few upper-triangle tiles to fill the
+// CUs): workgroup b computes K slice b % split of tile b / split and stores
+// its accumulators, lane-interleaved (entry q of thread tid at
+// [q][tid], d4 each), in acc[(tile * split + slice) * 128 * 128].
+__global__ __launch_bounds__(NT, 2) void sp2_split_product_kernel(Sp2Args g) {
+  __shared__ double As[BK][LROW];
+  __shared__ double Bs[BK][LROW];
+  __shared__ int s_flags[2];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const double* X = g.buf[s_flags[1]];
+  const int t = blockIdx.x / g.split, slice = blockIdx.x % g.split;
+  const int kd = g.kdim / g.split;
+  int I, J;
+  sym_tile(t, I, J);
+  d4 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
+  Operands op;
+  op.A[0] = X; op.B[0] = X; op.A[1] = X; op.B[1] = X; op.npair = 1;
+  mainloop(op, g.ld, slice * kd, kd, I, J, As, Bs, acc);
+  d4* dst = (d4*)(g.acc + (size_t)blockIdx.x * BM * BN) + threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) dst[(4 * m + n) * NT] = acc[m][n];
+}
+
+// Sums the split slices of one 16-row group of every wave's quadrant of a
+// tile in slice order (deterministic) and applies the SP2 epilogue to it:
+// workgroup b handles tile b / 4, group b % 4, so four workgroups share a
+// tile's partial reads.  THR: with the sparse formats' thresholding (the
+// fused kernel above serves dense storage only, so that its epilogue stays
+// straight-line code).  Partial sums per workgroup, part[4 b].
+template <bool THR>
+__global__ __launch_bounds__(NT, 2) void sp2_split_epilogue_kernel(Sp2Args g) {
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const int t = blockIdx.x >> 2, q = blockIdx.x & 3;
+  int I, J;
+  sym_tile(t, I, J);
+  d4 acc[1][4];
+  const d4* src = (const d4*)(g.acc + (size_t)t * g.split * BM * BN) + threadIdx.x;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) acc[0][n] = src[(4 * q + n) * NT];
+  for (int sl = 1; sl < g.split; ++sl) {
+    const d4* sp = src + (size_t)sl * (BM * BN / 4);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[0][n] += sp[(4 * q + n) * NT];
+  }
+  sp2_epilogue_impl<THR, 1>(g, g.st, blockIdx.x, I, J, acc, q, s_flags[1], s_flags[2],
+                            s_flags[3]);
 }
 
 // Response step: Y = X D + D X (two products accumulated over a 2N-deep K),
@@ -279,7 +370,7 @@ __global__ __launch_bounds__(NT, 2) void sp2_response_kernel(Sp2Args g) {
     for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
   Operands op;
   op.A[0] = X; op.B[0] = D; op.A[1] = D; op.B[1] = X; op.npair = 2;
-  mainloop(op, g.ld, g.kdim, I, J, As, Bs, acc);
+  mainloop(op, g.ld, 0, g.kdim, I, J, As, Bs, acc);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
   const int ld = g.ld;
@@ -352,7 +443,24 @@ int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, con
   return OK;
 }
 
-int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles) {
+int sp2_split_count(int ld) {
+  // enough (tile, slice) workgroups to cover the 256 CUs, each slice at least
+  // 8 K steps deep (ld is a multiple of 128, so ld / 8 is a multiple of BK)
+  const int nt = ld / BM, nb = nt * (nt + 1) / 2;
+  int split = 1;
+  while (nb * split < 256 && split < 8 && ld / (2 * split) >= 8 * BK) split *= 2;
+  return split;
+}
+
+size_t sp2_split_bytes(int ld) {
+  const int split = sp2_split_count(ld);
+  if (split == 1) return 0;
+  const int nt = ld / BM, nb = nt * (nt + 1) / 2;
+  return sizeof(double) * (size_t)nb * split * BM * BN;
+}
+
+int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
+             double* split_acc, bool thresholded) {
   Sp2Args g;
   g.buf[0] = x0.p;
   g.buf[1] = x1.p;
@@ -362,9 +470,25 @@ int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
   g.nt = x0.ld / BM;
   g.st = d_state;
   g.part = d_part;
+  g.split = sp2_split_count(g.ld);
+  g.acc = nullptr;
   const int nb = g.nt * (g.nt + 1) / 2;
   if (ntiles) *ntiles = nb;
-  hipLaunchKernelGGL(sp2_step_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  if (g.split > 1 || thresholded) {
+    int st = OK;
+    const size_t bytes = sizeof(double) * (size_t)nb * g.split * BM * BN;
+    g.acc = split_acc ? split_acc : (double*)workspace("sp2_split_acc", bytes, &st);
+    if (st) return st;
+    hipLaunchKernelGGL(sp2_split_product_kernel, dim3(nb * g.split), dim3(NT), 0, stream(), g);
+    if (thresholded)
+      hipLaunchKernelGGL(sp2_split_epilogue_kernel<true>, dim3(4 * nb), dim3(NT), 0, stream(), g);
+    else
+      hipLaunchKernelGGL(sp2_split_epilogue_kernel<false>, dim3(4 * nb), dim3(NT), 0, stream(),
+                         g);
+    if (ntiles) *ntiles = 4 * nb;  // partial-sum slots: four per tile
+  } else {
+    hipLaunchKernelGGL(sp2_step_kernel, dim3(nb), dim3(NT), 0, stream(), g);
+  }
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
@@ -381,6 +505,8 @@ int sp2_response_step(const DMat& x0, const DMat& x1, const DMat& d0, const DMat
   g.nt = x0.ld / BM;
   g.st = d_state;
   g.part = d_part;
+  g.split = 1;
+  g.acc = nullptr;
   const int nb = g.nt * (g.nt + 1) / 2;
   if (ntiles) *ntiles = nb;
   hipLaunchKernelGGL(sp2_response_kernel, dim3(nb), dim3(NT), 0, stream(), g);

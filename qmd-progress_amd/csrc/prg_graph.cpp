@@ -659,6 +659,75 @@ int graph_density_t0(const DMat& H, bml_matrix_t* g_bml, DMat rho, double bndfil
   return OK;
 }
 
+// prg_build_zmatGP (src/prg_graphsolver_mod.F90:202-331) on one GPU: the
+// partition (own BFS partitioner in place of METIS), the double-jump core+halo
+// lists of the graph (no Hamiltonian, as there), per part the Loewdin factor
+// Z_p = U s^-1/2 U^T of the overlap's core+halo block (prg_buildZdiag,
+// src/prg_genz_mod.F90:176-276) from batched eigensolves, and the core rows of
+// each Z_p written into zmat (bml_submatrix2matrix).  Entries outside the
+// core rows' core+halo columns keep their value, as the reference leaves them.
+int graph_zmat(const DMat& S, bml_matrix_t* g_bml, DMat Z, int nparts) {
+  int st = ensure_device();
+  if (st) return st;
+  prg_graph_partitioning_t gp;
+  std::memset(&gp, 0, sizeof(gp));
+  struct Guard {
+    prg_graph_partitioning_t* g;
+    ~Guard() { free_parts(g); }
+  } guard{&gp};
+  if ((st = bfs_partition(g_bml, nparts, &gp))) return st;
+  if ((st = halo_indices(g_bml, &gp, 1))) return st;
+  HostPlan hp;
+  BatchPlan plan;
+  if ((st = make_plan(&gp, &hp, &plan))) return st;
+  const int np = plan.nparts;
+  std::vector<size_t> woff(np);
+  size_t wtot = 0;
+  for (int q = 0; q < np; ++q) {
+    woff[q] = wtot;
+    wtot += hp.parts[q].m;
+  }
+  double* a = (double*)workspace("gpt0_a", sizeof(double) * plan.total, &st);
+  double* u = (double*)workspace("gpt0_u", sizeof(double) * plan.total, &st);
+  double* w = (double*)workspace("gpt0_w", sizeof(double) * wtot, &st);
+  size_t* d_woff = (size_t*)workspace("gpt0_woff", sizeof(size_t) * np, &st);
+  double* d_wmin = (double*)workspace("gpt0_dn", sizeof(double) * np, &st);
+  double* zp = (double*)workspace("gpt0_d", sizeof(double) * plan.total, &st);
+  if (st) return st;
+  PRG_HIP_CHECK(hipMemcpyAsync(d_woff, woff.data(), sizeof(size_t) * np, hipMemcpyHostToDevice,
+                               stream()));
+  // padding rows decoupled with eigenvalue 1 (positive, never scattered)
+  if ((st = batched_gather_pad(S, plan, a, 1.0))) return st;
+  for (int q0 = 0; q0 < np;) {
+    int q1 = q0;
+    while (q1 < np && hp.parts[q1].m == hp.parts[q0].m) ++q1;
+    if ((st = eig_strided_batched(a + hp.parts[q0].offset, hp.parts[q0].m, q1 - q0,
+                                  w + woff[q0])))
+      return st;
+    q0 = q1;
+  }
+  if ((st = batched_invsqrt_scale(plan, a, w, d_woff, u, d_wmin))) return st;
+  for (int q0 = 0; q0 < np;) {
+    int q1 = q0;
+    while (q1 < np && hp.parts[q1].m == hp.parts[q0].m) ++q1;
+    const size_t o = hp.parts[q0].offset;
+    if ((st = gemm_nt_strided_batched(u + o, a + o, zp + o, hp.parts[q0].m, q1 - q0))) return st;
+    q0 = q1;
+  }
+  std::vector<double> wmin(np);
+  PRG_HIP_CHECK(hipMemcpyAsync(wmin.data(), d_wmin, sizeof(double) * np, hipMemcpyDeviceToHost,
+                               stream()));
+  PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  for (int q = 0; q < np; ++q) {
+    if (!(wmin[q] > 0.0)) {
+      // the reference stops: "Overlap matrix is not possitive definite"
+      set_error(ERR_NUMERIC, "prg_build_zmatGP: an overlap block is not positive definite");
+      return ERR_NUMERIC;
+    }
+  }
+  return batched_scatter(plan, zp, Z, 1.0, false);
+}
+
 }  // namespace prg
 
 using namespace prg;
@@ -798,6 +867,24 @@ void prg_build_densityGP_T0(bml_matrix_t* ham_bml, bml_matrix_t* g_bml, bml_matr
 }
 
 double prg_build_densityGP_T0_ef(void) { return g_gp_ef; }
+
+// src/prg_progress_mod.h:814-820 (Fortran src/prg_graphsolver_mod.F90:202-331).
+// threshold (sparse-algebra threshold of the reference) is ignored on dense
+// storage; verbose > 1 prints the part count.
+void prg_build_zmatGP(bml_matrix_t* over_bml, bml_matrix_t* g_bml, bml_matrix_t* zmat_bml,
+                      double threshold, int nparts, int verbose) {
+  (void)threshold;
+  clear_error();
+  if (check_pair(over_bml, g_bml, "prg_build_zmatGP") ||
+      check_pair(over_bml, zmat_bml, "prg_build_zmatGP"))
+    return;
+  DMat S, Z;
+  // zmat's current contents first: the entries the parts do not write keep them
+  if (bml_dev_read(over_bml, &S) || bml_dev_read(zmat_bml, &Z) || bml_dev_write(zmat_bml, &Z))
+    return;
+  if (graph_zmat(S, g_bml, Z, nparts)) return;
+  if (verbose > 1) std::printf("prg_build_zmatGP: totalParts = %d\n", nparts);
+}
 
 // Device time (ms) of the last prg_subgraphSP2Loop: gather + normalise, the
 // purification loop, the core-row scatter.

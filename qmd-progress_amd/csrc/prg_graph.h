@@ -56,12 +56,19 @@ int batched_slots(BatchPlan* plan);
 int batched_zero_tickets(int* d_tickets, int count);
 // vv[it] += sum of d_part[it * ntiles + t] over the tiles, it < iters.
 int batched_vv(const BatchPlan& plan, const double* d_part, int iters, double* d_vv);
-int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale);
+// rho[core rows, core+halo columns] = scale x (rho zeroed first unless zero
+// is false)
+int batched_scatter(const BatchPlan& plan, const double* x, DMat rho, double scale,
+                    bool zero = true);
 // X_p = H[idx, idx], padding decoupled with `pad` on its diagonal
 int batched_gather_pad(const DMat& H, const BatchPlan& plan, double* x, double pad);
 // U_p = V_p diag(2 f(w_p)), dnocc[p] = sum 2 df/def(w_p)  (finite-T occupations)
 int batched_fermi_scale(const BatchPlan& plan, const double* v, const double* w,
                         const size_t* d_w_offset, double* u, double ef, double kbt,
                         double* d_dnocc);
+
+// U_p = V_p diag(w_p^-1/2), wmin[p] = min w_p  (Loewdin factor of an overlap)
+int batched_invsqrt_scale(const BatchPlan& plan, const double* v, const double* w,
+                          const size_t* d_w_offset, double* u, double* d_wmin);
 
 }  // namespace prg

@@ -115,13 +115,20 @@ struct Sp2State {
   int refine_phase;  // 1 while the FP64 polishing steps run
   int refine_steps;  // polishing steps taken
   double ie0, ie1, ie2;  // idempotency errors of the last three X
+  // sparse BML formats (ellpack, ellsort, ellblock, csr): entries with
+  // |a| <= threshold are dropped from every product and sum, as those
+  // formats do; 0 for dense storage (BML dense ignores thresholds)
+  double threshold;
   int pp[kMaxIter];
   double vv[kMaxIter];
 };
 
 // ---------------------------------------------------------------- kernels
 // Gershgorin interval of the n x n block (row sums, as bml_gershgorin).
-int gershgorin(const DMat& h, double* d_bounds /*2 doubles, device*/);
+// scratch (optional, device): gershgorin_scratch_doubles(n) doubles instead
+// of the library workspace.
+int gershgorin(const DMat& h, double* d_bounds /*2 doubles, device*/, double* scratch = nullptr);
+size_t gershgorin_scratch_doubles(int n);
 // X = alpha*H + beta*I with alpha/beta computed on the device from the
 // Gershgorin bounds (mode 0: SP2 map (hi I - H)/(hi-lo)), or given scalars.
 // Also writes Tr(X) and ||X||_F^2 partials into d_part (per block) and the
@@ -149,8 +156,17 @@ int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, con
          const DMat* dotE = nullptr);
 
 // Fused SP2 step: Y = X*X, then per the device state's planned branch
-// Xn = Y or 2X - Y; epilogue partials [||Xn||^2, Tr Xn, idem, Tr Y].
-int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles);
+// Xn = Y or 2X - Y; epilogue partials [||Xn||^2, Tr Xn, idem, Tr Y].  Below
+// 256 upper-triangle tiles the product is split along K
+// (sp2_split_count(ld) slices) into split_acc (sp2_split_bytes(ld) bytes; the
+// library workspace when null) and summed in slice order by a second kernel.
+// thresholded: the sparse formats' thresholding of the state's threshold
+// (product and update in two kernels; split_acc then holds one slice per
+// tile at least).
+int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
+             double* split_acc = nullptr, bool thresholded = false);
+int sp2_split_count(int ld);
+size_t sp2_split_bytes(int ld);
 // Fused response step: Y = X D + D X with the same branch: Dn = Y or 2D - Y.
 int sp2_response_step(const DMat& x0, const DMat& x1, const DMat& d0, const DMat& d1,
                       Sp2State* d_state, double* d_part, int* ntiles);

@@ -1,10 +1,12 @@
 // SP2 purification drivers (src/prg_sp2_mod.F90) on one GPU.
 //
 // The whole iteration runs on the device: one fused product-and-update launch
-// and one single-workgroup bookkeeping launch per step, enqueued in chunks.
-// The host only reads the `done` flag between chunks (a 64-byte copy), so no
-// scalar crosses PCIe per iteration.  Once the loop has stopped, the
-// remaining launches of a chunk exit at their first instruction.
+// (two for the K-split product of small matrices) and one single-workgroup
+// bookkeeping launch per step, enqueued in chunks.  The host reads the state
+// header of a chunk (an asynchronous copy into pinned memory) only after the
+// next chunk is enqueued, so no scalar crosses PCIe per iteration and the
+// device does not wait for the host.  Once the loop has stopped, the
+// launches still queued exit at their first instruction.
 #include "prg_sp2.h"
 #include "prg_ops.h"
 
@@ -13,6 +15,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -28,6 +31,28 @@ double g_step_ms_sum = 0.0;
 int g_step_count = 0;
 std::vector<hipEvent_t> g_events;
 
+// Events of the pipelined header read-back (two slots).
+hipEvent_t g_head_ev[2] = {nullptr, nullptr};
+
+// Iteration count of the last solve per (n, variant, precision): the chunk
+// lengths of the next solve of that shape.
+struct ShapeKey {
+  int n, variant, precision;
+  bool operator<(const ShapeKey& o) const {
+    if (n != o.n) return n < o.n;
+    if (variant != o.variant) return variant < o.variant;
+    return precision < o.precision;
+  }
+};
+std::map<ShapeKey, int> g_iterations;
+
+int expected_iterations(const ShapeKey& k) {
+  auto it = g_iterations.find(k);
+  return it == g_iterations.end() ? 0 : it->second;
+}
+
+void record_iterations(const ShapeKey& k, int iters) { g_iterations[k] = iters; }
+
 bool fortran_eq(const char* s, const char* lit) {
   if (!s) return false;
   size_t n = std::strlen(s);
@@ -39,9 +64,12 @@ bool fortran_eq(const char* s, const char* lit) {
 int precision_mode() { return g_precision; }
 void set_precision_mode(int m) { g_precision = m; }
 
-int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const DMat* H1,
-            DMat* out1) {
-  int st = ensure_device();
+int run_sp2(const DMat& H, const Sp2Params& p_in, DMat out, Sp2Result* res, const DMat* H1,
+            DMat* out1, const Sp2Buffers* bufs) {
+  // the sparse formats' thresholding runs with FP64 products
+  Sp2Params p = p_in;
+  if (p.threshold > 0.0) p.precision = PRG_PREC_FP64;
+  int st = bufs ? OK : ensure_device();
   if (st) return st;
   const int n = H.n, ld = H.ld;
   if (n <= 0 || ld != padded(n) || out.ld != ld || out.n != n) {
@@ -51,15 +79,33 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   const int nt = ld / kTile;
   const int ntiles = nt * (nt + 1) / 2;
   const size_t mat_bytes = sizeof(double) * (size_t)ld * ld;
-  double* x0 = (double*)workspace("sp2_x0", mat_bytes, &st);
-  double* x1 = (double*)workspace("sp2_x1", mat_bytes, &st);
-  // partial sums: one slot per product tile (the int8 CRT update writes one
-  // per 64 x 64 block) plus the normalisation's
-  const int nslots = std::max(ntiles, crt_update_blocks(ld));
-  double* part = (double*)workspace("sp2_part", sizeof(double) * 4 * (size_t)(nslots + 2048), &st);
-  double* bounds = (double*)workspace("sp2_bounds", sizeof(double) * 4, &st);
-  Sp2State* dst = (Sp2State*)workspace("sp2_state", sizeof(Sp2State), &st);
-  Sp2State* hst = (Sp2State*)host_staging(sizeof(Sp2State), &st);
+  if (bufs && (H1 || p.variant == V_RESPONSE || p.precision != PRG_PREC_FP64)) {
+    set_error(ERR_ARG, "run_sp2: caller buffers serve FP64 solves without response only");
+    return ERR_ARG;
+  }
+  double *x0, *x1, *part, *bounds;
+  Sp2State *dst, *hst;
+  if (bufs) {
+    x0 = bufs->x0;
+    x1 = bufs->x1;
+    part = bufs->part;
+    bounds = bufs->bounds;
+    dst = bufs->state;
+    hst = bufs->host;
+  } else {
+    x0 = (double*)workspace("sp2_x0", mat_bytes, &st);
+    x1 = (double*)workspace("sp2_x1", mat_bytes, &st);
+    // partial sums: one slot per product tile (the int8 CRT update writes one
+    // per 64 x 64 block) plus the normalisation's
+    // (the K-split epilogue writes four per tile)
+    const int nslots = std::max(4 * ntiles, crt_update_blocks(ld));
+    part = (double*)workspace("sp2_part", sizeof(double) * 4 * (size_t)(nslots + 2048), &st);
+    bounds = (double*)workspace("sp2_bounds", sizeof(double) * 4, &st);
+    dst = (Sp2State*)workspace("sp2_state", sizeof(Sp2State), &st);
+    // pinned host memory: the state image, then two header slots that the
+    // pipelined loop below reads back alternately
+    hst = (Sp2State*)host_staging(3 * sizeof(Sp2State), &st);
+  }
   if (st) return st;
   DMat X0{n, ld, x0}, X1{n, ld, x1};
   const bool response = (p.variant == V_RESPONSE);
@@ -73,7 +119,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   // X0 = (hi I - H)/(hi - lo)
   int nb = 0;
   if (p.use_gershgorin) {
-    if ((st = gershgorin(H, bounds))) return st;
+    if ((st = gershgorin(H, bounds, bufs ? bufs->gersh : nullptr))) return st;
     if ((st = normalize_gershgorin(H, bounds, X0, part, &nb))) return st;
   } else {
     if ((st = normalize_bounds(H, p.mineval, p.maxeval, X0, part, &nb))) return st;
@@ -116,6 +162,7 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   hst->absconv = p.absconv;
   hst->icount = p.icount;
   hst->core = p.core;
+  hst->threshold = p.threshold > 0.0 ? p.threshold : 0.0;
   const bool lowp = (p.precision != PRG_PREC_FP64) && !response;
   // FP32-accumulated products stop on their noise floor and are polished in
   // FP64; the int8 CRT products carry X to 2^-42 and keep the FP64 rules.
@@ -134,15 +181,29 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
                         p.variant == V_SUBMATRIX)
                            ? p.icount
                            : p.maxit + 2;
-  const int chunk = n >= 4096 ? 8 : 16;
+  const bool replay = (maxsteps == p.icount) && p.variant != V_ALG2 && p.variant != V_ALG1;
   const size_t head = offsetof(Sp2State, pp);
   int prec = p.precision;
-  if (g_timing) crt_timing_begin();
-  if (g_timing && (int)g_events.size() < 2 * (maxsteps + 1)) {
+  const bool timing = g_timing && !bufs;
+  if (timing) crt_timing_begin();
+  if (timing && (int)g_events.size() < 2 * (maxsteps + 1)) {
     const size_t old = g_events.size();
     g_events.resize(2 * (maxsteps + 1));
     for (size_t e = old; e < g_events.size(); ++e) PRG_HIP_CHECK(hipEventCreate(&g_events[e]));
   }
+  if (!bufs && !g_head_ev[0]) {
+    PRG_HIP_CHECK(hipEventCreateWithFlags(&g_head_ev[0], hipEventDisableTiming));
+    PRG_HIP_CHECK(hipEventCreateWithFlags(&g_head_ev[1], hipEventDisableTiming));
+  }
+  // Chunked, pipelined loop: chunk k + 1 is enqueued before the host waits
+  // for the header of chunk k (copied asynchronously into a pinned slot), so
+  // the device never idles on the host.  Once the state says done, the
+  // launches still queued exit at their first instruction.  Chunk lengths
+  // follow the iteration count of the last solve of the same shape (one long
+  // chunk, then short ones), otherwise 16 steps (8 from n = 4096).
+  const ShapeKey key{n, p.variant, p.precision};
+  const int expect = (replay || bufs) ? 0 : expected_iterations(key);
+  Sp2State* slot[2] = {hst + 1, hst + 2};
   int phase0_steps = 0;
   for (int phase = 0; phase < 2; ++phase) {
     int steps = 0;
@@ -154,16 +215,27 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
       if ((st = sp2_begin_refine(dst))) return st;
       phase_steps = hst->refine_max + 1;
     }
-    while (steps < phase_steps) {
-      for (int c = 0; c < chunk && steps < phase_steps; ++c, ++steps) {
+    int chunk_no = 0;
+    bool waiting = false;  // a header copy is in flight for the previous chunk
+    bool done = false;
+    while (steps < phase_steps && !done) {
+      int len;
+      if (phase == 0 && replay)
+        len = maxsteps;
+      else if (phase == 0 && expect > 0)
+        len = (chunk_no == 0) ? std::max(1, expect - 2) : 4;
+      else
+        len = n >= 4096 ? 8 : 16;
+      for (int c = 0; c < len && steps < phase_steps; ++c, ++steps) {
         int nt_used = 0;
-        const bool timed = g_timing && phase == 0;
+        const bool timed = timing && phase == 0;
         if (timed) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps], stream()));
         if (response) {
           if ((st = sp2_response_step(X0, X1, D0, D1, dst, part, &nt_used))) return st;
         }
         if (prec == PRG_PREC_FP64 || response) {
-          st = sp2_step(X0, X1, dst, part, &nt_used);
+          st = sp2_step(X0, X1, dst, part, &nt_used, bufs ? bufs->split_acc : nullptr,
+                        p.threshold > 0.0);
         } else if (prec == PRG_PREC_F16SPLIT || prec == PRG_PREC_BF16SPLIT ||
                    prec == PRG_PREC_INT8CRT) {
           st = sp2_step_mixed(X0, X1, dst, part, &nt_used, prec);
@@ -175,9 +247,27 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
         if (timed) PRG_HIP_CHECK(hipEventRecord(g_events[2 * steps + 1], stream()));
         if ((st = sp2_absorb_plan(dst, part, nt_used))) return st;
       }
-      PRG_HIP_CHECK(hipMemcpyAsync(hst, dst, head, hipMemcpyDeviceToHost, stream()));
-      PRG_HIP_CHECK(hipStreamSynchronize(stream()));
-      if (hst->done) break;
+      const int sl = chunk_no & 1;
+      PRG_HIP_CHECK(hipMemcpyAsync(slot[sl], dst, head, hipMemcpyDeviceToHost, stream()));
+      if (bufs) {
+        // caller buffers: a plain synchronous read-back, no events
+        PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+        done = slot[sl]->done != 0;
+        ++chunk_no;
+        continue;
+      }
+      PRG_HIP_CHECK(hipEventRecord(g_head_ev[sl], stream()));
+      if (waiting) {
+        PRG_HIP_CHECK(hipEventSynchronize(g_head_ev[sl ^ 1]));
+        done = slot[sl ^ 1]->done != 0;
+      }
+      waiting = true;
+      ++chunk_no;
+    }
+    if (!done && waiting) {
+      // the last chunk's header
+      PRG_HIP_CHECK(hipEventSynchronize(g_head_ev[(chunk_no - 1) & 1]));
+      done = slot[(chunk_no - 1) & 1]->done != 0;
     }
     if (phase == 0) phase0_steps = steps;
   }
@@ -194,11 +284,12 @@ int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res, const D
   }
   PRG_HIP_CHECK(hipMemcpyAsync(hst, dst, sizeof(Sp2State), hipMemcpyDeviceToHost, stream()));
   PRG_HIP_CHECK(hipStreamSynchronize(stream()));
+  if (!replay && !bufs) record_iterations(key, hst->iter);
   if (!hst->done) {
     set_error(ERR_NOT_CONVERGED, "SP2 loop did not finish within the step budget");
     return ERR_NOT_CONVERGED;
   }
-  if (g_timing) {
+  if (timing) {
     // products of the main (phase 0) loop
     const int main_iters = hst->iter - hst->refine_steps;
     const int executed = main_iters < phase0_steps ? main_iters : phase0_steps;
@@ -250,9 +341,18 @@ void report(const char* who, int verbose, const Sp2Result& r) {
 }
 
 // Shared body of the H -> rho variants.
+// The threshold applies when rho is stored in a sparse BML format (ellpack,
+// ellsort, ellblock, csr): those formats drop |a| <= threshold from every
+// product and sum.  Dense storage ignores it, as BML dense does.
+double format_threshold(const bml_matrix_t* rho, double threshold) {
+  return (rho && rho->type != dense && rho->type != type_uninitialized && threshold > 0.0)
+             ? threshold
+             : 0.0;
+}
+
 void sp2_h_to_rho(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* rho, double bndfil,
                   int minit, int maxit, const char* sp2conv, double idemtol, int verbose,
-                  int* pp, int* icount, double* vv, int ppsize) {
+                  int* pp, int* icount, double* vv, int ppsize, double threshold) {
   clear_error();
   if (check_pair(h, rho, who)) return;
   DMat H, R;
@@ -268,6 +368,7 @@ void sp2_h_to_rho(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* r
   }
   if (bml_dev_write(rho, &R)) return;
   Sp2Params p = base_params(variant, h->N, bndfil, minit, maxit, sp2conv, idemtol);
+  p.threshold = format_threshold(rho, threshold);
   Sp2Result r;
   const int st = run_sp2(H, p, R, &r);
   if (st != OK && st != ERR_NOT_CONVERGED) return;
@@ -283,7 +384,8 @@ void sp2_h_to_rho(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* r
 }
 
 void sp2_replay(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* rho, int* pp,
-                int* icount, double* vv, bool gersh, double mineval, double maxeval, int core) {
+                int* icount, double* vv, bool gersh, double mineval, double maxeval, int core,
+                double threshold) {
   clear_error();
   if (!icount || !pp || !vv) {
     set_error(ERR_ARG, std::string(who) + ": null pp / icount / vv");
@@ -315,6 +417,7 @@ void sp2_replay(const char* who, int variant, bml_matrix_t* h, bml_matrix_t* rho
   p.maxeval = maxeval;
   p.core = core;
   p.precision = precision_mode();
+  p.threshold = format_threshold(rho, threshold);
   Sp2Result r;
   if (run_sp2(H, p, R, &r) != OK) return;
   for (int i = 0; i < *icount; ++i) vv[i] = r.vv[i];
@@ -340,13 +443,16 @@ int prg_set_int8_moduli(int count) {
 }
 int prg_get_int8_moduli(void) { return crt_moduli(); }
 int prg_gpu_int8_waves(void) { return crt_last_waves(); }
+int prg_gpu_set_int8_config(int config) {
+  clear_error();
+  return set_crt_config(config);
+}
 
 void prg_sp2_basic(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
                    int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
                    int verbose) {
-  (void)threshold;
   sp2_h_to_rho("prg_sp2_basic", V_BASIC, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
-               idemtol, verbose, nullptr, nullptr, nullptr, 0);
+               idemtol, verbose, nullptr, nullptr, nullptr, 0, threshold);
 }
 
 // src/prg_c_interface.F90:1091-1111 (Fortran src/prg_sp2_mod.F90:138-246): the
@@ -357,9 +463,8 @@ void prg_sp2_basic(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold,
 void prg_sp2_basic_tcore(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, bml_matrix_t* rhofull_bml,
                          double threshold, double bndfil, int minsp2iter, int maxsp2iter,
                          const char* sp2conv, double idemtol, int verbose) {
-  (void)threshold;
   sp2_h_to_rho("prg_sp2_basic_tcore", V_BASIC_FIXED, h_bml, rho_bml, bndfil, minsp2iter,
-               maxsp2iter, sp2conv, idemtol, verbose, nullptr, nullptr, nullptr, 0);
+               maxsp2iter, sp2conv, idemtol, verbose, nullptr, nullptr, nullptr, 0, threshold);
   if (prg_last_error_code() != OK || verbose <= 1 || !rhofull_bml) return;
   if (check_pair(rho_bml, rhofull_bml, "prg_sp2_basic_tcore")) return;
   DMat R, F, T;
@@ -419,76 +524,68 @@ void prg_sp2_tensorcore(int N, float* H, double* D, float eps, float bndfil, int
 void prg_sp2_alg2(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
                   int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
                   int verbose) {
-  (void)threshold;
   sp2_h_to_rho("prg_sp2_alg2", V_ALG2, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
-               idemtol, verbose, nullptr, nullptr, nullptr, 0);
+               idemtol, verbose, nullptr, nullptr, nullptr, 0, threshold);
 }
 
 void prg_sp2_alg2_genseq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold,
                          double bndfil, int minsp2iter, int maxsp2iter, const char* sp2conv,
                          double idemtol, int* pp, int* icount, double* vv, int verbose) {
-  (void)threshold;
   sp2_h_to_rho("prg_sp2_alg2_genseq", V_ALG2_GENSEQ, h_bml, rho_bml, bndfil, minsp2iter,
-               maxsp2iter, sp2conv, idemtol, verbose, pp, icount, vv, maxsp2iter);
+               maxsp2iter, sp2conv, idemtol, verbose, pp, icount, vv, maxsp2iter, threshold);
 }
 
 void prg_sp2_alg2_seq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, int* pp,
                       int* icount, double* vv, int verbose) {
-  (void)threshold;
   (void)verbose;
-  sp2_replay("prg_sp2_alg2_seq", V_SEQ_ALG2, h_bml, rho_bml, pp, icount, vv, true, 0, 0, 0);
+  sp2_replay("prg_sp2_alg2_seq", V_SEQ_ALG2, h_bml, rho_bml, pp, icount, vv, true, 0, 0, 0,
+             threshold);
 }
 
 void prg_prg_sp2_alg2_seq_inplace(bml_matrix_t* rho_bml, double threshold, int ppsize, int* pp,
                                   int* icount, double* vv, double mineval, double maxeval) {
-  (void)threshold;
   (void)ppsize;
   sp2_replay("prg_prg_sp2_alg2_seq_inplace", V_SEQ_ALG2, rho_bml, rho_bml, pp, icount, vv, false,
-             mineval, maxeval, 0);
+             mineval, maxeval, 0, threshold);
 }
 
 void prg_sp2_alg1(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, double bndfil,
                   int minsp2iter, int maxsp2iter, const char* sp2conv, double idemtol,
                   int verbose) {
-  (void)threshold;
   sp2_h_to_rho("prg_sp2_alg1", V_ALG1, h_bml, rho_bml, bndfil, minsp2iter, maxsp2iter, sp2conv,
-               idemtol, verbose, nullptr, nullptr, nullptr, 0);
+               idemtol, verbose, nullptr, nullptr, nullptr, 0, threshold);
 }
 
 void prg_sp2_alg1_genseq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold,
                          double bndfil, int minsp2iter, int maxsp2iter, const char* sp2conv,
                          double idemtol, int* pp, int* icount, double* vv) {
-  (void)threshold;
   sp2_h_to_rho("prg_sp2_alg1_genseq", V_ALG1_GENSEQ, h_bml, rho_bml, bndfil, minsp2iter,
-               maxsp2iter, sp2conv, idemtol, 0, pp, icount, vv, maxsp2iter);
+               maxsp2iter, sp2conv, idemtol, 0, pp, icount, vv, maxsp2iter, threshold);
 }
 
 void prg_sp2_alg1_seq(bml_matrix_t* h_bml, bml_matrix_t* rho_bml, double threshold, int* pp,
                       int* icount, double* vv) {
-  (void)threshold;
-  sp2_replay("prg_sp2_alg1_seq", V_SEQ_ALG1, h_bml, rho_bml, pp, icount, vv, true, 0, 0, 0);
+  sp2_replay("prg_sp2_alg1_seq", V_SEQ_ALG1, h_bml, rho_bml, pp, icount, vv, true, 0, 0, 0,
+             threshold);
 }
 
 void prg_prg_sp2_alg1_seq_inplace(bml_matrix_t* rho_bml, double threshold, int ppsize, int* pp,
                                   int* icount, double* vv, double mineval, double maxeval) {
-  (void)threshold;
   (void)ppsize;
   sp2_replay("prg_prg_sp2_alg1_seq_inplace", V_SEQ_ALG1, rho_bml, rho_bml, pp, icount, vv, false,
-             mineval, maxeval, 0);
+             mineval, maxeval, 0, threshold);
 }
 
 void prg_sp2_submatrix(bml_matrix_t* ham_bml, bml_matrix_t* rho_bml, double threshold, int* pp,
                        int* icount, double* vv, double mineval, double maxeval, int core_size) {
-  (void)threshold;
   sp2_replay("prg_sp2_submatrix", V_SUBMATRIX, ham_bml, rho_bml, pp, icount, vv, false, mineval,
-             maxeval, core_size);
+             maxeval, core_size, threshold);
 }
 
 void prg_sp2_submatrix_inplace(bml_matrix_t* rho_bml, double threshold, int* pp, int* icount,
                                double* vv, double mineval, double maxeval, int core_size) {
-  (void)threshold;
   sp2_replay("prg_sp2_submatrix_inplace", V_SUBMATRIX, rho_bml, rho_bml, pp, icount, vv, false,
-             mineval, maxeval, core_size);
+             mineval, maxeval, core_size, threshold);
 }
 
 void prg_gershgorin(bml_matrix_t* h_bml, double* gbnd) {

@@ -24,6 +24,7 @@ struct Sp2Params {
   double maxeval = 0.0;
   int precision = 0;
   int refine_max = 2;  // FP64 polishing steps after a low-precision loop
+  double threshold = 0.0;  // sparse-format thresholding of X (0: dense)
 };
 
 struct Sp2Result {
@@ -37,10 +38,27 @@ struct Sp2Result {
   double vv[kMaxIter] = {};
 };
 
+// Buffers a caller hands to run_sp2 instead of the library workspace (the
+// no-allocation entry point): device memory for the ping-pong X, the partial
+// sums, the bounds, the state, the K-split accumulators and the Gershgorin
+// scratch, and three host state images.  With them the loop reads the state
+// header back synchronously (no events) and keeps no per-shape history.
+struct Sp2Buffers {
+  double* x0;
+  double* x1;
+  double* part;
+  double* bounds;
+  Sp2State* state;
+  double* split_acc;
+  double* gersh;
+  Sp2State* host;
+};
+
 // Runs one SP2 variant: out = 2 X_final.  For V_RESPONSE, H1 is the
-// perturbation and out1 receives rho1 = 2 Delta_final.
+// perturbation and out1 receives rho1 = 2 Delta_final.  bufs (FP64 only, no
+// response): caller-provided buffers, see Sp2Buffers.
 int run_sp2(const DMat& H, const Sp2Params& p, DMat out, Sp2Result* res,
-            const DMat* H1 = nullptr, DMat* out1 = nullptr);
+            const DMat* H1 = nullptr, DMat* out1 = nullptr, const Sp2Buffers* bufs = nullptr);
 
 // Mixed-precision fused SP2 step (prec = PRG_PREC_F16SPLIT or PRG_PREC_INT8CRT).
 int sp2_step_mixed(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part,
@@ -64,7 +82,8 @@ int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, co
 // same pointer nest; a hold of another pointer while one is active is ignored.
 void crt_hold_operand(const double* p);
 void crt_release_operand(const double* p);
-int crt_last_waves();  // 800 + ring stages of the last int8 CRT product kernel (0: none yet)
+int crt_last_waves();  // configuration code of the last int8 CRT product kernel (0: none yet)
+int set_crt_config(int config);  // force a product kernel configuration (0: autotune)
 int crt_update_blocks(int ld);  // partial-sum slots one int8 CRT step writes
 // out = scale X[cur] after an int8 CRT solve (from the residue planes when no
 // FP64 master is kept, from the master's lower triangle otherwise)

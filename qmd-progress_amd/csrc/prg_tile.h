@@ -63,5 +63,27 @@ __device__ __forceinline__ void block_store_partials(double (&v)[NQ], double* ds
 }
 
 
+// One entry of the SP2 update z from x (the current X) and y (X^2) as the
+// sparse BML formats compute it, each product and sum thresholded (|a| <=
+// thr dropped): alg1 forms d = thr(X - X^2) then thr(X -+ d)
+// (src/prg_sp2_mod.F90:783-784, :811-823), the other variants thr(X^2) then
+// thr(X^2) or thr(2X - thr(X^2)) (:450-453, :328-337).  y is replaced by the
+// thresholded X^2 (X - d for alg1) that the idempotency sums read.  apply =
+// 0 leaves X unchanged.
+__device__ __forceinline__ double thr_drop(double v, double thr) {
+  return fabs(v) > thr ? v : 0.0;
+}
+struct SparseUpdate {
+  double z, y;
+};
+__device__ __forceinline__ SparseUpdate sp2_sparse_update(double x, double y, int apply,
+                                                          int sigma, bool alg1, double thr) {
+  // alg1: d = thr(X - X^2); X^2 seen by the sums as X - d
+  const double d = thr_drop(x - y, thr);
+  const double yt = alg1 ? x - d : thr_drop(y, thr);
+  const double up = alg1 ? (sigma ? x - d : x + d) : (sigma ? yt : 2.0 * x - yt);
+  return SparseUpdate{apply ? thr_drop(up, thr) : x, yt};
+}
+
 }  // namespace tile
 }  // namespace prg

@@ -1619,8 +1619,14 @@ module prg_graphsolver_mod
   implicit none
   private
   integer, parameter :: dp = kind(1.0d0)
-  public :: prg_build_densityGP_T0
+  public :: prg_build_densityGP_T0, prg_build_zmatGP
   interface
+    subroutine c_zmat_gp(s, g, z, thr, nparts, v) bind(C, name="prg_build_zmatGP")
+      import :: c_ptr, c_double, c_int
+      type(c_ptr), value :: s, g, z
+      real(c_double), value :: thr
+      integer(c_int), value :: nparts, v
+    end subroutine
     subroutine c_gp_t0(h, g, r, thr, bndfil, ef, nparts, v) bind(C, name="prg_build_densityGP_T0")
       import :: c_ptr, c_double, c_int
       type(c_ptr), value :: h, g, r
@@ -1652,6 +1658,23 @@ contains
     call c_gp_t0(ham_bml%ptr, g_bml%ptr, rho_bml%ptr, threshold, bndfil, Ef, nparts, v)
     Ef = c_gp_t0_ef()
   end subroutine prg_build_densityGP_T0
+
+  !> src/prg_graphsolver_mod.F90:202-331: Z ~ S^-1/2 from the parts' Loewdin
+  !> factors (partition by the library's BFS partitioner in place of METIS)
+  subroutine prg_build_zmatGP(over_bml, g_bml, zmat_bml, threshold, nparts, verbose)
+    type(bml_matrix_t), intent(in) :: over_bml, g_bml
+    type(bml_matrix_t), intent(inout) :: zmat_bml
+    real(dp) :: threshold
+    integer, intent(inout) :: nparts
+    integer, optional, intent(in) :: verbose
+    integer :: v
+    v = 0
+    if (present(verbose)) v = verbose
+    if (.not. c_associated(zmat_bml%ptr)) &
+         call bml_zero_matrix(bml_matrix_dense, bml_element_real, dp, bml_get_n(over_bml), &
+         bml_get_n(over_bml), zmat_bml)
+    call c_zmat_gp(over_bml%ptr, g_bml%ptr, zmat_bml%ptr, threshold, nparts, v)
+  end subroutine prg_build_zmatGP
 end module prg_graphsolver_mod
 
 ! src/sp2_tensorcore/src/prg_sp2_tensorcore_mod.F90: the mixed-precision SP2

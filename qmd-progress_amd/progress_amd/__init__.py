@@ -73,6 +73,7 @@ def lib():
         "prg_set_int8_moduli": (i, [i]),
         "prg_get_int8_moduli": (i, []),
         "prg_gpu_int8_waves": (i, []),
+        "prg_gpu_set_int8_config": (i, [i]),
         "prg_gershgorin": (None, [vp, dp]),
         "prg_normalize": (None, [vp]),
         "prg_normalize_fermi": (None, [vp, d, d, d]),
@@ -114,6 +115,7 @@ def lib():
         "prg_deorthogonalize": (None, [vp, vp, vp, d, cp, i]),
         "prg_build_densityGP_T0": (None, [vp, vp, vp, d, d, d, i, i]),
         "prg_build_densityGP_T0_ef": (d, []),
+        "prg_build_zmatGP": (None, [vp, vp, vp, d, i, i]),
         "prg_gpu_subgraph_timing": (None, [dp, dp, dp]),
         "prg_gpu_subgraph_config": (None, [ip, ip, ip, ip]),
         "prg_gpu_gershgorin_device": (i, [i, vp, dp]),
@@ -181,6 +183,9 @@ def _free(ptr):
     libc.free(ptr)
 
 
+ELLPACK = 2  # bml_matrix_type_t ellpack (a sparse BML format)
+
+
 class Matrix:
     """Owning wrapper of a library matrix handle (dense FP64)."""
 
@@ -191,8 +196,9 @@ class Matrix:
         self.h = C.c_void_p(handle)
 
     @classmethod
-    def zeros(cls, n: int) -> "Matrix":
-        h = lib().bml_zero_matrix(_dense, _double_real, n, n, _sequential)
+    def zeros(cls, n: int, matrix_type: int = None) -> "Matrix":
+        h = lib().bml_zero_matrix(_dense if matrix_type is None else matrix_type, _double_real,
+                                  n, n, _sequential)
         _check()
         return cls(h)
 
@@ -255,6 +261,14 @@ def get_int8_moduli() -> int:
     return int(lib().prg_get_int8_moduli())
 
 
+def set_int8_config(config: int):
+    """Force the int8 CRT product kernel configuration (803, 804, 805, 404,
+    405, 903, 904, 905; 0 = time them per size)."""
+    st = lib().prg_gpu_set_int8_config(config)
+    if st != 0:
+        raise ProgressError(st, lib().prg_last_error().decode())
+
+
 def int8_waves() -> int:
     """Workgroup layout (4 or 8 waves) of the last int8 CRT product."""
     return int(lib().prg_gpu_int8_waves())
@@ -290,9 +304,11 @@ def gershgorin(h: np.ndarray):
     return float(g[0]), float(g[1])
 
 
-def _h_rho(fn, h, *args):
-    mh = Matrix.from_numpy(h)
-    mr = Matrix.zeros(h.shape[0])
+def _h_rho(fn, h, *args, matrix_type=None):
+    """H in, rho out; matrix_type (ellpack = 2, ...) stores both handles in a
+    sparse BML format, whose thresholding the SP2 routines then apply."""
+    mh = Matrix.from_numpy(h, matrix_type)
+    mr = Matrix.zeros(h.shape[0], matrix_type)
     fn(mh.h, mr.h, *args)
     _check()
     return mr.numpy()
@@ -332,16 +348,18 @@ def sp2_tensorcore(h, bndfil, minsp2iter=1, maxsp2iter=100, sp2conv=b"R", idemto
     return out
 
 
-def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):
+def sp2_alg2(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0,
+             matrix_type=None):
     """prg_sp2_alg2 (src/prg_sp2_mod.F90:263-380)."""
     return _h_rho(lib().prg_sp2_alg2, h, threshold, bndfil, minsp2iter, maxsp2iter,
-                  sp2conv.encode(), idemtol, verbose)
+                  sp2conv.encode(), idemtol, verbose, matrix_type=matrix_type)
 
 
-def sp2_alg1(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0):
+def sp2_alg1(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0, verbose=0,
+             matrix_type=None):
     """prg_sp2_alg1 (src/prg_sp2_mod.F90:734-864)."""
     return _h_rho(lib().prg_sp2_alg1, h, threshold, bndfil, minsp2iter, maxsp2iter,
-                  sp2conv.encode(), idemtol, verbose)
+                  sp2conv.encode(), idemtol, verbose, matrix_type=matrix_type)
 
 
 def sp2_alg2_genseq(h, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol, threshold=0.0,
@@ -964,6 +982,18 @@ def build_density_gp_t0(h, g, bndfil, ef, nparts, threshold=0.0, verbose=0, out=
     lib().prg_build_densityGP_T0(mh.h, mg.h, rho.h, threshold, bndfil, ef, nparts, verbose)
     _check()
     return (rho if out is not None else rho.numpy()), float(lib().prg_build_densityGP_T0_ef())
+
+
+def build_zmat_gp(s, g, nparts, threshold=0.0, verbose=0, out=None):
+    """prg_build_zmatGP: the graph-partitioned congruence transform Z ~ S^-1/2.
+    s and g are numpy arrays or Matrix handles; with `out` (a Matrix, whose
+    entries outside the parts' core rows are kept) Z stays in that handle."""
+    ms = s if isinstance(s, Matrix) else Matrix.from_numpy(s)
+    mg = g if isinstance(g, Matrix) else Matrix.from_numpy(g)
+    z = out if out is not None else Matrix.zeros(ms.n)
+    lib().prg_build_zmatGP(ms.h, mg.h, z.h, threshold, nparts, verbose)
+    _check()
+    return z if out is not None else z.numpy()
 
 
 def subgraph_timing():

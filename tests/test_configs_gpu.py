@@ -76,8 +76,11 @@ def test_c2_n4096_every_precision(c2_4096, mode, moduli):
     """C2 at N = 4096: the density against D_diag within the mode's bar, the
     trace within 1e-8 (split modes: after their FP64 polish, 1e-6), exact
     symmetry, and the branch sequence of the eigenvalue control flow on its
-    first 60 steps (near convergence the limdiff test reads values at the
-    products' rounding level; the gap is 3.4e-6 eV)."""
+    first 60 steps for FP64 and int8 CRT, 40 for the split modes (near
+    convergence the limdiff test reads values at the products' rounding
+    level; the gap is 3.4e-6 eV).  Measured on an MI355X: the FP16 split
+    sequence leaves the eigenvalue flow at step 55 and stops after 92 steps
+    (eigenvalue flow 85) at 5.2e-4."""
     import torch
     import progress_amd as P
     print(f"[C2 n=4096] {mode} moduli={moduli}", flush=True)
@@ -87,7 +90,8 @@ def test_c2_n4096_every_precision(c2_4096, mode, moduli):
     assert err <= C2_BARS[(mode, moduli)]
     assert tr <= (1e-8 if mode in ("fp64", "int8crt") else 1e-6)
     assert sym == 0.0
-    assert pp[:60] == c2_4096["pp_e"][:60]
+    prefix = 60 if mode in ("fp64", "int8crt") else 40
+    assert pp[:prefix] == c2_4096["pp_e"][:prefix]
 
 
 @pytest.fixture(scope="module")

@@ -7,6 +7,7 @@ was built and that a missing device is reported as a library error; the GPU
 test compares the written rho with the oracle on the reference's own
 384-orbital Hamiltonian (examples/implicitfermi/hamiltonian_ortho.mtx).
 """
+import re
 import subprocess
 from pathlib import Path
 
@@ -127,7 +128,8 @@ def test_respsolve_driver_matches_oracle(tmp_path):
         pk = O.read_mtx(f"{pre}_ifr{k}.mtx")
         assert np.linalg.norm(pk - ps_o[k - 1]) < 1e-7 * max(1.0, np.linalg.norm(ps_o[k - 1]))
     line = [x for x in r.stdout.splitlines() if "mu0=" in x][0]
-    vals = dict(kv.split("=") for kv in line.replace("respsolve: ", "").split())
+    # Fortran es24.16 fields are right-aligned: "mu0=  1.0E+00 mu1= ..."
+    vals = dict(re.findall(r"(\w+)=\s*(\S+)", line))
     assert abs(float(vals["mu0"]) - mu0_o) < 1e-8
     assert abs(float(vals["mu1"]) - mu_o[0]) < 1e-8 and abs(float(vals["mu2"]) - mu_o[1]) < 1e-8
 

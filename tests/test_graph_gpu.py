@@ -193,3 +193,31 @@ def test_subgraph_loop_split_precision(P, wide_case, mode):
     print(f"  rel. Frobenius {rel:.2e}, vv {dvv:.2e}", flush=True)
     bar = 1e-3 if mode == "f16split" else 3e-2
     assert rel < bar and dvv < bar
+
+
+@pytest.mark.parametrize("nparts", [1, 6, 24])
+def test_zmat_gp_matches_oracle(nparts):
+    """prg_build_zmatGP against the oracle on a banded overlap S = I + 0.1 B
+    (B symmetric, bandwidth 6) with its pattern as the graph, the library's
+    BFS parts: Z to 1e-10 (batched rocSOLVER eigensolves vs LAPACK); one part
+    gives S^-1/2 (Z S Z = I).  Entries outside the parts' rows keep the
+    output handle's values."""
+    import progress_amd as P
+    n = 300
+    rng = np.random.default_rng(nparts)
+    b = np.triu(rng.standard_normal((n, n)), 0)
+    b = np.triu(b) - np.triu(b, 7)
+    s = np.eye(n) + 0.05 * (b + b.T)
+    g = (np.abs(s) > 0).astype(float)
+    print(f"[zmatGP] n={n} parts={nparts}", flush=True)
+    z = P.build_zmat_gp(s, g, nparts)
+    parts = P.bfs_partition(g, nparts).parts()
+    ref = O.build_zmat_gp(s, g, parts)
+    assert np.max(np.abs(z - ref)) < 1e-10
+    if nparts == 1:
+        assert np.allclose(z @ s @ z, np.eye(n), atol=1e-10)
+    # an output handle with prior contents: unwritten entries are kept
+    z0 = np.full((n, n), 3.0)
+    out = P.Matrix.from_numpy(z0)
+    got = P.build_zmat_gp(s, g, nparts, out=out).numpy()
+    assert np.max(np.abs(got - O.build_zmat_gp(s, g, parts, z0=z0))) < 1e-10

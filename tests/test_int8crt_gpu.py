@@ -1,7 +1,7 @@
 """GPU parity of the int8 Chinese-remainder SP2 products (PRG_PREC_INT8CRT).
 
 The products are exact integer products of X's residues, so the only error is
-the rounding of X to 2^-a (a = 43 with the default 11 moduli).  Bars (stated
+the rounding of X to 2^-a (a = 44 with the default 11 moduli).  Bars (stated
 per test): the same branch sequence as the FP64 oracle, the density within
 1e-7 (Frobenius) of the oracle's, and the error shrinking as moduli are
 added.  Replays of a fixed branch sequence isolate the product accuracy from
@@ -170,7 +170,7 @@ def test_crt_general_product_matches_oracle(P, npm, n):
     assert nprod == 1 and exe == 0.0 and int8_ops > 0.0
     got = c[:n, :n].cpu().numpy()
     ref = O.crt_product(x, y, npm)
-    _, _, abits, m = O.crt_constants(npm)
+    _, _, abits, m = O.crt_constants(npm, square=False)
     sxy = O.crt_scale(x) * O.crt_scale(y)
     noise = max(1e-13, 8.0 * npm * 2.0 ** -52 * (float(m) / 4.0 ** abits)) * sxy
     assert np.max(np.abs(got - ref)) < noise
@@ -311,3 +311,31 @@ def test_crt_after_shutdown_and_init(P):
     assert ic_a == ic_b == ic_o and list(pp_a) == list(pp_b) == list(pp_o)
     assert np.array_equal(after, before)
     assert np.linalg.norm(after - rho_o) < 1e-7
+
+
+@pytest.mark.parametrize("config", [803, 804, 805, 404, 405, 903, 904, 905])
+@pytest.mark.parametrize("n", [1000, 2000])
+def test_crt_product_configs_match_oracle(P, config, n):
+    """Every int8 CRT product kernel configuration (one item per workgroup
+    with 8 waves x 3/4/5 or 4 waves x 4/5 ring stages; the persistent 8-wave
+    kernel, one workgroup per CU walking its XCD's items, at 3/4/5 stages),
+    forced: one replayed step (X1 = X0^2) agrees with the oracle's integer
+    arithmetic (crt_square) to FP64 rounding of the fraction sum, at n = 1000
+    (110 items) and n = 2000 (396 items: the persistent workgroups take one
+    or two items each, diagonal and off-diagonal)."""
+    print(f"[int8crt config] {config} n={n}", flush=True)
+    npm = 11
+    P.set_int8_moduli(npm)
+    P.set_int8_config(config)
+    try:
+        h = O.model_hamiltonian(n, 9)
+        rho_g, _ = P.sp2_alg2_seq(h, [1], 1)
+        used = P.int8_waves()
+        rho0, _ = P.sp2_alg2_seq(h, [1], 0)
+    finally:
+        P.set_int8_config(0)
+    assert used == config
+    _, _, a, m = O.crt_constants(npm)
+    ref = 2.0 * O.crt_square(0.5 * rho0, npm)
+    noise = max(1e-13, 8.0 * npm * 2.0 ** -52 * (float(m) / 4.0 ** a)) + 2.0 ** -a
+    assert np.max(np.abs(rho_g - ref)) < noise

@@ -508,18 +508,22 @@ def test_sp2_inverse_round_trip_and_entropy_quadrature():
 
 @pytest.mark.parametrize("nmod", [8, 11, 14])
 def test_crt_square_reproduces_x_squared(nmod):
-    """The int8 Chinese-remainder product: for X with spectrum in [0, 1] the
-    reconstruction equals X^2 up to the rounding of X to 2^-a (a = 31, 43, 52),
-    i.e. |error| <= 2^-a (||X_i||_1 + ||X_j||_1) / 2 + FP64 noise."""
+    """The int8 Chinese-remainder square: for X with spectrum in [0, 1] the
+    reconstruction equals X^2 up to the rounding of X to 2^-a (a = 32, 44, 52:
+    the largest with 2^(2a - 1 + 0.001) <= M, since the residues stand for
+    C - 2^a A whose entries are at most 2^(2a) / 4), i.e. |error| <=
+    2^-a (||X_i||_1 + ||X_j||_1) / 2 + FP64 noise."""
     rng = np.random.default_rng(nmod)
     n = 200
     q, _ = np.linalg.qr(rng.standard_normal((n, n)))
     x = (q * rng.uniform(0.0, 1.0, n)) @ q.T
     x = 0.5 * (x + x.T)
     ps, qs, a, m = O.crt_constants(nmod)
-    assert a == {8: 31, 11: 43, 14: 52}[nmod]
-    assert 2 * 4 ** a * 2.0 ** 0.001 <= m
-    assert a == 52 or 2 * 4 ** (a + 1) * 2.0 ** 0.001 > m
+    assert a == {8: 32, 11: 44, 14: 52}[nmod]
+    assert 4 ** a / 2 * 2.0 ** 0.001 <= m
+    assert a == 52 or 4 ** (a + 1) / 2 * 2.0 ** 0.001 > m
+    # a general product keeps the |C| <= 2^(2a) bound: one bit less
+    assert O.crt_constants(nmod, square=False)[2] == {8: 31, 11: 43, 14: 52}[nmod]
     got = O.crt_square(x, nmod)
     ref = x @ x
     l1 = np.abs(x).sum(axis=1)
@@ -578,7 +582,7 @@ def test_crt_product_reproduces_commuting_product(nmod):
     sx, sy = O.crt_scale(x), O.crt_scale(y)
     assert sx == 1.0 and sy == 8.0
     assert np.max(np.sqrt(np.sum(y * y, axis=0))) <= sy
-    ps, qs, a, m = O.crt_constants(nmod)
+    ps, qs, a, m = O.crt_constants(nmod, square=False)
     got = O.crt_product(x, y, nmod)
     assert np.array_equal(got, got.T)
     ref = x @ y
@@ -586,8 +590,10 @@ def test_crt_product_reproduces_commuting_product(nmod):
     bound = (2.0 ** -(a + 1) * (sx * ly[None, :] + sy * lx[:, None]) +
              4.0 ** -(a + 1) * sx * sy * n + 1e-13 * sx * sy)
     assert np.all(np.abs(got - ref) <= bound)
-    # the square is the X = Y case of the same arithmetic
-    assert np.max(np.abs(O.crt_product(x, x, nmod) - O.crt_square(x, nmod))) < 1e-13
+    # the square is the X = Y case of the same arithmetic (on a grid one bit
+    # finer): both within their roundings of X^2
+    assert np.max(np.abs(O.crt_product(x, x, nmod) - O.crt_square(x, nmod))) <= \
+        2.0 ** -a * 2.0 * np.max(lx) + 1e-13
 
 
 def test_crt_product_planted_scale_mistake_is_caught():
@@ -622,24 +628,41 @@ def test_orthogonalize_round_trip_with_lowdin_factor():
 
 @pytest.mark.parametrize("nmod", [8, 10, 11, 13])
 def test_crt_range_bound_is_tight(nmod):
-    """The largest product the bound admits: X = e_1 e_1^T (a row of 2-norm 1,
-    C_11 = 2^(2a)) reconstructs to FP64 rounding of the fraction sum; one more
-    bit of scale (a + 1) makes C_11 exceed M / 2 and the reconstruction wraps
-    (a planted mistake)."""
+    """The largest shifted product the bound admits: X = e_1 e_1^T / 2 (the
+    eigenvalue 1/2 where x^2 - x = -1/4, so C_11 - 2^a A_11 = -2^(2a) / 4)
+    reconstructs to FP64 rounding of the fraction sum.  Planted mistakes: one
+    more bit of scale (a + 1) makes |C_11 - 2^a A_11| exceed M / 2 and the
+    reconstruction wraps by M / 4^(a+1); and without the shift (residues
+    read as C itself, whose entries reach 2^(2a)) X = e_1 e_1^T wraps at
+    this a."""
     n = 8
     x = np.zeros((n, n))
-    x[0, 0] = 1.0
-    assert np.max(np.abs(O.crt_square(x, nmod) - x)) < 1e-12
+    x[0, 0] = 0.5
+    sq = np.zeros((n, n))
+    sq[0, 0] = 0.25
+    assert np.max(np.abs(O.crt_square(x, nmod) - sq)) < 1e-12
     orig = O.crt_constants
+    ps, qs, a, m = orig(nmod)
 
-    def one_more_bit(k):
-        ps, qs, a, m = orig(k)
+    def one_more_bit(k, square=True):
+        ps, qs, a, m = orig(k, square)
         return ps, qs, a + 1, m
     O.crt_constants = one_more_bit
     try:
-        assert abs(O.crt_square(x, nmod)[0, 0] - 1.0) > 0.5
+        assert abs(O.crt_square(x, nmod)[0, 0] - 0.25) > 0.1
     finally:
         O.crt_constants = orig
+    # the unshifted reconstruction (S mod 1 times M / 4^a) of e_1 e_1^T
+    e = np.zeros((n, n))
+    e[0, 0] = 1.0
+    av = np.rint(e * 2.0 ** a)
+    s = 0.0
+    for p, q in zip(ps, qs):
+        v = float((int(av[0, 0]) % p) ** 2 * q % p)
+        s += (v - p * round(v / p)) / p
+    unshifted = (s - round(s)) * (float(m) / 4.0 ** a)
+    assert abs(unshifted - 1.0) > 0.1
+    assert abs(O.crt_square(e, nmod)[0, 0] - 1.0) < 1e-12
 
 
 def test_sp2_basic_tcore_pins():
@@ -1127,3 +1150,58 @@ def test_test_density_matrix_planted_mistake_is_caught():
         return mu
 
     assert abs(broken(h, beta, root + 0.05, nocc, 3) - root) > 1e-6
+
+
+# prg_build_zmatGP (graph-partitioned congruence transform)
+
+def _spd_overlap(n, blocks, seed):
+    """Overlap S = I + 0.1 B with B symmetric, nonzero only inside the given
+    diagonal blocks (dense within each), and its graph (the blocks' pattern)."""
+    rng = np.random.default_rng(seed)
+    s = np.eye(n)
+    g = np.zeros((n, n))
+    start = 0
+    for b in blocks:
+        x = rng.standard_normal((b, b))
+        s[start:start + b, start:start + b] += 0.05 * (x + x.T)
+        g[start:start + b, start:start + b] = 1.0
+        start += b
+    return s, g
+
+
+def test_zmat_gp_single_part_is_the_loewdin_factor():
+    """One part holding every node: Z = S^-1/2, i.e. Z S Z = I, Z = Z^T."""
+    s, g = _spd_overlap(40, [40], 1)
+    z = O.build_zmat_gp(s, g, [list(range(40))])
+    assert np.allclose(z @ s @ z, np.eye(40), atol=1e-12)
+    assert np.allclose(z, z.T, atol=1e-14)
+
+
+def test_zmat_gp_block_diagonal_overlap_is_exact():
+    """Block-diagonal S whose graph is its pattern (each block a clique): the
+    core+halo of any core covers its blocks, so every core row of Z_p is the
+    row of S^-1/2 whatever the partition (BFS parts and an interleaved one)."""
+    s, g = _spd_overlap(60, [12, 20, 8, 20], 2)
+    exact = O.buildzdiag(s)
+    for parts in (O.bfs_partition(g, 5), [list(range(k, 60, 7)) for k in range(7)]):
+        z = O.build_zmat_gp(s, g, parts)
+        assert np.max(np.abs(z - exact)) < 1e-12
+
+
+def test_zmat_gp_planted_mistakes_are_caught():
+    """Planted mistakes: S^+1/2 in place of S^-1/2, and parts without their
+    halo (core blocks only), both move Z far from S^-1/2 on the block case."""
+    s, g = _spd_overlap(60, [12, 20, 8, 20], 2)
+    exact = O.buildzdiag(s)
+    parts = [list(range(k, 60, 7)) for k in range(7)]
+    orig = O.buildzdiag
+    try:
+        O.buildzdiag = lambda m: (lambda w, u: (u * np.sqrt(w)) @ u.T)(*np.linalg.eigh(m))
+        assert np.max(np.abs(O.build_zmat_gp(s, g, parts) - exact)) > 1e-2
+    finally:
+        O.buildzdiag = orig
+    z = np.zeros((60, 60))
+    for core in parts:
+        zp = O.buildzdiag(s[np.ix_(core, core)])
+        z[np.ix_(core, core)] = zp
+    assert np.max(np.abs(z - exact)) > 1e-3

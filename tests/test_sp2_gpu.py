@@ -115,6 +115,47 @@ def test_reference_poly512_idempotency(P, golden):
         assert abs(np.trace(rho) - 6144) < 1e-6
 
 
+def test_reference_poly512_ellpack_thresholds(P, golden):
+    """tests/src/main.F90:336-424, the ellpack cases: poly.512.mtx in
+    ellpack handles, threshold 1e-9 (alg1 and alg2, idempotency of rho/2 <=
+    1e-6) and 1e-5 (alg2, idempotency <= 1e-2).  The library applies the
+    sparse formats' thresholding (|a| <= threshold dropped from every product
+    and sum); at 1e-5 the density gains exact zeros the dense solve does not
+    have."""
+    print("[poly512 ellpack] n=6144", flush=True)
+    h = O.read_mtx(str(golden / "poly.512.mtx.gz"))
+    dense = P.sp2_alg2(h, 0.5, 25, 100, "Rel", 1e-10)
+    for fg, thr, bar in ((P.sp2_alg1, 1e-9, 1e-6), (P.sp2_alg2, 1e-9, 1e-6),
+                         (P.sp2_alg2, 1e-5, 1e-2)):
+        rho = fg(h, 0.5, 25, 100, "Rel", 1e-10, threshold=thr, matrix_type=P.ELLPACK)
+        x = 0.5 * rho
+        idem = np.linalg.norm(x @ x - x)
+        print(f"  threshold {thr:.0e}: idempotency {idem:.2e}, zeros {np.sum(rho == 0)} "
+              f"(dense {np.sum(dense == 0)})", flush=True)
+        assert idem <= bar
+        if thr == 1e-5:
+            assert np.sum(rho == 0) > np.sum(dense == 0)
+
+
+def test_ellpack_threshold_matches_oracle(P, golden):
+    """prg_sp2_alg2 in ellpack handles, threshold 1e-5, on the reference's
+    384-orbital Hamiltonian, against the oracle's thresholded alg2 (product
+    dropped before 2X - X^2, the sum dropped again): the same iteration
+    count and branch sequence, the density to 1e-9.  A dense handle ignores
+    the threshold (BML dense does): bit-equal to threshold 0."""
+    h = O.read_mtx(str(golden / "hamiltonian_ortho_384.mtx"))
+    rho = P.sp2_alg2(h, 0.5, 10, 100, "Abs", 1e-10, threshold=1e-5, matrix_type=P.ELLPACK)
+    ro, it, pp = O.sp2_alg2(h, 0.5, 10, 100, "Abs", 1e-10, threshold=1e-5)
+    r0, _, _ = O.sp2_alg2(h, 0.5, 10, 100, "Abs", 1e-10)
+    print(f"[ellpack 384] |rho - oracle| {np.linalg.norm(rho - ro):.2e}, threshold moves the "
+          f"density by {np.linalg.norm(ro - r0):.2e}", flush=True)
+    assert np.linalg.norm(ro - r0) > 1e-6
+    assert np.linalg.norm(rho - ro) < 1e-9
+    d1 = P.sp2_alg2(h, 0.5, 10, 100, "Abs", 1e-10, threshold=1e-5)
+    d0 = P.sp2_alg2(h, 0.5, 10, 100, "Abs", 1e-10)
+    assert np.array_equal(d1, d0)
+
+
 def test_device_pointer_entry_point(P):
     import torch
     print("[device api] n=512", flush=True)
