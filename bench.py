@@ -247,6 +247,8 @@ def main():
                "bf16split": PEAK["fp16_tflops"], "int8crt": PEAK["int8_tops"]}[kind]
     kms = hr.get("product_kernel_ms")
     alg = (2.0 * float(n) ** 3 / (kms * 1e-3) / 1e12) if kms else None
+    # the symmetric square's own count: N (N + 1) / 2 distinct outputs of 2N flop
+    alg_sym = (float(n) ** 2 * (n + 1) / (kms * 1e-3) / 1e12) if kms else None
     kern = hr.get("product_kernel_mfma_tops")
     roofline = {
         "bound": "mfma",
@@ -259,11 +261,14 @@ def main():
         "peak": peak_tf,
         "unit": "TFLOP/s",
         "frac": (alg / peak_tf) if alg else None,
+        "frac_symmetric": (alg_sym / peak_tf) if alg_sym else None,
         "traffic": traffic_bytes(kind, n, hmod),
         "executed": kern,
         "executed_frac": (kern / peak_tf) if kern else None,
         "note": "achieved = algorithmic 2N^3 flop of the X^2 the launch produces / the "
-                "kernel's HIP-event average duration on the library stream; peak = the dense "
+                "kernel's HIP-event average duration on the library stream (frac_symmetric: "
+                "the symmetric square's own count N^2 (N + 1), at most 1 for a kernel that "
+                "computes each distinct output once); peak = the dense "
                 "MFMA peak of the type that executes it (FP64 78.6 TF, FP16 2.5 PF, int8 "
                 "5.0 POP/s).  executed = MFMA operations the launch issues (upper-triangle "
                 "tiles only; one pass per modulus for int8crt, 3 for f16split) / the same "

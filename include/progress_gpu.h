@@ -417,8 +417,8 @@ int prg_get_precision(void);
 int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 /* Configuration the int8 CRT product kernel last ran with, coded
- * waves * 100 + ring stages (803, 804, 805: 8 waves; 404, 405: 4 waves;
- * 903, 904, 905: the persistent 8-wave kernel, one workgroup per CU):
+ * waves * 100 + ring stages (803, 804, 805: 8 waves, one item per
+ * workgroup; 904: the persistent 8-wave kernel, one workgroup per CU):
  * chosen per (padded size, moduli) by timing each on the first product, or
  * forced with the environment variable PRG_CRT_CONFIG.  0 before any int8
  * product.  Measurement support; no reference counterpart. */

@@ -1316,6 +1316,16 @@ def crt_constants(nmod, square=True):
     return ps, qs, a, m
 
 
+def _exact_int_product(a, b):
+    """A B^T of integer matrices with entries in [-128, 128]: every partial sum
+    is an integer of magnitude <= K 2^14 < 2^53 (K < 2^17 columns), so an
+    FP64 BLAS product is exact whatever its summation order; returned as
+    int64 (numpy's int64 matmul has no BLAS path and takes minutes at
+    K = 2000)."""
+    c = np.asarray(a, dtype=np.float64) @ np.asarray(b, dtype=np.float64).T
+    return np.rint(c).astype(np.int64)
+
+
 def crt_square(x, nmod):
     """X^2 the way the int8 CRT SP2 step forms it: A = rint(2^a X), per modulus
     the exact integer product of the symmetric residues, the fraction sum
@@ -1330,7 +1340,7 @@ def crt_square(x, nmod):
         r = av - p * np.rint(av / p)
         r = np.where(r > 127, r - p, r)
         r = np.where(r < -128, r + p, r).astype(np.int64)
-        c = r @ r.T
+        c = _exact_int_product(r, r)
         v = (c * q).astype(np.float64)
         md = v - p * np.rint(v / p)
         s += md / p
@@ -1368,7 +1378,7 @@ def crt_product(x, y, nmod):
 
     s = np.zeros_like(xv)
     for p, q in zip(ps, qs):
-        c = residues(xv, p) @ residues(yv, p).T
+        c = _exact_int_product(residues(xv, p), residues(yv, p))
         v = (c * q).astype(np.float64)
         md = v - p * np.rint(v / p)
         s += md / p

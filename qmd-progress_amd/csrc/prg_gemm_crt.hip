@@ -68,6 +68,13 @@ constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239,
 
 constexpr int kRecon = 6;  // moduli that rebuild A itself (M6 = 2^47.8)
 
+constexpr bool moduli_odd_after_first() {
+  for (int k = 1; k < kMaxModuli; ++k)
+    if (kModuli[k] % 2 == 0) return false;
+  return kModuli[0] == 256;
+}
+static_assert(moduli_odd_after_first(), "the residue shortcuts assume 256 first, then odd moduli");
+
 struct CrtConst {
   double p[kMaxModuli];    // moduli
   double inv[kMaxModuli];  // 1 / p
@@ -141,6 +148,24 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
   m = (m > 127.0) ? m - p : m;
   m = (m < -128.0) ? m + p : m;
   return (int)m;
+}
+
+// The same for |v| / p < 2^38 (the product residues C q, |v| < 2^39, and
+// X_next scaled by 2^a <= 2^45): v * inv is then within 2^-14 of v / p,
+// while for odd p the exact quotient of an integer v stays at least
+// 1 / (2p) >= 2^-9 from a half-integer, so rint gives the nearest multiple
+// and v - p rint(.) lies in [-(p - 1) / 2, (p - 1) / 2]: no correction.  For
+// p = 256 the quotient is exact and a tie (remainder +128) is folded to
+// -128 by the one check, which never fires for odd p.
+__device__ __forceinline__ int sym_residue_small(double v, double p, double inv) {
+  double m = fma(-p, rint(v * inv), v);
+  m = (m > 127.0) ? m - p : m;
+  return (int)m;
+}
+
+// sym_residue_small for an odd modulus (no tie to fold).
+__device__ __forceinline__ int sym_residue_odd(double v, double p, double inv) {
+  return (int)fma(-p, rint(v * inv), v);
 }
 
 // One (tile, modulus) item: C = A_I B_J^T mod p over the full K range (B = A
@@ -317,8 +342,8 @@ __device__ __forceinline__ void item_epilogue(
         int b4 = 0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^43
-          b4 |= (sym_residue(v, p, inv) & 0xff) << (8 * r);
+          const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^39
+          b4 |= (sym_residue_small(v, p, inv) & 0xff) << (8 * r);
         }
         out[n4] = b4;
       }
@@ -573,7 +598,14 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
     for (int n = 0; n < 4; ++n) {
       int b4 = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) b4 |= (sym_residue(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+      for (int r = 0; r < 4; ++r) {
+        // with RECON |zz| <= 2^45 (a <= 45); kModuli[0] = 256 is the one
+        // even modulus
+        const int res = !RECON ? sym_residue(zz[4 * n + r], p, inv)
+                        : (k == 0 ? sym_residue_small(zz[4 * n + r], p, inv)
+                                  : sym_residue_odd(zz[4 * n + r], p, inv));
+        b4 |= (res & 0xff) << (8 * r);
+      }
       *(int*)&buf[rl][16 * n + cl0] = b4;
     }
     // the writes of modulus k - 1 from this buffer's twin are done by every
@@ -936,14 +968,15 @@ CrtConst make_const(int np, bool square) {
 inline int plane_ld(int ld) { return ((ld + TM - 1) / TM) * TM; }
 
 // Product kernel configuration, coded W * 100 + NSTAGE: 8 waves with a 3-,
-// 4- or 5-stage ring (96, 128, 160 KB of LDS) or 4 waves with 4 or 5 stages;
-// 900 + NSTAGE is the persistent 8-wave kernel.
+// 4- or 5-stage ring (96, 128, 160 KB of LDS); 904 is the persistent 8-wave
+// kernel with a 4-stage ring.  (The 4-wave layouts and the 3- and 5-stage
+// persistent forms compile with register spills and are not offered.)
 // PRG_CRT_CONFIG forces one; otherwise the first product of each
 // (pld, moduli) pair times every configuration (twice each, the minimum
 // counts) and keeps the fastest.  The product kernel only writes its residue
 // buffer, so the trial launches leave no trace and the results do not depend
 // on the choice.
-constexpr int kConfigs[] = {803, 804, 805, 404, 405, 903, 904, 905};
+constexpr int kConfigs[] = {803, 804, 805, 904};
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 int g_forced = -1;                        // -1: not read yet; 0: autotune
 std::vector<std::array<int, 3>> g_tuned;  // (pld, np, config)
@@ -978,16 +1011,8 @@ int persistent_grid(int nitems) {
 
 void launch_product(const ProductArgs& g, int nitems, int config) {
   switch (config) {
-    case 903:
-      hipLaunchKernelGGL((crt_product_persistent_kernel<8, 3>), dim3(persistent_grid(nitems)),
-                         dim3(512), 0, stream(), g);
-      break;
     case 904:
       hipLaunchKernelGGL((crt_product_persistent_kernel<8, 4>), dim3(persistent_grid(nitems)),
-                         dim3(512), 0, stream(), g);
-      break;
-    case 905:
-      hipLaunchKernelGGL((crt_product_persistent_kernel<8, 5>), dim3(persistent_grid(nitems)),
                          dim3(512), 0, stream(), g);
       break;
     case 803:
@@ -995,12 +1020,6 @@ void launch_product(const ProductArgs& g, int nitems, int config) {
       break;
     case 805:
       hipLaunchKernelGGL((crt_product_kernel<8, 5>), dim3(nitems), dim3(512), 0, stream(), g);
-      break;
-    case 404:
-      hipLaunchKernelGGL((crt_product_kernel<4, 4>), dim3(nitems), dim3(256), 0, stream(), g);
-      break;
-    case 405:
-      hipLaunchKernelGGL((crt_product_kernel<4, 5>), dim3(nitems), dim3(256), 0, stream(), g);
       break;
     default:
       hipLaunchKernelGGL((crt_product_kernel<8, 4>), dim3(nitems), dim3(512), 0, stream(), g);

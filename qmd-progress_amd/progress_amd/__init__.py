@@ -262,8 +262,8 @@ def get_int8_moduli() -> int:
 
 
 def set_int8_config(config: int):
-    """Force the int8 CRT product kernel configuration (803, 804, 805, 404,
-    405, 903, 904, 905; 0 = time them per size)."""
+    """Force the int8 CRT product kernel configuration (803, 804, 805, 904;
+    0 = time them per size)."""
     st = lib().prg_gpu_set_int8_config(config)
     if st != 0:
         raise ProgressError(st, lib().prg_last_error().decode())

@@ -1205,3 +1205,39 @@ def test_zmat_gp_planted_mistakes_are_caught():
         zp = O.buildzdiag(s[np.ix_(core, core)])
         z[np.ix_(core, core)] = zp
     assert np.max(np.abs(z - exact)) > 1e-3
+
+
+def test_crt_symmetric_residue_without_correction():
+    """The int8 CRT kernels' residue shortcut: for an odd modulus p <= 255 and
+    an integer-valued double |v| / p < 2^38, v - p rint(v (1/p)) is already
+    the symmetric residue in [-(p-1)/2, (p-1)/2] (the FP64 quotient is within
+    2^-14 of v / p, which stays at least 1/(2p) from a half-integer); for p =
+    256 only the tie +128 needs folding to -128.  Checked in IEEE FP64 on
+    random values up to 2^45 and on every residue class near the top."""
+    rng = np.random.default_rng(11)
+    for p in O.CRT_MODULI:
+        inv = 1.0 / p
+        top = 2.0 ** 45
+        v = np.concatenate([np.rint(rng.uniform(-top, top, 20000)),
+                            np.floor(top / p) * p - np.arange(2 * p, dtype=float),
+                            -np.floor(top / p) * p + np.arange(2 * p, dtype=float)])
+        m = v - p * np.rint(v * inv)
+        if p == 256:
+            m = np.where(m > 127, m - p, m)
+        exact = np.array([((int(x) + p // 2) % p) - p // 2 for x in v], dtype=float)
+        if p == 256:
+            exact = np.where(exact > 127, exact - p, exact)
+        assert np.array_equal(m, exact), p
+        assert m.min() >= -128 and m.max() <= 127
+
+
+def test_crt_exact_int_product_equals_int64_matmul():
+    """The oracle's FP64 route to the exact residue products equals numpy's
+    int64 matmul on residue-range matrices (including all +-128 entries,
+    the largest partial sums)."""
+    rng = np.random.default_rng(4)
+    a = rng.integers(-128, 128, size=(150, 300))
+    b = rng.integers(-128, 128, size=(120, 300))
+    assert np.array_equal(O._exact_int_product(a, b), a @ b.T)
+    full = np.full((8, 1000), -128, dtype=np.int64)
+    assert np.array_equal(O._exact_int_product(full, full), full @ full.T)
