@@ -439,6 +439,20 @@ int prg_gpu_set_int8_config(int config);
 int prg_gpu_sp2_alg2_device(int N, const double *dH, double *dRho, double bndfil, int minsp2iter,
                             int maxsp2iter, const char *sp2conv, double idemtol, int precision,
                             int *iters, int *pp);
+/* prg_gpu_sp2_alg2_device in FP64 without any allocation: no counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving or MD loops, harnesses that forbid hipMalloc).  The
+ * ping-pong X, partial sums, state and K-split accumulators live in a static
+ * device array of 52.2 MiB held by every device the library runs on until
+ * the process ends; N is limited to 1024 (larger N returns status 1, bad
+ * argument).  prg_progress_init must have been called on the device (it
+ * resolves the array's address; otherwise status 6, no device).  The array
+ * keeps the last call's contents; calls that overlap on one device race.
+ * Same arguments and results as prg_gpu_sp2_alg2_device with precision
+ * PRG_PREC_FP64. */
+int prg_gpu_sp2_alg2_device_noalloc(int N, const double *dH, double *dRho, double bndfil,
+                                    int minsp2iter, int maxsp2iter, const char *sp2conv,
+                                    double idemtol, int *iters, int *pp);
 /* New C symbols for the linear solvers of prg_implicit_fermi_mod.F90 (Fortran
  * argument order; the auxiliary matrices are unused and may be NULL):
  * prg_conjgrad (:1006-1048) and prg_pcg (:934-996, Jacobi preconditioner)

@@ -55,6 +55,10 @@ struct DMat {
 // ---------------------------------------------------------------- runtime
 hipStream_t stream();
 int ensure_device();  // initialises the device once; returns Status
+bool device_ready();  // ensure_device has run and succeeded
+// Resolves the no-allocation scratch array on the current device (once per
+// device); called by prg_progress_init.
+int noalloc_setup();
 
 // Scratch buffer cache keyed by name; grows on demand and is reused across
 // calls on the library stream (calls are serialised on that stream).

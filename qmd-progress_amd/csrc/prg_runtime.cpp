@@ -66,6 +66,12 @@ int ensure_device() {
   return OK;
 }
 
+bool device_ready() {
+  Runtime& r = rt();
+  std::lock_guard<std::mutex> lk(r.mu);
+  return r.init && r.init_status == OK;
+}
+
 hipStream_t stream() { return rt().stream; }
 
 void* workspace(const char* name, size_t bytes, int* status) {
@@ -141,7 +147,7 @@ void prg_version(void) { std::printf("progress-mi355x density-matrix solvers (gf
 
 void prg_progress_init(void) {
   prg::clear_error();
-  prg::ensure_device();
+  if (prg::ensure_device() == prg::OK) prg::noalloc_setup();
 }
 
 void prg_progress_shutdown(void) {

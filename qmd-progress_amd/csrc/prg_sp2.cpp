@@ -22,6 +22,42 @@
 namespace prg {
 
 namespace {
+// Static device scratch of the no-allocation SP2 entry point
+// (prg_gpu_sp2_alg2_device_noalloc, FP64, N <= 1024): the ping-pong X, the
+// K-split accumulators, the partial sums, the bounds, the state and the
+// Gershgorin scratch, each at a 256-byte aligned offset.  Every device the
+// library runs on holds its own copy for the life of the process; calls that
+// overlap on one device share it and race.
+constexpr int kNoallocMaxLd = 1024;
+constexpr size_t kMatBytes = sizeof(double) * (size_t)kNoallocMaxLd * kNoallocMaxLd;
+// K-split accumulators: room for 288 partial 128 x 128 tiles (36 MiB), at
+// least what the split rule asks for with ld <= 1024; checked per call
+constexpr size_t kSplitBytes = sizeof(double) * 36 * 8 * 128 * 128;
+constexpr size_t kPartBytes = 128 * 1024;
+constexpr size_t kBoundsBytes = 256;
+constexpr size_t kStateBytes = 16 * 1024;
+constexpr size_t kGershBytes = 32 * 1024;
+constexpr size_t kOffX0 = 0;
+constexpr size_t kOffX1 = kOffX0 + kMatBytes;
+constexpr size_t kOffSplit = kOffX1 + kMatBytes;
+constexpr size_t kOffPart = kOffSplit + kSplitBytes;
+constexpr size_t kOffBounds = kOffPart + kPartBytes;
+constexpr size_t kOffState = kOffBounds + kBoundsBytes;
+constexpr size_t kOffGersh = kOffState + kStateBytes;
+constexpr size_t kScratchBytes = kOffGersh + kGershBytes;  // 52.2 MiB
+static_assert(sizeof(Sp2State) <= kStateBytes, "state slot too small");
+static_assert(kOffSplit % 256 == 0 && kOffPart % 256 == 0 && kOffState % 256 == 0 &&
+                  kOffGersh % 256 == 0,
+              "scratch offsets must stay 256-byte aligned");
+
+__device__ __align__(256) char scratch[kScratchBytes];
+
+// Device address of `scratch` per device (host code uses these, never the
+// array's name), and the host state images of the no-allocation loop.
+constexpr int kMaxDevices = 64;
+char* g_scratch_ptr[kMaxDevices] = {};
+Sp2State g_noalloc_host[3];
+
 int g_precision = PRG_PREC_FP64;
 
 // Optional per-launch timing of the product kernel (HIP events on the library
@@ -60,6 +96,24 @@ bool fortran_eq(const char* s, const char* lit) {
   return n == std::strlen(lit) && std::strncmp(s, lit, n) == 0;
 }
 }  // namespace
+
+int noalloc_setup() {
+  int dev = 0;
+  PRG_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error(ERR_ARG, "no-allocation scratch: device index out of range");
+    return ERR_ARG;
+  }
+  if (g_scratch_ptr[dev]) return OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return ERR_HIP;
+  }
+  g_scratch_ptr[dev] = (char*)ptr;
+  return OK;
+}
 
 int precision_mode() { return g_precision; }
 void set_precision_mode(int m) { g_precision = m; }
@@ -650,6 +704,53 @@ int prg_gpu_step_timing(double* ms_sum, int* count) {
 int prg_gpu_int8_product_timing(double* ms_sum, int* count) {
   crt_timing_get(ms_sum, count);
   return OK;
+}
+
+int prg_gpu_sp2_alg2_device_noalloc(int N, const double* dH, double* dRho, double bndfil,
+                                    int minsp2iter, int maxsp2iter, const char* sp2conv,
+                                    double idemtol, int* iters, int* pp) {
+  clear_error();
+  if (N <= 0 || !dH || !dRho) {
+    set_error(ERR_ARG, "prg_gpu_sp2_alg2_device_noalloc: bad arguments");
+    return ERR_ARG;
+  }
+  const int ld = padded(N);
+  if (ld > kNoallocMaxLd) {
+    set_error(ERR_ARG, "prg_gpu_sp2_alg2_device_noalloc: N exceeds the static scratch "
+                       "(N <= 1024)");
+    return ERR_ARG;
+  }
+  if (!device_ready()) {
+    set_error(ERR_NO_DEVICE, "prg_gpu_sp2_alg2_device_noalloc: call prg_progress_init first");
+    return ERR_NO_DEVICE;
+  }
+  int st = noalloc_setup();
+  if (st) return st;
+  int dev = 0;
+  PRG_HIP_CHECK(hipGetDevice(&dev));
+  char* base = g_scratch_ptr[dev];
+  if (sp2_split_bytes(ld) > kSplitBytes || gershgorin_scratch_doubles(N) * 8 > kGershBytes) {
+    set_error(ERR_ARG, "prg_gpu_sp2_alg2_device_noalloc: scratch layout too small");
+    return ERR_ARG;
+  }
+  Sp2Buffers b;
+  b.x0 = (double*)(base + kOffX0);
+  b.x1 = (double*)(base + kOffX1);
+  b.split_acc = (double*)(base + kOffSplit);
+  b.part = (double*)(base + kOffPart);
+  b.bounds = (double*)(base + kOffBounds);
+  b.state = (Sp2State*)(base + kOffState);
+  b.gersh = (double*)(base + kOffGersh);
+  b.host = g_noalloc_host;
+  DMat H{N, ld, const_cast<double*>(dH)}, R{N, ld, dRho};
+  Sp2Params p = base_params(V_ALG2, N, bndfil, minsp2iter, maxsp2iter, sp2conv, idemtol);
+  p.precision = PRG_PREC_FP64;
+  Sp2Result r;
+  st = run_sp2(H, p, R, &r, nullptr, nullptr, &b);
+  if (iters) *iters = r.iter;
+  if (pp)
+    for (int i = 0; i < r.iter && i < maxsp2iter; ++i) pp[i] = r.pp[i];
+  return st;
 }
 
 int prg_gpu_sp2_alg2_device(int N, const double* dH, double* dRho, double bndfil, int minsp2iter,
