@@ -255,14 +255,14 @@ def main():
         "kernel": {"fp64": "sp2_step_kernel (FP64 MFMA, fused SP2 update)",
                    "f16split": "sp2_f16split_kernel (FP16 MFMA, 3 products)",
                    "bf16split": "sp2_f16split_kernel<bf16> (BF16 MFMA, 3 products)",
-                   "int8crt": f"crt_product_kernel (int8 MFMA, {hmod} moduli x "
-                              "upper-triangle 256x256 tiles)"}[kind],
+                   "int8crt": f"{crt_kernel_name(hr.get('int8_config'))} (int8 MFMA, {hmod} "
+                              "moduli x upper-triangle 256x256 tiles)"}[kind],
         "achieved": alg,
         "peak": peak_tf,
         "unit": "TFLOP/s",
         "frac": (alg / peak_tf) if alg else None,
         "frac_symmetric": (alg_sym / peak_tf) if alg_sym else None,
-        "traffic": traffic_bytes(kind, n, hmod),
+        "traffic": traffic_bytes(kind, n, hmod, hr.get("int8_config")),
         "executed": kern,
         "executed_frac": (kern / peak_tf) if kern else None,
         "note": "achieved = algorithmic 2N^3 flop of the X^2 the launch produces / the "
@@ -333,6 +333,12 @@ def batched_tile_count(m, panel):
     return k * (k + 1) // 2
 
 
+def progress(msg):
+    """One line on stderr per setup phase or step, so that a long untimed
+    setup is visible while it runs (the JSON line stays alone on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def graph_bench(args, P, torch, np):
     """Graph-partitioned SP2 (prg_subgraphSP2Loop) on the water-box system:
     natoms atoms x 3 orbitals.  Setup (untimed): a dense SP2 solve of the whole
@@ -350,6 +356,7 @@ def graph_bench(args, P, torch, np):
     bndfil = 1.0 / 3.0 if gapped else 0.5
     rows, cols, vals, n, box = W.water_box(natoms=args.natoms, onsite=onsite, seed=args.seed)
     t_gen = time.time() - t0
+    progress(f"water box generated: {n} orbitals, {len(vals)} entries, {t_gen:.1f} s")
     dev = torch.device("cuda:0")
     ld = P.padded_ld(n)
     hd = torch.zeros((ld, ld), dtype=torch.float64, device=dev)
@@ -362,16 +369,20 @@ def graph_bench(args, P, torch, np):
                                precision=P.PREC_INT8CRT)
     torch.cuda.synchronize()
     t_dense = time.time() - t0
+    progress(f"dense int8 CRT solve: {it} iterations, {t_dense:.1f} s")
     lo, hi = P.gershgorin_device(hd, n)
     d_dense = rd[:n, :n]
     g = (d_dense.abs() >= args.graph_threshold).to(torch.float64).cpu().numpy()
     h = hd[:n, :n].cpu().numpy()
+    progress("graph |D| >= threshold copied to the host")
     gp = P.bfs_partition(g.T, args.nparts)
+    progress(f"BFS partition: {gp.totalParts} parts")
     # library handles built once; their device copies stay resident, so each
     # timed call is the whole prg_subgraphSP2Loop: core+halo extraction on the
     # GPU, the host batch plan, gather, batched purification, scatter
     mh, mg, mr = P.Matrix.from_numpy(h), P.Matrix.from_numpy(g), P.Matrix.zeros(n)
     del g
+    progress("library handles of H, the graph and rho built")
     times = []
     walls = []
     gprec = {"fp64": P.PREC_FP64, "f16split": P.PREC_F16SPLIT,
@@ -383,6 +394,7 @@ def graph_bench(args, P, torch, np):
             t0 = time.perf_counter()
             _, vv = P.subgraph_sp2_loop(mh, mg, gp, pp, it, lo, hi, out=mr)
             wall = time.perf_counter() - t0
+            progress(f"subgraph loop call {k}: {wall * 1e3:.1f} ms")
             ga, lp, sc = P.subgraph_timing()
             if k >= args.warmup:
                 times.append((ga + lp + sc) * 1e-3)
@@ -391,6 +403,7 @@ def graph_bench(args, P, torch, np):
     finally:
         P.set_precision(P.PREC_FP64)
     rho = mr.numpy()
+    progress("accuracy against the dense solve")
     ls = [gp.lsize[i] for i in range(gp.totalParts)]
     core = [gp.llsize[i] for i in range(gp.totalParts)]
     dt = sum(walls) / len(walls)
@@ -738,15 +751,25 @@ def executed_ops(ld, mode, moduli):
     return per_pass * {"fp64": 1, "f16split": 3, "bf16split": 3, "int8crt": moduli}[mode]
 
 
-def traffic_bytes(mode, n, moduli):
+def crt_kernel_name(config):
+    """The int8 product kernel a configuration code launches (9xx: the
+    persistent one-workgroup-per-CU form; 8xx: one workgroup per item)."""
+    if config is not None and int(config) >= 900:
+        return "crt_product_persistent_kernel"
+    return "crt_product_kernel"
+
+
+def traffic_bytes(mode, n, moduli, config=None):
     """HBM bytes per launch of the dominant kernel from profiles/traffic.json
     (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the calibration copy),
-    when that file holds this kernel at this size; otherwise None."""
+    when that file holds this kernel at this size (and int8 configuration,
+    where the entry names one); otherwise None."""
     path = ROOT / "profiles" / "traffic.json"
     if not path.exists():
         return None
     for e in json.loads(path.read_text()).get("kernels", []):
-        if e.get("mode") == mode and e.get("n") == n and e.get("moduli", moduli) == moduli:
+        if e.get("mode") == mode and e.get("n") == n and e.get("moduli", moduli) == moduli \
+                and e.get("config", config) == config:
             return e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024
     return None
 

@@ -15,6 +15,10 @@
 // Frobenius partial sums for the next branch decision, and the idempotency
 // sums ||X - X^2||_F^2 (genseq) or its core-row restriction (submatrix).
 #include "prg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include "prg_tile.h"
 
 namespace prg {
@@ -290,7 +294,9 @@ __global__ __launch_bounds__(NT, 2) void sp2_split_product_kernel(Sp2Args g) {
   if (!s_flags[0]) return;
   const double* X = g.buf[s_flags[1]];
   const int t = blockIdx.x / g.split, slice = blockIdx.x % g.split;
-  const int kd = g.kdim / g.split;
+  // slice's K steps: [slice ks / split, (slice + 1) ks / split)
+  const int ks = g.kdim / BK;
+  const int k0 = (slice * ks) / g.split, k1 = ((slice + 1) * ks) / g.split;
   int I, J;
   sym_tile(t, I, J);
   d4 acc[4][4];
@@ -300,7 +306,7 @@ __global__ __launch_bounds__(NT, 2) void sp2_split_product_kernel(Sp2Args g) {
     for (int n = 0; n < 4; ++n) acc[m][n] = d4{0.0, 0.0, 0.0, 0.0};
   Operands op;
   op.A[0] = X; op.B[0] = X; op.A[1] = X; op.B[1] = X; op.npair = 1;
-  mainloop(op, g.ld, slice * kd, kd, I, J, As, Bs, acc);
+  mainloop(op, g.ld, k0 * BK, (k1 - k0) * BK, I, J, As, Bs, acc);
   d4* dst = (d4*)(g.acc + (size_t)blockIdx.x * BM * BN) + threadIdx.x;
 #pragma unroll
   for (int m = 0; m < 4; ++m)
@@ -443,13 +449,37 @@ int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, con
   return OK;
 }
 
+// K slices of the SP2 square for a padded size ld.  Below 1024 upper-triangle
+// tiles the tiles leave CUs idle (36 at ld = 1024, 528 at 4096, for 256 CUs),
+// so the K range is cut into s slices (uneven, at least 4 K steps of 16
+// each, s <= 16) chosen by a makespan model: ceil(nb s / 256) slices per CU
+// of 1/s of a tile's MFMA time each (128 x 128 x ld at ~0.24 TFLOP/s per
+// CU), plus the partial tiles' round trip through HBM (2 x 128 KB per slice
+// at ~5 TB/s) and the extra launch (~5 us).  PRG_SP2_SPLIT=s forces a count.
 int sp2_split_count(int ld) {
-  // enough (tile, slice) workgroups to cover the 256 CUs, each slice at least
-  // 8 K steps deep (ld is a multiple of 128, so ld / 8 is a multiple of BK)
   const int nt = ld / BM, nb = nt * (nt + 1) / 2;
-  int split = 1;
-  while (nb * split < 256 && split < 8 && ld / (2 * split) >= 8 * BK) split *= 2;
-  return split;
+  const int ks = ld / BK;
+  const int smax = std::min(16, ks / 4);
+  static int forced = -1;
+  if (forced < 0) {
+    const char* e = std::getenv("PRG_SP2_SPLIT");
+    forced = e ? std::max(0, std::atoi(e)) : 0;
+  }
+  if (forced > 0) return std::max(1, std::min(forced, smax));
+  if (nb >= 1024) return 1;
+  const double tile_s = 2.0 * BM * BN * (double)ld / 0.24e12;
+  int best = 1;
+  double best_t = 1e30;
+  for (int sl = 1; sl <= smax; ++sl) {
+    const double rounds = std::ceil((double)nb * sl / 256.0);
+    double t = rounds * tile_s / sl;
+    if (sl > 1) t += 2.0 * nb * sl * (BM * BN * 8.0) / 5.0e12 + 5e-6;
+    if (t < best_t * 0.97) {  // a split must win by 3 %
+      best_t = t;
+      best = sl;
+    }
+  }
+  return best;
 }
 
 size_t sp2_split_bytes(int ld) {
