@@ -418,7 +418,8 @@ int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 /* Configuration the int8 CRT product kernel last ran with, coded
  * waves * 100 + ring stages (803, 804, 805: 8 waves, one item per
- * workgroup; 904: the persistent 8-wave kernel, one workgroup per CU):
+ * workgroup; 904: the persistent 8-wave kernel, one workgroup per CU; 914:
+ * 904 with its LDS fragment reads interleaved among the MFMAs):
  * chosen per (padded size, moduli) by timing each on the first product, or
  * forced with the environment variable PRG_CRT_CONFIG.  0 before any int8
  * product.  Measurement support; no reference counterpart. */
@@ -443,7 +444,7 @@ int prg_gpu_sp2_alg2_device(int N, const double *dH, double *dRho, double bndfil
  * the reference.  For callers that cannot allocate (code under HIP graph
  * capture, tight serving or MD loops, harnesses that forbid hipMalloc).  The
  * ping-pong X, partial sums, state and K-split accumulators live in a static
- * device array of 52.2 MiB held by every device the library runs on until
+ * device array of 52.3 MiB held by every device the library runs on until
  * the process ends; N is limited to 1024 (larger N returns status 1, bad
  * argument).  prg_progress_init must have been called on the device (it
  * resolves the array's address; otherwise status 6, no device).  The array

@@ -95,7 +95,7 @@ struct ProductArgs {
   int8_t* res;              // product residues, item (t, k) at (t * np + k) * kItemBytes
   const int4* items;        // (modulus, I, J, upper-triangle index of (I, J))
   const Sp2State* st;       // SP2 state (planned, current buffer), or null: planes[0]
-  const double* fold;       // (p, 1/p, q) per modulus
+  const double* fold;       // (p, 1/p, q, q/p) per modulus
   int pld, np;
   int nitems;               // items in the list (the persistent kernel's range)
 };
@@ -166,6 +166,17 @@ __device__ __forceinline__ int sym_residue_small(double v, double p, double inv)
 // sym_residue_small for an odd modulus (no tie to fold).
 __device__ __forceinline__ int sym_residue_odd(double v, double p, double inv) {
   return (int)fma(-p, rint(v * inv), v);
+}
+
+// 1.5 * 2^52: for an integer m with |m| < 2^51, the low 32 bits of the double
+// kMagic + m (exact) are m as a two's-complement int32, so a residue formed
+// as fma(-p, t, v + kMagic) is read without a conversion.
+constexpr double kMagic = 6755399441055744.0;
+
+// The low byte of the residue carried by kMagic + m (the int8 stored for m;
+// +128 and -128, the two readings of a p = 256 tie, share the byte 0x80).
+__device__ __forceinline__ uint32_t magic_byte(double w) {
+  return (uint32_t)__double2loint(w) & 0xffu;
 }
 
 // One (tile, modulus) item: C = A_I B_J^T mod p over the full K range (B = A
@@ -270,7 +281,7 @@ __device__ __forceinline__ void issue_prologue(const ItemSrc& src, size_t kstrid
 // s + NSTAGE, then run the MFMAs of stage s while reading stage s + 1.  The
 // first step starts from zero accumulators (peeled, so that the loop carries
 // only MFMA results and they stay in AGPRs).
-template <int W, int NSTAGE, bool DIAG>
+template <int W, int NSTAGE, bool DIAG, bool IL = false>
 __device__ __forceinline__ void item_mainloop(
     const ItemSrc& src, size_t kstride, const ProductLane<W>& L, int ksteps,
     const int8_t (*ring)[kStage], bool all_landed,
@@ -301,6 +312,17 @@ __device__ __forceinline__ void item_mainloop(
       for (int n = 0; n < NTL; ++n)
         acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
                                                           first ? zero : acc[m][n], 0, 0, 0);
+    if (IL) {
+      // issue order of the step: one fragment read of stage s + 1, then two
+      // MFMAs of stage s, for all MT + NTL reads; the remaining MFMAs follow
+      // and cover the reads' latency before the next barrier waits for them
+#pragma unroll
+      for (int i = 0; i < MT + NTL; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, MT * NTL - 2 * (MT + NTL), 0);
+    }
   };
   if (all_landed || NSTAGE - 1 >= ksteps)
     wait_vm_barrier<0>();
@@ -329,7 +351,12 @@ __device__ __forceinline__ void item_epilogue(
     const i4 (&acc)[ProductLane<W>::MT][ProductLane<W>::NTL]) {
   constexpr int MT = ProductLane<W>::MT, NTL = ProductLane<W>::NTL;
   const int k = it.x, t = it.w;
-  const double p = g.fold[3 * k], inv = g.fold[3 * k + 1], qm = g.fold[3 * k + 2];
+  // p, q and q / p of modulus k: C q = v is exact (|v| < 2^39), and v / p
+  // is read as C (q / p), within 2^-19 of v / p, which for odd p lies at
+  // least 2^-9 from a half-integer (for p = 256 both are exact), so the
+  // rounded quotient is rint(v / p) and fma(-p, t, v + kMagic) the
+  // symmetric residue, whose byte is what sym_residue_small would store
+  const double p = g.fold[4 * k], qm = g.fold[4 * k + 2], qmi = g.fold[4 * k + 3];
   i4* dst = (i4*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + L.lane;
 #pragma unroll
   for (int m = 0; m < MT; ++m) {
@@ -339,13 +366,14 @@ __device__ __forceinline__ void item_epilogue(
 #pragma unroll
       for (int n4 = 0; n4 < 4; ++n4) {
         const int n = 4 * h + n4;
-        int b4 = 0;
+        uint32_t b4 = 0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const double v = (double)acc[m][n][r] * qm;  // exact: |v| < 2^39
-          b4 |= (sym_residue_small(v, p, inv) & 0xff) << (8 * r);
+          const double a = (double)acc[m][n][r];
+          const double w = fma(-p, rint(a * qmi), fma(a, qm, kMagic));
+          b4 |= magic_byte(w) << (8 * r);
         }
-        out[n4] = b4;
+        out[n4] = (int)b4;
       }
       const int rw = L.wm + 2 * (L.wn * (NTL / 4) + h);
       dst[(rw * 8 + m) * 64] = out;
@@ -396,7 +424,7 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_kernel(ProductArgs g) {
 // (the prologue and the previous item's residue stores), so loads and
 // stores never share a partial vmcnt wait.  Every workgroup exits after its
 // last item.
-template <int W, int NSTAGE>
+template <int W, int NSTAGE, bool IL = false>
 __global__ __launch_bounds__(64 * W, 1) void crt_product_persistent_kernel(ProductArgs g) {
   __shared__ __attribute__((aligned(1024))) int8_t ring[NSTAGE][kStage];
   int cur = 0;
@@ -423,9 +451,9 @@ __global__ __launch_bounds__(64 * W, 1) void crt_product_persistent_kernel(Produ
   i4 acc[ProductLane<W>::MT][ProductLane<W>::NTL];
   while (true) {
     if (src.diag)
-      item_mainloop<W, NSTAGE, true>(src, kstride, L, ksteps, ring, true, acc);
+      item_mainloop<W, NSTAGE, true, IL>(src, kstride, L, ksteps, ring, true, acc);
     else
-      item_mainloop<W, NSTAGE, false>(src, kstride, L, ksteps, ring, true, acc);
+      item_mainloop<W, NSTAGE, false, IL>(src, kstride, L, ksteps, ring, true, acc);
     // every wave has read its last stage (no loads are outstanding): the
     // ring is free for the next item's prologue
     wait_vm_barrier<0>();
@@ -596,17 +624,22 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
     int8_t (*buf)[kPitch] = rb[k & 1];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      int b4 = 0;
+      uint32_t b4 = 0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        // with RECON |zz| <= 2^45 (a <= 45); kModuli[0] = 256 is the one
-        // even modulus
-        const int res = !RECON ? sym_residue(zz[4 * n + r], p, inv)
-                        : (k == 0 ? sym_residue_small(zz[4 * n + r], p, inv)
-                                  : sym_residue_odd(zz[4 * n + r], p, inv));
-        b4 |= (res & 0xff) << (8 * r);
+        if (RECON && k == 0) {
+          // |zz| <= 2^45 (a <= 45); p = kModuli[0] = 256, the one even
+          // modulus: the quotient is exact and its tie (+128 or -128) is
+          // the byte 0x80 either way, so no fold is needed
+          const double w = fma(-p, rint(zz[4 * n + r] * inv), zz[4 * n + r] + kMagic);
+          b4 |= magic_byte(w) << (8 * r);
+        } else if (RECON) {
+          b4 |= (uint32_t)(sym_residue_odd(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+        } else {
+          b4 |= (uint32_t)(sym_residue(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+        }
       }
-      *(int*)&buf[rl][16 * n + cl0] = b4;
+      *(int*)&buf[rl][16 * n + cl0] = (int)b4;
     }
     // the writes of modulus k - 1 from this buffer's twin are done by every
     // thread once all pass this barrier; modulus k + 1 fills the twin next
@@ -969,14 +1002,16 @@ inline int plane_ld(int ld) { return ((ld + TM - 1) / TM) * TM; }
 
 // Product kernel configuration, coded W * 100 + NSTAGE: 8 waves with a 3-,
 // 4- or 5-stage ring (96, 128, 160 KB of LDS); 904 is the persistent 8-wave
-// kernel with a 4-stage ring.  (The 4-wave layouts and the 3- and 5-stage
-// persistent forms compile with register spills and are not offered.)
+// kernel with a 4-stage ring, and 914 the same with the fragment reads of
+// each step interleaved with its first MFMAs by scheduling barriers.  (The
+// 4-wave layouts and the 3- and 5-stage persistent forms compile with
+// register spills and are not offered.)
 // PRG_CRT_CONFIG forces one; otherwise the first product of each
 // (pld, moduli) pair times every configuration (twice each, the minimum
 // counts) and keeps the fastest.  The product kernel only writes its residue
 // buffer, so the trial launches leave no trace and the results do not depend
 // on the choice.
-constexpr int kConfigs[] = {803, 804, 805, 904};
+constexpr int kConfigs[] = {803, 804, 805, 904, 914};
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 int g_forced = -1;                        // -1: not read yet; 0: autotune
 std::vector<std::array<int, 3>> g_tuned;  // (pld, np, config)
@@ -1015,6 +1050,11 @@ void launch_product(const ProductArgs& g, int nitems, int config) {
       hipLaunchKernelGGL((crt_product_persistent_kernel<8, 4>), dim3(persistent_grid(nitems)),
                          dim3(512), 0, stream(), g);
       break;
+    case 914:
+      hipLaunchKernelGGL((crt_product_persistent_kernel<8, 4, true>),
+                         dim3(persistent_grid(nitems)), dim3(512), 0, stream(), g);
+      break;
+
     case 803:
       hipLaunchKernelGGL((crt_product_kernel<8, 3>), dim3(nitems), dim3(512), 0, stream(), g);
       break;
@@ -1166,15 +1206,16 @@ const int4* item_list(int nt, int np, int* st) {
 double* fold_constants(int np, int* st) {
   static int cached = 0;
   static unsigned cached_gen = 0;
-  double* d = (double*)workspace("crt_fold", sizeof(double) * 3 * kMaxModuli, st);
+  double* d = (double*)workspace("crt_fold", sizeof(double) * 4 * kMaxModuli, st);
   const unsigned gen = workspace_generation();
   if (*st || (cached == np && cached_gen == gen)) return d;
   const CrtConst c = make_const(np, false);  // p, 1/p and q only
-  double h[3 * kMaxModuli] = {};
+  double h[4 * kMaxModuli] = {};
   for (int k = 0; k < np; ++k) {
-    h[3 * k] = c.p[k];
-    h[3 * k + 1] = c.inv[k];
-    h[3 * k + 2] = c.q[k];
+    h[4 * k] = c.p[k];
+    h[4 * k + 1] = c.inv[k];
+    h[4 * k + 2] = c.q[k];
+    h[4 * k + 3] = c.q[k] * c.inv[k];
   }
   if (hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, stream()) != hipSuccess ||
       hipStreamSynchronize(stream()) != hipSuccess) {

@@ -161,7 +161,7 @@ int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, con
 
 // Fused SP2 step: Y = X*X, then per the device state's planned branch
 // Xn = Y or 2X - Y; epilogue partials [||Xn||^2, Tr Xn, idem, Tr Y].  Below
-// 256 upper-triangle tiles the product is split along K
+// 1024 upper-triangle tiles the product is split along K
 // (sp2_split_count(ld) slices) into split_acc (sp2_split_bytes(ld) bytes; the
 // library workspace when null) and summed in slice order by a second kernel.
 // thresholded: the sparse formats' thresholding of the state's threshold

@@ -36,7 +36,7 @@ constexpr size_t kSplitBytes = sizeof(double) * 36 * 8 * 128 * 128;
 constexpr size_t kPartBytes = 128 * 1024;
 constexpr size_t kBoundsBytes = 256;
 constexpr size_t kStateBytes = 16 * 1024;
-constexpr size_t kGershBytes = 32 * 1024;
+constexpr size_t kGershBytes = 160 * 1024;
 constexpr size_t kOffX0 = 0;
 constexpr size_t kOffX1 = kOffX0 + kMatBytes;
 constexpr size_t kOffSplit = kOffX1 + kMatBytes;
@@ -44,7 +44,7 @@ constexpr size_t kOffPart = kOffSplit + kSplitBytes;
 constexpr size_t kOffBounds = kOffPart + kPartBytes;
 constexpr size_t kOffState = kOffBounds + kBoundsBytes;
 constexpr size_t kOffGersh = kOffState + kStateBytes;
-constexpr size_t kScratchBytes = kOffGersh + kGershBytes;  // 52.2 MiB
+constexpr size_t kScratchBytes = kOffGersh + kGershBytes;  // 52.3 MiB
 static_assert(sizeof(Sp2State) <= kStateBytes, "state slot too small");
 static_assert(kOffSplit % 256 == 0 && kOffPart % 256 == 0 && kOffState % 256 == 0 &&
                   kOffGersh % 256 == 0,

@@ -167,7 +167,12 @@ def test_gptest_driver_matches_oracle(golden, tmp_path):
     assert abs(float(got["trace_hp_sp2"]) - np.sum(h * rho_full)) < 1e-8
     lo, hi = O.gershgorin(h)
     eh, el, eg = O.homolumogap(vv, ic, pp, lo, hi)
-    assert abs(float(got["egap_sp2"]) - eg) < 1e-8 * max(1.0, abs(eg))
+    # The last vv (6.6e-11) is ||X - X^2||_F at the rounding level of X's
+    # entries, so it carries ~1e-4 relative rounding; the gap estimator then
+    # has two admissible answers 4.8e-8 (relative) apart, and any change of
+    # the product's summation order (the GPU's K slices against numpy) may
+    # pick either.
+    assert abs(float(got["egap_sp2"]) - eg) < 1e-7 * max(1.0, abs(eg))
     g = np.where(np.abs(rho_full) > 1e-5, rho_full, 0.0)
     parts = O.equal_partition(16, n)
     seq, imax = O.sp2sequence(lo, hi, eh, el, 1e-12)

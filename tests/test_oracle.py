@@ -1231,6 +1231,37 @@ def test_crt_symmetric_residue_without_correction():
         assert m.min() >= -128 and m.max() <= 127
 
 
+def test_crt_product_epilogue_magic_read():
+    """The int8 product epilogue's residue of v = a q (a the int32 MFMA sum,
+    |a| <= 2^31, q < p): t = rint(a fl(q fl(1/p))), then w = (a q + 1.5 2^52)
+    - p t, exact in FP64, whose low 32 bits are the residue as an int32.  Its
+    low byte equals the byte of the symmetric residue for every modulus
+    (including the p = 256 tie +-128, byte 0x80), on random sums and on the
+    residue classes next to the extremes.  A quotient taken as a plain
+    truncation (int(a q / p)) instead of rint fails it for every odd p."""
+    rng = np.random.default_rng(12)
+    magic = 6755399441055744.0
+    ps, qs, _, _ = O.crt_constants(11, square=False)
+    for p, qk in zip(ps, qs):
+        q = float(qk)
+        qmi = q * (1.0 / p)
+        top = 2.0 ** 31
+        a = np.concatenate([np.rint(rng.uniform(-top, top, 20000)),
+                            np.arange(-2 * p, 2 * p, dtype=float),
+                            top - 1 - np.arange(2 * p, dtype=float), -top + np.arange(2 * p, dtype=float)])
+        w = (a * q + magic) - p * np.rint(a * qmi)
+        lo = (w.view(np.uint64) & np.uint64(0xffffffff)).astype(np.int64)
+        got = lo & 0xff
+        exact = np.array([(((int(x) * int(q)) + p // 2) % p) - p // 2 for x in a])
+        assert np.array_equal(got, exact & 0xff), p
+        if p == 256:
+            continue  # any integer quotient leaves the same low byte for p = 256
+        # planted mistake: truncating the quotient gives a different byte
+        bad = (a * q + magic) - p * np.trunc(a * qmi)
+        badlo = (bad.view(np.uint64) & np.uint64(0xffffffff)).astype(np.int64) & 0xff
+        assert not np.array_equal(badlo, exact & 0xff), p
+
+
 def test_crt_exact_int_product_equals_int64_matmul():
     """The oracle's FP64 route to the exact residue products equals numpy's
     int64 matmul on residue-range matrices (including all +-128 entries,
