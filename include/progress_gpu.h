@@ -418,7 +418,7 @@ int prg_set_int8_moduli(int count);
 int prg_get_int8_moduli(void);
 /* Configuration the int8 CRT product kernel last ran with, coded
  * waves * 100 + ring stages (803, 804, 805: 8 waves, one item per
- * workgroup; 904: the persistent 8-wave kernel, one workgroup per CU; 914:
+ * workgroup; 904: the persistent 8-wave kernel, one workgroup per CU; 944:
  * 904 with its LDS fragment reads interleaved among the MFMAs):
  * chosen per (padded size, moduli) by timing each on the first product, or
  * forced with the environment variable PRG_CRT_CONFIG.  0 before any int8

@@ -120,15 +120,24 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
-// 16 bytes per lane from global memory straight into LDS at base + 16 * lane
-// (global_load_lds_dwordx4; base must be wave-uniform).  Written as inline
-// assembly so that the compiler does not add its conservative vmcnt(0) waits
-// before every LDS read: the waits are explicit (wait_vm_barrier).
-__device__ __forceinline__ void dma16(const int8_t* src, uint32_t lds_base) {
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+// 16 bytes per lane from global memory at gbase + voff straight into LDS at
+// lds_base + 16 * lane (global_load_lds_dwordx4 with a scalar base: gbase
+// and lds_base must be wave-uniform, voff is the lane's 32-bit offset).
+// Written as inline assembly so that the compiler does not add its
+// conservative vmcnt(0) waits before every LDS read: the waits are explicit
+// (wait_vm_barrier).
+__device__ __forceinline__ void dma16(uint64_t gbase, uint32_t voff, uint32_t lds_base) {
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
                :
-               : "v"(src), "s"(lds_base)
+               : "v"(voff), "s"(gbase), "s"(lds_base)
                : "memory", "m0");
+}
+
+// A 64-bit value that is uniform across the wave, moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // Waits until at most N vector-memory operations of this wave are pending and
@@ -199,7 +208,7 @@ struct ProductLane {
   static constexpr int NTL = TM / WN / 16;  // 16-column MFMA blocks per wave
   static constexpr int CPW = 32 / W;        // 1 KB DMA chunks per wave per stage
   int lane, w, wm, wn, drow, kc, fo, offa;
-  uint32_t dst0;
+  uint32_t dst0, voff;
   __device__ ProductLane(const int8_t* ring0) {
     const int tid = threadIdx.x;
     lane = tid & 63;
@@ -213,6 +222,7 @@ struct ProductLane {
     // leave the swizzle unchanged.
     drow = 16 * w + (lane >> 2);
     kc = 16 * ((lane & 3) ^ ((drow >> 1) & 3));
+    voff = (uint32_t)(drow * BK + kc);
     dst0 = lds_addr(ring0) + 1024 * w;
     // LDS read offsets of this lane's fragments: row r = lane % 16 of each
     // 16-row block, 16-byte chunk lane / 16; the block offsets are multiples
@@ -222,22 +232,23 @@ struct ProductLane {
   }
 };
 
-// Operand panels of one (tile, modulus) item for this lane's DMA chunks.
+// Operand panels of one (tile, modulus) item: the wave-uniform addresses of
+// their first rows (each lane adds its ProductLane::voff).
 struct ItemSrc {
-  const int8_t* a;
-  const int8_t* b;
+  uint64_t a;
+  uint64_t b;
   bool diag;  // I == J and B = A: one panel, loaded once
 };
 
 template <int W>
 __device__ __forceinline__ ItemSrc item_src(const ProductArgs& g, int cur, const int4& it,
-                                            const ProductLane<W>& L) {
+                                            const ProductLane<W>&) {
   const size_t plane = (size_t)g.pld * g.pld;
   const int8_t* R = g.planes[cur] + (size_t)it.x * plane;
   const int8_t* RB = g.planes_b ? g.planes_b + (size_t)it.x * plane : R;
   ItemSrc s;
-  s.a = R + (size_t)(it.y * TM + L.drow) * BK + L.kc;
-  s.b = RB + (size_t)(it.z * TM + L.drow) * BK + L.kc;
+  s.a = uniform64((uint64_t)(uintptr_t)(R + (size_t)it.y * TM * BK));
+  s.b = uniform64((uint64_t)(uintptr_t)(RB + (size_t)it.z * TM * BK));
   s.diag = (it.y == it.z) && !g.planes_b;
   return s;
 }
@@ -254,7 +265,7 @@ __device__ __forceinline__ void issue_stage(const ItemSrc& src, size_t kstride,
   for (int q = 0; q < NQ; ++q) {
     const int panel = (W * q) >> 4;
     const int rows = 16 * W * q - TM * panel;
-    dma16((panel ? src.b : src.a) + off + rows * BK, base + 1024 * W * q);
+    dma16((panel ? src.b : src.a) + off + (size_t)rows * BK, L.voff, base + 1024 * W * q);
   }
 }
 
@@ -297,7 +308,25 @@ __device__ __forceinline__ void item_mainloop(
 #pragma unroll
     for (int n = 0; n < NTL; ++n) f[MT + n] = *(const i4*)(st + offb + 1024 * n);
   };
+  // IL: the interleaved issue order.  PRE MFMAs of the step go before its
+  // barrier: they read only fragments already in registers, so a wave that
+  // reaches the barrier early leaves work queued in its SIMD's MFMA pipe
+  // while it waits.  After the barrier, each of the MT + NTL fragment reads
+  // of stage s + 1 is followed by two MFMAs of stage s, so that the reads
+  // overlap the MFMAs and have completed by the next barrier.
+  constexpr int PRE = IL ? 8 : 0;
+  static_assert(PRE + 2 * (MT + NTL) <= MT * NTL, "schedule groups");
   auto step = [&](int s, i4 (&use)[MT + NTL], i4 (&next)[MT + NTL], bool first) {
+    const i4 zero = {0, 0, 0, 0};
+    if constexpr (PRE > 0) {
+#pragma unroll
+      for (int i = 0; i < PRE; ++i) {
+        const int m = i / NTL, n = i % NTL;
+        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
+                                                          first ? zero : acc[m][n], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if (s + NSTAGE - 1 < ksteps)
       wait_vm_barrier<NQ * (NSTAGE - 2)>();
     else
@@ -305,23 +334,19 @@ __device__ __forceinline__ void item_mainloop(
     if (s + NSTAGE < ksteps) issue_stage<W, NSTAGE, DIAG>(src, kstride, L, s + NSTAGE);
     // after the last stage this reads a stale slot that is never used
     read_frags(s + 1, next);
-    const i4 zero = {0, 0, 0, 0};
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int n = 0; n < NTL; ++n)
-        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
-                                                          first ? zero : acc[m][n], 0, 0, 0);
-    if (IL) {
-      // issue order of the step: one fragment read of stage s + 1, then two
-      // MFMAs of stage s, for all MT + NTL reads; the remaining MFMAs follow
-      // and cover the reads' latency before the next barrier waits for them
+    for (int i = PRE; i < MT * NTL; ++i) {
+      const int m = i / NTL, n = i % NTL;
+      acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(use[MT + n], use[m],
+                                                        first ? zero : acc[m][n], 0, 0, 0);
+    }
+    if constexpr (IL) {
 #pragma unroll
       for (int i = 0; i < MT + NTL; ++i) {
         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
         __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
       }
-      __builtin_amdgcn_sched_group_barrier(0x008, MT * NTL - 2 * (MT + NTL), 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, MT * NTL - PRE - 2 * (MT + NTL), 0);
     }
   };
   if (all_landed || NSTAGE - 1 >= ksteps)
@@ -1002,16 +1027,18 @@ inline int plane_ld(int ld) { return ((ld + TM - 1) / TM) * TM; }
 
 // Product kernel configuration, coded W * 100 + NSTAGE: 8 waves with a 3-,
 // 4- or 5-stage ring (96, 128, 160 KB of LDS); 904 is the persistent 8-wave
-// kernel with a 4-stage ring, and 914 the same with the fragment reads of
-// each step interleaved with its first MFMAs by scheduling barriers.  (The
-// 4-wave layouts and the 3- and 5-stage persistent forms compile with
-// register spills and are not offered.)
+// kernel with a 4-stage ring, and 944 the same with the interleaved issue
+// order of item_mainloop (IL).  (Other read / MFMA patterns measured within
+// 1.5 % of 944 or slower, or spilled registers; the interleaved form with a
+// 5-stage ring, 160 KB, was 6 % slower than 944 at N = 8192.)  (The
+// 4-wave layouts, the 3-stage persistent form and the 5-stage one without
+// the interleaved order compile with register spills and are not offered.)
 // PRG_CRT_CONFIG forces one; otherwise the first product of each
 // (pld, moduli) pair times every configuration (twice each, the minimum
 // counts) and keeps the fastest.  The product kernel only writes its residue
 // buffer, so the trial launches leave no trace and the results do not depend
 // on the choice.
-constexpr int kConfigs[] = {803, 804, 805, 904, 914};
+constexpr int kConfigs[] = {803, 804, 805, 904, 944};
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 int g_forced = -1;                        // -1: not read yet; 0: autotune
 std::vector<std::array<int, 3>> g_tuned;  // (pld, np, config)
@@ -1050,11 +1077,10 @@ void launch_product(const ProductArgs& g, int nitems, int config) {
       hipLaunchKernelGGL((crt_product_persistent_kernel<8, 4>), dim3(persistent_grid(nitems)),
                          dim3(512), 0, stream(), g);
       break;
-    case 914:
+    case 944:
       hipLaunchKernelGGL((crt_product_persistent_kernel<8, 4, true>),
                          dim3(persistent_grid(nitems)), dim3(512), 0, stream(), g);
       break;
-
     case 803:
       hipLaunchKernelGGL((crt_product_kernel<8, 3>), dim3(nitems), dim3(512), 0, stream(), g);
       break;

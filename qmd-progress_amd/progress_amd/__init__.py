@@ -262,7 +262,7 @@ def get_int8_moduli() -> int:
 
 
 def set_int8_config(config: int):
-    """Force the int8 CRT product kernel configuration (803, 804, 805, 904, 914;
+    """Force the int8 CRT product kernel configuration (803, 804, 805, 904, 944;
     0 = time them per size)."""
     st = lib().prg_gpu_set_int8_config(config)
     if st != 0:

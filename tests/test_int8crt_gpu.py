@@ -313,7 +313,7 @@ def test_crt_after_shutdown_and_init(P):
     assert np.linalg.norm(after - rho_o) < 1e-7
 
 
-@pytest.mark.parametrize("config", [803, 804, 805, 904, 914])
+@pytest.mark.parametrize("config", [803, 804, 805, 904, 944])
 @pytest.mark.parametrize("n", [1000, 2000])
 def test_crt_product_configs_match_oracle(P, config, n):
     """Every int8 CRT product kernel configuration (one item per workgroup
