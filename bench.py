@@ -253,7 +253,10 @@ def main():
     kern = hr.get("product_kernel_mfma_tops")
     roofline = {
         "bound": "mfma",
-        "kernel": {"fp64": "sp2_step_kernel (FP64 MFMA, fused SP2 update)",
+        "kernel": {"fp64": ("sp2_step_kernel (FP64 MFMA, fused SP2 update)"
+                            if P.sp2_split_slices(n) == 1 else
+                            f"sp2_split_product_kernel + sp2_split_epilogue_kernel (FP64 MFMA, "
+                            f"K in {P.sp2_split_slices(n)} slices; the pair's time)"),
                    "f16split": "sp2_f16split_kernel (FP16 MFMA, 3 products)",
                    "bf16split": "sp2_f16split_kernel<bf16> (BF16 MFMA, 3 products)",
                    "int8crt": f"{crt_kernel_name(hr.get('int8_config'))} (int8 MFMA, {hmod} "

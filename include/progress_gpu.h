@@ -424,6 +424,11 @@ int prg_get_int8_moduli(void);
  * forced with the environment variable PRG_CRT_CONFIG.  0 before any int8
  * product.  Measurement support; no reference counterpart. */
 int prg_gpu_int8_waves(void);
+/* K slices of the FP64 SP2 step at size N: 1 for the fused one-kernel step
+ * (sp2_step_kernel), s > 1 for the split product and its epilogue kernel
+ * (fewer than 1024 upper-triangle 128 x 128 tiles).  Measurement support;
+ * no reference counterpart. */
+int prg_gpu_sp2_split_slices(int N);
 /* Forces the int8 CRT product kernel configuration (one of the codes above;
  * 0 returns to timing them per size).  Returns a status (1 for an unknown
  * code).  Test and measurement support; no reference counterpart. */

@@ -497,6 +497,7 @@ int prg_set_int8_moduli(int count) {
 }
 int prg_get_int8_moduli(void) { return crt_moduli(); }
 int prg_gpu_int8_waves(void) { return crt_last_waves(); }
+int prg_gpu_sp2_split_slices(int N) { return N > 0 ? sp2_split_count(padded(N)) : 0; }
 int prg_gpu_set_int8_config(int config) {
   clear_error();
   return set_crt_config(config);

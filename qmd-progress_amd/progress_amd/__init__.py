@@ -96,6 +96,7 @@ def lib():
         "prg_sp2sequence": (None, [ip, ip, d, d, d, d, d, i]),
         "prg_gpu_sp2_alg2_device": (i, [i, vp, vp, d, i, i, cp, d, i, ip, ip]),
         "prg_gpu_padded_ld": (i, [i]),
+        "prg_gpu_sp2_split_slices": (i, [i]),
         "prg_gpu_timing": (None, [i]),
         "prg_equalPartition": (None, [vp, i, i]),
         "prg_filePartition": (None, [vp, cp]),
@@ -267,6 +268,11 @@ def set_int8_config(config: int):
     st = lib().prg_gpu_set_int8_config(config)
     if st != 0:
         raise ProgressError(st, lib().prg_last_error().decode())
+
+
+def sp2_split_slices(n: int) -> int:
+    """K slices of the FP64 SP2 step at size n (1: the fused one-kernel step)."""
+    return int(lib().prg_gpu_sp2_split_slices(n))
 
 
 def int8_waves() -> int:
