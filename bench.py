@@ -625,7 +625,7 @@ def fermi_bench(args, P, torch, np):
         metric = "implicit Fermi solves/s (prg_implicit_fermi, FP64 MFMA)"
         dtype = "f64"
     else:
-        roof = {"bound": "mfma", "kernel": "crt_product_kernel (int8 MFMA)",
+        roof = {"bound": "mfma", "kernel": "int8 CRT product kernels (crt_product_kernel / crt_product_persistent_kernel, configuration timed per shape; int8 MFMA)",
                 "achieved": i8 / dt / 1e12, "peak": PEAK["int8_tops"], "unit": "TOP/s",
                 "frac": i8 / dt / 1e12 / PEAK["int8_tops"], "traffic": None,
                 "note": "int8 MFMA operations executed by the solve's CRT products / "

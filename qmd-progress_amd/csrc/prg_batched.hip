@@ -346,6 +346,173 @@ __device__ __forceinline__ void split_tile(const BatchArgs& g, int t, int4 tl, T
   block_store_partials<1>(s, g.part_sums + t);
 }
 
+// Split-precision tile of 128-wide panels: TM x TN (128 x 128, 128 x 64 or
+// 64 x 64) with four waves of (TM/2) x (TN/2), so that each pair of fragment
+// reads feeds up to NJ x 3 MFMAs (twice the reuse of the 64 x 64 tile's
+// 32 x 32 waves).  32-deep K slabs of the four operand planes through LDS
+// (A hi, A lo: TM rows; B hi, B lo: TN rows), loaded by all threads, eight
+// halves per 16-byte chunk, with a register prefetch of the next slab.  An
+// LDS row is 64 bytes, four chunks; chunk c of row r sits in slot
+// c ^ ((r >> 1) & 3), which puts the 16-byte reads of 8 consecutive rows
+// (and the stores of 2 rows) on distinct banks.  Products, update and sums
+// as split_tile.
+constexpr int WBK = 32;          // K slab of the wide split tiles (halves)
+constexpr int kSplitWideLds = 512 * WBK * 2;  // bytes: 2 x 128 + 2 x 128 rows
+
+__device__ __forceinline__ int wide_slot(int row, int chunk) {
+  return chunk ^ ((row >> 1) & 3);
+}
+
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void split_tile_wide(const BatchArgs& g, int t, int4 tl, T* smem) {
+  typedef typename SplitT<T>::v8 h8;
+  constexpr double kScale = SplitT<T>::kScale;
+  constexpr int MI = TM / 32, NJ = TN / 32;      // 16 x 16 blocks per wave
+  constexpr int ROWS = 2 * TM + 2 * TN;          // LDS rows: A hi, A lo, B hi, B lo
+  constexpr int NQ = ROWS * (WBK / 8) / NT;      // 16-byte chunks per thread
+  static_assert(NQ * NT == ROWS * (WBK / 8) && (NT / (WBK / 8)) == 64, "wide split loader");
+  T (*Sp)[WBK] = reinterpret_cast<T (*)[WBK]>(smem);
+  const PartDesc pd = g.parts[tl.x];
+  const int r0 = tl.y, c0 = tl.z;
+  const int m = pd.m;
+  const double* X = g.x + pd.offset;
+  double* Xn = g.xn + pd.offset;
+  const T* hi = (const T*)g.hi + pd.offset;
+  const T* lo = (const T*)g.lo + pd.offset;
+  T* hin = (T*)g.hin + pd.offset;
+  T* lon = (T*)g.lon + pd.offset;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+  const bool diag = (r0 == c0);
+  const bool idle = diag && wm == 1 && wn == 0;
+  // loader: chunk q of thread tid is chunk tid % 4 (halves 8 (tid % 4) ..)
+  // of LDS row 64 q + tid / 4; each group of 64 rows lies in one plane, and
+  // 64 q leaves the slot swizzle unchanged
+  const int lr = tid >> 2, lc = 8 * (tid & 3);
+  const int ls = 8 * wide_slot(lr, tid & 3);
+  i4 pre[NQ];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int row = 64 * q + lr;  // compile-time plane per q
+      const T* src;
+      int prow;
+      if (64 * q < TM) { src = hi; prow = r0 + row; }
+      else if (64 * q < 2 * TM) { src = lo; prow = r0 + row - TM; }
+      else if (64 * q < 2 * TM + TN) { src = hi; prow = c0 + row - 2 * TM; }
+      else { src = lo; prow = c0 + row - 2 * TM - TN; }
+      pre[q] = *(const i4*)(src + (size_t)prow * m + k0 + lc);
+    }
+  };
+  f4 acc_hh[MI][NJ], acc_x[MI][NJ];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      acc_hh[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      acc_x[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  load(0);
+  for (int k0 = 0; k0 < m; k0 += WBK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) *(i4*)(&Sp[64 * q + lr][ls]) = pre[q];
+    __syncthreads();
+    if (k0 + WBK < m) load(k0 + WBK);
+    if (!idle) {
+      // this lane's K chunk lane / 16 of rows whose offsets inside their
+      // plane are multiples of 16 plus lane % 16: one slot for all of them
+      const int ks = 8 * wide_slot(lane & 15, lane >> 4);
+      h8 bh[NJ], bl[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int rb = wn * (TN / 2) + j * 16 + (lane & 15);
+        bh[j] = *(const h8*)(&Sp[2 * TM + rb][ks]);
+        bl[j] = *(const h8*)(&Sp[2 * TM + TN + rb][ks]);
+      }
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int ra = wm * (TM / 2) + i * 16 + (lane & 15);
+        const h8 ah = *(const h8*)(&Sp[ra][ks]);
+        const h8 al = *(const h8*)(&Sp[TM + ra][ks]);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          acc_hh[i][j] = SplitT<T>::mfma(bh[j], ah, acc_hh[i][j]);
+          acc_x[i][j] = SplitT<T>::mfma(bl[j], ah, acc_x[i][j]);
+          acc_x[i][j] = SplitT<T>::mfma(bh[j], al, acc_x[i][j]);
+        }
+      }
+    }
+  }
+  // epilogue as split_tile: this lane holds (X^2)[row][col] with row = lane
+  // % 16, col = 4 (lane / 16) + r inside each 16 x 16 block; in the diagonal
+  // quadrants of a diagonal tile only row <= col is formed and mirrored
+  const double ca = g.sigma ? 1.0 : -1.0, cb = g.sigma ? 0.0 : 2.0;
+  const int core = pd.core;
+  double s[1] = {0.0};
+  if (!idle) {
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int row = r0 + wm * (TM / 2) + i * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = c0 + wn * (TN / 2) + j * 16 + 4 * (lane >> 4) + r;
+          if (row > col) continue;
+          const bool mir = (row != col);
+          const size_t idx = row + (size_t)col * m;
+          const size_t tdx = mir ? col + (size_t)row * m : idx;
+          const double y = (double)acc_hh[i][j][r] + (1.0 / kScale) * (double)acc_x[i][j][r];
+          const double x = X[idx];
+          const double z = fma(cb, x, ca * y);
+          const double d = x - y;
+          const double wi = ((row < core) ? 1.0 : 0.0) + ((mir && col < core) ? 1.0 : 0.0);
+          s[0] += wi * d * d;
+          const T h = (T)z;
+          const T l = (T)((z - (double)h) * kScale);
+          Xn[idx] = z;
+          Xn[tdx] = z;
+          hin[idx] = h;
+          hin[tdx] = h;
+          lon[idx] = l;
+          lon[tdx] = l;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  block_store_partials<1>(s, g.part_sums + t);
+}
+
+// Persistent work-queue launch of the wide split tiles (shapes as in
+// batched_sp2_kernel<true, ...>: 3 = 128 x 128, 1 = 128 x 64, 0 = 64 x 64).
+template <typename T>
+__global__ __launch_bounds__(NT, 2) void batched_split_wide_kernel(BatchArgs g) {
+  __shared__ __attribute__((aligned(16))) T smem[kSplitWideLds / sizeof(T)];
+  __shared__ int s_ticket;
+  const int home = blockIdx.x & 7;
+  for (int q = 0; q < 8; ++q) {
+    const int qi = (home + q) & 7;
+    const int first = g.queue_off[qi], count = g.queue_off[qi + 1] - first;
+    for (;;) {
+      __syncthreads();  // every thread has read the previous ticket
+      if (threadIdx.x == 0) s_ticket = atomicAdd(&g.tickets[qi], 1);
+      __syncthreads();
+      const int k = s_ticket;
+      if (k >= count) break;
+      const int t = first + k;
+      const int4 tl = g.tiles[t];
+      if (tl.w == 3)
+        split_tile_wide<T, 128, 128>(g, t, tl, smem);
+      else if (tl.w == 1)
+        split_tile_wide<T, 128, 64>(g, t, tl, smem);
+      else
+        split_tile_wide<T, 64, 64>(g, t, tl, smem);
+    }
+  }
+}
+
 // Persistent work-queue launch of split tiles (the queue walk of
 // batched_sp2_kernel over the 64-wide tile list).
 template <typename T>
@@ -383,12 +550,16 @@ __global__ void split_planes_kernel(const double* __restrict__ x, T* __restrict_
 typedef void (*BatchKernel)(BatchArgs);
 
 // The compiled configurations: (panel, slabs of prefetch, slab depth); the
-// split-precision kernels have one (64, 1, 64).
+// split-precision kernels have two, (64, 1, 64) and (128, 1, 32).
 BatchKernel batch_kernel(const BatchPlan& p) {
-  if (p.split)
-    return (!p.wide && p.depth == 1 && p.kslab == SBK)
-               ? (p.split == 2 ? batched_split_kernel<__bf16> : batched_split_kernel<_Float16>)
-               : nullptr;
+  if (p.split) {
+    if (!p.wide && p.depth == 1 && p.kslab == SBK)
+      return p.split == 2 ? batched_split_kernel<__bf16> : batched_split_kernel<_Float16>;
+    if (p.wide && p.depth == 1 && p.kslab == WBK)
+      return p.split == 2 ? batched_split_wide_kernel<__bf16>
+                          : batched_split_wide_kernel<_Float16>;
+    return nullptr;
+  }
   if (p.wide) return (p.depth == 1 && p.kslab == 16) ? batched_sp2_kernel<true, 1, 16> : nullptr;
   if (p.kslab == 32) return p.depth == 1 ? batched_sp2_kernel<false, 1, 32> : nullptr;
   if (p.kslab == 16)

@@ -199,6 +199,9 @@ struct BatchConfig {
   int panel = 0, depth = 0, kslab = 0;
 };
 constexpr BatchConfig kCandidates[] = {{64, 1, 16}, {64, 2, 16}, {64, 1, 32}, {128, 1, 16}};
+// split-precision products: 64 x 64 tiles (64-deep slabs) or 128-wide panels
+// (32-deep slabs)
+constexpr BatchConfig kSplitCandidates[] = {{64, 1, 64}, {128, 1, 32}};
 
 std::map<std::pair<uint64_t, int>, BatchConfig>& config_cache() {
   static std::map<std::pair<uint64_t, int>, BatchConfig> c;
@@ -358,25 +361,65 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
   double* d_vv = (double*)workspace("gp_vv", sizeof(double) * 100, &st);
   if (st) return st;
   // split-precision products (prg_set_precision FP16 / BF16 split): the
-  // 64-wide split kernel, X also carried as hi / lo planes; int8 CRT has no
+  // split kernels, X also carried as hi / lo planes; int8 CRT has no
   // batched kernel, so its parts run with FP64 products
   const int pm = precision_mode();
   plan.split = pm == PRG_PREC_F16SPLIT ? 1 : (pm == PRG_PREC_BF16SPLIT ? 2 : 0);
+  BatchConfig force;
+  if (const char* e = std::getenv("PRG_BATCH_TILE")) force.panel = std::atoi(e);
   if (plan.split) {
+    // two split kernels: 64 x 64 tiles with 64-deep slabs, or tiles of
+    // 128-wide panels with 32-deep slabs; timed per batch like the FP64
+    // candidates (PRG_BATCH_TILE restricts the panel width)
     char* pl = (char*)workspace("gp_planes", 4 * sizeof(uint16_t) * off, &st);
     if (st) return st;
     for (int k = 0; k < 4; ++k) plan.planes[k] = pl + k * sizeof(uint16_t) * off;
-    if ((st = upload_tiles(tiles64, 64, &plan))) return st;
-    plan.depth = 1;
-    plan.kslab = 64;
+    std::vector<BatchConfig> scand;
+    for (const BatchConfig& c : kSplitCandidates)
+      if (!force.panel || c.panel == force.panel) scand.push_back(c);
+    if (scand.empty()) {
+      set_error(ERR_ARG, "PRG_BATCH_TILE: no compiled split-precision batched SP2 kernel has "
+                         "this panel width");
+      return ERR_ARG;
+    }
+    const auto skey = std::make_pair(size_signature(hp), -(plan.split * 256 + force.panel));
+    BatchConfig cfg = scand[0];
+    auto hit = config_cache().find(skey);
+    if (hit != config_cache().end()) {
+      cfg = hit->second;
+    } else if (scand.size() > 1) {
+      if ((st = batched_gather(H, plan, x0, gp->mineval, gp->maxeval)) ||
+          (st = batched_split_prepare(plan, x0)))
+        return st;
+      const TileQueues tiles128 = make_tiles(hp, 128);
+      float best = 0.f;
+      for (size_t k = 0; k < scand.size(); ++k) {
+        const BatchConfig& c = scand[k];
+        if ((st = upload_tiles(c.panel == 128 ? tiles128 : tiles64, c.panel, &plan))) return st;
+        plan.depth = c.depth;
+        plan.kslab = c.kslab;
+        float ms = 0.f;
+        if ((st = batched_slots(&plan)) ||
+            (st = time_step(plan, x0, x1, d_part + (size_t)iters * maxtiles,
+                            d_tickets + 8 * iters, &ms)))
+          return st;
+        if (k == 0 || ms < best) {
+          best = ms;
+          cfg = c;
+        }
+      }
+    }
+    config_cache()[skey] = cfg;
+    if ((st = upload_tiles(cfg.panel == 128 ? make_tiles(hp, 128) : tiles64, cfg.panel, &plan)))
+      return st;
+    plan.depth = cfg.depth;
+    plan.kslab = cfg.kslab;
     if ((st = batched_slots(&plan))) return st;
-    g_cfg[0] = 64;
-    g_cfg[2] = 1;
-    g_cfg[3] = 64;
+    g_cfg[0] = cfg.panel;
+    g_cfg[2] = cfg.depth;
+    g_cfg[3] = cfg.kslab;
     return run_batch(H, rho, gp, plan, x0, x1, d_part, d_tickets, d_vv);
   }
-  BatchConfig force;
-  if (const char* e = std::getenv("PRG_BATCH_TILE")) force.panel = std::atoi(e);
   if (const char* e = std::getenv("PRG_BATCH_DEPTH")) force.depth = std::atoi(e);
   if (const char* e = std::getenv("PRG_BATCH_KSLAB")) force.kslab = std::atoi(e);
   std::vector<BatchConfig> cand;
