@@ -183,12 +183,13 @@ struct Sp2Args {
 // when the product is only measured (X unchanged); the tile's partial sums
 // [||Xn||^2, Tr Xn, ||X - X^2||^2 (core rows for submatrix), Tr Y] go to
 // part[4 t].
-// MN of the tile's 16-row MFMA groups, starting at group m0 (MN = 4: the
-// whole tile); the partial sums go to part[4 slot].
-template <bool THR, int MN>
+// MN of the tile's 16-row MFMA groups, starting at group m0, and NN of its
+// column groups, starting at n0 (MN = NN = 4: the whole tile); the partial
+// sums go to part[4 slot].
+template <bool THR, int MN, int NN = 4>
 __device__ __forceinline__ void sp2_epilogue_impl(const Sp2Args& g, const Sp2State* st, int slot,
-                                                  int I, int J, const d4 (&acc)[MN][4], int m0,
-                                                  int apply, int sigma, int cur) {
+                                                  int I, int J, const d4 (&acc)[MN][NN], int m0,
+                                                  int apply, int sigma, int cur, int n0 = 0) {
   const int core = st->core;
   const int variant = st->variant;
   const double* X = g.buf[cur];
@@ -208,13 +209,13 @@ __device__ __forceinline__ void sp2_epilogue_impl(const Sp2Args& g, const Sp2Sta
 #pragma unroll
   for (int m = 0; m < MN; ++m) {
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
+    for (int n = 0; n < NN; ++n) {
       const int row = PRG_ROW(I, m0 + m);
       const double wrow = (row < core_lim) ? 1.0 : 0.0;
       double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int col = PRG_COL(J, n, r);
+        const int col = PRG_COL(J, n0 + n, r);
         const size_t idx = row + (size_t)col * ld;
         double y = acc[m][n][r];
         const double x = X[idx];
@@ -315,12 +316,14 @@ __global__ __launch_bounds__(NT, 2) void sp2_split_product_kernel(Sp2Args g) {
 }
 
 // Sums the split slices of one 16-row group of every wave's quadrant of a
-// tile in slice order (deterministic) and applies the SP2 epilogue to it:
-// workgroup b handles tile b / 4, group b % 4, so four workgroups share a
-// tile's partial reads.  THR: with the sparse formats' thresholding (the
-// fused kernel above serves dense storage only, so that its epilogue stays
-// straight-line code).  Partial sums per workgroup, part[4 b].
-template <bool THR>
+// tile (all four of its 16-column groups for Q = 4, two for Q = 8) in slice
+// order (deterministic) and applies the SP2 epilogue to it: workgroup b
+// handles tile b / Q, part b % Q, so Q workgroups share a tile's partial
+// reads (Q = 8 when four per tile would leave CUs idle).  THR: with the
+// sparse formats' thresholding (the fused kernel above serves dense storage
+// only, so that its epilogue stays straight-line code).  Partial sums per
+// workgroup, part[4 b].
+template <bool THR, int Q>
 __global__ __launch_bounds__(NT, 2) void sp2_split_epilogue_kernel(Sp2Args g) {
   __shared__ int s_flags[4];
   if (threadIdx.x == 0) {
@@ -331,20 +334,23 @@ __global__ __launch_bounds__(NT, 2) void sp2_split_epilogue_kernel(Sp2Args g) {
   }
   __syncthreads();
   if (!s_flags[0]) return;
-  const int t = blockIdx.x >> 2, q = blockIdx.x & 3;
+  static_assert(Q == 4 || Q == 8, "four or eight workgroups per tile");
+  constexpr int NN = 16 / Q;  // 16-column groups per workgroup
+  const int t = blockIdx.x / Q, q = blockIdx.x % Q;
+  const int mg = q / (Q / 4), n0 = (q % (Q / 4)) * NN;
   int I, J;
   sym_tile(t, I, J);
-  d4 acc[1][4];
+  d4 acc[1][NN];
   const d4* src = (const d4*)(g.acc + (size_t)t * g.split * BM * BN) + threadIdx.x;
 #pragma unroll
-  for (int n = 0; n < 4; ++n) acc[0][n] = src[(4 * q + n) * NT];
+  for (int n = 0; n < NN; ++n) acc[0][n] = src[(4 * mg + n0 + n) * NT];
   for (int sl = 1; sl < g.split; ++sl) {
     const d4* sp = src + (size_t)sl * (BM * BN / 4);
 #pragma unroll
-    for (int n = 0; n < 4; ++n) acc[0][n] += sp[(4 * q + n) * NT];
+    for (int n = 0; n < NN; ++n) acc[0][n] += sp[(4 * mg + n0 + n) * NT];
   }
-  sp2_epilogue_impl<THR, 1>(g, g.st, blockIdx.x, I, J, acc, q, s_flags[1], s_flags[2],
-                            s_flags[3]);
+  sp2_epilogue_impl<THR, 1, NN>(g, g.st, blockIdx.x, I, J, acc, mg, s_flags[1], s_flags[2],
+                                s_flags[3], n0);
 }
 
 // Response step: Y = X D + D X (two products accumulated over a 2N-deep K),
@@ -482,6 +488,11 @@ int sp2_split_count(int ld) {
   return best;
 }
 
+int sp2_epilogue_parts(int ld) {
+  const int nt = ld / BM, nb = nt * (nt + 1) / 2;
+  return (4 * nb < 256) ? 8 : 4;
+}
+
 size_t sp2_split_bytes(int ld) {
   const int split = sp2_split_count(ld);
   if (split == 1) return 0;
@@ -510,12 +521,15 @@ int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, 
     g.acc = split_acc ? split_acc : (double*)workspace("sp2_split_acc", bytes, &st);
     if (st) return st;
     hipLaunchKernelGGL(sp2_split_product_kernel, dim3(nb * g.split), dim3(NT), 0, stream(), g);
-    if (thresholded)
-      hipLaunchKernelGGL(sp2_split_epilogue_kernel<true>, dim3(4 * nb), dim3(NT), 0, stream(), g);
-    else
-      hipLaunchKernelGGL(sp2_split_epilogue_kernel<false>, dim3(4 * nb), dim3(NT), 0, stream(),
-                         g);
-    if (ntiles) *ntiles = 4 * nb;  // partial-sum slots: four per tile
+    // eight epilogue workgroups per tile while four per tile would leave
+    // half the 256 CUs idle
+    const int q = sp2_epilogue_parts(g.ld);
+    void (*epi)(Sp2Args) =
+        thresholded ? (q == 8 ? sp2_split_epilogue_kernel<true, 8> : sp2_split_epilogue_kernel<true, 4>)
+                    : (q == 8 ? sp2_split_epilogue_kernel<false, 8>
+                              : sp2_split_epilogue_kernel<false, 4>);
+    hipLaunchKernelGGL(epi, dim3(q * nb), dim3(NT), 0, stream(), g);
+    if (ntiles) *ntiles = q * nb;  // partial-sum slots: q per tile
   } else {
     hipLaunchKernelGGL(sp2_step_kernel, dim3(nb), dim3(NT), 0, stream(), g);
   }

@@ -170,6 +170,9 @@ int gemm(int npair, const DMat* A, const DMat* B, double alpha, double beta, con
 int sp2_step(const DMat& x0, const DMat& x1, Sp2State* d_state, double* d_part, int* ntiles,
              double* split_acc = nullptr, bool thresholded = false);
 int sp2_split_count(int ld);
+// Workgroups per tile of the split product's epilogue (4, or 8 at small ld);
+// each writes one record of partial sums.
+int sp2_epilogue_parts(int ld);
 size_t sp2_split_bytes(int ld);
 // Fused response step: Y = X D + D X with the same branch: Dn = Y or 2D - Y.
 int sp2_response_step(const DMat& x0, const DMat& x1, const DMat& d0, const DMat& d1,

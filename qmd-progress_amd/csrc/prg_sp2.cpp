@@ -151,8 +151,8 @@ int run_sp2(const DMat& H, const Sp2Params& p_in, DMat out, Sp2Result* res, cons
     x1 = (double*)workspace("sp2_x1", mat_bytes, &st);
     // partial sums: one slot per product tile (the int8 CRT update writes one
     // per 64 x 64 block) plus the normalisation's
-    // (the K-split epilogue writes four per tile)
-    const int nslots = std::max(4 * ntiles, crt_update_blocks(ld));
+    // (the K-split epilogue writes four or eight per tile)
+    const int nslots = std::max(8 * ntiles, crt_update_blocks(ld));
     part = (double*)workspace("sp2_part", sizeof(double) * 4 * (size_t)(nslots + 2048), &st);
     bounds = (double*)workspace("sp2_bounds", sizeof(double) * 4, &st);
     dst = (Sp2State*)workspace("sp2_state", sizeof(Sp2State), &st);
