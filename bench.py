@@ -113,7 +113,7 @@ def errors_vs_diag(h, rho, evals, evecs, n):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "one GPU per run: multi-GPU runs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np

@@ -16,7 +16,7 @@
  *   - out-parameters the reference loses through `value` (mu, occiter,
  *     totns) are pointers.
  *
- * The matrix handle: BML (lanl/bml) is not available to this build, so the
+ * The matrix handle: BML (lanl/bml) is not linked, so the
  * library carries its own bml_matrix_t holding a dense double-precision
  * column-major matrix with a lazily synchronised HBM mirror.  The handful of
  * bml_* functions below have BML's names and C signatures (bml.h) for the

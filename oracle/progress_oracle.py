@@ -1,7 +1,7 @@
 """CPU oracle for the PROGRESS density-matrix solvers (TEST INFRASTRUCTURE ONLY).
 
 This module is a plain numpy FP64 restatement of the reference algorithms.  It is
-the checker for the HIP library, never part of the product: only ``tests/``,
+the oracle the HIP library is tested against, never part of the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import it.  Every function cites the reference routine it restates
 (paths relative to the lanl/qmd-progress tree).

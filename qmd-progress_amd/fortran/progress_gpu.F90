@@ -5,7 +5,7 @@
 ! src/prg_normalize_mod.F90, src/prg_homolumo_mod.F90,
 ! src/prg_densitymatrix_mod.F90, src/prg_implicit_fermi_mod.F90 and
 ! src/prg_response_mod.F90.  The `bml` module here is the minimal dense
-! subset those callers use (BML itself is not available to this build); its
+! subset those callers use (BML itself is not linked); its
 ! handle wraps the library's C matrix handle.
 
 module bml

@@ -66,7 +66,7 @@ def test_m0_is_only_touched_by_the_lds_dma_helper(tmp_path):
 
 
 def test_m0_checker_rejects_a_foreign_m0_use():
-    """The checker itself: a body with one more m0 access (an indexed move
+    """The m0 scan itself: a body with one more m0 access (an indexed move
     through m0) is rejected; the helper's own pattern is accepted."""
     good = ("_ZN3prg12_GLOBAL__N_118crt_product_kernelILi8ELi4EEEvNS0_11ProductArgsE:\n"
             "  s_mov_b32 m0, s4\n  s_nop 0\n  global_load_lds_dwordx4 v[0:1], off\n"
