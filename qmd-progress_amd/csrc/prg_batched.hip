@@ -15,6 +15,7 @@
 // the fused update X <- X + (1 - 2 pp)(X - X^2) with the core-row sum of
 // (X - X^2)^2 for vv.
 #include "prg_graph.h"
+#include "prg_lds_dma.h"
 #include "prg_tile.h"
 
 namespace prg {
@@ -513,6 +514,201 @@ __global__ __launch_bounds__(NT, 2) void batched_split_wide_kernel(BatchArgs g) 
   }
 }
 
+// Ring form of the wide split tile: 512 threads, 8 waves in a 2 x 4 grid of
+// (TM/2) x (TN/4), and the 32-deep operand slabs copied global -> LDS
+// directly (dma16) into a 4-stage ring, 3 slabs ahead of the MFMAs, with no
+// register staging.  A slab is ROWS = 2 TM + 2 TN rows of 64 bytes (A hi,
+// A lo, B hi, B lo), ROWS / 16 chunks of 1 KB; chunk c = w + 8 q goes to
+// bytes 1024 c of its ring slot, lane l writing row 16 c + l / 4, slot l % 4,
+// and loading global K-chunk (l % 4) ^ ((l / 8) & 3): the slot swizzle of
+// split_tile_wide, so the 16-byte fragment reads of 8 consecutive rows fall
+// on distinct banks.  Each step waits for its slab in every wave (one
+// barrier), refills the slot just read with the slab 4 ahead, reads the next
+// slab's fragments and runs this slab's MFMAs.  Products, update and sums as
+// split_tile.
+constexpr int RNT = 512;                // threads of the ring kernel
+constexpr int RSTAGE = 4;               // ring slots
+constexpr int kRingSlot = 512 * 64;     // bytes: up to 512 rows of 64 bytes
+
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 tl,
+                                                const int8_t (*ring)[kRingSlot]) {
+  typedef typename SplitT<T>::v8 h8;
+  constexpr double kScale = SplitT<T>::kScale;
+  constexpr int ROWS = 2 * TM + 2 * TN;
+  constexpr int NQ = ROWS / 16 / 8;       // DMA chunks per wave per slab
+  constexpr int WMR = TM / 2, WNC = TN / 4;
+  constexpr int MI = WMR / 16, NJ = WNC / 16;
+  static_assert(NQ * 16 * 8 == ROWS && MI >= 1 && NJ >= 1, "ring tile shape");
+  const PartDesc pd = g.parts[tl.x];
+  const int r0 = tl.y, c0 = tl.z;
+  const int m = pd.m;
+  const double* X = g.x + pd.offset;
+  double* Xn = g.xn + pd.offset;
+  const T* hi = (const T*)g.hi + pd.offset;
+  const T* lo = (const T*)g.lo + pd.offset;
+  T* hin = (T*)g.hin + pd.offset;
+  T* lon = (T*)g.lon + pd.offset;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+  const bool diag = (r0 == c0);
+  // a wave whose rows all lie below its columns in a diagonal tile
+  const bool idle = diag && (wm * WMR >= (wn + 1) * WNC);
+  // the previous tile's epilogue stores are done before this tile's copies
+  // are counted (the waits below count this wave's vector-memory operations)
+  lds_dma::wait_vm_barrier<0>();
+  // DMA: the lane's byte offset inside a chunk's 16 rows of one plane
+  const int lrow = lane >> 2;
+  const uint32_t voff =
+      (uint32_t)(lrow * m * (int)sizeof(T) + 16 * ((lane & 3) ^ ((lrow >> 1) & 3)));
+  const uint32_t ring0 = lds_dma::lds_addr(&ring[0][0]);
+  auto issue = [&](int s) {
+    const uint32_t slot = ring0 + kRingSlot * (s % RSTAGE);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int row = 16 * (w + 8 * q);  // first LDS row of the chunk (wave-uniform)
+      const T* src;
+      int prow;
+      if (row < TM) { src = hi; prow = r0 + row; }
+      else if (row < 2 * TM) { src = lo; prow = r0 + row - TM; }
+      else if (row < 2 * TM + TN) { src = hi; prow = c0 + row - 2 * TM; }
+      else { src = lo; prow = c0 + row - 2 * TM - TN; }
+      const uint64_t base =
+          lds_dma::uniform64((uint64_t)(uintptr_t)(src + (size_t)prow * m + (size_t)s * 32));
+      lds_dma::dma16(base, voff, __builtin_amdgcn_readfirstlane(slot + 64 * row));
+    }
+  };
+  // fragment reads: row (lane % 16) of a 16-row block, K-chunk lane / 16,
+  // whose slot is swizzled by the row's bits 1-2 (block starts are multiples
+  // of 16 rows)
+  const int ks = 16 * ((lane >> 4) ^ (((lane & 15) >> 1) & 3));
+  const int ra0 = wm * WMR + (lane & 15), rb0 = 2 * TM + wn * WNC + (lane & 15);
+  struct Frags {
+    h8 ah[MI], al[MI], bh[NJ], bl[NJ];
+  };
+  auto read_frags = [&](int s, Frags& f) {
+    const int8_t* st = ring[s % RSTAGE];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      f.ah[i] = *(const h8*)(st + 64 * (ra0 + 16 * i) + ks);
+      f.al[i] = *(const h8*)(st + 64 * (ra0 + TM + 16 * i) + ks);
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      f.bh[j] = *(const h8*)(st + 64 * (rb0 + 16 * j) + ks);
+      f.bl[j] = *(const h8*)(st + 64 * (rb0 + TN + 16 * j) + ks);
+    }
+  };
+  f4 acc_hh[MI][NJ], acc_x[MI][NJ];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      acc_hh[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      acc_x[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  const int ksteps = m / 32;  // m is a multiple of 64
+#pragma unroll
+  for (int s = 0; s < RSTAGE; ++s)
+    if (s < ksteps) issue(s);
+  if (RSTAGE - 1 >= ksteps)
+    lds_dma::wait_vm_barrier<0>();
+  else
+    lds_dma::wait_vm_barrier<NQ * (RSTAGE - 1)>();
+  Frags fa, fb;
+  if (!idle) read_frags(0, fa);
+  auto step = [&](int s, Frags& use, Frags& next) {
+    // slab s + 1 has landed in every wave and every wave's reads of slab s
+    // are done: its slot takes slab s + RSTAGE
+    if (s + RSTAGE - 1 < ksteps)
+      lds_dma::wait_vm_barrier<NQ * (RSTAGE - 2)>();
+    else
+      lds_dma::wait_vm_barrier<0>();
+    if (s + RSTAGE < ksteps) issue(s + RSTAGE);
+    if (!idle) {
+      // past the last slab this reads a stale slot that is never used
+      read_frags(s + 1, next);
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          acc_hh[i][j] = SplitT<T>::mfma(use.bh[j], use.ah[i], acc_hh[i][j]);
+          acc_x[i][j] = SplitT<T>::mfma(use.bl[j], use.ah[i], acc_x[i][j]);
+          acc_x[i][j] = SplitT<T>::mfma(use.bh[j], use.al[i], acc_x[i][j]);
+        }
+    }
+  };
+  // ksteps is even
+  for (int s = 0; s < ksteps; s += 2) {
+    step(s, fa, fb);
+    step(s + 1, fb, fa);
+  }
+  // every wave is done with the ring before the next tile's prologue
+  lds_dma::wait_vm_barrier<0>();
+  const double ca = g.sigma ? 1.0 : -1.0, cb = g.sigma ? 0.0 : 2.0;
+  const int core = pd.core;
+  double sum[1] = {0.0};
+  if (!idle) {
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int row = r0 + wm * WMR + i * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = c0 + wn * WNC + j * 16 + 4 * (lane >> 4) + r;
+          if (row > col) continue;
+          const bool mir = (row != col);
+          const size_t idx = row + (size_t)col * m;
+          const size_t tdx = mir ? col + (size_t)row * m : idx;
+          const double y = (double)acc_hh[i][j][r] + (1.0 / kScale) * (double)acc_x[i][j][r];
+          const double x = X[idx];
+          const double z = fma(cb, x, ca * y);
+          const double d = x - y;
+          const double wi = ((row < core) ? 1.0 : 0.0) + ((mir && col < core) ? 1.0 : 0.0);
+          sum[0] += wi * d * d;
+          const T h = (T)z;
+          const T l = (T)((z - (double)h) * kScale);
+          Xn[idx] = z;
+          Xn[tdx] = z;
+          hin[idx] = h;
+          hin[tdx] = h;
+          lon[idx] = l;
+          lon[tdx] = l;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  block_store_partials<1, RNT>(sum, g.part_sums + t);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RNT, 1) void batched_split_ring_kernel(BatchArgs g) {
+  __shared__ __attribute__((aligned(1024))) int8_t ring[RSTAGE][kRingSlot];
+  __shared__ int s_ticket;
+  const int home = blockIdx.x & 7;
+  for (int q = 0; q < 8; ++q) {
+    const int qi = (home + q) & 7;
+    const int first = g.queue_off[qi], count = g.queue_off[qi + 1] - first;
+    for (;;) {
+      __syncthreads();  // every thread has read the previous ticket
+      if (threadIdx.x == 0) s_ticket = atomicAdd(&g.tickets[qi], 1);
+      __syncthreads();
+      const int k = s_ticket;
+      if (k >= count) break;
+      const int t = first + k;
+      const int4 tl = g.tiles[t];
+      if (tl.w == 3)
+        split_tile_ring<T, 128, 128>(g, t, tl, ring);
+      else if (tl.w == 1)
+        split_tile_ring<T, 128, 64>(g, t, tl, ring);
+      else
+        split_tile_ring<T, 64, 64>(g, t, tl, ring);
+    }
+  }
+}
+
 // Persistent work-queue launch of split tiles (the queue walk of
 // batched_sp2_kernel over the 64-wide tile list).
 template <typename T>
@@ -549,8 +745,14 @@ __global__ void split_planes_kernel(const double* __restrict__ x, T* __restrict_
 
 typedef void (*BatchKernel)(BatchArgs);
 
+// Threads per workgroup of the configured kernel.
+int batch_threads(const BatchPlan& p) {
+  return (p.split && p.wide && p.depth == RSTAGE && p.kslab == WBK) ? RNT : NT;
+}
+
 // The compiled configurations: (panel, slabs of prefetch, slab depth); the
-// split-precision kernels have two, (64, 1, 64) and (128, 1, 32).
+// split-precision kernels have three, (64, 1, 64), (128, 1, 32) and the ring
+// form (128, 4, 32).
 BatchKernel batch_kernel(const BatchPlan& p) {
   if (p.split) {
     if (!p.wide && p.depth == 1 && p.kslab == SBK)
@@ -558,6 +760,9 @@ BatchKernel batch_kernel(const BatchPlan& p) {
     if (p.wide && p.depth == 1 && p.kslab == WBK)
       return p.split == 2 ? batched_split_wide_kernel<__bf16>
                           : batched_split_wide_kernel<_Float16>;
+    if (p.wide && p.depth == RSTAGE && p.kslab == WBK)
+      return p.split == 2 ? batched_split_ring_kernel<__bf16>
+                          : batched_split_ring_kernel<_Float16>;
     return nullptr;
   }
   if (p.wide) return (p.depth == 1 && p.kslab == 16) ? batched_sp2_kernel<true, 1, 16> : nullptr;
@@ -761,7 +966,7 @@ int batched_step(const BatchPlan& plan, const double* x, double* xn, int sigma, 
     return ERR_ARG;
   }
   const int grid = plan.grid;
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(NT), 0, stream(), g);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(batch_threads(plan)), 0, stream(), g);
   PRG_HIP_CHECK(hipGetLastError());
   return OK;
 }
@@ -775,7 +980,8 @@ int batched_slots(BatchPlan* plan) {
     return ERR_ARG;
   }
   int nb = 0, cus = 0, dev = 0;
-  PRG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, 0));
+  PRG_HIP_CHECK(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, batch_threads(*plan), 0));
   PRG_HIP_CHECK(hipGetDevice(&dev));
   PRG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   // one workgroup per resident slot, a multiple of 8 (workgroup b runs on XCD b mod 8)

@@ -34,6 +34,7 @@
 // up to 256), so a 256-row x 64-byte operand stage is one contiguous 16 KB
 // panel and a 64 x 64 block of a plane is one contiguous 4 KB.
 #include "prg_sp2.h"
+#include "prg_lds_dma.h"
 #include "prg_tile.h"
 
 #include <algorithm>
@@ -110,42 +111,10 @@ struct UpdateArgs {
   CrtConst c;
 };
 
-// M0 is clobbered by the DMA helper below; no compiler-generated code in this
-// file keeps a value in M0 (checked in the generated assembly), so the
-// reserved-register clobber warning is silenced here.
-#pragma clang diagnostic ignored "-Winline-asm"
-
-// LDS byte address of a __shared__ object.
-__device__ __forceinline__ uint32_t lds_addr(const void* p) {
-  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
-}
-
-// 16 bytes per lane from global memory at gbase + voff straight into LDS at
-// lds_base + 16 * lane (global_load_lds_dwordx4 with a scalar base: gbase
-// and lds_base must be wave-uniform, voff is the lane's 32-bit offset).
-// Written as inline assembly so that the compiler does not add its
-// conservative vmcnt(0) waits before every LDS read: the waits are explicit
-// (wait_vm_barrier).
-__device__ __forceinline__ void dma16(uint64_t gbase, uint32_t voff, uint32_t lds_base) {
-  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
-               :
-               : "v"(voff), "s"(gbase), "s"(lds_base)
-               : "memory", "m0");
-}
-
-// A 64-bit value that is uniform across the wave, moved to scalar registers.
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Waits until at most N vector-memory operations of this wave are pending and
-// its LDS reads have completed, then joins the workgroup barrier.
-template <int N>
-__device__ __forceinline__ void wait_vm_barrier() {
-  asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" : : "n"(N) : "memory");
-}
+// The LDS DMA helpers (dma16, wait_vm_barrier, lds_addr, uniform64).  M0 is
+// clobbered by dma16; no compiler-generated code in this file keeps a value
+// in M0 (checked in the generated assembly).
+using namespace lds_dma;
 
 // Symmetric residue of the integer-valued double v (|v| <= 2^53) modulo p,
 // in [-128, 127]: v - p rint(v / p), formed exactly by the fma.  v * inv

@@ -199,9 +199,10 @@ struct BatchConfig {
   int panel = 0, depth = 0, kslab = 0;
 };
 constexpr BatchConfig kCandidates[] = {{64, 1, 16}, {64, 2, 16}, {64, 1, 32}, {128, 1, 16}};
-// split-precision products: 64 x 64 tiles (64-deep slabs) or 128-wide panels
-// (32-deep slabs)
-constexpr BatchConfig kSplitCandidates[] = {{64, 1, 64}, {128, 1, 32}};
+// split-precision products: 64 x 64 tiles (64-deep slabs), 128-wide panels
+// (32-deep slabs through registers), or 128-wide panels with the slabs copied
+// into a 4-slot LDS ring (depth 4)
+constexpr BatchConfig kSplitCandidates[] = {{64, 1, 64}, {128, 1, 32}, {128, 4, 32}};
 
 std::map<std::pair<uint64_t, int>, BatchConfig>& config_cache() {
   static std::map<std::pair<uint64_t, int>, BatchConfig> c;
@@ -375,14 +376,17 @@ int subgraph_sp2_loop(const DMat& H, DMat rho, prg_graph_partitioning_t* gp) {
     if (st) return st;
     for (int k = 0; k < 4; ++k) plan.planes[k] = pl + k * sizeof(uint16_t) * off;
     std::vector<BatchConfig> scand;
+    if (const char* e = std::getenv("PRG_BATCH_DEPTH")) force.depth = std::atoi(e);
     for (const BatchConfig& c : kSplitCandidates)
-      if (!force.panel || c.panel == force.panel) scand.push_back(c);
+      if ((!force.panel || c.panel == force.panel) && (!force.depth || c.depth == force.depth))
+        scand.push_back(c);
     if (scand.empty()) {
-      set_error(ERR_ARG, "PRG_BATCH_TILE: no compiled split-precision batched SP2 kernel has "
-                         "this panel width");
+      set_error(ERR_ARG, "PRG_BATCH_TILE / PRG_BATCH_DEPTH: no compiled split-precision "
+                         "batched SP2 kernel has this configuration");
       return ERR_ARG;
     }
-    const auto skey = std::make_pair(size_signature(hp), -(plan.split * 256 + force.panel));
+    const auto skey = std::make_pair(size_signature(hp),
+                                     -((plan.split * 256 + force.panel) * 8 + force.depth));
     BatchConfig cfg = scand[0];
     auto hit = config_cache().find(skey);
     if (hit != config_cache().end()) {

@@ -37,10 +37,11 @@ __device__ __forceinline__ void full_tile(int t, int nt, int& I, int& J) {
   J = in / gsize;
 }
 
-// Deterministic workgroup sum of NQ per-thread values; thread 0 stores them.
-template <int NQ>
+// Deterministic workgroup sum of NQ per-thread values (BT threads); threads
+// 0 .. NQ - 1 store them.
+template <int NQ, int BT = kThreads>
 __device__ __forceinline__ void block_store_partials(double (&v)[NQ], double* dst) {
-  __shared__ double red[kThreads / 64][NQ];
+  __shared__ double red[BT / 64][NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     double x = v[q];
@@ -57,7 +58,7 @@ __device__ __forceinline__ void block_store_partials(double (&v)[NQ], double* ds
   if (threadIdx.x < NQ) {
     double s = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < kThreads / 64; ++ww) s += red[ww][threadIdx.x];
+    for (int ww = 0; ww < BT / 64; ++ww) s += red[ww][threadIdx.x];
     dst[threadIdx.x] = s;
   }
 }

@@ -1,13 +1,14 @@
-"""Checks on the gfx950 assembly of the library's own int8 CRT kernels.
+"""Checks on the gfx950 assembly of the library's own LDS-ring kernels.
 
-The product kernels copy operand stages global -> LDS with
-global_load_lds_dwordx4 written as inline assembly that sets M0 itself (so
-that the compiler adds no conservative vmcnt waits before LDS reads).  LLVM
-treats M0 as reserved and does not honour a clobber of it, so correctness
-needs every M0 access in those kernels to be the helper's own: an
-`s_mov_b32 m0, sN` followed by `s_nop 0` and the DMA.  This test compiles
-csrc/prg_gemm_crt.hip for gfx950 (device code only, to assembly) and fails if
-any other instruction of a product kernel reads or writes m0.
+The int8 CRT product kernels and the ring form of the batched split-precision
+SP2 kernel copy operand stages global -> LDS with global_load_lds_dwordx4
+written as inline assembly that sets M0 itself (so that the compiler adds no
+conservative vmcnt waits before LDS reads).  LLVM treats M0 as reserved and
+does not honour a clobber of it, so correctness needs every M0 access in
+those kernels to be the helper's own: an `s_mov_b32 m0, sN` followed by
+`s_nop 0` and the DMA.  This test compiles csrc/prg_gemm_crt.hip and
+csrc/prg_batched.hip for gfx950 (device code only, to assembly) and fails if
+any other instruction of those kernels reads or writes m0.
 """
 import re
 import shutil
@@ -17,16 +18,19 @@ from pathlib import Path
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
-SRC = ROOT / "qmd-progress_amd" / "csrc" / "prg_gemm_crt.hip"
+CSRC = ROOT / "qmd-progress_amd" / "csrc"
+# (source file, mangled-name pattern of its LDS-ring kernels)
+RING_KERNELS = [("prg_gemm_crt.hip", r"_Z\S*crt_product\S*kernel\S*"),
+                ("prg_batched.hip", r"_Z\S*batched_split_ring_kernel\S*")]
 HIPCC = Path("/opt/rocm/bin/hipcc")
 
 
-def _kernel_bodies(asm):
-    """{symbol: [instruction lines]} of the int8 CRT product kernels."""
+def _kernel_bodies(asm, pattern=r"_Z\S*crt_product\S*kernel\S*"):
+    """{symbol: [instruction lines]} of the kernels whose symbol matches."""
     bodies = {}
     cur = None
     for line in asm.splitlines():
-        m = re.match(r"^(_Z\S*crt_product\S*kernel\S*):", line)
+        m = re.match(r"^(" + pattern + r"):", line)
         if m:
             cur = m.group(1)
             bodies[cur] = []
@@ -42,15 +46,16 @@ def _kernel_bodies(asm):
 
 
 @pytest.mark.skipif(not HIPCC.exists() or shutil.which("bash") is None, reason="hipcc not found")
-def test_m0_is_only_touched_by_the_lds_dma_helper(tmp_path):
-    out = tmp_path / "crt.s"
+@pytest.mark.parametrize("src,pattern", RING_KERNELS)
+def test_m0_is_only_touched_by_the_lds_dma_helper(tmp_path, src, pattern):
+    out = tmp_path / "ring.s"
     r = subprocess.run([str(HIPCC), "-O3", "-std=c++17", "--offload-arch=gfx950",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                        "--offload-device-only", "-S", str(SRC), "-o", str(out)],
+                        "--offload-device-only", "-S", str(CSRC / src), "-o", str(out)],
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
-    bodies = _kernel_bodies(out.read_text())
-    assert bodies, "no product kernel found in the assembly"
+    bodies = _kernel_bodies(out.read_text(), pattern)
+    assert bodies, "no LDS-ring kernel found in the assembly"
     for name, ins in bodies.items():
         dmas = 0
         for i, s in enumerate(ins):

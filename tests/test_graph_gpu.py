@@ -163,22 +163,25 @@ def test_build_density_gp_t0_one_part_is_exact(P):
     assert np.linalg.norm(rho_g - d) < 1e-9
 
 
-@pytest.mark.parametrize("panel", [64, 128])
+@pytest.mark.parametrize("panel,depth", [(64, 1), (128, 1), (128, 4)])
 @pytest.mark.parametrize("mode", ["f16split", "bf16split"])
-def test_subgraph_loop_split_precision(P, wide_case, mode, panel, monkeypatch):
+def test_subgraph_loop_split_precision(P, wide_case, mode, panel, depth, monkeypatch):
     """The batched loop with FP16 / BF16 hi+lo split products (prg_set_precision
     F16SPLIT / BF16SPLIT; three 16-bit MFMA products with FP32 accumulation
     per tile, FP64 update and core-row sums) on the n = 1200, 8-part case,
-    with each of the two split kernels (PRG_BATCH_TILE: 64 x 64 tiles, or
-    128-wide panels giving 128 x 128, 128 x 64 and 64 x 64 tiles): the same
-    core+halo lists as the FP64 loop, and the density and vv of the oracle's
-    FP64 loop within the split products' accuracy (relative Frobenius bars
-    1e-3 FP16, 3e-2 BF16; vv within 1e-3 / 3e-2 of its largest entry)."""
-    print(f"[subgraph split] {mode} panel={panel}", flush=True)
-    monkeypatch.setenv("PRG_BATCH_TILE", str(panel))
+    with each of the three split kernels (PRG_BATCH_TILE / PRG_BATCH_DEPTH:
+    64 x 64 tiles; 128-wide panels giving 128 x 128, 128 x 64 and 64 x 64
+    tiles, slabs staged through registers; the same with a 4-slot LDS ring):
+    the same core+halo lists as the FP64 loop, and the density and vv of the
+    oracle's FP64 loop within the split products' accuracy (relative
+    Frobenius bars 1e-3 FP16, 3e-2 BF16; vv within 1e-3 / 3e-2 of its
+    largest entry)."""
+    print(f"[subgraph split] {mode} panel={panel} depth={depth}", flush=True)
     c = wide_case
     gp64 = P.equal_partition(150, c["n"])
     P.subgraph_sp2_loop(c["h"], c["g"], gp64, c["seq"], c["imax"], c["lo"], c["hi"])
+    monkeypatch.setenv("PRG_BATCH_TILE", str(panel))
+    monkeypatch.setenv("PRG_BATCH_DEPTH", str(depth))
     prec = {"f16split": P.PREC_F16SPLIT, "bf16split": P.PREC_BF16SPLIT}[mode]
     P.set_precision(prec)
     try:
@@ -188,7 +191,7 @@ def test_subgraph_loop_split_precision(P, wide_case, mode, panel, monkeypatch):
         cfg = P.subgraph_config()
     finally:
         P.set_precision(P.PREC_FP64)
-    assert cfg[0] == panel
+    assert cfg[0] == panel and cfg[1] == depth
     assert gp.halos() == gp64.halos()
     rel = np.linalg.norm(rho_g - c["rho"]) / np.linalg.norm(c["rho"])
     vv_o = np.asarray(c["vv"])
