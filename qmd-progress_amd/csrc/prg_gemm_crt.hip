@@ -148,13 +148,32 @@ __device__ __forceinline__ int sym_residue_odd(double v, double p, double inv) {
 
 // 1.5 * 2^52: for an integer m with |m| < 2^51, the low 32 bits of the double
 // kMagic + m (exact) are m as a two's-complement int32, so a residue formed
-// as fma(-p, t, v + kMagic) is read without a conversion.
+// as fma(-p, t, v + kMagic) is read without a conversion.  The int8 stored
+// for m is the low byte of that dword (+128 and -128, the two readings of a
+// p = 256 tie, share the byte 0x80).
 constexpr double kMagic = 6755399441055744.0;
 
-// The low byte of the residue carried by kMagic + m (the int8 stored for m;
-// +128 and -128, the two readings of a p = 256 tie, share the byte 0x80).
-__device__ __forceinline__ uint32_t magic_byte(double w) {
-  return (uint32_t)__double2loint(w) & 0xffu;
+// The low bytes of four int32 values as one dword (e0 lowest).  v_perm picks
+// bytes of its operand pair: selector byte s < 4 takes byte s of the second
+// operand, 4 + s byte s of the first, and 0x0c gives zero.
+__device__ __forceinline__ uint32_t pack_low_bytes(int e0, int e1, int e2, int e3) {
+  const uint32_t lo = __builtin_amdgcn_perm((uint32_t)e1, (uint32_t)e0, 0x0c0c0400u);
+  const uint32_t hi = __builtin_amdgcn_perm((uint32_t)e3, (uint32_t)e2, 0x0c0c0400u);
+  return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+// Transpose of a 4 x 4 byte block given as its rows a, b, c, d (byte j of a
+// row is column j): col[j] = (a_j, b_j, c_j, d_j).
+__device__ __forceinline__ void transpose_4x4(uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                              uint32_t (&col)[4]) {
+  const uint32_t ab01 = __builtin_amdgcn_perm(b, a, 0x05010400u);  // a0 b0 a1 b1
+  const uint32_t ab23 = __builtin_amdgcn_perm(b, a, 0x07030602u);  // a2 b2 a3 b3
+  const uint32_t cd01 = __builtin_amdgcn_perm(d, c, 0x05010400u);
+  const uint32_t cd23 = __builtin_amdgcn_perm(d, c, 0x07030602u);
+  col[0] = __builtin_amdgcn_perm(cd01, ab01, 0x05040100u);
+  col[1] = __builtin_amdgcn_perm(cd01, ab01, 0x07060302u);
+  col[2] = __builtin_amdgcn_perm(cd23, ab23, 0x05040100u);
+  col[3] = __builtin_amdgcn_perm(cd23, ab23, 0x07060302u);
 }
 
 // One (tile, modulus) item: C = A_I B_J^T mod p over the full K range (B = A
@@ -360,14 +379,13 @@ __device__ __forceinline__ void item_epilogue(
 #pragma unroll
       for (int n4 = 0; n4 < 4; ++n4) {
         const int n = 4 * h + n4;
-        uint32_t b4 = 0;
+        int e[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const double a = (double)acc[m][n][r];
-          const double w = fma(-p, rint(a * qmi), fma(a, qm, kMagic));
-          b4 |= magic_byte(w) << (8 * r);
+          e[r] = __double2loint(fma(-p, rint(a * qmi), fma(a, qm, kMagic)));
         }
-        out[n4] = (int)b4;
+        out[n4] = (int)pack_low_bytes(e[0], e[1], e[2], e[3]);
       }
       const int rw = L.wm + 2 * (L.wn * (NTL / 4) + h);
       dst[(rw * 8 + m) * 64] = out;
@@ -605,35 +623,36 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
   // Residue planes of X_next, one modulus at a time through a double-buffered
   // LDS block rb[row][col] (4 columns per dword).  Block (I, J) of a plane:
   // rows SB I + [0, 64), columns SB J + [0, 64) at J * pld * 64 +
-  // (SB I + row) * 64 + col; block (J, I) holds the transpose.
+  // (SB I + row) * 64 + col; block (J, I) holds the transpose, written by
+  // each thread as one transposed 4 x 4 byte block (rows qr .. qr + 3,
+  // columns qc .. qc + 3 of rb), 64 contiguous bytes per 16 lanes.
   int8_t* P = g.planes[cur ^ 1];
   const size_t plane = (size_t)pld * pld;
   const int dr = tid >> 2, dc = 16 * (tid & 3);     // direct: row dr, bytes dc..dc+15
-  const int tc = tid & 63, tr = 16 * (tid >> 6);    // transposed: column tc, rows tr..tr+15
+  const int qr = 4 * (tid & 15), qc = 4 * (tid >> 4);
   int8_t* pd = P + (size_t)J * pld * SB + (size_t)(SB * I + dr) * SB + dc;
-  int8_t* pt = P + (size_t)I * pld * SB + (size_t)(SB * J + tc) * SB + tr;
+  int8_t* pt = P + (size_t)I * pld * SB + (size_t)(SB * J + qc) * SB + qr;
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
     const double p = g.c.p[k], inv = g.c.inv[k];
     int8_t (*buf)[kPitch] = rb[k & 1];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      uint32_t b4 = 0;
+      int e[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (RECON && k == 0) {
           // |zz| <= 2^45 (a <= 45); p = kModuli[0] = 256, the one even
           // modulus: the quotient is exact and its tie (+128 or -128) is
           // the byte 0x80 either way, so no fold is needed
-          const double w = fma(-p, rint(zz[4 * n + r] * inv), zz[4 * n + r] + kMagic);
-          b4 |= magic_byte(w) << (8 * r);
+          e[r] = __double2loint(fma(-p, rint(zz[4 * n + r] * inv), zz[4 * n + r] + kMagic));
         } else if (RECON) {
-          b4 |= (uint32_t)(sym_residue_odd(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+          e[r] = sym_residue_odd(zz[4 * n + r], p, inv);
         } else {
-          b4 |= (uint32_t)(sym_residue(zz[4 * n + r], p, inv) & 0xff) << (8 * r);
+          e[r] = sym_residue(zz[4 * n + r], p, inv);
         }
       }
-      *(int*)&buf[rl][16 * n + cl0] = (int)b4;
+      *(uint32_t*)&buf[rl][16 * n + cl0] = pack_low_bytes(e[0], e[1], e[2], e[3]);
     }
     // the writes of modulus k - 1 from this buffer's twin are done by every
     // thread once all pass this barrier; modulus k + 1 fills the twin next
@@ -641,14 +660,11 @@ __global__ __launch_bounds__(NT, 3) void crt_update_kernel(UpdateArgs g) {
     const int* s4 = (const int*)&buf[dr][dc];
     *(i4*)(pd + (size_t)k * plane) = i4{s4[0], s4[1], s4[2], s4[3]};
     if (mirror) {
-      int v[4];
+      uint32_t col[4];
+      transpose_4x4(*(const uint32_t*)&buf[qr][qc], *(const uint32_t*)&buf[qr + 1][qc],
+                    *(const uint32_t*)&buf[qr + 2][qc], *(const uint32_t*)&buf[qr + 3][qc], col);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v[j] = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[j] |= (int)(uint8_t)buf[tr + 4 * j + i][tc] << (8 * i);
-      }
-      *(i4*)(pt + (size_t)k * plane) = i4{v[0], v[1], v[2], v[3]};
+      for (int j = 0; j < 4; ++j) *(uint32_t*)(pt + (size_t)k * plane + (size_t)j * SB) = col[j];
     }
   }
   // block sums (fixed order); off-diagonal blocks stand for their mirror too
