@@ -65,8 +65,13 @@ def parse():
     ap.add_argument("--graph-precision", default="fp64", choices=["fp64", "f16split", "bf16split"],
                     help="graph workload: products of the batched subgraph kernel")
     ap.add_argument("--products", default="fp64",
-                    help="fermi workload: arithmetic of the symmetric products, fp64 or "
-                         "int8crt<m> (int8 CRT products with m moduli)")
+                    help="fermi workload: arithmetic of the symmetric products, fp64, "
+                         "int8crt<m> (int8 CRT products with m moduli) or bf16split "
+                         "(BF16 hi+lo split products, FP32 accumulation; pass an --ns-tol "
+                         "above their noise floor, e.g. 1e-4)")
+    ap.add_argument("--mu-steps", type=int, default=None,
+                    help="fermi workload: fixed Newton steps on mu (0: until |Tr P - N/2| "
+                         "<= 1e-6; default 0, and 4 with --products bf16split)")
     ap.add_argument("--ns-tol", type=float, default=1e-10,
                     help="fermi workload: Newton-Schulz tolerance ||I - Y Inv||_F")
     ap.add_argument("--adjoint-of", default="sp2", choices=["sp2", "fermi"],
@@ -553,11 +558,12 @@ def _line(metric, value, unit, dt, args, dtype, config, extra, higher=True):
 def fermi_bench(args, P, torch, np):
     """prg_implicit_fermi (recursive Fermi-operator expansion, CG / Newton-Schulz
     inverses) on the model Hamiltonian at --n, beta = 20 eV^-1 (kT 0.05),
-    8 levels, k = 2, occupation N/2, its symmetric products in FP64 or (with
-    --products int8crt<m>) as int8 CRT products.  The library matrix handles
-    stay resident across the timed calls (H and P in HBM).  Accuracy: P
-    against the Fermi function of an FP64 eigendecomposition at the converged
-    mu, and (int8 products) against the FP64-product solve."""
+    8 levels, k = 2, occupation N/2, its symmetric products in FP64, as int8
+    CRT products (--products int8crt<m>) or as BF16 hi+lo split products
+    (--products bf16split).  The library matrix handles stay resident across
+    the timed calls (H and P in HBM).  Accuracy: P against the Fermi function
+    of an FP64 eigendecomposition at the converged mu, and (int8 or BF16
+    products) against the FP64-product solve at the same tolerance."""
     n = args.n
     h = model_hamiltonian_torch(n, args.seed, torch.device("cuda:0"))
     hh = h.cpu().numpy()
@@ -568,28 +574,36 @@ def fermi_bench(args, P, torch, np):
     w = torch.linalg.eigvalsh(h)
     mu = 0.5 * (w[n // 2 - 1] + w[n // 2]).item()
     products = args.products
-    if products == "fp64":
-        moduli = None
-    elif products.startswith("int8crt") and products[7:].isdigit():
+    moduli = None
+    if products.startswith("int8crt") and products[7:].isdigit():
         moduli = int(products[7:])
-    else:
-        raise SystemExit(f"--products: fp64 or int8crt<m>, not {products}")
+    elif products not in ("fp64", "bf16split"):
+        raise SystemExit(f"--products: fp64, int8crt<m> or bf16split, not {products}")
+
+    # osteps = 0: Newton steps on mu until |Tr P - nocc| <= 1e-6.  BF16 split
+    # products carry about 16 significant bits, so the trace cannot reach that
+    # limit: their solves (and the FP64 solve they are compared with) take four
+    # Newton steps on mu instead, unless --mu-steps says otherwise.
+    osteps = args.mu_steps if args.mu_steps is not None else (4 if products == "bf16split" else 0)
 
     def solve(mu0, method):
-        # osteps = 0: Newton steps on mu until |Tr P - nocc| <= 1e-6
-        L.prg_implicit_fermi(mh.h, mp.h, nsteps, 2, nocc, mu0, beta, method, 0, 1e-6, 0.0,
+        L.prg_implicit_fermi(mh.h, mp.h, nsteps, 2, nocc, mu0, beta, method, osteps, 1e-6, 0.0,
                              args.ns_tol)
         P._check()
+        progress(f"fermi: {products} solve done")
         return L.prg_implicit_fermi_last_mu()
 
     method = 1  # Newton-Schulz inverses (CG fallback)
     p_fp64 = None
-    if moduli is not None:
+    if products != "fp64":
         P.set_product_precision(P.PREC_FP64)
         solve(mu, method)
         p_fp64 = torch.from_numpy(mp.numpy()).cuda()
-        P.set_int8_moduli(moduli)
-        P.set_product_precision(P.PREC_INT8CRT)
+        if moduli is not None:
+            P.set_int8_moduli(moduli)
+            P.set_product_precision(P.PREC_INT8CRT)
+        else:
+            P.set_product_precision(P.PREC_BF16SPLIT)
     try:
         for _ in range(args.warmup):
             solve(mu, method)
@@ -614,7 +628,22 @@ def fermi_bench(args, P, torch, np):
     acc = {"fro_err_vs_fermi_eigh": err, "trace_err": tr, "band_energy_err": eb}
     if p_fp64 is not None:
         acc["fro_diff_vs_fp64_products"] = torch.linalg.norm(pm - p_fp64).item()
-    if moduli is None:
+    if products == "bf16split":
+        # executed BF16 MFMA flops: three products over the upper-triangle
+        # 128 x 128 tiles of the padded size per symmetric product
+        ld = P.padded_ld(n)
+        nt = ld // 128
+        ex16 = nprod * 3 * nt * (nt + 1) / 2 * 2.0 * 128 * 128 * ld
+        roof = {"bound": "mfma", "kernel": "split_gemm_kernel<bf16> (BF16 MFMA, 3 products)",
+                "achieved": ex16 / dt / 1e12, "peak": PEAK["fp16_tflops"],
+                "unit": "TFLOP/s", "frac": ex16 / dt / 1e12 / PEAK["fp16_tflops"],
+                "traffic": None,
+                "note": "BF16 MFMA flops executed by the solve's split products / "
+                        "whole-solve wall time (plane splits, host recurrences and syncs "
+                        "included: a lower bound of the kernel rate)"}
+        metric = "implicit Fermi solves/s (prg_implicit_fermi, BF16 hi+lo split products)"
+        dtype = "bf16/f32/f64"
+    elif moduli is None:
         roof = {"bound": "mfma", "kernel": "gemm_kernel (FP64 MFMA)",
                 "achieved": exe / dt / 1e12, "peak": PEAK["fp64_tflops"],
                 "unit": "TFLOP/s", "frac": exe / dt / 1e12 / PEAK["fp64_tflops"],
@@ -637,7 +666,8 @@ def fermi_bench(args, P, torch, np):
     _line(metric, 1.0 / dt, "solves/s", dt, args, dtype,
           {"workload": f"prg_implicit_fermi model tight-binding H, N={n}, beta=20 (kT 0.05 eV), "
                        f"8 levels, k=2, method 1 (Newton-Schulz, tol {args.ns_tol:g}), "
-                       f"seed={args.seed}", "n": n, "products": products,
+                       + (f"{osteps} Newton steps on mu, " if osteps else "")
+                       + f"seed={args.seed}", "n": n, "products": products,
            "mu_start": mu, "mu": mu1},
           {"accuracy": acc, "products_per_solve": nprod, "alg_tflops": alg / dt / 1e12,
            "cpu_baseline": cpu, "roofline": roof})

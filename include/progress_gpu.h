@@ -543,11 +543,14 @@ void prg_gpu_gemm_counters(int reset, double *products, double *alg_flops, doubl
 /* Arithmetic of the symmetric products of commuting symmetric matrices in
  * the iterative solvers other than the SP2 loops -- implicit Fermi (X^2,
  * the Newton-Schulz and CG products, Inv X^2), its saved-inverse variant,
- * SP2-Fermi, the adjoint's forward recomputation: PRG_PREC_FP64 (default) or
+ * SP2-Fermi, the adjoint's forward recomputation: PRG_PREC_FP64 (default),
  * PRG_PREC_INT8CRT (exact int8 products modulo prg_get_int8_moduli() moduli,
  * each operand scaled by the power of two above its largest row 2-norm,
- * FP64 reconstruction).  Other products stay FP64.  Sets the error state for
- * any other mode.  No counterpart in the reference. */
+ * FP64 reconstruction) or PRG_PREC_BF16SPLIT (BF16 hi+lo split products
+ * hi*hi + 2^-8 (hi*lo + lo*hi) with FP32 accumulation, FP64 output: about 16
+ * significant bits, so Newton-Schulz and CG tolerances must sit above that
+ * noise floor).  Other products stay FP64.  Sets the error state for any
+ * other mode.  No counterpart in the reference. */
 void prg_set_product_precision(int mode);
 int prg_get_product_precision(void);
 /* C = A B on device pointers (padded layout, prg_gpu_padded_ld(N)) for

@@ -128,20 +128,11 @@ __device__ __forceinline__ int sym_residue(double v, double p, double inv) {
   return (int)m;
 }
 
-// The same for |v| / p < 2^38 (the product residues C q, |v| < 2^39, and
-// X_next scaled by 2^a <= 2^45): v * inv is then within 2^-14 of v / p,
-// while for odd p the exact quotient of an integer v stays at least
+// The same for an odd modulus p and |v| / p < 2^38 (the product residues
+// C q, |v| < 2^39, and X_next scaled by 2^a <= 2^45): v * inv is then within
+// 2^-14 of v / p, while the exact quotient of an integer v stays at least
 // 1 / (2p) >= 2^-9 from a half-integer, so rint gives the nearest multiple
-// and v - p rint(.) lies in [-(p - 1) / 2, (p - 1) / 2]: no correction.  For
-// p = 256 the quotient is exact and a tie (remainder +128) is folded to
-// -128 by the one check, which never fires for odd p.
-__device__ __forceinline__ int sym_residue_small(double v, double p, double inv) {
-  double m = fma(-p, rint(v * inv), v);
-  m = (m > 127.0) ? m - p : m;
-  return (int)m;
-}
-
-// sym_residue_small for an odd modulus (no tie to fold).
+// and v - p rint(.) lies in [-(p - 1) / 2, (p - 1) / 2]: no correction.
 __device__ __forceinline__ int sym_residue_odd(double v, double p, double inv) {
   return (int)fma(-p, rint(v * inv), v);
 }
@@ -368,7 +359,7 @@ __device__ __forceinline__ void item_epilogue(
   // is read as C (q / p), within 2^-19 of v / p, which for odd p lies at
   // least 2^-9 from a half-integer (for p = 256 both are exact), so the
   // rounded quotient is rint(v / p) and fma(-p, t, v + kMagic) the
-  // symmetric residue, whose byte is what sym_residue_small would store
+  // symmetric residue (a p = 256 tie, +128, stores the byte of -128)
   const double p = g.fold[4 * k], qm = g.fold[4 * k + 2], qmi = g.fold[4 * k + 3];
   i4* dst = (i4*)(g.res + ((size_t)t * g.np + k) * kItemBytes) + L.lane;
 #pragma unroll

@@ -63,37 +63,19 @@ struct F16Args {
   double* part;
 };
 
+// The 128 x 128 tile (I, J) of A B for symmetric A and B given as hi/lo
+// planes (ld x ld): rows of a panel are read as contiguous column chunks.
+// Three segments over K: (hi, hi) -> acc_hh; (hi, lo), (lo, hi) -> acc_x.
+// Lane (w, lane) holds (A B)[row][col] of each 16 x 16 block (m, n) of wave
+// w's 64 x 64 quarter, row = lane % 16, col = 4 (lane / 16) + r.
 template <typename T>
-__global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
+__device__ __forceinline__ void split_tile_product(const T* ahi, const T* alo, const T* bhi,
+                                                   const T* blo, int ld, int I, int J,
+                                                   T (*As)[HROW], T (*Bs)[HROW],
+                                                   f4 (&acc_hh)[4][4], f4 (&acc_x)[4][4]) {
   typedef typename Split<T>::v8 h8;
-  constexpr double kScale = Split<T>::kScale;
-  __shared__ T As[BM][HROW];
-  __shared__ T Bs[BM][HROW];
-  __shared__ int s_flags[4];
-  if (threadIdx.x == 0) {
-    s_flags[0] = g.st->planned;
-    s_flags[1] = g.st->apply;
-    s_flags[2] = g.st->sigma;
-    s_flags[3] = g.st->cur;
-  }
-  __syncthreads();
-  if (!s_flags[0]) return;
-  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
-  const int ld = g.ld;
-  const double* X = g.buf[cur];
-  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
-  const T* hi = (const T*)g.hi[cur];
-  const T* lo = (const T*)g.lo[cur];
-  T* hin = (T*)const_cast<void*>(g.hi[cur ^ 1]);
-  T* lon = (T*)const_cast<void*>(g.lo[cur ^ 1]);
-
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  int I, J;
-  sym_tile(t, I, J);
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = tid >> 6, wm = w & 1, wn = w >> 1;
-
-  f4 acc_hh[4][4], acc_x[4][4];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -101,8 +83,6 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
       acc_hh[m][n] = f4{0.f, 0.f, 0.f, 0.f};
       acc_x[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     }
-
-  // three segments over K: (hi,hi) -> acc_hh ; (hi,lo), (lo,hi) -> acc_x
   const int ksteps = ld / BK;
   const int total = 3 * ksteps;
   const int c = tid >> 1, kh = (tid & 1) * 32;
@@ -110,8 +90,8 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
   auto load = [&](int s) {
     const int seg = s / ksteps;
     const int k0 = (s - seg * ksteps) * BK;
-    const T* Ap = (seg == 2) ? lo : hi;
-    const T* Bp = (seg == 1) ? lo : hi;
+    const T* Ap = (seg == 2) ? alo : ahi;
+    const T* Bp = (seg == 1) ? blo : bhi;
     const T* pa = Ap + (size_t)(I * BM + c) * ld + k0 + kh;
     const T* pb = Bp + (size_t)(J * BM + c) * ld + k0 + kh;
 #pragma unroll
@@ -154,6 +134,39 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
       }
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
+  constexpr double kScale = Split<T>::kScale;
+  __shared__ T As[BM][HROW];
+  __shared__ T Bs[BM][HROW];
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
+  const int ld = g.ld;
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
+  const T* hi = (const T*)g.hi[cur];
+  const T* lo = (const T*)g.lo[cur];
+  T* hin = (T*)const_cast<void*>(g.hi[cur ^ 1]);
+  T* lon = (T*)const_cast<void*>(g.lo[cur ^ 1]);
+
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+
+  f4 acc_hh[4][4], acc_x[4][4];
+  split_tile_product<T>(hi, lo, hi, lo, ld, I, J, As, Bs, acc_hh, acc_x);
 
   // Epilogue.  Operands were swapped, so this lane holds (X^2)[row][col]
   // with row = lane%16 and col = 4*(lane/16) + r within each 16 x 16 block.
@@ -207,6 +220,64 @@ __global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
     }
   }
   block_store_partials<4>(sums, g.part + 4 * (size_t)t);
+}
+
+struct SplitGemmArgs {
+  const void* ahi;
+  const void* alo;
+  const void* bhi;
+  const void* blo;
+  const double* Cin;  // beta term, or null
+  const double* E;    // <C, E> partial, or null
+  double* C;
+  double* part;       // per tile [Tr C, ||C||_F^2, <C, E>, 0], or null
+  double alpha, beta, gamma;
+  int ld, n;
+};
+
+// C = alpha A B + beta Cin + gamma I (gamma on the n x n diagonal) for
+// symmetric, commuting A and B from their hi/lo planes: one workgroup per
+// upper-triangle 128 x 128 tile, the tile mirrored; inside a diagonal tile
+// the upper half is mirrored onto the lower one, so C is exactly symmetric.
+template <typename T>
+__global__ __launch_bounds__(NT, 1) void split_gemm_kernel(SplitGemmArgs g) {
+  constexpr double kScale = Split<T>::kScale;
+  __shared__ T As[BM][HROW];
+  __shared__ T Bs[BM][HROW];
+  const int ld = g.ld;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
+  f4 acc_hh[4][4], acc_x[4][4];
+  split_tile_product<T>((const T*)g.ahi, (const T*)g.alo, (const T*)g.bhi, (const T*)g.blo, ld, I,
+                        J, As, Bs, acc_hh, acc_x);
+  double sums[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int row = I * BM + wm * 64 + m * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = J * BM + wn * 64 + n * 16 + 4 * (lane >> 4) + r;
+        if (row > col) continue;  // lower half of a diagonal tile
+        const bool mirror = (row != col), diag = !mirror;
+        const size_t idx = row + (size_t)col * ld;
+        double v = g.alpha * ((double)acc_hh[m][n][r] + (1.0 / kScale) * (double)acc_x[m][n][r]);
+        if (g.Cin) v += g.beta * g.Cin[idx];
+        v += (diag && row < g.n) ? g.gamma : 0.0;
+        const double we = mirror ? 2.0 : 1.0;
+        sums[0] += diag ? v : 0.0;
+        sums[1] += we * v * v;
+        if (g.E) sums[2] += we * v * g.E[idx];
+        g.C[idx] = v;
+        if (mirror) g.C[col + (size_t)row * ld] = v;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if (g.part) block_store_partials<4>(sums, g.part + 4 * (size_t)t);
 }
 
 // hi/lo planes of an FP64 matrix (whole ld x ld buffer, padding stays zero).
@@ -279,6 +350,82 @@ int sp2_mixed_prepare(const DMat& x0, int prec) {
   char* planes = (char*)workspace(bf16 ? "bf16_planes" : "f16_planes", 4 * plane, &st);
   if (st) return st;
   return split_f16(x0, planes, planes + plane, bf16);
+}
+
+// One operand that a caller promises not to change while held (the matrix of
+// a Newton-Schulz or CG solve): its planes are formed once and reused by
+// every split product that names it until the hold ends.
+namespace {
+struct SplitHeld {
+  const double* key = nullptr;
+  int depth = 0;
+  bool valid = false;
+  int ld = 0;
+};
+SplitHeld g_split_held;
+}  // namespace
+
+void split_hold_operand(const double* p) {
+  if (g_split_held.depth == 0) {
+    g_split_held.key = p;
+    g_split_held.valid = false;
+  }
+  if (g_split_held.key == p) ++g_split_held.depth;
+}
+
+void split_release_operand(const double* p) {
+  if (g_split_held.depth > 0 && g_split_held.key == p && --g_split_held.depth == 0) {
+    g_split_held.key = nullptr;
+    g_split_held.valid = false;
+  }
+}
+
+int split_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
+               double gamma, double* part, int* nparts, const DMat* dotE) {
+  const int ld = C.ld;
+  if (A.ld != ld || B.ld != ld || (Cin && Cin->ld != ld) || (dotE && dotE->ld != ld) ||
+      ld % BM != 0) {
+    set_error(ERR_ARG, "BF16 split products: operand leading dimensions differ");
+    return ERR_ARG;
+  }
+  const size_t plane = 2 * (size_t)ld * ld;  // bytes of one BF16 plane
+  const bool same = (A.p == B.p);
+  const bool held_a = g_split_held.depth > 0 && A.p == g_split_held.key;
+  const bool held_b = g_split_held.depth > 0 && B.p == g_split_held.key && !same;
+  int st = OK;
+  char* pa = (char*)workspace(held_a ? "splitg_planes_held" : "splitg_planes_a", 2 * plane, &st);
+  char* pb = same ? pa
+                  : (char*)workspace(held_b ? "splitg_planes_held" : "splitg_planes_b",
+                                     2 * plane, &st);
+  if (st) return st;
+  if ((held_a || held_b) && g_split_held.ld != ld) g_split_held.valid = false;
+  if (!(held_a && g_split_held.valid) && (st = split_f16(A, pa, pa + plane, true))) return st;
+  if (!same && !(held_b && g_split_held.valid) && (st = split_f16(B, pb, pb + plane, true)))
+    return st;
+  if (held_a || held_b) {
+    g_split_held.valid = true;
+    g_split_held.ld = ld;
+  }
+  SplitGemmArgs g;
+  g.ahi = pa;
+  g.alo = pa + plane;
+  g.bhi = pb;
+  g.blo = pb + plane;
+  g.Cin = (Cin && beta != 0.0) ? Cin->p : nullptr;
+  g.E = dotE ? dotE->p : nullptr;
+  g.C = C.p;
+  g.part = part;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.gamma = gamma;
+  g.ld = ld;
+  g.n = C.n;
+  const int nt = ld / BM;
+  const int nb = nt * (nt + 1) / 2;
+  if (nparts) *nparts = nb;
+  hipLaunchKernelGGL(split_gemm_kernel<__bf16>, dim3(nb), dim3(NT), 0, stream(), g);
+  PRG_HIP_CHECK(hipGetLastError());
+  return OK;
 }
 
 }  // namespace prg

@@ -20,7 +20,7 @@ namespace ops {
 // flops (2 n^3 each), FP64 MFMA flops executed (symmetric products compute
 // the upper-triangle 128 x 128 tiles only) and int8 MFMA operations executed
 // (int8 CRT products: upper-triangle 256 x 256 tiles of the padded size, once
-// per modulus).
+// per modulus).  BF16 split products count in the first two only.
 namespace {
 double g_products = 0.0, g_alg_flops = 0.0, g_exec_flops = 0.0, g_int8_ops = 0.0;
 int g_product_precision = PRG_PREC_FP64;
@@ -52,16 +52,23 @@ void gemm_counters(double* products, double* alg_flops, double* exec_flops, bool
 int product_precision() { return g_product_precision; }
 
 int set_product_precision(int mode) {
-  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_INT8CRT) {
-    set_error(ERR_ARG, "prg_set_product_precision: PRG_PREC_FP64 or PRG_PREC_INT8CRT");
+  if (mode != PRG_PREC_FP64 && mode != PRG_PREC_INT8CRT && mode != PRG_PREC_BF16SPLIT) {
+    set_error(ERR_ARG,
+              "prg_set_product_precision: PRG_PREC_FP64, PRG_PREC_INT8CRT or PRG_PREC_BF16SPLIT");
     return ERR_ARG;
   }
   g_product_precision = mode;
   return OK;
 }
 
-HoldOperand::HoldOperand(const DMat& A) : p(A.p) { crt_hold_operand(p); }
-HoldOperand::~HoldOperand() { crt_release_operand(p); }
+HoldOperand::HoldOperand(const DMat& A) : p(A.p) {
+  crt_hold_operand(p);
+  split_hold_operand(p);
+}
+HoldOperand::~HoldOperand() {
+  crt_release_operand(p);
+  split_release_operand(p);
+}
 
 int temp(const char* name, int n, DMat* out) {
   int st = OK;
@@ -103,6 +110,10 @@ int mm(const DMat& A, const DMat& B, DMat C, bool sym, double alpha, double beta
     const int np = crt_moduli();
     if ((st = crt_gemm(A, B, C, alpha, beta, Cin, gamma, part, &nt, dotE, np))) return st;
     count_crt_product(C, np);
+  } else if (sym && g_product_precision == PRG_PREC_BF16SPLIT) {
+    if ((st = split_gemm(A, B, C, alpha, beta, Cin, gamma, part, &nt, dotE))) return st;
+    g_products += 1;
+    g_alg_flops += 2.0 * C.n * (double)C.n * C.n;
   } else {
     if ((st = gemm(1, &A, &B, alpha, beta, Cin, gamma, C, sym ? 1 : 0, part, &nt, dotE)))
       return st;

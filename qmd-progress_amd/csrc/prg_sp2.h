@@ -77,6 +77,15 @@ int crt_moduli();
 // ||C||_F^2, <C, dotE>, 0], *nparts blocks.  C is exactly symmetric.
 int crt_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
              double gamma, double* part, int* nparts, const DMat* dotE, int np);
+// C = alpha A B + beta Cin + gamma I for symmetric, commuting A and B from
+// BF16 hi+lo split products with FP32 accumulation (hi*hi + 2^-8 (hi*lo +
+// lo*hi)); part (if not null) receives per-128x128-tile [Tr C, ||C||_F^2,
+// <C, dotE>, 0], *nparts tiles.  C is exactly symmetric.
+int split_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, const DMat* Cin,
+               double gamma, double* part, int* nparts, const DMat* dotE);
+// Hold operand p across split_gemm calls (its planes are formed once).
+void split_hold_operand(const double* p);
+void split_release_operand(const double* p);
 // Hold operand p across crt_gemm calls (its planes are formed once); the
 // caller must not change the matrix until the matching release.  Holds of the
 // same pointer nest; a hold of another pointer while one is active is ignored.

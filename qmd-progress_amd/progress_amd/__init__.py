@@ -497,7 +497,9 @@ def gemm_counters(reset=False, int8=False):
 
 def set_product_precision(mode):
     """Arithmetic of the symmetric products of commuting symmetric matrices in
-    implicit Fermi, SP2-Fermi and the adjoint: PREC_FP64 or PREC_INT8CRT."""
+    implicit Fermi, SP2-Fermi and the adjoint: PREC_FP64, PREC_INT8CRT or
+    PREC_BF16SPLIT (about 16 significant bits: solver tolerances must sit
+    above that noise floor)."""
     lib().prg_set_product_precision(int(mode))
     _check()
 
