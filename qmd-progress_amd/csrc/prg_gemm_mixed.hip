@@ -63,19 +63,37 @@ struct F16Args {
   double* part;
 };
 
-// The 128 x 128 tile (I, J) of A B for symmetric A and B given as hi/lo
-// planes (ld x ld): rows of a panel are read as contiguous column chunks.
-// Three segments over K: (hi, hi) -> acc_hh; (hi, lo), (lo, hi) -> acc_x.
-// Lane (w, lane) holds (A B)[row][col] of each 16 x 16 block (m, n) of wave
-// w's 64 x 64 quarter, row = lane % 16, col = 4 (lane / 16) + r.
 template <typename T>
-__device__ __forceinline__ void split_tile_product(const T* ahi, const T* alo, const T* bhi,
-                                                   const T* blo, int ld, int I, int J,
-                                                   T (*As)[HROW], T (*Bs)[HROW],
-                                                   f4 (&acc_hh)[4][4], f4 (&acc_x)[4][4]) {
+__global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
   typedef typename Split<T>::v8 h8;
+  constexpr double kScale = Split<T>::kScale;
+  __shared__ T As[BM][HROW];
+  __shared__ T Bs[BM][HROW];
+  __shared__ int s_flags[4];
+  if (threadIdx.x == 0) {
+    s_flags[0] = g.st->planned;
+    s_flags[1] = g.st->apply;
+    s_flags[2] = g.st->sigma;
+    s_flags[3] = g.st->cur;
+  }
+  __syncthreads();
+  if (!s_flags[0]) return;
+  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
+  const int ld = g.ld;
+  const double* X = g.buf[cur];
+  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
+  const T* hi = (const T*)g.hi[cur];
+  const T* lo = (const T*)g.lo[cur];
+  T* hin = (T*)const_cast<void*>(g.hi[cur ^ 1]);
+  T* lon = (T*)const_cast<void*>(g.lo[cur ^ 1]);
+
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  int I, J;
+  sym_tile(t, I, J);
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+
+  f4 acc_hh[4][4], acc_x[4][4];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -83,6 +101,8 @@ __device__ __forceinline__ void split_tile_product(const T* ahi, const T* alo, c
       acc_hh[m][n] = f4{0.f, 0.f, 0.f, 0.f};
       acc_x[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     }
+
+  // three segments over K: (hi,hi) -> acc_hh ; (hi,lo), (lo,hi) -> acc_x
   const int ksteps = ld / BK;
   const int total = 3 * ksteps;
   const int c = tid >> 1, kh = (tid & 1) * 32;
@@ -90,8 +110,8 @@ __device__ __forceinline__ void split_tile_product(const T* ahi, const T* alo, c
   auto load = [&](int s) {
     const int seg = s / ksteps;
     const int k0 = (s - seg * ksteps) * BK;
-    const T* Ap = (seg == 2) ? alo : ahi;
-    const T* Bp = (seg == 1) ? blo : bhi;
+    const T* Ap = (seg == 2) ? lo : hi;
+    const T* Bp = (seg == 1) ? lo : hi;
     const T* pa = Ap + (size_t)(I * BM + c) * ld + k0 + kh;
     const T* pb = Bp + (size_t)(J * BM + c) * ld + k0 + kh;
 #pragma unroll
@@ -134,39 +154,6 @@ __device__ __forceinline__ void split_tile_product(const T* ahi, const T* alo, c
       }
     }
   }
-}
-
-template <typename T>
-__global__ __launch_bounds__(NT, 1) void sp2_f16split_kernel(F16Args g) {
-  constexpr double kScale = Split<T>::kScale;
-  __shared__ T As[BM][HROW];
-  __shared__ T Bs[BM][HROW];
-  __shared__ int s_flags[4];
-  if (threadIdx.x == 0) {
-    s_flags[0] = g.st->planned;
-    s_flags[1] = g.st->apply;
-    s_flags[2] = g.st->sigma;
-    s_flags[3] = g.st->cur;
-  }
-  __syncthreads();
-  if (!s_flags[0]) return;
-  const int apply = s_flags[1], sigma = s_flags[2], cur = s_flags[3];
-  const int ld = g.ld;
-  const double* X = g.buf[cur];
-  double* Xn = const_cast<double*>(g.buf[cur ^ 1]);
-  const T* hi = (const T*)g.hi[cur];
-  const T* lo = (const T*)g.lo[cur];
-  T* hin = (T*)const_cast<void*>(g.hi[cur ^ 1]);
-  T* lon = (T*)const_cast<void*>(g.lo[cur ^ 1]);
-
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  int I, J;
-  sym_tile(t, I, J);
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
-
-  f4 acc_hh[4][4], acc_x[4][4];
-  split_tile_product<T>(hi, lo, hi, lo, ld, I, J, As, Bs, acc_hh, acc_x);
 
   // Epilogue.  Operands were swapped, so this lane holds (X^2)[row][col]
   // with row = lane%16 and col = 4*(lane/16) + r within each 16 x 16 block.
@@ -248,10 +235,76 @@ __global__ __launch_bounds__(NT, 1) void split_gemm_kernel(SplitGemmArgs g) {
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   int I, J;
   sym_tile(t, I, J);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
+  const T* ahi = (const T*)g.ahi;
+  const T* alo = (const T*)g.alo;
+  const T* bhi = (const T*)g.bhi;
+  const T* blo = (const T*)g.blo;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = tid >> 6, wm = w & 1, wn = w >> 1;
+  // the main loop of sp2_f16split_kernel on the operand planes, three
+  // segments over K: (hi, hi) -> acc_hh; (hi, lo), (lo, hi) -> acc_x.  (An
+  // inline copy: behind a shared function the compiler moved the
+  // accumulators between AGPRs and VGPRs inside the loop, a third slower.)
+  typedef typename Split<T>::v8 h8;
   f4 acc_hh[4][4], acc_x[4][4];
-  split_tile_product<T>((const T*)g.ahi, (const T*)g.alo, (const T*)g.bhi, (const T*)g.blo, ld, I,
-                        J, As, Bs, acc_hh, acc_x);
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      acc_hh[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+      acc_x[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  const int ksteps = ld / BK;
+  const int total = 3 * ksteps;
+  const int c = tid >> 1, kh = (tid & 1) * 32;
+  i4 ra[4], rb[4];
+  auto load = [&](int s) {
+    const int seg = s / ksteps;
+    const int k0 = (s - seg * ksteps) * BK;
+    const T* Ap = (seg == 2) ? alo : ahi;
+    const T* Bp = (seg == 1) ? blo : bhi;
+    const T* pa = Ap + (size_t)(I * BM + c) * ld + k0 + kh;
+    const T* pb = Bp + (size_t)(J * BM + c) * ld + k0 + kh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ra[q] = *(const i4*)(pa + 8 * q);
+      rb[q] = *(const i4*)(pb + 8 * q);
+    }
+  };
+  load(0);
+  for (int s = 0; s < total; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      *(i4*)(&As[c][kh + 8 * q]) = ra[q];
+      *(i4*)(&Bs[c][kh + 8 * q]) = rb[q];
+    }
+    __syncthreads();
+    if (s + 1 < total) load(s + 1);
+    const bool cross = s >= ksteps;
+#pragma unroll
+    for (int kk = 0; kk < BK / 32; ++kk) {
+      const int kc = kk * 32 + 8 * (lane >> 4);
+      h8 a[4], b[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a[m] = *(const h8*)(&As[wm * 64 + m * 16 + (lane & 15)][kc]);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) b[n] = *(const h8*)(&Bs[wn * 64 + n * 16 + (lane & 15)][kc]);
+      if (cross) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int n = 0; n < 4; ++n)
+            acc_x[m][n] = Split<T>::mfma(b[n], a[m], acc_x[m][n]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int n = 0; n < 4; ++n)
+            acc_hh[m][n] = Split<T>::mfma(b[n], a[m], acc_hh[m][n]);
+      }
+    }
+  }
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
