@@ -529,10 +529,15 @@ __global__ __launch_bounds__(NT, 2) void batched_split_wide_kernel(BatchArgs g) 
 constexpr int RNT = 512;                // threads of the ring kernel
 constexpr int RSTAGE = 4;               // ring slots
 constexpr int kRingSlot = 512 * 64;     // bytes: up to 512 rows of 64 bytes
+// LDS of the ring kernel: the ring, reused after a tile's main loop to stage
+// X_next of an off-diagonal tile for its mirrored store, 128 columns of 129
+// doubles (one more than the ring holds)
+constexpr int kRingLds = 128 * 129 * 8;
+static_assert(kRingLds >= RSTAGE * kRingSlot, "ring kernel LDS");
 
 template <typename T, int TM, int TN>
 __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 tl,
-                                                const int8_t (*ring)[kRingSlot]) {
+                                                const int8_t (*ring)[kRingSlot], double* stage) {
   typedef typename SplitT<T>::v8 h8;
   constexpr double kScale = SplitT<T>::kScale;
   constexpr int ROWS = 2 * TM + 2 * TN;
@@ -643,10 +648,12 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
     step(s, fa, fb);
     step(s + 1, fb, fa);
   }
-  // every wave is done with the ring before the next tile's prologue
+  // every wave is done with the ring before the next tile's prologue (and
+  // before an off-diagonal tile stages X_next in it)
   lds_dma::wait_vm_barrier<0>();
   const double ca = g.sigma ? 1.0 : -1.0, cb = g.sigma ? 0.0 : 2.0;
   const int core = pd.core;
+  constexpr int SP = TM + 1;  // staging pitch (doubles): column c of the tile at c * SP
   double sum[1] = {0.0};
   if (!idle) {
 #pragma unroll
@@ -660,7 +667,6 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
           if (row > col) continue;
           const bool mir = (row != col);
           const size_t idx = row + (size_t)col * m;
-          const size_t tdx = mir ? col + (size_t)row * m : idx;
           const double y = (double)acc_hh[i][j][r] + (1.0 / kScale) * (double)acc_x[i][j][r];
           const double x = X[idx];
           const double z = fma(cb, x, ca * y);
@@ -670,14 +676,37 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
           const T h = (T)z;
           const T l = (T)((z - (double)h) * kScale);
           Xn[idx] = z;
-          Xn[tdx] = z;
           hin[idx] = h;
-          hin[tdx] = h;
           lon[idx] = l;
-          lon[tdx] = l;
+          if (diag) {
+            // inside a diagonal tile the upper half is mirrored directly
+            const size_t tdx = mir ? col + (size_t)row * m : idx;
+            Xn[tdx] = z;
+            hin[tdx] = h;
+            lon[tdx] = l;
+          } else {
+            stage[(col - c0) * SP + (row - r0)] = z;
+          }
         }
         __builtin_amdgcn_sched_barrier(0);
       }
+    }
+  }
+  if (!diag) {
+    // the mirror block (rows c0 .., columns r0 ..): thread tid takes tile
+    // column c = tid % TN, i.e. memory row c0 + c, for every (RNT / TN)-th
+    // tile row, so that consecutive lanes store consecutive addresses
+    __syncthreads();
+    const int c = tid % TN;
+#pragma unroll 4
+    for (int rr = tid / TN; rr < TM; rr += RNT / TN) {
+      const double z = stage[c * SP + rr];
+      const T h = (T)z;
+      const T l = (T)((z - (double)h) * kScale);
+      const size_t tdx = (c0 + c) + (size_t)(r0 + rr) * m;
+      Xn[tdx] = z;
+      hin[tdx] = h;
+      lon[tdx] = l;
     }
   }
   block_store_partials<1, RNT>(sum, g.part_sums + t);
@@ -685,7 +714,9 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
 
 template <typename T>
 __global__ __launch_bounds__(RNT, 1) void batched_split_ring_kernel(BatchArgs g) {
-  __shared__ __attribute__((aligned(1024))) int8_t ring[RSTAGE][kRingSlot];
+  __shared__ __attribute__((aligned(1024))) int8_t lds[kRingLds];
+  const int8_t(*ring)[kRingSlot] = (const int8_t(*)[kRingSlot])lds;
+  double* stage = (double*)lds;
   __shared__ int s_ticket;
   const int home = blockIdx.x & 7;
   for (int q = 0; q < 8; ++q) {
@@ -700,11 +731,11 @@ __global__ __launch_bounds__(RNT, 1) void batched_split_ring_kernel(BatchArgs g)
       const int t = first + k;
       const int4 tl = g.tiles[t];
       if (tl.w == 3)
-        split_tile_ring<T, 128, 128>(g, t, tl, ring);
+        split_tile_ring<T, 128, 128>(g, t, tl, ring, stage);
       else if (tl.w == 1)
-        split_tile_ring<T, 128, 64>(g, t, tl, ring);
+        split_tile_ring<T, 128, 64>(g, t, tl, ring, stage);
       else
-        split_tile_ring<T, 64, 64>(g, t, tl, ring);
+        split_tile_ring<T, 64, 64>(g, t, tl, ring, stage);
     }
   }
 }
