@@ -438,7 +438,8 @@ int split_gemm(const DMat& A, const DMat& B, DMat C, double alpha, double beta, 
   const int ld = C.ld;
   if (A.ld != ld || B.ld != ld || (Cin && Cin->ld != ld) || (dotE && dotE->ld != ld) ||
       ld % BM != 0) {
-    set_error(ERR_ARG, "BF16 split products: operand leading dimensions differ");
+    set_error(ERR_ARG,
+              "BF16 split products: operand leading dimensions differ or are not multiples of 128");
     return ERR_ARG;
   }
   const size_t plane = 2 * (size_t)ld * ld;  // bytes of one BF16 plane
