@@ -679,9 +679,10 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
           hin[idx] = h;
           lon[idx] = l;
           if (diag) {
-            // inside a diagonal tile the upper half is mirrored directly
+            // inside a diagonal tile the upper half of the planes is mirrored
+            // directly (the FP64 master is kept in the upper triangle only:
+            // every reader takes that half)
             const size_t tdx = mir ? col + (size_t)row * m : idx;
-            Xn[tdx] = z;
             hin[tdx] = h;
             lon[tdx] = l;
           } else {
@@ -693,9 +694,11 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
     }
   }
   if (!diag) {
-    // the mirror block (rows c0 .., columns r0 ..): thread tid takes tile
-    // column c = tid % TN, i.e. memory row c0 + c, for every (RNT / TN)-th
-    // tile row, so that consecutive lanes store consecutive addresses
+    // the mirror block of the hi / lo planes (rows c0 .., columns r0 ..; the
+    // operand copies read whole rows of a plane, the FP64 master is read in
+    // its upper triangle only): thread tid takes tile column c = tid % TN,
+    // i.e. memory row c0 + c, for every (RNT / TN)-th tile row, so that
+    // consecutive lanes store consecutive addresses
     __syncthreads();
     const int c = tid % TN;
 #pragma unroll 4
@@ -704,7 +707,6 @@ __device__ __forceinline__ void split_tile_ring(const BatchArgs& g, int t, int4 
       const T h = (T)z;
       const T l = (T)((z - (double)h) * kScale);
       const size_t tdx = (c0 + c) + (size_t)(r0 + rr) * m;
-      Xn[tdx] = z;
       hin[tdx] = h;
       lon[tdx] = l;
     }
@@ -836,8 +838,11 @@ __global__ void scatter_core_kernel(const double* __restrict__ x,
   const double* X = x + pd.offset;
   for (int c = blockIdx.x; c < pd.n; c += gridDim.x) {
     const int gc = idx[c];
+    // X is read in its upper triangle: the split kernels keep only that half
+    // of the FP64 master current (the FP64 kernels write both)
     for (int r = threadIdx.x; r < pd.core; r += NT)
-      rho[idx[r] + (size_t)gc * ldr] = scale * X[r + (size_t)c * pd.m];
+      rho[idx[r] + (size_t)gc * ldr] =
+          scale * X[std::min(r, c) + (size_t)std::max(r, c) * pd.m];
   }
 }
 
